@@ -1,0 +1,226 @@
+"""CPU oracle for the Wilson / twisted-mass solve path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the
+checker: the product library never calls it.
+
+Two legs:
+
+* ``liboracle.so`` (``oracle/qqcd_oracle.c``): a plain-C restatement of the
+  reference's operators (Hsingle, Hdouble, Hdbletm, gammamult, vecdot, the
+  full-system operator) and of its blocked/checkerboarded storage order.
+* ``oracle/_ref/<lattice>/libqqcdref.so`` (``oracle/ref/Makefile``): the
+  reference Fortran itself, compiled from its sources with flang, reached
+  through the project-written ``oracle/ref/ref_driver.f90``.  It exists only
+  where the reference sources were present at build time.
+
+Canonical (oracle) arrays: spinors are complex128 of shape (V, 4, 3)
+(site, Dirac, colour), links complex128 of shape (V, 4, 3, 3) (site, mu,
+row, col), with site = x + nx*(y + ny*(z + nz*t)).  Reference arrays are
+flat float64 vectors in Fortran order: spinors (6, ntotal, 4, 2, 8), links
+(18, ntotal, 4, 2, 16).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def lib():
+    """Load (building on first use if needed) the C oracle."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        src = os.path.join(HERE, "qqcd_oracle.c")
+        if not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(src):
+            subprocess.check_call(["make", "-s", "-f", os.path.join(HERE, "Makefile")])
+        L = ctypes.CDLL(path)
+        L.orc_ref_nvhalf.restype = ctypes.c_long
+        L.orc_ref_ntotal.restype = ctypes.c_long
+        _LIB = L
+    return _LIB
+
+
+def dims(n):
+    return np.ascontiguousarray(n, dtype=np.int32)
+
+
+def volume(n):
+    return int(np.prod(n))
+
+
+def ref_sizes(n):
+    """(nvhalf, ntotal) of the reference's single-process layout (basics.f90:20-30)."""
+    L = lib()
+    d = dims(n)
+    return int(L.orc_ref_nvhalf(_iptr(d))), int(L.orc_ref_ntotal(_iptr(d)))
+
+
+def spinor_ref_len(n):
+    return 6 * ref_sizes(n)[1] * 64
+
+
+def link_ref_len(n):
+    return 18 * ref_sizes(n)[1] * 128
+
+
+def ref_site(n, x):
+    """1-based (i, ieo, ibl, ibleo, gblclr) of global site x in the reference layout."""
+    L = lib()
+    d = dims(n)
+    xx = np.ascontiguousarray(x, dtype=np.int32)
+    out = [ctypes.c_int() for _ in range(5)]
+    L.orc_ref_site(_iptr(d), _iptr(xx), *[ctypes.byref(o) for o in out])
+    return tuple(o.value for o in out)
+
+
+def spinor_canon_to_ref(n, psi):
+    L = lib()
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    be = np.zeros(spinor_ref_len(n))
+    bo = np.zeros(spinor_ref_len(n))
+    L.orc_spinor_canon_to_ref(_iptr(dims(n)), _ptr(psi.view(np.float64)), _ptr(be), _ptr(bo))
+    return be, bo
+
+
+def spinor_ref_to_canon(n, be=None, bo=None):
+    L = lib()
+    psi = np.zeros((volume(n), 4, 3), dtype=np.complex128)
+    null = ctypes.cast(None, _dp)
+    be_p = _ptr(np.ascontiguousarray(be, dtype=np.float64)) if be is not None else null
+    bo_p = _ptr(np.ascontiguousarray(bo, dtype=np.float64)) if bo is not None else null
+    L.orc_spinor_ref_to_canon(_iptr(dims(n)), be_p, bo_p, _ptr(psi.view(np.float64)))
+    return psi
+
+
+def links_ref_to_canon(n, u):
+    L = lib()
+    U = np.zeros((volume(n), 4, 3, 3), dtype=np.complex128)
+    L.orc_links_ref_to_canon(_iptr(dims(n)), _ptr(np.ascontiguousarray(u, dtype=np.float64)),
+                             _ptr(U.view(np.float64)))
+    return U
+
+
+def links_canon_to_ref(n, U):
+    L = lib()
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    u = np.zeros(link_ref_len(n))
+    L.orc_links_canon_to_ref(_iptr(dims(n)), _ptr(U.view(np.float64)), _ptr(u))
+    return u
+
+
+def gamma_table():
+    """gamma_1..gamma_5 as a (5, 4, 4) complex array (diracops.f90:61-71)."""
+    out = np.zeros(5 * 4 * 4 * 2)
+    lib().orc_gamma_table(_ptr(out))
+    return out.reshape(5, 4, 4, 2) @ np.array([1.0, 1.0j])
+
+
+def coact_array(c1=(1.0, 1.0, 1.0, 1.0), c2=(1.0, 1.0, 1.0, 1.0)):
+    """Flat Fortran-order coact(4,4,4) with coact(3,1,mu)=c1, coact(3,2,mu)=c2."""
+    co = np.zeros((4, 4, 4), order="F")
+    for mu in range(4):
+        co[2, 0, mu] = c1[mu]
+        co[2, 1, mu] = c2[mu]
+    return np.asfortranarray(co).ravel(order="F")
+
+
+def _cc(c):
+    return np.ascontiguousarray(c, dtype=np.float64)
+
+
+def hop(n, U, bc, c1, c2, idag, par_out, psi, out=None):
+    """Hopping term onto the sites of global parity par_out (0 even, 1 odd).
+
+    Sites of the other parity in ``out`` are left as they were (zeros for a
+    fresh output)."""
+    L = lib()
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    if out is None:
+        out = np.zeros_like(psi)
+    L.orc_hop(_iptr(dims(n)), _ptr(U.view(np.float64)), _iptr(dims(bc)), _ptr(_cc(c1)),
+              _ptr(_cc(c2)), int(idag), int(par_out), _ptr(psi.view(np.float64)),
+              _ptr(out.view(np.float64)))
+    return out
+
+
+def schur(n, U, bc, c1, c2, idag, kappa, psi):
+    """Even-odd Schur operator (Hdouble / Hdbletm) on the even sites of psi."""
+    L = lib()
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    out = np.zeros_like(psi)
+    t1 = np.zeros_like(psi)
+    t2 = np.zeros_like(psi)
+    L.orc_schur(_iptr(dims(n)), _ptr(U.view(np.float64)), _iptr(dims(bc)), _ptr(_cc(c1)),
+                _ptr(_cc(c2)), int(idag), _ptr(_cc(kappa)), _ptr(psi.view(np.float64)),
+                _ptr(out.view(np.float64)), _ptr(t1.view(np.float64)), _ptr(t2.view(np.float64)))
+    return out
+
+
+def full_m(n, U, bc, c1, c2, kappa, x):
+    """Full-lattice M x = (1 + 2 kappa mu i g5) x - kappa H x."""
+    L = lib()
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(x)
+    tmp = np.zeros_like(x)
+    L.orc_full_m(_iptr(dims(n)), _ptr(U.view(np.float64)), _iptr(dims(bc)), _ptr(_cc(c1)),
+                 _ptr(_cc(c2)), _ptr(_cc(kappa)), _ptr(x.view(np.float64)),
+                 _ptr(out.view(np.float64)), _ptr(tmp.view(np.float64)))
+    return out
+
+
+def gammamult(n, mu, psi):
+    L = lib()
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    out = np.zeros_like(psi)
+    L.orc_gammamult(_iptr(dims(n)), int(mu), _ptr(psi.view(np.float64)), _ptr(out.view(np.float64)))
+    return out
+
+
+def hsingle_ref(n, u, coact, bc, gblclr, idag, gin):
+    L = lib()
+    gout = np.zeros(spinor_ref_len(n))
+    L.orc_hsingle_ref(_iptr(dims(n)), _ptr(np.ascontiguousarray(u, dtype=np.float64)),
+                      _ptr(_cc(coact)), _iptr(dims(bc)), int(gblclr), int(idag),
+                      _ptr(np.ascontiguousarray(gin, dtype=np.float64)), _ptr(gout))
+    return gout
+
+
+def hdbletm_ref(n, u, coact, bc, idag, kappa, ge):
+    L = lib()
+    he = np.zeros(spinor_ref_len(n))
+    L.orc_hdbletm_ref(_iptr(dims(n)), _ptr(np.ascontiguousarray(u, dtype=np.float64)),
+                      _ptr(_cc(coact)), _iptr(dims(bc)), int(idag), _ptr(_cc(kappa)),
+                      _ptr(np.ascontiguousarray(ge, dtype=np.float64)), _ptr(he))
+    return he
+
+
+def parity_mask(n):
+    """Boolean (V,) mask of odd global parity."""
+    V = volume(n)
+    s = np.arange(V)
+    x = s % n[0]
+    y = (s // n[0]) % n[1]
+    z = (s // (n[0] * n[1])) % n[2]
+    t = s // (n[0] * n[1] * n[2])
+    return ((x + y + z + t) & 1).astype(bool)

@@ -1,0 +1,257 @@
+! ref_driver.f90 -- C entry points onto the reference Fortran, for the oracle.
+!
+! TEST INFRASTRUCTURE ONLY: this project-written module is compiled together
+! with the unmodified reference sources (BaylorLQCD "qqcd", compiled from
+! /root/reference by oracle/ref/Makefile into oracle/_ref/) and lets the
+! golden-vector script and the CPU baseline call the reference's own
+! routines through ctypes.  It holds no physics of its own: every entry point
+! forwards to one reference procedure with the geometry tables that the
+! reference's initlat builds (qqcd-test/cfgsprops/glue/lattice.f90:33-59).
+!
+! Single process only (npx=npy=npz=npt=1 in the latdims copy the recipe
+! writes), myid = 0.
+
+module refdrv
+   use iso_c_binding
+   use kinds
+   use latdims
+   use basics
+   use lattice
+   use diracops
+#ifdef WITH_SOLVERS
+   use gmresrhs
+   use inverters
+   use quark
+   use gaugetools
+   use heatbath
+   use lagfib
+#endif
+   implicit none
+   include 'mpif.h'
+
+   integer(kind=KI), dimension(4)           :: gbc
+   integer(kind=KI), dimension(4,2)         :: nn
+   logical,          dimension(4)           :: ldiv
+   integer(kind=KI), dimension(4)           :: nms
+   integer(kind=KI), dimension(nvhalf,4,2)  :: lv, lvbc
+   integer(kind=KI), dimension(nbmax,4,2,2) :: ib
+   logical,          dimension(16,4)        :: lbd
+   integer(kind=KI), dimension(16,4)        :: iblv
+   integer(kind=KI), dimension(2,8)         :: vecbl
+   integer(kind=KI), dimension(2,16)        :: vecblinv
+   character(len=512), dimension(1)         :: rwdir
+   integer(kind=KI), parameter              :: myid = 0
+
+contains
+
+   ! Lattice sizes compiled into this build: n(4), nvhalf, ntotal.
+   subroutine ref_dims(n, nvh, ntot) bind(C, name="ref_dims")
+      integer(c_int), intent(out) :: n(4), nvh, ntot
+      n = (/ nx, ny, nz, nt /)
+      nvh = nvhalf
+      ntot = ntotal
+   end subroutine ref_dims
+
+   ! Build the geometry tables for boundary conditions bc(4); dir is the
+   ! directory the reference solvers append CFGSPROPS.LOG to.
+   subroutine ref_init(bc, dir, ndir) bind(C, name="ref_init")
+      integer(c_int), intent(in) :: bc(4)
+      integer(c_int), value      :: ndir
+      character(kind=c_char), intent(in) :: dir(ndir)
+      integer :: k
+      gbc = bc
+      call initlat(myid, gbc, nn, ldiv, nms, lv, lvbc, ib, lbd, iblv, vecbl, vecblinv)
+      rwdir(1) = ' '
+      do k = 1, ndir
+         rwdir(1)(k:k) = dir(k)
+      end do
+   end subroutine ref_init
+
+   ! Export the geometry tables (for the layout tests).
+   subroutine ref_tables(o_lvbc, o_vecbl, o_vecblinv, o_iblv, o_lbd, o_nms) bind(C, name="ref_tables")
+      integer(c_int), intent(out) :: o_lvbc(nvhalf,4,2), o_vecbl(2,8), o_vecblinv(2,16)
+      integer(c_int), intent(out) :: o_iblv(16,4), o_lbd(16,4), o_nms(4)
+      o_lvbc = lvbc
+      o_vecbl = vecbl
+      o_vecblinv = vecblinv
+      o_iblv = iblv
+      o_lbd = merge(1, 0, lbd)
+      o_nms = nms
+   end subroutine ref_tables
+
+   subroutine ref_hsingle(gout, u, gin, idag, coact, gblclr) bind(C, name="ref_hsingle")
+      real(c_double), intent(inout) :: gout(6,ntotal,4,2,8), gin(6,ntotal,4,2,8)
+      real(c_double), intent(in)    :: u(18,ntotal,4,2,16), coact(4,4,4)
+      integer(c_int), value         :: idag, gblclr
+      integer(kind=KI) :: MRT
+      MRT = MPI_DOUBLE_PRECISION
+      call Hsingle(gout, u, gin, idag, coact, gbc, gblclr, vecbl, vecblinv, myid, nn, &
+                   ldiv, nms, lvbc, ib, lbd, iblv, MRT)
+   end subroutine ref_hsingle
+
+   subroutine ref_hdouble(he, u, ge, idag, coact, kappa) bind(C, name="ref_hdouble")
+      real(c_double), intent(inout) :: he(6,ntotal,4,2,8), ge(6,ntotal,4,2,8)
+      real(c_double), intent(in)    :: u(18,ntotal,4,2,16), coact(4,4,4)
+      integer(c_int), value         :: idag
+      real(c_double), value         :: kappa
+      integer(kind=KI) :: MRT
+      MRT = MPI_DOUBLE_PRECISION
+      call Hdouble(he, u, ge, idag, coact, kappa, gbc, vecbl, vecblinv, myid, nn, ldiv, &
+                   nms, lvbc, ib, lbd, iblv, MRT)
+   end subroutine ref_hdouble
+
+   subroutine ref_hdbletm(he, u, ge, idag, coact, kappa, iflag) bind(C, name="ref_hdbletm")
+      real(c_double), intent(inout) :: he(6,ntotal,4,2,8), ge(6,ntotal,4,2,8)
+      real(c_double), intent(in)    :: u(18,ntotal,4,2,16), coact(4,4,4), kappa(2)
+      integer(c_int), value         :: idag, iflag
+      real(kind=KR) :: gdummy(18,1,1,1,1)
+      integer(kind=KI) :: MRT
+      MRT = MPI_DOUBLE_PRECISION
+      gdummy = 0.0_KR
+      call Hdbletm(he, u, gdummy, ge, idag, coact, kappa, iflag, gbc, vecbl, vecblinv, &
+                   myid, nn, ldiv, nms, lvbc, ib, lbd, iblv, MRT)
+   end subroutine ref_hdbletm
+
+   subroutine ref_gammamult(ein, oin, eout, oout, mu, ieo, ibleo) bind(C, name="ref_gammamult")
+      real(c_double), intent(in)    :: ein(6,ntotal,4,2,8), oin(6,ntotal,4,2,8)
+      real(c_double), intent(inout) :: eout(6,ntotal,4,2,8), oout(6,ntotal,4,2,8)
+      integer(c_int), value         :: mu, ieo, ibleo
+      call gammamult(ein, oin, eout, oout, mu, ieo, ibleo)
+   end subroutine ref_gammamult
+
+   subroutine ref_gam5x(a) bind(C, name="ref_gam5x")
+      real(c_double), intent(inout) :: a(6,ntotal,4,2,8)
+      call gam5x(a)
+   end subroutine ref_gam5x
+
+#ifdef WITH_SOLVERS
+   subroutine ref_vecdot(a, b, adb) bind(C, name="ref_vecdot")
+      real(c_double), intent(in)  :: a(6,ntotal,4,2,8), b(6,ntotal,4,2,8)
+      real(c_double), intent(out) :: adb(2)
+      integer(kind=KI) :: MRT2
+      MRT2 = MPI_DOUBLE_PRECISION
+      call vecdot(a, b, adb, MRT2)
+   end subroutine ref_vecdot
+
+   ! gmresdr / psc / bicgstab on the even-odd system phi -> x, iflag=-1.
+   subroutine ref_solve(which, phi, x, nk, resmax, itermin, itercount, u, coact, kappa, &
+                        isig) bind(C, name="ref_solve")
+      integer(c_int), value         :: which, itermin, isig
+      real(c_double), intent(inout) :: phi(6,ntotal,4,2,8), x(6,ntotal,4,2,8)
+      integer(c_int), intent(in)    :: nk(2)
+      real(c_double), value         :: resmax
+      integer(c_int), intent(out)   :: itercount
+      real(c_double), intent(in)    :: u(18,ntotal,4,2,16), coact(4,4,4), kappa(2)
+      real(kind=KR) :: gdummy(18,1,1,1,1)
+      integer(kind=KI) :: MRT, MRT2, iflag, gm(2)
+      MRT = MPI_DOUBLE_PRECISION
+      MRT2 = MPI_DOUBLE_PRECISION
+      iflag = -1
+      gm = nk
+      gdummy = 0.0_KR
+      isignal = isig
+      select case (which)
+      case (0)
+         call bicgstab(rwdir, phi, x, resmax, itermin, itercount, u, gdummy, iflag, kappa, &
+                       coact, gbc, vecbl, vecblinv, myid, nn, ldiv, nms, lvbc, ib, lbd, iblv, &
+                       MRT, MRT2)
+      case (5)
+         call gmresdr(rwdir, phi, x, gm, resmax, itermin, itercount, u, gdummy, iflag, &
+                      kappa, coact, gbc, vecbl, vecblinv, myid, nn, ldiv, nms, lvbc, ib, lbd, &
+                      iblv, MRT, MRT2)
+      case (6)
+         call psc(rwdir, phi, x, gm, resmax, itermin, itercount, u, gdummy, iflag, &
+                  kappa, coact, gbc, vecbl, vecblinv, myid, nn, ldiv, nms, lvbc, ib, lbd, &
+                  iblv, MRT, MRT2)
+      end select
+   end subroutine ref_solve
+
+   ! fermprop for twisted/Wilson (nkappa=-1): be, bo -> xe, xo.
+   subroutine ref_fermprop(be, bo, kappa, inverter, coact, resmax, itermin, nk, u, xe, xo, &
+                           isig) bind(C, name="ref_fermprop")
+      real(c_double), intent(inout) :: be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8)
+      real(c_double), intent(inout) :: xe(6,ntotal,4,2,8,1), xo(6,ntotal,4,2,8,1)
+      real(c_double), intent(in)    :: kappa(2), coact(4,4,4)
+      real(c_double), intent(inout) :: u(18,ntotal,4,2,16)
+      integer(c_int), value         :: inverter, itermin, isig
+      real(c_double), value         :: resmax
+      integer(c_int), intent(in)    :: nk(2)
+      integer(kind=KI) :: MRT, MRT2, vectype, kg, ntm, hopt
+      real(kind=KR2)   :: om
+      real(kind=KR)    :: csw
+      MRT = MPI_DOUBLE_PRECISION
+      MRT2 = MPI_DOUBLE_PRECISION
+      vectype = nk(1)
+      kg = nk(2)
+      ntm = 0
+      hopt = 0
+      om = 1.0_KR2
+      csw = 0.0_KR
+      isignal = isig
+      call fermprop(rwdir, be, bo, -1, kappa, csw, inverter, coact, gbc, resmax, itermin, om, &
+                    u, vectype, kg, xe, xo, myid, nn, ldiv, nms, lvbc, lv, ib, lbd, iblv, vecbl, &
+                    vecblinv, ntm, hopt, MRT, MRT2)
+   end subroutine ref_fermprop
+
+   ! pointsource(be,bo,src,icsrc,idsrc,...), quark.f90:46-108
+   subroutine ref_pointsource(be, bo, src, icsrc, idsrc) bind(C, name="ref_pointsource")
+      real(c_double), intent(inout) :: be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8)
+      integer(c_int), intent(in)    :: src(4)
+      integer(c_int), value         :: icsrc, idsrc
+      integer(kind=KI) :: s(4)
+      s = src
+      call pointsource(be, bo, s, icsrc, idsrc, myid, iblv, vecblinv)
+   end subroutine ref_pointsource
+
+   ! nsweep pseudo-heatbath sweeps at the plaquette action held in coact,
+   ! following sweep (qqcd-test/cfgsprops/cfgsprops.f90:1973-2003), with the
+   ! lagged-Fibonacci stream seeded by initlagfib(0, iseed).
+   subroutine ref_heatbath(u, coact, nsweep, nhit, iseed) bind(C, name="ref_heatbath")
+      real(c_double), intent(inout) :: u(18,ntotal,4,2,16)
+      real(c_double), intent(in)    :: coact(4,4,4)
+      integer(c_int), value         :: nsweep, nhit, iseed
+      integer(kind=KI) :: isw, ieo, ibl, mu, MRT
+      integer(kind=KI) :: gaction(4)
+      real(kind=KR)    :: c(18,nvhalf)
+      MRT = MPI_DOUBLE_PRECISION
+      gaction = 1
+      call initlagfib(myid, iseed)
+      do isw = 1, nsweep
+         do ieo = 1, 2
+            do ibl = 1, 16
+               do mu = 1, 4
+                  call staple(mu, ieo, ibl, coact, gaction, c, u, myid, nn, ldiv, nms, lv, ib, &
+                              lbd, iblv, MRT)
+                  call phbup(mu, ieo, ibl, c, u, nhit)
+               end do
+            end do
+         end do
+      end do
+   end subroutine ref_heatbath
+
+   ! average plaquette, aveplaq (qqcd-test/cfgsprops/glue/gaugetools.f90:1436-1474)
+   subroutine ref_aveplaq(u, plaq) bind(C, name="ref_aveplaq")
+      real(c_double), intent(inout) :: u(18,ntotal,4,2,16)
+      real(c_double), intent(out)   :: plaq
+      integer(kind=KI) :: MRT
+      real(kind=KR)    :: p
+      MRT = MPI_DOUBLE_PRECISION
+      call aveplaq(p, u, myid, nn, ldiv, nms, lv, ib, lbd, iblv, MRT)
+      plaq = p
+   end subroutine ref_aveplaq
+#endif
+
+   ! coact for the isotropic, untadpoled plaquette + Wilson action
+   ! (fixaction, qqcd-test/cfgsprops/glue/lattice.f90:358-437)
+   subroutine ref_fixaction(beta, coact) bind(C, name="ref_fixaction")
+      real(c_double), value       :: beta
+      real(c_double), intent(out) :: coact(4,4,4)
+      integer(kind=KI) :: gaction(4)
+      real(kind=KR)    :: alat(4), tad(4)
+      gaction = 1
+      alat = 1.0_KR
+      tad = 1.0_KR
+      call fixaction(gaction, beta, alat, tad, coact)
+   end subroutine ref_fixaction
+
+end module refdrv

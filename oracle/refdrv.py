@@ -1,0 +1,187 @@
+"""ctypes access to the reference Fortran compiled by oracle/ref/Makefile.
+
+TEST INFRASTRUCTURE ONLY (golden-vector generation and the reference CPU
+baseline).  A build exists only where /root/reference was present when
+oracle/ref/Makefile ran; ``available()`` says whether one is on disk.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+from . import HERE
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+REF_SRC = "/root/reference/qqcd-test"
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _ip_(a):
+    return a.ctypes.data_as(_ip)
+
+
+def tag(n, solvers=True):
+    return "x".join(str(int(v)) for v in n) + ("" if solvers else "-ds")
+
+
+def lib_path(n, solvers=True):
+    return os.path.join(HERE, "_ref", tag(n, solvers), "libqqcdref.so")
+
+
+def available(n, solvers=True):
+    return os.path.exists(lib_path(n, solvers))
+
+
+def build(n, solvers=True, jobs=4):
+    """Compile the reference for lattice n (needs /root/reference and flang)."""
+    if not os.path.isdir(REF_SRC):
+        raise FileNotFoundError("reference sources not present: " + REF_SRC)
+    subprocess.check_call(
+        ["make", "-s", "-f", os.path.join(HERE, "ref", "Makefile"), f"-j{jobs}",
+         f"NX={n[0]}", f"NY={n[1]}", f"NZ={n[2]}", f"NT={n[3]}", f"SOLVERS={1 if solvers else 0}"],
+        stdout=subprocess.DEVNULL)
+    return lib_path(n, solvers)
+
+
+class RefLattice:
+    """One compiled reference build (fixed lattice) with its geometry tables."""
+
+    def __init__(self, n, bc=(1, 1, 1, -1), solvers=True, workdir=None):
+        path = lib_path(n, solvers)
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.L = ctypes.CDLL(path)
+        nn = np.zeros(4, np.int32)
+        nvh = ctypes.c_int()
+        ntot = ctypes.c_int()
+        self.L.ref_dims(_ip_(nn), ctypes.byref(nvh), ctypes.byref(ntot))
+        self.n = tuple(int(v) for v in nn)
+        if self.n != tuple(int(v) for v in n):
+            raise ValueError(f"build holds {self.n}, asked for {tuple(n)}")
+        self.nvhalf, self.ntotal = nvh.value, ntot.value
+        self.solvers = solvers
+        self.workdir = workdir or tempfile.mkdtemp(prefix="qqcdref_")
+        # the reference solvers append to <rwdir>CFGSPROPS.LOG, opened status="old"
+        open(os.path.join(self.workdir, "CFGSPROPS.LOG"), "a").close()
+        self.set_bc(bc)
+
+    def set_bc(self, bc):
+        self.bc = np.ascontiguousarray(bc, dtype=np.int32)
+        d = (self.workdir.rstrip("/") + "/").encode()
+        self.L.ref_init(_ip_(self.bc), ctypes.c_char_p(d), ctypes.c_int(len(d)))
+
+    @property
+    def slen(self):
+        return 6 * self.ntotal * 64
+
+    @property
+    def ulen(self):
+        return 18 * self.ntotal * 128
+
+    def tables(self):
+        nvh = self.nvhalf
+        lvbc = np.zeros((2, 4, nvh), np.int32)
+        vecbl = np.zeros((8, 2), np.int32)
+        vecblinv = np.zeros((16, 2), np.int32)
+        iblv = np.zeros((4, 16), np.int32)
+        lbd = np.zeros((4, 16), np.int32)
+        nms = np.zeros(4, np.int32)
+        self.L.ref_tables(_ip_(lvbc), _ip_(vecbl), _ip_(vecblinv), _ip_(iblv), _ip_(lbd), _ip_(nms))
+        return dict(lvbc=lvbc, vecbl=vecbl, vecblinv=vecblinv, iblv=iblv, lbd=lbd, nms=nms)
+
+    def hsingle(self, u, gin, coact, gblclr, idag=0):
+        gout = np.zeros(self.slen)
+        g = np.array(gin, dtype=np.float64, copy=True)
+        self.L.ref_hsingle(_p(gout), _p(np.ascontiguousarray(u)), _p(g), int(idag),
+                           _p(np.ascontiguousarray(coact)), int(gblclr))
+        return gout
+
+    def hdouble(self, u, ge, coact, kappa, idag=0):
+        he = np.zeros(self.slen)
+        g = np.array(ge, dtype=np.float64, copy=True)
+        self.L.ref_hdouble(_p(he), _p(np.ascontiguousarray(u)), _p(g), int(idag),
+                           _p(np.ascontiguousarray(coact)), ctypes.c_double(kappa))
+        return he
+
+    def hdbletm(self, u, ge, coact, kappa, idag=0, iflag=-1):
+        he = np.zeros(self.slen)
+        g = np.array(ge, dtype=np.float64, copy=True)
+        k = np.ascontiguousarray(kappa, dtype=np.float64)
+        self.L.ref_hdbletm(_p(he), _p(np.ascontiguousarray(u)), _p(g), int(idag),
+                           _p(np.ascontiguousarray(coact)), _p(k), int(iflag))
+        return he
+
+    def gammamult(self, ein, oin, mu, ieo, ibleo, eout=None, oout=None):
+        eout = np.zeros(self.slen) if eout is None else np.array(eout, copy=True)
+        oout = np.zeros(self.slen) if oout is None else np.array(oout, copy=True)
+        self.L.ref_gammamult(_p(np.ascontiguousarray(ein)), _p(np.ascontiguousarray(oin)),
+                             _p(eout), _p(oout), int(mu), int(ieo), int(ibleo))
+        return eout, oout
+
+    def gam5x(self, a):
+        a = np.array(a, dtype=np.float64, copy=True)
+        self.L.ref_gam5x(_p(a))
+        return a
+
+    # ---- solver build only -------------------------------------------
+    def vecdot(self, a, b):
+        out = np.zeros(2)
+        self.L.ref_vecdot(_p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(b)), _p(out))
+        return complex(out[0], out[1])
+
+    def solve(self, which, phi, u, coact, kappa, nk=(40, 10), resmax=1e-10, itermin=0,
+              x0=None, isignal=1):
+        """which: 0 bicgstab, 5 gmresdr, 6 psc (with isignal 1 or 2)."""
+        x = np.zeros(self.slen) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        ph = np.array(phi, dtype=np.float64, copy=True)
+        it = ctypes.c_int()
+        nkk = np.ascontiguousarray(nk, dtype=np.int32)
+        self.L.ref_solve(int(which), _p(ph), _p(x), _ip_(nkk), ctypes.c_double(resmax),
+                         int(itermin), ctypes.byref(it), _p(np.ascontiguousarray(u)),
+                         _p(np.ascontiguousarray(coact)),
+                         _p(np.ascontiguousarray(kappa, dtype=np.float64)), int(isignal))
+        return x, it.value
+
+    def fermprop(self, be, bo, u, coact, kappa, inverter, nk=(40, 10), resmax=1e-10, itermin=0,
+                 isignal=1, xe0=None):
+        xe = np.zeros(self.slen) if xe0 is None else np.array(xe0, dtype=np.float64, copy=True)
+        xo = np.zeros(self.slen)
+        b_e = np.array(be, dtype=np.float64, copy=True)
+        b_o = np.array(bo, dtype=np.float64, copy=True)
+        uu = np.array(u, dtype=np.float64, copy=True)
+        nkk = np.ascontiguousarray(nk, dtype=np.int32)
+        self.L.ref_fermprop(_p(b_e), _p(b_o), _p(np.ascontiguousarray(kappa, dtype=np.float64)),
+                            int(inverter), _p(np.ascontiguousarray(coact)), ctypes.c_double(resmax),
+                            int(itermin), _ip_(nkk), _p(uu), _p(xe), _p(xo), int(isignal))
+        return xe, xo
+
+    def pointsource(self, src, icsrc, idsrc):
+        be = np.zeros(self.slen)
+        bo = np.zeros(self.slen)
+        s = np.ascontiguousarray(src, dtype=np.int32)
+        self.L.ref_pointsource(_p(be), _p(bo), _ip_(s), int(icsrc), int(idsrc))
+        return be, bo
+
+    def heatbath(self, u, coact, nsweep, nhit=1, iseed=1):
+        uu = np.array(u, dtype=np.float64, copy=True)
+        self.L.ref_heatbath(_p(uu), _p(np.ascontiguousarray(coact)), int(nsweep), int(nhit), int(iseed))
+        return uu
+
+    def aveplaq(self, u):
+        uu = np.array(u, dtype=np.float64, copy=True)
+        p = ctypes.c_double()
+        self.L.ref_aveplaq(_p(uu), ctypes.byref(p))
+        return p.value
+
+    def fixaction(self, beta):
+        co = np.zeros(64)
+        self.L.ref_fixaction(ctypes.c_double(beta), _p(co))
+        return co
