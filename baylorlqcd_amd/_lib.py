@@ -1,0 +1,98 @@
+"""Loader of the HIP library ``baylorlqcd_amd/libqqcdhip.so``.
+
+The library is built in-tree (``make -C baylorlqcd_amd/csrc``) and loaded
+from this folder only.  There is no fallback: if the library or a GPU is
+missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqqcdhip.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+_L = None
+
+# (name, argtypes) of every exported entry point of include/qqcd_hip.h
+SIGNATURES = {
+    "qqcd_init": [_ip, _ip, ctypes.c_int],
+    "qqcd_finalize": [],
+    "qqcd_last_error": [],
+    "qqcd_ref_sizes": [_lp, _lp],
+    "qqcd_gauge_load": [_dp, _dp],
+    "qqcd_gauge_load_canonical": [_dp, _dp],
+    "qqcd_hsingle": [_dp, _dp, _dp, ctypes.c_int, _dp, _ip, ctypes.c_int],
+    "qqcd_hdouble": [_dp, _dp, _dp, ctypes.c_int, _dp, ctypes.c_double, _ip],
+    "qqcd_hdbletm": [_dp, _dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip],
+    "qqcd_gammamult": [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qqcd_gam5x": [_dp],
+    "qqcd_vecdot": [_dp, _dp, _dp],
+    "qqcd_pointsource": [_dp, _dp, _ip, ctypes.c_int, ctypes.c_int],
+    "qqcd_gmresdr": [_dp, _dp, _ip, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int, _ip],
+    "qqcd_gmresproject": [_dp, _dp, _ip, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int, _ip],
+    "qqcd_psc": [_dp, _dp, _ip, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int, _ip,
+                 ctypes.c_int, ctypes.c_double],
+    "qqcd_bicgstab": [_dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int, _ip],
+    "qqcd_fermprop": [_dp, _dp, _dp, ctypes.c_int, _dp, _ip, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int,
+                      _dp, _dp, _dp, ctypes.c_int, _ip],
+    "qqcd_bench_dslash": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp],
+    "qqcd_hop_canonical": [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qqcd_schur_canonical": [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qqcd_set_xcd_remap": [ctypes.c_int],
+    "qqcd_gauge_random": [ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp],
+    "qqcd_gauge_get_canonical": [_dp],
+}
+
+
+class QQCDError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library; raises if it was not built."""
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            raise QQCDError(f"{LIB_PATH} is missing: build it with `make -C baylorlqcd_amd/csrc` "
+                            "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.argtypes = args
+            f.restype = ctypes.c_char_p if name == "qqcd_last_error" else ctypes.c_int
+        _L = L
+    return _L
+
+
+def check(rc):
+    if rc != 0:
+        raise QQCDError(lib().qqcd_last_error().decode())
+    return rc
+
+
+def dp(a):
+    if a is None:
+        return ctypes.cast(None, _dp)
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and
+            (a.flags.c_contiguous or a.flags.f_contiguous)):
+        raise TypeError("expected a contiguous float64 array")
+    return a.ctypes.data_as(_dp)
+
+
+def ip(a):
+    if a is None:
+        return ctypes.cast(None, _ip)
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a.ctypes.data_as(_ip), a
+
+
+def ints(v):
+    return np.ascontiguousarray(v, dtype=np.int32)

@@ -1,0 +1,56 @@
+// qqcd_blas.h -- level-1 operations on device vectors (see qqcd_blas.hip).
+#pragma once
+#include <complex>
+#include "qqcd_internal.h"
+
+namespace qqcd {
+
+// A flat run of n complex numbers in FP64 or FP32.
+struct Vec {
+  void *data = nullptr;
+  Prec prec = PREC_DOUBLE;
+  long n = 0;
+  size_t bytes() const { return (size_t)n * (prec == PREC_DOUBLE ? 16 : 8); }
+};
+
+inline Vec as_vec(const Spinor &s) {
+  Vec v;
+  v.data = s.data;
+  v.prec = s.prec;
+  v.n = 12 * s.stride;
+  return v;
+}
+
+class Blas {
+ public:
+  explicit Blas(hipStream_t s);
+  ~Blas();
+  Blas(const Blas &) = delete;
+  Blas &operator=(const Blas &) = delete;
+
+  // out[i] = <v_i, w> = sum conj(v_i) w
+  void multi_dot(int k, const Vec *v, const Vec &w, std::complex<double> *out);
+  std::complex<double> dot(const Vec &a, const Vec &b);
+  double norm2(const Vec &a);
+  // w -= sum_i h_i v_i ; returns ||w||^2 when want_norm
+  double multi_axpy(int k, const std::complex<double> *h, const Vec *v, Vec &w, bool want_norm);
+  // y = a x + b y ; returns ||y||^2 when want_norm
+  double axpby(std::complex<double> a, const Vec &x, std::complex<double> b, Vec &y, bool want_norm = false);
+  void copy(const Vec &x, Vec &y);
+  void zero(Vec &x);
+  // Y_j (+)= sum_{i<m} V_i Z(i, j), j < k; Z column-major with leading dim ldz
+  void combine(int m, const Vec *V, int k, const std::complex<double> *Z, int ldz, Vec *Y, bool accumulate);
+
+  hipStream_t stream;
+
+ private:
+  void fetch(int nq);
+  double *part = nullptr;
+  double *res = nullptr;
+  double *hres = nullptr;
+  double *zbuf = nullptr;
+  void **vptr = nullptr;
+  static constexpr long zcap = 128 * 128;
+};
+
+}  // namespace qqcd

@@ -1,0 +1,86 @@
+// qqcd_context.h -- the library's single per-process state.
+//
+// One context per process, one HIP stream, synchronous at every C entry
+// point: the reference is single-threaded per rank and keeps its solver
+// state in module globals (qqcd-test/cfgsprops/quark/shift.f90:20-45,
+// gmresrhs.f90:10), and the library mirrors that contract.
+#pragma once
+#include <map>
+#include <memory>
+#include <vector>
+#include "qqcd_blas.h"
+
+namespace qqcd {
+
+struct Context {
+  Geom g{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<Blas> blas;
+
+  // master gauge field: FP64, 18 reals, resident
+  Gauge U64;
+  uint64_t gauge_gen = 0;  // bumped on every load
+  const void *u_host = nullptr;
+  double coact[64] = {0};
+  int have_gauge = 0;
+  // lower-precision / compressed copies, keyed by prec*100+recon
+  struct Copy {
+    Gauge U;
+    uint64_t gen = ~0ull;
+  };
+  std::map<int, Copy> copies;
+
+  // staging for reference-layout spinors (6*ntotal*64 doubles each)
+  double *dref[3] = {nullptr, nullptr, nullptr};
+
+  // deflation space carried from the first RHS to later ones (the
+  // reference's shift::vtemp / hcnew, qqcd-test/cfgsprops/quark/shift.f90)
+  std::vector<Spinor> defl_V;      // k+1 vectors (FP64)
+  std::vector<std::complex<double>> defl_H;  // (k+1) x k column-major
+  int defl_k = 0;
+  uint64_t defl_gen = 0;
+  double defl_kappa[2] = {0, 0};
+
+  ~Context();
+  const Gauge &gauge(Prec p, int recon);
+};
+
+extern Context *g_ctx;
+Context &ctx();
+
+Spinor alloc_spinor(const Geom &g, Prec p);
+void free_spinor(Spinor &s);
+Gauge alloc_gauge(const Geom &g, Prec p, int recon);
+void free_gauge(Gauge &u);
+
+// scratch spinors that free themselves
+struct SpinorBuf {
+  Spinor s;
+  SpinorBuf(const Geom &g, Prec p) : s(alloc_spinor(g, p)) {}
+  ~SpinorBuf() { free_spinor(s); }
+  SpinorBuf(const SpinorBuf &) = delete;
+  SpinorBuf &operator=(const SpinorBuf &) = delete;
+  SpinorBuf(SpinorBuf &&o) noexcept : s(o.s) { o.s.data = nullptr; o.s.norm = nullptr; }
+};
+
+// geometry updates from the reference arguments
+void set_coact(Context &c, const double *coact);
+void set_bc(Context &c, const int *bc);
+void ensure_gauge(Context &c, const double *u_ref, const double *coact);
+
+// Schur operators on device spinors of checkerboard 0:
+//   Wilson   (mu < 1e-8): out = in/k^2 - H_eo H_oe in
+//   twisted  (a=2 k mu):  out = (1 + a i g5) in/k^2 - H_eo (1 - a i g5) H_oe in/(1+a^2)
+// (Hdouble / Hdbletm, qqcd-test/cfgsprops/quark/diracops.f90:264-346, 435-550)
+void schur_apply(Context &c, const Gauge &U, const double kappa[2], int idag, const Spinor &in, Spinor &out,
+                 Spinor &tmp_odd);
+
+void canon_to_spinor(const Geom &g, const double *dcn, Spinor &out, int par, hipStream_t s);
+void spinor_to_canon(const Geom &g, const Spinor &in, double *dcn, int par, hipStream_t s);
+void gamma_ref(const Geom &g, const double *in, double *out, int mu, int nslices, const int *slices,
+               hipStream_t s);
+void random_spinor(const Geom &g, Spinor &s, uint64_t seed, hipStream_t st);
+void set_xcd_remap(int v);
+
+}  // namespace qqcd

@@ -1,0 +1,439 @@
+// qqcd_dslash.hip -- the Wilson hopping operator on one checkerboard.
+//
+// Computes, for every site x of checkerboard par_out,
+//   D(x) = sum_mu [ (c1 - s c2 g_mu) U_mu(x) psi(x+mu)
+//                 + (c1 + s c2 g_mu) U_mu(x-mu)^dag psi(x-mu) ]
+// with s = +1 (idag=0) or -1 (idag=1), hops across the lattice edge in mu
+// scaled by bc[mu], and then the local epilogue
+//   out = a0 x + a1 (i g5 x) + b0 D + b1 (i g5 D).
+// This is the reference's Hsingle (qqcd-test/cfgsprops/quark/diracops.f90:
+// 554-599; mulbac :950, mulfor :1163, boundary signs :1684-1694, 1730-1740)
+// and, through the epilogue, the local terms of Hdouble (:340-344) and
+// Hdbletm (:484-514).
+//
+// MI355X mapping: one lane per output site, 256-lane workgroups, sites in
+// x-fastest order so neighbour loads of a wavefront are contiguous runs.  The
+// projectors (1 -/+ g_mu) have rank 2 in the reference's gamma basis
+// (diracops.f90:61-71), so every hop multiplies a 2-spinor (12 reals) by the
+// link.  In that basis (1 +/- g_4) touches only the upper or the lower spin
+// pair, so the t hops load half of the neighbour spinor.  The link loads,
+// spinor loads and the FP16 per-site norm are all coalesced 64-lane runs.
+// Arithmetic is FP64 for FP64 storage and FP32 for FP32 and FP16 storage.
+#include <stdexcept>
+#include "qqcd_internal.h"
+
+namespace qqcd {
+
+template <typename S> struct RealOf { typedef float type; };
+template <> struct RealOf<double> { typedef double type; };
+
+// ---- storage access --------------------------------------------------
+template <typename S> struct SIO;
+
+template <> struct SIO<double> {
+  typedef double R;
+  static __device__ __forceinline__ cx<R> ld(const void *b, long stride, int k, int i, float) {
+    const double2 v = ((const double2 *)b)[(long)k * stride + i];
+    return cx<R>{v.x, v.y};
+  }
+  static __device__ __forceinline__ float norm(const float *, int) { return 1.f; }
+  static __device__ __forceinline__ void st(void *b, float *, long stride, int i, const cx<R> *v) {
+    double2 *p = (double2 *)b;
+#pragma unroll
+    for (int k = 0; k < 12; k++) p[(long)k * stride + i] = make_double2(v[k].re, v[k].im);
+  }
+};
+
+template <> struct SIO<float> {
+  typedef float R;
+  static __device__ __forceinline__ cx<R> ld(const void *b, long stride, int k, int i, float) {
+    const float2 v = ((const float2 *)b)[(long)k * stride + i];
+    return cx<R>{v.x, v.y};
+  }
+  static __device__ __forceinline__ float norm(const float *, int) { return 1.f; }
+  static __device__ __forceinline__ void st(void *b, float *, long stride, int i, const cx<R> *v) {
+    float2 *p = (float2 *)b;
+#pragma unroll
+    for (int k = 0; k < 12; k++) p[(long)k * stride + i] = make_float2(v[k].re, v[k].im);
+  }
+};
+
+typedef f16 h2 __attribute__((ext_vector_type(2)));
+
+template <> struct SIO<f16> {
+  typedef float R;
+  static __device__ __forceinline__ cx<R> ld(const void *b, long stride, int k, int i, float nrm) {
+    const h2 v = ((const h2 *)b)[(long)k * stride + i];
+    return cx<R>{nrm * (float)v.x, nrm * (float)v.y};
+  }
+  static __device__ __forceinline__ float norm(const float *n, int i) { return n[i]; }
+  static __device__ __forceinline__ void st(void *b, float *nrm, long stride, int i, const cx<R> *v) {
+    float m = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
+    nrm[i] = m;
+    const float inv = m > 0.f ? 1.f / m : 0.f;
+    h2 *p = (h2 *)b;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      h2 w;
+      w.x = (f16)(v[k].re * inv);
+      w.y = (f16)(v[k].im * inv);
+      p[(long)k * stride + i] = w;
+    }
+  }
+};
+
+// links: component run (par, mu, k) at ((par*4 + mu)*NC + k)*stride
+template <typename L, int RECON> struct LIO;
+
+template <typename L> __device__ __forceinline__ cx<typename RealOf<L>::type> ldc(const void *b, long off);
+template <> __device__ __forceinline__ cx<double> ldc<double>(const void *b, long off) {
+  const double2 v = ((const double2 *)b)[off];
+  return cx<double>{v.x, v.y};
+}
+template <> __device__ __forceinline__ cx<float> ldc<float>(const void *b, long off) {
+  const float2 v = ((const float2 *)b)[off];
+  return cx<float>{v.x, v.y};
+}
+template <> __device__ __forceinline__ cx<float> ldc<f16>(const void *b, long off) {
+  const h2 v = ((const h2 *)b)[off];
+  return cx<float>{(float)v.x, (float)v.y};
+}
+
+template <typename L, int RECON>
+__device__ __forceinline__ void load_link(const void *b, long stride, int par, int mu, int i,
+                                          cx<typename RealOf<L>::type> u[9]) {
+  typedef typename RealOf<L>::type R;
+  constexpr int NC = RECON == 18 ? 9 : 6;
+  const long base = (long)((par * 4 + mu) * NC) * stride + i;
+#pragma unroll
+  for (int k = 0; k < NC; k++) u[k] = ldc<L>(b, base + (long)k * stride);
+  if (RECON == 12) {
+    // row 2 = conj(row 0 x row 1)
+    u[6] = mk<R>(u[1].re * u[5].re - u[1].im * u[5].im - (u[2].re * u[4].re - u[2].im * u[4].im),
+                 -(u[1].re * u[5].im + u[1].im * u[5].re - (u[2].re * u[4].im + u[2].im * u[4].re)));
+    u[7] = mk<R>(u[2].re * u[3].re - u[2].im * u[3].im - (u[0].re * u[5].re - u[0].im * u[5].im),
+                 -(u[2].re * u[3].im + u[2].im * u[3].re - (u[0].re * u[5].im + u[0].im * u[5].re)));
+    u[8] = mk<R>(u[0].re * u[4].re - u[0].im * u[4].im - (u[1].re * u[3].re - u[1].im * u[3].im),
+                 -(u[0].re * u[4].im + u[0].im * u[4].re - (u[1].re * u[3].im + u[1].im * u[3].re)));
+  }
+}
+
+template <typename R> __device__ __forceinline__ void mv(const cx<R> u[9], const cx<R> *h, cx<R> *w) {
+#pragma unroll
+  for (int r = 0; r < 3; r++) w[r] = cmul(u[3 * r], h[0]) + cmul(u[3 * r + 1], h[1]) + cmul(u[3 * r + 2], h[2]);
+}
+template <typename R> __device__ __forceinline__ void mdv(const cx<R> u[9], const cx<R> *h, cx<R> *w) {
+#pragma unroll
+  for (int r = 0; r < 3; r++) w[r] = cjmul(u[r], h[0]) + cjmul(u[3 + r], h[1]) + cjmul(u[6 + r], h[2]);
+}
+
+// Spin projection (1 + sig g_mu) psi -> (h0, h1) [2 spins x 3 colours], and
+// reconstruction of the 4 spins of (1 + sig g_mu) U psi from U h.
+// 0-based spins; gamma basis of diracops.f90:61-71:
+//   mu=0: h0 = p0 + sig p3, h1 = p1 + sig p2;   r = (h0, h1, sig h1, sig h0)
+//   mu=1: h0 = p0 - i sig p3, h1 = p1 + i sig p2; r = (h0, h1, -i sig h1, i sig h0)
+//   mu=2: h0 = p0 + sig p2, h1 = p1 - sig p3;   r = (h0, h1, sig h0, -sig h1)
+//   mu=3: sig=+1: h = 2(p0, p1), r = (h0, h1, 0, 0); sig=-1: h = 2(p2, p3), r = (0, 0, h0, h1)
+template <int MU, typename R>
+__device__ __forceinline__ void project(const cx<R> p[12], R sig, cx<R> h[6]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (MU == 0) {
+      h[c] = p[c] + sig * p[9 + c];
+      h[3 + c] = p[3 + c] + sig * p[6 + c];
+    } else if (MU == 1) {
+      h[c] = p[c] - sig * mul_i(p[9 + c]);
+      h[3 + c] = p[3 + c] + sig * mul_i(p[6 + c]);
+    } else if (MU == 2) {
+      h[c] = p[c] + sig * p[6 + c];
+      h[3 + c] = p[3 + c] - sig * p[9 + c];
+    }
+  }
+}
+
+template <int MU, typename R>
+__device__ __forceinline__ void accum(cx<R> acc[12], const cx<R> w[6], R sig, R scl) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const cx<R> a = scl * w[c], b = scl * w[3 + c];
+    if (MU == 0) {
+      acc[c] = acc[c] + a;
+      acc[3 + c] = acc[3 + c] + b;
+      acc[6 + c] = acc[6 + c] + sig * b;
+      acc[9 + c] = acc[9 + c] + sig * a;
+    } else if (MU == 1) {
+      acc[c] = acc[c] + a;
+      acc[3 + c] = acc[3 + c] + b;
+      acc[6 + c] = acc[6 + c] - sig * mul_i(b);
+      acc[9 + c] = acc[9 + c] + sig * mul_i(a);
+    } else if (MU == 2) {
+      acc[c] = acc[c] + a;
+      acc[3 + c] = acc[3 + c] + b;
+      acc[6 + c] = acc[6 + c] + sig * a;
+      acc[9 + c] = acc[9 + c] - sig * b;
+    }
+  }
+}
+
+struct DslashArgs {
+  const void *in;
+  const float *in_norm;
+  void *out;
+  float *out_norm;
+  const void *x;
+  const float *x_norm;
+  const void *links;
+  long sstride;  // spinor component stride
+  long lstride;  // link component stride
+  int n[4];
+  int Vh;
+  int par;       // output checkerboard
+  float bcf[4];  // edge factor per direction
+  // projector form: sig_f (forward projector sign), scl (c1)
+  float sig_f[4];
+  double scl[4];
+  // split form
+  double fw_p[4], fw_m[4], bw_p[4], bw_m[4];
+  double a0, a1, b0, b1;
+  int has_x;
+  int xcd_remap;
+};
+
+// One hop in direction MU from neighbour spinor at checkerboard index j
+// (opposite parity), with link (lpar, li).  FWD: U h ; else U^dag h.
+template <typename S, typename L, int RECON, int MU, bool FWD, typename R>
+__device__ __forceinline__ void hop(const DslashArgs &a, cx<R> acc[12], int j, int lpar, int li, R sig,
+                                    R scl, R edge) {
+  cx<R> u[9];
+  load_link<L, RECON>(a.links, a.lstride, lpar, MU, li, u);
+  const float nrm = SIO<S>::norm(a.in_norm, j);
+  cx<R> h[6], w[6];
+  if (MU == 3) {
+    // (1 + sig g_4) keeps the upper (sig=+1) or lower (sig=-1) spin pair, doubled
+    const int s0 = sig > 0 ? 0 : 6;
+#pragma unroll
+    for (int c = 0; c < 6; c++) h[c] = R(2) * SIO<S>::ld(a.in, a.sstride, s0 + c, j, nrm);
+  } else {
+    cx<R> p[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) p[k] = SIO<S>::ld(a.in, a.sstride, k, j, nrm);
+    project<MU>(p, sig, h);
+  }
+  if (FWD) {
+    mv(u, h, w);
+    mv(u, h + 3, w + 3);
+  } else {
+    mdv(u, h, w);
+    mdv(u, h + 3, w + 3);
+  }
+  const R f = scl * edge;
+  if (MU == 3) {
+    const int s0 = sig > 0 ? 0 : 6;
+    if (s0 == 0) {
+#pragma unroll
+      for (int c = 0; c < 6; c++) acc[c] = acc[c] + f * w[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 6; c++) acc[6 + c] = acc[6 + c] + f * w[c];
+    }
+  } else {
+    accum<MU>(acc, w, sig, f);
+  }
+}
+
+template <typename S, typename L, int RECON, bool PROJ, int MU, typename R>
+__device__ __forceinline__ void direction(const DslashArgs &a, cx<R> acc[12], int i, int jf, int jb, R ef,
+                                          R eb) {
+  const int opar = a.par ^ 1;
+  if (PROJ) {
+    const R sf = (R)a.sig_f[MU];
+    const R sc = (R)a.scl[MU];
+    hop<S, L, RECON, MU, true>(a, acc, jf, a.par, i, sf, sc, ef);
+    hop<S, L, RECON, MU, false>(a, acc, jb, opar, jb, -sf, sc, eb);
+  } else {
+    if (a.fw_p[MU] != 0.0) hop<S, L, RECON, MU, true>(a, acc, jf, a.par, i, R(1), (R)a.fw_p[MU], ef);
+    if (a.fw_m[MU] != 0.0) hop<S, L, RECON, MU, true>(a, acc, jf, a.par, i, R(-1), (R)a.fw_m[MU], ef);
+    if (a.bw_p[MU] != 0.0) hop<S, L, RECON, MU, false>(a, acc, jb, opar, jb, R(1), (R)a.bw_p[MU], eb);
+    if (a.bw_m[MU] != 0.0) hop<S, L, RECON, MU, false>(a, acc, jb, opar, jb, R(-1), (R)a.bw_m[MU], eb);
+  }
+}
+
+template <typename S, typename L, int RECON, bool PROJ>
+__global__ void __launch_bounds__(256) k_dslash(const DslashArgs a) {
+  typedef typename SIO<S>::R R;
+  int blk = blockIdx.x;
+  if (a.xcd_remap) {
+    // workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+    // contiguous run of sites so neighbour reuse stays in its own L2
+    const int nb = gridDim.x, per = (nb + 7) / 8;
+    const int xcd = blk & 7, k = blk >> 3;
+    const int full = nb - 8 * (per - 1);  // XCDs that get `per` blocks
+    blk = xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
+  }
+  const int i = blk * blockDim.x + threadIdx.x;
+  if (i >= a.Vh) return;
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2], nt = a.n[3];
+  const int r = 2 * i;
+  const int x0 = r % nx;
+  int q = r / nx;
+  const int y = q % ny;
+  q /= ny;
+  const int z = q % nz;
+  const int t = q / nz;
+  const int x = x0 + ((a.par + y + z + t) & 1);
+  const int yzt = y + ny * (z + nz * t);
+
+  cx<R> acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) acc[k] = mk<R>(0, 0);
+
+  {  // x
+    const int xf = x + 1 == nx ? 0 : x + 1, xb = x == 0 ? nx - 1 : x - 1;
+    const R ef = x + 1 == nx ? (R)a.bcf[0] : R(1), eb = x == 0 ? (R)a.bcf[0] : R(1);
+    direction<S, L, RECON, PROJ, 0>(a, acc, i, (xf + nx * yzt) >> 1, (xb + nx * yzt) >> 1, ef, eb);
+  }
+  {  // y
+    const int yf = y + 1 == ny ? 0 : y + 1, yb = y == 0 ? ny - 1 : y - 1;
+    const R ef = y + 1 == ny ? (R)a.bcf[1] : R(1), eb = y == 0 ? (R)a.bcf[1] : R(1);
+    const int zt = z + nz * t;
+    direction<S, L, RECON, PROJ, 1>(a, acc, i, (x + nx * (yf + ny * zt)) >> 1, (x + nx * (yb + ny * zt)) >> 1,
+                                    ef, eb);
+  }
+  {  // z
+    const int zf = z + 1 == nz ? 0 : z + 1, zb = z == 0 ? nz - 1 : z - 1;
+    const R ef = z + 1 == nz ? (R)a.bcf[2] : R(1), eb = z == 0 ? (R)a.bcf[2] : R(1);
+    direction<S, L, RECON, PROJ, 2>(a, acc, i, (x + nx * (y + ny * (zf + nz * t))) >> 1,
+                                    (x + nx * (y + ny * (zb + nz * t))) >> 1, ef, eb);
+  }
+  {  // t
+    const int tf = t + 1 == nt ? 0 : t + 1, tb = t == 0 ? nt - 1 : t - 1;
+    const R ef = t + 1 == nt ? (R)a.bcf[3] : R(1), eb = t == 0 ? (R)a.bcf[3] : R(1);
+    direction<S, L, RECON, PROJ, 3>(a, acc, i, (x + nx * (y + ny * (z + nz * tf))) >> 1,
+                                    (x + nx * (y + ny * (z + nz * tb))) >> 1, ef, eb);
+  }
+
+  // epilogue: out = a0 x + a1 (i g5 x) + b0 D + b1 (i g5 D);  i g5 v = (v2, v3, -v0, -v1)
+  const R b0 = (R)a.b0, b1 = (R)a.b1;
+  cx<R> o[12];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    o[c] = b0 * acc[c] + b1 * acc[6 + c];
+    o[3 + c] = b0 * acc[3 + c] + b1 * acc[9 + c];
+    o[6 + c] = b0 * acc[6 + c] - b1 * acc[c];
+    o[9 + c] = b0 * acc[9 + c] - b1 * acc[3 + c];
+  }
+  if (a.has_x) {
+    const R a0 = (R)a.a0, a1 = (R)a.a1;
+    const float nrm = SIO<S>::norm(a.x_norm, i);
+    cx<R> xv[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) xv[k] = SIO<S>::ld(a.x, a.sstride, k, i, nrm);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      o[c] = o[c] + a0 * xv[c] + a1 * xv[6 + c];
+      o[3 + c] = o[3 + c] + a0 * xv[3 + c] + a1 * xv[9 + c];
+      o[6 + c] = o[6 + c] + a0 * xv[6 + c] - a1 * xv[c];
+      o[9 + c] = o[9 + c] + a0 * xv[9 + c] - a1 * xv[3 + c];
+    }
+  }
+  SIO<S>::st(a.out, a.out_norm, a.sstride, i, o);
+}
+
+HopCoef make_hopcoef(const Geom &g, int idag) {
+  HopCoef h{};
+  const double s = idag ? -1.0 : 1.0;
+  h.proj = 1;
+  for (int mu = 0; mu < 4; mu++) {
+    const double c1 = g.c1[mu], c2 = s * g.c2[mu];
+    h.fw_p[mu] = 0.5 * (c1 - c2);
+    h.fw_m[mu] = 0.5 * (c1 + c2);
+    h.bw_p[mu] = 0.5 * (c1 + c2);
+    h.bw_m[mu] = 0.5 * (c1 - c2);
+    if (h.fw_p[mu] != 0.0 && h.fw_m[mu] != 0.0) h.proj = 0;
+  }
+  return h;
+}
+
+static int g_xcd_remap = 1;
+void set_xcd_remap(int v) { g_xcd_remap = v; }
+
+template <typename S, typename L, int RECON>
+static void launch_t(const DslashArgs &a, bool proj, hipStream_t s) {
+  const int nb = (a.Vh + 255) / 256;
+  if (proj)
+    hipLaunchKernelGGL((k_dslash<S, L, RECON, true>), dim3(nb), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_dslash<S, L, RECON, false>), dim3(nb), dim3(256), 0, s, a);
+  QQCD_HIP(hipGetLastError());
+}
+
+void dslash(const Geom &g, const Gauge &U, const Spinor &in, Spinor &out, int par_out, int idag,
+            const Epilogue &ep, const Spinor *x, hipStream_t s) {
+  if (in.prec != out.prec || (x && x->prec != out.prec))
+    throw std::runtime_error("dslash: spinor precisions differ");
+  if (in.stride != g.stride || out.stride != g.stride || U.stride != g.stride)
+    throw std::runtime_error("dslash: field stride does not match the geometry");
+  if (in.data == out.data) throw std::runtime_error("dslash: in and out alias");
+  DslashArgs a{};
+  a.in = in.data;
+  a.in_norm = in.norm;
+  a.out = out.data;
+  a.out_norm = out.norm;
+  a.x = x ? x->data : nullptr;
+  a.x_norm = x ? x->norm : nullptr;
+  a.has_x = x ? 1 : 0;
+  a.links = U.data;
+  a.sstride = g.stride;
+  a.lstride = U.stride;
+  for (int m = 0; m < 4; m++) {
+    a.n[m] = g.n[m];
+    a.bcf[m] = (float)g.bc[m];
+  }
+  a.Vh = (int)g.Vh;
+  a.par = par_out;
+  HopCoef h = make_hopcoef(g, idag);
+  for (int m = 0; m < 4; m++) {
+    a.fw_p[m] = h.fw_p[m];
+    a.fw_m[m] = h.fw_m[m];
+    a.bw_p[m] = h.bw_p[m];
+    a.bw_m[m] = h.bw_m[m];
+    // projector form: forward is c1 (1 + sig g) with sig = +1 when fw_m == 0
+    a.sig_f[m] = h.fw_m[m] == 0.0 ? 1.f : -1.f;
+    a.scl[m] = h.fw_p[m] + h.fw_m[m];
+  }
+  a.a0 = ep.a0;
+  a.a1 = ep.a1;
+  a.b0 = ep.b0;
+  a.b1 = ep.b1;
+  a.xcd_remap = g_xcd_remap;
+  const bool proj = h.proj != 0;
+  if (in.prec == PREC_DOUBLE) {
+    if (U.prec != PREC_DOUBLE) throw std::runtime_error("dslash: FP64 spinors need FP64 links");
+    if (U.recon == 18) launch_t<double, double, 18>(a, proj, s);
+    else launch_t<double, double, 12>(a, proj, s);
+  } else if (in.prec == PREC_SINGLE) {
+    if (U.prec == PREC_SINGLE) {
+      if (U.recon == 18) launch_t<float, float, 18>(a, proj, s);
+      else launch_t<float, float, 12>(a, proj, s);
+    } else if (U.prec == PREC_HALF) {
+      if (U.recon == 18) launch_t<float, f16, 18>(a, proj, s);
+      else launch_t<float, f16, 12>(a, proj, s);
+    } else {
+      throw std::runtime_error("dslash: FP32 spinors need FP32 or FP16 links");
+    }
+  } else {
+    if (U.prec == PREC_HALF) {
+      if (U.recon == 18) launch_t<f16, f16, 18>(a, proj, s);
+      else launch_t<f16, f16, 12>(a, proj, s);
+    } else if (U.prec == PREC_SINGLE) {
+      if (U.recon == 18) launch_t<f16, float, 18>(a, proj, s);
+      else launch_t<f16, float, 12>(a, proj, s);
+    } else {
+      throw std::runtime_error("dslash: FP16 spinors need FP32 or FP16 links");
+    }
+  }
+}
+
+}  // namespace qqcd

@@ -1,0 +1,114 @@
+// qqcd_internal.h -- shared definitions of the MI355X solve-path library.
+//
+// Device data layout (all fields resident in HBM):
+//   * even-odd: a field lives on one global checkerboard (parity p = 0 for
+//     x+y+z+t even -- the reference's gblclr=1 -- and p = 1 for odd);
+//   * site order inside a checkerboard: cb = (x + nx*(y + ny*(z + nz*t))) / 2,
+//     so consecutive lanes of a wavefront hold x-neighbouring sites;
+//   * spinors are site-fastest structure-of-arrays: component k = 3*spin +
+//     colour (12 complex components) is a contiguous run of `stride` complex
+//     numbers, so each of the 12 loads of a wavefront is one contiguous
+//     64-lane transaction.  FP16 spinors carry one float norm per site;
+//   * links per (parity, mu): 9 (18-real) or 6 (12-real: rows 0 and 1, row 2
+//     rebuilt as conj(row0 x row1)) complex component runs of `stride`.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+namespace qqcd {
+
+enum Prec { PREC_DOUBLE = 0, PREC_SINGLE = 1, PREC_HALF = 2 };
+enum Recon { RECON_18 = 18, RECON_12 = 12 };
+
+typedef _Float16 f16;
+
+template <typename T> struct cx { T re, im; };
+
+template <typename T> __host__ __device__ inline cx<T> mk(T a, T b) { return cx<T>{a, b}; }
+template <typename T> __host__ __device__ inline cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T> __host__ __device__ inline cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T> __host__ __device__ inline cx<T> operator*(T s, cx<T> a) { return {s * a.re, s * a.im}; }
+template <typename T> __host__ __device__ inline cx<T> cmul(cx<T> a, cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// conj(a) * b
+template <typename T> __host__ __device__ inline cx<T> cjmul(cx<T> a, cx<T> b) {
+  return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+// i * a
+template <typename T> __host__ __device__ inline cx<T> mul_i(cx<T> a) { return {-a.im, a.re}; }
+
+// Lattice geometry of one process / one GPU.
+struct Geom {
+  int n[4];      // nx, ny, nz, nt (all even, nx divisible by 4 for the reference layout)
+  long V;        // sites
+  long Vh;       // sites per checkerboard
+  long stride;   // allocation stride of one component run (>= Vh, multiple of 64)
+  int bc[4];     // 1 periodic, -1 antiperiodic, 0 fixed (hop across the edge scaled by bc)
+  double c1[4];  // coact(3,1,mu)
+  double c2[4];  // coact(3,2,mu)
+};
+
+// A spinor field on one checkerboard, any precision.
+struct Spinor {
+  Prec prec = PREC_DOUBLE;
+  long stride = 0;
+  void *data = nullptr;   // 12 component runs
+  float *norm = nullptr;  // FP16 only: per-site scale
+  size_t bytes() const;
+};
+
+// Gauge field on the device: links[parity][mu] component runs.
+struct Gauge {
+  Prec prec = PREC_DOUBLE;
+  int recon = 18;
+  long stride = 0;
+  void *data = nullptr;
+  size_t bytes() const;
+};
+
+// Coefficients of the local spin matrix applied in a Dslash epilogue:
+//   out = a0*x + a1*(i g5 x) + b0*D + b1*(i g5 D)
+// with D the hopping sum and x a spinor on the output checkerboard.
+struct Epilogue {
+  double a0 = 0, a1 = 0, b0 = 1, b1 = 0;
+};
+
+// Per-direction hopping coefficients as the kernel consumes them.  In
+// projector form (|c1| == |c2|) the forward hop is c1 (1 - sig g_mu) U psi(x+mu)
+// and the backward hop c1 (1 + sig g_mu) U^dag psi(x-mu), sig = s*c2/c1.
+// Otherwise each hop is split into its (1+g) and (1-g) projections.
+struct HopCoef {
+  float dummy;
+  double fw_p[4], fw_m[4];  // forward:  fw_p (1+g) + fw_m (1-g)
+  double bw_p[4], bw_m[4];  // backward: bw_p (1+g) + bw_m (1-g)
+  int proj;                 // 1 if every direction is a single projector
+};
+HopCoef make_hopcoef(const Geom &g, int idag);
+
+// error handling: every entry point returns 0 or a negative code and stores
+// a message retrievable with qqcd_last_error().
+void set_error(const std::string &msg);
+#define QQCD_HIP(call)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      throw std::runtime_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                           \
+  } while (0)
+
+// ---- kernels (qqcd_dslash.hip) ----
+void dslash(const Geom &g, const Gauge &U, const Spinor &in, Spinor &out, int par_out, int idag,
+            const Epilogue &ep, const Spinor *x, hipStream_t s);
+
+// ---- layout (qqcd_layout.hip) ----
+long ref_nvhalf(const Geom &g);
+long ref_ntotal(const Geom &g);
+void ref_to_spinor(const Geom &g, const double *dref, Spinor &out, int par, hipStream_t s);
+void spinor_to_ref(const Geom &g, const Spinor &in, double *dref, int par, hipStream_t s);
+void ref_to_gauge(const Geom &g, const double *dref, Gauge &U, hipStream_t s);
+void canon_to_gauge(const Geom &g, const double *dcanon, Gauge &U, hipStream_t s);
+void spinor_copy_convert(const Geom &g, const Spinor &in, Spinor &out, hipStream_t s);
+
+}  // namespace qqcd

@@ -1,0 +1,175 @@
+/*
+ * qqcd_hip.h -- C ABI of the MI355X Wilson / twisted-mass solve path.
+ *
+ * Drop-in boundary for the Fortran host of BaylorLQCD "qqcd": the entry
+ * points below replace the reference's module procedures of the Dirac
+ * operator and inverter layer.  Arrays are the reference's own host arrays
+ * in its own storage order (Fortran, column-major):
+ *   spinor   real(8) g(6, ntotal, 4, 2, 8)     one global checkerboard
+ *   links    real(8) u(18, ntotal, 4, 2, 16)
+ *   coact    real(8) coact(4, 4, 4)            only coact(3,1:2,mu) is read
+ *   bc       integer(4) bc(4)                   1, -1 or 0 per direction
+ * with nvhalf = V/32, ntotal = nvhalf + max_mu(2 nvhalf / n_mu) for one
+ * process (qqcd-test/cfgsprops/basics.f90:20-30).  Rows nvhalf+1..ntotal are
+ * halo scratch in the reference and carry no data here.
+ *
+ * One context per process, one HIP stream, not thread-safe; every call is
+ * synchronous at return.  Every entry point returns 0 on success and a
+ * negative code on error; qqcd_last_error() returns the message.
+ *
+ * The geometry arguments of the reference procedures (vecbl, vecblinv,
+ * myid, nn, ldiv, nms, lvbc, ib, lbd, iblv, MRT) are not passed: the
+ * Fortran shim checks that they describe one process with the canonical
+ * block tables and drops them.
+ */
+#ifndef QQCD_HIP_H
+#define QQCD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- life cycle ------------------------------------------------------- */
+
+/* Set up the lattice n[4] = (nx, ny, nz, nt) with boundary conditions bc[4]
+ * on HIP device `device`.  Replaces the single-process part of initlat
+ * (qqcd-test/cfgsprops/glue/lattice.f90:33-59).  Requires every n even and
+ * nx, ny, nz, nt divisible by 4 (the reference's rule nx/4/npx integer,
+ * qqcd-test/user/latdims.f90:11-13). */
+int qqcd_init(const int n[4], const int bc[4], int device);
+int qqcd_finalize(void);
+const char *qqcd_last_error(void);
+/* sizes of the reference arrays: nvhalf, ntotal */
+int qqcd_ref_sizes(long *nvhalf, long *ntotal);
+
+/* Upload the link field u (reference order) and coact, and make it the
+ * resident field.  Operator and solver calls that pass the same u address
+ * reuse the resident copy; call this again after changing u in place
+ * (the reference reads u afresh in every Hsingle,
+ * qqcd-test/cfgsprops/quark/diracops.f90:554). */
+int qqcd_gauge_load(const double *u, const double *coact);
+/* Same, from canonical order U[((s*4 + mu)*3 + r)*3 + c] complex (re, im),
+ * s = x + nx*(y + ny*(z + nz*t)); for hosts that do not hold the reference
+ * layout. */
+int qqcd_gauge_load_canonical(const double *U, const double *coact);
+
+/* Seeded SU(3) field generated on the device and made resident (no host
+ * array): kind 0 Haar-random, kind 1 reunitarised 1 + eps*Gaussian.  The
+ * project's own generator; the reference reads fields from disk or makes
+ * them with its heatbath. */
+int qqcd_gauge_random(int kind, double eps, unsigned long long seed, const double *coact);
+/* Copy the resident FP64 field back in canonical order (see above). */
+int qqcd_gauge_get_canonical(double *U);
+
+/* ---- operators (qqcd-test/cfgsprops/quark/diracops.f90) ---------------- */
+
+/* Hsingle(gout,u,gin,idag,coact,bc,gblclr,...), diracops.f90:554-599:
+ * gout = H_eo gin (gblclr=1) or H_oe gin (gblclr=2); all of gout is written
+ * (zero outside rows 1..nvhalf).  idag=1 applies H^dagger = g5 H g5. */
+int qqcd_hsingle(double *gout, const double *u, const double *gin, int idag, const double *coact,
+                 const int *bc, int gblclr);
+
+/* Hdouble(he,u,ge,idag,coact,kappa,bc,...), diracops.f90:264-346:
+ * he = (1/kappa^2 - H_eo H_oe) ge on the gblclr=1 checkerboard. */
+int qqcd_hdouble(double *he, const double *u, const double *ge, int idag, const double *coact, double kappa,
+                 const int *bc);
+
+/* Hdbletm(he,u,GeeGooinv,ge,idag,coact,kappa,iflag,bc,...),
+ * diracops.f90:435-550, Wilson twisted mass (iflag = -1), a = 2 kappa(1)
+ * kappa(2): he = (1 + a i g5) ge / kappa(1)^2
+ *               - H_eo (1 - a i g5) H_oe ge / (1 + a^2);
+ * kappa(2) < 1e-8 (signed test, :467) falls back to Hdouble.  The clover
+ * case (iflag = -2) is not supported and returns an error. */
+int qqcd_hdbletm(double *he, const double *u, const double *ge, int idag, const double *coact,
+                 const double kappa[2], int iflag, const int *bc);
+
+/* gammamult(evenIN,oddIN,evenOUT,oddOUT,mu,ieo,ibleo), diracops.f90:105-195:
+ * gamma_mu (mu = 1..5) on rows 1..nvhalf of slice (ieo, ibleo) of both
+ * arrays; the rest of the outputs is left unchanged. */
+int qqcd_gammamult(const double *even_in, const double *odd_in, double *even_out, double *odd_out, int mu,
+                   int ieo, int ibleo);
+
+/* gam5x(a), qqcd-test/cfgsprops/basics.f90:408-454: a = gamma_5 a in place on
+ * rows 1..nvhalf of every slice. */
+int qqcd_gam5x(double *a);
+
+/* vecdot(a,b,adb,MRT2), qqcd-test/cfgsprops/quark/inverters.f90:13485-13530:
+ * adb = sum over rows 1..nvhalf of conj(a) b, as (re, im). */
+int qqcd_vecdot(const double *a, const double *b, double adb[2]);
+
+/* pointsource(be,bo,src,icsrc,idsrc,...), qqcd-test/cfgsprops/quark/
+ * quark.f90:46-108: unit source at 1-based global site src[4], colour icsrc,
+ * Dirac idsrc; be and bo are zeroed first. */
+int qqcd_pointsource(double *be, double *bo, const int src[4], int icsrc, int idsrc);
+
+/* ---- solvers (qqcd-test/cfgsprops/quark/inverters.f90) ----------------- */
+
+/* Common arguments: phi/b the even-odd right-hand side, x the solution
+ * (input: initial guess where the reference uses one), kappa[2] = (kappa,
+ * mu), iflag = -1.  Convergence is the reference's: ||r|| / ||r0|| < resmax
+ * on the even-odd system with the true residual r = phi - M x recomputed in
+ * FP64 at every cycle end.  *itercount receives the number of operator
+ * applications in the Krylov iterations, as the reference counts them. */
+
+/* gmresdr(rwdir,phi,x,GMRES,resmax,itermin,itercount,u,GeeGooinv,iflag,
+ * kappa,coact,bc,...), inverters.f90:2871-3572: GMRES-DR(m = GMRES[0],
+ * k = GMRES[1]), x0 as given.  Keeps the deflation space (V_{k+1}, Hbar_k)
+ * of the last cycle resident for qqcd_gmresproject / qqcd_psc. */
+int qqcd_gmresdr(const double *phi, double *x, const int GMRES[2], double resmax, int itermin, int *itercount,
+                 const double *u, const double *coact, const double kappa[2], int iflag, const int *bc);
+
+/* gmresproject(...), inverters.f90:7183-8304: GMRES-Proj(m - k, k) with
+ * the resident deflation space; x0 = 0 (:7347). */
+int qqcd_gmresproject(const double *b, double *x, const int GMRES[2], double resmax, int itermin,
+                      int *itercount, const double *u, const double *coact, const double kappa[2], int iflag,
+                      const int *bc);
+
+/* psc(...), inverters.f90:4310-4465: isignal = 1 runs GMRES-DR(m, k) and
+ * keeps its deflation space; otherwise GMRES-Proj(m-k, k) with the
+ * tolerance the reference hard-codes there (1e-8, :4442) unless
+ * resmax_proj > 0 is given. */
+int qqcd_psc(const double *b, double *x, const int GMRES[2], double resmax, int itermin, int *itercount,
+             const double *u, const double *coact, const double kappa[2], int iflag, const int *bc, int isignal,
+             double resmax_proj);
+
+/* bicgstab(...), inverters.f90:302-487, from the given x. */
+int qqcd_bicgstab(const double *phi, double *x, double resmax, int itermin, int *itercount, const double *u,
+                  const double *coact, const double kappa[2], int iflag, const int *bc);
+
+/* fermprop(...) for nkappa = -1 (Wilson / twisted), qqcd-test/cfgsprops/
+ * quark/quark.f90:458-842: even-odd source preparation, solve with
+ * inverter 0 (BiCGStab from the constant start of :632-640 when
+ * vectype == 1, else from xe), 5 (GMRES-DR) or 6 (psc with isignal), and
+ * odd-site reconstruction.  nGMRES = (m, k).  ntmqcd rotations are not
+ * applied (ntmqcd = 0). */
+int qqcd_fermprop(const double *be, const double *bo, const double kappa[2], int inverter, const double *coact,
+                  const int *bc, double resmax, int itermin, const int nGMRES[2], int vectype, const double *u,
+                  double *xe, double *xo, int isignal, int *itercount);
+
+/* ---- native (device-resident) interface: benchmarks and tests ---------- */
+
+/* prec: 0 FP64, 1 FP32, 2 FP16 storage; recon: 18 or 12. */
+
+/* Time `niter` back-to-back Dslash launches (after `nwarm`) on seeded random
+ * spinors with the resident gauge field in precision (prec, recon).  Returns
+ * the average time of one launch in *ms (HIP events on the library stream)
+ * and the relative error of the result against the FP64/18 Dslash of the
+ * same input in *relerr.  which = 0: one hop H_oe; 1: the full Schur
+ * operator (two hops + epilogue). */
+int qqcd_bench_dslash(int prec, int recon, int which, int nwarm, int niter, double *ms, double *relerr);
+
+/* Apply the hop H (par_out = 0: H_eo, 1: H_oe) to a canonical-order spinor
+ * psi[((s*4 + d)*3 + c)] complex, at storage precision (prec, recon);
+ * the sites of parity par_out of out are written, the others left alone. */
+int qqcd_hop_canonical(const double *psi, double *out, int par_out, int idag, int prec, int recon);
+
+/* Schur operator (Hdbletm semantics) on the even sites of canonical arrays. */
+int qqcd_schur_canonical(const double *psi, double *out, const double kappa[2], int idag, int prec, int recon);
+
+/* Toggle the XCD-aware workgroup order of the Dslash (default on). */
+int qqcd_set_xcd_remap(int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -6,10 +6,12 @@ oracle/ref/Makefile ran; ``available()`` says whether one is on disk.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import ctypes
 import os
 import subprocess
 import tempfile
+import threading
 
 import numpy as np
 
@@ -18,6 +20,26 @@ from . import HERE
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 REF_SRC = "/root/reference/qqcd-test"
+
+
+# The reference keeps large work arrays as local variables (for example
+# htmp(6,ntotal,4,2,8) in Hdouble, qqcd-test/cfgsprops/quark/diracops.f90:
+# 326) and is run with `ulimit -s unlimited`.  Every call into it therefore
+# runs on one worker thread with a 3 GiB stack.
+_EXEC = None
+
+
+def _run(fn, *args):
+    global _EXEC
+    if _EXEC is None:
+        old = threading.stack_size()
+        threading.stack_size(3 << 30)
+        try:
+            _EXEC = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+            _EXEC.submit(lambda: None).result()
+        finally:
+            threading.stack_size(old)
+    return _EXEC.submit(fn, *args).result()
 
 
 def _p(a):
@@ -62,7 +84,7 @@ class RefLattice:
         nn = np.zeros(4, np.int32)
         nvh = ctypes.c_int()
         ntot = ctypes.c_int()
-        self.L.ref_dims(_ip_(nn), ctypes.byref(nvh), ctypes.byref(ntot))
+        _run(self.L.ref_dims, _ip_(nn), ctypes.byref(nvh), ctypes.byref(ntot))
         self.n = tuple(int(v) for v in nn)
         if self.n != tuple(int(v) for v in n):
             raise ValueError(f"build holds {self.n}, asked for {tuple(n)}")
@@ -76,7 +98,7 @@ class RefLattice:
     def set_bc(self, bc):
         self.bc = np.ascontiguousarray(bc, dtype=np.int32)
         d = (self.workdir.rstrip("/") + "/").encode()
-        self.L.ref_init(_ip_(self.bc), ctypes.c_char_p(d), ctypes.c_int(len(d)))
+        _run(self.L.ref_init, _ip_(self.bc), ctypes.c_char_p(d), ctypes.c_int(len(d)))
 
     @property
     def slen(self):
@@ -94,20 +116,20 @@ class RefLattice:
         iblv = np.zeros((4, 16), np.int32)
         lbd = np.zeros((4, 16), np.int32)
         nms = np.zeros(4, np.int32)
-        self.L.ref_tables(_ip_(lvbc), _ip_(vecbl), _ip_(vecblinv), _ip_(iblv), _ip_(lbd), _ip_(nms))
+        _run(self.L.ref_tables, _ip_(lvbc), _ip_(vecbl), _ip_(vecblinv), _ip_(iblv), _ip_(lbd), _ip_(nms))
         return dict(lvbc=lvbc, vecbl=vecbl, vecblinv=vecblinv, iblv=iblv, lbd=lbd, nms=nms)
 
     def hsingle(self, u, gin, coact, gblclr, idag=0):
         gout = np.zeros(self.slen)
         g = np.array(gin, dtype=np.float64, copy=True)
-        self.L.ref_hsingle(_p(gout), _p(np.ascontiguousarray(u)), _p(g), int(idag),
+        _run(self.L.ref_hsingle, _p(gout), _p(np.ascontiguousarray(u)), _p(g), int(idag),
                            _p(np.ascontiguousarray(coact)), int(gblclr))
         return gout
 
     def hdouble(self, u, ge, coact, kappa, idag=0):
         he = np.zeros(self.slen)
         g = np.array(ge, dtype=np.float64, copy=True)
-        self.L.ref_hdouble(_p(he), _p(np.ascontiguousarray(u)), _p(g), int(idag),
+        _run(self.L.ref_hdouble, _p(he), _p(np.ascontiguousarray(u)), _p(g), int(idag),
                            _p(np.ascontiguousarray(coact)), ctypes.c_double(kappa))
         return he
 
@@ -115,26 +137,26 @@ class RefLattice:
         he = np.zeros(self.slen)
         g = np.array(ge, dtype=np.float64, copy=True)
         k = np.ascontiguousarray(kappa, dtype=np.float64)
-        self.L.ref_hdbletm(_p(he), _p(np.ascontiguousarray(u)), _p(g), int(idag),
+        _run(self.L.ref_hdbletm, _p(he), _p(np.ascontiguousarray(u)), _p(g), int(idag),
                            _p(np.ascontiguousarray(coact)), _p(k), int(iflag))
         return he
 
     def gammamult(self, ein, oin, mu, ieo, ibleo, eout=None, oout=None):
         eout = np.zeros(self.slen) if eout is None else np.array(eout, copy=True)
         oout = np.zeros(self.slen) if oout is None else np.array(oout, copy=True)
-        self.L.ref_gammamult(_p(np.ascontiguousarray(ein)), _p(np.ascontiguousarray(oin)),
+        _run(self.L.ref_gammamult, _p(np.ascontiguousarray(ein)), _p(np.ascontiguousarray(oin)),
                              _p(eout), _p(oout), int(mu), int(ieo), int(ibleo))
         return eout, oout
 
     def gam5x(self, a):
         a = np.array(a, dtype=np.float64, copy=True)
-        self.L.ref_gam5x(_p(a))
+        _run(self.L.ref_gam5x, _p(a))
         return a
 
     # ---- solver build only -------------------------------------------
     def vecdot(self, a, b):
         out = np.zeros(2)
-        self.L.ref_vecdot(_p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(b)), _p(out))
+        _run(self.L.ref_vecdot, _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(b)), _p(out))
         return complex(out[0], out[1])
 
     def solve(self, which, phi, u, coact, kappa, nk=(40, 10), resmax=1e-10, itermin=0,
@@ -144,7 +166,7 @@ class RefLattice:
         ph = np.array(phi, dtype=np.float64, copy=True)
         it = ctypes.c_int()
         nkk = np.ascontiguousarray(nk, dtype=np.int32)
-        self.L.ref_solve(int(which), _p(ph), _p(x), _ip_(nkk), ctypes.c_double(resmax),
+        _run(self.L.ref_solve, int(which), _p(ph), _p(x), _ip_(nkk), ctypes.c_double(resmax),
                          int(itermin), ctypes.byref(it), _p(np.ascontiguousarray(u)),
                          _p(np.ascontiguousarray(coact)),
                          _p(np.ascontiguousarray(kappa, dtype=np.float64)), int(isignal))
@@ -158,7 +180,7 @@ class RefLattice:
         b_o = np.array(bo, dtype=np.float64, copy=True)
         uu = np.array(u, dtype=np.float64, copy=True)
         nkk = np.ascontiguousarray(nk, dtype=np.int32)
-        self.L.ref_fermprop(_p(b_e), _p(b_o), _p(np.ascontiguousarray(kappa, dtype=np.float64)),
+        _run(self.L.ref_fermprop, _p(b_e), _p(b_o), _p(np.ascontiguousarray(kappa, dtype=np.float64)),
                             int(inverter), _p(np.ascontiguousarray(coact)), ctypes.c_double(resmax),
                             int(itermin), _ip_(nkk), _p(uu), _p(xe), _p(xo), int(isignal))
         return xe, xo
@@ -167,21 +189,21 @@ class RefLattice:
         be = np.zeros(self.slen)
         bo = np.zeros(self.slen)
         s = np.ascontiguousarray(src, dtype=np.int32)
-        self.L.ref_pointsource(_p(be), _p(bo), _ip_(s), int(icsrc), int(idsrc))
+        _run(self.L.ref_pointsource, _p(be), _p(bo), _ip_(s), int(icsrc), int(idsrc))
         return be, bo
 
     def heatbath(self, u, coact, nsweep, nhit=1, iseed=1):
         uu = np.array(u, dtype=np.float64, copy=True)
-        self.L.ref_heatbath(_p(uu), _p(np.ascontiguousarray(coact)), int(nsweep), int(nhit), int(iseed))
+        _run(self.L.ref_heatbath, _p(uu), _p(np.ascontiguousarray(coact)), int(nsweep), int(nhit), int(iseed))
         return uu
 
     def aveplaq(self, u):
         uu = np.array(u, dtype=np.float64, copy=True)
         p = ctypes.c_double()
-        self.L.ref_aveplaq(_p(uu), ctypes.byref(p))
+        _run(self.L.ref_aveplaq, _p(uu), ctypes.byref(p))
         return p.value
 
     def fixaction(self, beta):
         co = np.zeros(64)
-        self.L.ref_fixaction(ctypes.c_double(beta), _p(co))
+        _run(self.L.ref_fixaction, ctypes.c_double(beta), _p(co))
         return co
