@@ -47,6 +47,12 @@ SIGNATURES = {
     "qqcd_set_xcd_remap": [ctypes.c_int],
     "qqcd_gauge_random": [ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp],
     "qqcd_gauge_get_canonical": [_dp],
+    "qqcd_last_solve_stats": [_dp, _ip, _ip, _ip, _ip],
+    "qqcd_deflation_clear": [],
+    "qqcd_dense_schur": [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp],
+    "qqcd_dense_reorder": [ctypes.c_int, _dp, _dp, ctypes.c_int],
+    "qqcd_dense_lsq": [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp],
+    "qqcd_dense_harmonic_ritz": [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp],
 }
 
 

@@ -29,6 +29,7 @@ extern "C" const char *qqcd_last_error(void) { return g_err.c_str(); }
 
 extern "C" int qqcd_init(const int n[4], const int bc[4], int device) {
   API_BEGIN
+  qqcd_deflation_clear();
   if (g_ctx) {
     delete g_ctx;
     g_ctx = nullptr;
@@ -62,6 +63,7 @@ extern "C" int qqcd_init(const int n[4], const int bc[4], int device) {
 
 extern "C" int qqcd_finalize(void) {
   API_BEGIN
+  qqcd_deflation_clear();
   delete g_ctx;
   g_ctx = nullptr;
   API_END

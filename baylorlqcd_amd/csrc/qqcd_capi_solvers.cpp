@@ -1,0 +1,255 @@
+// qqcd_capi_solvers.cpp -- extern "C" solver entry points on reference-order
+// host arrays: gmresdr, gmresproject, psc, bicgstab, fermprop.
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/qqcd_hip.h"
+#include "qqcd_solvers.h"
+
+using namespace qqcd;
+using qqcd::dense::zd;
+
+#define API_BEGIN try {
+#define API_END                     \
+  }                                 \
+  catch (const std::exception &e) { \
+    set_error(e.what());            \
+    return -1;                      \
+  }                                 \
+  return 0;
+
+namespace {
+
+// The deflation space kept between calls, as the reference keeps vtemp and
+// hcnew in module shift (qqcd-test/cfgsprops/quark/shift.f90:20-45).
+Deflation *g_defl = nullptr;
+SolveStats g_last;
+
+// the reference's cycle limit kcyclim (qqcd-test/user/latdims.f90:70)
+constexpr int KCYCLIM = 5000;
+
+Deflation &defl() {
+  if (!g_defl) g_defl = new Deflation();
+  return *g_defl;
+}
+
+void to_device(Context &c, const double *h, Spinor &s, int par) {
+  const size_t bytes = sizeof(double) * 6 * ref_ntotal(c.g) * 64;
+  QQCD_HIP(hipMemcpyAsync(c.dref[2], h, bytes, hipMemcpyHostToDevice, c.stream));
+  ref_to_spinor(c.g, c.dref[2], s, par, c.stream);
+}
+
+void to_host(Context &c, const Spinor &s, double *h, int par) {
+  const size_t bytes = sizeof(double) * 6 * ref_ntotal(c.g) * 64;
+  spinor_to_ref(c.g, s, c.dref[2], par, c.stream);
+  QQCD_HIP(hipMemcpyAsync(h, c.dref[2], bytes, hipMemcpyDeviceToHost, c.stream));
+  QQCD_HIP(hipStreamSynchronize(c.stream));
+}
+
+void prepare(Context &c, const double *u, const double *coact, const int *bc, int iflag) {
+  if (iflag != -1) throw std::runtime_error("only iflag = -1 (Wilson / twisted mass) is supported");
+  set_bc(c, bc);
+  ensure_gauge(c, u, coact);
+}
+
+void check_mk(int m, int k) {
+  if (m < 2 || k < 0 || k >= m || m > 120) throw std::runtime_error("GMRES sizes need 0 <= k < m <= 120");
+}
+
+SolveStats run_gmresdr(Context &c, const Spinor &b, Spinor &x, const int GMRES[2], double resmax, int itermin,
+                       const double kappa[2]) {
+  check_mk(GMRES[0], GMRES[1]);
+  SchurOp A(c, PREC_DOUBLE, 18, kappa);
+  LinOp op = A.op();
+  return gmresdr(c, op, b, x, GMRES[0], GMRES[1], resmax, itermin, KCYCLIM * GMRES[0], &defl());
+}
+
+SolveStats run_gmresproj(Context &c, const Spinor &b, Spinor &x, int m, int k, double resmax, int itermin,
+                         const double kappa[2]) {
+  Deflation &d = defl();
+  if (d.k == 0 || (int)d.V.size() != d.k + 1)
+    throw std::runtime_error("GMRES-Proj needs the deflation space of an earlier GMRES-DR solve");
+  if (k != d.k) throw std::runtime_error("GMRES-Proj k differs from the stored deflation space");
+  if (m < 1 || m > 120) throw std::runtime_error("GMRES-Proj cycle length must be 1..120");
+  SchurOp A(c, PREC_DOUBLE, 18, kappa);
+  LinOp op = A.op();
+  return gmresproj(c, op, b, x, m, d, resmax, itermin, KCYCLIM * m);
+}
+
+}  // namespace
+
+extern "C" int qqcd_gmresdr(const double *phi, double *x, const int GMRES[2], double resmax, int itermin,
+                            int *itercount, const double *u, const double *coact, const double kappa[2], int iflag,
+                            const int *bc) {
+  API_BEGIN
+  Context &c = ctx();
+  prepare(c, u, coact, bc, iflag);
+  SpinorBuf b(c.g, PREC_DOUBLE), X(c.g, PREC_DOUBLE);
+  to_device(c, phi, b.s, 0);
+  to_device(c, x, X.s, 0);
+  g_last = run_gmresdr(c, b.s, X.s, GMRES, resmax, itermin, kappa);
+  to_host(c, X.s, x, 0);
+  *itercount = g_last.iters;
+  API_END
+}
+
+extern "C" int qqcd_gmresproject(const double *bh, double *x, const int GMRES[2], double resmax, int itermin,
+                                 int *itercount, const double *u, const double *coact, const double kappa[2],
+                                 int iflag, const int *bc) {
+  API_BEGIN
+  Context &c = ctx();
+  prepare(c, u, coact, bc, iflag);
+  SpinorBuf b(c.g, PREC_DOUBLE), X(c.g, PREC_DOUBLE);
+  to_device(c, bh, b.s, 0);
+  g_last = run_gmresproj(c, b.s, X.s, GMRES[0], GMRES[1], resmax, itermin, kappa);
+  to_host(c, X.s, x, 0);
+  *itercount = g_last.iters;
+  API_END
+}
+
+extern "C" int qqcd_psc(const double *bh, double *x, const int GMRES[2], double resmax, int itermin,
+                        int *itercount, const double *u, const double *coact, const double kappa[2], int iflag,
+                        const int *bc, int isignal, double resmax_proj) {
+  API_BEGIN
+  Context &c = ctx();
+  prepare(c, u, coact, bc, iflag);
+  check_mk(GMRES[0], GMRES[1]);
+  SpinorBuf b(c.g, PREC_DOUBLE), X(c.g, PREC_DOUBLE);
+  to_device(c, bh, b.s, 0);
+  if (isignal == 1) {
+    to_device(c, x, X.s, 0);
+    g_last = run_gmresdr(c, b.s, X.s, GMRES, resmax, itermin, kappa);
+  } else {
+    // inverters.f90:4439-4458: GMRES-Proj(m-k, k), tolerance 1e-8 hard-coded
+    const double tol = resmax_proj > 0 ? resmax_proj : 1e-8;
+    g_last = run_gmresproj(c, b.s, X.s, GMRES[0] - GMRES[1], GMRES[1], tol, itermin, kappa);
+  }
+  to_host(c, X.s, x, 0);
+  *itercount = g_last.iters;
+  API_END
+}
+
+extern "C" int qqcd_bicgstab(const double *phi, double *x, double resmax, int itermin, int *itercount,
+                             const double *u, const double *coact, const double kappa[2], int iflag, const int *bc) {
+  API_BEGIN
+  Context &c = ctx();
+  prepare(c, u, coact, bc, iflag);
+  SpinorBuf b(c.g, PREC_DOUBLE), X(c.g, PREC_DOUBLE);
+  to_device(c, phi, b.s, 0);
+  to_device(c, x, X.s, 0);
+  SchurOp A(c, PREC_DOUBLE, 18, kappa);
+  LinOp op = A.op();
+  g_last = bicgstab(c, op, b.s, X.s, resmax, itermin, 1000000);
+  to_host(c, X.s, x, 0);
+  *itercount = g_last.iters;
+  API_END
+}
+
+// fermprop for nkappa = -1, qqcd-test/cfgsprops/quark/quark.f90:570-617
+// (source), :630-692 (solve), :786-819 (reconstruction).  a = 2 kappa mu is
+// taken with the sign of mu as given, while the operator falls back to
+// Wilson for mu < 1e-8 (diracops.f90:467), exactly as in the reference.
+extern "C" int qqcd_fermprop(const double *be, const double *bo, const double kappa[2], int inverter,
+                             const double *coact, const int *bc, double resmax, int itermin, const int nGMRES[2],
+                             int vectype, const double *u, double *xe, double *xo, int isignal, int *itercount) {
+  API_BEGIN
+  Context &c = ctx();
+  prepare(c, u, coact, bc, -1);
+  const double k = kappa[0], a = 2.0 * kappa[0] * kappa[1];
+  SpinorBuf b_e(c.g, PREC_DOUBLE), b_o(c.g, PREC_DOUBLE), t_o(c.g, PREC_DOUBLE), src(c.g, PREC_DOUBLE);
+  SpinorBuf x_e(c.g, PREC_DOUBLE), x_o(c.g, PREC_DOUBLE);
+  to_device(c, be, b_e.s, 0);
+  to_device(c, bo, b_o.s, 1);
+  // vecsrc = b_e/kappa^2 + H_eo (1 - a i g5) b_o / (kappa (1 + a^2))
+  local_spin(c.g, b_o.s, t_o.s, 1.0, -a, c.stream);
+  Epilogue es;
+  es.a0 = 1.0 / (k * k);
+  es.b0 = 1.0 / (k * (1.0 + a * a));
+  dslash(c.g, c.U64, t_o.s, src.s, 0, 0, es, &b_e.s, c.stream);
+  // initial guess (quark.f90:630-640)
+  if (inverter > 3) {
+    Vec xv = as_vec(x_e.s);
+    c.blas->zero(xv);
+  } else if (inverter == 0) {
+    if (vectype == 1) fill_spinor(c.g, x_e.s, 1.0, 0.0, c.stream);
+    else to_device(c, xe, x_e.s, 0);
+  } else {
+    throw std::runtime_error("fermprop: inverter must be 0 (BiCGStab), 5 (GMRES-DR) or 6 (psc)");
+  }
+  if (inverter == 0) {
+    SchurOp A(c, PREC_DOUBLE, 18, kappa);
+    LinOp op = A.op();
+    g_last = bicgstab(c, op, src.s, x_e.s, resmax, itermin, 1000000);
+  } else if (inverter == 5) {
+    g_last = run_gmresdr(c, src.s, x_e.s, nGMRES, resmax, itermin, kappa);
+  } else if (inverter == 6) {
+    if (isignal == 1) {
+      g_last = run_gmresdr(c, src.s, x_e.s, nGMRES, resmax, itermin, kappa);
+    } else {
+      check_mk(nGMRES[0], nGMRES[1]);
+      g_last = run_gmresproj(c, src.s, x_e.s, nGMRES[0] - nGMRES[1], nGMRES[1], 1e-8, itermin, kappa);
+    }
+  } else {
+    throw std::runtime_error("fermprop: inverter must be 0 (BiCGStab), 5 (GMRES-DR) or 6 (psc)");
+  }
+  // x_o = (1 - a i g5)(b_o + kappa H_oe x_e) / (1 + a^2)
+  Epilogue er;
+  const double f = 1.0 / (1.0 + a * a);
+  er.a0 = f;
+  er.a1 = -a * f;
+  er.b0 = k * f;
+  er.b1 = -a * k * f;
+  dslash(c.g, c.U64, x_e.s, x_o.s, 1, 0, er, &b_o.s, c.stream);
+  to_host(c, x_e.s, xe, 0);
+  to_host(c, x_o.s, xo, 1);
+  *itercount = g_last.iters;
+  API_END
+}
+
+extern "C" int qqcd_last_solve_stats(double *rel_res, int *iters, int *matvecs, int *cycles, int *converged) {
+  API_BEGIN
+  *rel_res = g_last.rel_res;
+  *iters = g_last.iters;
+  *matvecs = g_last.matvecs;
+  *cycles = g_last.cycles;
+  *converged = g_last.converged;
+  API_END
+}
+
+extern "C" int qqcd_deflation_clear(void) {
+  API_BEGIN
+  if (g_defl) {
+    delete g_defl;
+    g_defl = nullptr;
+  }
+  API_END
+}
+
+// ---- host dense linear algebra of the solvers, exported for verification ----
+extern "C" int qqcd_dense_schur(int n, const double *a, int lda, double *t, double *q) {
+  API_BEGIN
+  if (!dense::schur(n, (const zd *)a, lda, (zd *)t, (zd *)q)) throw std::runtime_error("schur: QR did not converge");
+  API_END
+}
+
+extern "C" int qqcd_dense_reorder(int n, double *t, double *q, int k) {
+  API_BEGIN
+  dense::reorder_smallest(n, (zd *)t, (zd *)q, k);
+  API_END
+}
+
+extern "C" int qqcd_dense_lsq(int rows, int cols, const double *h, int ldh, const double *c, double *y,
+                              double *resid) {
+  API_BEGIN
+  *resid = dense::lsq(rows, cols, (const zd *)h, ldh, (const zd *)c, (zd *)y);
+  API_END
+}
+
+extern "C" int qqcd_dense_harmonic_ritz(int m, const double *hbar, int ldh, int k, double *p, double *theta) {
+  API_BEGIN
+  if (!dense::harmonic_ritz(m, (const zd *)hbar, ldh, k, (zd *)p, (zd *)theta))
+    throw std::runtime_error("harmonic_ritz failed");
+  API_END
+}

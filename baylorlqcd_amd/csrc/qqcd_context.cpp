@@ -62,7 +62,6 @@ void free_gauge(Gauge &u) {
 
 Context::~Context() {
   if (stream) hipStreamSynchronize(stream);
-  for (auto &s : defl_V) free_spinor(s);
   for (auto &kv : copies) free_gauge(kv.second.U);
   free_gauge(U64);
   for (double *&p : dref)

@@ -34,14 +34,6 @@ struct Context {
   // staging for reference-layout spinors (6*ntotal*64 doubles each)
   double *dref[3] = {nullptr, nullptr, nullptr};
 
-  // deflation space carried from the first RHS to later ones (the
-  // reference's shift::vtemp / hcnew, qqcd-test/cfgsprops/quark/shift.f90)
-  std::vector<Spinor> defl_V;      // k+1 vectors (FP64)
-  std::vector<std::complex<double>> defl_H;  // (k+1) x k column-major
-  int defl_k = 0;
-  uint64_t defl_gen = 0;
-  double defl_kappa[2] = {0, 0};
-
   ~Context();
   const Gauge &gauge(Prec p, int recon);
 };

@@ -241,3 +241,43 @@ void gauge_random(const Geom &g, Gauge &U, int kind, double eps, uint64_t seed, 
 }
 
 }  // namespace qqcd
+
+namespace qqcd {
+
+// out = a0 in + a1 (i g5) in, FP64;  i g5 v = (v2, v3, -v0, -v1) per colour
+__global__ void k_local_spin(int Vh, long stride, const double2 *in, double2 *out, double a0, double a1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Vh) return;
+  double2 v[12];
+  for (int k = 0; k < 12; k++) v[k] = in[(long)k * stride + i];
+  for (int c = 0; c < 3; c++) {
+    const double2 s0 = v[c], s1 = v[3 + c], s2 = v[6 + c], s3 = v[9 + c];
+    out[(long)c * stride + i] = make_double2(a0 * s0.x + a1 * s2.x, a0 * s0.y + a1 * s2.y);
+    out[(long)(3 + c) * stride + i] = make_double2(a0 * s1.x + a1 * s3.x, a0 * s1.y + a1 * s3.y);
+    out[(long)(6 + c) * stride + i] = make_double2(a0 * s2.x - a1 * s0.x, a0 * s2.y - a1 * s0.y);
+    out[(long)(9 + c) * stride + i] = make_double2(a0 * s3.x - a1 * s1.x, a0 * s3.y - a1 * s1.y);
+  }
+}
+
+void local_spin(const Geom &g, const Spinor &in, Spinor &out, double a0, double a1, hipStream_t s) {
+  if (in.prec != PREC_DOUBLE || out.prec != PREC_DOUBLE) throw std::runtime_error("local_spin: FP64 only");
+  const int nb = (int)((g.Vh + 255) / 256);
+  hipLaunchKernelGGL(k_local_spin, dim3(nb), dim3(256), 0, s, (int)g.Vh, g.stride, (const double2 *)in.data,
+                     (double2 *)out.data, a0, a1);
+  QQCD_HIP(hipGetLastError());
+}
+
+__global__ void k_fill(int Vh, long stride, double2 *x, double re, double im) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Vh) return;
+  for (int k = 0; k < 12; k++) x[(long)k * stride + i] = make_double2(re, im);
+}
+
+void fill_spinor(const Geom &g, Spinor &x, double re, double im, hipStream_t s) {
+  if (x.prec != PREC_DOUBLE) throw std::runtime_error("fill_spinor: FP64 only");
+  const int nb = (int)((g.Vh + 255) / 256);
+  hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, s, (int)g.Vh, g.stride, (double2 *)x.data, re, im);
+  QQCD_HIP(hipGetLastError());
+}
+
+}  // namespace qqcd

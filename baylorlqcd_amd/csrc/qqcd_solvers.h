@@ -1,0 +1,77 @@
+// qqcd_solvers.h -- Krylov solvers on device-resident spinors.
+#pragma once
+#include <functional>
+#include <vector>
+
+#include "qqcd_context.h"
+#include "qqcd_dense.h"
+
+namespace qqcd {
+
+// A linear operator on checkerboard-0 spinors of one precision.
+struct LinOp {
+  Prec prec = PREC_DOUBLE;
+  std::function<void(const Spinor &, Spinor &)> apply;
+};
+
+// The even-odd Schur operator (Hdbletm semantics) with storage (prec, recon).
+struct SchurOp {
+  Context &c;
+  const Gauge &U;
+  double kappa[2];
+  int idag;
+  SpinorBuf tmp;
+  SchurOp(Context &c_, Prec p, int recon, const double k[2], int idag_ = 0)
+      : c(c_), U(c_.gauge(p, recon)), idag(idag_), tmp(c_.g, p) {
+    kappa[0] = k[0];
+    kappa[1] = k[1];
+  }
+  void operator()(const Spinor &in, Spinor &out) { schur_apply(c, U, kappa, idag, in, out, tmp.s); }
+  LinOp op() {
+    LinOp o;
+    o.prec = tmp.s.prec;
+    o.apply = [this](const Spinor &a, Spinor &b) { (*this)(a, b); };
+    return o;
+  }
+};
+
+struct SolveStats {
+  int iters = 0;       // Arnoldi / BiCGStab steps (operator applications in the Krylov iterations)
+  int matvecs = 0;     // all operator applications
+  int cycles = 0;
+  double rel_res = 0;  // final ||b - A x|| / ||r0|| with the true residual
+  int converged = 0;
+};
+
+// Deflation space carried between solves: A V_k = V_{k+1} Hbar_k.
+struct Deflation {
+  std::vector<Spinor> V;  // k+1 vectors
+  std::vector<dense::zd> H;  // (k+1) x k, column-major
+  int k = 0;
+  Prec prec = PREC_DOUBLE;
+  void clear();
+  ~Deflation() { clear(); }
+};
+
+// GMRES-DR(m, k): x holds x0 on entry.  Stops when ||b - A x|| < tol ||b - A x0||.
+// When defl is given, the deflation space of the last restart (or of the
+// final cycle) is stored there.
+SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int k, double tol, int itermin,
+                   int maxiter, Deflation *defl);
+
+// GMRES-Proj(m, k) with deflation space defl: alternates a projection over
+// (V_{k+1}, Hbar_k) with a GMRES(m) cycle.  x holds x0 on entry.
+SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, const Deflation &defl, double tol,
+                     int itermin, int maxiter);
+
+// BiCGStab from x (Frommer et al.; the reference's bicgstab).  Convergence
+// on the recursive residual, then confirmed on the true residual (restarting
+// from it if the two disagree).
+SolveStats bicgstab(Context &c, LinOp &A, const Spinor &b, Spinor &x, double tol, int itermin, int maxiter);
+
+// out = a0 in + a1 (i g5) in, FP64, on one checkerboard
+void local_spin(const Geom &g, const Spinor &in, Spinor &out, double a0, double a1, hipStream_t s);
+// x = (re, im) in every complex component of every site (padding untouched)
+void fill_spinor(const Geom &g, Spinor &x, double re, double im, hipStream_t s);
+
+}  // namespace qqcd

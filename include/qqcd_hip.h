@@ -146,6 +146,12 @@ int qqcd_fermprop(const double *be, const double *bo, const double kappa[2], int
                   const int *bc, double resmax, int itermin, const int nGMRES[2], int vectype, const double *u,
                   double *xe, double *xo, int isignal, int *itercount);
 
+/* Statistics of the last solver call: final ||r|| / ||r0|| (true residual),
+ * Krylov steps, all operator applications, cycles, converged flag. */
+int qqcd_last_solve_stats(double *rel_res, int *iters, int *matvecs, int *cycles, int *converged);
+/* Drop the resident deflation space (the next psc must use isignal = 1). */
+int qqcd_deflation_clear(void);
+
 /* ---- native (device-resident) interface: benchmarks and tests ---------- */
 
 /* prec: 0 FP64, 1 FP32, 2 FP16 storage; recon: 18 or 12. */
@@ -168,6 +174,21 @@ int qqcd_schur_canonical(const double *psi, double *out, const double kappa[2], 
 
 /* Toggle the XCD-aware workgroup order of the Dslash (default on). */
 int qqcd_set_xcd_remap(int on);
+
+/* ---- host dense linear algebra of the solvers (verification hooks) ------
+ * Column-major complex matrices as interleaved (re, im) doubles.  They
+ * replace the reference's pseudolapack routines used by gmresdr
+ * (qqcd-test/cfgsprops/quark/pseudolapack.f90: hessenberg :700, qgenerator
+ * :774, evalues/QRdouble :868/:2196, orgschur/schurswap :252/:389). */
+/* A = Q T Q^H, T upper triangular (complex Schur form) */
+int qqcd_dense_schur(int n, const double *a, int lda, double *t, double *q);
+/* move the k eigenvalues of smallest modulus to the leading positions */
+int qqcd_dense_reorder(int n, double *t, double *q, int k);
+/* min ||c - H y|| for rows x cols H; *resid = the minimum */
+int qqcd_dense_lsq(int rows, int cols, const double *h, int ldh, const double *c, double *y, double *resid);
+/* GMRES-DR harmonic Ritz step: k Schur vectors (m x k) of
+ * H_m + |h_{m+1,m}|^2 H_m^{-H} e_m e_m^T for the k values of smallest modulus */
+int qqcd_dense_harmonic_ritz(int m, const double *hbar, int ldh, int k, double *p, double *theta);
 
 #ifdef __cplusplus
 }

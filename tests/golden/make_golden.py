@@ -1,0 +1,153 @@
+"""Write the golden fixtures of the solve path with the reference Fortran.
+
+Runs only where /root/reference is present (this script builds it with
+oracle/ref/Makefile, then calls it through oracle/refdrv.py).  Output:
+tests/golden/golden_4x4x4x4.npz and tests/golden/golden_8x8x8x16.npz.
+
+Inputs are regenerated from seeds by the tests (spinors: numpy
+default_rng(seed) Gaussians; links: baylorlqcd_amd.gauge.haar), except the
+8^3x16 heatbath field, which is stored (12 reals per link; row 2 is
+conj(row0 x row1) and the reference consumed that reconstructed field).
+
+Canonical orders: spinor psi[s, spin, colour], s = x + nx*(y + ny*(z + nz*t));
+"even" arrays hold the global-parity-0 sites in increasing s.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+from oracle import refdrv  # noqa: E402
+from baylorlqcd_amd import gauge  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def spinor(n, seed):
+    rng = np.random.default_rng(seed)
+    V = int(np.prod(n))
+    return rng.standard_normal((V, 4, 3)) + 1j * rng.standard_normal((V, 4, 3))
+
+
+def canon_even(n, be):
+    psi = O.spinor_ref_to_canon(n, be, None)
+    return psi[~O.parity_mask(n)]
+
+
+def canon_odd(n, bo):
+    psi = O.spinor_ref_to_canon(n, None, bo)
+    return psi[O.parity_mask(n)]
+
+
+def ops_block(R, n, U, tag, out):
+    u = O.links_canon_to_ref(n, U)
+    co = O.coact_array()
+    psi = spinor(n, 101)
+    be, bo = O.spinor_canon_to_ref(n, psi)
+    for gbl, gin in ((1, bo), (2, be)):
+        for idag in (0, 1):
+            g = R.hsingle(u, gin, co, gbl, idag)
+            par = gbl - 1
+            pm = O.parity_mask(n) == bool(par)
+            out[f"{tag}hsingle_g{gbl}_d{idag}"] = O.spinor_ref_to_canon(
+                n, g if gbl == 1 else None, g if gbl == 2 else None)[pm]
+    for name, kap in (("wilson", (0.150, 0.0)), ("tm", (0.156, 0.005)), ("tmneg", (0.156, -0.005))):
+        for idag in (0, 1):
+            he = R.hdbletm(u, be, co, kap, idag)
+            out[f"{tag}hdbletm_{name}_d{idag}"] = canon_even(n, he)
+    out[f"{tag}hdouble_d0"] = canon_even(n, R.hdouble(u, be, co, 0.150, 0))
+    # anisotropic coefficients (c1 != c2)
+    co2 = O.coact_array((0.9, 1.1, 1.0, 0.7), (1.2, 0.8, 1.0, 0.7))
+    g = R.hsingle(u, be, co2, 2, 0)
+    out[f"{tag}hsingle_aniso_g2_d0"] = canon_odd(n, g)
+
+
+def main():
+    # 4^4 builds without the inverters: their off-path debug prints index
+    # sites up to 128 with constant subscripts, which flang rejects when
+    # ntotal is smaller (qqcd-test/cfgsprops/quark/inverters.f90:2325-2333).
+    for n, solvers in (((4, 4, 4, 4), False), ((8, 8, 8, 16), True)):
+        refdrv.build(n, solvers=solvers)
+
+    # ---------------- 4^4: operators in both boundary conditions, solves --------
+    n = (4, 4, 4, 4)
+    out = {}
+    for bc in ((1, 1, 1, -1), (1, 1, 1, 1)):
+        R = refdrv.RefLattice(n, bc=bc, solvers=False)
+        tag = "ap_" if bc[3] == -1 else "p_"
+        U = gauge.haar(n, 7)
+        ops_block(R, n, U, tag, out)
+        if bc[3] == -1:
+            t = R.tables()
+            for k, v in t.items():
+                out["tables_" + k] = v
+            psi = spinor(n, 202)
+            be, bo = O.spinor_canon_to_ref(n, psi)
+            out["gam5x"] = canon_even(n, R.gam5x(be))
+            for mu in range(1, 6):
+                eo, oo = R.gammamult(be, bo, mu, 2, 5)
+                out[f"gammamult_mu{mu}_e"] = canon_even(n, eo)
+                out[f"gammamult_mu{mu}_o"] = canon_odd(n, oo)
+    np.savez_compressed(os.path.join(HERE, "golden_4x4x4x4.npz"), **out)
+    print("wrote golden_4x4x4x4.npz", len(out), "arrays")
+
+    # ---------------- 8^3x16: the config-1 field, operators and solves ----------
+    n = (8, 8, 8, 16)
+    bc = (1, 1, 1, -1)
+    R = refdrv.RefLattice(n, bc=bc)
+    out = {}
+    co6 = R.fixaction(6.0)
+    U0 = gauge.haar(n, 20261202)
+    u = R.heatbath(O.links_canon_to_ref(n, U0), co6, 200, nhit=1, iseed=11)
+    U = O.links_ref_to_canon(n, u)
+    U12 = gauge.compress12(U)
+    U = gauge.reconstruct12(U12)
+    out["links12"] = U12
+    out["plaquette_ref"] = np.array(R.aveplaq(O.links_canon_to_ref(n, U)))
+    u = O.links_canon_to_ref(n, U)
+    co = O.coact_array()
+    psi = spinor(n, 101)
+    be, bo = O.spinor_canon_to_ref(n, psi)
+    out["hsingle_g2_d0"] = canon_odd(n, R.hsingle(u, be, co, 2, 0))
+    out["hdbletm_tm_d0"] = canon_even(n, R.hdbletm(u, be, co, (0.156, 0.005), 0))
+    psi2 = spinor(n, 202)
+    be2, bo2 = O.spinor_canon_to_ref(n, psi2)
+    out["vecdot"] = np.array(R.vecdot(be2, bo2))
+    ps_be, ps_bo = R.pointsource((3, 2, 4, 1), 2, 3)
+    out["pointsource_be"] = ps_be
+    out["pointsource_bo"] = ps_bo
+    # config 1: point source at the origin, spin 1 colour 1, Wilson kappa = 0.150
+    ps_e, ps_o = R.pointsource((1, 1, 1, 1), 1, 1)
+    cases = {
+        "gmresdr_k150": dict(kap=(0.150, 0.0), inverter=5, nk=(40, 10), src=(ps_e, ps_o)),
+        "bicgstab_k150": dict(kap=(0.150, 0.0), inverter=0, nk=(1, 0), src=(ps_e, ps_o)),
+        "gmresdr_tm": dict(kap=(0.156, 0.005), inverter=5, nk=(40, 10), src=(ps_e, ps_o)),
+    }
+    for name, cs in cases.items():
+        b_e, b_o = cs["src"]
+        R.set_bc(bc)
+        log0 = open(os.path.join(R.workdir, "CFGSPROPS.LOG")).read()
+        xe, xo = R.fermprop(b_e, b_o, u, co, cs["kap"], cs["inverter"], nk=cs["nk"], resmax=1e-11, isignal=1)
+        log = open(os.path.join(R.workdir, "CFGSPROPS.LOG")).read()[len(log0):]
+        out[f"{name}_xe"] = canon_even(n, xe)
+        if name == "gmresdr_k150":
+            out[f"{name}_xo"] = canon_odd(n, xo)
+        print(name, "log tail:", log.strip().splitlines()[-1:] if log.strip() else "")
+    # psc: first RHS by GMRES-DR (isignal=1), a second RHS by GMRES-Proj
+    b2_e, b2_o = R.pointsource((1, 1, 1, 1), 2, 3)
+    xe1, _ = R.fermprop(ps_e, ps_o, u, co, (0.150, 0.0), 6, nk=(40, 10), resmax=1e-11, isignal=1)
+    xe2, xo2 = R.fermprop(b2_e, b2_o, u, co, (0.150, 0.0), 6, nk=(40, 10), resmax=1e-11, isignal=2)
+    out["psc_rhs1_xe"] = canon_even(n, xe1)[:64]
+    out["psc_rhs2_xe"] = canon_even(n, xe2)
+    np.savez_compressed(os.path.join(HERE, "golden_8x8x8x16.npz"), **out)
+    print("wrote golden_8x8x8x16.npz", len(out), "arrays; plaquette", float(out["plaquette_ref"]))
+
+
+if __name__ == "__main__":
+    main()
