@@ -47,8 +47,15 @@ SIGNATURES = {
     "qqcd_set_xcd_remap": [ctypes.c_int],
     "qqcd_gauge_random": [ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp],
     "qqcd_gauge_get_canonical": [_dp],
+    "qqcd_gauge_heatbath": [ctypes.c_double, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_int, _dp],
+    "qqcd_gauge_plaquette": [_dp],
     "qqcd_last_solve_stats": [_dp, _ip, _ip, _ip, _ip],
     "qqcd_deflation_clear": [],
+    "qqcd_solve_mixed": [_dp, _dp, _ip, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                         ctypes.c_int, _dp, _dp, _dp, _ip, _ip],
+    "qqcd_bench_solve": [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_double, ctypes.c_int,
+                         ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _ip, _ip, _dp],
+    "qqcd_bench_copy": [ctypes.c_double, ctypes.c_int, _dp],
     "qqcd_dense_schur": [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp],
     "qqcd_dense_reorder": [ctypes.c_int, _dp, _dp, ctypes.c_int],
     "qqcd_dense_lsq": [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp],

@@ -51,6 +51,9 @@ class Blas {
   double *zbuf = nullptr;
   void **vptr = nullptr;
   static constexpr long zcap = 128 * 128;
+
+ public:
+  static constexpr int kmax = 136;  // vectors per multi_dot
 };
 
 }  // namespace qqcd

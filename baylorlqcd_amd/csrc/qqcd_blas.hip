@@ -89,18 +89,16 @@ __global__ void __launch_bounds__(RB) k_mdot(long n, PtrPack v, const void *w, d
   block_sum<2 * K>(acc, part + (long)blockIdx.x * 2 * K);
 }
 
-// final pass: res[q] = sum_b part[b][q], q < nq, fixed order
+// final pass: res[q] = sum_b part[b*nq + q], one workgroup per q, fixed order
 __global__ void __launch_bounds__(RB) k_final(int nb, int nq, const double *part, double *res) {
-  for (int q = 0; q < nq; q++) {
-    double s = 0;
-    for (int b = threadIdx.x; b < nb; b += RB) s += part[(long)b * nq + q];
-    double v[1] = {s};
-    __shared__ double o[1];
-    block_sum<1>(v, o);
-    __syncthreads();
-    if (threadIdx.x == 0) res[q] = o[0];
-    __syncthreads();
-  }
+  const int q = blockIdx.x;
+  double s = 0;
+  for (int b = threadIdx.x; b < nb; b += RB) s += part[(long)b * nq + q];
+  double v[1] = {s};
+  __shared__ double o[1];
+  block_sum<1>(v, o);
+  __syncthreads();
+  if (threadIdx.x == 0) res[q] = o[0];
 }
 
 // ---- w = w - sum_k h_k v_k, returns ||w||^2 (optional) ----
@@ -183,9 +181,9 @@ __global__ void __launch_bounds__(RB) k_combine(long n, int m, const void *const
 // host side
 
 Blas::Blas(hipStream_t s) : stream(s) {
-  QQCD_HIP(hipMalloc(&part, sizeof(double) * RG * 16));
-  QQCD_HIP(hipMalloc(&res, sizeof(double) * 64));
-  QQCD_HIP(hipHostMalloc(&hres, sizeof(double) * 64, hipHostMallocDefault));
+  QQCD_HIP(hipMalloc(&part, sizeof(double) * RG * 2 * kmax));
+  QQCD_HIP(hipMalloc(&res, sizeof(double) * 2 * kmax));
+  QQCD_HIP(hipHostMalloc(&hres, sizeof(double) * 2 * kmax, hipHostMallocDefault));
   QQCD_HIP(hipMalloc(&zbuf, sizeof(double) * 2 * zcap));
   QQCD_HIP(hipMalloc(&vptr, sizeof(void *) * 128));
 }
@@ -199,7 +197,7 @@ Blas::~Blas() {
 }
 
 void Blas::fetch(int nq) {
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, stream, RG, nq, part, res);
+  hipLaunchKernelGGL(k_final, dim3(nq), dim3(RB), 0, stream, RG, nq, part, res);
   QQCD_HIP(hipGetLastError());
   QQCD_HIP(hipMemcpyAsync(hres, res, sizeof(double) * nq, hipMemcpyDeviceToHost, stream));
   QQCD_HIP(hipStreamSynchronize(stream));
@@ -223,6 +221,12 @@ static void launch_mdot(hipStream_t s, long n, const PtrPack &pp, const void *w,
 }
 
 void Blas::multi_dot(int k, const Vec *v, const Vec &w, std::complex<double> *out) {
+  if (k > kmax) throw std::runtime_error("multi_dot: too many vectors");
+  // every chunk of <= 8 vectors writes its partials into its own slab of
+  // part (RG rows of 2*kk); one final pass and one host synchronisation
+  long off = 0;
+  std::vector<long> offs;
+  std::vector<int> kks;
   for (int k0 = 0; k0 < k; k0 += 8) {
     const int kk = std::min(8, k - k0);
     PtrPack pp{};
@@ -230,23 +234,34 @@ void Blas::multi_dot(int k, const Vec *v, const Vec &w, std::complex<double> *ou
       if (v[k0 + q].n != w.n) throw std::runtime_error("multi_dot: length mismatch");
       pp.p[q] = v[k0 + q].data;
     }
+    double *pt = part + off;
     tsw(v[k0].prec, w.prec, [&](auto ta, auto tb) {
       TYPES_OF(ta, tb)
       switch (kk) {
-        case 1: launch_mdot<1, TA, TB>(stream, w.n, pp, w.data, part); break;
-        case 2: launch_mdot<2, TA, TB>(stream, w.n, pp, w.data, part); break;
-        case 3: launch_mdot<3, TA, TB>(stream, w.n, pp, w.data, part); break;
-        case 4: launch_mdot<4, TA, TB>(stream, w.n, pp, w.data, part); break;
-        case 5: launch_mdot<5, TA, TB>(stream, w.n, pp, w.data, part); break;
-        case 6: launch_mdot<6, TA, TB>(stream, w.n, pp, w.data, part); break;
-        case 7: launch_mdot<7, TA, TB>(stream, w.n, pp, w.data, part); break;
-        default: launch_mdot<8, TA, TB>(stream, w.n, pp, w.data, part); break;
+        case 1: launch_mdot<1, TA, TB>(stream, w.n, pp, w.data, pt); break;
+        case 2: launch_mdot<2, TA, TB>(stream, w.n, pp, w.data, pt); break;
+        case 3: launch_mdot<3, TA, TB>(stream, w.n, pp, w.data, pt); break;
+        case 4: launch_mdot<4, TA, TB>(stream, w.n, pp, w.data, pt); break;
+        case 5: launch_mdot<5, TA, TB>(stream, w.n, pp, w.data, pt); break;
+        case 6: launch_mdot<6, TA, TB>(stream, w.n, pp, w.data, pt); break;
+        case 7: launch_mdot<7, TA, TB>(stream, w.n, pp, w.data, pt); break;
+        default: launch_mdot<8, TA, TB>(stream, w.n, pp, w.data, pt); break;
       }
     });
     QQCD_HIP(hipGetLastError());
-    fetch(2 * kk);
-    for (int q = 0; q < kk; q++) out[k0 + q] = std::complex<double>(hres[2 * q], hres[2 * q + 1]);
+    offs.push_back(off);
+    kks.push_back(kk);
+    off += (long)RG * 2 * kk;
   }
+  for (size_t c = 0; c < offs.size(); c++)
+    hipLaunchKernelGGL(k_final, dim3(2 * kks[c]), dim3(RB), 0, stream, RG, 2 * kks[c], part + offs[c],
+                       res + 16 * c);
+  QQCD_HIP(hipGetLastError());
+  QQCD_HIP(hipMemcpyAsync(hres, res, sizeof(double) * 16 * offs.size(), hipMemcpyDeviceToHost, stream));
+  QQCD_HIP(hipStreamSynchronize(stream));
+  for (size_t c = 0; c < offs.size(); c++)
+    for (int q = 0; q < kks[c]; q++)
+      out[8 * c + q] = std::complex<double>(hres[16 * c + 2 * q], hres[16 * c + 2 * q + 1]);
 }
 
 std::complex<double> Blas::dot(const Vec &a, const Vec &b) {

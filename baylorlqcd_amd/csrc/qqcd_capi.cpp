@@ -371,3 +371,39 @@ extern "C" int qqcd_gauge_get_canonical(double *U) {
   hipFree(d);
   API_END
 }
+
+namespace qqcd {
+void heatbath(Context &c, double beta, int nsweep, uint64_t seed, int hot_start);
+double plaquette(Context &c);
+}  // namespace qqcd
+
+extern "C" int qqcd_gauge_heatbath(double beta, int nsweep, unsigned long long seed, int start,
+                                   const double *coact) {
+  API_BEGIN
+  Context &c = ctx();
+  if (beta <= 0 || nsweep < 0) throw std::runtime_error("heatbath: need beta > 0 and nsweep >= 0");
+  if (start < -1 || start > 1) throw std::runtime_error("heatbath: start is -1 (continue), 0 (cold) or 1 (hot)");
+  set_coact(c, coact);
+  heatbath(c, beta, nsweep, seed, start);
+  API_END
+}
+
+extern "C" int qqcd_gauge_plaquette(double *plaq) {
+  API_BEGIN
+  Context &c = ctx();
+  if (!c.have_gauge) throw std::runtime_error("no gauge field loaded");
+  *plaq = plaquette(c);
+  API_END
+}
+
+namespace qqcd {
+double bench_copy(size_t bytes, int iters, hipStream_t s);
+}
+
+extern "C" int qqcd_bench_copy(double bytes, int iters, double *ms) {
+  API_BEGIN
+  Context &c = ctx();
+  if (bytes < 16 || iters < 1) throw std::runtime_error("bench_copy: bad size");
+  *ms = bench_copy((size_t)bytes, iters, c.stream);
+  API_END
+}

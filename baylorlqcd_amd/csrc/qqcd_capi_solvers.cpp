@@ -1,5 +1,6 @@
 // qqcd_capi_solvers.cpp -- extern "C" solver entry points on reference-order
 // host arrays: gmresdr, gmresproject, psc, bicgstab, fermprop.
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -25,6 +26,7 @@ namespace {
 // The deflation space kept between calls, as the reference keeps vtemp and
 // hcnew in module shift (qqcd-test/cfgsprops/quark/shift.f90:20-45).
 Deflation *g_defl = nullptr;
+Deflation *g_defl_mixed = nullptr;  // deflation space of the mixed solver
 SolveStats g_last;
 
 // the reference's cycle limit kcyclim (qqcd-test/user/latdims.f90:70)
@@ -224,6 +226,10 @@ extern "C" int qqcd_deflation_clear(void) {
     delete g_defl;
     g_defl = nullptr;
   }
+  if (g_defl_mixed) {
+    delete g_defl_mixed;
+    g_defl_mixed = nullptr;
+  }
   API_END
 }
 
@@ -251,5 +257,118 @@ extern "C" int qqcd_dense_harmonic_ritz(int m, const double *hbar, int ldh, int 
   API_BEGIN
   if (!dense::harmonic_ritz(m, (const zd *)hbar, ldh, k, (zd *)p, (zd *)theta))
     throw std::runtime_error("harmonic_ritz failed");
+  API_END
+}
+
+// ---- mixed precision ----------------------------------------------------
+
+namespace {
+
+MixedParams mixed_params(const int GMRES[2], double tol, int basis, int op_prec, int op_recon, double inner_tol) {
+  check_mk(GMRES[0], GMRES[1]);
+  if (basis != 0 && basis != 1) throw std::runtime_error("basis precision must be 0 (FP64) or 1 (FP32)");
+  if (op_prec < 0 || op_prec > 2 || (op_recon != 18 && op_recon != 12))
+    throw std::runtime_error("bad operator precision / link format");
+  if (op_prec == 0 && basis == 1) throw std::runtime_error("an FP64 operator needs an FP64 basis");
+  MixedParams p;
+  p.m = GMRES[0];
+  p.k = GMRES[1];
+  p.tol = tol;
+  p.basis = (Prec)basis;
+  p.op_prec = (Prec)op_prec;
+  p.op_recon = op_recon;
+  if (inner_tol > 0) p.inner_tol = inner_tol;
+  else p.inner_tol = op_prec == 2 ? 1e-2 : (op_prec == 1 ? 1e-6 : 1e-10);
+  return p;
+}
+
+// unit source at 0-based global site x, Dirac d, colour c (pointsource)
+void point_source_dev(Context &c, Spinor &b_e, Spinor &b_o, const int src[4], int d, int col) {
+  const Geom &g = c.g;
+  int x[4];
+  for (int m = 0; m < 4; m++) {
+    if (src[m] < 1 || src[m] > g.n[m]) throw std::runtime_error("source site outside the lattice");
+    x[m] = src[m] - 1;
+  }
+  if (d < 1 || d > 4 || col < 1 || col > 3) throw std::runtime_error("bad Dirac / colour index");
+  const int par = (x[0] + x[1] + x[2] + x[3]) & 1;
+  const long cb = (x[0] + (long)g.n[0] * (x[1] + (long)g.n[1] * (x[2] + (long)g.n[2] * x[3]))) / 2;
+  Spinor &b = par == 0 ? b_e : b_o;
+  const double one[2] = {1.0, 0.0};
+  const long k = 3 * (d - 1) + (col - 1);
+  QQCD_HIP(hipMemcpyAsync((double2 *)b.data + k * b.stride + cb, one, sizeof(one), hipMemcpyHostToDevice, c.stream));
+  QQCD_HIP(hipStreamSynchronize(c.stream));
+}
+}  // namespace
+
+extern "C" int qqcd_solve_mixed(const double *phi, double *x, const int GMRES[2], double tol, int basis_prec,
+                                int op_prec, int op_recon, double inner_tol, int keep_deflation, const double *u,
+                                const double *coact, const double kappa[2], const int *bc, int *itercount) {
+  API_BEGIN
+  Context &c = ctx();
+  prepare(c, u, coact, bc, -1);
+  MixedParams p = mixed_params(GMRES, tol, basis_prec, op_prec, op_recon, inner_tol);
+  SpinorBuf b(c.g, PREC_DOUBLE), X(c.g, PREC_DOUBLE);
+  to_device(c, phi, b.s, 0);
+  to_device(c, x, X.s, 0);
+  if (!keep_deflation && g_defl_mixed) {
+    delete g_defl_mixed;
+    g_defl_mixed = nullptr;
+  }
+  if (!g_defl_mixed) g_defl_mixed = new Deflation();
+  MixedStats ms = solve_mixed(c, kappa, b.s, X.s, p, g_defl_mixed);
+  to_host(c, X.s, x, 0);
+  g_last = ms.inner;
+  g_last.rel_res = ms.rel_res;
+  g_last.converged = ms.converged;
+  g_last.cycles = ms.outer;
+  *itercount = ms.inner.iters;
+  API_END
+}
+
+// Device-resident solve of M x = (point source) for timing: method 0 is the
+// plain FP64 GMRES-DR(m,k), method 1 the mixed-precision solver.  The timed
+// region covers source preparation, the solve and odd-site reconstruction
+// (inputs resident, no host transfers); the true full-system residual is
+// computed afterwards in FP64.
+extern "C" int qqcd_bench_solve(int method, const int src[4], int spin, int colour, const double kappa[2],
+                                const int GMRES[2], double tol, int op_prec, int op_recon, double inner_tol,
+                                int keep_deflation, double *seconds, int *iters, int *outer, double *true_res) {
+  API_BEGIN
+  Context &c = ctx();
+  if (!c.have_gauge) throw std::runtime_error("no gauge field loaded");
+  SpinorBuf b_e(c.g, PREC_DOUBLE), b_o(c.g, PREC_DOUBLE), src_e(c.g, PREC_DOUBLE);
+  SpinorBuf x_e(c.g, PREC_DOUBLE), x_o(c.g, PREC_DOUBLE);
+  point_source_dev(c, b_e.s, b_o.s, src, spin, colour);
+  if (!keep_deflation && g_defl_mixed) {
+    delete g_defl_mixed;
+    g_defl_mixed = nullptr;
+  }
+  if (!g_defl_mixed) g_defl_mixed = new Deflation();
+  QQCD_HIP(hipDeviceSynchronize());
+  auto t0 = std::chrono::steady_clock::now();
+  prep_source(c, kappa, b_e.s, b_o.s, src_e.s);
+  if (method == 0) {
+    check_mk(GMRES[0], GMRES[1]);
+    SchurOp A(c, PREC_DOUBLE, 18, kappa);
+    LinOp op = A.op();
+    // x0 = 0, so ||r0|| = ||src||; with b_o = 0 the even-odd relative
+    // residual equals the full-system one
+    SolveStats st = gmresdr(c, op, src_e.s, x_e.s, GMRES[0], GMRES[1], tol, 0, KCYCLIM * GMRES[0], nullptr);
+    *iters = st.iters;
+    *outer = st.cycles;
+  } else if (method == 1) {
+    MixedParams p = mixed_params(GMRES, tol, 1, op_prec, op_recon, inner_tol);
+    MixedStats ms = solve_mixed(c, kappa, src_e.s, x_e.s, p, g_defl_mixed);
+    *iters = ms.inner.iters;
+    *outer = ms.outer;
+  } else {
+    throw std::runtime_error("bench_solve: method must be 0 (FP64 GMRES-DR) or 1 (mixed)");
+  }
+  reconstruct_odd(c, kappa, b_o.s, x_e.s, x_o.s);
+  QQCD_HIP(hipDeviceSynchronize());
+  auto t1 = std::chrono::steady_clock::now();
+  *seconds = std::chrono::duration<double>(t1 - t0).count();
+  *true_res = full_residual(c, kappa, b_e.s, b_o.s, x_e.s, x_o.s);
   API_END
 }

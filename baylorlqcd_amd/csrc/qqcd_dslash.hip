@@ -201,26 +201,50 @@ struct DslashArgs {
   int xcd_remap;
 };
 
-// One hop in direction MU from neighbour spinor at checkerboard index j
-// (opposite parity), with link (lpar, li).  FWD: U h ; else U^dag h.
+// Pointers and strides a hop reads (scalars only, so that nothing of the
+// kernel argument block is copied to private memory).
+struct HopIn {
+  const void *in;
+  const float *in_norm;
+  const void *links;
+  long sstride, lstride;
+};
+
+// One hop in direction MU = 0, 1, 2 from the neighbour spinor at
+// checkerboard index j (opposite parity), with link (lpar, li).
+// FWD: U h ; else U^dag h.
 template <typename S, typename L, int RECON, int MU, bool FWD, typename R>
-__device__ __forceinline__ void hop(const DslashArgs &a, cx<R> acc[12], int j, int lpar, int li, R sig,
-                                    R scl, R edge) {
+__device__ __forceinline__ void hop(const HopIn &a, cx<R> acc[12], int j, int lpar, int li, R sig, R scl,
+                                    R edge) {
   cx<R> u[9];
   load_link<L, RECON>(a.links, a.lstride, lpar, MU, li, u);
   const float nrm = SIO<S>::norm(a.in_norm, j);
-  cx<R> h[6], w[6];
-  if (MU == 3) {
-    // (1 + sig g_4) keeps the upper (sig=+1) or lower (sig=-1) spin pair, doubled
-    const int s0 = sig > 0 ? 0 : 6;
+  cx<R> p[12], h[6], w[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) h[c] = R(2) * SIO<S>::ld(a.in, a.sstride, s0 + c, j, nrm);
+  for (int k = 0; k < 12; k++) p[k] = SIO<S>::ld(a.in, a.sstride, k, j, nrm);
+  project<MU>(p, sig, h);
+  if (FWD) {
+    mv(u, h, w);
+    mv(u, h + 3, w + 3);
   } else {
-    cx<R> p[12];
-#pragma unroll
-    for (int k = 0; k < 12; k++) p[k] = SIO<S>::ld(a.in, a.sstride, k, j, nrm);
-    project<MU>(p, sig, h);
+    mdv(u, h, w);
+    mdv(u, h + 3, w + 3);
   }
+  accum<MU>(acc, w, sig, scl * edge);
+}
+
+// One t hop.  (1 + g_4) keeps the upper spin pair, (1 - g_4) the lower one,
+// each doubled; UPPER selects which, at compile time, so only that half of
+// the neighbour spinor is loaded and the accumulator is indexed statically.
+template <typename S, typename L, int RECON, bool FWD, bool UPPER, typename R>
+__device__ __forceinline__ void hop_t(const HopIn &a, cx<R> acc[12], int j, int lpar, int li, R scl, R edge) {
+  cx<R> u[9];
+  load_link<L, RECON>(a.links, a.lstride, lpar, 3, li, u);
+  const float nrm = SIO<S>::norm(a.in_norm, j);
+  constexpr int s0 = UPPER ? 0 : 6;
+  cx<R> h[6], w[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) h[c] = R(2) * SIO<S>::ld(a.in, a.sstride, s0 + c, j, nrm);
   if (FWD) {
     mv(u, h, w);
     mv(u, h + 3, w + 3);
@@ -229,38 +253,45 @@ __device__ __forceinline__ void hop(const DslashArgs &a, cx<R> acc[12], int j, i
     mdv(u, h + 3, w + 3);
   }
   const R f = scl * edge;
+#pragma unroll
+  for (int c = 0; c < 6; c++) acc[s0 + c] = acc[s0 + c] + f * w[c];
+}
+
+// Both hops of direction MU.  Projector form: forward c1 (1 + sf g), backward
+// c1 (1 - sf g); split form: fp (1+g) + fm (1-g) forward, bp (1+g) + bm (1-g)
+// backward.
+// Both hops of direction MU.  SIG = +1 / -1: projector form with the same
+// sign in every direction, forward c1 (1 + SIG g), backward c1 (1 - SIG g),
+// sc = c1; SIG = 0: split form, forward fp (1+g) + fm (1-g), backward
+// bp (1+g) + bm (1-g).
+template <typename S, typename L, int RECON, int SIG, int MU, typename R>
+__device__ __forceinline__ void direction(const HopIn &h, int par, cx<R> acc[12], int i, int jf, int jb, R ef,
+                                          R eb, R sc, R fp, R fm, R bp, R bm) {
+  const int opar = par ^ 1;
   if (MU == 3) {
-    const int s0 = sig > 0 ? 0 : 6;
-    if (s0 == 0) {
-#pragma unroll
-      for (int c = 0; c < 6; c++) acc[c] = acc[c] + f * w[c];
+    if (SIG != 0) {
+      hop_t<S, L, RECON, true, (SIG > 0)>(h, acc, jf, par, i, sc, ef);
+      hop_t<S, L, RECON, false, (SIG < 0)>(h, acc, jb, opar, jb, sc, eb);
     } else {
-#pragma unroll
-      for (int c = 0; c < 6; c++) acc[6 + c] = acc[6 + c] + f * w[c];
+      if (fp != R(0)) hop_t<S, L, RECON, true, true>(h, acc, jf, par, i, fp, ef);
+      if (fm != R(0)) hop_t<S, L, RECON, true, false>(h, acc, jf, par, i, fm, ef);
+      if (bp != R(0)) hop_t<S, L, RECON, false, true>(h, acc, jb, opar, jb, bp, eb);
+      if (bm != R(0)) hop_t<S, L, RECON, false, false>(h, acc, jb, opar, jb, bm, eb);
     }
+    return;
+  }
+  if (SIG != 0) {
+    hop<S, L, RECON, MU, true>(h, acc, jf, par, i, R(SIG), sc, ef);
+    hop<S, L, RECON, MU, false>(h, acc, jb, opar, jb, R(-SIG), sc, eb);
   } else {
-    accum<MU>(acc, w, sig, f);
+    if (fp != R(0)) hop<S, L, RECON, MU, true>(h, acc, jf, par, i, R(1), fp, ef);
+    if (fm != R(0)) hop<S, L, RECON, MU, true>(h, acc, jf, par, i, R(-1), fm, ef);
+    if (bp != R(0)) hop<S, L, RECON, MU, false>(h, acc, jb, opar, jb, R(1), bp, eb);
+    if (bm != R(0)) hop<S, L, RECON, MU, false>(h, acc, jb, opar, jb, R(-1), bm, eb);
   }
 }
 
-template <typename S, typename L, int RECON, bool PROJ, int MU, typename R>
-__device__ __forceinline__ void direction(const DslashArgs &a, cx<R> acc[12], int i, int jf, int jb, R ef,
-                                          R eb) {
-  const int opar = a.par ^ 1;
-  if (PROJ) {
-    const R sf = (R)a.sig_f[MU];
-    const R sc = (R)a.scl[MU];
-    hop<S, L, RECON, MU, true>(a, acc, jf, a.par, i, sf, sc, ef);
-    hop<S, L, RECON, MU, false>(a, acc, jb, opar, jb, -sf, sc, eb);
-  } else {
-    if (a.fw_p[MU] != 0.0) hop<S, L, RECON, MU, true>(a, acc, jf, a.par, i, R(1), (R)a.fw_p[MU], ef);
-    if (a.fw_m[MU] != 0.0) hop<S, L, RECON, MU, true>(a, acc, jf, a.par, i, R(-1), (R)a.fw_m[MU], ef);
-    if (a.bw_p[MU] != 0.0) hop<S, L, RECON, MU, false>(a, acc, jb, opar, jb, R(1), (R)a.bw_p[MU], eb);
-    if (a.bw_m[MU] != 0.0) hop<S, L, RECON, MU, false>(a, acc, jb, opar, jb, R(-1), (R)a.bw_m[MU], eb);
-  }
-}
-
-template <typename S, typename L, int RECON, bool PROJ>
+template <typename S, typename L, int RECON, int SIG>
 __global__ void __launch_bounds__(256) k_dslash(const DslashArgs a) {
   typedef typename SIO<S>::R R;
   int blk = blockIdx.x;
@@ -284,6 +315,9 @@ __global__ void __launch_bounds__(256) k_dslash(const DslashArgs a) {
   const int t = q / nz;
   const int x = x0 + ((a.par + y + z + t) & 1);
   const int yzt = y + ny * (z + nz * t);
+  const HopIn h{a.in, a.in_norm, a.links, a.sstride, a.lstride};
+  const int par = a.par;
+#define DIRARGS(M) (R) a.scl[M], (R) a.fw_p[M], (R) a.fw_m[M], (R) a.bw_p[M], (R) a.bw_m[M]
 
   cx<R> acc[12];
 #pragma unroll
@@ -292,27 +326,29 @@ __global__ void __launch_bounds__(256) k_dslash(const DslashArgs a) {
   {  // x
     const int xf = x + 1 == nx ? 0 : x + 1, xb = x == 0 ? nx - 1 : x - 1;
     const R ef = x + 1 == nx ? (R)a.bcf[0] : R(1), eb = x == 0 ? (R)a.bcf[0] : R(1);
-    direction<S, L, RECON, PROJ, 0>(a, acc, i, (xf + nx * yzt) >> 1, (xb + nx * yzt) >> 1, ef, eb);
+    direction<S, L, RECON, SIG, 0>(h, par, acc, i, (xf + nx * yzt) >> 1, (xb + nx * yzt) >> 1, ef, eb,
+                                    DIRARGS(0));
   }
   {  // y
     const int yf = y + 1 == ny ? 0 : y + 1, yb = y == 0 ? ny - 1 : y - 1;
     const R ef = y + 1 == ny ? (R)a.bcf[1] : R(1), eb = y == 0 ? (R)a.bcf[1] : R(1);
     const int zt = z + nz * t;
-    direction<S, L, RECON, PROJ, 1>(a, acc, i, (x + nx * (yf + ny * zt)) >> 1, (x + nx * (yb + ny * zt)) >> 1,
-                                    ef, eb);
+    direction<S, L, RECON, SIG, 1>(h, par, acc, i, (x + nx * (yf + ny * zt)) >> 1,
+                                    (x + nx * (yb + ny * zt)) >> 1, ef, eb, DIRARGS(1));
   }
   {  // z
     const int zf = z + 1 == nz ? 0 : z + 1, zb = z == 0 ? nz - 1 : z - 1;
     const R ef = z + 1 == nz ? (R)a.bcf[2] : R(1), eb = z == 0 ? (R)a.bcf[2] : R(1);
-    direction<S, L, RECON, PROJ, 2>(a, acc, i, (x + nx * (y + ny * (zf + nz * t))) >> 1,
-                                    (x + nx * (y + ny * (zb + nz * t))) >> 1, ef, eb);
+    direction<S, L, RECON, SIG, 2>(h, par, acc, i, (x + nx * (y + ny * (zf + nz * t))) >> 1,
+                                    (x + nx * (y + ny * (zb + nz * t))) >> 1, ef, eb, DIRARGS(2));
   }
   {  // t
     const int tf = t + 1 == nt ? 0 : t + 1, tb = t == 0 ? nt - 1 : t - 1;
     const R ef = t + 1 == nt ? (R)a.bcf[3] : R(1), eb = t == 0 ? (R)a.bcf[3] : R(1);
-    direction<S, L, RECON, PROJ, 3>(a, acc, i, (x + nx * (y + ny * (z + nz * tf))) >> 1,
-                                    (x + nx * (y + ny * (z + nz * tb))) >> 1, ef, eb);
+    direction<S, L, RECON, SIG, 3>(h, par, acc, i, (x + nx * (y + ny * (z + nz * tf))) >> 1,
+                                    (x + nx * (y + ny * (z + nz * tb))) >> 1, ef, eb, DIRARGS(3));
   }
+#undef DIRARGS
 
   // epilogue: out = a0 x + a1 (i g5 x) + b0 D + b1 (i g5 D);  i g5 v = (v2, v3, -v0, -v1)
   const R b0 = (R)a.b0, b1 = (R)a.b1;
@@ -360,12 +396,14 @@ static int g_xcd_remap = 1;
 void set_xcd_remap(int v) { g_xcd_remap = v; }
 
 template <typename S, typename L, int RECON>
-static void launch_t(const DslashArgs &a, bool proj, hipStream_t s) {
+static void launch_t(const DslashArgs &a, int sig, hipStream_t s) {
   const int nb = (a.Vh + 255) / 256;
-  if (proj)
-    hipLaunchKernelGGL((k_dslash<S, L, RECON, true>), dim3(nb), dim3(256), 0, s, a);
+  if (sig > 0)
+    hipLaunchKernelGGL((k_dslash<S, L, RECON, 1>), dim3(nb), dim3(256), 0, s, a);
+  else if (sig < 0)
+    hipLaunchKernelGGL((k_dslash<S, L, RECON, -1>), dim3(nb), dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((k_dslash<S, L, RECON, false>), dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_dslash<S, L, RECON, 0>), dim3(nb), dim3(256), 0, s, a);
   QQCD_HIP(hipGetLastError());
 }
 
@@ -408,7 +446,13 @@ void dslash(const Geom &g, const Gauge &U, const Spinor &in, Spinor &out, int pa
   a.b0 = ep.b0;
   a.b1 = ep.b1;
   a.xcd_remap = g_xcd_remap;
-  const bool proj = h.proj != 0;
+  // one projector sign for every direction, or the split form
+  int proj = 0;
+  if (h.proj) {
+    proj = a.sig_f[0] > 0 ? 1 : -1;
+    for (int m = 1; m < 4; m++)
+      if ((a.sig_f[m] > 0 ? 1 : -1) != proj) proj = 0;
+  }
   if (in.prec == PREC_DOUBLE) {
     if (U.prec != PREC_DOUBLE) throw std::runtime_error("dslash: FP64 spinors need FP64 links");
     if (U.recon == 18) launch_t<double, double, 18>(a, proj, s);

@@ -281,3 +281,35 @@ void fill_spinor(const Geom &g, Spinor &x, double re, double im, hipStream_t s) 
 }
 
 }  // namespace qqcd
+
+namespace qqcd {
+
+// plain streaming copy, the calibration kernel for byte counters
+__global__ void __launch_bounds__(256) k_copy(long n, const double2 *__restrict__ a, double2 *__restrict__ b) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) b[i] = a[i];
+}
+
+double bench_copy(size_t bytes, int iters, hipStream_t s) {
+  const long n = (long)(bytes / 16);
+  double2 *a = nullptr, *b = nullptr;
+  QQCD_HIP(hipMalloc(&a, n * 16));
+  QQCD_HIP(hipMalloc(&b, n * 16));
+  QQCD_HIP(hipMemsetAsync(a, 0, n * 16, s));
+  hipEvent_t e0, e1;
+  QQCD_HIP(hipEventCreate(&e0));
+  QQCD_HIP(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, n, a, b);
+  QQCD_HIP(hipEventRecord(e0, s));
+  for (int k = 0; k < iters; k++) hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, n, a, b);
+  QQCD_HIP(hipEventRecord(e1, s));
+  QQCD_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QQCD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return (double)ms / iters;
+}
+
+}  // namespace qqcd

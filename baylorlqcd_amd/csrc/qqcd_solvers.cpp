@@ -23,6 +23,7 @@
 #include "qqcd_solvers.h"
 
 #include <cmath>
+#include <memory>
 #include <stdexcept>
 
 namespace qqcd {
@@ -52,19 +53,27 @@ static double residual(Context &c, LinOp &A, const Spinor &b, const Spinor &x, S
   return std::sqrt(c.blas->axpby(1.0, vv(b), -1.0, wv, true));
 }
 
-// One Arnoldi step: V[j+1] = A V[j] orthonormalised against V[0..j] (CGS2);
-// the Hessenberg column goes to h[0..j+1].  Returns the new norm.
+// One Arnoldi step: V[j+1] = A V[j] orthonormalised against V[0..j] by
+// classical Gram-Schmidt with selective reorthogonalisation (the DGKS
+// criterion: a second pass only when the first removed more than half of
+// ||w||^2, i.e. when cancellation may have spoiled orthogonality).  The
+// first pass computes <V_i, w> and <w, w> in one sweep; the Hessenberg
+// column goes to h[0..j+1].  Returns the new norm.
 static double arnoldi_step(Context &c, LinOp &A, std::vector<Spinor> &V, int j, zd *h) {
   A.apply(V[j], V[j + 1]);
-  std::vector<Vec> Vb = vecs(V, j + 1);
+  std::vector<Vec> Vb = vecs(V, j + 2);  // V[0..j] and w itself
   Vec w = vv(V[j + 1]);
-  std::vector<zd> h1(j + 1), h2(j + 1);
-  c.blas->multi_dot(j + 1, Vb.data(), w, h1.data());
-  c.blas->multi_axpy(j + 1, h1.data(), Vb.data(), w, false);
-  c.blas->multi_dot(j + 1, Vb.data(), w, h2.data());
-  const double n2 = c.blas->multi_axpy(j + 1, h2.data(), Vb.data(), w, true);
+  std::vector<zd> h1(j + 2), h2(j + 1);
+  c.blas->multi_dot(j + 2, Vb.data(), w, h1.data());
+  const double w2 = h1[j + 1].real();
+  double n2 = c.blas->multi_axpy(j + 1, h1.data(), Vb.data(), w, true);
+  for (int i = 0; i <= j; i++) h[i] = h1[i];
+  if (n2 < 0.5 * w2) {
+    c.blas->multi_dot(j + 1, Vb.data(), w, h2.data());
+    n2 = c.blas->multi_axpy(j + 1, h2.data(), Vb.data(), w, true);
+    for (int i = 0; i <= j; i++) h[i] += h2[i];
+  }
   const double hn = std::sqrt(n2);
-  for (int i = 0; i <= j; i++) h[i] = h1[i] + h2[i];
   h[j + 1] = hn;
   if (hn > 0) c.blas->axpby(1.0 / hn, w, 0.0, w, false);
   return hn;
@@ -358,6 +367,149 @@ SolveStats bicgstab(Context &c, LinOp &A, const Spinor &b, Spinor &x, double tol
   }
   st.converged = st.rel_res < tol;
   return st;
+}
+
+}  // namespace qqcd
+
+namespace qqcd {
+
+MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spinor &x, const MixedParams &p,
+                       Deflation *defl) {
+  if (p.basis == PREC_HALF) throw std::runtime_error("solve_mixed: the Krylov basis is FP64 or FP32");
+  MixedStats ms;
+  SchurOp A64(c, PREC_DOUBLE, 18, kappa);
+  LinOp op64 = A64.op();
+  // operator levels: the requested storage, then FP32, then FP64 (fallback
+  // when an outer step stalls)
+  Prec op_prec = p.op_prec;
+  std::unique_ptr<SchurOp> Alo(new SchurOp(c, op_prec, p.op_recon, kappa));
+  std::unique_ptr<SpinorBuf> in_lo, out_lo;
+  auto make_inner = [&]() {
+    LinOp o;
+    o.prec = p.basis;
+    if (op_prec == p.basis) {
+      o.apply = [&](const Spinor &a, Spinor &r) { (*Alo)(a, r); };
+    } else {
+      in_lo.reset(new SpinorBuf(c.g, op_prec));
+      out_lo.reset(new SpinorBuf(c.g, op_prec));
+      o.apply = [&](const Spinor &a, Spinor &r) {
+        spinor_copy_convert(c.g, a, in_lo->s, c.stream);
+        (*Alo)(in_lo->s, out_lo->s);
+        spinor_copy_convert(c.g, out_lo->s, r, c.stream);
+      };
+    }
+    return o;
+  };
+  LinOp inner = make_inner();
+  double inner_tol = p.inner_tol;
+  SpinorBuf r(c.g, PREC_DOUBLE), rl(c.g, p.basis), el(c.g, p.basis);
+  SolveStats dummy;
+  const double bn = std::sqrt(c.blas->norm2(as_vec(b)));
+  double rn = residual(c, op64, b, x, r.s, dummy);
+  ms.fp64_matvecs++;
+  Deflation local;
+  Deflation &D = defl ? *defl : local;
+  if (D.k > 0 && (D.prec != p.basis || D.k != p.k)) D.clear();
+  while (rn >= p.tol * bn && ms.outer < p.max_outer && ms.inner.iters < p.maxiter) {
+    if (op_prec == PREC_DOUBLE && p.basis == PREC_DOUBLE && ms.outer > 0 && ms.stalled) {
+      // last resort: plain FP64 GMRES-DR on the true residual equation
+      SolveStats st = gmresdr(c, op64, b, x, p.m, p.k, p.tol * bn / rn * 0.5, 0, p.maxiter - ms.inner.iters, nullptr);
+      ms.inner.iters += st.iters;
+      ms.inner.matvecs += st.matvecs;
+      ms.fallback = 2;
+      rn = residual(c, op64, b, x, r.s, dummy);
+      ms.outer++;
+      break;
+    }
+    Vec rv = as_vec(r.s), rlv = as_vec(rl.s), elv = as_vec(el.s), xv = as_vec(x);
+    c.blas->copy(rv, rlv);
+    c.blas->zero(elv);
+    // never ask the inner solve for more than the outer target needs
+    const double itol = std::max(inner_tol, 0.5 * p.tol * bn / rn);
+    SolveStats st;
+    const int budget = p.maxiter - ms.inner.iters;
+    if (D.k == 0)
+      st = gmresdr(c, inner, rl.s, el.s, p.m, p.k, itol, 0, std::min(budget, p.first_cycles * p.m), &D);
+    else
+      st = gmresproj(c, inner, rl.s, el.s, p.m - p.k, D, itol, 0, std::min(budget, p.later_cycles * (p.m - p.k)));
+    ms.inner.iters += st.iters;
+    ms.inner.matvecs += st.matvecs;
+    ms.inner.cycles += st.cycles;
+    c.blas->axpby(1.0, elv, 1.0, xv, false);
+    const double rn_old = rn;
+    rn = residual(c, op64, b, x, r.s, dummy);
+    ms.fp64_matvecs++;
+    ms.outer++;
+    ms.stalled = rn > p.stall * rn_old;
+    if (ms.stalled) {
+      // the low-precision operator no longer reduces the true residual:
+      // climb to the next operator precision
+      if (op_prec == PREC_HALF) {
+        op_prec = PREC_SINGLE;
+        ms.fallback = 1;
+      } else {
+        op_prec = PREC_DOUBLE;
+        ms.fallback = 2;
+      }
+      if (op_prec == PREC_DOUBLE && p.basis != PREC_DOUBLE) {
+        // the FP32 basis cannot carry an FP64 operator: finish with plain FP64 GMRES-DR
+        SolveStats st2 = gmresdr(c, op64, b, x, p.m, p.k, 0.5 * p.tol * bn / rn, 0, p.maxiter - ms.inner.iters,
+                                 nullptr);
+        ms.inner.iters += st2.iters;
+        ms.inner.matvecs += st2.matvecs;
+        rn = residual(c, op64, b, x, r.s, dummy);
+        ms.outer++;
+        break;
+      }
+      Alo.reset(new SchurOp(c, op_prec, op_prec == PREC_DOUBLE ? 18 : p.op_recon, kappa));
+      inner = make_inner();
+      inner_tol = op_prec == PREC_SINGLE ? std::min(inner_tol, 1e-6) : inner_tol;
+    }
+  }
+  ms.rel_res = bn > 0 ? rn / bn : 0.0;
+  ms.converged = rn < p.tol * bn;
+  return ms;
+}
+
+void prep_source(Context &c, const double kappa[2], const Spinor &b_e, const Spinor &b_o, Spinor &src) {
+  const double k = kappa[0], a = 2.0 * kappa[0] * kappa[1];
+  SpinorBuf t(c.g, PREC_DOUBLE);
+  local_spin(c.g, b_o, t.s, 1.0, -a, c.stream);
+  Epilogue es;
+  es.a0 = 1.0 / (k * k);
+  es.b0 = 1.0 / (k * (1.0 + a * a));
+  dslash(c.g, c.U64, t.s, src, 0, 0, es, &b_e, c.stream);
+}
+
+void reconstruct_odd(Context &c, const double kappa[2], const Spinor &b_o, const Spinor &x_e, Spinor &x_o) {
+  const double k = kappa[0], a = 2.0 * kappa[0] * kappa[1];
+  const double f = 1.0 / (1.0 + a * a);
+  Epilogue er;
+  er.a0 = f;
+  er.a1 = -a * f;
+  er.b0 = k * f;
+  er.b1 = -a * k * f;
+  dslash(c.g, c.U64, x_e, x_o, 1, 0, er, &b_o, c.stream);
+}
+
+// ||b - M x|| / ||b|| for M = (1 + a i g5) - kappa H on the whole lattice
+double full_residual(Context &c, const double kappa[2], const Spinor &b_e, const Spinor &b_o, const Spinor &x_e,
+                     const Spinor &x_o) {
+  const double k = kappa[0], a = 2.0 * kappa[0] * kappa[1];
+  Epilogue e;
+  e.a0 = -1.0;
+  e.a1 = -a;
+  e.b0 = k;
+  SpinorBuf t(c.g, PREC_DOUBLE);
+  double num = 0, den = 0;
+  const Spinor *xs[2] = {&x_e, &x_o}, *bs[2] = {&b_e, &b_o};
+  for (int par = 0; par < 2; par++) {
+    dslash(c.g, c.U64, *xs[1 - par], t.s, par, 0, e, xs[par], c.stream);
+    Vec tv = as_vec(t.s);
+    num += c.blas->axpby(1.0, as_vec(*bs[par]), 1.0, tv, true);
+    den += c.blas->norm2(as_vec(*bs[par]));
+  }
+  return den > 0 ? std::sqrt(num / den) : 0.0;
 }
 
 }  // namespace qqcd

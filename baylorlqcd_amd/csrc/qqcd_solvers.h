@@ -75,3 +75,50 @@ void local_spin(const Geom &g, const Spinor &in, Spinor &out, double a0, double 
 void fill_spinor(const Geom &g, Spinor &x, double re, double im, hipStream_t s);
 
 }  // namespace qqcd
+
+namespace qqcd {
+
+// Mixed-precision solve of the even-odd system (defect correction with
+// reliable updates): the solution, the true residual and every outer update
+// are FP64; each correction is solved with an inner GMRES-DR(m,k) (first
+// time) or GMRES-Proj(m-k,k) over the kept deflation space (afterwards) on
+// vectors of precision `basis`, with the operator stored in (op_prec,
+// op_recon).  Stops when ||b - A x|| < tol ||b||.  An outer step that does
+// not halve the true residual raises the operator precision (FP16 -> FP32
+// -> plain FP64 GMRES-DR), so the FP64 solver is the fallback and the FP64
+// true residual the judge of every result.
+struct MixedParams {
+  int m = 40, k = 16;
+  Prec basis = PREC_SINGLE;
+  Prec op_prec = PREC_SINGLE;
+  int op_recon = 12;
+  double tol = 1e-10;
+  double inner_tol = 1e-6;
+  int max_outer = 60;
+  int maxiter = 200000;
+  int first_cycles = 30;  // GMRES-DR cycles allowed in the first inner solve
+  int later_cycles = 3;   // GMRES-Proj cycles per later inner solve
+  double stall = 0.5;     // an outer step that shrinks ||r|| by less stalls
+};
+
+struct MixedStats {
+  SolveStats inner;  // summed over outer iterations
+  int outer = 0;
+  int fp64_matvecs = 0;
+  double rel_res = 0;  // ||b - A x|| / ||b|| in FP64
+  int converged = 0;
+  int stalled = 0;
+  int fallback = 0;  // 1: operator raised to FP32, 2: finished in plain FP64
+};
+
+MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spinor &x, const MixedParams &p,
+                       Deflation *defl);
+
+// even-odd source preparation / reconstruction / full-system residual of
+// fermprop (qqcd-test/cfgsprops/quark/quark.f90:600-617, 797-819), FP64
+void prep_source(Context &c, const double kappa[2], const Spinor &b_e, const Spinor &b_o, Spinor &src);
+void reconstruct_odd(Context &c, const double kappa[2], const Spinor &b_o, const Spinor &x_e, Spinor &x_o);
+double full_residual(Context &c, const double kappa[2], const Spinor &b_e, const Spinor &b_o, const Spinor &x_e,
+                     const Spinor &x_o);
+
+}  // namespace qqcd

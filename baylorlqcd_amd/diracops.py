@@ -101,3 +101,15 @@ def gauge_get_canonical(n):
     U = np.zeros((V, 4, 3, 3), dtype=np.complex128)
     _lib.check(_lib.lib().qqcd_gauge_get_canonical(_lib.dp(U.view(np.float64))))
     return U
+
+
+def gauge_heatbath(beta, nsweep, seed, start, coact):
+    """Device pseudo-heatbath (start: 0 cold, 1 hot, -1 continue)."""
+    _lib.check(_lib.lib().qqcd_gauge_heatbath(float(beta), int(nsweep), int(seed), int(start),
+                                              _lib.dp(np.ascontiguousarray(coact, dtype=np.float64).ravel(order="F"))))
+
+
+def gauge_plaquette():
+    p = np.zeros(1)
+    _lib.check(_lib.lib().qqcd_gauge_plaquette(_lib.dp(p)))
+    return float(p[0])

@@ -1,0 +1,434 @@
+! qqcd_hip.f90 -- Fortran drop-in for the Dirac-operator and inverter layer.
+!
+! The procedures below carry the names and dummy-argument lists of the
+! reference's module procedures (BaylorLQCD "qqcd"):
+!   Hsingle, Hdouble, Hdbletm, gammamult  qqcd-test/cfgsprops/quark/diracops.f90
+!   gam5x                                 qqcd-test/cfgsprops/basics.f90
+!   bicgstab, gmresdr, psc, gmresproject, vecdot
+!                                         qqcd-test/cfgsprops/quark/inverters.f90
+!   pointsource, fermprop                 qqcd-test/cfgsprops/quark/quark.f90
+! and forward to the HIP library through its C ABI (include/qqcd_hip.h).  A
+! host program replaces `use diracops` / `use inverters` (and the two quark
+! procedures) with `use qqcd_hip`; everything else in the host is unchanged.
+!
+! The lattice extents come from the host's own latdims module (nx, ny, nz,
+! nt), the library starts on the first call.  Only one process is
+! supported: the geometry arguments (vecbl, vecblinv, myid, nn, ldiv, nms,
+! lvbc, ib, lbd, iblv) are checked against the single-process tables and
+! otherwise unused.  Errors are reported as the reference reports them: a
+! <MODULE>.ERROR file with the message, then stop
+! (qqcd-test/cfgsprops/quark/diracops.f90:1077-1082).
+!
+! After the host changes the link array in place (heatbath sweep, GFread),
+! it calls qqcd_gauge_changed(u, coact) so the resident copy is refreshed;
+! an array at a new address is picked up without it.
+
+module qqcd_hip
+   use iso_c_binding
+   use latdims, only: nx, ny, nz, nt
+   implicit none
+   private
+
+   public :: Hsingle, Hdouble, Hdbletm, gammamult, gam5x, vecdot, bicgstab, gmresdr, psc, &
+             gmresproject, pointsource, fermprop, qqcd_gauge_changed, qqcd_shutdown, &
+             qqcd_solve_mixed_f
+
+   logical, save :: started = .false.
+
+   interface
+      integer(c_int) function c_init(n, bc, device) bind(C, name="qqcd_init")
+         import :: c_int
+         integer(c_int), intent(in) :: n(4), bc(4)
+         integer(c_int), value :: device
+      end function
+      integer(c_int) function c_finalize() bind(C, name="qqcd_finalize")
+         import :: c_int
+      end function
+      type(c_ptr) function c_last_error() bind(C, name="qqcd_last_error")
+         import :: c_ptr
+      end function
+      integer(c_int) function c_gauge_load(u, coact) bind(C, name="qqcd_gauge_load")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: u, coact
+      end function
+      integer(c_int) function c_hsingle(gout, u, gin, idag, coact, bc, gblclr) bind(C, name="qqcd_hsingle")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: gout, u, gin, coact, bc
+         integer(c_int), value :: idag, gblclr
+      end function
+      integer(c_int) function c_hdouble(he, u, ge, idag, coact, kappa, bc) bind(C, name="qqcd_hdouble")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: he, u, ge, coact, bc
+         integer(c_int), value :: idag
+         real(c_double), value :: kappa
+      end function
+      integer(c_int) function c_hdbletm(he, u, ge, idag, coact, kappa, iflag, bc) bind(C, name="qqcd_hdbletm")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: he, u, ge, coact, kappa, bc
+         integer(c_int), value :: idag, iflag
+      end function
+      integer(c_int) function c_gammamult(ei, oi, eo, oo, mu, ieo, ibleo) bind(C, name="qqcd_gammamult")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ei, oi, eo, oo
+         integer(c_int), value :: mu, ieo, ibleo
+      end function
+      integer(c_int) function c_gam5x(a) bind(C, name="qqcd_gam5x")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: a
+      end function
+      integer(c_int) function c_vecdot(a, b, adb) bind(C, name="qqcd_vecdot")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: a, b, adb
+      end function
+      integer(c_int) function c_pointsource(be, bo, src, icsrc, idsrc) bind(C, name="qqcd_pointsource")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: be, bo, src
+         integer(c_int), value :: icsrc, idsrc
+      end function
+      integer(c_int) function c_gmresdr(phi, x, gm, resmax, itermin, itercount, u, coact, kappa, iflag, bc) &
+            bind(C, name="qqcd_gmresdr")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: phi, x, gm, itercount, u, coact, kappa, bc
+         real(c_double), value :: resmax
+         integer(c_int), value :: itermin, iflag
+      end function
+      integer(c_int) function c_gmresproject(b, x, gm, resmax, itermin, itercount, u, coact, kappa, iflag, bc) &
+            bind(C, name="qqcd_gmresproject")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: b, x, gm, itercount, u, coact, kappa, bc
+         real(c_double), value :: resmax
+         integer(c_int), value :: itermin, iflag
+      end function
+      integer(c_int) function c_psc(b, x, gm, resmax, itermin, itercount, u, coact, kappa, iflag, bc, &
+                                    isignal, resproj) bind(C, name="qqcd_psc")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: b, x, gm, itercount, u, coact, kappa, bc
+         real(c_double), value :: resmax, resproj
+         integer(c_int), value :: itermin, iflag, isignal
+      end function
+      integer(c_int) function c_bicgstab(phi, x, resmax, itermin, itercount, u, coact, kappa, iflag, bc) &
+            bind(C, name="qqcd_bicgstab")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: phi, x, itercount, u, coact, kappa, bc
+         real(c_double), value :: resmax
+         integer(c_int), value :: itermin, iflag
+      end function
+      integer(c_int) function c_fermprop(be, bo, kappa, inverter, coact, bc, resmax, itermin, ngm, vectype, u, &
+                                         xe, xo, isignal, itercount) bind(C, name="qqcd_fermprop")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: be, bo, kappa, coact, bc, ngm, u, xe, xo, itercount
+         real(c_double), value :: resmax
+         integer(c_int), value :: inverter, itermin, vectype, isignal
+      end function
+      integer(c_int) function c_solve_mixed(phi, x, gm, tol, basis, opp, recon, itol, keep, u, coact, kappa, &
+                                            bc, itercount) bind(C, name="qqcd_solve_mixed")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: phi, x, gm, u, coact, kappa, bc, itercount
+         real(c_double), value :: tol, itol
+         integer(c_int), value :: basis, opp, recon, keep
+      end function
+   end interface
+
+contains
+
+   subroutine fail(where, rc)
+      character(len=*), intent(in) :: where
+      integer(c_int), intent(in) :: rc
+      character(kind=c_char), pointer :: msg(:)
+      integer :: k, unit
+      character(len=1024) :: text
+      if (rc == 0) return
+      text = ' '
+      call c_f_pointer(c_last_error(), msg, [1024])
+      do k = 1, 1024
+         if (msg(k) == c_null_char) exit
+         text(k:k) = msg(k)
+      end do
+      open(newunit=unit, file=where//".ERROR", action="write", status="replace", form="formatted")
+      write(unit, fmt=*) trim(text)
+      close(unit, status="keep")
+      stop
+   end subroutine fail
+
+   subroutine ensure_started(bc)
+      integer, intent(in) :: bc(:)
+      integer(c_int), target :: n(4), b(4)
+      if (started) return
+      n = [nx, ny, nz, nt]
+      b = bc(1:4)
+      call fail("QQCD_HIP", c_init(n, b, 0_c_int))
+      started = .true.
+   end subroutine ensure_started
+
+   subroutine check_geometry(where, myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      character(len=*), intent(in) :: where
+      integer, intent(in) :: myid
+      logical, intent(in) :: ldiv(:)
+      integer, intent(in), optional :: vecbl(:,:), vecblinv(:,:), iblv(:,:)
+      logical, intent(in), optional :: lbd(:,:)
+      integer :: ibl, mu, unit
+      logical :: ok
+      ok = myid == 0 .and. .not. any(ldiv)
+      if (present(vecbl)) ok = ok .and. all(vecbl(1,1:8) == [1, 4, 6, 7, 10, 11, 13, 16]) &
+                                  .and. all(vecbl(2,1:8) == [2, 3, 5, 8, 9, 12, 14, 15])
+      if (present(vecblinv)) then
+         do ibl = 1, 16
+            ok = ok .and. vecblinv(2,ibl) == (ibl + 1)/2
+         end do
+      end if
+      if (present(iblv) .and. present(lbd)) then
+         do ibl = 1, 16
+            do mu = 1, 4
+               ok = ok .and. iblv(ibl,mu) == 1 + ieor(ibl - 1, ishft(1, mu - 1)) &
+                       .and. (lbd(ibl,mu) .eqv. iand(ishft(ibl - 1, -(mu - 1)), 1) == 0)
+            end do
+         end do
+      end if
+      if (.not. ok) then
+         open(newunit=unit, file=where//".ERROR", action="write", status="replace", form="formatted")
+         write(unit, fmt=*) "qqcd_hip supports one process with the canonical block tables only"
+         close(unit, status="keep")
+         stop
+      end if
+   end subroutine check_geometry
+
+   ! Refresh the resident links after the host changed u in place.
+   subroutine qqcd_gauge_changed(u, coact, bc)
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:)
+      real(kind=8), intent(in), target, contiguous :: coact(:,:,:)
+      integer, intent(in) :: bc(:)
+      call ensure_started(bc)
+      call fail("QQCD_HIP", c_gauge_load(c_loc(u), c_loc(coact)))
+   end subroutine qqcd_gauge_changed
+
+   subroutine qqcd_shutdown()
+      if (started) call fail("QQCD_HIP", c_finalize())
+      started = .false.
+   end subroutine qqcd_shutdown
+
+   ! ---- diracops ------------------------------------------------------
+
+   subroutine Hsingle(gout,u,gin,idag,coact,bc,gblclr,vecbl,vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd,iblv,MRT)
+      real(kind=8), intent(out), target, contiguous :: gout(:,:,:,:,:)
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: gin(:,:,:,:,:)
+      integer, intent(in) :: idag, gblclr, myid, MRT
+      real(kind=8), intent(in), target, contiguous :: coact(:,:,:)
+      integer, intent(in), target :: bc(:)
+      integer, intent(in) :: nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int), target :: b(4)
+      call check_geometry("DIRACOPS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      call ensure_started(bc)
+      b = bc(1:4)
+      call fail("DIRACOPS", c_hsingle(c_loc(gout), c_loc(u), c_loc(gin), idag, c_loc(coact), c_loc(b), gblclr))
+   end subroutine Hsingle
+
+   subroutine Hdouble(he,u,ge,idag,coact,kappa,bc,vecbl,vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd,iblv,MRT)
+      real(kind=8), intent(out), target, contiguous :: he(:,:,:,:,:)
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: ge(:,:,:,:,:)
+      integer, intent(in) :: idag, myid, MRT
+      real(kind=8), intent(in), target, contiguous :: coact(:,:,:)
+      real(kind=8), intent(in) :: kappa
+      integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int), target :: b(4)
+      call check_geometry("DIRACOPS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      call ensure_started(bc)
+      b = bc(1:4)
+      call fail("DIRACOPS", c_hdouble(c_loc(he), c_loc(u), c_loc(ge), idag, c_loc(coact), kappa, c_loc(b)))
+   end subroutine Hdouble
+
+   subroutine Hdbletm(he,u,GeeGooinv,ge,idag,coact,kappa,iflag,bc,vecbl,vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd, &
+                      iblv,MRT)
+      real(kind=8), intent(out), target, contiguous :: he(:,:,:,:,:)
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:)
+      real(kind=8), intent(in) :: GeeGooinv(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: ge(:,:,:,:,:)
+      integer, intent(in) :: idag, iflag, myid, MRT
+      real(kind=8), intent(in), target, contiguous :: coact(:,:,:)
+      real(kind=8), intent(in), target, contiguous :: kappa(:)
+      integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int), target :: b(4)
+      call check_geometry("DIRACOPS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      call ensure_started(bc)
+      b = bc(1:4)
+      call fail("DIRACOPS", c_hdbletm(c_loc(he), c_loc(u), c_loc(ge), idag, c_loc(coact), c_loc(kappa), &
+                                      iflag, c_loc(b)))
+   end subroutine Hdbletm
+
+   subroutine gammamult(evenIN, oddIN, evenOUT, oddOUT, mu, ieo, ibleo)
+      real(kind=8), intent(in), target, contiguous :: evenIN(:,:,:,:,:), oddIN(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: evenOUT(:,:,:,:,:), oddOUT(:,:,:,:,:)
+      integer, intent(in) :: mu, ieo, ibleo
+      if (.not. started) call fail("DIRACOPS", -1_c_int)
+      call fail("DIRACOPS", c_gammamult(c_loc(evenIN), c_loc(oddIN), c_loc(evenOUT), c_loc(oddOUT), mu, ieo, ibleo))
+   end subroutine gammamult
+
+   subroutine gam5x(a)
+      real(kind=8), intent(inout), target, contiguous :: a(:,:,:,:,:)
+      if (.not. started) call fail("BASICS", -1_c_int)
+      call fail("BASICS", c_gam5x(c_loc(a)))
+   end subroutine gam5x
+
+   ! ---- inverters ------------------------------------------------------
+
+   subroutine vecdot(a, b, adb, MRT2)
+      real(kind=8), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
+      real(kind=8), intent(out), target :: adb(2)
+      integer, intent(in) :: MRT2
+      if (.not. started) call fail("INVERTERS", -1_c_int)
+      call fail("INVERTERS", c_vecdot(c_loc(a), c_loc(b), c_loc(adb)))
+   end subroutine vecdot
+
+   subroutine bicgstab(rwdir,phi,x,resmax,itermin,itercount,u,GeeGooinv,iflag,kappa,coact,bc,vecbl,vecblinv, &
+                       myid,nn,ldiv,nms,lvbc,ib,lbd,iblv,MRT,MRT2)
+      character(len=*), intent(in) :: rwdir(:)
+      real(kind=8), intent(in), target, contiguous :: phi(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: x(:,:,:,:,:)
+      real(kind=8), intent(in) :: resmax
+      integer, intent(in) :: itermin, iflag, myid, MRT, MRT2
+      integer, intent(out), target :: itercount
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:), kappa(:), coact(:,:,:)
+      real(kind=8), intent(in) :: GeeGooinv(:,:,:,:,:)
+      integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int), target :: b(4)
+      call check_geometry("INVERTERS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      call ensure_started(bc)
+      b = bc(1:4)
+      call fail("INVERTERS", c_bicgstab(c_loc(phi), c_loc(x), resmax, itermin, c_loc(itercount), c_loc(u), &
+                                        c_loc(coact), c_loc(kappa), iflag, c_loc(b)))
+   end subroutine bicgstab
+
+   subroutine gmresdr(rwdir,phi,x,GMRES,resmax,itermin,itercount,u,GeeGooinv,iflag,kappa,coact,bc,vecbl, &
+                      vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd,iblv,MRT,MRT2)
+      character(len=*), intent(in) :: rwdir(:)
+      real(kind=8), intent(in), target, contiguous :: phi(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: x(:,:,:,:,:)
+      integer, intent(in), target :: GMRES(:)
+      real(kind=8), intent(in) :: resmax
+      integer, intent(in) :: itermin, iflag, myid, MRT, MRT2
+      integer, intent(out), target :: itercount
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:), kappa(:), coact(:,:,:)
+      real(kind=8), intent(in) :: GeeGooinv(:,:,:,:,:)
+      integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int), target :: b(4), gm(2)
+      call check_geometry("INVERTERS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      call ensure_started(bc)
+      b = bc(1:4)
+      gm = GMRES(1:2)
+      call fail("INVERTERS", c_gmresdr(c_loc(phi), c_loc(x), c_loc(gm), resmax, itermin, c_loc(itercount), &
+                                       c_loc(u), c_loc(coact), c_loc(kappa), iflag, c_loc(b)))
+   end subroutine gmresdr
+
+   subroutine psc(rwdir,b,x,GMRES,resmax,itermin,itercount,u,GeeGooinv,iflag,kappa,coact,bc,vecbl, &
+                  vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd,iblv,MRT,MRT2)
+      use gmresrhs, only: isignal
+      character(len=*), intent(in) :: rwdir(:)
+      real(kind=8), intent(in), target, contiguous :: b(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: x(:,:,:,:,:)
+      integer, intent(in), target :: GMRES(:)
+      real(kind=8), intent(in) :: resmax
+      integer, intent(in) :: itermin, iflag, myid, MRT, MRT2
+      integer, intent(out), target :: itercount
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:), kappa(:), coact(:,:,:)
+      real(kind=8), intent(in) :: GeeGooinv(:,:,:,:,:)
+      integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int), target :: bb(4), gm(2)
+      call check_geometry("INVERTERS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      call ensure_started(bc)
+      bb = bc(1:4)
+      gm = GMRES(1:2)
+      call fail("INVERTERS", c_psc(c_loc(b), c_loc(x), c_loc(gm), resmax, itermin, c_loc(itercount), c_loc(u), &
+                                   c_loc(coact), c_loc(kappa), iflag, c_loc(bb), isignal, 0.0_c_double))
+   end subroutine psc
+
+   subroutine gmresproject(rwdir,b,xshift,GMRES,resmax,itermin,itercount,u,GeeGooinv,iflag,kappa,coact,bc, &
+                           vecbl,vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd,iblv,MRT,MRT2)
+      character(len=*), intent(in) :: rwdir(:)
+      real(kind=8), intent(in), target, contiguous :: b(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: xshift(:,:,:,:,:)
+      integer, intent(in), target :: GMRES(:)
+      real(kind=8), intent(in) :: resmax
+      integer, intent(in) :: itermin, iflag, myid, MRT, MRT2
+      integer, intent(out), target :: itercount
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:), kappa(:), coact(:,:,:)
+      real(kind=8), intent(in) :: GeeGooinv(:,:,:,:,:)
+      integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int), target :: bb(4), gm(2)
+      call check_geometry("INVERTERS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      call ensure_started(bc)
+      bb = bc(1:4)
+      gm = GMRES(1:2)
+      call fail("INVERTERS", c_gmresproject(c_loc(b), c_loc(xshift), c_loc(gm), resmax, itermin, &
+                                            c_loc(itercount), c_loc(u), c_loc(coact), c_loc(kappa), iflag, c_loc(bb)))
+   end subroutine gmresproject
+
+   ! Mixed-precision solve of the even-odd system (no counterpart in the
+   ! reference): tol on ||phi - M x||/||phi||, GMRES = (m, k), operator
+   ! storage opprec (1 FP32, 2 FP16) with recon 18 or 12 link reals.
+   subroutine qqcd_solve_mixed_f(phi, x, GMRES, tol, opprec, recon, keep, u, kappa, coact, bc, itercount)
+      real(kind=8), intent(in), target, contiguous :: phi(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: x(:,:,:,:,:)
+      integer, intent(in), target :: GMRES(:)
+      real(kind=8), intent(in) :: tol
+      integer, intent(in) :: opprec, recon, keep
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:), kappa(:), coact(:,:,:)
+      integer, intent(in) :: bc(:)
+      integer, intent(out), target :: itercount
+      integer(c_int), target :: bb(4), gm(2)
+      call ensure_started(bc)
+      bb = bc(1:4)
+      gm = GMRES(1:2)
+      call fail("INVERTERS", c_solve_mixed(c_loc(phi), c_loc(x), c_loc(gm), tol, 1_c_int, opprec, recon, &
+                                           0.0_c_double, keep, c_loc(u), c_loc(coact), c_loc(kappa), c_loc(bb), &
+                                           c_loc(itercount)))
+   end subroutine qqcd_solve_mixed_f
+
+   ! ---- quark ----------------------------------------------------------
+
+   subroutine pointsource(be,bo,src,icsrc,idsrc,myid,iblv,vecblinv)
+      real(kind=8), intent(out), target, contiguous :: be(:,:,:,:,:), bo(:,:,:,:,:)
+      integer, intent(in), target :: src(:)
+      integer, intent(in) :: icsrc, idsrc, myid
+      integer, intent(in) :: iblv(:,:), vecblinv(:,:)
+      integer(c_int), target :: s(4)
+      if (.not. started) call fail("QUARK", -1_c_int)
+      s = src(1:4)
+      call fail("QUARK", c_pointsource(c_loc(be), c_loc(bo), c_loc(s), icsrc, idsrc))
+   end subroutine pointsource
+
+   subroutine fermprop(rwdir,be,bo,nkappa,kappa,cSW,inverter,coact,bc,resmax,itermin,omegaM3R,u,vectype, &
+                       kGMRES,xe,xo,myid,nn,ldiv,nms,lvbc,lvcl,ib,lbd,iblv,vecbl,vecblinv,ntmqcd,hoption,MRT,MRT2)
+      use gmresrhs, only: isignal
+      character(len=*), intent(in) :: rwdir(:)
+      real(kind=8), intent(in), target, contiguous :: be(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: bo(:,:,:,:,:)
+      integer, intent(in) :: nkappa, hoption, inverter, kGMRES, ntmqcd, itermin, myid, MRT, MRT2
+      real(kind=8), intent(in), target, contiguous :: kappa(:)
+      real(kind=8), intent(in) :: cSW, resmax, omegaM3R
+      real(kind=8), intent(in), target, contiguous :: coact(:,:,:)
+      integer, intent(in) :: bc(:), nms(:)
+      real(kind=8), intent(inout), target, contiguous :: u(:,:,:,:,:)
+      integer, intent(inout) :: vectype
+      real(kind=8), intent(inout), target, contiguous :: xe(:,:,:,:,:,:), xo(:,:,:,:,:,:)
+      integer, intent(in) :: nn(:,:), iblv(:,:), vecbl(:,:), vecblinv(:,:), lvbc(:,:,:), lvcl(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int), target :: b(4), ngm(2), it
+      if (nkappa /= -1 .or. ntmqcd /= 0) call fail("QUARK", -1_c_int)
+      call check_geometry("QUARK", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      call ensure_started(bc)
+      b = bc(1:4)
+      ngm = [vectype, kGMRES]
+      call fail("QUARK", c_fermprop(c_loc(be), c_loc(bo), c_loc(kappa), inverter, c_loc(coact), c_loc(b), resmax, &
+                                    itermin, c_loc(ngm), vectype, c_loc(u), c_loc(xe), c_loc(xo), isignal, c_loc(it)))
+      if (inverter == 0 .and. vectype == 1) vectype = 0
+   end subroutine fermprop
+
+end module qqcd_hip

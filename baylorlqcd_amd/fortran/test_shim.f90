@@ -1,0 +1,110 @@
+! test_shim.f90 -- a Fortran host calling the drop-in with the reference's
+! argument lists (8^3x16, one process).  Reads u.bin, be.bin, bo.bin (raw
+! float64 in the reference order) from the working directory, writes the
+! outputs next to them; tests/test_gpu_fortran.py compares them with the
+! oracle.
+program test_shim
+   use qqcd_hip
+   use gmresrhs, only: isignal
+   use latdims, only: nx, ny, nz, nt
+   implicit none
+   integer, parameter :: nvhalf = nx*ny*nz*nt/32
+   integer, parameter :: nbmax = max(2*nvhalf/nx, 2*nvhalf/ny, 2*nvhalf/nz, 2*nvhalf/nt)
+   integer, parameter :: ntotal = nvhalf + nbmax
+   real(kind=8), allocatable :: u(:,:,:,:,:), be(:,:,:,:,:), bo(:,:,:,:,:), g(:,:,:,:,:), h(:,:,:,:,:)
+   real(kind=8), allocatable :: xe(:,:,:,:,:,:), xo(:,:,:,:,:,:), ps_e(:,:,:,:,:), ps_o(:,:,:,:,:)
+   real(kind=8) :: coact(4,4,4), kappa(2), adb(2), gdummy(18,1,1,1,1)
+   integer :: bc(4), vecbl(2,8), vecblinv(2,16), nn(4,2), nms(4), iblv(16,4), ibl, mu, itercount, vectype
+   integer, allocatable :: lvbc(:,:,:), ib(:,:,:,:)
+   logical :: ldiv(4), lbd(16,4)
+   character(len=8) :: rwdir(1)
+   integer :: src(4)
+
+   allocate(u(18,ntotal,4,2,16), be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8), g(6,ntotal,4,2,8), h(6,ntotal,4,2,8))
+   allocate(xe(6,ntotal,4,2,8,1), xo(6,ntotal,4,2,8,1), ps_e(6,ntotal,4,2,8), ps_o(6,ntotal,4,2,8))
+   allocate(lvbc(nvhalf,4,2), ib(nbmax,4,2,2))
+   call readbin("u.bin", u, size(u))
+   call readbin("be.bin", be, size(be))
+   call readbin("bo.bin", bo, size(bo))
+   ! single-process geometry tables (setblk / gblchker of the reference)
+   vecbl(1,:) = [1, 4, 6, 7, 10, 11, 13, 16]
+   vecbl(2,:) = [2, 3, 5, 8, 9, 12, 14, 15]
+   do ibl = 1, 16
+      vecblinv(1,ibl) = 1 + mod(popcnt(ibl - 1), 2)
+      vecblinv(2,ibl) = (ibl + 1)/2
+      do mu = 1, 4
+         iblv(ibl,mu) = 1 + ieor(ibl - 1, ishft(1, mu - 1))
+         lbd(ibl,mu) = iand(ishft(ibl - 1, -(mu - 1)), 1) == 0
+      end do
+   end do
+   nn = 0
+   nms = 0
+   lvbc = 0
+   ib = 0
+   ldiv = .false.
+   bc = [1, 1, 1, -1]
+   rwdir(1) = "./"
+   coact = 0.0d0
+   coact(3,1,:) = 1.0d0
+   coact(3,2,:) = 1.0d0
+   gdummy = 0.0d0
+
+   call Hsingle(g, u, be, 0, coact, bc, 2, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0)
+   call writebin("hsingle.bin", g, size(g))
+   kappa = [0.156d0, 0.005d0]
+   call Hdbletm(h, u, gdummy, be, 0, coact, kappa, -1, bc, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0)
+   call writebin("hdbletm.bin", h, size(h))
+   call Hdouble(h, u, be, 1, coact, 0.15d0, bc, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0)
+   call writebin("hdouble_dag.bin", h, size(h))
+   g = be
+   call gam5x(g)
+   call writebin("gam5x.bin", g, size(g))
+   call vecdot(be, bo, adb, 0)
+   call writebin("vecdot.bin", adb, 2)
+
+   ! config-1 solve: point source at the origin, GMRES-DR(40,10), Wilson kappa 0.150
+   src = [1, 1, 1, 1]
+   call pointsource(ps_e, ps_o, src, 1, 1, 0, iblv, vecblinv)
+   kappa = [0.150d0, 0.0d0]
+   vectype = 40
+   isignal = 1
+   call fermprop(rwdir, ps_e, ps_o, -1, kappa, 0.0d0, 5, coact, bc, 1.0d-10, 0, 1.0d0, u, vectype, 10, xe, xo, &
+                 0, nn, ldiv, nms, lvbc, lvbc, ib, lbd, iblv, vecbl, vecblinv, 0, 0, 0, 0)
+   call writebin("xe.bin", xe, size(xe))
+   call writebin("xo.bin", xo, size(xo))
+   ! psc: the same source again with isignal=2 runs GMRES-Proj over the deflation space
+   isignal = 2
+   vectype = 40
+   call fermprop(rwdir, ps_e, ps_o, -1, kappa, 0.0d0, 6, coact, bc, 1.0d-10, 0, 1.0d0, u, vectype, 10, xe, xo, &
+                 0, nn, ldiv, nms, lvbc, lvbc, ib, lbd, iblv, vecbl, vecblinv, 0, 0, 0, 0)
+   call writebin("xe_proj.bin", xe, size(xe))
+   ! mixed precision on the even-odd right-hand side of the first solve
+   g = 0.0d0
+   call qqcd_solve_mixed_f(be, g, [40, 16], 1.0d-10, 2, 12, 0, u, kappa, coact, bc, itercount)
+   call writebin("x_mixed.bin", g, size(g))
+   call qqcd_shutdown()
+   print *, "test_shim done"
+
+contains
+
+   subroutine readbin(name, a, n)
+      character(len=*), intent(in) :: name
+      integer, intent(in) :: n
+      real(kind=8), intent(out) :: a(n)
+      integer :: unit
+      open(newunit=unit, file=name, access="stream", form="unformatted", status="old", action="read")
+      read(unit) a
+      close(unit)
+   end subroutine readbin
+
+   subroutine writebin(name, a, n)
+      character(len=*), intent(in) :: name
+      integer, intent(in) :: n
+      real(kind=8), intent(in) :: a(n)
+      integer :: unit
+      open(newunit=unit, file=name, access="stream", form="unformatted", status="replace", action="write")
+      write(unit) a
+      close(unit)
+   end subroutine writebin
+
+end program test_shim

@@ -11,7 +11,15 @@ norms; 18- and 12-real links) are measured on the same input and reported in
 Algorithmic bytes per output site: 8*24*s_psi + 8*L*s_U + 24*s_psi
 (L = 18 or 12 link reals); FP16 spinors add 9*4 bytes of per-site norms.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+A second section times the solve path on the same lattice: a seeded
+beta = 6.0 heatbath field (device heatbath), a point source, twisted-mass
+kappa = 0.1565, mu = 0.005, solved to a true full-system residual
+||b - M x|| / ||b|| < 1e-10 by (a) the plain FP64 GMRES-DR(40,16) and (b) the
+mixed-precision solver (FP32 Krylov basis, FP16 / 12-real operator, FP64
+reliable updates); both times, iteration counts and residuals are reported
+in ``solve``.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-solve]
 """
 from __future__ import annotations
 
@@ -86,6 +94,43 @@ def cpu_baseline(U, vh):
             "sample": f"{reps} applications of the C oracle hopping term on 32^3x64 ({dt*1e3:.0f} ms each)"}
 
 
+def bench_solve(method, kappa, op_prec=2, recon=12, mk=(40, 16), tol=1e-10):
+    sec, res = np.zeros(1), np.zeros(1)
+    its, outer = np.zeros(1, np.int32), np.zeros(1, np.int32)
+    _lib.check(_lib.lib().qqcd_bench_solve(
+        int(method), _lib.ints((1, 1, 1, 1)).ctypes.data_as(_lib._ip), 1, 1, _lib.dp(np.asarray(kappa, float)),
+        _lib.ints(mk).ctypes.data_as(_lib._ip), float(tol), int(op_prec), int(recon), 0.0, 0, _lib.dp(sec),
+        its.ctypes.data_as(_lib._ip), outer.ctypes.data_as(_lib._ip), _lib.dp(res)))
+    return {"seconds": round(float(sec[0]), 4), "iterations": int(its[0]), "outer": int(outer[0]),
+            "true_residual": float(f"{res[0]:.3e}")}
+
+
+def solve_section():
+    """Time to a 1e-10 true residual on a 32^3x64 beta=6.0 heatbath field."""
+    co = np.zeros(64)
+    for mu in range(4):
+        co[2 + 4 * (0 + 4 * mu)] = 1.0
+        co[2 + 4 * (1 + 4 * mu)] = 1.0
+    t0 = time.perf_counter()
+    diracops.gauge_heatbath(6.0, 200, 2024, 1, co)
+    plaq = diracops.gauge_plaquette()
+    t_hb = time.perf_counter() - t0
+    print(f"[bench] heatbath 200 sweeps {t_hb:.1f}s, plaquette {plaq:.4f}", file=sys.stderr, flush=True)
+    kappa = (0.1565, 0.005)
+    plain = bench_solve(0, kappa)
+    print(f"[bench] plain FP64 GMRES-DR(40,16): {plain}", file=sys.stderr, flush=True)
+    mixed = bench_solve(1, kappa, 2, 12)
+    print(f"[bench] mixed FP32 basis / FP16-12 operator: {mixed}", file=sys.stderr, flush=True)
+    mixed32 = bench_solve(1, kappa, 1, 12)
+    print(f"[bench] mixed FP32 basis / FP32-12 operator: {mixed32}", file=sys.stderr, flush=True)
+    return {"lattice": list(N), "beta": 6.0, "heatbath_sweeps": 200, "plaquette": round(plaq, 5),
+            "kappa": kappa[0], "mu": kappa[1], "source": "point, origin, spin 1 colour 1",
+            "target": "||b - M x|| / ||b|| < 1e-10 (FP64, full system)",
+            "plain_fp64_gmresdr_40_16": plain, "mixed_fp16op_fp32basis": mixed, "mixed_fp32op_fp32basis": mixed32,
+            "speedup_mixed_vs_plain": round(plain["seconds"] / mixed["seconds"], 2),
+            "learned_preconditioner": "not built yet"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +138,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="also time a 1 GiB plain copy kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
@@ -126,6 +173,18 @@ def main():
         sweep.append({"op": "Hdbletm (2 hops + twisted epilogue), per hop", "storage": "f64", "links": 18,
                       "us": round(m2 * 1e3, 2)})
 
+    copy = None
+    if args.copy_probe:
+        cms = np.zeros(1)
+        _lib.check(_lib.lib().qqcd_bench_copy(float(1 << 30), 20, _lib.dp(cms)))
+        copy = {"bytes_per_launch": 2 * (1 << 30), "us": round(cms[0] * 1e3, 1),
+                "GB/s": round(2 * (1 << 30) / (cms[0] * 1e-3) / 1e9, 1)}
+        print(f"[bench] copy {copy}", file=sys.stderr, flush=True)
+
+    solve = None
+    if not args.no_solve:
+        solve = solve_section()
+
     prof = None
     ppath = os.path.join(ROOT, "profiles", "traffic_dslash_f64_18_32x32x32x64.json")
     if os.path.exists(ppath):
@@ -158,7 +217,10 @@ def main():
                      "frac": round(gbs / HBM_PEAK_GBS, 3), "traffic": prof},
         "cpu_baseline": cpu,
         "sweep": sweep,
+        "solve": solve,
     }
+    if copy:
+        line["copy_probe"] = copy
     print(json.dumps(line))
     lattice.finalize()
 

@@ -58,6 +58,17 @@ int qqcd_gauge_load_canonical(const double *U, const double *coact);
  * project's own generator; the reference reads fields from disk or makes
  * them with its heatbath. */
 int qqcd_gauge_random(int kind, double eps, unsigned long long seed, const double *coact);
+/* Quenched pseudo-heatbath at coupling beta on the device (Cabibbo-Marinari
+ * with Kennedy-Pendleton SU(2) updates, the algorithm of the reference's
+ * sweep: qqcd-test/cfgsprops/cfgsprops.f90:1973-2003,
+ * qqcd-test/cfgsprops/glue/heatbath.f90:33-467), nsweep sweeps from a cold
+ * (start 0) or hot (start 1, Haar) field, or continuing the resident one
+ * (start -1).  A counter-based generator replaces the reference's
+ * lagged-Fibonacci stream (same distribution, different trajectory). */
+int qqcd_gauge_heatbath(double beta, int nsweep, unsigned long long seed, int start, const double *coact);
+/* Average plaquette (1/3) Re Tr U_P of the resident field, as the
+ * reference's aveplaq (qqcd-test/cfgsprops/glue/gaugetools.f90:1436). */
+int qqcd_gauge_plaquette(double *plaq);
 /* Copy the resident FP64 field back in canonical order (see above). */
 int qqcd_gauge_get_canonical(double *U);
 
@@ -146,6 +157,19 @@ int qqcd_fermprop(const double *be, const double *bo, const double kappa[2], int
                   const int *bc, double resmax, int itermin, const int nGMRES[2], int vectype, const double *u,
                   double *xe, double *xo, int isignal, int *itercount);
 
+/* Mixed-precision solve of the even-odd system phi (gblclr=1 array) to
+ * ||phi - M x|| < tol ||phi||: FP64 solution, true residuals and reliable
+ * updates; inner GMRES-DR(GMRES[0], GMRES[1]) and then GMRES-Proj over its
+ * deflation space on a basis of basis_prec (0 FP64, 1 FP32) with the
+ * operator in (op_prec, op_recon) storage (0 FP64, 1 FP32, 2 FP16; 18 or 12
+ * link reals).  inner_tol <= 0 picks a default per op_prec.  keep_deflation
+ * = 1 reuses the deflation space of the previous mixed solve (same gauge
+ * field and kappa): the reference's first-RHS / later-RHS split of psc.  No
+ * counterpart in the reference, which is FP64 only. */
+int qqcd_solve_mixed(const double *phi, double *x, const int GMRES[2], double tol, int basis_prec, int op_prec,
+                     int op_recon, double inner_tol, int keep_deflation, const double *u, const double *coact,
+                     const double kappa[2], const int *bc, int *itercount);
+
 /* Statistics of the last solver call: final ||r|| / ||r0|| (true residual),
  * Krylov steps, all operator applications, cycles, converged flag. */
 int qqcd_last_solve_stats(double *rel_res, int *iters, int *matvecs, int *cycles, int *converged);
@@ -171,6 +195,19 @@ int qqcd_hop_canonical(const double *psi, double *out, int par_out, int idag, in
 
 /* Schur operator (Hdbletm semantics) on the even sites of canonical arrays. */
 int qqcd_schur_canonical(const double *psi, double *out, const double kappa[2], int idag, int prec, int recon);
+
+/* Time one device-resident solve of M x = point source (1-based site src,
+ * Dirac spin, colour) on the resident gauge field: method 0 plain FP64
+ * GMRES-DR(m,k), method 1 the mixed solver of qqcd_solve_mixed.  *seconds
+ * covers source preparation, solve and reconstruction; *true_res is the
+ * FP64 ||b - M x|| / ||b|| of the full system. */
+int qqcd_bench_solve(int method, const int src[4], int spin, int colour, const double kappa[2], const int GMRES[2],
+                     double tol, int op_prec, int op_recon, double inner_tol, int keep_deflation, double *seconds,
+                     int *iters, int *outer, double *true_res);
+
+/* Time a plain device-to-device copy kernel (read + write `bytes` each
+ * launch): the calibration point for hardware byte counters. */
+int qqcd_bench_copy(double bytes, int iters, double *ms);
 
 /* Toggle the XCD-aware workgroup order of the Dslash (default on). */
 int qqcd_set_xcd_remap(int on);

@@ -1,0 +1,80 @@
+"""The Fortran drop-in (baylorlqcd_amd/fortran/qqcd_hip.f90) called by a
+Fortran host with the reference's argument lists, on the GPU.
+
+The host program (baylorlqcd_amd/fortran/test_shim.f90, built by
+__graft_entry__.build()) reads the config-1 links and a seeded spinor pair,
+calls Hsingle, Hdbletm, Hdouble(idag=1), gam5x, vecdot, pointsource +
+fermprop (GMRES-DR, then psc with isignal=2) and the mixed-precision solve;
+every output is compared with the oracle or the reference's golden solution.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from baylorlqcd_amd import gauge
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "baylorlqcd_amd", "fortran", "_build", "test_shim")
+GOLD = os.path.join(ROOT, "tests", "golden", "golden_8x8x8x16.npz")
+N, BC = (8, 8, 8, 16), (1, 1, 1, -1)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def rows(a):
+    nvh, ntot = O.ref_sizes(N)
+    return a.reshape(6, ntot, 64, order="F")[:, :nvh]
+
+
+def test_fortran_host(tmp_path):
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    g = np.load(GOLD)
+    U = gauge.reconstruct12(g["links12"])
+    u = O.links_canon_to_ref(N, U)
+    rng = np.random.default_rng(31)
+    V = int(np.prod(N))
+    psi = rng.standard_normal((V, 4, 3)) + 1j * rng.standard_normal((V, 4, 3))
+    be, bo = O.spinor_canon_to_ref(N, psi)
+    u.tofile(tmp_path / "u.bin")
+    be.tofile(tmp_path / "be.bin")
+    bo.tofile(tmp_path / "bo.bin")
+    print("[fortran] running test_shim", flush=True)
+    out = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(out.stdout[-2000:], out.stderr[-2000:])
+    assert out.returncode == 0 and "test_shim done" in out.stdout
+    for f in tmp_path.glob("*.ERROR"):
+        pytest.fail(f.read_text())
+
+    def load(name):
+        return np.fromfile(tmp_path / name)
+
+    co = O.coact_array()
+    assert rel(rows(load("hsingle.bin")), rows(O.hsingle_ref(N, u, co, BC, 2, 0, be))) < 1e-13
+    assert rel(rows(load("hdbletm.bin")), rows(O.hdbletm_ref(N, u, co, BC, 0, (0.156, 0.005), be))) < 1e-13
+    assert rel(rows(load("hdouble_dag.bin")), rows(O.hdbletm_ref(N, u, co, BC, 1, (0.15, 0.0), be))) < 1e-13
+    G5 = O.gamma_table()[4]
+    w5, _ = O.spinor_canon_to_ref(N, np.einsum("de,sec->sdc", G5, psi))
+    assert rel(rows(load("gam5x.bin")), rows(w5)) < 1e-15
+    a, b = rows(be), rows(bo)
+    want = np.sum((a[0::2] - 1j * a[1::2]) * (b[0::2] + 1j * b[1::2]))
+    got = load("vecdot.bin")
+    assert abs(complex(got[0], got[1]) - want) < 1e-12 * abs(want)
+    # config-1 solve through fermprop
+    odd = O.parity_mask(N)
+    x = O.spinor_ref_to_canon(N, load("xe.bin"), load("xo.bin"))
+    assert rel(x[~odd], g["gmresdr_k150_xe"]) < 1e-8
+    assert rel(x[odd], g["gmresdr_k150_xo"]) < 1e-8
+    xp = O.spinor_ref_to_canon(N, load("xe_proj.bin"), None)
+    assert rel(xp[~odd], g["gmresdr_k150_xe"]) < 1e-6
+    # mixed precision on the even-odd system M_hat x = be
+    xm = O.spinor_ref_to_canon(N, load("x_mixed.bin"), None)
+    bb = O.spinor_ref_to_canon(N, be, None)
+    r = bb - O.schur(N, U, BC, [1] * 4, [1] * 4, 0, (0.150, 0.0), xm)
+    assert np.linalg.norm(r[~odd]) / np.linalg.norm(bb[~odd]) < 1e-10
