@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libqqcdhip.so")
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _lp = ctypes.POINTER(ctypes.c_long)
+_dpp = ctypes.POINTER(_dp)
 _L = None
 
 # (name, argtypes) of every exported entry point of include/qqcd_hip.h
@@ -60,6 +61,22 @@ SIGNATURES = {
     "qqcd_dense_reorder": [ctypes.c_int, _dp, _dp, ctypes.c_int],
     "qqcd_dense_lsq": [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp],
     "qqcd_dense_harmonic_ritz": [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp],
+    "qqcd_z4source": [_dp, _dp, ctypes.c_ulonglong, ctypes.c_long, ctypes.c_double],
+    "qqcd_fermprop_multi": [ctypes.c_int, _dpp, _dpp, _dp, _dp, _ip, ctypes.c_double, _ip, _dp, _dpp, _dpp, _ip,
+                            _dp],
+    "qqcd_noise_loop": [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, _dp, _dp, _ip, _dp, _ip,
+                        ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _dp, _ip],
+    "qqcd_trace16": [_dp, _dp, _dp, _dp, _dp],
+    "qqcd_batch_set": [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                       ctypes.c_int],
+    "qqcd_batch_stats": [_ip, _ip, _lp, _lp, _ip, _ip],
+    "qqcd_hop_multi_canonical": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp],
+    "qqcd_schur_multi_canonical": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                   ctypes.c_int, ctypes.c_int, _dp, _dp],
+    "qqcd_bench_dslash_multi": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
+    "qqcd_autotune_multi": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp],
 }
 
 
@@ -105,6 +122,12 @@ def ip(a):
         return ctypes.cast(None, _ip)
     a = np.ascontiguousarray(a, dtype=np.int32)
     return a.ctypes.data_as(_ip), a
+
+
+def dpp(arrays):
+    """A C array of double* for a list of contiguous float64 arrays."""
+    ptrs = (_dp * len(arrays))(*[dp(a) for a in arrays])
+    return ctypes.cast(ptrs, _dpp), ptrs
 
 
 def ints(v):

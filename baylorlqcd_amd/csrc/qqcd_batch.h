@@ -1,0 +1,78 @@
+// qqcd_batch.h -- batched solves of many right-hand sides on one gauge field.
+#pragma once
+#include <vector>
+
+#include "qqcd_multi.h"
+#include "qqcd_solvers.h"
+
+namespace qqcd {
+
+// The even-odd Schur operator (schur_apply) on the active slots of multi-RHS fields.
+struct MultiSchur {
+  Context &c;
+  const Gauge &U;
+  double kappa[2];
+  int idag;
+  MultiBuf tmp;
+  MultiTune tune;
+  MultiSchur(Context &c_, Prec p, int recon, const double k[2], int cap, int nb, const MultiTune &t,
+             int idag_ = 0);
+  void operator()(const MultiSpinor &in, MultiSpinor &out, const int *nact);
+};
+
+// Batched GMRES-Proj with reliable updates.  Every right-hand side b_r is
+// solved to ||b_r - A x_r|| < tol ||b_r|| in FP64: an outer loop recomputes
+// the true residuals with the FP64 operator and adds corrections from an
+// inner batched GMRES-Proj(m) over the FP32 deflation space of an earlier
+// GMRES-DR solve (projection over (V_{k+1}, Hbar_k), then a GMRES(m) cycle;
+// the reference's gmresproject, qqcd-test/cfgsprops/quark/inverters.f90:
+// 7183-8304, with psc's split m-k / k, :4439-4458).  All right-hand sides
+// advance in lockstep: one multi-RHS operator application serves every
+// active slot, the Gram-Schmidt coefficients stay on the device, and a slot
+// whose residual meets its target is retired (swapped behind the active
+// slots, so later launches skip it).  Right-hand sides that do not
+// converge within max_outer outer steps are finished one by one with the
+// single-RHS mixed solver (whose last resort is plain FP64 GMRES-DR).
+struct BatchParams {
+  int m = 40;             // GMRES cycle length
+  double tol = 1e-10;     // per right-hand side, FP64 true residual
+  double inner_tol = 1e-5;
+  int inner_cycles = 4;   // GMRES-Proj cycles per inner solve
+  int max_outer = 30;
+  int reorth = 0;         // 0: one Gram-Schmidt pass; 1: second pass per slot when needed; 2: always
+  int use_graph = 1;      // replay the Arnoldi cycle from a captured graph
+  Prec op_prec = PREC_SINGLE;
+  int op_recon = 12;
+  MultiTune tune;
+};
+
+struct BatchStats {
+  int outer = 0;
+  int cycles = 0;
+  long slot_matvecs = 0;     // operator applications summed over right-hand sides
+  long batched_matvecs = 0;  // multi-RHS operator launches (pairs of hopping kernels)
+  int graph_replays = 0;
+  int fallback = 0;          // right-hand sides finished by the single-RHS solver
+  std::vector<int> iters;    // per right-hand side: Arnoldi steps while active
+  std::vector<double> rel_res;
+  int converged = 0;
+};
+
+// Solves A x_r = b_r for the right-hand sides in slots [0, n) of B (FP64);
+// X (FP64, same layout) holds the initial guesses and gets the solutions.
+// Slots are permuted while right-hand sides retire; on return slot s holds
+// right-hand side id[s] (id is the identity on entry unless given).
+BatchStats batched_solve(Context &c, const double kappa[2], MultiSpinor &B, MultiSpinor &X, int n,
+                         const Deflation &D, const BatchParams &p, std::vector<int> &id);
+
+// Autotuner of the multi-RHS hopping kernel: times every RHS block nb in
+// {8, 16, 32, 64} (<= the power of two above nrhs) and tile order for the
+// given storage and reports the fastest.  Results are cached per shape.
+struct TuneResult {
+  int nb = 16;
+  MultiTune t;
+  double ms = 0;
+};
+TuneResult autotune_multi(Context &c, Prec p, int recon, int nrhs);
+
+}  // namespace qqcd

@@ -262,9 +262,8 @@ extern "C" int qqcd_dense_harmonic_ritz(int m, const double *hbar, int ldh, int 
 
 // ---- mixed precision ----------------------------------------------------
 
-namespace {
-
-MixedParams mixed_params(const int GMRES[2], double tol, int basis, int op_prec, int op_recon, double inner_tol) {
+MixedParams qqcd::mixed_params(const int GMRES[2], double tol, int basis, int op_prec, int op_recon,
+                               double inner_tol) {
   check_mk(GMRES[0], GMRES[1]);
   if (basis != 0 && basis != 1) throw std::runtime_error("basis precision must be 0 (FP64) or 1 (FP32)");
   if (op_prec < 0 || op_prec > 2 || (op_recon != 18 && op_recon != 12))
@@ -281,6 +280,8 @@ MixedParams mixed_params(const int GMRES[2], double tol, int basis, int op_prec,
   else p.inner_tol = op_prec == 2 ? 1e-2 : (op_prec == 1 ? 1e-6 : 1e-10);
   return p;
 }
+
+namespace {
 
 // unit source at 0-based global site x, Dirac d, colour c (pointsource)
 void point_source_dev(Context &c, Spinor &b_e, Spinor &b_o, const int src[4], int d, int col) {

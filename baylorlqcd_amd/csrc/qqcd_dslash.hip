@@ -19,7 +19,9 @@
 // pair, so the t hops load half of the neighbour spinor.  The link loads,
 // spinor loads and the FP16 per-site norm are all coalesced 64-lane runs.
 // Arithmetic is FP64 for FP64 storage and FP32 for FP32 and FP16 storage.
+#include <algorithm>
 #include <stdexcept>
+#include <type_traits>
 #include "qqcd_internal.h"
 
 namespace qqcd {
@@ -32,6 +34,7 @@ template <typename S> struct SIO;
 
 template <> struct SIO<double> {
   typedef double R;
+  typedef double2 E;
   static __device__ __forceinline__ cx<R> ld(const void *b, long stride, int k, int i, float) {
     const double2 v = ((const double2 *)b)[(long)k * stride + i];
     return cx<R>{v.x, v.y};
@@ -46,6 +49,7 @@ template <> struct SIO<double> {
 
 template <> struct SIO<float> {
   typedef float R;
+  typedef float2 E;
   static __device__ __forceinline__ cx<R> ld(const void *b, long stride, int k, int i, float) {
     const float2 v = ((const float2 *)b)[(long)k * stride + i];
     return cx<R>{v.x, v.y};
@@ -62,6 +66,7 @@ typedef f16 h2 __attribute__((ext_vector_type(2)));
 
 template <> struct SIO<f16> {
   typedef float R;
+  typedef h2 E;
   static __device__ __forceinline__ cx<R> ld(const void *b, long stride, int k, int i, float nrm) {
     const h2 v = ((const h2 *)b)[(long)k * stride + i];
     return cx<R>{nrm * (float)v.x, nrm * (float)v.y};
@@ -264,47 +269,46 @@ __device__ __forceinline__ void hop_t(const HopIn &a, cx<R> acc[12], int j, int 
 // sign in every direction, forward c1 (1 + SIG g), backward c1 (1 - SIG g),
 // sc = c1; SIG = 0: split form, forward fp (1+g) + fm (1-g), backward
 // bp (1+g) + bm (1-g).
+// i: link index of the output site; jb: link index of the backward
+// neighbour; jfs / jbs: spinor element indices of the two neighbours (equal
+// to the site indices for one right-hand side, site*nrhs + r for several).
 template <typename S, typename L, int RECON, int SIG, int MU, typename R>
-__device__ __forceinline__ void direction(const HopIn &h, int par, cx<R> acc[12], int i, int jf, int jb, R ef,
-                                          R eb, R sc, R fp, R fm, R bp, R bm) {
+__device__ __forceinline__ void direction(const HopIn &h, int par, cx<R> acc[12], int i, int jfs, int jb, int jbs,
+                                          R ef, R eb, R sc, R fp, R fm, R bp, R bm) {
   const int opar = par ^ 1;
+  const int jf = jfs;
   if (MU == 3) {
     if (SIG != 0) {
       hop_t<S, L, RECON, true, (SIG > 0)>(h, acc, jf, par, i, sc, ef);
-      hop_t<S, L, RECON, false, (SIG < 0)>(h, acc, jb, opar, jb, sc, eb);
+      hop_t<S, L, RECON, false, (SIG < 0)>(h, acc, jbs, opar, jb, sc, eb);
     } else {
       if (fp != R(0)) hop_t<S, L, RECON, true, true>(h, acc, jf, par, i, fp, ef);
       if (fm != R(0)) hop_t<S, L, RECON, true, false>(h, acc, jf, par, i, fm, ef);
-      if (bp != R(0)) hop_t<S, L, RECON, false, true>(h, acc, jb, opar, jb, bp, eb);
-      if (bm != R(0)) hop_t<S, L, RECON, false, false>(h, acc, jb, opar, jb, bm, eb);
+      if (bp != R(0)) hop_t<S, L, RECON, false, true>(h, acc, jbs, opar, jb, bp, eb);
+      if (bm != R(0)) hop_t<S, L, RECON, false, false>(h, acc, jbs, opar, jb, bm, eb);
     }
     return;
   }
   if (SIG != 0) {
     hop<S, L, RECON, MU, true>(h, acc, jf, par, i, R(SIG), sc, ef);
-    hop<S, L, RECON, MU, false>(h, acc, jb, opar, jb, R(-SIG), sc, eb);
+    hop<S, L, RECON, MU, false>(h, acc, jbs, opar, jb, R(-SIG), sc, eb);
   } else {
     if (fp != R(0)) hop<S, L, RECON, MU, true>(h, acc, jf, par, i, R(1), fp, ef);
     if (fm != R(0)) hop<S, L, RECON, MU, true>(h, acc, jf, par, i, R(-1), fm, ef);
-    if (bp != R(0)) hop<S, L, RECON, MU, false>(h, acc, jb, opar, jb, R(1), bp, eb);
-    if (bm != R(0)) hop<S, L, RECON, MU, false>(h, acc, jb, opar, jb, R(-1), bm, eb);
+    if (bp != R(0)) hop<S, L, RECON, MU, false>(h, acc, jbs, opar, jb, R(1), bp, eb);
+    if (bm != R(0)) hop<S, L, RECON, MU, false>(h, acc, jbs, opar, jb, R(-1), bm, eb);
   }
 }
 
-template <typename S, typename L, int RECON, int SIG>
-__global__ void __launch_bounds__(256) k_dslash(const DslashArgs a) {
+// Output site i of checkerboard a.par.  MULTI: the fields hold nb = 2^lognb
+// right-hand sides innermost (element (site j, rhs rr) at j*nb + rr) and this
+// lane computes right-hand side rr; h, out and x point at the lane's block.
+// Link indices are site indices in both cases, so the lanes of one site load
+// the same link words (one broadcast transaction per wavefront).
+template <typename S, typename L, int RECON, int SIG, bool MULTI>
+__device__ __forceinline__ void dslash_site(const DslashArgs &a, const HopIn &h, void *out, float *out_norm,
+                                            const void *xin, const float *x_norm, int i, int rr, int lognb) {
   typedef typename SIO<S>::R R;
-  int blk = blockIdx.x;
-  if (a.xcd_remap) {
-    // workgroups are dealt round-robin to the 8 XCDs; give each XCD a
-    // contiguous run of sites so neighbour reuse stays in its own L2
-    const int nb = gridDim.x, per = (nb + 7) / 8;
-    const int xcd = blk & 7, k = blk >> 3;
-    const int full = nb - 8 * (per - 1);  // XCDs that get `per` blocks
-    blk = xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
-  }
-  const int i = blk * blockDim.x + threadIdx.x;
-  if (i >= a.Vh) return;
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2], nt = a.n[3];
   const int r = 2 * i;
   const int x0 = r % nx;
@@ -315,8 +319,8 @@ __global__ void __launch_bounds__(256) k_dslash(const DslashArgs a) {
   const int t = q / nz;
   const int x = x0 + ((a.par + y + z + t) & 1);
   const int yzt = y + ny * (z + nz * t);
-  const HopIn h{a.in, a.in_norm, a.links, a.sstride, a.lstride};
   const int par = a.par;
+#define SIDX(j) (MULTI ? ((j) << lognb) + rr : (j))
 #define DIRARGS(M) (R) a.scl[M], (R) a.fw_p[M], (R) a.fw_m[M], (R) a.bw_p[M], (R) a.bw_m[M]
 
   cx<R> acc[12];
@@ -326,27 +330,27 @@ __global__ void __launch_bounds__(256) k_dslash(const DslashArgs a) {
   {  // x
     const int xf = x + 1 == nx ? 0 : x + 1, xb = x == 0 ? nx - 1 : x - 1;
     const R ef = x + 1 == nx ? (R)a.bcf[0] : R(1), eb = x == 0 ? (R)a.bcf[0] : R(1);
-    direction<S, L, RECON, SIG, 0>(h, par, acc, i, (xf + nx * yzt) >> 1, (xb + nx * yzt) >> 1, ef, eb,
-                                    DIRARGS(0));
+    const int jf = (xf + nx * yzt) >> 1, jb = (xb + nx * yzt) >> 1;
+    direction<S, L, RECON, SIG, 0>(h, par, acc, i, SIDX(jf), jb, SIDX(jb), ef, eb, DIRARGS(0));
   }
   {  // y
     const int yf = y + 1 == ny ? 0 : y + 1, yb = y == 0 ? ny - 1 : y - 1;
     const R ef = y + 1 == ny ? (R)a.bcf[1] : R(1), eb = y == 0 ? (R)a.bcf[1] : R(1);
     const int zt = z + nz * t;
-    direction<S, L, RECON, SIG, 1>(h, par, acc, i, (x + nx * (yf + ny * zt)) >> 1,
-                                    (x + nx * (yb + ny * zt)) >> 1, ef, eb, DIRARGS(1));
+    const int jf = (x + nx * (yf + ny * zt)) >> 1, jb = (x + nx * (yb + ny * zt)) >> 1;
+    direction<S, L, RECON, SIG, 1>(h, par, acc, i, SIDX(jf), jb, SIDX(jb), ef, eb, DIRARGS(1));
   }
   {  // z
     const int zf = z + 1 == nz ? 0 : z + 1, zb = z == 0 ? nz - 1 : z - 1;
     const R ef = z + 1 == nz ? (R)a.bcf[2] : R(1), eb = z == 0 ? (R)a.bcf[2] : R(1);
-    direction<S, L, RECON, SIG, 2>(h, par, acc, i, (x + nx * (y + ny * (zf + nz * t))) >> 1,
-                                    (x + nx * (y + ny * (zb + nz * t))) >> 1, ef, eb, DIRARGS(2));
+    const int jf = (x + nx * (y + ny * (zf + nz * t))) >> 1, jb = (x + nx * (y + ny * (zb + nz * t))) >> 1;
+    direction<S, L, RECON, SIG, 2>(h, par, acc, i, SIDX(jf), jb, SIDX(jb), ef, eb, DIRARGS(2));
   }
   {  // t
     const int tf = t + 1 == nt ? 0 : t + 1, tb = t == 0 ? nt - 1 : t - 1;
     const R ef = t + 1 == nt ? (R)a.bcf[3] : R(1), eb = t == 0 ? (R)a.bcf[3] : R(1);
-    direction<S, L, RECON, SIG, 3>(h, par, acc, i, (x + nx * (y + ny * (z + nz * tf))) >> 1,
-                                    (x + nx * (y + ny * (z + nz * tb))) >> 1, ef, eb, DIRARGS(3));
+    const int jf = (x + nx * (y + ny * (z + nz * tf))) >> 1, jb = (x + nx * (y + ny * (z + nz * tb))) >> 1;
+    direction<S, L, RECON, SIG, 3>(h, par, acc, i, SIDX(jf), jb, SIDX(jb), ef, eb, DIRARGS(3));
   }
 #undef DIRARGS
 
@@ -360,12 +364,13 @@ __global__ void __launch_bounds__(256) k_dslash(const DslashArgs a) {
     o[6 + c] = b0 * acc[6 + c] - b1 * acc[c];
     o[9 + c] = b0 * acc[9 + c] - b1 * acc[3 + c];
   }
+  const int io = SIDX(i);
   if (a.has_x) {
     const R a0 = (R)a.a0, a1 = (R)a.a1;
-    const float nrm = SIO<S>::norm(a.x_norm, i);
+    const float nrm = SIO<S>::norm(x_norm, io);
     cx<R> xv[12];
 #pragma unroll
-    for (int k = 0; k < 12; k++) xv[k] = SIO<S>::ld(a.x, a.sstride, k, i, nrm);
+    for (int k = 0; k < 12; k++) xv[k] = SIO<S>::ld(xin, h.sstride, k, io, nrm);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       o[c] = o[c] + a0 * xv[c] + a1 * xv[6 + c];
@@ -374,7 +379,74 @@ __global__ void __launch_bounds__(256) k_dslash(const DslashArgs a) {
       o[9 + c] = o[9 + c] + a0 * xv[9 + c] - a1 * xv[3 + c];
     }
   }
-  SIO<S>::st(a.out, a.out_norm, a.sstride, i, o);
+#undef SIDX
+  SIO<S>::st(out, out_norm, h.sstride, io, o);
+}
+
+template <typename S, typename L, int RECON, int SIG>
+__global__ void __launch_bounds__(256) k_dslash(const DslashArgs a) {
+  int blk = blockIdx.x;
+  if (a.xcd_remap) {
+    // workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+    // contiguous run of sites so neighbour reuse stays in its own L2
+    const int nb = gridDim.x, per = (nb + 7) / 8;
+    const int xcd = blk & 7, k = blk >> 3;
+    const int full = nb - 8 * (per - 1);  // XCDs that get `per` blocks
+    blk = xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
+  }
+  const int i = blk * blockDim.x + threadIdx.x;
+  if (i >= a.Vh) return;
+  const HopIn h{a.in, a.in_norm, a.links, a.sstride, a.lstride};
+  dslash_site<S, L, RECON, SIG, false>(a, h, a.out, a.out_norm, a.x, a.x_norm, i, 0, 0);
+}
+
+// Several right-hand sides, RHS-innermost in blocks of nb = 2^lognb
+// (element (rhs r, component k, site j) of a field at
+// ((r/nb * 12 + k) * stride + j) * nb + r%nb).  A work tile is 256 lanes of
+// one RHS block: nb lanes per site, 256/nb consecutive sites.  The kernel is
+// persistent: a fixed grid (a multiple of 8 workgroups) walks the tiles of
+// the *active* blocks, and the active RHS count is read from device memory
+// at launch, so a launch captured once in a graph keeps working while
+// converged right-hand sides are retired (moved behind the active ones).
+// Tiles are dealt so that each XCD works on one contiguous range of tiles
+// per sweep.  order 0: RHS block outermost; 1: RHS block innermost (the
+// blocks of one site range run back to back and share the link loads in L2).
+struct MultiArgs {
+  const int *nact;
+  int lognb;
+  int order;
+  int site_tiles;
+  long blk_elems;  // elements of one RHS block (12 * stride * nb)
+  long blk_norms;  // norms of one RHS block (stride * nb)
+};
+
+template <typename S, typename L, int RECON, int SIG>
+__global__ void __launch_bounds__(256) k_dslash_multi(const DslashArgs a, const MultiArgs m) {
+  typedef typename SIO<S>::E E;
+  const int nact = *m.nact;
+  const int nb = 1 << m.lognb;
+  const int nblk = (nact + nb - 1) >> m.lognb;
+  const long ntiles = (long)m.site_tiles * nblk;
+  const int G = gridDim.x;
+  const int vw = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+  for (long t = vw; t < ntiles; t += G) {
+    int blk, st;
+    if (m.order == 0) {
+      blk = (int)(t / m.site_tiles);
+      st = (int)(t - (long)blk * m.site_tiles);
+    } else {
+      st = (int)(t / nblk);
+      blk = (int)(t - (long)st * nblk);
+    }
+    const int lane = st * 256 + threadIdx.x;
+    const int i = lane >> m.lognb, rr = lane & (nb - 1);
+    if (i >= a.Vh || (blk << m.lognb) + rr >= nact) continue;
+    const long eo = (long)blk * m.blk_elems, no = (long)blk * m.blk_norms;
+    const HopIn h{(const E *)a.in + eo, a.in_norm ? a.in_norm + no : nullptr, a.links, a.sstride, a.lstride};
+    dslash_site<S, L, RECON, SIG, true>(a, h, (E *)a.out + eo, a.out_norm ? a.out_norm + no : nullptr,
+                                        a.x ? (const E *)a.x + eo : nullptr, a.x_norm ? a.x_norm + no : nullptr, i,
+                                        rr, m.lognb);
+  }
 }
 
 HopCoef make_hopcoef(const Geom &g, int idag) {
@@ -407,21 +479,35 @@ static void launch_t(const DslashArgs &a, int sig, hipStream_t s) {
   QQCD_HIP(hipGetLastError());
 }
 
-void dslash(const Geom &g, const Gauge &U, const Spinor &in, Spinor &out, int par_out, int idag,
-            const Epilogue &ep, const Spinor *x, hipStream_t s) {
-  if (in.prec != out.prec || (x && x->prec != out.prec))
-    throw std::runtime_error("dslash: spinor precisions differ");
-  if (in.stride != g.stride || out.stride != g.stride || U.stride != g.stride)
-    throw std::runtime_error("dslash: field stride does not match the geometry");
-  if (in.data == out.data) throw std::runtime_error("dslash: in and out alias");
-  DslashArgs a{};
-  a.in = in.data;
-  a.in_norm = in.norm;
-  a.out = out.data;
-  a.out_norm = out.norm;
-  a.x = x ? x->data : nullptr;
-  a.x_norm = x ? x->norm : nullptr;
-  a.has_x = x ? 1 : 0;
+// resident workgroups of a persistent kernel: occupancy x CUs, a multiple of 8
+template <typename K> static int persistent_grid(K kern) {
+  int dev = 0, cus = 0, per = 0;
+  QQCD_HIP(hipGetDevice(&dev));
+  QQCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  QQCD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0));
+  const int g = std::max(8, (cus * std::max(per, 1)) / 8 * 8);
+  return g;
+}
+
+template <typename S, typename L, int RECON, int SIG>
+static void launch_multi_sig(const DslashArgs &a, const MultiArgs &m, int grid, hipStream_t s) {
+  static int g0 = 0;
+  if (!g0) g0 = persistent_grid(k_dslash_multi<S, L, RECON, SIG>);
+  const int g = grid > 0 ? (grid + 7) / 8 * 8 : g0;
+  hipLaunchKernelGGL((k_dslash_multi<S, L, RECON, SIG>), dim3(g), dim3(256), 0, s, a, m);
+  QQCD_HIP(hipGetLastError());
+}
+
+template <typename S, typename L, int RECON>
+static void launch_multi(const DslashArgs &a, const MultiArgs &m, int sig, int grid, hipStream_t s) {
+  if (sig > 0) launch_multi_sig<S, L, RECON, 1>(a, m, grid, s);
+  else if (sig < 0) launch_multi_sig<S, L, RECON, -1>(a, m, grid, s);
+  else launch_multi_sig<S, L, RECON, 0>(a, m, grid, s);
+}
+
+// Kernel arguments shared by the single- and multi-RHS launches; returns
+// the projector sign (+1 / -1 in every direction, or 0 for the split form).
+static int make_args(const Geom &g, const Gauge &U, int par_out, int idag, const Epilogue &ep, DslashArgs &a) {
   a.links = U.data;
   a.sstride = g.stride;
   a.lstride = U.stride;
@@ -446,38 +532,111 @@ void dslash(const Geom &g, const Gauge &U, const Spinor &in, Spinor &out, int pa
   a.b0 = ep.b0;
   a.b1 = ep.b1;
   a.xcd_remap = g_xcd_remap;
-  // one projector sign for every direction, or the split form
   int proj = 0;
   if (h.proj) {
     proj = a.sig_f[0] > 0 ? 1 : -1;
     for (int m = 1; m < 4; m++)
       if ((a.sig_f[m] > 0 ? 1 : -1) != proj) proj = 0;
   }
-  if (in.prec == PREC_DOUBLE) {
-    if (U.prec != PREC_DOUBLE) throw std::runtime_error("dslash: FP64 spinors need FP64 links");
-    if (U.recon == 18) launch_t<double, double, 18>(a, proj, s);
-    else launch_t<double, double, 12>(a, proj, s);
-  } else if (in.prec == PREC_SINGLE) {
+  return proj;
+}
+
+// Storage combinations: FP64 spinors with FP64 links; FP32 spinors with FP32
+// or FP16 links; FP16 spinors with FP16 or FP32 links.
+template <typename F>
+static void dispatch_prec(Prec sp, const Gauge &U, const char *who, F &&f) {
+  auto bad = [&](const char *m) { throw std::runtime_error(std::string(who) + ": " + m); };
+  const bool r18 = U.recon == 18;
+  if (sp == PREC_DOUBLE) {
+    if (U.prec != PREC_DOUBLE) bad("FP64 spinors need FP64 links");
+    if (r18) f((double *)0, (double *)0, std::integral_constant<int, 18>());
+    else f((double *)0, (double *)0, std::integral_constant<int, 12>());
+  } else if (sp == PREC_SINGLE) {
     if (U.prec == PREC_SINGLE) {
-      if (U.recon == 18) launch_t<float, float, 18>(a, proj, s);
-      else launch_t<float, float, 12>(a, proj, s);
+      if (r18) f((float *)0, (float *)0, std::integral_constant<int, 18>());
+      else f((float *)0, (float *)0, std::integral_constant<int, 12>());
     } else if (U.prec == PREC_HALF) {
-      if (U.recon == 18) launch_t<float, f16, 18>(a, proj, s);
-      else launch_t<float, f16, 12>(a, proj, s);
+      if (r18) f((float *)0, (f16 *)0, std::integral_constant<int, 18>());
+      else f((float *)0, (f16 *)0, std::integral_constant<int, 12>());
     } else {
-      throw std::runtime_error("dslash: FP32 spinors need FP32 or FP16 links");
+      bad("FP32 spinors need FP32 or FP16 links");
     }
   } else {
     if (U.prec == PREC_HALF) {
-      if (U.recon == 18) launch_t<f16, f16, 18>(a, proj, s);
-      else launch_t<f16, f16, 12>(a, proj, s);
+      if (r18) f((f16 *)0, (f16 *)0, std::integral_constant<int, 18>());
+      else f((f16 *)0, (f16 *)0, std::integral_constant<int, 12>());
     } else if (U.prec == PREC_SINGLE) {
-      if (U.recon == 18) launch_t<f16, float, 18>(a, proj, s);
-      else launch_t<f16, float, 12>(a, proj, s);
+      if (r18) f((f16 *)0, (float *)0, std::integral_constant<int, 18>());
+      else f((f16 *)0, (float *)0, std::integral_constant<int, 12>());
     } else {
-      throw std::runtime_error("dslash: FP16 spinors need FP32 or FP16 links");
+      bad("FP16 spinors need FP32 or FP16 links");
     }
   }
+}
+
+void dslash(const Geom &g, const Gauge &U, const Spinor &in, Spinor &out, int par_out, int idag,
+            const Epilogue &ep, const Spinor *x, hipStream_t s) {
+  if (in.prec != out.prec || (x && x->prec != out.prec))
+    throw std::runtime_error("dslash: spinor precisions differ");
+  if (in.stride != g.stride || out.stride != g.stride || U.stride != g.stride)
+    throw std::runtime_error("dslash: field stride does not match the geometry");
+  if (in.data == out.data) throw std::runtime_error("dslash: in and out alias");
+  DslashArgs a{};
+  a.in = in.data;
+  a.in_norm = in.norm;
+  a.out = out.data;
+  a.out_norm = out.norm;
+  a.x = x ? x->data : nullptr;
+  a.x_norm = x ? x->norm : nullptr;
+  a.has_x = x ? 1 : 0;
+  const int proj = make_args(g, U, par_out, idag, ep, a);
+  dispatch_prec(in.prec, U, "dslash", [&](auto *sp, auto *lp, auto rc) {
+    typedef typename std::remove_pointer<decltype(sp)>::type S;
+    typedef typename std::remove_pointer<decltype(lp)>::type L;
+    launch_t<S, L, decltype(rc)::value>(a, proj, s);
+  });
+}
+
+size_t MultiSpinor::bytes() const {
+  const size_t e = prec == PREC_DOUBLE ? 16 : prec == PREC_SINGLE ? 8 : 4;
+  return (size_t)elems() * e + (prec == PREC_HALF ? (size_t)stride * nb * nblk() * sizeof(float) : 0);
+}
+
+void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpinor &out, int par_out, int idag,
+                  const Epilogue &ep, const MultiSpinor *x, const int *d_nact, const MultiTune &t, hipStream_t s) {
+  if (in.prec != out.prec || (x && x->prec != out.prec))
+    throw std::runtime_error("dslash_multi: spinor precisions differ");
+  if (in.stride != g.stride || out.stride != g.stride || U.stride != g.stride)
+    throw std::runtime_error("dslash_multi: field stride does not match the geometry");
+  if (in.nb != out.nb || in.cap != out.cap || (x && (x->nb != in.nb || x->cap != in.cap)))
+    throw std::runtime_error("dslash_multi: right-hand-side layouts differ");
+  if (in.nb < 1 || in.nb > 256 || (in.nb & (in.nb - 1)) || (1 << in.lognb) != in.nb)
+    throw std::runtime_error("dslash_multi: RHS block must be a power of two <= 256");
+  if (in.data == out.data) throw std::runtime_error("dslash_multi: in and out alias");
+  if (!d_nact) throw std::runtime_error("dslash_multi: no active count");
+  DslashArgs a{};
+  a.in = in.data;
+  a.in_norm = in.norm;
+  a.out = out.data;
+  a.out_norm = out.norm;
+  a.x = x ? x->data : nullptr;
+  a.x_norm = x ? x->norm : nullptr;
+  a.has_x = x ? 1 : 0;
+  const int proj = make_args(g, U, par_out, idag, ep, a);
+  a.sstride = g.stride * in.nb;
+  MultiArgs m{};
+  m.nact = d_nact;
+  m.lognb = in.lognb;
+  m.order = t.order;
+  m.site_tiles = (int)((g.Vh * in.nb + 255) / 256);
+  m.blk_elems = in.blk_elems();
+  m.blk_norms = g.stride * in.nb;
+  if (in.prec == PREC_HALF) throw std::runtime_error("dslash_multi: FP64 or FP32 right-hand sides");
+  dispatch_prec(in.prec, U, "dslash_multi", [&](auto *sp, auto *lp, auto rc) {
+    typedef typename std::remove_pointer<decltype(sp)>::type S;
+    typedef typename std::remove_pointer<decltype(lp)>::type L;
+    if constexpr (!std::is_same<S, f16>::value) launch_multi<S, L, decltype(rc)::value>(a, m, proj, t.grid, s);
+  });
 }
 
 }  // namespace qqcd

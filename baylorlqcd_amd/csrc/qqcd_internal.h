@@ -68,6 +68,31 @@ struct Gauge {
   size_t bytes() const;
 };
 
+// Several right-hand sides on one checkerboard, RHS-innermost: slots are
+// grouped in blocks of nb = 2^lognb, and element (slot r, component k, site
+// j) sits at ((r/nb * 12 + k) * stride + j) * nb + r%nb, so the nb lanes of
+// one site are adjacent and share every link load.  FP16 fields carry one
+// float norm per (site, slot) at (r/nb * stride + j) * nb + r%nb.
+struct MultiSpinor {
+  Prec prec = PREC_SINGLE;
+  int cap = 0;  // right-hand-side slots
+  int nb = 1;
+  int lognb = 0;
+  long stride = 0;
+  void *data = nullptr;
+  float *norm = nullptr;
+  int nblk() const { return (cap + nb - 1) / nb; }
+  long blk_elems() const { return 12L * stride * nb; }
+  long elems() const { return blk_elems() * nblk(); }
+  size_t bytes() const;
+};
+
+// Launch shape of the multi-RHS hopping kernel (picked by the autotuner).
+struct MultiTune {
+  int order = 1;  // 0: RHS block outermost; 1: innermost
+  int grid = 0;   // persistent workgroups (0: occupancy x CUs)
+};
+
 // Coefficients of the local spin matrix applied in a Dslash epilogue:
 //   out = a0*x + a1*(i g5 x) + b0*D + b1*(i g5 D)
 // with D the hopping sum and x a spinor on the output checkerboard.
@@ -101,6 +126,11 @@ void set_error(const std::string &msg);
 // ---- kernels (qqcd_dslash.hip) ----
 void dslash(const Geom &g, const Gauge &U, const Spinor &in, Spinor &out, int par_out, int idag,
             const Epilogue &ep, const Spinor *x, hipStream_t s);
+
+// Hopping operator on the active slots [0, *d_nact) of a multi-RHS field
+// (d_nact in device memory); same epilogue as dslash.
+void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpinor &out, int par_out, int idag,
+                  const Epilogue &ep, const MultiSpinor *x, const int *d_nact, const MultiTune &t, hipStream_t s);
 
 // ---- layout (qqcd_layout.hip) ----
 long ref_nvhalf(const Geom &g);

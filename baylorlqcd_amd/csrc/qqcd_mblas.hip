@@ -1,0 +1,617 @@
+// qqcd_mblas.hip -- level-1 kernels on multi-right-hand-side fields.
+//
+// The batched counterparts of qqcd_blas.hip for the noise solves (many
+// right-hand sides on one gauge field; the reference runs them one after
+// the other, qqcd-test/cfgsprops/quark/disconloops.f90:2460-2511).  Every
+// per-slot dot product and norm accumulates in FP64 and is deterministic: a
+// workgroup of 256 lanes covers 256/nb sites x nb slots, the lanes of one
+// slot are combined by cross-lane shuffles over the site bits and a fixed
+// LDS pass, and one final pass sums the workgroup partials in a fixed order.
+// Coefficients are read from device arrays [q * cap + slot] and the active
+// count from device memory, so the kernels can be replayed from a graph.
+#include <algorithm>
+#include <functional>
+#include <stdexcept>
+#include <type_traits>
+
+#include "qqcd_multi.h"
+
+namespace qqcd {
+
+static constexpr int MB = 256;  // threads per workgroup
+
+template <typename T> struct E2;
+template <> struct E2<double> {
+  static __device__ __forceinline__ double2 ld(const void *p, long i) { return ((const double2 *)p)[i]; }
+  // stores (re, im) and returns the stored value
+  static __device__ __forceinline__ double2 st(void *p, long i, double re, double im) {
+    ((double2 *)p)[i] = make_double2(re, im);
+    return make_double2(re, im);
+  }
+};
+template <> struct E2<float> {
+  static __device__ __forceinline__ double2 ld(const void *p, long i) {
+    const float2 v = ((const float2 *)p)[i];
+    return make_double2(v.x, v.y);
+  }
+  static __device__ __forceinline__ double2 st(void *p, long i, double re, double im) {
+    const float2 v = make_float2((float)re, (float)im);
+    ((float2 *)p)[i] = v;
+    return make_double2(v.x, v.y);
+  }
+};
+
+struct MShape {
+  long rows;       // 12 * stride
+  long blk_elems;  // rows * nb
+  const int *nact;
+  int lognb;
+  int cap;
+  int gx;  // workgroups per RHS block
+};
+
+struct MPtr {
+  const void *p[8];
+};
+
+// Sum NV per-lane values over the lanes of each slot in the workgroup;
+// thread (k, r) with k < NV, r < nb then writes its total to
+// part[((k * cap + slot0 + r) * gx + bx].
+template <int NV>
+__device__ __forceinline__ void slot_sum(double (&v)[NV], int lognb, double *part, int cap, int slot0, int gx) {
+  __shared__ double sm[MB / 64][NV][64];
+  const int nb = 1 << lognb;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = v[k];
+    for (int o = 32; o >= nb; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane < nb) sm[w][k][lane] = s;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NV * nb; t += MB) {
+    const int k = t >> lognb, r = t & (nb - 1);
+    // the last block may hold fewer than nb slots
+    if (slot0 + r >= cap) continue;
+    double s = 0;
+#pragma unroll
+    for (int q = 0; q < MB / 64; q++) s += sm[q][k][r];
+    part[((long)k * cap + slot0 + r) * gx + blockIdx.x] = s;
+  }
+}
+
+// partials: part[((k * cap + slot) * gx + bx] for workgroup bx of the slot's block
+template <typename TV, typename TW, int K, bool SHARED>
+__global__ void __launch_bounds__(MB) k_mdot(MShape a, MPtr v, const void *w, double *part, const int *any) {
+  if (any && !*any) return;
+  const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
+  if ((blk << a.lognb) >= nact) return;
+  const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
+  double acc[2 * K];
+#pragma unroll
+  for (int k = 0; k < 2 * K; k++) acc[k] = 0;
+  if (slot < nact) {
+    const int per = MB >> a.lognb;
+    const long base = (long)blk * a.blk_elems;
+    for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
+      const long e = base + (p << a.lognb) + rr;
+      const double2 b = E2<TW>::ld(w, e);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const double2 x = E2<TV>::ld(v.p[k], SHARED ? p : e);
+        acc[2 * k] += x.x * b.x + x.y * b.y;
+        acc[2 * k + 1] += x.x * b.y - x.y * b.x;
+      }
+    }
+  }
+  slot_sum<2 * K>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
+}
+
+// One wavefront per (component c, slot): the sum over bx of the partials
+// part[((c * cap + slot) * gx + bx], lane-strided then folded by shuffles
+// (a fixed order).  Complex results (nv = 2 nq components) go to
+// res[2 ((q0 + c/2) * cap + slot) + c%2], real ones to res[(q0 + c) * cap + slot].
+__global__ void __launch_bounds__(64) k_mfinal(int nv, int cap, int gx, const int *nact, const double *part,
+                                               double *res, int q0, int is_complex) {
+  const int slot = blockIdx.x, c = blockIdx.y;
+  if (slot >= *nact) return;
+  const double *p = part + ((long)c * cap + slot) * gx;
+  double s = 0;
+  for (int b = threadIdx.x; b < gx; b += 64) s += p[b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (threadIdx.x == 0) {
+    if (is_complex) res[2 * ((long)(q0 + c / 2) * cap + slot) + (c & 1)] = s;
+    else res[(long)(q0 + c) * cap + slot] = s;
+  }
+}
+
+// W += sign * sum_k c_k V_k  per slot (optionally masked by flag), optional norm partials
+template <typename TV, typename TW, int K, bool SHARED, bool NORM>
+__global__ void __launch_bounds__(MB) k_maxpy(MShape a, MPtr v, const double *coef, int q0, double sign, void *w,
+                                              double *part, const int *flag, const int *any) {
+  if (any && !*any) return;
+  const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
+  if ((blk << a.lognb) >= nact) return;
+  const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
+  double acc[1] = {0};
+  bool act = slot < nact;
+  if (act && flag) act = flag[slot] != 0;
+  if (act) {
+    double cr[K], ci[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      cr[k] = sign * coef[2 * ((long)(q0 + k) * a.cap + slot)];
+      ci[k] = sign * coef[2 * ((long)(q0 + k) * a.cap + slot) + 1];
+    }
+    const int per = MB >> a.lognb;
+    const long base = (long)blk * a.blk_elems;
+    for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
+      const long e = base + (p << a.lognb) + rr;
+      double2 b = E2<TW>::ld(w, e);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const double2 x = E2<TV>::ld(v.p[k], SHARED ? p : e);
+        b.x += cr[k] * x.x - ci[k] * x.y;
+        b.y += cr[k] * x.y + ci[k] * x.x;
+      }
+      // the norm of what was stored (FP32 fields: of the rounded value)
+      const double2 s = E2<TW>::st(w, e, b.x, b.y);
+      if (NORM) acc[0] += s.x * s.x + s.y * s.y;
+    }
+  }
+  if (NORM) slot_sum<1>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
+}
+
+template <typename TX, typename TY, bool NORM>
+__global__ void __launch_bounds__(MB) k_maxpby(MShape a, double ca, const void *x, double cb, void *y, double *part) {
+  const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
+  if ((blk << a.lognb) >= nact) return;
+  const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
+  double acc[1] = {0};
+  if (slot < nact) {
+    const int per = MB >> a.lognb;
+    const long base = (long)blk * a.blk_elems;
+    for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
+      const long e = base + (p << a.lognb) + rr;
+      const double2 u = E2<TX>::ld(x, e);
+      double2 b = make_double2(0, 0);
+      if (cb != 0.0) b = E2<TY>::ld(y, e);
+      const double2 s = E2<TY>::st(y, e, ca * u.x + cb * b.x, ca * u.y + cb * b.y);
+      if (NORM) acc[0] += s.x * s.x + s.y * s.y;
+    }
+  }
+  if (NORM) slot_sum<1>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(MB) k_mnorm(MShape a, const void *w, double *part) {
+  const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
+  if ((blk << a.lognb) >= nact) return;
+  const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
+  double acc[1] = {0};
+  if (slot < nact) {
+    const int per = MB >> a.lognb;
+    const long base = (long)blk * a.blk_elems;
+    for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
+      const double2 s = E2<T>::ld(w, base + (p << a.lognb) + rr);
+      acc[0] += s.x * s.x + s.y * s.y;
+    }
+  }
+  slot_sum<1>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
+}
+
+template <typename TX, typename TW>
+__global__ void __launch_bounds__(MB) k_mscale(MShape a, const void *x, void *w, const double *n2) {
+  const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
+  if ((blk << a.lognb) >= nact) return;
+  const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
+  if (slot >= nact) return;
+  const double n = n2[slot];
+  const double s = n > 0.0 ? 1.0 / sqrt(n) : 0.0;
+  const int per = MB >> a.lognb;
+  const long base = (long)blk * a.blk_elems;
+  for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
+    const long e = base + (p << a.lognb) + rr;
+    const double2 u = E2<TX>::ld(x, e);
+    E2<TW>::st(w, e, s * u.x, s * u.y);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(MB) k_mswap(long rows, int lognb, long blk_elems, void *w, int sa, int sb) {
+  const long p = blockIdx.x * (long)MB + threadIdx.x;
+  if (p >= rows) return;
+  const int nb = 1 << lognb;
+  const long ea = (long)(sa >> lognb) * blk_elems + (p << lognb) + (sa & (nb - 1));
+  const long eb = (long)(sb >> lognb) * blk_elems + (p << lognb) + (sb & (nb - 1));
+  const double2 u = E2<T>::ld(w, ea), v = E2<T>::ld(w, eb);
+  E2<T>::st(w, ea, v.x, v.y);
+  E2<T>::st(w, eb, u.x, u.y);
+}
+
+template <typename TS, typename TW>
+__global__ void __launch_bounds__(MB) k_mput(long rows, int lognb, long blk_elems, const void *s, void *w, int slot) {
+  const long p = blockIdx.x * (long)MB + threadIdx.x;
+  if (p >= rows) return;
+  const int nb = 1 << lognb;
+  const long e = (long)(slot >> lognb) * blk_elems + (p << lognb) + (slot & (nb - 1));
+  const double2 u = E2<TS>::ld(s, p);
+  E2<TW>::st(w, e, u.x, u.y);
+}
+
+template <typename TW, typename TS>
+__global__ void __launch_bounds__(MB) k_mget(long rows, int lognb, long blk_elems, const void *w, void *s, int slot) {
+  const long p = blockIdx.x * (long)MB + threadIdx.x;
+  if (p >= rows) return;
+  const int nb = 1 << lognb;
+  const long e = (long)(slot >> lognb) * blk_elems + (p << lognb) + (slot & (nb - 1));
+  const double2 u = E2<TW>::ld(w, e);
+  E2<TS>::st(s, p, u.x, u.y);
+}
+
+__global__ void k_arnoldi_flags(const double *n2, const double *w2, int *flag, int *any, const int *nact, int cap,
+                                int force) {
+  const int r = threadIdx.x;
+  int f = 0;
+  if (r < cap) {
+    if (nact && r < *nact) f = force ? 1 : n2[r] < 0.5 * w2[2 * r];
+    flag[r] = f;
+  }
+  const int a = __syncthreads_or(f);
+  if (r == 0) *any = a;
+}
+
+// column j of the batched Hessenberg matrix, and the final squared norm in nf
+__global__ void k_arnoldi_column(double *H, int ldh, int j, const double *h1, const double *h2, const int *flag,
+                                 const double *n2a, const double *n2b, double *nf, const int *nact, int cap) {
+  const int t = blockIdx.x * MB + threadIdx.x;
+  const int q = t / cap, r = t - q * cap;
+  if (q > j + 1 || r >= *nact) return;
+  const int f = flag[r];
+  double *h = H + 2 * (((long)j * ldh + q) * cap + r);
+  if (q <= j) {
+    double re = h1[2 * ((long)q * cap + r)], im = h1[2 * ((long)q * cap + r) + 1];
+    if (f) {
+      re += h2[2 * ((long)q * cap + r)];
+      im += h2[2 * ((long)q * cap + r) + 1];
+    }
+    h[0] = re;
+    h[1] = im;
+  } else {
+    const double n = f ? n2b[r] : n2a[r];
+    nf[r] = n;
+    h[0] = n > 0.0 ? sqrt(n) : 0.0;
+    h[1] = 0.0;
+  }
+}
+
+// ===================================================================== //
+
+static size_t esize(Prec p) { return p == PREC_DOUBLE ? 16 : p == PREC_SINGLE ? 8 : 4; }
+
+MultiSpinor alloc_multi(const Geom &g, Prec p, int cap, int nb) {
+  if (nb < 1 || nb > 64 || (nb & (nb - 1))) throw std::runtime_error("alloc_multi: RHS block must be 1..64, a power of two");
+  if (cap < 1 || cap > 256) throw std::runtime_error("alloc_multi: 1..256 right-hand sides");
+  MultiSpinor m;
+  m.prec = p;
+  m.cap = cap;
+  m.nb = nb;
+  m.lognb = 0;
+  while ((1 << m.lognb) < nb) m.lognb++;
+  m.stride = g.stride;
+  QQCD_HIP(hipMalloc(&m.data, (size_t)m.elems() * esize(p)));
+  QQCD_HIP(hipMemset(m.data, 0, (size_t)m.elems() * esize(p)));
+  if (p == PREC_HALF) {
+    const size_t nn = (size_t)m.stride * nb * m.nblk();
+    QQCD_HIP(hipMalloc(&m.norm, nn * sizeof(float)));
+    QQCD_HIP(hipMemset(m.norm, 0, nn * sizeof(float)));
+  }
+  return m;
+}
+
+void free_multi(MultiSpinor &m) {
+  if (m.data) hipFree(m.data);
+  if (m.norm) hipFree(m.norm);
+  m.data = nullptr;
+  m.norm = nullptr;
+}
+
+MBlas::MBlas(hipStream_t s, int cap_) : stream(s), cap(cap_) {}
+MBlas::~MBlas() {
+  if (part) hipFree(part);
+}
+
+void MBlas::reserve(size_t n) {
+  if (n <= part_cap) return;
+  // only outside graph capture: the first call of each shape sizes the buffer
+  if (part) {
+    QQCD_HIP(hipStreamSynchronize(stream));
+    hipFree(part);
+  }
+  QQCD_HIP(hipMalloc(&part, n * sizeof(double)));
+  part_cap = n;
+}
+
+static int gx_of(long rows, int nb, int nblk) {
+  // about 2048 workgroups in all, at most one row group per lane
+  const long groups = (rows * nb + MB - 1) / MB;
+  return (int)std::max(1L, std::min(groups, (long)std::max(16, 2048 / nblk)));
+}
+
+// gx <= 2048 and at most 16 partial values per slot
+size_t MBlas::need() const { return (size_t)2048 * 16 * cap + 64; }
+
+static MShape shape(const MultiSpinor &W, const int *nact) {
+  MShape a;
+  a.rows = 12 * W.stride;
+  a.blk_elems = W.blk_elems();
+  a.nact = nact;
+  a.lognb = W.lognb;
+  a.cap = W.cap;
+  a.gx = gx_of(a.rows, W.nb, W.nblk());
+  return a;
+}
+
+static void need_fp(const MultiSpinor &W, const char *who) {
+  if (W.prec == PREC_HALF) throw std::runtime_error(std::string(who) + ": FP16 multi-RHS fields are hopping-only");
+}
+
+template <typename F> static void two_types(Prec a, Prec b, F &&f) {
+  if (a == PREC_DOUBLE && b == PREC_DOUBLE) f((double *)0, (double *)0);
+  else if (a == PREC_DOUBLE && b == PREC_SINGLE) f((double *)0, (float *)0);
+  else if (a == PREC_SINGLE && b == PREC_DOUBLE) f((float *)0, (double *)0);
+  else if (a == PREC_SINGLE && b == PREC_SINGLE) f((float *)0, (float *)0);
+  else throw std::runtime_error("multi BLAS: FP64 / FP32 fields only");
+}
+
+#define T_OF(p) typename std::remove_pointer<decltype(p)>::type
+
+template <bool SHARED>
+static void dot_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, const MultiSpinor &W, const int *nact,
+                     double *res, double *&part, const int *any, std::function<void(size_t)> reserve) {
+  const MShape a = shape(W, nact);
+  const int nblk = W.nblk();
+  reserve((size_t)a.gx * 16 * W.cap);
+  for (int q0 = 0; q0 < nq; q0 += 8) {
+    const int K = std::min(8, nq - q0);
+    MPtr pk{};
+    for (int k = 0; k < K; k++) pk.p[k] = vp[q0 + k];
+    two_types(vprec, W.prec, [&](auto *tv, auto *tw) {
+      typedef T_OF(tv) TV;
+      typedef T_OF(tw) TW;
+      dim3 grid(a.gx, nblk);
+#define L_(KK) hipLaunchKernelGGL((k_mdot<TV, TW, KK, SHARED>), grid, dim3(MB), 0, mb.stream, a, pk, W.data, part, any)
+      switch (K) {
+        case 1: L_(1); break;
+        case 2: L_(2); break;
+        case 3: L_(3); break;
+        case 4: L_(4); break;
+        case 5: L_(5); break;
+        case 6: L_(6); break;
+        case 7: L_(7); break;
+        default: L_(8); break;
+      }
+#undef L_
+    });
+    QQCD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mfinal, dim3(W.cap, 2 * K), dim3(64), 0, mb.stream, 2 * K, W.cap, a.gx, nact,
+                       (const double *)part, res, q0, 1);
+    QQCD_HIP(hipGetLastError());
+  }
+}
+
+void MBlas::dot(int nq, const MultiSpinor *V, const MultiSpinor &W, const int *nact, double *res, const int *any) {
+  need_fp(W, "MBlas::dot");
+  std::vector<const void *> vp(nq);
+  for (int q = 0; q < nq; q++) {
+    if (V[q].prec != V[0].prec || V[q].nb != W.nb || V[q].cap != W.cap)
+      throw std::runtime_error("MBlas::dot: layouts differ");
+    vp[q] = V[q].data;
+  }
+  if (nq == 0) return;
+  dot_impl<false>(*this, nq, vp.data(), V[0].prec, W, nact, res, part, any, [this](size_t n) { reserve(n); });
+}
+
+void MBlas::dot_shared(int nq, const Spinor *D, const MultiSpinor &W, const int *nact, double *res) {
+  need_fp(W, "MBlas::dot_shared");
+  std::vector<const void *> vp(nq);
+  for (int q = 0; q < nq; q++) {
+    if (D[q].prec != D[0].prec || D[q].stride != W.stride) throw std::runtime_error("MBlas::dot_shared: layouts differ");
+    vp[q] = D[q].data;
+  }
+  if (nq == 0) return;
+  dot_impl<true>(*this, nq, vp.data(), D[0].prec, W, nact, res, part, nullptr, [this](size_t n) { reserve(n); });
+}
+
+template <bool SHARED>
+static void axpy_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, const double *coef, int q0g,
+                      double sign, MultiSpinor &W, const int *nact, double *n2, const int *flag, const int *any,
+                      double *part) {
+  const MShape a = shape(W, nact);
+  const int nblk = W.nblk();
+  for (int q0 = 0; q0 < nq; q0 += 8) {
+    const int K = std::min(8, nq - q0);
+    const bool last = q0 + K >= nq;
+    MPtr pk{};
+    for (int k = 0; k < K; k++) pk.p[k] = vp[q0 + k];
+    two_types(vprec, W.prec, [&](auto *tv, auto *tw) {
+      typedef T_OF(tv) TV;
+      typedef T_OF(tw) TW;
+      dim3 grid(a.gx, nblk);
+#define L_(KK, NR) \
+  hipLaunchKernelGGL((k_maxpy<TV, TW, KK, SHARED, NR>), grid, dim3(MB), 0, mb.stream, a, pk, coef, q0g + q0, sign, W.data, part, flag, any)
+#define LK_(NR)            \
+  switch (K) {             \
+    case 1: L_(1, NR); break; \
+    case 2: L_(2, NR); break; \
+    case 3: L_(3, NR); break; \
+    case 4: L_(4, NR); break; \
+    case 5: L_(5, NR); break; \
+    case 6: L_(6, NR); break; \
+    case 7: L_(7, NR); break; \
+    default: L_(8, NR); break; \
+  }
+      if (n2 && last) {
+        LK_(true)
+      } else {
+        LK_(false)
+      }
+#undef LK_
+#undef L_
+    });
+    QQCD_HIP(hipGetLastError());
+  }
+  if (n2) {
+    hipLaunchKernelGGL(k_mfinal, dim3(W.cap, 1), dim3(64), 0, mb.stream, 1, W.cap, a.gx, nact,
+                       (const double *)part, n2, 0, 0);
+    QQCD_HIP(hipGetLastError());
+  }
+}
+
+void MBlas::axpy(int nq, const MultiSpinor *V, const double *coef, int q0, double sign, MultiSpinor &W,
+                 const int *nact, double *n2, const int *flag, const int *any) {
+  need_fp(W, "MBlas::axpy");
+  std::vector<const void *> vp(nq);
+  for (int q = 0; q < nq; q++) {
+    if (V[q].prec != V[0].prec || V[q].nb != W.nb || V[q].cap != W.cap)
+      throw std::runtime_error("MBlas::axpy: layouts differ");
+    vp[q] = V[q].data;
+  }
+  if (nq == 0) {
+    if (n2) norm2(W, nact, n2);
+    return;
+  }
+  reserve((size_t)shape(W, nact).gx * 16 * W.cap);
+  axpy_impl<false>(*this, nq, vp.data(), V[0].prec, coef, q0, sign, W, nact, n2, flag, any, part);
+}
+
+void MBlas::axpy_shared(int nq, const Spinor *D, const double *coef, int q0, double sign, MultiSpinor &W,
+                        const int *nact, double *n2) {
+  need_fp(W, "MBlas::axpy_shared");
+  std::vector<const void *> vp(nq);
+  for (int q = 0; q < nq; q++) {
+    if (D[q].prec != D[0].prec || D[q].stride != W.stride)
+      throw std::runtime_error("MBlas::axpy_shared: layouts differ");
+    vp[q] = D[q].data;
+  }
+  if (nq == 0) {
+    if (n2) norm2(W, nact, n2);
+    return;
+  }
+  reserve((size_t)shape(W, nact).gx * 16 * W.cap);
+  axpy_impl<true>(*this, nq, vp.data(), D[0].prec, coef, q0, sign, W, nact, n2, nullptr, nullptr, part);
+}
+
+void MBlas::norm2(const MultiSpinor &W, const int *nact, double *n2) {
+  need_fp(W, "MBlas::norm2");
+  const MShape a = shape(W, nact);
+  reserve((size_t)a.gx * 16 * W.cap);
+  if (W.prec == PREC_DOUBLE)
+    hipLaunchKernelGGL(k_mnorm<double>, dim3(a.gx, W.nblk()), dim3(MB), 0, stream, a, W.data, part);
+  else
+    hipLaunchKernelGGL(k_mnorm<float>, dim3(a.gx, W.nblk()), dim3(MB), 0, stream, a, W.data, part);
+  QQCD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mfinal, dim3(W.cap, 1), dim3(64), 0, stream, 1, W.cap, a.gx, nact, (const double *)part,
+                     n2, 0, 0);
+  QQCD_HIP(hipGetLastError());
+}
+
+void MBlas::axpby(double ca, const MultiSpinor &X, double cb, MultiSpinor &Y, const int *nact, double *n2) {
+  need_fp(Y, "MBlas::axpby");
+  if (X.nb != Y.nb || X.cap != Y.cap || X.stride != Y.stride) throw std::runtime_error("MBlas::axpby: layouts differ");
+  const MShape a = shape(Y, nact);
+  reserve((size_t)a.gx * 16 * Y.cap);
+  two_types(X.prec, Y.prec, [&](auto *tx, auto *ty) {
+    typedef T_OF(tx) TX;
+    typedef T_OF(ty) TY;
+    dim3 grid(a.gx, Y.nblk());
+    if (n2)
+      hipLaunchKernelGGL((k_maxpby<TX, TY, true>), grid, dim3(MB), 0, stream, a, ca, X.data, cb, Y.data, part);
+    else
+      hipLaunchKernelGGL((k_maxpby<TX, TY, false>), grid, dim3(MB), 0, stream, a, ca, X.data, cb, Y.data, part);
+  });
+  QQCD_HIP(hipGetLastError());
+  if (n2) {
+    hipLaunchKernelGGL(k_mfinal, dim3(Y.cap, 1), dim3(64), 0, stream, 1, Y.cap, a.gx, nact, (const double *)part,
+                       n2, 0, 0);
+    QQCD_HIP(hipGetLastError());
+  }
+}
+
+void MBlas::scale_inv_norm(const MultiSpinor &X, MultiSpinor &W, const double *n2, const int *nact) {
+  need_fp(W, "MBlas::scale_inv_norm");
+  const MShape a = shape(W, nact);
+  two_types(X.prec, W.prec, [&](auto *tx, auto *tw) {
+    typedef T_OF(tx) TX;
+    typedef T_OF(tw) TW;
+    hipLaunchKernelGGL((k_mscale<TX, TW>), dim3(a.gx, W.nblk()), dim3(MB), 0, stream, a, X.data, W.data, n2);
+  });
+  QQCD_HIP(hipGetLastError());
+}
+
+void MBlas::swap(MultiSpinor &W, int sa, int sb) {
+  need_fp(W, "MBlas::swap");
+  if (sa == sb) return;
+  if (sa < 0 || sb < 0 || sa >= W.cap || sb >= W.cap) throw std::runtime_error("MBlas::swap: slot out of range");
+  const long rows = 12 * W.stride;
+  const dim3 grid((unsigned)((rows + MB - 1) / MB));
+  if (W.prec == PREC_DOUBLE)
+    hipLaunchKernelGGL(k_mswap<double>, grid, dim3(MB), 0, stream, rows, W.lognb, W.blk_elems(), W.data, sa, sb);
+  else
+    hipLaunchKernelGGL(k_mswap<float>, grid, dim3(MB), 0, stream, rows, W.lognb, W.blk_elems(), W.data, sa, sb);
+  QQCD_HIP(hipGetLastError());
+}
+
+void MBlas::put(const Spinor &s, MultiSpinor &W, int slot) {
+  need_fp(W, "MBlas::put");
+  if (slot < 0 || slot >= W.cap || s.stride != W.stride) throw std::runtime_error("MBlas::put: bad slot or layout");
+  const long rows = 12 * W.stride;
+  two_types(s.prec, W.prec, [&](auto *ts, auto *tw) {
+    typedef T_OF(ts) TS;
+    typedef T_OF(tw) TW;
+    hipLaunchKernelGGL((k_mput<TS, TW>), dim3((unsigned)((rows + MB - 1) / MB)), dim3(MB), 0, stream, rows, W.lognb,
+                       W.blk_elems(), s.data, W.data, slot);
+  });
+  QQCD_HIP(hipGetLastError());
+}
+
+void MBlas::get(const MultiSpinor &W, int slot, Spinor &s) {
+  need_fp(W, "MBlas::get");
+  if (slot < 0 || slot >= W.cap || s.stride != W.stride) throw std::runtime_error("MBlas::get: bad slot or layout");
+  const long rows = 12 * W.stride;
+  two_types(W.prec, s.prec, [&](auto *tw, auto *ts) {
+    typedef T_OF(tw) TW;
+    typedef T_OF(ts) TS;
+    hipLaunchKernelGGL((k_mget<TW, TS>), dim3((unsigned)((rows + MB - 1) / MB)), dim3(MB), 0, stream, rows, W.lognb,
+                       W.blk_elems(), W.data, s.data, slot);
+  });
+  QQCD_HIP(hipGetLastError());
+}
+
+void MBlas::zero(MultiSpinor &W) {
+  QQCD_HIP(hipMemsetAsync(W.data, 0, (size_t)W.elems() * esize(W.prec), stream));
+}
+
+void arnoldi_flags(const double *n2, const double *w2, int *flag, int *any, const int *nact, int cap, int force,
+                   hipStream_t s) {
+  if (cap > MB) throw std::runtime_error("arnoldi_flags: at most 256 slots");
+  hipLaunchKernelGGL(k_arnoldi_flags, dim3(1), dim3(MB), 0, s, n2, w2, flag, any, nact, cap, force);
+  QQCD_HIP(hipGetLastError());
+}
+
+void clear_flags(int *flag, int *any, int cap, hipStream_t s) {
+  if (cap > MB) throw std::runtime_error("clear_flags: at most 256 slots");
+  hipLaunchKernelGGL(k_arnoldi_flags, dim3(1), dim3(MB), 0, s, nullptr, nullptr, flag, any, nullptr, cap, 0);
+  QQCD_HIP(hipGetLastError());
+}
+
+void arnoldi_column(double *H, int ldh, int j, const double *h1, const double *h2, const int *flag,
+                    const double *n2a, const double *n2b, double *nf, const int *nact, int cap, hipStream_t s) {
+  const int nt = (j + 2) * cap;
+  hipLaunchKernelGGL(k_arnoldi_column, dim3((nt + MB - 1) / MB), dim3(MB), 0, s, H, ldh, j, h1, h2, flag, n2a, n2b,
+                     nf, nact, cap);
+  QQCD_HIP(hipGetLastError());
+}
+
+}  // namespace qqcd

@@ -42,6 +42,17 @@ static GammaRow gamma_rows() {
   return g;
 }
 
+void gamma_dense(int mu, double re[4][4], double im[4][4]) {
+  if (mu < 1 || mu > 5) throw std::runtime_error("gamma_dense: mu must be 1..5");
+  static const GammaRow gr = gamma_rows();
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++) re[r][c] = im[r][c] = 0.0;
+  for (int r = 0; r < 4; r++) {
+    re[r][gr.col[mu - 1][r]] = gr.re[mu - 1][r];
+    im[r][gr.col[mu - 1][r]] = gr.im[mu - 1][r];
+  }
+}
+
 struct SliceList {
   int n;
   int sl[16];  // slice index ieo + 2*ibleo (0-based)

@@ -114,6 +114,10 @@ struct MixedStats {
 MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spinor &x, const MixedParams &p,
                        Deflation *defl);
 
+// MixedParams from the C-ABI arguments (GMRES = (m, k); basis 0 FP64 / 1 FP32;
+// op_prec 0/1/2; inner_tol <= 0 picks 1e-2 (FP16), 1e-6 (FP32), 1e-10 (FP64))
+MixedParams mixed_params(const int GMRES[2], double tol, int basis, int op_prec, int op_recon, double inner_tol);
+
 // even-odd source preparation / reconstruction / full-system residual of
 // fermprop (qqcd-test/cfgsprops/quark/quark.f90:600-617, 797-819), FP64
 void prep_source(Context &c, const double kappa[2], const Spinor &b_e, const Spinor &b_o, Spinor &src);

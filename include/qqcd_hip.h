@@ -170,6 +170,72 @@ int qqcd_solve_mixed(const double *phi, double *x, const int GMRES[2], double to
                      int op_recon, double inner_tol, int keep_deflation, const double *u, const double *coact,
                      const double kappa[2], const int *bc, int *itercount);
 
+/* ---- many right-hand sides on one gauge field (disconnected loops) ------ */
+
+/* Z4 noise source {1, -1, i, -i} * scale on every (site, spin, colour) of be
+ * (gblclr=1) and bo (gblclr=2), either may be NULL.  Replaces z4source
+ * (qqcd-test/cfgsprops/quark/disconloops.f90:4359, active branch
+ * :4518-4569), which maps one uniform number per component to the four
+ * values with the same thresholds; the uniform numbers here come from a
+ * counter-based generator keyed by (seed, rhs, global site, component)
+ * instead of the reference's lagged-Fibonacci stream rnds. */
+int qqcd_z4source(double *be, double *bo, unsigned long long seed, long rhs, double scale);
+
+/* Solve nrhs systems M x_r = b_r on one gauge field, as the reference's noise
+ * loops call fermprop(inverter=6) once per source (testFUNC / discon,
+ * qqcd-test/cfgsprops/quark/disconloops.f90:2460-2511, 3566; psc
+ * qqcd-test/cfgsprops/quark/inverters.f90:4310-4465): source 1 by the mixed
+ * GMRES-DR(GMRES[0], GMRES[1]) keeping its deflation space, sources 2..nrhs
+ * together by the batched GMRES-Proj(GMRES[0]-GMRES[1]) over it (FP32
+ * right-hand-side-innermost fields, FP64 reliable updates), each to
+ * ||b - M x|| < resmax ||b|| on its even-odd system.  be/bo/xe/xo: arrays of
+ * nrhs pointers to (6,ntotal,4,2,8) arrays; itercount[r]: Krylov steps
+ * spent on source r; rel_res[r] (may be NULL): whole-lattice
+ * ||b - M x|| / ||b||. */
+int qqcd_fermprop_multi(int nrhs, const double *const *be, const double *const *bo, const double kappa[2],
+                        const double *coact, const int *bc, double resmax, const int GMRES[2], const double *u,
+                        double *const *xe, double *const *xo, int *itercount, double *rel_res);
+
+/* The noise loop on the device: nrhs Z4 sources (qqcd_z4source with seed,
+ * rhs = 0..nrhs-1, scale), solved as in qqcd_fermprop_multi, and the local
+ * trace estimators of the disconnected loops (loopops / spacesum,
+ * qqcd-test/cfgsprops/quark/disconloops.f90:7769-7910, 9012-9124) for the
+ * 16 Dirac matrices Gamma_n (n = 0: 1; 1..4: gamma_mu; 5: gamma_5; 6..9:
+ * gamma_mu gamma_5; 10..15: sigma_munu = (i/2)[gamma_mu, gamma_nu] for
+ * (1,2),(1,3),(1,4),(2,3),(2,4),(3,4)):
+ *   traces[(t*16 + n)*2 + {re, im}] = (1/(nrhs scale^2)) sum_r sum_{x, x_4 = t} eta_r(x)^dag Gamma_n x_r(x),
+ * an estimate of Tr(Gamma_n M^-1) on timeslice t.  u may be NULL to use the
+ * resident gauge field.  nb = 0 lets the autotuner pick the RHS block.
+ * res_full[r], iters[r]: per source; times = {first solve, batched solves,
+ * total} seconds; stats[8] = {outer steps, GMRES cycles, batched operator
+ * applications, graph replays, sources finished by the fallback, RHS block,
+ * tile order, outer steps of the first solve}.  method 0: the batched path
+ * above; 1: one source after the other in plain FP64 (GMRES-DR, then
+ * GMRES-Proj per later source: the reference's psc sequence on the GPU);
+ * 2: one source after the other with the single-RHS mixed solver.  No
+ * reference counterpart as one call (the reference loops over fermprop). */
+int qqcd_noise_loop(int nrhs, unsigned long long seed, double scale, const double kappa[2], const double *coact,
+                    const int *bc, const double *u, const int GMRES[2], double tol, int nb, int method,
+                    double *traces, double *res_full, int *iters, double *times, int *stats);
+
+/* The 16 local trace estimators of one source: eta and x on both
+ * checkerboards (reference layout); traces as in qqcd_noise_loop for
+ * nrhs = 1, scale = 1. */
+int qqcd_trace16(const double *eta_e, const double *eta_o, const double *x_e, const double *x_o, double *traces);
+
+/* Knobs of the batched solver: reorth 0 (one classical Gram-Schmidt pass),
+ * 1 (a second pass for the slots that need it), 2 (always two); use_graph
+ * replays the Arnoldi cycle from a captured HIP graph; inner_tol relative
+ * tolerance of each FP32 inner solve; inner_cycles GMRES-Proj cycles per
+ * inner solve; max_outer FP64 reliable updates before the single-RHS
+ * fallback; op_prec 1 (FP32 links) / op_recon 12 or 18 for the inner
+ * operator. */
+int qqcd_batch_set(int reorth, int use_graph, double inner_tol, int inner_cycles, int max_outer, int op_prec,
+                   int op_recon);
+/* Statistics of the last batched solve. */
+int qqcd_batch_stats(int *outer, int *cycles, long *slot_matvecs, long *batched_matvecs, int *graph_replays,
+                     int *fallback);
+
 /* Statistics of the last solver call: final ||r|| / ||r0|| (true residual),
  * Krylov steps, all operator applications, cycles, converged flag. */
 int qqcd_last_solve_stats(double *rel_res, int *iters, int *matvecs, int *cycles, int *converged);
@@ -204,6 +270,22 @@ int qqcd_schur_canonical(const double *psi, double *out, const double kappa[2], 
 int qqcd_bench_solve(int method, const int src[4], int spin, int colour, const double kappa[2], const int GMRES[2],
                      double tol, int op_prec, int op_recon, double inner_tol, int keep_deflation, double *seconds,
                      int *iters, int *outer, double *true_res);
+
+/* Multi-right-hand-side hop / Schur operator on nrhs canonical arrays (each
+ * as in qqcd_hop_canonical, back to back) through the RHS-innermost fields
+ * (prec 0 FP64 or 1 FP32; RHS block nb, 0 = the power of two >= nrhs, at
+ * most 64; order 0/1 tile order).  Only the first nact right-hand sides are
+ * active: the others are left as they are in out. */
+int qqcd_hop_multi_canonical(int nrhs, int nact, int nb, int prec, int recon, int idag, int par_out, int order,
+                             const double *in, double *out);
+int qqcd_schur_multi_canonical(int nrhs, int nact, int nb, int prec, int recon, const double kappa[2], int idag,
+                               int order, const double *in, double *out);
+/* Time the multi-RHS hop (average ms per launch) with nact of nrhs slots active. */
+int qqcd_bench_dslash_multi(int prec, int recon, int nrhs, int nact, int nb, int order, int nwarm, int niter,
+                            double *ms);
+/* The autotuner's choice of RHS block and tile order for nrhs right-hand
+ * sides on the resident gauge field, and its time per launch. */
+int qqcd_autotune_multi(int prec, int recon, int nrhs, int *nb, int *order, double *ms);
 
 /* Time a plain device-to-device copy kernel (read + write `bytes` each
  * launch): the calibration point for hardware byte counters. */

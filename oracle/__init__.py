@@ -224,3 +224,43 @@ def parity_mask(n):
     z = (s // (n[0] * n[1])) % n[2]
     t = s // (n[0] * n[1] * n[2])
     return ((x + y + z + t) & 1).astype(bool)
+
+
+def z4(n, seed, rhs=0, scale=1.0):
+    """Z4 noise of right-hand side ``rhs`` on the whole lattice, canonical
+    (V, 4, 3) complex (the library's counter-based z4source)."""
+    L = lib()
+    V = volume(n)
+    out = np.zeros((V, 4, 3), dtype=np.complex128)
+    L.orc_z4(_iptr(dims(n)), ctypes.c_uint64(int(seed)), ctypes.c_long(int(rhs)), ctypes.c_double(scale),
+             _ptr(out.view(np.float64)))
+    return out
+
+
+GAMMA16_NAMES = ("1", "g1", "g2", "g3", "g4", "g5", "g1g5", "g2g5", "g3g5", "g4g5",
+                 "s12", "s13", "s14", "s23", "s24", "s34")
+
+
+def gamma16():
+    """The 16 local Dirac matrices of the trace estimators as dense (16, 4, 4)
+    complex matrices, built by matrix products of the reference's gamma_mu:
+    1; gamma_1..4; gamma_5; gamma_mu gamma_5; sigma_munu = (i/2)[g_mu, g_nu]."""
+    g = gamma_table()
+    out = [np.eye(4, dtype=np.complex128)] + [g[m] for m in range(5)]
+    out += [g[m] @ g[4] for m in range(4)]
+    for a, b in ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)):
+        out.append(0.5j * (g[a] @ g[b] - g[b] @ g[a]))
+    return np.array(out)
+
+
+def trace16(n, eta, x):
+    """(nt, 16) complex: sum over each timeslice of eta(x)^dag Gamma_n x(x) for
+    canonical (V, 4, 3) fields (dense 4x4 products per site)."""
+    G = gamma16()
+    V = volume(n)
+    t = np.arange(V) // (n[0] * n[1] * n[2])
+    # per site and Gamma: sum_{a,b,c} conj(eta_ac) G_ab x_bc
+    per = np.einsum("sac,nab,sbc->sn", np.conj(eta), G, x)
+    out = np.zeros((n[3], 16), dtype=np.complex128)
+    np.add.at(out, t, per)
+    return out

@@ -40,6 +40,7 @@
  * gauge covariance, dense brute force on 4^4); see tests/.
  */
 #include <complex.h>
+#include <stdint.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -495,4 +496,38 @@ void orc_hdbletm_ref(const int n[4], const double *u, const double *coact, const
     orc_schur(n, U, bc, c1, c2, idag, kappa, a, b, t1, t2);
     orc_spinor_canon_to_ref(n, b, he, NULL);
     free(U); free(a); free(b); free(t1); free(t2);
+}
+
+/* ------------------------------------------------------------------------
+ * Z4 noise (the library's counter-based restatement of z4source,
+ * qqcd-test/cfgsprops/quark/disconloops.f90:4518-4569): component k =
+ * 3*spin + colour of site s of right-hand side rhs takes the uniform number
+ *   u = (mix(seed + G*(1 + (rhs*V + s)*12 + k)) >> 11) * 2^-53,
+ * mix = the splitmix64 finaliser, G = 0x9E3779B97F4A7C15, and the value
+ * 1 (u > 0.75), -1 (0.5 < u < 0.75), i (0.25 < u < 0.5), -i (otherwise),
+ * times scale.  Canonical output psi[(s*4 + spin)*3 + colour].
+ * ---------------------------------------------------------------------- */
+static uint64_t orc_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void orc_z4(const int n[4], uint64_t seed, long rhs, double scale, double *psi_ri)
+{
+    const long V = vol(n);
+    for (long s = 0; s < V; s++)
+        for (int k = 0; k < 12; k++) {
+            const uint64_t c = ((uint64_t)rhs * (uint64_t)V + (uint64_t)s) * 12ull + (uint64_t)k;
+            const double u = (double)(orc_mix(seed + 0x9E3779B97F4A7C15ull * (1ull + c)) >> 11) *
+                             (1.0 / 9007199254740992.0);
+            double re = 0, im = 0;
+            if (u > 0.75) re = 1;
+            else if (0.5 < u && u < 0.75) re = -1;
+            else if (0.25 < u && u < 0.5) im = 1;
+            else im = -1;
+            psi_ri[2 * (s * 12 + k)] = scale * re;
+            psi_ri[2 * (s * 12 + k) + 1] = scale * im;
+        }
 }
