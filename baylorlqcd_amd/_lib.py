@@ -77,6 +77,17 @@ SIGNATURES = {
     "qqcd_bench_dslash_multi": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
     "qqcd_autotune_multi": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp],
+    "qqcd_nn_init": [ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong],
+    "qqcd_nn_set_weights": [ctypes.c_int, _ip, _dp],
+    "qqcd_nn_get_weights": [_ip, _ip, _dp],
+    "qqcd_nn_layer_canonical": [ctypes.c_int, ctypes.c_int, _dp, _dp],
+    "qqcd_nn_wgrad_canonical": [ctypes.c_int, _dp, _dp, _dp],
+    "qqcd_nn_apply_canonical": [_dp, _dp],
+    "qqcd_nn_train": [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                      ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int,
+                      ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp, _dp],
+    "qqcd_bench_learned": [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_double, ctypes.c_int,
+                           ctypes.c_int, _dp, _ip, _ip, _dp, _ip, _ip],
 }
 
 

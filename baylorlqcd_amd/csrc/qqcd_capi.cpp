@@ -61,9 +61,14 @@ extern "C" int qqcd_init(const int n[4], const int bc[4], int device) {
   API_END
 }
 
+namespace qqcd {
+void nn_release_device();
+}
+
 extern "C" int qqcd_finalize(void) {
   API_BEGIN
   qqcd_deflation_clear();
+  nn_release_device();
   delete g_ctx;
   g_ctx = nullptr;
   API_END

@@ -281,10 +281,8 @@ MixedParams qqcd::mixed_params(const int GMRES[2], double tol, int basis, int op
   return p;
 }
 
-namespace {
-
-// unit source at 0-based global site x, Dirac d, colour c (pointsource)
-void point_source_dev(Context &c, Spinor &b_e, Spinor &b_o, const int src[4], int d, int col) {
+// unit source at 1-based global site src, Dirac d, colour col (pointsource)
+void qqcd::point_source_dev(Context &c, Spinor &b_e, Spinor &b_o, const int src[4], int d, int col) {
   const Geom &g = c.g;
   int x[4];
   for (int m = 0; m < 4; m++) {
@@ -300,7 +298,7 @@ void point_source_dev(Context &c, Spinor &b_e, Spinor &b_o, const int src[4], in
   QQCD_HIP(hipMemcpyAsync((double2 *)b.data + k * b.stride + cb, one, sizeof(one), hipMemcpyHostToDevice, c.stream));
   QQCD_HIP(hipStreamSynchronize(c.stream));
 }
-}  // namespace
+
 
 extern "C" int qqcd_solve_mixed(const double *phi, double *x, const int GMRES[2], double tol, int basis_prec,
                                 int op_prec, int op_recon, double inner_tol, int keep_deflation, const double *u,

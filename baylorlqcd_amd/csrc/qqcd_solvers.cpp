@@ -59,8 +59,15 @@ static double residual(Context &c, LinOp &A, const Spinor &b, const Spinor &x, S
 // ||w||^2, i.e. when cancellation may have spoiled orthogonality).  The
 // first pass computes <V_i, w> and <w, w> in one sweep; the Hessenberg
 // column goes to h[0..j+1].  Returns the new norm.
-static double arnoldi_step(Context &c, LinOp &A, std::vector<Spinor> &V, int j, zd *h) {
-  A.apply(V[j], V[j + 1]);
+static double arnoldi_step(Context &c, LinOp &A, std::vector<Spinor> &V, int j, zd *h, int reorth,
+                           LinOp *M = nullptr, std::vector<Spinor> *Z = nullptr) {
+  if (M) {
+    // flexible right preconditioning: z_j = M v_j, w = A z_j
+    M->apply(V[j], (*Z)[j]);
+    A.apply((*Z)[j], V[j + 1]);
+  } else {
+    A.apply(V[j], V[j + 1]);
+  }
   std::vector<Vec> Vb = vecs(V, j + 2);  // V[0..j] and w itself
   Vec w = vv(V[j + 1]);
   std::vector<zd> h1(j + 2), h2(j + 1);
@@ -68,7 +75,7 @@ static double arnoldi_step(Context &c, LinOp &A, std::vector<Spinor> &V, int j, 
   const double w2 = h1[j + 1].real();
   double n2 = c.blas->multi_axpy(j + 1, h1.data(), Vb.data(), w, true);
   for (int i = 0; i <= j; i++) h[i] = h1[i];
-  if (n2 < 0.5 * w2) {
+  if (reorth && n2 < 0.5 * w2) {
     c.blas->multi_dot(j + 1, Vb.data(), w, h2.data());
     n2 = c.blas->multi_axpy(j + 1, h2.data(), Vb.data(), w, true);
     for (int i = 0; i <= j; i++) h[i] += h2[i];
@@ -98,7 +105,7 @@ struct BasisGuard {
 // new (k+1) x k matrix and right-hand side.
 static bool deflate(Context &c, std::vector<Spinor> &V, std::vector<Spinor> &W, int j, int k,
                     const std::vector<zd> &H, int ldh, const std::vector<zd> &cv, const std::vector<zd> &y,
-                    std::vector<zd> &Hnew, std::vector<zd> &cnew) {
+                    std::vector<zd> &Hnew, std::vector<zd> &cnew, std::vector<zd> *Zout = nullptr) {
   const int rows = j + 1;
   std::vector<zd> srv(rows), P((size_t)j * k), theta(j);
   for (int i = 0; i < rows; i++) {
@@ -144,6 +151,7 @@ static bool deflate(Context &c, std::vector<Spinor> &V, std::vector<Spinor> &W, 
     for (int i = 0; i < rows; i++) s += std::conj(Z[i + (size_t)rows * p]) * srv[i];
     cnew[p] = s;
   }
+  if (Zout) *Zout = Z;
   // V[0..k] <- V[0..j] Z  (through W, then swap)
   std::vector<Vec> Vin = vecs(V, rows), Wout = vecs(W, k + 1);
   c.blas->combine(rows, Vin.data(), k + 1, Z.data(), rows, Wout.data(), false);
@@ -175,14 +183,20 @@ static void store_deflation(Context &c, Deflation *defl, const std::vector<Spino
 }
 
 SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int k, double tol, int itermin,
-                   int maxiter, Deflation *defl) {
+                   int maxiter, Deflation *defl, int reorth, LinOp *M, const CycleHook *hook) {
   if (m < 2 || k < 0 || k >= m) throw std::runtime_error("gmresdr: need 0 <= k < m and m >= 2");
+  if (M && defl) throw std::runtime_error("gmresdr: no deflation space is kept with a preconditioner");
   SolveStats st;
   const Prec p = A.prec;
-  std::vector<Spinor> V, W;
-  BasisGuard g1{V}, g2{W};
+  std::vector<Spinor> V, W, Z, ZW;
+  BasisGuard g1{V}, g2{W}, g3{Z}, g4{ZW};
   alloc_basis(c, V, m + 1, p);
   alloc_basis(c, W, k + 1, p);
+  if (M) {
+    alloc_basis(c, Z, m, p);
+    alloc_basis(c, ZW, k, p);
+  }
+  std::vector<zd> Zmat;
   SpinorBuf r(c.g, p);
   double rn = residual(c, A, b, x, r.s, st);
   const double r0 = rn;
@@ -205,7 +219,7 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
   for (;;) {
     // Arnoldi
     while (j < m) {
-      const double hn = arnoldi_step(c, A, V, j, col.data());
+      const double hn = arnoldi_step(c, A, V, j, col.data(), reorth, M, M ? &Z : nullptr);
       st.matvecs++;
       st.iters++;
       for (int i = 0; i <= j + 1; i++) H[i + (size_t)ldh * j] = col[i];
@@ -213,16 +227,17 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
       const double est = dense::lsq(j + 1, j, H.data(), ldh, cv.data(), y.data());
       if ((est < tol * r0 && st.iters >= itermin) || hn == 0.0 || st.iters >= maxiter) break;
     }
-    // solution update and true residual
+    // solution update (x += V y, or Z y when preconditioned) and true residual
     dense::lsq(j + 1, j, H.data(), ldh, cv.data(), y.data());
     {
-      std::vector<Vec> Vb = vecs(V, j);
+      std::vector<Vec> Vb = vecs(M ? Z : V, j);
       Vec xv = vv(x);
       c.blas->combine(j, Vb.data(), 1, y.data(), j, &xv, true);
     }
     rn = residual(c, A, b, x, r.s, st);
     st.cycles++;
     st.rel_res = rn / r0;
+    if (hook && *hook) (*hook)(x, r.s, st);
     const bool done = (rn < tol * r0 && st.iters >= itermin) || st.iters >= maxiter;
     if (done) {
       st.converged = rn < tol * r0;
@@ -234,7 +249,13 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
       if (defl && !have_defl) defl->clear();
       return st;
     }
-    if (j > k && k > 0 && deflate(c, V, W, j, k, H, ldh, cv, y, Hnew, cnew)) {
+    if (j > k && k > 0 && deflate(c, V, W, j, k, H, ldh, cv, y, Hnew, cnew, M ? &Zmat : nullptr)) {
+      if (M) {
+        // Z_k <- Z_j P_k (the first k columns of the restart combination; A Z_k = V_{k+1} Hbar_k)
+        std::vector<Vec> Zin = vecs(Z, j), Zo = vecs(ZW, k);
+        c.blas->combine(j, Zin.data(), k, Zmat.data(), j + 1, Zo.data(), false);
+        for (int q = 0; q < k; q++) std::swap(Z[q], ZW[q]);
+      }
       std::fill(H.begin(), H.end(), zd(0.0));
       std::fill(cv.begin(), cv.end(), zd(0.0));
       for (int q = 0; q < k; q++)
@@ -258,7 +279,7 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
 }
 
 SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, const Deflation &defl, double tol,
-                     int itermin, int maxiter) {
+                     int itermin, int maxiter, int reorth) {
   SolveStats st;
   const Prec p = A.prec;
   const int k = defl.k;
@@ -298,7 +319,7 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
     cv[0] = rn;
     int j = 0;
     while (j < m) {
-      const double hn = arnoldi_step(c, A, V, j, col.data());
+      const double hn = arnoldi_step(c, A, V, j, col.data(), reorth);
       st.matvecs++;
       st.iters++;
       for (int i = 0; i <= j + 1; i++) H[i + (size_t)ldh * j] = col[i];
@@ -409,6 +430,7 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
   ms.fp64_matvecs++;
   Deflation local;
   Deflation &D = defl ? *defl : local;
+  bool use_pre = p.precond != nullptr;
   if (D.k > 0 && (D.prec != p.basis || D.k != p.k)) D.clear();
   while (rn >= p.tol * bn && ms.outer < p.max_outer && ms.inner.iters < p.maxiter) {
     if (op_prec == PREC_DOUBLE && p.basis == PREC_DOUBLE && ms.outer > 0 && ms.stalled) {
@@ -428,10 +450,15 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
     const double itol = std::max(inner_tol, 0.5 * p.tol * bn / rn);
     SolveStats st;
     const int budget = p.maxiter - ms.inner.iters;
-    if (D.k == 0)
-      st = gmresdr(c, inner, rl.s, el.s, p.m, p.k, itol, 0, std::min(budget, p.first_cycles * p.m), &D);
+    if (use_pre) {
+      st = gmresdr(c, inner, rl.s, el.s, p.m, p.k, itol, 0, std::min(budget, p.first_cycles * p.m), nullptr,
+                   p.reorth, p.precond);
+      ms.precond_steps += st.iters;
+    } else if (D.k == 0)
+      st = gmresdr(c, inner, rl.s, el.s, p.m, p.k, itol, 0, std::min(budget, p.first_cycles * p.m), &D, p.reorth);
     else
-      st = gmresproj(c, inner, rl.s, el.s, p.m - p.k, D, itol, 0, std::min(budget, p.later_cycles * (p.m - p.k)));
+      st = gmresproj(c, inner, rl.s, el.s, p.m - p.k, D, itol, 0, std::min(budget, p.later_cycles * (p.m - p.k)),
+                     p.reorth);
     ms.inner.iters += st.iters;
     ms.inner.matvecs += st.matvecs;
     ms.inner.cycles += st.cycles;
@@ -441,7 +468,12 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
     ms.fp64_matvecs++;
     ms.outer++;
     ms.stalled = rn > p.stall * rn_old;
-    if (ms.stalled) {
+    if (ms.stalled && use_pre) {
+      // the unpreconditioned solver takes over
+      use_pre = false;
+      ms.precond_dropped = 1;
+      ms.stalled = 0;
+    } else if (ms.stalled) {
       // the low-precision operator no longer reduces the true residual:
       // climb to the next operator precision
       if (op_prec == PREC_HALF) {

@@ -56,13 +56,23 @@ struct Deflation {
 // GMRES-DR(m, k): x holds x0 on entry.  Stops when ||b - A x|| < tol ||b - A x0||.
 // When defl is given, the deflation space of the last restart (or of the
 // final cycle) is stored there.
+// Called after every cycle with the current iterate, its true residual and
+// the statistics so far.
+typedef std::function<void(const Spinor &x, const Spinor &r, const SolveStats &st)> CycleHook;
+
+// reorth = 1: a second classical Gram-Schmidt pass whenever the first
+// removed more than half of ||w||^2 (the DGKS criterion); 0: one pass.
+// M: right preconditioner, applied flexibly (FGMRES-DR: z_j = M v_j is kept,
+// A Z_m = V_{m+1} Hbar_m, x += Z y, and the deflated restart combines Z with
+// the same harmonic Ritz coefficients as V); no deflation space is kept then.
 SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int k, double tol, int itermin,
-                   int maxiter, Deflation *defl);
+                   int maxiter, Deflation *defl, int reorth = 1, LinOp *M = nullptr,
+                   const CycleHook *hook = nullptr);
 
 // GMRES-Proj(m, k) with deflation space defl: alternates a projection over
 // (V_{k+1}, Hbar_k) with a GMRES(m) cycle.  x holds x0 on entry.
 SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, const Deflation &defl, double tol,
-                     int itermin, int maxiter);
+                     int itermin, int maxiter, int reorth = 1);
 
 // BiCGStab from x (Frommer et al.; the reference's bicgstab).  Convergence
 // on the recursive residual, then confirmed on the true residual (restarting
@@ -99,6 +109,10 @@ struct MixedParams {
   int first_cycles = 30;  // GMRES-DR cycles allowed in the first inner solve
   int later_cycles = 3;   // GMRES-Proj cycles per later inner solve
   double stall = 0.5;     // an outer step that shrinks ||r|| by less stalls
+  int reorth = 0;         // Gram-Schmidt of the inner (low-precision) solves: 0 one pass, 1 DGKS
+  // right preconditioner of the inner solves (flexible GMRES-DR, restarted
+  // fresh in every outer step); dropped for good at the first stalled outer step
+  LinOp *precond = nullptr;
 };
 
 struct MixedStats {
@@ -109,6 +123,8 @@ struct MixedStats {
   int converged = 0;
   int stalled = 0;
   int fallback = 0;  // 1: operator raised to FP32, 2: finished in plain FP64
+  int precond_dropped = 0;  // the preconditioner stalled and was switched off
+  int precond_steps = 0;    // Krylov steps taken with the preconditioner
 };
 
 MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spinor &x, const MixedParams &p,
@@ -117,6 +133,9 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
 // MixedParams from the C-ABI arguments (GMRES = (m, k); basis 0 FP64 / 1 FP32;
 // op_prec 0/1/2; inner_tol <= 0 picks 1e-2 (FP16), 1e-6 (FP32), 1e-10 (FP64))
 MixedParams mixed_params(const int GMRES[2], double tol, int basis, int op_prec, int op_recon, double inner_tol);
+
+// unit source at 1-based global site src[4], Dirac d, colour col (both checkerboards zeroed first by the caller)
+void point_source_dev(Context &c, Spinor &b_e, Spinor &b_o, const int src[4], int d, int col);
 
 // even-odd source preparation / reconstruction / full-system residual of
 // fermprop (qqcd-test/cfgsprops/quark/quark.f90:600-617, 797-819), FP64

@@ -236,6 +236,55 @@ int qqcd_batch_set(int reorth, int use_graph, double inner_tol, int inner_cycles
 int qqcd_batch_stats(int *outer, int *cycles, long *slot_matvecs, long *batched_matvecs, int *graph_replays,
                      int *fallback);
 
+/* ---- learned preconditioner (no counterpart in the reference) ---------
+ * A gauge-equivariant network of parallel-transport convolution layers:
+ *   out_a(x) = sum_{b, p} W[a][b][p] (T_p psi_b)(x),
+ * p over {0, +x, -x, +y, -y, +z, -z, +t, -t} (0, 1+2mu, 2+2mu), T_0 = 1,
+ * (T_+mu psi)(x) = U_mu(x) psi(x+mu), (T_-mu psi)(x) = U_mu(x-mu)^dag psi(x-mu)
+ * (edge hops times bc[mu]); W[a][b][p] a complex 4x4 spin matrix (identity on
+ * colour).  Channel counts channels[0..nlayers] with channels[0] =
+ * channels[nlayers] = 1 and every count 1 or 8.  Weights W of layer l are
+ * complex (re, im) in the order ((a*cin + b)*9 + p)*16 + 4 s + s', layers back
+ * to back.  Activations BF16, weights BF16 on the matrix cores, FP32
+ * accumulation.  The host weights survive qqcd_finalize / qqcd_init, so a
+ * network trained on one lattice can be applied on another. */
+/* Start near the point-Jacobi preconditioner scale * 1, plus noise * Gaussian weights. */
+int qqcd_nn_init(int nlayers, const int *channels, double scale, double noise, unsigned long long seed);
+int qqcd_nn_set_weights(int nlayers, const int *channels, const double *W);
+/* nlayers, channels[nlayers + 1]; W may be NULL to query the sizes only */
+int qqcd_nn_get_weights(int *nlayers, int *channels, double *W);
+/* One layer (adjoint = 1: its adjoint, reversed paths and W^dag) on canonical
+ * arrays in[C][V][4][3], out[C'][V][4][3] complex, with the resident links. */
+int qqcd_nn_layer_canonical(int layer, int adjoint, const double *in, double *out);
+/* dL/dconj(W) of layer l: sum_{x, colour} g_a(x) (T_p psi_b)(x)^dag, complex,
+ * weight order; psi: the layer input, g: the gradient at its output. */
+int qqcd_nn_wgrad_canonical(int layer, const double *psi, const double *g, double *dW);
+/* z = N(r) on the even sites of canonical arrays (the preconditioner as the solver applies it). */
+int qqcd_nn_apply_canonical(const double *r, double *z);
+/* Train on nconf seeded heatbath configurations of the current lattice:
+ * (residual, error) pairs from the cycle ends of plain FP64 GMRES-DR(m, k)
+ * solves of nsrc Gaussian sources per configuration (kappa uniform in
+ * [kappa_lo, kappa_hi]), at most max_pairs; loss ||N(r) - e||^2 / ||e||^2 with
+ * ||r|| = 1; `steps` Adam steps (lr, betas 0.9 / 0.999) on minibatches of
+ * `batch` pairs, gradients from the device forward / adjoint / weight-gradient
+ * kernels.  keep = 0 starts from the kappa^2-Jacobi initialisation with the
+ * given channels.  loss[steps]: minibatch losses; stats[5] = {pairs, harvest
+ * seconds, training seconds, mean loss before, mean loss after}.  Leaves the
+ * last training configuration resident. */
+int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo, double kappa_hi, int nsrc, int m, int k,
+                  double harvest_tol, int max_pairs, int nlayers, const int *channels, int keep, int steps, int batch,
+                  double lr, unsigned long long seed, double *loss, double *stats);
+/* Time one solve of M x = point source on the resident field: method 0
+ * plain FP64 GMRES-DR(m,k); 1 the mixed solver (qqcd_solve_mixed with an FP32
+ * basis and (op_prec, op_recon) operator); 2 the same with the network as
+ * the flexible right preconditioner of every inner solve (FGMRES-DR),
+ * dropped for the unpreconditioned solver at the first outer step that does
+ * not halve the true residual.  nn_steps: Krylov steps taken with the
+ * network; dropped: 1 if it was switched off. */
+int qqcd_bench_learned(int method, const int src[4], int spin, int colour, const double kappa[2], const int GMRES[2],
+                       double tol, int op_prec, int op_recon, double *seconds, int *iters, int *outer,
+                       double *true_res, int *nn_steps, int *dropped);
+
 /* Statistics of the last solver call: final ||r|| / ||r0|| (true residual),
  * Krylov steps, all operator applications, cycles, converged flag. */
 int qqcd_last_solve_stats(double *rel_res, int *iters, int *matvecs, int *cycles, int *converged);
