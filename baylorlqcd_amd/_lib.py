@@ -86,6 +86,7 @@ SIGNATURES = {
     "qqcd_nn_train": [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                       ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int,
                       ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp, _dp],
+    "qqcd_bench_nn": [ctypes.c_int, ctypes.c_int, _dp],
     "qqcd_bench_learned": [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_double, ctypes.c_int,
                            ctypes.c_int, _dp, _ip, _ip, _dp, _ip, _ip],
 }

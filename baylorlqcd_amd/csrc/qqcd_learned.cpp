@@ -88,10 +88,10 @@ struct Pair {
 };
 
 struct TrainSet {
-  std::vector<Gauge> U32;  // one FP32 18-real field per configuration
+  std::vector<NNLinks> U;  // the BF16 links of every configuration
   std::vector<Pair> pairs;
   ~TrainSet() {
-    for (auto &u : U32) free_gauge(u);
+    for (auto &u : U) nn_free_links(u);
     for (auto &p : pairs) {
       free_spinor(p.r);
       free_spinor(p.e);
@@ -155,7 +155,7 @@ static void harvest(Context &c, const double kappa[2], int nsrc, int m, int k, d
 
 // One pass over a pair: forward, loss ||N(r) - e||^2 / ||e||^2, backward
 // into dW (accumulated).  Returns the loss.
-static double pair_grad(Context &c, NNModel &M, const Gauge &U, const Pair &p, double *d_scr,
+static double pair_grad(Context &c, NNModel &M, const NNLinks &U, const Pair &p, double *d_scr,
                         std::vector<std::vector<std::complex<double>>> &dW) {
   const int L = (int)M.layers.size();
   nn_from_spinor(c.g, p.r, M.acts[0], 1.f, c.stream);
@@ -174,7 +174,7 @@ static double pair_grad(Context &c, NNModel &M, const Gauge &U, const Pair &p, d
 }
 
 // loss only (evaluation)
-static double pair_loss(Context &c, NNModel &M, const Gauge &U, const Pair &p, double *d_scr) {
+static double pair_loss(Context &c, NNModel &M, const NNLinks &U, const Pair &p, double *d_scr) {
   const int L = (int)M.layers.size();
   nn_from_spinor(c.g, p.r, M.acts[0], 1.f, c.stream);
   for (int l = 0; l < L; l++) nn_layer(c.g, U, M.dev[l], 0, M.acts[l], M.acts[l + 1], c.stream);
@@ -254,7 +254,7 @@ extern "C" int qqcd_nn_layer_canonical(int layer, int adjoint, const double *in,
       QQCD_HIP(hipMemcpyAsync(d, in + ch * n, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
       nn_from_canon(c.g, d, fi, ch, c.stream);
     }
-    nn_layer(c.g, c.gauge(PREC_SINGLE, 18), M.dev[layer], adjoint, fi, fo, c.stream);
+    nn_layer(c.g, M.links_for(c), M.dev[layer], adjoint, fi, fo, c.stream);
     for (int ch = 0; ch < co; ch++) {
       nn_to_canon(c.g, fo, ch, d, c.stream);
       QQCD_HIP(hipMemcpyAsync(out + ch * n, d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -293,7 +293,7 @@ extern "C" int qqcd_nn_wgrad_canonical(int layer, const double *psi, const doubl
       QQCD_HIP(hipMemcpyAsync(d, g + ch * n, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
       nn_from_canon(c.g, d, fg, ch, c.stream);
     }
-    nn_wgrad(c.g, c.gauge(PREC_SINGLE, 18), ci, co, fi, fg, (std::complex<double> *)dW, c.stream);
+    nn_wgrad(c.g, M.links_for(c), ci, co, fi, fg, (std::complex<double> *)dW, c.stream);
   } catch (...) {
     free_nn(fi);
     free_nn(fg);
@@ -341,11 +341,8 @@ extern "C" int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo
     const double kap[2] = {kappa_lo + (kappa_hi - kappa_lo) * uni(rng), 0.0};
     const int before = (int)T.pairs.size();
     harvest(c, kap, nsrc, m, k, harvest_tol, seed + 31ull * conf, conf, before + per_conf, T);
-    // keep this configuration's FP32 links
-    const Gauge &U = c.gauge(PREC_SINGLE, 18);
-    Gauge keepU = alloc_gauge(c.g, PREC_SINGLE, 18);
-    QQCD_HIP(hipMemcpyAsync(keepU.data, U.data, U.bytes(), hipMemcpyDeviceToDevice, c.stream));
-    T.U32.push_back(keepU);
+    // keep this configuration's links
+    T.U.push_back(nn_links(c.g, c.gauge(PREC_SINGLE, 18), c.stream));
   }
   QQCD_HIP(hipStreamSynchronize(c.stream));
   const auto t1 = std::chrono::steady_clock::now();
@@ -358,7 +355,7 @@ extern "C" int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo
   QQCD_HIP(hipMalloc(&d_scr, sizeof(double) * ((2 * c.g.Vh + 255) / 256 + 16)));
   auto mean_loss = [&]() {
     double s = 0;
-    for (auto &p : T.pairs) s += pair_loss(c, M, T.U32[p.conf], p, d_scr);
+    for (auto &p : T.pairs) s += pair_loss(c, M, T.U[p.conf], p, d_scr);
     return s / T.pairs.size();
   };
   try {
@@ -378,7 +375,7 @@ extern "C" int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo
       double ls = 0;
       for (int bi = 0; bi < batch; bi++) {
         const Pair &p = T.pairs[pick(rng)];
-        ls += pair_grad(c, M, T.U32[p.conf], p, d_scr, dW);
+        ls += pair_grad(c, M, T.U[p.conf], p, d_scr, dW);
       }
       loss[step] = ls / batch;
       const double bc1 = 1.0 - std::pow(b1, step + 1), bc2 = 1.0 - std::pow(b2, step + 1);
@@ -486,5 +483,30 @@ extern "C" int qqcd_bench_learned(int method, const int src[4], int spin, int co
   const auto t1 = std::chrono::steady_clock::now();
   *seconds = std::chrono::duration<double>(t1 - t0).count();
   *true_res = full_residual(c, kappa, b_e.s, b_o.s, x_e.s, x_o.s);
+  API_END
+}
+
+// Average time of one network application z = N(r) (device-resident, seeded
+// Gaussian r on the even sites) over niter launches after nwarm.
+extern "C" int qqcd_bench_nn(int nwarm, int niter, double *ms) {
+  API_BEGIN
+  Context &c = ctx();
+  if (!c.have_gauge) throw std::runtime_error("no gauge field loaded");
+  NNModel &M = nn_model(c);
+  SpinorBuf r(c.g, PREC_SINGLE), z(c.g, PREC_SINGLE);
+  random_spinor(c.g, r.s, 99, c.stream);
+  for (int i = 0; i < nwarm; i++) nn_apply(c, M, r.s, z.s);
+  hipEvent_t e0, e1;
+  QQCD_HIP(hipEventCreate(&e0));
+  QQCD_HIP(hipEventCreate(&e1));
+  QQCD_HIP(hipEventRecord(e0, c.stream));
+  for (int i = 0; i < niter; i++) nn_apply(c, M, r.s, z.s);
+  QQCD_HIP(hipEventRecord(e1, c.stream));
+  QQCD_HIP(hipEventSynchronize(e1));
+  float f = 0;
+  QQCD_HIP(hipEventElapsedTime(&f, e0, e1));
+  *ms = f / std::max(niter, 1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   API_END
 }

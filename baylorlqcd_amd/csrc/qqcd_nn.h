@@ -41,21 +41,29 @@ struct NNLayer {
 // kernel consumes (BF16), for the forward layer and for its adjoint.
 struct NNLayerDev {
   int cin = 0, cout = 0;
-  uint16_t *B = nullptr;     // forward
+  uint16_t *B = nullptr;     // forward (transport first)
   uint16_t *Badj = nullptr;  // adjoint (reversed paths, W^dag)
-  size_t nB = 0, nBadj = 0;
+  uint16_t *Bmix = nullptr;  // the 8 -> 1 direction as mix-then-gather (forward of 8->1, adjoint of 1->8)
+  size_t nB = 0, nBadj = 0, nBmix = 0;
 };
 void nn_upload(const NNLayer &L, NNLayerDev &D, hipStream_t s);
 void nn_free(NNLayerDev &D);
 
-// out = layer(in) (adjoint = 0) or layer^dag(in) (adjoint = 1); links: the
-// FP32 18-real gauge copy.
-void nn_layer(const Geom &g, const Gauge &U32, const NNLayerDev &D, int adjoint, const NNField &in, NNField &out,
+// Links as BF16 complex pairs in the 18-real component-run layout.
+struct NNLinks {
+  uint32_t *data = nullptr;
+  long stride = 0;
+};
+NNLinks nn_links(const Geom &g, const Gauge &U32, hipStream_t s);
+void nn_free_links(NNLinks &L);
+
+// out = layer(in) (adjoint = 0) or layer^dag(in) (adjoint = 1)
+void nn_layer(const Geom &g, const NNLinks &U, const NNLayerDev &D, int adjoint, const NNField &in, NNField &out,
               hipStream_t s);
 // Weight gradient dL/dconj(W) of one layer, summed over the lattice:
 // dW[a][b][p][s][s'] = sum_x,c g_a(x)[s,c] conj((T_p psi_b)(x)[s',c]); written
 // as complex doubles in the weight order into host array dW (cout*cin*9*16).
-void nn_wgrad(const Geom &g, const Gauge &U32, int cin, int cout, const NNField &psi, const NNField &gout,
+void nn_wgrad(const Geom &g, const NNLinks &U, int cin, int cout, const NNField &psi, const NNField &gout,
               std::complex<double> *dW, hipStream_t s);
 
 // even-checkerboard FP32/FP64 spinor <-> channel 0 of a network field (the
@@ -77,6 +85,9 @@ struct NNModel {
   std::vector<NNField> acts;  // activations of the last forward pass (L + 1 fields)
   std::vector<NNField> grads; // gradients with respect to the activations (training only)
   bool ready = false;
+  NNLinks links;  // BF16 links of the context's gauge field
+  uint64_t links_gen = ~0ull;
+  const NNLinks &links_for(Context &c);
   void init(const std::vector<int> &channels, uint64_t seed);
   void upload(hipStream_t s);
   void alloc(const Geom &g);

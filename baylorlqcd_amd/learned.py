@@ -100,3 +100,9 @@ def bench(method, kappa, GMRES=(40, 16), tol=1e-10, op_prec=1, op_recon=12, src=
         _lib.dp(res), nns.ctypes.data_as(_lib._ip), drop.ctypes.data_as(_lib._ip)))
     return {"seconds": float(sec[0]), "iterations": int(it[0]), "outer": int(outer[0]),
             "true_residual": float(res[0]), "nn_steps": int(nns[0]), "dropped": int(drop[0])}
+
+
+def bench_nn(nwarm=2, niter=10):
+    ms = ctypes.c_double()
+    _lib.check(_lib.lib().qqcd_bench_nn(int(nwarm), int(niter), ctypes.byref(ms)))
+    return ms.value

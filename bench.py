@@ -11,15 +11,18 @@ norms; 18- and 12-real links) are measured on the same input and reported in
 Algorithmic bytes per output site: 8*24*s_psi + 8*L*s_U + 24*s_psi
 (L = 18 or 12 link reals); FP16 spinors add 9*4 bytes of per-site norms.
 
-A second section times the solve path on the same lattice: a seeded
-beta = 6.0 heatbath field (device heatbath), a point source, twisted-mass
-kappa = 0.1565, mu = 0.005, solved to a true full-system residual
-||b - M x|| / ||b|| < 1e-10 by (a) the plain FP64 GMRES-DR(40,16) and (b) the
-mixed-precision solver (FP32 Krylov basis, FP16 / 12-real operator, FP64
-reliable updates); both times, iteration counts and residuals are reported
-in ``solve``.
+A second section (``solve``) times the solve path on a seeded beta = 6.0
+heatbath field (device heatbath), point source, to a true full-system
+residual ||b - M x|| / ||b|| < 1e-10: at Wilson kappa = 0.1565 the plain
+FP64 GMRES-DR(40,16), the mixed-precision solver (FP32 Krylov basis, FP32 or
+FP16 12-real operator, FP64 reliable updates) and the mixed solver with the
+learned preconditioner (trained first on 16^3x32 heatbath configurations,
+flexible GMRES-DR); at twisted mass kappa = 0.1565, mu = 0.005 plain and
+mixed.  A third (``noise_batch``) runs the disconnected-loop workload on
+24^3x48: 64 Z4 sources, the first by GMRES-DR(60,20), the rest together by
+the batched GMRES-Proj, against the sequential plain FP64 and mixed solves.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-solve]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-solve] [--no-noise]
 """
 from __future__ import annotations
 
@@ -34,7 +37,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from baylorlqcd_amd import _lib, diracops, gauge, lattice  # noqa: E402
+from baylorlqcd_amd import _lib, diracops, disconloops, gauge, lattice, learned  # noqa: E402
 
 N = (32, 32, 32, 64)
 BC = (1, 1, 1, -1)
@@ -105,30 +108,105 @@ def bench_solve(method, kappa, op_prec=2, recon=12, mk=(40, 16), tol=1e-10):
             "true_residual": float(f"{res[0]:.3e}")}
 
 
-def solve_section():
-    """Time to a 1e-10 true residual on a 32^3x64 beta=6.0 heatbath field."""
+def heatbath_field(seed=2024, sweeps=200):
     co = np.zeros(64)
     for mu in range(4):
         co[2 + 4 * (0 + 4 * mu)] = 1.0
         co[2 + 4 * (1 + 4 * mu)] = 1.0
     t0 = time.perf_counter()
-    diracops.gauge_heatbath(6.0, 200, 2024, 1, co)
+    diracops.gauge_heatbath(6.0, sweeps, seed, 1, co)
     plaq = diracops.gauge_plaquette()
-    t_hb = time.perf_counter() - t0
-    print(f"[bench] heatbath 200 sweeps {t_hb:.1f}s, plaquette {plaq:.4f}", file=sys.stderr, flush=True)
-    kappa = (0.1565, 0.005)
-    plain = bench_solve(0, kappa)
-    print(f"[bench] plain FP64 GMRES-DR(40,16): {plain}", file=sys.stderr, flush=True)
-    mixed = bench_solve(1, kappa, 2, 12)
-    print(f"[bench] mixed FP32 basis / FP16-12 operator: {mixed}", file=sys.stderr, flush=True)
-    mixed32 = bench_solve(1, kappa, 1, 12)
-    print(f"[bench] mixed FP32 basis / FP32-12 operator: {mixed32}", file=sys.stderr, flush=True)
+    print(f"[bench] heatbath {sweeps} sweeps {time.perf_counter() - t0:.1f}s, plaquette {plaq:.4f}",
+          file=sys.stderr, flush=True)
+    return co, plaq
+
+
+def solve_section(train_steps=300):
+    """Time to a 1e-10 true residual on a 32^3x64 beta=6.0 heatbath field:
+    plain FP64 GMRES-DR, the mixed solver, and the mixed solver with the
+    learned preconditioner trained on 16^3x32 configurations (Wilson kappa in
+    [0.150, 0.157], pairs from plain GMRES-DR iterates), at Wilson kappa
+    0.1565; plus the twisted-mass point kappa 0.1565, mu 0.005."""
+    lattice.finalize()
+    lattice.init((16, 16, 16, 32), BC)
+    t0 = time.perf_counter()
+    tr = learned.train(nconf=4, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=1, m=40, k=16, harvest_tol=1e-8,
+                       max_pairs=256, channels=(1, 8, 8, 8, 1), steps=train_steps, batch=4, lr=1e-3, seed=11)
+    train = {"lattice": [16, 16, 16, 32], "configs": 4, "pairs": tr["pairs"], "adam_steps": train_steps,
+             "batch": 4, "lr": 1e-3, "loss_before": round(tr["loss_before"], 4),
+             "loss_after": round(tr["loss_after"], 4), "harvest_s": round(tr["harvest_s"], 2),
+             "train_s": round(tr["train_s"], 2), "wall_s": round(time.perf_counter() - t0, 2)}
+    print(f"[bench] training {train}", file=sys.stderr, flush=True)
+    lattice.finalize()
+    lattice.init(N, BC)
+    co, plaq = heatbath_field()
+    nn_ms = learned.bench_nn(2, 10)
+    print(f"[bench] network application {nn_ms:.3f} ms", file=sys.stderr, flush=True)
+    wil = (0.1565, 0.0)
+    plain = learned.bench(0, wil, GMRES=(40, 16))
+    print(f"[bench] Wilson plain FP64 GMRES-DR(40,16): {plain}", file=sys.stderr, flush=True)
+    mixed = learned.bench(1, wil, GMRES=(40, 16), op_prec=1, op_recon=12)
+    print(f"[bench] Wilson mixed FP32: {mixed}", file=sys.stderr, flush=True)
+    mixed16 = learned.bench(1, wil, GMRES=(40, 16), op_prec=2, op_recon=12)
+    print(f"[bench] Wilson mixed FP16 operator: {mixed16}", file=sys.stderr, flush=True)
+    lrn = learned.bench(2, wil, GMRES=(40, 16), op_prec=1, op_recon=12)
+    print(f"[bench] Wilson mixed FP32 + learned: {lrn}", file=sys.stderr, flush=True)
+    tm = (0.1565, 0.005)
+    tplain = bench_solve(0, tm)
+    print(f"[bench] twisted plain FP64: {tplain}", file=sys.stderr, flush=True)
+    tmixed = bench_solve(1, tm, 1, 12)
+    print(f"[bench] twisted mixed FP32: {tmixed}", file=sys.stderr, flush=True)
+    best = min((mixed, mixed16, lrn), key=lambda r: r["seconds"])
     return {"lattice": list(N), "beta": 6.0, "heatbath_sweeps": 200, "plaquette": round(plaq, 5),
-            "kappa": kappa[0], "mu": kappa[1], "source": "point, origin, spin 1 colour 1",
+            "source": "point, origin, spin 1 colour 1",
             "target": "||b - M x|| / ||b|| < 1e-10 (FP64, full system)",
-            "plain_fp64_gmresdr_40_16": plain, "mixed_fp16op_fp32basis": mixed, "mixed_fp32op_fp32basis": mixed32,
-            "speedup_mixed_vs_plain": round(plain["seconds"] / mixed["seconds"], 2),
-            "learned_preconditioner": "not built yet"}
+            "wilson_kappa_0.1565": {"plain_fp64_gmresdr_40_16": plain, "mixed_fp32": mixed,
+                                    "mixed_fp16_operator": mixed16, "mixed_fp32_learned": lrn,
+                                    "speedup_best_vs_plain": round(plain["seconds"] / best["seconds"], 2),
+                                    "speedup_learned_vs_plain": round(plain["seconds"] / lrn["seconds"], 2),
+                                    "iteration_ratio_learned_vs_mixed": round(lrn["iterations"] /
+                                                                               max(1, mixed["iterations"]), 3)},
+            "twisted_kappa_0.1565_mu_0.005": {"plain_fp64_gmresdr_40_16": tplain, "mixed_fp32": tmixed,
+                                              "speedup_mixed_vs_plain": round(tplain["seconds"] /
+                                                                              tmixed["seconds"], 2)},
+            "learned_preconditioner": {"training": train, "network_ms_per_application": round(nn_ms, 3),
+                                       "layers": "4 parallel-transport convolutions, channels 1-8-8-8-1, "
+                                                 "BF16 MFMA, FP32 accumulation"}}
+
+
+def noise_section(nrhs=64, seq=3):
+    """The disconnected-loop workload: 24^3x48 beta=6.0 heatbath field,
+    Wilson kappa 0.1565, nrhs Z4 sources, GMRES-DR(60,20) on the first and
+    the batched GMRES-Proj(40) over its deflation space on the rest, every
+    source to a 1e-10 true residual, 16 local trace estimators; against the
+    same sources one after the other in plain FP64 (psc sequence), timed on
+    `seq` sources and scaled to nrhs - 1."""
+    lattice.finalize()
+    n3 = (24, 24, 24, 48)
+    lattice.init(n3, BC)
+    co, plaq = heatbath_field()
+    kappa = (0.1565, 0.0)
+    r = disconloops.noise_loop(nrhs, 99, kappa, (60, 20), tol=1e-10, coact=co, bc=BC)
+    print(f"[bench] batched noise loop: times {r['times']} stats {r['stats']}", file=sys.stderr, flush=True)
+    s = disconloops.noise_loop(seq, 99, kappa, (60, 20), tol=1e-10, coact=co, bc=BC, method=1)
+    per = s["times"][1] / max(1, seq - 1)
+    print(f"[bench] sequential plain FP64: times {s['times']}", file=sys.stderr, flush=True)
+    m = disconloops.noise_loop(seq, 99, kappa, (60, 20), tol=1e-10, coact=co, bc=BC, method=2)
+    perm = m["times"][1] / max(1, seq - 1)
+    print(f"[bench] sequential mixed: times {m['times']}", file=sys.stderr, flush=True)
+    nb, order, tms = disconloops.autotune_multi(1, 12, nrhs - 1)
+    return {"lattice": list(n3), "kappa": kappa[0], "sources": nrhs, "noise": "Z4 {1,-1,i,-i}",
+            "plaquette": round(plaq, 5),
+            "batched": {"first_source_s": round(r["times"][0], 3), "batched_s": round(r["times"][1], 3),
+                        "total_s": round(r["times"][2], 3), "max_rel_residual": float(np.max(r["res_full"])),
+                        "iterations_per_source": int(np.median(r["iters"][1:])) if nrhs > 1 else None,
+                        "stats": r["stats"]},
+            "autotune": {"rhs_block": nb, "tile_order": order, "ms_per_hop": round(tms, 4)},
+            "sequential_plain_fp64_s_per_source": round(per, 3),
+            "sequential_mixed_s_per_source": round(perm, 3),
+            "speedup_batched_vs_sequential_plain": round(per * (nrhs - 1) / r["times"][1], 2),
+            "speedup_batched_vs_sequential_mixed": round(perm * (nrhs - 1) / r["times"][1], 2),
+            "traces_timeslice0_gamma5": [float(r["traces"][0, 5].real), float(r["traces"][0, 5].imag)]}
 
 
 def main():
@@ -139,6 +217,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-noise", action="store_true", help="skip the 24^3x48 batched noise-source section")
+    ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--copy-probe", action="store_true", help="also time a 1 GiB plain copy kernel")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -181,10 +261,6 @@ def main():
                 "GB/s": round(2 * (1 << 30) / (cms[0] * 1e-3) / 1e9, 1)}
         print(f"[bench] copy {copy}", file=sys.stderr, flush=True)
 
-    solve = None
-    if not args.no_solve:
-        solve = solve_section()
-
     prof = None
     ppath = os.path.join(ROOT, "profiles", "traffic_dslash_f64_18_32x32x32x64.json")
     if os.path.exists(ppath):
@@ -197,6 +273,13 @@ def main():
             cpu = cpu_baseline(diracops.gauge_get_canonical(N), vh)
         except Exception as e:  # the baseline is reported, never required
             cpu = {"value": None, "unit": "GB/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
+
+    solve = None
+    if not args.no_solve:
+        solve = solve_section(args.train_steps)
+    noise = None
+    if not args.no_noise:
+        noise = noise_section()
 
     line = {
         "metric": "Dslash HBM GB/s (algorithmic bytes / kernel time), 32^3x64 Wilson hopping term",
@@ -218,6 +301,7 @@ def main():
         "cpu_baseline": cpu,
         "sweep": sweep,
         "solve": solve,
+        "noise_batch": noise,
     }
     if copy:
         line["copy_probe"] = copy

@@ -285,6 +285,10 @@ int qqcd_bench_learned(int method, const int src[4], int spin, int colour, const
                        double tol, int op_prec, int op_recon, double *seconds, int *iters, int *outer,
                        double *true_res, int *nn_steps, int *dropped);
 
+/* Average time (ms, HIP events on the library stream) of one application of
+ * the network on the resident field. */
+int qqcd_bench_nn(int nwarm, int niter, double *ms);
+
 /* Statistics of the last solver call: final ||r|| / ||r0|| (true residual),
  * Krylov steps, all operator applications, cycles, converged flag. */
 int qqcd_last_solve_stats(double *rel_res, int *iters, int *matvecs, int *cycles, int *converged);
