@@ -7,6 +7,7 @@
 !   bicgstab, gmresdr, psc, gmresproject, vecdot
 !                                         qqcd-test/cfgsprops/quark/inverters.f90
 !   pointsource, fermprop                 qqcd-test/cfgsprops/quark/quark.f90
+!   z4source                              qqcd-test/cfgsprops/quark/disconloops.f90
 ! and forward to the HIP library through its C ABI (include/qqcd_hip.h).  A
 ! host program replaces `use diracops` / `use inverters` (and the two quark
 ! procedures) with `use qqcd_hip`; everything else in the host is unchanged.
@@ -31,9 +32,13 @@ module qqcd_hip
 
    public :: Hsingle, Hdouble, Hdbletm, gammamult, gam5x, vecdot, bicgstab, gmresdr, psc, &
              gmresproject, pointsource, fermprop, qqcd_gauge_changed, qqcd_shutdown, &
-             qqcd_solve_mixed_f
+             qqcd_solve_mixed_f, z4source, qqcd_z4_seed
 
    logical, save :: started = .false.
+   ! the Z4 stream: seed and the index of the next source (the reference's
+   ! z4source advances its lagged-Fibonacci stream on every call)
+   integer(c_long_long), save :: z4_seed = 1_c_long_long
+   integer(c_long), save :: z4_next = 0_c_long
 
    interface
       integer(c_int) function c_init(n, bc, device) bind(C, name="qqcd_init")
@@ -126,6 +131,13 @@ module qqcd_hip
          type(c_ptr), value :: phi, x, gm, u, coact, kappa, bc, itercount
          real(c_double), value :: tol, itol
          integer(c_int), value :: basis, opp, recon, keep
+      end function
+      integer(c_int) function c_z4source(be, bo, seed, rhs, scale) bind(C, name="qqcd_z4source")
+         import :: c_int, c_ptr, c_double, c_long_long, c_long
+         type(c_ptr), value :: be, bo
+         integer(c_long_long), value :: seed
+         integer(c_long), value :: rhs
+         real(c_double), value :: scale
       end function
    end interface
 
@@ -403,6 +415,26 @@ contains
       s = src(1:4)
       call fail("QUARK", c_pointsource(c_loc(be), c_loc(bo), c_loc(s), icsrc, idsrc))
    end subroutine pointsource
+
+   ! z4source(be,bo,nvhalf,myid), qqcd-test/cfgsprops/quark/disconloops.f90:4359: Z4 noise
+   ! {1, -1, i, -i} on every (site, spin, colour) of both checkerboards; successive calls give
+   ! successive sources of the stream set by qqcd_z4_seed.
+   subroutine z4source(be,bo,nvhalf,myid)
+      real(kind=8), intent(out), target, contiguous :: be(:,:,:,:,:), bo(:,:,:,:,:)
+      integer, intent(in) :: nvhalf, myid
+      if (.not. started) call fail("DISCONLOOPS", -1_c_int)
+      if (myid /= 0 .or. size(be, 2) < nvhalf) call fail("DISCONLOOPS", -1_c_int)
+      call fail("DISCONLOOPS", c_z4source(c_loc(be), c_loc(bo), z4_seed, z4_next, 1.0d0))
+      z4_next = z4_next + 1
+   end subroutine z4source
+
+   ! restart the Z4 stream: the next z4source call gives source `first` of seed `seed`
+   subroutine qqcd_z4_seed(seed, first)
+      integer(kind=8), intent(in) :: seed
+      integer, intent(in) :: first
+      z4_seed = int(seed, c_long_long)
+      z4_next = int(first, c_long)
+   end subroutine qqcd_z4_seed
 
    subroutine fermprop(rwdir,be,bo,nkappa,kappa,cSW,inverter,coact,bc,resmax,itermin,omegaM3R,u,vectype, &
                        kGMRES,xe,xo,myid,nn,ldiv,nms,lvbc,lvcl,ib,lbd,iblv,vecbl,vecblinv,ntmqcd,hoption,MRT,MRT2)

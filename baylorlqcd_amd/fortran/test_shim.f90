@@ -82,6 +82,12 @@ program test_shim
    g = 0.0d0
    call qqcd_solve_mixed_f(be, g, [40, 16], 1.0d-10, 2, 12, 0, u, kappa, coact, bc, itercount)
    call writebin("x_mixed.bin", g, size(g))
+   ! two Z4 sources of seed 5: the second one is written out
+   call qqcd_z4_seed(5_8, 0)
+   call z4source(g, h, nvhalf, 0)
+   call z4source(g, h, nvhalf, 0)
+   call writebin("z4_e.bin", g, size(g))
+   call writebin("z4_o.bin", h, size(h))
    call qqcd_shutdown()
    print *, "test_shim done"
 

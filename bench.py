@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-noise", action="store_true", help="skip the 24^3x48 batched noise-source section")
     ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--noise-rhs", type=int, default=64)
+    ap.add_argument("--no-graph", action="store_true", help="batched solver without graph replays (profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="also time a 1 GiB plain copy kernel")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -279,7 +281,9 @@ def main():
         solve = solve_section(args.train_steps)
     noise = None
     if not args.no_noise:
-        noise = noise_section()
+        if args.no_graph:
+            disconloops.batch_set(use_graph=0)
+        noise = noise_section(args.noise_rhs)
 
     line = {
         "metric": "Dslash HBM GB/s (algorithmic bytes / kernel time), 32^3x64 Wilson hopping term",

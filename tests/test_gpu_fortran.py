@@ -4,7 +4,8 @@ Fortran host with the reference's argument lists, on the GPU.
 The host program (baylorlqcd_amd/fortran/test_shim.f90, built by
 __graft_entry__.build()) reads the config-1 links and a seeded spinor pair,
 calls Hsingle, Hdbletm, Hdouble(idag=1), gam5x, vecdot, pointsource +
-fermprop (GMRES-DR, then psc with isignal=2) and the mixed-precision solve;
+fermprop (GMRES-DR, then psc with isignal=2), the mixed-precision solve and
+z4source;
 every output is compared with the oracle or the reference's golden solution.
 """
 import os
@@ -78,3 +79,6 @@ def test_fortran_host(tmp_path):
     bb = O.spinor_ref_to_canon(N, be, None)
     r = bb - O.schur(N, U, BC, [1] * 4, [1] * 4, 0, (0.150, 0.0), xm)
     assert np.linalg.norm(r[~odd]) / np.linalg.norm(bb[~odd]) < 1e-10
+    # z4source: the second source of seed 5 (stream index 1), bit-exact
+    z = O.spinor_ref_to_canon(N, load("z4_e.bin"), load("z4_o.bin"))
+    assert np.array_equal(z, O.z4(N, 5, 1))
