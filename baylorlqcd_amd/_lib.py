@@ -62,6 +62,7 @@ SIGNATURES = {
     "qqcd_dense_lsq": [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp],
     "qqcd_dense_harmonic_ritz": [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp],
     "qqcd_z4source": [_dp, _dp, ctypes.c_ulonglong, ctypes.c_long, ctypes.c_double],
+    "qqcd_z2source": [_dp, _dp, ctypes.c_ulonglong, ctypes.c_long, ctypes.c_double],
     "qqcd_fermprop_multi": [ctypes.c_int, _dpp, _dpp, _dp, _dp, _ip, ctypes.c_double, _ip, _dp, _dpp, _dpp, _ip,
                             _dp],
     "qqcd_noise_loop": [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, _dp, _dp, _ip, _dp, _ip,

@@ -133,6 +133,19 @@ extern "C" int qqcd_z4source(double *be, double *bo, unsigned long long seed, lo
   API_END
 }
 
+extern "C" int qqcd_z2source(double *be, double *bo, unsigned long long seed, long rhs, double scale) {
+  API_BEGIN
+  Context &c = ctx();
+  SpinorBuf s(c.g, PREC_DOUBLE);
+  double *outs[2] = {be, bo};
+  for (int par = 0; par < 2; par++) {
+    if (!outs[par]) continue;
+    noise_source(c.g, s.s, par, seed, rhs, scale, 1, c.stream);
+    to_host(c, s.s, outs[par], par);
+  }
+  API_END
+}
+
 extern "C" int qqcd_hop_multi_canonical(int nrhs, int nact, int nb, int prec, int recon, int idag, int par_out,
                                         int order, const double *in, double *out) {
   API_BEGIN

@@ -10,7 +10,8 @@
 //   u = (splitmix64(seed + G (1 + (r V + s) 12 + k)) >> 11) 2^-53,
 // G = 0x9E3779B97F4A7C15, so a source does not depend on the layout or on
 // how many right-hand sides are generated; the same mapping to {1,-1,i,-i}
-// follows, times `scale`.
+// follows, times `scale`.  Z2 noise (z2source) maps the same uniform number
+// to 1 (u > 0.5) or -1 in the real part.
 //
 // trace16: per timeslice t and local Dirac matrix Gamma_n, the estimator
 //   T_n(t) = sum_{x in t} sum_{a,b,c} conj(eta_ac(x)) (Gamma_n)_ab x_bc(x)
@@ -38,8 +39,10 @@ __device__ __forceinline__ uint64_t nmix(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-__global__ void k_z4(int n0, int n1, int n2, int Vh, long V, int par, long stride, uint64_t seed, long rhs,
-                     double scale, double2 *out) {
+// kind 0: Z4 {1, -1, i, -i}; kind 1: Z2 {1, -1} (z2source, disconloops.f90:4583-4649: 1 for
+// u > 0.5, else -1, imaginary part zero)
+__global__ void k_noise(int n0, int n1, int n2, int Vh, long V, int par, long stride, uint64_t seed, long rhs,
+                        double scale, int kind, double2 *out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Vh) return;
   const int r2 = 2 * i;
@@ -54,7 +57,8 @@ __global__ void k_z4(int n0, int n1, int n2, int Vh, long V, int par, long strid
     const uint64_t c = (uint64_t)(((uint64_t)rhs * (uint64_t)V + (uint64_t)s) * 12ull + (uint64_t)k);
     const double u = (double)(nmix(seed + 0x9E3779B97F4A7C15ull * (1ull + c)) >> 11) * (1.0 / 9007199254740992.0);
     double re = 0, im = 0;
-    if (u > 0.75) re = 1;
+    if (kind == 1) re = u > 0.5 ? 1 : -1;
+    else if (u > 0.75) re = 1;
     else if (0.5 < u && u < 0.75) re = -1;
     else if (0.25 < u && u < 0.5) im = 1;
     else im = -1;
@@ -62,12 +66,18 @@ __global__ void k_z4(int n0, int n1, int n2, int Vh, long V, int par, long strid
   }
 }
 
-void z4_source(const Geom &g, Spinor &out, int par, uint64_t seed, long rhs, double scale, hipStream_t s) {
-  if (out.prec != PREC_DOUBLE) throw std::runtime_error("z4_source: FP64 spinor expected");
+void noise_source(const Geom &g, Spinor &out, int par, uint64_t seed, long rhs, double scale, int kind,
+                  hipStream_t s) {
+  if (out.prec != PREC_DOUBLE) throw std::runtime_error("noise_source: FP64 spinor expected");
+  if (kind != 0 && kind != 1) throw std::runtime_error("noise_source: kind 0 (Z4) or 1 (Z2)");
   const int nb = (int)((g.Vh + 255) / 256);
-  hipLaunchKernelGGL(k_z4, dim3(nb), dim3(256), 0, s, g.n[0], g.n[1], g.n[2], (int)g.Vh, g.V, par, out.stride,
-                     seed, rhs, scale, (double2 *)out.data);
+  hipLaunchKernelGGL(k_noise, dim3(nb), dim3(256), 0, s, g.n[0], g.n[1], g.n[2], (int)g.Vh, g.V, par, out.stride,
+                     seed, rhs, scale, kind, (double2 *)out.data);
   QQCD_HIP(hipGetLastError());
+}
+
+void z4_source(const Geom &g, Spinor &out, int par, uint64_t seed, long rhs, double scale, hipStream_t s) {
+  noise_source(g, out, par, seed, rhs, scale, 0, s);
 }
 
 static constexpr int TB = 256;

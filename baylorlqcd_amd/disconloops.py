@@ -24,6 +24,11 @@ def z4source(be, bo, seed, rhs=0, scale=1.0):
     _lib.check(_lib.lib().qqcd_z4source(_lib.dp(be), _lib.dp(bo), int(seed), int(rhs), float(scale)))
 
 
+def z2source(be, bo, seed, rhs=0, scale=1.0):
+    """Z2 noise {1,-1}*scale in the real part on both checkerboards."""
+    _lib.check(_lib.lib().qqcd_z2source(_lib.dp(be), _lib.dp(bo), int(seed), int(rhs), float(scale)))
+
+
 def trace16(eta_e, eta_o, x_e, x_o):
     """(nt, 16) complex: sum over each timeslice of eta^dag Gamma_n x."""
     L = _lib.lib()

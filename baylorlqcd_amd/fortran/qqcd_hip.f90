@@ -7,7 +7,7 @@
 !   bicgstab, gmresdr, psc, gmresproject, vecdot
 !                                         qqcd-test/cfgsprops/quark/inverters.f90
 !   pointsource, fermprop                 qqcd-test/cfgsprops/quark/quark.f90
-!   z4source                              qqcd-test/cfgsprops/quark/disconloops.f90
+!   z4source, z2source                    qqcd-test/cfgsprops/quark/disconloops.f90
 ! and forward to the HIP library through its C ABI (include/qqcd_hip.h).  A
 ! host program replaces `use diracops` / `use inverters` (and the two quark
 ! procedures) with `use qqcd_hip`; everything else in the host is unchanged.
@@ -32,7 +32,7 @@ module qqcd_hip
 
    public :: Hsingle, Hdouble, Hdbletm, gammamult, gam5x, vecdot, bicgstab, gmresdr, psc, &
              gmresproject, pointsource, fermprop, qqcd_gauge_changed, qqcd_shutdown, &
-             qqcd_solve_mixed_f, z4source, qqcd_z4_seed
+             qqcd_solve_mixed_f, z4source, z2source, qqcd_z4_seed
 
    logical, save :: started = .false.
    ! the Z4 stream: seed and the index of the next source (the reference's
@@ -133,6 +133,13 @@ module qqcd_hip
          integer(c_int), value :: basis, opp, recon, keep
       end function
       integer(c_int) function c_z4source(be, bo, seed, rhs, scale) bind(C, name="qqcd_z4source")
+         import :: c_int, c_ptr, c_double, c_long_long, c_long
+         type(c_ptr), value :: be, bo
+         integer(c_long_long), value :: seed
+         integer(c_long), value :: rhs
+         real(c_double), value :: scale
+      end function
+      integer(c_int) function c_z2source(be, bo, seed, rhs, scale) bind(C, name="qqcd_z2source")
          import :: c_int, c_ptr, c_double, c_long_long, c_long
          type(c_ptr), value :: be, bo
          integer(c_long_long), value :: seed
@@ -428,7 +435,18 @@ contains
       z4_next = z4_next + 1
    end subroutine z4source
 
-   ! restart the Z4 stream: the next z4source call gives source `first` of seed `seed`
+   ! z2source(be,bo,nvhalf), qqcd-test/cfgsprops/quark/disconloops.f90:4583: Z2 noise {1, -1}
+   ! in the real part, from the same stream as z4source
+   subroutine z2source(be,bo,nvhalf)
+      real(kind=8), intent(out), target, contiguous :: be(:,:,:,:,:), bo(:,:,:,:,:)
+      integer, intent(in) :: nvhalf
+      if (.not. started) call fail("DISCONLOOPS", -1_c_int)
+      if (size(be, 2) < nvhalf) call fail("DISCONLOOPS", -1_c_int)
+      call fail("DISCONLOOPS", c_z2source(c_loc(be), c_loc(bo), z4_seed, z4_next, 1.0d0))
+      z4_next = z4_next + 1
+   end subroutine z2source
+
+   ! restart the noise stream: the next z4source / z2source call gives source `first` of seed `seed`
    subroutine qqcd_z4_seed(seed, first)
       integer(kind=8), intent(in) :: seed
       integer, intent(in) :: first

@@ -88,6 +88,10 @@ program test_shim
    call z4source(g, h, nvhalf, 0)
    call writebin("z4_e.bin", g, size(g))
    call writebin("z4_o.bin", h, size(h))
+   ! the next source of the stream as Z2 noise (index 2)
+   call z2source(g, h, nvhalf)
+   call writebin("z2_e.bin", g, size(g))
+   call writebin("z2_o.bin", h, size(h))
    call qqcd_shutdown()
    print *, "test_shim done"
 

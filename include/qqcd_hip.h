@@ -180,6 +180,11 @@ int qqcd_solve_mixed(const double *phi, double *x, const int GMRES[2], double to
  * counter-based generator keyed by (seed, rhs, global site, component)
  * instead of the reference's lagged-Fibonacci stream rnds. */
 int qqcd_z4source(double *be, double *bo, unsigned long long seed, long rhs, double scale);
+/* Z2 noise {1, -1} * scale in the real part of every (site, spin, colour):
+ * replaces z2source (qqcd-test/cfgsprops/quark/disconloops.f90:4583-4649,
+ * 1 for a uniform number above 0.5, else -1), with the uniform numbers of
+ * qqcd_z4source's generator. */
+int qqcd_z2source(double *be, double *bo, unsigned long long seed, long rhs, double scale);
 
 /* Solve nrhs systems M x_r = b_r on one gauge field, as the reference's noise
  * loops call fermprop(inverter=6) once per source (testFUNC / discon,

@@ -237,6 +237,16 @@ def z4(n, seed, rhs=0, scale=1.0):
     return out
 
 
+def z2(n, seed, rhs=0, scale=1.0):
+    """Z2 noise (the library's counter-based z2source), canonical (V, 4, 3)."""
+    L = lib()
+    V = volume(n)
+    out = np.zeros((V, 4, 3), dtype=np.complex128)
+    L.orc_noise(_iptr(dims(n)), ctypes.c_uint64(int(seed)), ctypes.c_long(int(rhs)), ctypes.c_double(scale),
+                ctypes.c_int(1), _ptr(out.view(np.float64)))
+    return out
+
+
 GAMMA16_NAMES = ("1", "g1", "g2", "g3", "g4", "g5", "g1g5", "g2g5", "g3g5", "g4g5",
                  "s12", "s13", "s14", "s23", "s24", "s34")
 

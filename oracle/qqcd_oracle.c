@@ -514,7 +514,16 @@ static uint64_t orc_mix(uint64_t z)
     return z ^ (z >> 31);
 }
 
+/* kind 0: Z4 (as above); kind 1: Z2, 1 for u > 0.5 else -1 in the real part
+ * (z2source, qqcd-test/cfgsprops/quark/disconloops.f90:4583-4649) */
+void orc_noise(const int n[4], uint64_t seed, long rhs, double scale, int kind, double *psi_ri);
+
 void orc_z4(const int n[4], uint64_t seed, long rhs, double scale, double *psi_ri)
+{
+    orc_noise(n, seed, rhs, scale, 0, psi_ri);
+}
+
+void orc_noise(const int n[4], uint64_t seed, long rhs, double scale, int kind, double *psi_ri)
 {
     const long V = vol(n);
     for (long s = 0; s < V; s++)
@@ -523,7 +532,8 @@ void orc_z4(const int n[4], uint64_t seed, long rhs, double scale, double *psi_r
             const double u = (double)(orc_mix(seed + 0x9E3779B97F4A7C15ull * (1ull + c)) >> 11) *
                              (1.0 / 9007199254740992.0);
             double re = 0, im = 0;
-            if (u > 0.75) re = 1;
+            if (kind == 1) re = u > 0.5 ? 1 : -1;
+            else if (u > 0.75) re = 1;
             else if (0.5 < u && u < 0.75) re = -1;
             else if (0.25 < u && u < 0.5) im = 1;
             else im = -1;

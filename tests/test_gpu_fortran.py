@@ -82,3 +82,5 @@ def test_fortran_host(tmp_path):
     # z4source: the second source of seed 5 (stream index 1), bit-exact
     z = O.spinor_ref_to_canon(N, load("z4_e.bin"), load("z4_o.bin"))
     assert np.array_equal(z, O.z4(N, 5, 1))
+    z = O.spinor_ref_to_canon(N, load("z2_e.bin"), load("z2_o.bin"))
+    assert np.array_equal(z, O.z2(N, 5, 2))

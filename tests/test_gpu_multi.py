@@ -118,6 +118,14 @@ def test_z4source_bitexact(lat):
         assert np.array_equal(got, want), rhs
 
 
+def test_z2source_bitexact(lat):
+    n, bc, U, u, co = lat
+    be = lattice.new_spinor(n).ravel(order="F")
+    bo = lattice.new_spinor(n).ravel(order="F")
+    disconloops.z2source(be, bo, 77, 3, 1.0)
+    assert np.array_equal(O.spinor_ref_to_canon(n, be, bo), O.z2(n, 77, 3))
+
+
 def test_trace16(lat):
     n, bc, U, u, co = lat
     eta, x = spinors(n, 2, 17)

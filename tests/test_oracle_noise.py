@@ -61,6 +61,8 @@ def test_z4_threshold_mapping():
     want = np.where(u > 0.75, 1, np.where((u > 0.5) & (u < 0.75), -1,
                                            np.where((u > 0.25) & (u < 0.5), 1j, -1j)))
     assert np.array_equal(O.z4(N, seed, rhs).reshape(V, 12), want)
+    # Z2 (z2source): the same uniform numbers, 1 above 0.5, else -1, real
+    assert np.array_equal(O.z2(N, seed, rhs).reshape(V, 12), np.where(u > 0.5, 1.0, -1.0) + 0j)
 
 
 def test_gamma16_algebra():
