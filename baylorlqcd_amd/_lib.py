@@ -66,7 +66,10 @@ SIGNATURES = {
     "qqcd_fermprop_multi": [ctypes.c_int, _dpp, _dpp, _dp, _dp, _ip, ctypes.c_double, _ip, _dp, _dpp, _dpp, _ip,
                             _dp],
     "qqcd_noise_loop": [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, _dp, _dp, _ip, _dp, _ip,
-                        ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _dp, _ip],
+                        ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _ip, _dp,
+                        _ip],
+    "qqcd_loopops": [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp],
+    "qqcd_loopsum": [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, ctypes.c_double, _dp],
     "qqcd_trace16": [_dp, _dp, _dp, _dp, _dp],
     "qqcd_batch_set": [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                        ctypes.c_int],

@@ -11,6 +11,7 @@
 
 #include "../../include/qqcd_hip.h"
 #include "qqcd_batch.h"
+#include "qqcd_loops.h"
 #include "qqcd_noise.h"
 
 using namespace qqcd;
@@ -179,6 +180,77 @@ extern "C" int qqcd_trace16(const double *eta_e, const double *eta_o, const doub
   API_END
 }
 
+namespace {
+
+// upload the four reference-layout fields of a loop-operator call
+void loop_fields(Context &c, const double *xe, const double *xo, const double *u, const double *be, const double *bo,
+                 const int *bc, SpinorBuf *b, SpinorBuf *x) {
+  if (!xe || !xo || !be || !bo) throw std::runtime_error("loop operators: xe, xo, be and bo are required");
+  if (bc) set_bc(c, bc);
+  if (u) ensure_gauge(c, u, nullptr);
+  else if (!c.have_gauge) throw std::runtime_error("no gauge field loaded");
+  to_device(c, be, b[0].s, 0);
+  to_device(c, bo, b[1].s, 1);
+  to_device(c, xe, x[0].s, 0);
+  to_device(c, xo, x[1].s, 1);
+}
+
+}  // namespace
+
+extern "C" int qqcd_loopops(int ksub, const double *xe, const double *xo, const double *u, const double *be,
+                            const double *bo, const int *bc, double *Je, double *Jo) {
+  API_BEGIN
+  Context &c = ctx();
+  if (ksub != 0 && ksub != 1) throw std::runtime_error("loopops: ksub is 0 or 1");
+  SpinorBuf b[2] = {SpinorBuf(c.g, PREC_DOUBLE), SpinorBuf(c.g, PREC_DOUBLE)};
+  SpinorBuf x[2] = {SpinorBuf(c.g, PREC_DOUBLE), SpinorBuf(c.g, PREC_DOUBLE)};
+  SpinorBuf J[2] = {SpinorBuf(c.g, PREC_DOUBLE), SpinorBuf(c.g, PREC_DOUBLE)};
+  loop_fields(c, xe, xo, u, be, bo, bc, b, x);
+  DevArr part(nullptr, (size_t)2 * c.g.n[3] * loopsum_gx(c.g) * 58, c.stream);
+  const Spinor bs[2] = {b[0].s, b[1].s}, xs[2] = {x[0].s, x[1].s};
+  Spinor js[2] = {J[0].s, J[1].s};
+  loopsum(c.g, c.U64, bs, xs, ksub, nullptr, 1, 1.0, 1.0, nullptr, 0, part.p, js, c.stream);
+  // the per-site operators sit in components 0..8 (= 3 spin + colour) of a
+  // reference-layout spinor; Je(2,nvhalf,9,2,8) takes them slice by slice
+  const long ntot = ref_ntotal(c.g), nvh = c.g.Vh / 16;
+  std::vector<double> h((size_t)6 * ntot * 64);
+  double *outs[2] = {Je, Jo};
+  for (int par = 0; par < 2; par++) {
+    to_host(c, js[par], h.data(), par);
+    for (long sl = 0; sl < 16; sl++)
+      for (int op = 0; op < 9; op++)
+        for (long i = 0; i < nvh; i++)
+          for (int ri = 0; ri < 2; ri++)
+            outs[par][ri + 2 * (i + nvh * (op + 9 * sl))] =
+                h[(2 * (op % 3) + ri) + 6 * (i + ntot * ((op / 3) + 4 * sl))];
+  }
+  API_END
+}
+
+extern "C" int qqcd_loopsum(int ksub, const double *xe, const double *xo, const double *u, const double *be,
+                            const double *bo, const int *bc, const int *io, double cur_scale, double *Jtime) {
+  API_BEGIN
+  Context &c = ctx();
+  if (ksub != 0 && ksub != 1) throw std::runtime_error("loopsum: ksub is 0 or 1");
+  SpinorBuf b[2] = {SpinorBuf(c.g, PREC_DOUBLE), SpinorBuf(c.g, PREC_DOUBLE)};
+  SpinorBuf x[2] = {SpinorBuf(c.g, PREC_DOUBLE), SpinorBuf(c.g, PREC_DOUBLE)};
+  loop_fields(c, xe, xo, u, be, bo, bc, b, x);
+  const int nt = c.g.n[3];
+  DevArr part(nullptr, (size_t)2 * nt * loopsum_gx(c.g) * 58, c.stream);
+  DevArr out(nullptr, (size_t)nt * 90, c.stream);
+  const Spinor bs[2] = {b[0].s, b[1].s}, xs[2] = {x[0].s, x[1].s};
+  loopsum(c.g, c.U64, bs, xs, ksub, io, io ? 5 : 1, 1.0, cur_scale, out.p, 0, part.p, nullptr, c.stream);
+  std::vector<double> h((size_t)nt * 90);
+  QQCD_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  QQCD_HIP(hipStreamSynchronize(c.stream));
+  // Jtime(2, nt, 5, 9), Fortran order
+  for (int t = 0; t < nt; t++)
+    for (int m = 0; m < 5; m++)
+      for (int op = 0; op < 9; op++)
+        for (int ri = 0; ri < 2; ri++) Jtime[ri + 2 * (t + nt * (m + 5 * op))] = h[((t * 5 + m) * 9 + op) * 2 + ri];
+  API_END
+}
+
 extern "C" int qqcd_batch_set(int reorth, int use_graph, double inner_tol, int inner_cycles, int max_outer,
                               int op_prec, int op_recon) {
   API_BEGIN
@@ -266,8 +338,8 @@ extern "C" int qqcd_fermprop_multi(int nrhs, const double *const *be, const doub
 // replays, fallback, nb, order, first-solve outer steps}.
 extern "C" int qqcd_noise_loop(int nrhs, unsigned long long seed, double scale, const double kappa[2],
                                const double *coact, const int *bc, const double *u, const int GMRES[2], double tol,
-                               int nb, int method, double *traces, double *res_full, int *iters, double *times,
-                               int *stats) {
+                               int nb, int method, double *traces, int ksub, const int *io, double *loops,
+                               double *res_full, int *iters, double *times, int *stats) {
   API_BEGIN
   Context &c = ctx();
   if (nrhs < 1 || nrhs > 257) throw std::runtime_error("noise_loop: 1..257 right-hand sides");
@@ -285,6 +357,12 @@ extern "C" int qqcd_noise_loop(int nrhs, unsigned long long seed, double scale, 
   DevArr part(nullptr, (size_t)trace16_gx(c.g) * nt * 32, c.stream);
   SpinorBuf e_e(c.g, PREC_DOUBLE), e_o(c.g, PREC_DOUBLE), src(c.g, PREC_DOUBLE), x_e(c.g, PREC_DOUBLE),
       x_o(c.g, PREC_DOUBLE);
+  if (ksub != 0 && ksub != 1) throw std::runtime_error("noise_loop: ksub is 0 or 1");
+  std::unique_ptr<DevArr> lo, lpart;
+  if (loops) {
+    lo.reset(new DevArr(nullptr, (size_t)nt * 90, c.stream));
+    lpart.reset(new DevArr(nullptr, (size_t)2 * nt * loopsum_gx(c.g) * 58, c.stream));
+  }
   auto noise = [&](long r) {
     z4_source(c.g, e_e.s, 0, seed, r, scale, c.stream);
     z4_source(c.g, e_o.s, 1, seed, r, scale, c.stream);
@@ -295,6 +373,12 @@ extern "C" int qqcd_noise_loop(int nrhs, unsigned long long seed, double scale, 
     res_full[r] = full_residual(c, kappa, e_e.s, e_o.s, x_e.s, x_o.s);
     trace16(c.g, e_e.s, x_e.s, tr.p, !first, part.p, c.stream);
     trace16(c.g, e_o.s, x_o.s, tr.p, 1, part.p, c.stream);
+    if (loops) {
+      // the reference's operators of this source (nsaverage: currents times kappa)
+      const Spinor bs[2] = {e_e.s, e_o.s}, xs[2] = {x_e.s, x_o.s};
+      loopsum(c.g, c.U64, bs, xs, ksub, io, io ? 5 : 1, 1.0, kappa[0], lo->p, !first, lpart->p, nullptr,
+              c.stream);
+    }
   };
   Deflation D;
   std::memset(stats, 0, 8 * sizeof(int));
@@ -371,6 +455,11 @@ extern "C" int qqcd_noise_loop(int nrhs, unsigned long long seed, double scale, 
   QQCD_HIP(hipMemcpy(h.data(), tr.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
   const double f = 1.0 / (nrhs * scale * scale);
   for (size_t i = 0; i < h.size(); i++) traces[i] = f * h[i];
+  if (loops) {
+    std::vector<double> hl((size_t)nt * 90);
+    QQCD_HIP(hipMemcpy(hl.data(), lo->p, hl.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < hl.size(); i++) loops[i] = f * hl[i];
+  }
   times[0] = std::chrono::duration<double>(t1 - t0).count();
   times[1] = std::chrono::duration<double>(t2 - t1).count();
   times[2] = std::chrono::duration<double>(t2 - t0).count();

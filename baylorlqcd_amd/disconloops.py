@@ -1,9 +1,10 @@
 """Python mirror of the noise-source path of the reference's ``disconloops``
-module (qqcd-test/cfgsprops/quark/disconloops.f90): Z4 sources (z4source
-:4359), the loop over noise solves (testFUNC :1980-2569, discon
-:3445-3608), and local trace estimators (loopops :7769-7910, spacesum
-:9012-9124) -- on the GPU, with all right-hand sides after the first solved
-together by the batched GMRES-Proj.
+module (qqcd-test/cfgsprops/quark/disconloops.f90): Z4/Z2 sources (z4source
+:4359, z2source :4583), the loop over noise solves (testFUNC :1980-2569,
+discon :3445-3608), the reference's loop operators (loopops :7769-7910,
+spacesum :9012-9124, momfacs :8925-9010, nsaverage :3079-3143) and 16
+local trace estimators -- on the GPU, with all right-hand sides after the
+first solved together by the batched GMRES-Proj.
 
 Local Dirac matrices of the trace estimators, in order (the reference's
 gamma basis, qqcd-test/cfgsprops/quark/diracops.f90:61-71):
@@ -38,6 +39,37 @@ def trace16(eta_e, eta_o, x_e, x_o):
     return out.reshape(nt, 16, 2) @ np.array([1.0, 1j])
 
 
+LOOP_OPS = ("J1", "J2", "J3", "J4", "scalar", "V1", "V2", "V3", "V4")
+
+
+def loopops(ksub, xe, xo, be, bo, u=None, bc=None):
+    """The reference's loop operators (loopops, disconloops.f90:7769-7910) of
+    sink x and source b (reference-layout arrays) on every (ieo, ibleo) slice:
+    returns Je, Jo as flat Fortran (2, nvhalf, 9, 2, 8) arrays."""
+    from . import lattice
+    nvh = lattice.ref_sizes()[0]
+    Je = np.zeros(2 * nvh * 9 * 16)
+    Jo = np.zeros(2 * nvh * 9 * 16)
+    bcp = None if bc is None else _lib.ints(bc).ctypes.data_as(_lib._ip)
+    _lib.check(_lib.lib().qqcd_loopops(int(ksub), _lib.dp(xe), _lib.dp(xo), _lib.dp(u), _lib.dp(be), _lib.dp(bo),
+                                       bcp, _lib.dp(Je), _lib.dp(Jo)))
+    return Je, Jo
+
+
+def loopsum(ksub, xe, xo, be, bo, io=(1, 1, 1), cur_scale=1.0, u=None, bc=None):
+    """Timeslice and momentum sums of the loop operators (spacesum with the
+    factors of momfacs for origin io; io None: momentum 1 only), currents
+    times cur_scale: (nt, 5, 9) complex."""
+    nt = _nt()
+    out = np.zeros(2 * nt * 5 * 9)
+    bcp = None if bc is None else _lib.ints(bc).ctypes.data_as(_lib._ip)
+    iop = None if io is None else _lib.ints(io).ctypes.data_as(_lib._ip)
+    _lib.check(_lib.lib().qqcd_loopsum(int(ksub), _lib.dp(xe), _lib.dp(xo), _lib.dp(u), _lib.dp(be), _lib.dp(bo),
+                                       bcp, iop, float(cur_scale), _lib.dp(out)))
+    J = out.reshape(9, 5, nt, 2) @ np.array([1.0, 1j])
+    return J.transpose(2, 1, 0)
+
+
 def _nt():
     from . import lattice
     return lattice.dims()[3]
@@ -64,14 +96,18 @@ def fermprop_multi(be, bo, kappa, coact, bc, resmax, GMRES, u):
 
 
 def noise_loop(nrhs, seed, kappa, GMRES, tol=1e-10, scale=1.0, coact=None, bc=(1, 1, 1, -1), u=None, nb=0,
-               method=0):
+               method=0, loops=True, ksub=0, io=None):
     """The device noise loop.  Returns a dict with traces ((nt, 16) complex),
-    res_full, iters, times (first, rest, total) and stats.  method 0: batched;
-    1: sequential plain FP64 (GMRES-DR then GMRES-Proj per source); 2:
-    sequential single-RHS mixed precision."""
+    loops ((nt, 5, 9) complex: the reference's operators averaged over the
+    sources, currents times kappa; None when loops is False), res_full, iters,
+    times (first, rest, total) and stats.  method 0: batched; 1: sequential
+    plain FP64 (GMRES-DR then GMRES-Proj per source); 2: sequential
+    single-RHS mixed precision."""
     L = _lib.lib()
     nt = _nt()
     tr = np.zeros(nt * 32)
+    lo = np.zeros(nt * 90) if loops else None
+    iop = None if io is None else _lib.ints(io).ctypes.data_as(_lib._ip)
     res = np.zeros(nrhs)
     it = np.zeros(nrhs, np.int32)
     times = np.zeros(3)
@@ -80,11 +116,12 @@ def noise_loop(nrhs, seed, kappa, GMRES, tol=1e-10, scale=1.0, coact=None, bc=(1
     _lib.check(L.qqcd_noise_loop(
         int(nrhs), int(seed), float(scale), _lib.dp(np.ascontiguousarray(kappa, dtype=np.float64)), _lib.dp(co),
         _lib.ints(bc).ctypes.data_as(_lib._ip), _lib.dp(u), _lib.ints(GMRES).ctypes.data_as(_lib._ip), float(tol),
-        int(nb), int(method), _lib.dp(tr), _lib.dp(res), it.ctypes.data_as(_lib._ip), _lib.dp(times),
-        stats.ctypes.data_as(_lib._ip)))
+        int(nb), int(method), _lib.dp(tr), int(ksub), iop, _lib.dp(lo), _lib.dp(res), it.ctypes.data_as(_lib._ip),
+        _lib.dp(times), stats.ctypes.data_as(_lib._ip)))
     keys = ("outer", "cycles", "batched_matvecs", "graph_replays", "fallback", "nb", "order", "first_outer")
-    return {"traces": tr.reshape(nt, 16, 2) @ np.array([1.0, 1j]), "res_full": res, "iters": it,
-            "times": times, "stats": dict(zip(keys, (int(s) for s in stats)))}
+    return {"traces": tr.reshape(nt, 16, 2) @ np.array([1.0, 1j]),
+            "loops": None if lo is None else lo.reshape(nt, 5, 9, 2) @ np.array([1.0, 1j]),
+            "res_full": res, "iters": it, "times": times, "stats": dict(zip(keys, (int(s) for s in stats)))}
 
 
 def batch_set(reorth=0, use_graph=1, inner_tol=1e-5, inner_cycles=4, max_outer=30, op_prec=1, op_recon=12):

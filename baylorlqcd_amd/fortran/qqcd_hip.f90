@@ -7,7 +7,8 @@
 !   bicgstab, gmresdr, psc, gmresproject, vecdot
 !                                         qqcd-test/cfgsprops/quark/inverters.f90
 !   pointsource, fermprop                 qqcd-test/cfgsprops/quark/quark.f90
-!   z4source, z2source                    qqcd-test/cfgsprops/quark/disconloops.f90
+!   z4source, z2source, loopops, nsaverage
+!                                         qqcd-test/cfgsprops/quark/disconloops.f90
 ! and forward to the HIP library through its C ABI (include/qqcd_hip.h).  A
 ! host program replaces `use diracops` / `use inverters` (and the two quark
 ! procedures) with `use qqcd_hip`; everything else in the host is unchanged.
@@ -32,7 +33,7 @@ module qqcd_hip
 
    public :: Hsingle, Hdouble, Hdbletm, gammamult, gam5x, vecdot, bicgstab, gmresdr, psc, &
              gmresproject, pointsource, fermprop, qqcd_gauge_changed, qqcd_shutdown, &
-             qqcd_solve_mixed_f, z4source, z2source, qqcd_z4_seed
+             qqcd_solve_mixed_f, z4source, z2source, qqcd_z4_seed, loopops, loopops_all, nsaverage
 
    logical, save :: started = .false.
    ! the Z4 stream: seed and the index of the next source (the reference's
@@ -145,6 +146,17 @@ module qqcd_hip
          integer(c_long_long), value :: seed
          integer(c_long), value :: rhs
          real(c_double), value :: scale
+      end function
+      integer(c_int) function c_loopops(ksub, xe, xo, u, be, bo, bc, Je, Jo) bind(C, name="qqcd_loopops")
+         import :: c_int, c_ptr
+         integer(c_int), value :: ksub
+         type(c_ptr), value :: xe, xo, u, be, bo, bc, Je, Jo
+      end function
+      integer(c_int) function c_loopsum(ksub, xe, xo, u, be, bo, bc, io, cur, Jtime) bind(C, name="qqcd_loopsum")
+         import :: c_int, c_ptr, c_double
+         integer(c_int), value :: ksub
+         type(c_ptr), value :: xe, xo, u, be, bo, bc, io, Jtime
+         real(c_double), value :: cur
       end function
    end interface
 
@@ -453,6 +465,65 @@ contains
       z4_seed = int(seed, c_long_long)
       z4_next = int(first, c_long)
    end subroutine qqcd_z4_seed
+
+   ! loopops(ksub,xe,xo,u,be,bo,Je,Jo,ieo,ibleo,bc,...), disconloops.f90:7769: the nine loop
+   ! operators (point-split currents J_1..J_4, local scalar, local vector currents) of sink x and
+   ! source b on the sites of slice (ieo, ibleo): Je for the gblclr=1 sites, Jo for gblclr=2.
+   ! Every call computes all slices on the device; loopops_all hands them over in one call.
+   subroutine loopops(ksub,xe,xo,u,be,bo,Je,Jo,ieo,ibleo,bc,vecbl,vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd,iblv,MRT)
+      integer, intent(in) :: ksub, ieo, ibleo, myid, MRT
+      real(kind=8), intent(inout), target, contiguous :: xe(:,:,:,:,:), xo(:,:,:,:,:), be(:,:,:,:,:), bo(:,:,:,:,:)
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:)
+      real(kind=8), intent(out) :: Je(:,:,:), Jo(:,:,:)
+      integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      real(kind=8), allocatable :: ae(:,:,:,:,:), ao(:,:,:,:,:)
+      integer :: nvh
+      call check_geometry("DISCONLOOPS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      nvh = nx*ny*nz*nt/32
+      if (size(Je, 1) /= 2 .or. size(Je, 2) < nvh .or. size(Je, 3) < 9) call fail("DISCONLOOPS", -1_c_int)
+      allocate(ae(2,nvh,9,2,8), ao(2,nvh,9,2,8))
+      call loopops_all(ksub, xe, xo, u, be, bo, ae, ao, bc)
+      Je(:,1:nvh,1:9) = ae(:,:,:,ieo,ibleo)
+      Jo(:,1:nvh,1:9) = ao(:,:,:,ieo,ibleo)
+   end subroutine loopops
+
+   ! All (ieo, ibleo) slices of loopops at once: Je, Jo(2,nvhalf,9,2,8).
+   subroutine loopops_all(ksub, xe, xo, u, be, bo, Je, Jo, bc)
+      integer, intent(in) :: ksub
+      real(kind=8), intent(in), target, contiguous :: xe(:,:,:,:,:), xo(:,:,:,:,:), be(:,:,:,:,:), bo(:,:,:,:,:)
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:)
+      real(kind=8), intent(out), target, contiguous :: Je(:,:,:,:,:), Jo(:,:,:,:,:)
+      integer, intent(in) :: bc(:)
+      integer(c_int), target :: b(4)
+      call ensure_started(bc)
+      b = bc(1:4)
+      call fail("DISCONLOOPS", c_loopops(int(ksub, c_int), c_loc(xe), c_loc(xo), c_loc(u), c_loc(be), c_loc(bo), &
+                                         c_loc(b), c_loc(Je), c_loc(Jo)))
+   end subroutine loopops_all
+
+   ! nsaverage(Jtime,u,z2e,z2o,xe,xo,kappa,coact,bc,...,MRT2,ESHFES), disconloops.f90:3079:
+   ! loopops and spacesum over every slice, momentum factors zero, the currents times kappa;
+   ! Jtime(2,nt,5,9).  Only ESHFES = 1 (no eigenmode subtraction).
+   subroutine nsaverage(Jtime,u,z2e,z2o,xe,xo,kappa,coact,bc,vecbl,vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd, &
+                        iblv,MRT2,ESHFES)
+      real(kind=8), intent(out), target, contiguous :: Jtime(:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: u(:,:,:,:,:)
+      real(kind=8), intent(in) :: kappa
+      real(kind=8), intent(inout), target, contiguous :: z2e(:,:,:,:,:), z2o(:,:,:,:,:), xe(:,:,:,:,:), xo(:,:,:,:,:)
+      real(kind=8), intent(in) :: coact(:,:,:)
+      integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer, intent(in) :: myid, MRT2, ESHFES
+      integer(c_int), target :: b(4)
+      if (ESHFES /= 1) call fail("DISCONLOOPS", -1_c_int)
+      if (size(Jtime) /= 2*nt*5*9) call fail("DISCONLOOPS", -1_c_int)
+      call check_geometry("DISCONLOOPS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      call ensure_started(bc)
+      b = bc(1:4)
+      call fail("DISCONLOOPS", c_loopsum(0_c_int, c_loc(xe), c_loc(xo), c_loc(u), c_loc(z2e), c_loc(z2o), &
+                                         c_loc(b), c_null_ptr, kappa, c_loc(Jtime)))
+   end subroutine nsaverage
 
    subroutine fermprop(rwdir,be,bo,nkappa,kappa,cSW,inverter,coact,bc,resmax,itermin,omegaM3R,u,vectype, &
                        kGMRES,xe,xo,myid,nn,ldiv,nms,lvbc,lvcl,ib,lbd,iblv,vecbl,vecblinv,ntmqcd,hoption,MRT,MRT2)

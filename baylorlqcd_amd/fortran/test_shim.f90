@@ -14,6 +14,7 @@ program test_shim
    real(kind=8), allocatable :: u(:,:,:,:,:), be(:,:,:,:,:), bo(:,:,:,:,:), g(:,:,:,:,:), h(:,:,:,:,:)
    real(kind=8), allocatable :: xe(:,:,:,:,:,:), xo(:,:,:,:,:,:), ps_e(:,:,:,:,:), ps_o(:,:,:,:,:)
    real(kind=8) :: coact(4,4,4), kappa(2), adb(2), gdummy(18,1,1,1,1)
+   real(kind=8), allocatable :: jse(:,:,:), jso(:,:,:), jtime(:,:,:,:)
    integer :: bc(4), vecbl(2,8), vecblinv(2,16), nn(4,2), nms(4), iblv(16,4), ibl, mu, itercount, vectype
    integer, allocatable :: lvbc(:,:,:), ib(:,:,:,:)
    logical :: ldiv(4), lbd(16,4)
@@ -92,6 +93,15 @@ program test_shim
    call z2source(g, h, nvhalf)
    call writebin("z2_e.bin", g, size(g))
    call writebin("z2_o.bin", h, size(h))
+   ! loop operators of sink g/h (the Z2 source above) and source be/bo: slice (ieo, ibleo) = (2, 3)
+   ! of loopops with ksub = 1, and nsaverage at kappa 0.15
+   allocate(jse(2,nvhalf,9), jso(2,nvhalf,9), jtime(2,nt,5,9))
+   call loopops(1, g, h, u, be, bo, jse, jso, 2, 3, bc, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0)
+   call writebin("loopops_e.bin", jse, size(jse))
+   call writebin("loopops_o.bin", jso, size(jso))
+   call nsaverage(jtime, u, be, bo, g, h, 0.15d0, coact, bc, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, &
+                  iblv, 0, 1)
+   call writebin("nsaverage.bin", jtime, size(jtime))
    call qqcd_shutdown()
    print *, "test_shim done"
 

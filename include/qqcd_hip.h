@@ -209,8 +209,14 @@ int qqcd_fermprop_multi(int nrhs, const double *const *be, const double *const *
  * gamma_mu gamma_5; 10..15: sigma_munu = (i/2)[gamma_mu, gamma_nu] for
  * (1,2),(1,3),(1,4),(2,3),(2,4),(3,4)):
  *   traces[(t*16 + n)*2 + {re, im}] = (1/(nrhs scale^2)) sum_r sum_{x, x_4 = t} eta_r(x)^dag Gamma_n x_r(x),
- * an estimate of Tr(Gamma_n M^-1) on timeslice t.  u may be NULL to use the
- * resident gauge field.  nb = 0 lets the autotuner pick the RHS block.
+ * an estimate of Tr(Gamma_n M^-1) on timeslice t.  loops (optional, may be
+ * NULL): the reference's own nine loop operators of each source (as
+ * qqcd_loopsum with ksub, io) averaged the same way, currents times kappa[0]
+ * as nsaverage does (disconloops.f90:3079-3143):
+ *   loops[((t*5 + m)*9 + op)*2 + {re, im}], op = J_1..J_4, scalar, V_1..V_4,
+ * m = momentum 1..5 (io NULL: momentum 1 only, m > 0 left at zero).
+ * u may be NULL to use the resident gauge field.  nb = 0 lets the autotuner
+ * pick the RHS block.
  * res_full[r], iters[r]: per source; times = {first solve, batched solves,
  * total} seconds; stats[8] = {outer steps, GMRES cycles, batched operator
  * applications, graph replays, sources finished by the fallback, RHS block,
@@ -221,7 +227,31 @@ int qqcd_fermprop_multi(int nrhs, const double *const *be, const double *const *
  * reference counterpart as one call (the reference loops over fermprop). */
 int qqcd_noise_loop(int nrhs, unsigned long long seed, double scale, const double kappa[2], const double *coact,
                     const int *bc, const double *u, const int GMRES[2], double tol, int nb, int method,
-                    double *traces, double *res_full, int *iters, double *times, int *stats);
+                    double *traces, int ksub, const int *io, double *loops, double *res_full, int *iters,
+                    double *times, int *stats);
+
+/* The reference's loop operators of one (sink x, source b) pair, replacing
+ * loopops (qqcd-test/cfgsprops/quark/disconloops.f90:7769-7910) called over
+ * every (ieo, ibleo) slice: xe/xo the sink, be/bo the source (reference
+ * layout, both checkerboards), u the links (NULL: the resident field), bc the
+ * boundary conditions (NULL: as set).  Je/Jo(2, nvhalf, 9, 2, 8) receive,
+ * slice by slice, what loopops writes into Je/Jo(2, nvhalf, 9) for (ieo,
+ * ibleo): the point-split currents J_1..J_4
+ *   J_mu(x) = conj( f x(x+mu)^dag (1 - gamma_mu) U_mu(x)^dag b(x)
+ *                 - f x(x)^dag (1 + gamma_mu) U_mu(x) b(x+mu) ),
+ * the local scalar conj(x^dag b) (+12 when ksub = 1) and the local vector
+ * currents conj(x^dag gamma_mu b); f = bc(mu) on hops across the edge. */
+int qqcd_loopops(int ksub, const double *xe, const double *xo, const double *u, const double *be, const double *bo,
+                 const int *bc, double *Je, double *Jo);
+
+/* The loop operators summed per timeslice with the momentum factors of
+ * momfacs for the origin io(3) (1-based; NULL: momentum 1 only), replacing
+ * loopops + spacesum over all slices (disconloops.f90:8925-9124) without
+ * forming the per-site operators: Jtime(2, nt, 5, 9) in Fortran order, the
+ * four currents multiplied by cur_scale (1: spacesum; kappa: nsaverage,
+ * :3079-3143).  Arguments otherwise as qqcd_loopops. */
+int qqcd_loopsum(int ksub, const double *xe, const double *xo, const double *u, const double *be, const double *bo,
+                 const int *bc, const int *io, double cur_scale, double *Jtime);
 
 /* The 16 local trace estimators of one source: eta and x on both
  * checkerboards (reference layout); traces as in qqcd_noise_loop for

@@ -274,3 +274,124 @@ def trace16(n, eta, x):
     out = np.zeros((n[3], 16), dtype=np.complex128)
     np.add.at(out, t, per)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Loop operators of the disconnected-loop path
+
+
+def coords(n):
+    """(V, 4) integer coordinates (x, y, z, t) of the canonical sites."""
+    s = np.arange(volume(n))
+    return np.stack([s % n[0], (s // n[0]) % n[1], (s // (n[0] * n[1])) % n[2], s // (n[0] * n[1] * n[2])], 1)
+
+
+def forward(n, bc, mu):
+    """Canonical index of x + mu for every site x, and the factor of that hop:
+    bc[mu] where it crosses the lattice edge, else 1."""
+    c = coords(n)
+    c[:, mu] += 1
+    wrap = c[:, mu] == n[mu]
+    c[wrap, mu] = 0
+    idx = c[:, 0] + n[0] * (c[:, 1] + n[1] * (c[:, 2] + n[2] * c[:, 3]))
+    return idx, np.where(wrap, float(bc[mu]), 1.0)
+
+
+LOOP_OPS = ("J1", "J2", "J3", "J4", "scalar", "V1", "V2", "V3", "V4")
+
+
+def loopops(n, U, bc, b, x, ksub=0):
+    """(V, 9) complex: the reference's loop operators per site for source b and
+    sink x (canonical fields), qqcd-test/cfgsprops/quark/disconloops.f90:
+    7769-7910 with mulfor2 (:8374-8465), mulbac2 (:8574-8675), vv (:8788-8825)
+    and vgv (:8829-8921):
+      J_mu(x)   = conj( f x(x+mu)^dag (1 - g_mu) U_mu(x)^dag b(x)
+                      - f x(x)^dag (1 + g_mu) U_mu(x) b(x+mu) ),
+      scalar(x) = conj( x(x)^dag b(x) ) + 12 ksub,
+      V_mu(x)   = conj( x(x)^dag g_mu b(x) ),
+    f = bc[mu] on hops across the lattice edge."""
+    g = gamma_table()
+    one = np.eye(4)
+    J = np.zeros((volume(n), 9), dtype=np.complex128)
+    for mu in range(4):
+        nb, f = forward(n, bc, mu)
+        Um = U[:, mu]
+        udb = np.einsum("sji,saj->sai", np.conj(Um), b)  # U^dag b(x)
+        t1 = f * np.einsum("sac,ab,sbc->s", np.conj(x[nb]), one - g[mu], udb)
+        ub = np.einsum("sij,saj->sai", Um, b[nb]) * f[:, None, None]  # U b(x+mu)
+        t2 = -np.einsum("sac,ab,sbc->s", np.conj(x), one + g[mu], ub)
+        J[:, mu] = np.conj(t1 + t2)
+        J[:, 5 + mu] = np.conj(np.einsum("sac,ab,sbc->s", np.conj(x), g[mu], b))
+    J[:, 4] = np.conj(np.einsum("sac,sac->s", np.conj(x), b)) + 12.0 * ksub
+    return J
+
+
+# twoPi in momfacs is a default-real literal: single precision, then widened
+TWO_PI = float(np.float32(6.283185307179586))
+
+
+def momfacs(n, io=(1, 1, 1)):
+    """(V, 4, 5) momentum factors of momfacs (disconloops.f90:8925-9010) for
+    the origin io (1-based global coordinates); d_k = twoPi (x_k + 1 - io_k) / n_k."""
+    c = coords(n)
+    d = [TWO_PI * (c[:, k] + 1 - io[k]) / n[k] for k in range(3)]
+    s = [np.sin(v) for v in d]
+    co = [np.cos(v) for v in d]
+    s2 = [np.sin(2.0 * v) for v in d]
+    c2 = [np.cos(2.0 * v) for v in d]
+    F = np.zeros((volume(n), 4, 5))
+    F[:, 0, :4] = np.stack([s[0], s[1], s[2], (co[0] + co[1] + co[2]) / 3.0], 1)
+    F[:, 1, :4] = np.stack([s[0] * (co[1] + co[2]) / 2.0, s[1] * (co[0] + co[2]) / 2.0,
+                            s[2] * (co[0] + co[1]) / 2.0,
+                            (co[0] * co[1] + co[0] * co[2] + co[1] * co[2]) / 3.0], 1)
+    F[:, 2, :4] = np.stack([s[0] * co[1] * co[2], s[1] * co[0] * co[2], s[2] * co[0] * co[1],
+                            co[0] * co[1] * co[2]], 1)
+    F[:, 3, :4] = np.stack([s2[0], s2[1], s2[2], (c2[0] + c2[1] + c2[2]) / 3.0], 1)
+    F[:, :, 4] = F[:, :, 3]
+    return F
+
+
+def spacesum(n, J, io=(1, 1, 1), nmom=5):
+    """(nt, nmom, 9) complex timeslice sums of per-site operators J (V, 9),
+    spacesum (disconloops.f90:9012-9124): momentum 1 sums every operator;
+    momentum m = 2..nmom weights the currents by the factors of momfacs:
+      J_1 -> F3 Im J_1 + i F2 Im J_1,  J_2 -> F1 Im J_2 + i F3 Im J_2,
+      J_3 -> F2 Im J_3 + i F1 Im J_3,  J_4 -> F4 J_4,  scalar -> F5 scalar,
+    the local vector currents only at momentum 1."""
+    t = coords(n)[:, 3]
+    out = np.zeros((n[3], nmom, 9), dtype=np.complex128)
+    np.add.at(out[:, 0, :], t, J)
+    if nmom > 1:
+        F = momfacs(n, io)
+        for m in range(1, nmom):
+            f = F[:, m - 1, :]
+            per = np.zeros((volume(n), 5), dtype=np.complex128)
+            per[:, 0] = f[:, 2] * J[:, 0].imag + 1j * f[:, 1] * J[:, 0].imag
+            per[:, 1] = f[:, 0] * J[:, 1].imag + 1j * f[:, 2] * J[:, 1].imag
+            per[:, 2] = f[:, 1] * J[:, 2].imag + 1j * f[:, 0] * J[:, 2].imag
+            per[:, 3] = f[:, 3] * J[:, 3]
+            per[:, 4] = f[:, 4] * J[:, 4]
+            np.add.at(out[:, m, :5], t, per)
+    return out
+
+
+def loops_ref_to_canon(n, Je, Jo):
+    """Per-site operators (V, 9) complex from the reference's flat Fortran
+    Je/Jo(2, nvhalf, 9, 2, 8) over every (ieo, ibleo) slice (even sites from
+    Je, odd from Jo): each is placed in a reference-layout spinor at
+    component 3 spin + colour = op and read back through the layout map."""
+    nvh, ntot = ref_sizes(n)
+    out = np.zeros((volume(n), 9), dtype=np.complex128)
+    par = parity_mask(n)
+    for J, odd in ((Je, False), (Jo, True)):
+        J5 = np.asarray(J).reshape((2, nvh, 9, 2, 8), order="F")
+        A = np.zeros((6, ntot, 4, 2, 8), order="F")
+        for op in range(9):
+            sp, col = divmod(op, 3)
+            A[2 * col, :nvh, sp] = J5[0, :, op]
+            A[2 * col + 1, :nvh, sp] = J5[1, :, op]
+        flat = A.ravel(order="F")
+        psi = spinor_ref_to_canon(n, None, flat) if odd else spinor_ref_to_canon(n, flat, None)
+        sel = par if odd else ~par
+        out[sel] = psi[sel].reshape(-1, 12)[:, :9]
+    return out

@@ -25,6 +25,7 @@ module refdrv
    use gaugetools
    use heatbath
    use lagfib
+   use disconloops, only: loopops, momfacs, spacesum, nsaverage
 #endif
    implicit none
    include 'mpif.h'
@@ -239,6 +240,70 @@ contains
       call aveplaq(p, u, myid, nn, ldiv, nms, lv, ib, lbd, iblv, MRT)
       plaq = p
    end subroutine ref_aveplaq
+
+   ! The loop operators of one (sink x, source b) pair on every (ieo, ibleo)
+   ! slice: Je/Jo(:,:,:,ieo,ibleo) is loopops' output for that slice
+   ! (qqcd-test/cfgsprops/quark/disconloops.f90:7769-7910).
+   subroutine ref_loopops(ksub, xe, xo, u, be, bo, Je, Jo) bind(C, name="ref_loopops")
+      integer(c_int), value         :: ksub
+      real(c_double), intent(inout) :: xe(6,ntotal,4,2,8), xo(6,ntotal,4,2,8)
+      real(c_double), intent(inout) :: be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8)
+      real(c_double), intent(in)    :: u(18,ntotal,4,2,16)
+      real(c_double), intent(out)   :: Je(2,nvhalf,9,2,8), Jo(2,nvhalf,9,2,8)
+      real(kind=KR) :: je1(2,nvhalf,9), jo1(2,nvhalf,9)
+      integer(kind=KI) :: ieo, ibleo, ks, MRT
+      MRT = MPI_DOUBLE_PRECISION
+      ks = ksub
+      do ibleo = 1, 8
+         do ieo = 1, 2
+            call loopops(ks, xe, xo, u, be, bo, je1, jo1, ieo, ibleo, gbc, vecbl, vecblinv, &
+                         myid, nn, ldiv, nms, lvbc, ib, lbd, iblv, MRT)
+            Je(:,:,:,ieo,ibleo) = je1
+            Jo(:,:,:,ieo,ibleo) = jo1
+         end do
+      end do
+   end subroutine ref_loopops
+
+   ! Timeslice and momentum sums of loop operators: the momentum factors of
+   ! momfacs for origin io(3) (disconloops.f90:8925-9010), then spacesum
+   ! (:9012-9124) over every slice into Jtime(2,nt,5,9).
+   subroutine ref_spacesum(Je, Jo, io, Jtime) bind(C, name="ref_spacesum")
+      real(c_double), intent(in)  :: Je(2,nvhalf,9,2,8), Jo(2,nvhalf,9,2,8)
+      integer(c_int), intent(in)  :: io(3)
+      real(c_double), intent(out) :: Jtime(2,nt,5,9)
+      real(kind=KR), allocatable :: momfac(:,:,:,:,:)
+      real(kind=KR) :: je1(2,nvhalf,9), jo1(2,nvhalf,9)
+      integer(kind=KI) :: ieo, ibleo, MRT, io3(3)
+      MRT = MPI_DOUBLE_PRECISION
+      allocate(momfac(nvhalf,2,16,4,9))
+      momfac = 0.0_KR
+      io3 = io
+      call momfacs(io3, momfac, myid, iblv)
+      Jtime = 0.0_KR
+      do ibleo = 1, 8
+         do ieo = 1, 2
+            je1 = Je(:,:,:,ieo,ibleo)
+            jo1 = Jo(:,:,:,ieo,ibleo)
+            call spacesum(Jtime, je1, jo1, ieo, ibleo, momfac, 5, 9, myid, vecbl, vecblinv, MRT)
+         end do
+      end do
+      deallocate(momfac)
+   end subroutine ref_spacesum
+
+   ! nsaverage (disconloops.f90:3079-3143) with ESHFES = 1: loopops and
+   ! spacesum over every slice with zero momentum factors, currents times kappa.
+   subroutine ref_nsaverage(Jtime, u, be, bo, xe, xo, kappa, coact) bind(C, name="ref_nsaverage")
+      real(c_double), intent(out)   :: Jtime(2,nt,5,9)
+      real(c_double), intent(inout) :: u(18,ntotal,4,2,16)
+      real(c_double), intent(inout) :: be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8)
+      real(c_double), intent(inout) :: xe(6,ntotal,4,2,8), xo(6,ntotal,4,2,8)
+      real(c_double), value         :: kappa
+      real(c_double), intent(in)    :: coact(4,4,4)
+      integer(kind=KI) :: MRT
+      MRT = MPI_DOUBLE_PRECISION
+      call nsaverage(Jtime, u, be, bo, xe, xo, kappa, coact, gbc, vecbl, vecblinv, myid, nn, &
+                     ldiv, nms, lvbc, ib, lbd, iblv, MRT, 1)
+   end subroutine ref_nsaverage
 #endif
 
    ! coact for the isotropic, untadpoled plaquette + Wilson action

@@ -203,6 +203,30 @@ class RefLattice:
         _run(self.L.ref_aveplaq, _p(uu), ctypes.byref(p))
         return p.value
 
+    def loopops(self, ksub, xe, xo, u, be, bo):
+        """Je, Jo as (2, 8, 9, nvhalf, 2) C-order views of the Fortran
+        (2, nvhalf, 9, 2, 8) arrays."""
+        nvh = self.nvhalf
+        Je = np.zeros(2 * nvh * 9 * 16)
+        Jo = np.zeros(2 * nvh * 9 * 16)
+        args = [np.array(a, dtype=np.float64, copy=True) for a in (xe, xo, u, be, bo)]
+        _run(self.L.ref_loopops, int(ksub), *[_p(a) for a in args], _p(Je), _p(Jo))
+        return Je, Jo
+
+    def spacesum(self, Je, Jo, io=(1, 1, 1)):
+        """Jtime flat Fortran (2, nt, 5, 9)."""
+        out = np.zeros(2 * self.n[3] * 5 * 9)
+        iov = np.ascontiguousarray(io, dtype=np.int32)
+        _run(self.L.ref_spacesum, _p(np.ascontiguousarray(Je)), _p(np.ascontiguousarray(Jo)), _ip_(iov), _p(out))
+        return out
+
+    def nsaverage(self, u, be, bo, xe, xo, kappa, coact):
+        out = np.zeros(2 * self.n[3] * 5 * 9)
+        args = [np.array(a, dtype=np.float64, copy=True) for a in (u, be, bo, xe, xo)]
+        _run(self.L.ref_nsaverage, _p(out), *[_p(a) for a in args], ctypes.c_double(kappa),
+             _p(np.ascontiguousarray(coact)))
+        return out
+
     def fixaction(self, beta):
         co = np.zeros(64)
         _run(self.L.ref_fixaction, ctypes.c_double(beta), _p(co))

@@ -2,7 +2,9 @@
 
 Runs only where /root/reference is present (this script builds it with
 oracle/ref/Makefile, then calls it through oracle/refdrv.py).  Output:
-tests/golden/golden_4x4x4x4.npz and tests/golden/golden_8x8x8x16.npz.
+tests/golden/golden_4x4x4x4.npz and tests/golden/golden_8x8x8x16.npz;
+with the argument ``loops`` only tests/golden/golden_loops_8x8x8x16.npz
+(the loop operators loopops / spacesum / nsaverage of the noise loop).
 
 Inputs are regenerated from seeds by the tests (spinors: numpy
 default_rng(seed) Gaussians; links: baylorlqcd_amd.gauge.haar), except the
@@ -149,5 +151,53 @@ def main():
     print("wrote golden_8x8x8x16.npz", len(out), "arrays; plaquette", float(out["plaquette_ref"]))
 
 
+LOOP_CASES = {
+    # name: (bc, ksub, momentum origin io)
+    "a": ((1, 1, 1, -1), 0, (1, 1, 1)),
+    "b": ((-1, 1, 0, 1), 1, (3, 2, 5)),
+}
+
+
+def loop_sites(n):
+    """Canonical sites whose per-site operators are stored: timeslices 0 and
+    nt-1 (the time hop of the last one crosses the lattice edge)."""
+    ls = n[0] * n[1] * n[2]
+    return np.concatenate([np.arange(ls), np.arange((n[3] - 1) * ls, n[3] * ls)])
+
+
+def loops():
+    n = (8, 8, 8, 16)
+    refdrv.build(n, solvers=True)
+    U = gauge.haar(n, 3)
+    u = O.links_canon_to_ref(n, U)
+    b, x = spinor(n, 301), spinor(n, 302)
+    be, bo = O.spinor_canon_to_ref(n, b)
+    xe, xo = O.spinor_canon_to_ref(n, x)
+    c = O.coords(n)
+    sites = loop_sites(n)
+    out = {}
+    for name, (bc, ksub, io) in LOOP_CASES.items():
+        R = refdrv.RefLattice(n, bc=bc)
+        Je, Jo = R.loopops(ksub, xe, xo, u, be, bo)
+        nvh = R.nvhalf
+        Je4, Jo4 = Je.reshape(8, 2, 9, nvh, 2), Jo.reshape(8, 2, 9, nvh, 2)
+        J = np.zeros((len(sites), 9), dtype=np.complex128)
+        for k, s in enumerate(sites):
+            i, ieo, _, ibleo, gc = O.ref_site(n, c[s])
+            A = Je4 if gc == 1 else Jo4
+            J[k] = A[ibleo - 1, ieo - 1, :, i - 1, 0] + 1j * A[ibleo - 1, ieo - 1, :, i - 1, 1]
+        out[f"loopops_{name}"] = J
+        Jt = R.spacesum(Je, Jo, io).reshape(9, 5, n[3], 2)
+        out[f"spacesum_{name}"] = (Jt[..., 0] + 1j * Jt[..., 1]).transpose(2, 1, 0)
+    R = refdrv.RefLattice(n, bc=LOOP_CASES["a"][0])
+    Jn = R.nsaverage(u, be, bo, xe, xo, 0.15, O.coact_array()).reshape(9, 5, n[3], 2)
+    out["nsaverage_a_k150"] = (Jn[..., 0] + 1j * Jn[..., 1]).transpose(2, 1, 0)[:, 0, :]
+    np.savez_compressed(os.path.join(HERE, "golden_loops_8x8x8x16.npz"), **out)
+    print("wrote golden_loops_8x8x8x16.npz", len(out), "arrays")
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["loops"]:
+        loops()
+    else:
+        main()

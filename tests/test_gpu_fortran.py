@@ -5,7 +5,7 @@ The host program (baylorlqcd_amd/fortran/test_shim.f90, built by
 __graft_entry__.build()) reads the config-1 links and a seeded spinor pair,
 calls Hsingle, Hdbletm, Hdouble(idag=1), gam5x, vecdot, pointsource +
 fermprop (GMRES-DR, then psc with isignal=2), the mixed-precision solve and
-z4source;
+z4source, z2source, loopops (one slice) and nsaverage;
 every output is compared with the oracle or the reference's golden solution.
 """
 import os
@@ -84,3 +84,22 @@ def test_fortran_host(tmp_path):
     assert np.array_equal(z, O.z4(N, 5, 1))
     z = O.spinor_ref_to_canon(N, load("z2_e.bin"), load("z2_o.bin"))
     assert np.array_equal(z, O.z2(N, 5, 2))
+    # loop operators (sink: that Z2 source, source: psi): loopops slice (ieo, ibleo) = (2, 3)
+    # with ksub = 1, and nsaverage at kappa 0.15
+    nvh = V // 32
+    Jw = O.loopops(N, U, BC, psi, z, 1)
+    full_e, full_o = np.zeros((2, nvh, 9, 2, 8), order="F"), np.zeros((2, nvh, 9, 2, 8), order="F")
+    mask_e, mask_o = np.zeros_like(full_e), np.zeros_like(full_o)
+    full_e[:, :, :, 1, 2] = load("loopops_e.bin").reshape((2, nvh, 9), order="F")
+    full_o[:, :, :, 1, 2] = load("loopops_o.bin").reshape((2, nvh, 9), order="F")
+    mask_e[:, :, :, 1, 2] = 1.0
+    mask_o[:, :, :, 1, 2] = 1.0
+    got = O.loops_ref_to_canon(N, full_e.ravel(order="F"), full_o.ravel(order="F"))
+    sel = O.loops_ref_to_canon(N, mask_e.ravel(order="F"), mask_o.ravel(order="F"))[:, 0].real > 0
+    assert sel.sum() == 2 * nvh
+    assert rel(got[sel], Jw[sel]) < 1e-12
+    Jt = load("nsaverage.bin").reshape((2, N[3], 5, 9), order="F")
+    Jt = Jt[0] + 1j * Jt[1]
+    wt = O.spacesum(N, O.loopops(N, U, BC, psi, z, 0), nmom=1)[:, 0, :]
+    wt[:, :4] *= 0.15
+    assert rel(Jt[:, 0, :], wt) < 1e-12 and np.all(Jt[:, 1:, :] == 0)

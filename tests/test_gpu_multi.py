@@ -4,6 +4,8 @@
   block and tile order, partial activity) against the C oracle per RHS;
 * Z4 sources bit-exact against the oracle's restatement;
 * the 16 local trace estimators against the oracle;
+* the reference's loop operators (qqcd_loopops per site, qqcd_loopsum per
+  timeslice and momentum) against the oracle, bar 1e-12 relative;
 * batched solves (qqcd_fermprop_multi, qqcd_noise_loop): every source
   reaches a true FP64 residual below 1e-10, the point source among them
   matches the reference Fortran's solution (golden fixture) within 1e-8,
@@ -136,6 +138,27 @@ def test_trace16(lat):
     assert rel(got, want) < 1e-13
 
 
+@pytest.mark.parametrize("ksub", [0, 1])
+def test_loopops_and_loopsum(lat, ksub):
+    n, bc, U, u, co = lat
+    b, x = spinors(n, 2, 23 + ksub)
+    be, bo = O.spinor_canon_to_ref(n, b)
+    xe, xo = O.spinor_canon_to_ref(n, x)
+    print(f"[multi] loopops ksub={ksub}", flush=True)
+    Je, Jo = disconloops.loopops(ksub, xe, xo, be, bo)
+    want = O.loopops(n, U, bc, b, x, ksub)
+    assert rel(O.loops_ref_to_canon(n, Je, Jo), want) < 1e-12
+    for io, cur in (((1, 1, 1), 1.0), ((2, 1, 3), 0.5), (None, 0.15)):
+        print(f"[multi] loopsum io={io}", flush=True)
+        got = disconloops.loopsum(ksub, xe, xo, be, bo, io=io, cur_scale=cur)
+        w = O.spacesum(n, want, io if io else (1, 1, 1), nmom=5 if io else 1)
+        w[:, :, :4] *= cur
+        if io is None:
+            assert np.all(got[:, 1:] == 0)
+            got = got[:, :1]
+        assert rel(got, w) < 1e-12
+
+
 @pytest.fixture(scope="module")
 def cfg1():
     n, bc = (8, 8, 8, 16), (1, 1, 1, -1)
@@ -212,6 +235,7 @@ def test_noise_loop_traces_and_graph(cfg1):
     # graph replays run the same kernels with the same arguments as the eager launches
     assert np.array_equal(a["iters"], b["iters"])
     assert rel(a["traces"], b["traces"]) < 1e-12
+    assert rel(a["loops"], b["loops"]) < 1e-12
     assert a["stats"]["graph_replays"] > 0 and b["stats"]["graph_replays"] == 0
     # the traces are those of the solutions of the same sources
     srcs_e, srcs_o = [], []
@@ -223,9 +247,15 @@ def test_noise_loop_traces_and_graph(cfg1):
         srcs_o.append(bo)
     xe, xo, it, rr = disconloops.fermprop_multi(srcs_e, srcs_o, kappa, co, bc, 1e-10, (40, 10), u)
     want = np.zeros((n[3], 16), dtype=np.complex128)
+    wloop = np.zeros((n[3], 9), dtype=np.complex128)
     for r in range(nrhs):
         eta = O.spinor_ref_to_canon(n, srcs_e[r], srcs_o[r])
         x = O.spinor_ref_to_canon(n, xe[r], xo[r])
         want += O.trace16(n, eta, x) / nrhs
+        wloop += O.spacesum(n, O.loopops(n, U, bc, eta, x, 0), nmom=1)[:, 0, :] / nrhs
+    wloop[:, :4] *= kappa[0]
     # both sets of solutions meet 1e-10 relative residual; traces agree to that order
     assert rel(a["traces"], want) < 1e-8
+    # the reference's operators as nsaverage forms them (currents times kappa)
+    assert rel(a["loops"][:, 0, :], wloop) < 1e-8
+    assert np.all(a["loops"][:, 1:] == 0)
