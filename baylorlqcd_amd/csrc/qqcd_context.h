@@ -24,6 +24,7 @@ struct Context {
   const void *u_host = nullptr;
   double coact[64] = {0};
   int have_gauge = 0;
+  int hb_sweep = 0;  // sweeps of the current heatbath chain (keys its random stream)
   // lower-precision / compressed copies, keyed by prec*100+recon
   struct Copy {
     Gauge U;

@@ -299,9 +299,11 @@ void heatbath(Context &c, double beta, int nsweep, uint64_t seed, int hot_start)
   a.beta = beta;
   a.seed = seed;
   const int nb = (int)((g.Vh + 127) / 128);
-  static int sweep_counter = 0;
+  // the sweep index keys the random stream: a hot or cold start begins a new
+  // chain at sweep 0, a continuation carries on from the chain's last sweep
+  if (hot_start >= 0) c.hb_sweep = 0;
   for (int sw = 0; sw < nsweep; sw++) {
-    a.sweep = sweep_counter++;
+    a.sweep = c.hb_sweep++;
     for (int par = 0; par < 2; par++)
       for (int mu = 0; mu < 4; mu++) {
         a.par = par;

@@ -57,9 +57,13 @@ struct NNLinks {
 NNLinks nn_links(const Geom &g, const Gauge &U32, hipStream_t s);
 void nn_free_links(NNLinks &L);
 
-// out = layer(in) (adjoint = 0) or layer^dag(in) (adjoint = 1)
+// out = layer(in) (adjoint = 0) or layer^dag(in) (adjoint = 1).  sparse:
+// NN_SRC_EVEN promises that `in` is zero on the odd sites (their transports
+// are skipped); NN_OUT_EVEN asks for the even sites of `out` only (the odd
+// ones are left undefined; honoured by the 8 -> 1 layer).
+constexpr int NN_SRC_EVEN = 1, NN_OUT_EVEN = 2;
 void nn_layer(const Geom &g, const NNLinks &U, const NNLayerDev &D, int adjoint, const NNField &in, NNField &out,
-              hipStream_t s);
+              hipStream_t s, int sparse = 0);
 // Weight gradient dL/dconj(W) of one layer, summed over the lattice:
 // dW[a][b][p][s][s'] = sum_x,c g_a(x)[s,c] conj((T_p psi_b)(x)[s',c]); written
 // as complex doubles in the weight order into host array dW (cout*cin*9*16).

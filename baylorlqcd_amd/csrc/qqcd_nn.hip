@@ -65,6 +65,7 @@ struct NNGeo {
   long stride;   // network-field site stride per parity
   long lstride;  // link component stride
   float bcf[4];
+  int src_even;  // the input field is zero on the odd sites: transports from them are skipped
 };
 
 template <int CIN> struct ChunkMap {
@@ -124,9 +125,10 @@ __device__ __forceinline__ void transport_load(const NNGeo &g, const uint32_t *i
                                                int valid, int par, int cb, const int x[4], int pe, int b, TRaw &r) {
   // every load is issued on every path (an unused path reads valid
   // addresses and is masked), so the count of loads in flight is static
-  const bool ok = valid && pe < NN_PATHS;
-  const int pe2 = ok ? pe : 0;
+  const bool ok0 = valid && pe < NN_PATHS;
+  const int pe2 = ok0 ? pe : 0;
   const Hop h = hop_of(g, par, cb, x, pe2);
+  const bool ok = ok0 && !(g.src_even && h.npar == 1);
   const uint4 *src = (const uint4 *)(in + ((long)(h.npar * g.stride + h.ncb) * C + b) * 12);
   const uint4 w0 = src[0], w1 = src[1], w2 = src[2];
   const uint32_t q[12] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
@@ -549,6 +551,7 @@ struct MixArgs {
   uint32_t *phi;       // 9 path fields
   const uint16_t *B;   // [96][64] bf16
   NNGeo g;
+  int even_out;        // only the even sites' output is wanted: phi_0 on even, phi_1..8 on odd sites
 };
 
 __global__ void __launch_bounds__(64 * NNW) k_ptc_mix(const MixArgs a) {
@@ -623,8 +626,8 @@ __global__ void __launch_bounds__(64 * NNW) k_ptc_mix(const MixArgs a) {
           const float re = acc[c][nb][i];
           const float im = __shfl_xor(re, 1, 64);
           const int n = nb * 32 + (lane & 31), jj = n >> 1;
-          if (!(lane & 1) && jj < 4 * NN_PATHS && v2) {
-            const int pth = jj >> 2, sp = jj & 3;
+          const int pth = jj >> 2, sp = jj & 3;
+          if (!(lane & 1) && jj < 4 * NN_PATHS && v2 && (!a.even_out || (p2 == 0) == (pth == 0))) {
             a.phi[((long)(p2 * g.stride + cb2) * NN_PATHS + pth) * 12 + 3 * sp + c] = pack_bf(re, im);
           }
         }
@@ -633,9 +636,11 @@ __global__ void __launch_bounds__(64 * NNW) k_ptc_mix(const MixArgs a) {
 }
 
 // out(x) = sum_p T_pe phi_p(x), pe = p (or its reverse for the adjoint layer)
+// sites q < nsite (2 Vh: all; Vh: the even checkerboard only)
 __global__ void __launch_bounds__(256) k_ptc_gather(const uint32_t *phi, uint32_t *out, const uint32_t *links,
-                                                    NNGeo g, int adjoint) {
+                                                    NNGeo g, int adjoint, long nsite) {
   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= nsite) return;
   int valid, par, cb, x[4];
   site_of(g, q, valid, par, cb, x);
   if (!valid) return;
@@ -1017,12 +1022,14 @@ void nn_free_links(NNLinks &L) {
 }
 
 void nn_layer(const Geom &g, const NNLinks &U, const NNLayerDev &D, int adjoint, const NNField &in, NNField &out,
-              hipStream_t s) {
+              hipStream_t s, int sparse) {
   const int ci = adjoint ? D.cout : D.cin, co = adjoint ? D.cin : D.cout;
   if (in.C != ci || out.C != co) throw std::runtime_error("nn_layer: channel counts do not match the layer");
   if (in.data == out.data) throw std::runtime_error("nn_layer: in and out alias");
   NNGeo geo = nngeo(g, nullptr);
   geo.lstride = U.stride;
+  geo.src_even = (sparse & NN_SRC_EVEN) ? 1 : 0;
+  const bool even_out = (sparse & NN_OUT_EVEN) != 0;
   if (ci == 8 && co == 1) {
     // mix, then gather: the nine path fields live in a scratch field kept between calls
     static NNField phi;
@@ -1036,11 +1043,13 @@ void nn_layer(const Geom &g, const NNLinks &U, const NNLayerDev &D, int adjoint,
     m.phi = phi.data;
     m.B = D.Bmix;
     m.g = geo;
+    m.g.src_even = 0;  // the mix reads its own site only; zero odd input gives zero there
+    m.even_out = even_out;
     hipLaunchKernelGGL(k_ptc_mix, dim3(nn_grid()), dim3(64 * NNW), 0, s, m);
     QQCD_HIP(hipGetLastError());
-    const long n = 2 * g.Vh;
+    const long n = even_out ? g.Vh : 2 * g.Vh;
     hipLaunchKernelGGL(k_ptc_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const uint32_t *)phi.data,
-                       out.data, (const uint32_t *)U.data, geo, adjoint);
+                       out.data, (const uint32_t *)U.data, geo, adjoint, n);
     QQCD_HIP(hipGetLastError());
     return;
   }
@@ -1254,7 +1263,13 @@ void nn_apply(Context &c, NNModel &M, const Spinor &r, Spinor &z) {
   M.alloc(c.g);
   const NNLinks &U = M.links_for(c);
   nn_from_spinor(c.g, r, M.acts[0], 1.f, c.stream);
-  for (size_t l = 0; l < M.layers.size(); l++) nn_layer(c.g, U, M.dev[l], 0, M.acts[l], M.acts[l + 1], c.stream);
+  // the input lives on the even sites and only the even sites of the output
+  // are read: the first layer skips the transports out of odd sites, the
+  // last one forms the even sites only
+  const size_t L = M.layers.size();
+  for (size_t l = 0; l < L; l++)
+    nn_layer(c.g, U, M.dev[l], 0, M.acts[l], M.acts[l + 1], c.stream,
+             (l == 0 ? NN_SRC_EVEN : 0) | (l + 1 == L ? NN_OUT_EVEN : 0));
   nn_to_spinor(c.g, M.acts.back(), z, 1.f, c.stream);
 }
 

@@ -121,7 +121,7 @@ def heatbath_field(seed=2024, sweeps=200):
     return co, plaq
 
 
-def solve_section(train_steps=300):
+def solve_section(train_steps=1500):
     """Time to a 1e-10 true residual on a 32^3x64 beta=6.0 heatbath field:
     plain FP64 GMRES-DR, the mixed solver, and the mixed solver with the
     learned preconditioner trained on 16^3x32 configurations (Wilson kappa in
@@ -130,10 +130,10 @@ def solve_section(train_steps=300):
     lattice.finalize()
     lattice.init((16, 16, 16, 32), BC)
     t0 = time.perf_counter()
-    tr = learned.train(nconf=4, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=1, m=40, k=16, harvest_tol=1e-8,
-                       max_pairs=256, channels=(1, 8, 8, 8, 1), steps=train_steps, batch=4, lr=1e-3, seed=11)
-    train = {"lattice": [16, 16, 16, 32], "configs": 4, "pairs": tr["pairs"], "adam_steps": train_steps,
-             "batch": 4, "lr": 1e-3, "loss_before": round(tr["loss_before"], 4),
+    tr = learned.train(nconf=10, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=2, m=40, k=16, harvest_tol=1e-8,
+                       max_pairs=800, channels=(1, 8, 8, 8, 1), steps=train_steps, batch=8, lr=1e-3, seed=11)
+    train = {"lattice": [16, 16, 16, 32], "configs": 10, "pairs": tr["pairs"], "adam_steps": train_steps,
+             "batch": 8, "lr": 1e-3, "loss_before": round(tr["loss_before"], 4),
              "loss_after": round(tr["loss_after"], 4), "harvest_s": round(tr["harvest_s"], 2),
              "train_s": round(tr["train_s"], 2), "wall_s": round(time.perf_counter() - t0, 2)}
     print(f"[bench] training {train}", file=sys.stderr, flush=True)
@@ -218,7 +218,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-noise", action="store_true", help="skip the 24^3x48 batched noise-source section")
-    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--noise-rhs", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true", help="batched solver without graph replays (profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="also time a 1 GiB plain copy kernel")

@@ -6,13 +6,15 @@
   in tests/test_oracle_nn.py); bar: relative Frobenius error below 1.5e-2,
   the BF16 rounding of the operands;
 * training lowers the loss on harvested (residual, error) pairs;
-* the trained network as the flexible right preconditioner reaches the
-  1e-10 true residual and needs fewer Krylov steps than the same mixed
-  solver without it.
+* the trained network (four 16^3x32 training configurations) as the
+  flexible right preconditioner reaches the 1e-10 true residual on a
+  held-out configuration and needs fewer than 0.8x the Krylov steps of the
+  same mixed solver without it.
 """
 import numpy as np
 import pytest
 
+import oracle as O
 import oracle.nn as NN
 from baylorlqcd_amd import diracops, gauge, lattice, learned
 
@@ -96,13 +98,41 @@ def test_network_gradient_matches_oracle(lat):
             g = learned.layer_canonical(l, g, adjoint=1)
 
 
+def bf16(a):
+    """Round to the nearest BF16 value (ties to even), real and imaginary parts."""
+    def r(x):
+        u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+        return u.astype(np.uint32).view(np.float32).astype(np.float64)
+    return r(a.real) + 1j * r(a.imag)
+
+
+def test_apply_matches_layer_chain(lat):
+    # the solver's application skips the transports out of the (zero) odd
+    # sites in the first layer and forms only the even sites in the last one;
+    # on BF16-exact input it must give the dense layer chain's even sites exactly
+    n, bc, U, Ws = lat
+    odd = O.parity_mask(n)
+    psi = bf16(field(n, 1, 60)[0])
+    psi[odd] = 0
+    print("[nn] sparse application vs dense layer chain", flush=True)
+    got = learned.apply_canonical(psi)
+    acts = psi[None]
+    for l in range(len(Ws)):
+        acts = learned.layer_canonical(l, acts)
+    assert frob(got[~odd], acts[0][~odd]) < 1e-6
+    # and the network itself against the oracle
+    want = NN.forward(n, U, bc, Ws, psi)[-1][0]
+    assert frob(got[~odd], want[~odd]) < 3 * TOL
+
+
 @pytest.fixture(scope="module")
 def trained():
-    n, bc = (8, 8, 8, 16), (1, 1, 1, -1)
-    print("[nn] training on 8^3x16", flush=True)
+    n, bc = (16, 16, 16, 32), (1, 1, 1, -1)
+    print("[nn] training on 16^3x32", flush=True)
     lattice.init(n, bc)
-    out = learned.train(nconf=2, beta=6.0, nsweep=100, kappa=(0.150, 0.157), nsrc=1, m=40, k=16,
-                        harvest_tol=1e-8, max_pairs=64, channels=(1, 8, 8, 8, 1), steps=120, batch=4, lr=1e-3,
+    out = learned.train(nconf=4, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=1, m=40, k=16,
+                        harvest_tol=1e-8, max_pairs=128, channels=(1, 8, 8, 8, 1), steps=300, batch=4, lr=1e-3,
                         seed=7)
     print("[nn] train", {k: v for k, v in out.items() if k != "loss"}, "loss[0], loss[-1]", out["loss"][0],
           out["loss"][-1], flush=True)
@@ -123,7 +153,7 @@ def test_learned_solve_gains(trained):
     for mu in range(4):
         co[2 + 4 * (0 + 4 * mu)] = 1.0
         co[2 + 4 * (1 + 4 * mu)] = 1.0
-    diracops.gauge_heatbath(6.0, 100, 4242, 1, co)
+    diracops.gauge_heatbath(6.0, 200, 4242, 1, co)
     kappa = (0.1565, 0.0)
     plain = learned.bench(1, kappa, GMRES=(40, 16))
     nn = learned.bench(2, kappa, GMRES=(40, 16))
@@ -131,4 +161,5 @@ def test_learned_solve_gains(trained):
     print("[nn] mixed with network", nn, flush=True)
     assert plain["true_residual"] < 1e-10 and nn["true_residual"] < 1e-10
     assert nn["nn_steps"] > 0
-    assert nn["iterations"] < plain["iterations"]
+    # measured on this field: about half the Krylov steps
+    assert nn["iterations"] < 0.8 * plain["iterations"]
