@@ -57,6 +57,7 @@ SIGNATURES = {
     "qqcd_bench_solve": [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_double, ctypes.c_int,
                          ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _ip, _ip, _dp],
     "qqcd_bench_copy": [ctypes.c_double, ctypes.c_int, _dp],
+    "qqcd_bench_blas": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
     "qqcd_dense_schur": [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp],
     "qqcd_dense_reorder": [ctypes.c_int, _dp, _dp, ctypes.c_int],
     "qqcd_dense_lsq": [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp],
@@ -69,6 +70,8 @@ SIGNATURES = {
                         ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _ip, _dp,
                         _ip],
     "qqcd_loopops": [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp],
+    "qqcd_propagator": [ctypes.c_int, _ip, _dp, _ip, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                        ctypes.c_void_p, ctypes.c_void_p, _ip, _dp, _dp, _ip],
     "qqcd_loopsum": [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, ctypes.c_double, _dp],
     "qqcd_trace16": [_dp, _dp, _dp, _dp, _dp],
     "qqcd_batch_set": [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -5,12 +5,17 @@
 
 namespace qqcd {
 
-// A flat run of n complex numbers in FP64 or FP32.
+// A flat run of n = 12 * stride complex numbers (the 12 component runs of a
+// checkerboard field) in FP64, FP32 or FP16.  FP16 vectors are the FP16
+// spinors of the operator: element (k, site) is its half-precision pair times
+// norm[site]; they are read by every operation and written only through the
+// site-wise conversion (spinor_copy_convert), which forms the norms.
 struct Vec {
   void *data = nullptr;
   Prec prec = PREC_DOUBLE;
   long n = 0;
-  size_t bytes() const { return (size_t)n * (prec == PREC_DOUBLE ? 16 : 8); }
+  const float *norm = nullptr;  // FP16 only
+  size_t bytes() const { return (size_t)n * (prec == PREC_DOUBLE ? 16 : (prec == PREC_SINGLE ? 8 : 4)); }
 };
 
 inline Vec as_vec(const Spinor &s) {
@@ -18,6 +23,7 @@ inline Vec as_vec(const Spinor &s) {
   v.data = s.data;
   v.prec = s.prec;
   v.n = 12 * s.stride;
+  v.norm = s.norm;
   return v;
 }
 
@@ -55,5 +61,9 @@ class Blas {
  public:
   static constexpr int kmax = 136;  // vectors per multi_dot
 };
+
+// Kernel time (ms) of one multi-dot (op 0) or multi-axpy (op 1) pass over k
+// vectors of n complex numbers with `grid` workgroups (tuning).
+double bench_blas(long n, Prec p, int k, int op, int grid, int niter, hipStream_t s);
 
 }  // namespace qqcd

@@ -11,8 +11,8 @@
 // quark/inverters.f90:13485-13530), the MGS dot/axpy pairs of gmresdr
 // (:3120-3152), the per-step solution update (:3242-3255) and the restart
 // V <- V Z (:3461-3485) with fused, bandwidth-bound passes: one multi-dot
-// pass reads w and up to 8 basis vectors, one multi-axpy pass updates w and
-// returns its norm.
+// pass reads w and up to 16 basis vectors, one multi-axpy pass updates w and
+// returns its norm; every lane moves 16 bytes per vector and element pair.
 #include <stdexcept>
 #include <type_traits>
 #include <vector>
@@ -23,24 +23,69 @@ namespace qqcd {
 static constexpr int RB = 256;   // threads per reduction workgroup
 static constexpr int RG = 1024;  // reduction workgroups (grid-stride)
 
+// Element access in units of U consecutive elements (U = 4 for FP16
+// vectors, else 2), so that every lane moves at least 16 bytes per vector
+// and pass: two double2, one float4, or four half-precision complex numbers
+// (16 bytes) with the float4 of their four sites' norms.  Element e belongs to
+// site e % stride; the stride is a multiple of 64, so a unit never leaves a
+// component run.
 template <typename T> struct C2;
 template <> struct C2<double> {
-  typedef double2 V;
-  static __device__ __forceinline__ double2 ld(const void *p, long i) { return ((const double2 *)p)[i]; }
-  static __device__ __forceinline__ void st(void *p, long i, double re, double im) {
-    ((double2 *)p)[i] = make_double2(re, im);
+  template <int U>
+  static __device__ __forceinline__ void ld(const void *p, const float *, unsigned, long i, double2 (&e)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; u++) e[u] = ((const double2 *)p)[U * i + u];
+  }
+  template <int U> static __device__ __forceinline__ void st(void *p, long i, const double2 (&e)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; u++) ((double2 *)p)[U * i + u] = e[u];
   }
 };
 template <> struct C2<float> {
-  typedef float2 V;
-  static __device__ __forceinline__ double2 ld(const void *p, long i) {
-    const float2 v = ((const float2 *)p)[i];
-    return make_double2(v.x, v.y);
+  template <int U>
+  static __device__ __forceinline__ void ld(const void *p, const float *, unsigned, long i, double2 (&e)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; u += 2) {
+      const float4 v = ((const float4 *)p)[(U * i + u) / 2];
+      e[u] = make_double2(v.x, v.y);
+      e[u + 1] = make_double2(v.z, v.w);
+    }
   }
-  static __device__ __forceinline__ void st(void *p, long i, double re, double im) {
-    ((float2 *)p)[i] = make_float2((float)re, (float)im);
+  template <int U> static __device__ __forceinline__ void st(void *p, long i, const double2 (&e)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; u += 2)
+      ((float4 *)p)[(U * i + u) / 2] =
+          make_float4((float)e[u].x, (float)e[u].y, (float)e[u + 1].x, (float)e[u + 1].y);
   }
 };
+typedef f16 h8 __attribute__((ext_vector_type(8)));
+template <> struct C2<f16> {
+  template <int U>
+  static __device__ __forceinline__ void ld(const void *p, const float *nrm, unsigned stride, long i,
+                                            double2 (&e)[U]) {
+    static_assert(U == 4, "FP16 vectors are read four elements at a time");
+    const h8 v = ((const h8 *)p)[i];
+    const unsigned site = (4u * (unsigned)i) % stride;
+    const float4 m = *(const float4 *)(nrm + site);
+    e[0] = make_double2((double)(m.x * (float)v[0]), (double)(m.x * (float)v[1]));
+    e[1] = make_double2((double)(m.y * (float)v[2]), (double)(m.y * (float)v[3]));
+    e[2] = make_double2((double)(m.z * (float)v[4]), (double)(m.z * (float)v[5]));
+    e[3] = make_double2((double)(m.w * (float)v[6]), (double)(m.w * (float)v[7]));
+  }
+};
+template <typename T> struct UnitOf {
+  static constexpr int U = std::is_same<T, f16>::value ? 4 : 2;
+};
+
+// conj(a) b accumulated into (re, im)
+__device__ __forceinline__ void cdot_acc(double2 a, double2 b, double &re, double &im) {
+  re += a.x * b.x + a.y * b.y;
+  im += a.x * b.y - a.y * b.x;
+}
+// b - h a
+__device__ __forceinline__ double2 csub_mul(double2 b, double hr, double hi, double2 a) {
+  return make_double2(b.x - (hr * a.x - hi * a.y), b.y - (hr * a.y + hi * a.x));
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -67,27 +112,35 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *out) {
   }
 }
 
-// ---- multi-dot: out[k] = sum conj(v_k) w, k < K (<= 8), one pass ----
+// ---- multi-dot: out[k] = sum conj(v_k) w, k < K (<= KMAXP), one pass ----
+// nu = n / U units (vector lengths are 12 runs of a stride that is a multiple of 64)
+static constexpr int KMAXP = 16;
 struct PtrPack {
-  const void *p[8];
+  const void *p[KMAXP];
+  const float *n[KMAXP];  // norms of FP16 vectors
 };
 
 template <typename TV, typename TW, int K>
-__global__ void __launch_bounds__(RB) k_mdot(long n, PtrPack v, const void *w, double *part) {
+__global__ void __launch_bounds__(RB) k_mdot(long nu, PtrPack v, const void *w, const float *wn, unsigned stride,
+                                            double *part) {
+  constexpr int U = UnitOf<TV>::U;
   double acc[2 * K];
 #pragma unroll
   for (int k = 0; k < 2 * K; k++) acc[k] = 0;
-  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < n; i += (long)gridDim.x * RB) {
-    const double2 b = C2<TW>::ld(w, i);
+  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
+    double2 b[U];
+    C2<TW>::template ld<U>(w, wn, stride, i, b);
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      const double2 a = C2<TV>::ld(v.p[k], i);
-      acc[2 * k] += a.x * b.x + a.y * b.y;
-      acc[2 * k + 1] += a.x * b.y - a.y * b.x;
+      double2 a[U];
+      C2<TV>::template ld<U>(v.p[k], v.n[k], stride, i, a);
+#pragma unroll
+      for (int u = 0; u < U; u++) cdot_acc(a[u], b[u], acc[2 * k], acc[2 * k + 1]);
     }
   }
   block_sum<2 * K>(acc, part + (long)blockIdx.x * 2 * K);
 }
+
 
 // final pass: res[q] = sum_b part[b*nq + q], one workgroup per q, fixed order
 __global__ void __launch_bounds__(RB) k_final(int nb, int nq, const double *part, double *res) {
@@ -103,89 +156,114 @@ __global__ void __launch_bounds__(RB) k_final(int nb, int nq, const double *part
 
 // ---- w = w - sum_k h_k v_k, returns ||w||^2 (optional) ----
 struct CoefPack {
-  double re[8], im[8];
+  double re[KMAXP], im[KMAXP];
 };
 
 template <typename TV, typename TW, int K, bool NORM>
-__global__ void __launch_bounds__(RB) k_maxpy(long n, PtrPack v, CoefPack h, void *w, double *part) {
+__global__ void __launch_bounds__(RB) k_maxpy(long nu, PtrPack v, CoefPack h, void *w, unsigned stride,
+                                             double *part) {
+  constexpr int U = UnitOf<TV>::U;
   double acc[1] = {0};
-  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < n; i += (long)gridDim.x * RB) {
-    double2 b = C2<TW>::ld(w, i);
+  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
+    double2 b[U];
+    C2<TW>::template ld<U>(w, nullptr, stride, i, b);
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      const double2 a = C2<TV>::ld(v.p[k], i);
-      b.x -= h.re[k] * a.x - h.im[k] * a.y;
-      b.y -= h.re[k] * a.y + h.im[k] * a.x;
+      double2 a[U];
+      C2<TV>::template ld<U>(v.p[k], v.n[k], stride, i, a);
+#pragma unroll
+      for (int u = 0; u < U; u++) b[u] = csub_mul(b[u], h.re[k], h.im[k], a[u]);
     }
-    C2<TW>::st(w, i, b.x, b.y);
-    if (NORM) acc[0] += b.x * b.x + b.y * b.y;
+    C2<TW>::template st<U>(w, i, b);
+    if (NORM)
+#pragma unroll
+      for (int u = 0; u < U; u++) acc[0] += b[u].x * b[u].x + b[u].y * b[u].y;
   }
   if (NORM) block_sum<1>(acc, part + blockIdx.x);
 }
 
 // ---- y = a x + b y (+ norm of y) ----
+__device__ __forceinline__ double2 caxpby(double ar, double ai, double2 a, double br, double bi, double2 b) {
+  return make_double2(ar * a.x - ai * a.y + br * b.x - bi * b.y, ar * a.y + ai * a.x + br * b.y + bi * b.x);
+}
+
 template <typename TX, typename TY, bool NORM>
-__global__ void __launch_bounds__(RB) k_axpby(long n, double ar, double ai, const void *x, double br, double bi,
-                                             void *y, double *part) {
+__global__ void __launch_bounds__(RB) k_axpby(long nu, double ar, double ai, const void *x, const float *xn,
+                                             double br, double bi, void *y, unsigned stride, double *part) {
+  constexpr int U = UnitOf<TX>::U;
   double acc[1] = {0};
-  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < n; i += (long)gridDim.x * RB) {
-    const double2 a = C2<TX>::ld(x, i);
-    double2 b = make_double2(0, 0);
-    if (br != 0.0 || bi != 0.0) b = C2<TY>::ld(y, i);
-    const double re = ar * a.x - ai * a.y + br * b.x - bi * b.y;
-    const double im = ar * a.y + ai * a.x + br * b.y + bi * b.x;
-    C2<TY>::st(y, i, re, im);
-    if (NORM) acc[0] += re * re + im * im;
+  const bool ready = br != 0.0 || bi != 0.0;
+  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
+    double2 a[U], b[U];
+    C2<TX>::template ld<U>(x, xn, stride, i, a);
+    if (ready) {
+      C2<TY>::template ld<U>(y, nullptr, stride, i, b);
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; u++) b[u] = make_double2(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      b[u] = caxpby(ar, ai, a[u], br, bi, b[u]);
+      if (NORM) acc[0] += b[u].x * b[u].x + b[u].y * b[u].y;
+    }
+    C2<TY>::template st<U>(y, i, b);
   }
   if (NORM) block_sum<1>(acc, part + blockIdx.x);
 }
 
 // ---- Y_j = sum_i V_i Z(i,j) (+ add to Y if accumulate) -----------------
-// Z is column-major m x k complex on the device.  Each lane walks its site
+// Z is column-major m x k complex on the device.  Each lane walks its unit
 // over the m inputs once per chunk of up to 8 outputs.
 struct OutPack {
   void *p[8];
 };
 template <typename TV, typename TY, int KO>
-__global__ void __launch_bounds__(RB) k_combine(long n, int m, const void *const *V, const double *Z, int ldz,
-                                               int j0, OutPack Y, int accumulate) {
-  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < n; i += (long)gridDim.x * RB) {
-    double sr[KO], si[KO];
+__global__ void __launch_bounds__(RB) k_combine(long nu, int m, const void *const *V, const float *const *VN,
+                                               const double *Z, int ldz, int j0, OutPack Y, int accumulate,
+                                               unsigned stride) {
+  constexpr int U = UnitOf<TV>::U;
+  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
+    double2 s[KO][U];
 #pragma unroll
-    for (int q = 0; q < KO; q++) {
-      sr[q] = 0;
-      si[q] = 0;
-    }
+    for (int q = 0; q < KO; q++)
+#pragma unroll
+      for (int u = 0; u < U; u++) s[q][u] = make_double2(0, 0);
     for (int r = 0; r < m; r++) {
-      const double2 a = C2<TV>::ld(V[r], i);
+      double2 a[U];
+      C2<TV>::template ld<U>(V[r], VN[r], stride, i, a);
 #pragma unroll
       for (int q = 0; q < KO; q++) {
         const double zr = Z[2 * (r + (long)ldz * (j0 + q))], zi = Z[2 * (r + (long)ldz * (j0 + q)) + 1];
-        sr[q] += a.x * zr - a.y * zi;
-        si[q] += a.x * zi + a.y * zr;
+#pragma unroll
+        for (int u = 0; u < U; u++) s[q][u] = caxpby(zr, zi, a[u], 1.0, 0.0, s[q][u]);
       }
     }
 #pragma unroll
     for (int q = 0; q < KO; q++) {
       if (accumulate) {
-        const double2 b = C2<TY>::ld(Y.p[q], i);
-        C2<TY>::st(Y.p[q], i, b.x + sr[q], b.y + si[q]);
-      } else {
-        C2<TY>::st(Y.p[q], i, sr[q], si[q]);
+        double2 b[U];
+        C2<TY>::template ld<U>(Y.p[q], nullptr, stride, i, b);
+#pragma unroll
+        for (int u = 0; u < U; u++) s[q][u] = make_double2(b[u].x + s[q][u].x, b[u].y + s[q][u].y);
       }
+      C2<TY>::template st<U>(Y.p[q], i, s[q]);
     }
   }
 }
+
 
 // ===================================================================== //
 // host side
 
 Blas::Blas(hipStream_t s) : stream(s) {
   QQCD_HIP(hipMalloc(&part, sizeof(double) * RG * 2 * kmax));
-  QQCD_HIP(hipMalloc(&res, sizeof(double) * 2 * kmax));
-  QQCD_HIP(hipHostMalloc(&hres, sizeof(double) * 2 * kmax, hipHostMallocDefault));
+  // one 2*KMAXP slab of results per pass of up to KMAXP vectors
+  const long nres = 2L * KMAXP * ((kmax + KMAXP - 1) / KMAXP);
+  QQCD_HIP(hipMalloc(&res, sizeof(double) * nres));
+  QQCD_HIP(hipHostMalloc(&hres, sizeof(double) * nres, hipHostMallocDefault));
   QQCD_HIP(hipMalloc(&zbuf, sizeof(double) * 2 * zcap));
-  QQCD_HIP(hipMalloc(&vptr, sizeof(void *) * 128));
+  QQCD_HIP(hipMalloc(&vptr, sizeof(void *) * 256));  // 128 vector and 128 norm pointers
 }
 
 Blas::~Blas() {
@@ -204,20 +282,49 @@ void Blas::fetch(int nq) {
 }
 
 // call f(TA*, TB*) with the element types of a precision pair
+// (FP16 only as the read side: FP16 vectors are written by the site-wise conversion)
 template <typename F> static void tsw(Prec pa, Prec pb, F &&f) {
   if (pa == PREC_DOUBLE && pb == PREC_DOUBLE) f((double *)nullptr, (double *)nullptr);
   else if (pa == PREC_SINGLE && pb == PREC_DOUBLE) f((float *)nullptr, (double *)nullptr);
   else if (pa == PREC_DOUBLE && pb == PREC_SINGLE) f((double *)nullptr, (float *)nullptr);
   else if (pa == PREC_SINGLE && pb == PREC_SINGLE) f((float *)nullptr, (float *)nullptr);
+  else if (pa == PREC_HALF && pb == PREC_SINGLE) f((f16 *)nullptr, (float *)nullptr);
+  else if (pa == PREC_HALF && pb == PREC_DOUBLE) f((f16 *)nullptr, (double *)nullptr);
   else throw std::runtime_error("blas: unsupported precision pair");
+}
+
+static unsigned stride_of(const Vec &v) {
+  if (v.n % 12 || v.n / 12 > 0x7FFFFFFFL) throw std::runtime_error("blas: vector is not a checkerboard field");
+  return (unsigned)(v.n / 12);
+}
+static void check_half(const Vec &v) {
+  if (v.prec == PREC_HALF && !v.norm) throw std::runtime_error("blas: FP16 vector without norms");
 }
 #define TYPES_OF(ta, tb)                                   \
   typedef typename std::remove_pointer<decltype(ta)>::type TA; \
   typedef typename std::remove_pointer<decltype(tb)>::type TB;
 
+// kernel units of a vector of n elements read as TA
+template <typename TA> static long units(long n) {
+  if (n % UnitOf<TA>::U) throw std::runtime_error("blas: vector length is not a whole number of units");
+  return n / UnitOf<TA>::U;
+}
+
+// f(std::integral_constant<int, K>) for K = k (1..16)
+template <typename F> static void with_k(int k, F &&f) {
+  switch (k) {
+#define KC(K) \
+  case K: f(std::integral_constant<int, K>()); break;
+    KC(1) KC(2) KC(3) KC(4) KC(5) KC(6) KC(7) KC(8) KC(9) KC(10) KC(11) KC(12) KC(13) KC(14) KC(15) KC(16)
+#undef KC
+    default: throw std::runtime_error("blas: 1..16 vectors per pass");
+  }
+}
+
 template <int K, typename TA, typename TB>
-static void launch_mdot(hipStream_t s, long n, const PtrPack &pp, const void *w, double *part) {
-  hipLaunchKernelGGL((k_mdot<TA, TB, K>), dim3(RG), dim3(RB), 0, s, n, pp, w, part);
+static void launch_mdot(hipStream_t s, const Vec &w, const PtrPack &pp, double *part) {
+  hipLaunchKernelGGL((k_mdot<TA, TB, K>), dim3(RG), dim3(RB), 0, s, units<TA>(w.n), pp, w.data, w.norm, stride_of(w),
+                     part);
 }
 
 void Blas::multi_dot(int k, const Vec *v, const Vec &w, std::complex<double> *out) {
@@ -227,26 +334,19 @@ void Blas::multi_dot(int k, const Vec *v, const Vec &w, std::complex<double> *ou
   long off = 0;
   std::vector<long> offs;
   std::vector<int> kks;
-  for (int k0 = 0; k0 < k; k0 += 8) {
-    const int kk = std::min(8, k - k0);
+  for (int k0 = 0; k0 < k; k0 += KMAXP) {
+    const int kk = std::min(KMAXP, k - k0);
     PtrPack pp{};
     for (int q = 0; q < kk; q++) {
-      if (v[k0 + q].n != w.n) throw std::runtime_error("multi_dot: length mismatch");
+      if (v[k0 + q].n != w.n || v[k0 + q].prec != v[k0].prec) throw std::runtime_error("multi_dot: mixed vectors");
+      check_half(v[k0 + q]);
       pp.p[q] = v[k0 + q].data;
+      pp.n[q] = v[k0 + q].norm;
     }
     double *pt = part + off;
     tsw(v[k0].prec, w.prec, [&](auto ta, auto tb) {
       TYPES_OF(ta, tb)
-      switch (kk) {
-        case 1: launch_mdot<1, TA, TB>(stream, w.n, pp, w.data, pt); break;
-        case 2: launch_mdot<2, TA, TB>(stream, w.n, pp, w.data, pt); break;
-        case 3: launch_mdot<3, TA, TB>(stream, w.n, pp, w.data, pt); break;
-        case 4: launch_mdot<4, TA, TB>(stream, w.n, pp, w.data, pt); break;
-        case 5: launch_mdot<5, TA, TB>(stream, w.n, pp, w.data, pt); break;
-        case 6: launch_mdot<6, TA, TB>(stream, w.n, pp, w.data, pt); break;
-        case 7: launch_mdot<7, TA, TB>(stream, w.n, pp, w.data, pt); break;
-        default: launch_mdot<8, TA, TB>(stream, w.n, pp, w.data, pt); break;
-      }
+      with_k(kk, [&](auto K) { launch_mdot<decltype(K)::value, TA, TB>(stream, w, pp, pt); });
     });
     QQCD_HIP(hipGetLastError());
     offs.push_back(off);
@@ -255,13 +355,13 @@ void Blas::multi_dot(int k, const Vec *v, const Vec &w, std::complex<double> *ou
   }
   for (size_t c = 0; c < offs.size(); c++)
     hipLaunchKernelGGL(k_final, dim3(2 * kks[c]), dim3(RB), 0, stream, RG, 2 * kks[c], part + offs[c],
-                       res + 16 * c);
+                       res + 2 * KMAXP * c);
   QQCD_HIP(hipGetLastError());
-  QQCD_HIP(hipMemcpyAsync(hres, res, sizeof(double) * 16 * offs.size(), hipMemcpyDeviceToHost, stream));
+  QQCD_HIP(hipMemcpyAsync(hres, res, sizeof(double) * 2 * KMAXP * offs.size(), hipMemcpyDeviceToHost, stream));
   QQCD_HIP(hipStreamSynchronize(stream));
   for (size_t c = 0; c < offs.size(); c++)
     for (int q = 0; q < kks[c]; q++)
-      out[8 * c + q] = std::complex<double>(hres[16 * c + 2 * q], hres[16 * c + 2 * q + 1]);
+      out[KMAXP * c + q] = std::complex<double>(hres[2 * KMAXP * c + 2 * q], hres[2 * KMAXP * c + 2 * q + 1]);
 }
 
 std::complex<double> Blas::dot(const Vec &a, const Vec &b) {
@@ -273,39 +373,33 @@ std::complex<double> Blas::dot(const Vec &a, const Vec &b) {
 double Blas::norm2(const Vec &a) { return dot(a, a).real(); }
 
 template <int K, bool NORM, typename TA, typename TB>
-static void launch_maxpy(hipStream_t s, long n, const PtrPack &pp, const CoefPack &cp, void *w, double *part) {
-  hipLaunchKernelGGL((k_maxpy<TA, TB, K, NORM>), dim3(RG), dim3(RB), 0, s, n, pp, cp, w, part);
+static void launch_maxpy(hipStream_t s, Vec &w, const PtrPack &pp, const CoefPack &cp, double *part) {
+  hipLaunchKernelGGL((k_maxpy<TA, TB, K, NORM>), dim3(RG), dim3(RB), 0, s, units<TA>(w.n), pp, cp, w.data,
+                     stride_of(w), part);
 }
 
 double Blas::multi_axpy(int k, const std::complex<double> *h, const Vec *v, Vec &w, bool want_norm) {
   double nrm = 0;
-  for (int k0 = 0; k0 < k; k0 += 8) {
-    const int kk = std::min(8, k - k0);
-    const bool last = k0 + 8 >= k;
+  for (int k0 = 0; k0 < k; k0 += KMAXP) {
+    const int kk = std::min(KMAXP, k - k0);
+    const bool last = k0 + KMAXP >= k;
     PtrPack pp{};
     CoefPack cp{};
     for (int q = 0; q < kk; q++) {
+      if (v[k0 + q].n != w.n || v[k0 + q].prec != v[k0].prec) throw std::runtime_error("multi_axpy: mixed vectors");
+      check_half(v[k0 + q]);
       pp.p[q] = v[k0 + q].data;
+      pp.n[q] = v[k0 + q].norm;
       cp.re[q] = h[k0 + q].real();
       cp.im[q] = h[k0 + q].imag();
     }
     const bool nm = want_norm && last;
     tsw(v[k0].prec, w.prec, [&](auto ta, auto tb) {
       TYPES_OF(ta, tb)
-#define MX(KK)                                                                   \
-  if (nm) launch_maxpy<KK, true, TA, TB>(stream, w.n, pp, cp, w.data, part);    \
-  else launch_maxpy<KK, false, TA, TB>(stream, w.n, pp, cp, w.data, part);
-      switch (kk) {
-        case 1: MX(1); break;
-        case 2: MX(2); break;
-        case 3: MX(3); break;
-        case 4: MX(4); break;
-        case 5: MX(5); break;
-        case 6: MX(6); break;
-        case 7: MX(7); break;
-        default: MX(8); break;
-      }
-#undef MX
+      with_k(kk, [&](auto K) {
+        if (nm) launch_maxpy<decltype(K)::value, true, TA, TB>(stream, w, pp, cp, part);
+        else launch_maxpy<decltype(K)::value, false, TA, TB>(stream, w, pp, cp, part);
+      });
     });
     QQCD_HIP(hipGetLastError());
     if (nm) {
@@ -319,14 +413,16 @@ double Blas::multi_axpy(int k, const std::complex<double> *h, const Vec *v, Vec 
 
 double Blas::axpby(std::complex<double> a, const Vec &x, std::complex<double> b, Vec &y, bool want_norm) {
   if (x.n != y.n) throw std::runtime_error("axpby: length mismatch");
+  check_half(x);
+  const unsigned st = stride_of(x);
   tsw(x.prec, y.prec, [&](auto ta, auto tb) {
     TYPES_OF(ta, tb)
     if (want_norm)
-      hipLaunchKernelGGL((k_axpby<TA, TB, true>), dim3(RG), dim3(RB), 0, stream, x.n, a.real(), a.imag(), x.data,
-                         b.real(), b.imag(), y.data, part);
+      hipLaunchKernelGGL((k_axpby<TA, TB, true>), dim3(RG), dim3(RB), 0, stream, units<TA>(x.n), a.real(), a.imag(),
+                         x.data, x.norm, b.real(), b.imag(), y.data, st, part);
     else
-      hipLaunchKernelGGL((k_axpby<TA, TB, false>), dim3(RG), dim3(RB), 0, stream, x.n, a.real(), a.imag(), x.data,
-                         b.real(), b.imag(), y.data, part);
+      hipLaunchKernelGGL((k_axpby<TA, TB, false>), dim3(RG), dim3(RB), 0, stream, units<TA>(x.n), a.real(), a.imag(),
+                         x.data, x.norm, b.real(), b.imag(), y.data, st, part);
   });
   QQCD_HIP(hipGetLastError());
   if (!want_norm) return 0;
@@ -338,26 +434,37 @@ void Blas::copy(const Vec &x, Vec &y) {
   if (x.prec == y.prec) {
     if (x.n != y.n) throw std::runtime_error("copy: length mismatch");
     QQCD_HIP(hipMemcpyAsync(y.data, x.data, x.bytes(), hipMemcpyDeviceToDevice, stream));
+    if (x.prec == PREC_HALF) {
+      check_half(x);
+      check_half(y);
+      QQCD_HIP(hipMemcpyAsync((void *)y.norm, x.norm, sizeof(float) * stride_of(x), hipMemcpyDeviceToDevice, stream));
+    }
   } else {
     axpby(1.0, x, 0.0, y, false);
   }
 }
 
+// (an FP16 vector with zero halves is zero whatever its norms)
 void Blas::zero(Vec &x) { QQCD_HIP(hipMemsetAsync(x.data, 0, x.bytes(), stream)); }
 
 template <int KO, typename TA, typename TB>
-static void launch_combine(hipStream_t s, long n, int m, void **vp, const double *Z, int ldz, int j0,
-                           const OutPack &op, int acc) {
-  hipLaunchKernelGGL((k_combine<TA, TB, KO>), dim3(RG), dim3(RB), 0, s, n, m, (const void *const *)vp, Z, ldz, j0,
-                     op, acc);
+static void launch_combine(hipStream_t s, const Vec &y, int m, void **vp, void **np, const double *Z, int ldz,
+                           int j0, const OutPack &op, int acc) {
+  hipLaunchKernelGGL((k_combine<TA, TB, KO>), dim3(RG), dim3(RB), 0, s, units<TA>(y.n), m, (const void *const *)vp,
+                     (const float *const *)np, Z, ldz, j0, op, acc, stride_of(y));
 }
 
 void Blas::combine(int m, const Vec *V, int k, const std::complex<double> *Z, int ldz, Vec *Y, bool accumulate) {
   if (m == 0 || k == 0) return;
   if ((long)ldz * k > zcap || m > 128) throw std::runtime_error("combine: coefficient block too large");
-  std::vector<void *> hp(m);
-  for (int r = 0; r < m; r++) hp[r] = V[r].data;
-  QQCD_HIP(hipMemcpyAsync(vptr, hp.data(), sizeof(void *) * m, hipMemcpyHostToDevice, stream));
+  std::vector<void *> hp(2 * m);
+  for (int r = 0; r < m; r++) {
+    if (V[r].prec != V[0].prec || V[r].n != Y[0].n) throw std::runtime_error("combine: mixed vectors");
+    check_half(V[r]);
+    hp[r] = V[r].data;
+    hp[m + r] = (void *)V[r].norm;
+  }
+  QQCD_HIP(hipMemcpyAsync(vptr, hp.data(), sizeof(void *) * 2 * m, hipMemcpyHostToDevice, stream));
   QQCD_HIP(hipMemcpyAsync(zbuf, Z, sizeof(double) * 2 * ldz * k, hipMemcpyHostToDevice, stream));
   for (int j0 = 0; j0 < k; j0 += 8) {
     const int kk = std::min(8, k - j0);
@@ -366,20 +473,81 @@ void Blas::combine(int m, const Vec *V, int k, const std::complex<double> *Z, in
     tsw(V[0].prec, Y[0].prec, [&](auto ta, auto tb) {
       TYPES_OF(ta, tb)
       switch (kk) {
-        case 1: launch_combine<1, TA, TB>(stream, Y[0].n, m, vptr, zbuf, ldz, j0, op, accumulate); break;
-        case 2: launch_combine<2, TA, TB>(stream, Y[0].n, m, vptr, zbuf, ldz, j0, op, accumulate); break;
-        case 3: launch_combine<3, TA, TB>(stream, Y[0].n, m, vptr, zbuf, ldz, j0, op, accumulate); break;
-        case 4: launch_combine<4, TA, TB>(stream, Y[0].n, m, vptr, zbuf, ldz, j0, op, accumulate); break;
-        case 5: launch_combine<5, TA, TB>(stream, Y[0].n, m, vptr, zbuf, ldz, j0, op, accumulate); break;
-        case 6: launch_combine<6, TA, TB>(stream, Y[0].n, m, vptr, zbuf, ldz, j0, op, accumulate); break;
-        case 7: launch_combine<7, TA, TB>(stream, Y[0].n, m, vptr, zbuf, ldz, j0, op, accumulate); break;
-        default: launch_combine<8, TA, TB>(stream, Y[0].n, m, vptr, zbuf, ldz, j0, op, accumulate); break;
+        case 1: launch_combine<1, TA, TB>(stream, Y[0], m, vptr, vptr + m, zbuf, ldz, j0, op, accumulate); break;
+        case 2: launch_combine<2, TA, TB>(stream, Y[0], m, vptr, vptr + m, zbuf, ldz, j0, op, accumulate); break;
+        case 3: launch_combine<3, TA, TB>(stream, Y[0], m, vptr, vptr + m, zbuf, ldz, j0, op, accumulate); break;
+        case 4: launch_combine<4, TA, TB>(stream, Y[0], m, vptr, vptr + m, zbuf, ldz, j0, op, accumulate); break;
+        case 5: launch_combine<5, TA, TB>(stream, Y[0], m, vptr, vptr + m, zbuf, ldz, j0, op, accumulate); break;
+        case 6: launch_combine<6, TA, TB>(stream, Y[0], m, vptr, vptr + m, zbuf, ldz, j0, op, accumulate); break;
+        case 7: launch_combine<7, TA, TB>(stream, Y[0], m, vptr, vptr + m, zbuf, ldz, j0, op, accumulate); break;
+        default: launch_combine<8, TA, TB>(stream, Y[0], m, vptr, vptr + m, zbuf, ldz, j0, op, accumulate); break;
       }
     });
     QQCD_HIP(hipGetLastError());
   }
   // the host copies of the pointer and coefficient tables go out of scope
   QQCD_HIP(hipStreamSynchronize(stream));
+}
+
+// Time one fused pass over k lattice vectors of precision p (FP64 or FP32)
+// and grid workgroups: op 0 the multi-dot, 1 the multi-axpy with norm.
+// Average kernel time in ms over niter launches (tuning only).
+double bench_blas(long n, Prec p, int k, int op, int grid, int niter, hipStream_t s) {
+  if (k < 1 || k > KMAXP || grid < 1 || grid > 8192 || niter < 1) throw std::runtime_error("bench_blas: bad sizes");
+  if (p != PREC_DOUBLE && p != PREC_SINGLE) throw std::runtime_error("bench_blas: FP64 or FP32");
+  const size_t bytes = (size_t)n * (p == PREC_DOUBLE ? 16 : 8);
+  std::vector<void *> v(k + 1, nullptr);
+  double *part = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float ms = 0;
+  try {
+    for (auto &q : v) {
+      QQCD_HIP(hipMalloc(&q, bytes));
+      QQCD_HIP(hipMemsetAsync(q, 0, bytes, s));
+    }
+    QQCD_HIP(hipMalloc(&part, sizeof(double) * grid * 2 * KMAXP));
+    PtrPack pp{};
+    CoefPack cp{};
+    for (int q = 0; q < k; q++) {
+      pp.p[q] = v[q];
+      cp.re[q] = 1e-3;
+    }
+    QQCD_HIP(hipEventCreate(&e0));
+    QQCD_HIP(hipEventCreate(&e1));
+    auto run = [&]() {
+      tsw(p, p, [&](auto ta, auto tb) {
+        TYPES_OF(ta, tb)
+        with_k(k, [&](auto K) {
+          constexpr int KK = decltype(K)::value;
+          if (op == 0)
+            hipLaunchKernelGGL((k_mdot<TA, TB, KK>), dim3(grid), dim3(RB), 0, s, units<TA>(n), pp, v[k], nullptr,
+                               (unsigned)(n / 12), part);
+          else
+            hipLaunchKernelGGL((k_maxpy<TA, TB, KK, true>), dim3(grid), dim3(RB), 0, s, units<TA>(n), pp, cp, v[k],
+                               (unsigned)(n / 12), part);
+        });
+      });
+      QQCD_HIP(hipGetLastError());
+    };
+    run();
+    QQCD_HIP(hipEventRecord(e0, s));
+    for (int it = 0; it < niter; it++) run();
+    QQCD_HIP(hipEventRecord(e1, s));
+    QQCD_HIP(hipEventSynchronize(e1));
+    QQCD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  } catch (...) {
+    for (auto &q : v)
+      if (q) (void)hipFree(q);
+    if (part) (void)hipFree(part);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    throw;
+  }
+  for (auto &q : v) (void)hipFree(q);
+  (void)hipFree(part);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return (double)ms / niter;
 }
 
 }  // namespace qqcd

@@ -405,6 +405,13 @@ namespace qqcd {
 double bench_copy(size_t bytes, int iters, hipStream_t s);
 }
 
+extern "C" int qqcd_bench_blas(int prec, int k, int op, int grid, int niter, double *ms) {
+  API_BEGIN
+  Context &c = ctx();
+  *ms = bench_blas(12 * c.g.stride, (Prec)prec, k, op, grid, niter, c.stream);
+  API_END
+}
+
 extern "C" int qqcd_bench_copy(double bytes, int iters, double *ms) {
   API_BEGIN
   Context &c = ctx();

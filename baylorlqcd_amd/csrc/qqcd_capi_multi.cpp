@@ -11,6 +11,7 @@
 
 #include "../../include/qqcd_hip.h"
 #include "qqcd_batch.h"
+#include "qqcd_solvers.h"
 #include "qqcd_loops.h"
 #include "qqcd_noise.h"
 
@@ -460,6 +461,111 @@ extern "C" int qqcd_noise_loop(int nrhs, unsigned long long seed, double scale, 
     QQCD_HIP(hipMemcpy(hl.data(), lo->p, hl.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < hl.size(); i++) loops[i] = f * hl[i];
   }
+  times[0] = std::chrono::duration<double>(t1 - t0).count();
+  times[1] = std::chrono::duration<double>(t2 - t1).count();
+  times[2] = std::chrono::duration<double>(t2 - t0).count();
+  API_END
+}
+
+// The full propagator of one source site: nrhs (<= 12) point sources
+// (spin r/3 + 1, colour r%3 + 1, pointsource's numbering) solved on the
+// resident links to ||b - M x|| < tol ||b|| each.  The first runs through
+// the mixed solver (GMRES-DR(m, k) inner solves with op_prec / op_recon
+// links, FP32 basis, FP64 reliable updates) and keeps its deflation space;
+// the rest: method 0 all together through the batched GMRES-Proj(m - k)
+// over it, 1 one by one through the mixed solver with GMRES-Proj inner
+// solves over it, 2 one by one through the mixed solver with the learned
+// network as the flexible right preconditioner (dropped for the
+// unpreconditioned solver when an outer step stalls).
+extern "C" int qqcd_propagator(int nrhs, const int src[4], const double kappa[2], const int GMRES[2], double tol,
+                               int op_prec, int op_recon, int method, double *const *xe, double *const *xo,
+                               int *iters, double *res_full, double *times, int *stats) {
+  API_BEGIN
+  Context &c = ctx();
+  if (!c.have_gauge) throw std::runtime_error("no gauge field loaded");
+  if (nrhs < 1 || nrhs > 12) throw std::runtime_error("propagator: 1..12 right-hand sides (spin x colour)");
+  if (GMRES[1] < 1 || GMRES[1] >= GMRES[0] || GMRES[0] > 120) throw std::runtime_error("GMRES sizes need 0 < k < m <= 120");
+  if (method < 0 || method > 2) throw std::runtime_error("propagator: method 0, 1 or 2");
+  if (op_prec != PREC_SINGLE && op_prec != PREC_HALF) throw std::runtime_error("propagator: FP32 or FP16 operator");
+  QQCD_HIP(hipDeviceSynchronize());
+  const auto t0 = std::chrono::steady_clock::now();
+  SpinorBuf b_e(c.g, PREC_DOUBLE), b_o(c.g, PREC_DOUBLE), srcv(c.g, PREC_DOUBLE), x_e(c.g, PREC_DOUBLE),
+      x_o(c.g, PREC_DOUBLE);
+  auto source = [&](int r) {
+    fill_spinor(c.g, b_e.s, 0.0, 0.0, c.stream);
+    fill_spinor(c.g, b_o.s, 0.0, 0.0, c.stream);
+    point_source_dev(c, b_e.s, b_o.s, src, r / 3 + 1, r % 3 + 1);
+  };
+  auto finish = [&](int r) {
+    reconstruct_odd(c, kappa, b_o.s, x_e.s, x_o.s);
+    res_full[r] = full_residual(c, kappa, b_e.s, b_o.s, x_e.s, x_o.s);
+    if (xe && xe[r]) to_host(c, x_e.s, xe[r], 0);
+    if (xo && xo[r]) to_host(c, x_o.s, xo[r], 1);
+  };
+  std::memset(stats, 0, 8 * sizeof(int));
+  Deflation D;
+  MixedParams mp = mixed_params(GMRES, tol, PREC_SINGLE, op_prec, op_recon, 0.0);
+  source(0);
+  prep_source(c, kappa, b_e.s, b_o.s, srcv.s);
+  fill_spinor(c.g, x_e.s, 0.0, 0.0, c.stream);
+  MixedStats ms = solve_mixed(c, kappa, srcv.s, x_e.s, mp, &D);
+  iters[0] = ms.inner.iters;
+  stats[7] = ms.outer;
+  finish(0);
+  QQCD_HIP(hipDeviceSynchronize());
+  const auto t1 = std::chrono::steady_clock::now();
+  const int rest = nrhs - 1;
+  if (rest > 0 && method == 0) {
+    // the largest RHS block <= 16 that pads the slot count by at most 1/8:
+    // every slot of a block is allocated in each of the m + 1 basis fields
+    int nb = 1;
+    for (int cand = 16; cand >= 1; cand >>= 1)
+      if (((rest + cand - 1) / cand * cand - rest) * 8 <= rest) {
+        nb = cand;
+        break;
+      }
+    MultiBuf B(c.g, PREC_DOUBLE, rest, nb), X(c.g, PREC_DOUBLE, rest, nb);
+    MBlas mb(c.stream, rest);
+    for (int r = 1; r < nrhs; r++) {
+      source(r);
+      prep_source(c, kappa, b_e.s, b_o.s, srcv.s);
+      mb.put(srcv.s, B.m, r - 1);
+    }
+    std::vector<int> id;
+    BatchParams bp = batch_params(GMRES, tol, rest);
+    g_bst = batched_solve(c, kappa, B.m, X.m, rest, D, bp, id);
+    for (int s = 0; s < rest; s++) {
+      const int r = id[s] + 1;
+      mb.get(X.m, s, x_e.s);
+      source(r);
+      iters[r] = g_bst.iters[id[s]];
+      finish(r);
+    }
+    stats[0] = g_bst.outer;
+    stats[1] = g_bst.cycles;
+    stats[2] = (int)g_bst.batched_matvecs;
+    stats[3] = g_bst.graph_replays;
+    stats[4] = g_bst.fallback;
+    stats[5] = nb;
+  } else if (rest > 0) {
+    LinOp pre;
+    if (method == 2) pre = nn_precond(c);
+    for (int r = 1; r < nrhs; r++) {
+      source(r);
+      prep_source(c, kappa, b_e.s, b_o.s, srcv.s);
+      fill_spinor(c.g, x_e.s, 0.0, 0.0, c.stream);
+      MixedParams q = mp;
+      if (method == 2) q.precond = &pre;
+      MixedStats m2 = solve_mixed(c, kappa, srcv.s, x_e.s, q, method == 2 ? nullptr : &D);
+      iters[r] = m2.inner.iters;
+      stats[0] += m2.outer;
+      stats[4] += m2.fallback != 0;
+      stats[6] += m2.precond_dropped;
+      finish(r);
+    }
+  }
+  QQCD_HIP(hipDeviceSynchronize());
+  const auto t2 = std::chrono::steady_clock::now();
   times[0] = std::chrono::duration<double>(t1 - t0).count();
   times[1] = std::chrono::duration<double>(t2 - t1).count();
   times[2] = std::chrono::duration<double>(t2 - t0).count();

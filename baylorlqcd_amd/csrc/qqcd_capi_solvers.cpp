@@ -265,10 +265,10 @@ extern "C" int qqcd_dense_harmonic_ritz(int m, const double *hbar, int ldh, int 
 MixedParams qqcd::mixed_params(const int GMRES[2], double tol, int basis, int op_prec, int op_recon,
                                double inner_tol) {
   check_mk(GMRES[0], GMRES[1]);
-  if (basis != 0 && basis != 1) throw std::runtime_error("basis precision must be 0 (FP64) or 1 (FP32)");
+  if (basis < 0 || basis > 2) throw std::runtime_error("basis precision must be 0 (FP64), 1 (FP32) or 2 (FP16)");
   if (op_prec < 0 || op_prec > 2 || (op_recon != 18 && op_recon != 12))
     throw std::runtime_error("bad operator precision / link format");
-  if (op_prec == 0 && basis == 1) throw std::runtime_error("an FP64 operator needs an FP64 basis");
+  if (op_prec == 0 && basis != 0) throw std::runtime_error("an FP64 operator needs an FP64 basis");
   MixedParams p;
   p.m = GMRES[0];
   p.k = GMRES[1];
@@ -356,13 +356,13 @@ extern "C" int qqcd_bench_solve(int method, const int src[4], int spin, int colo
     SolveStats st = gmresdr(c, op, src_e.s, x_e.s, GMRES[0], GMRES[1], tol, 0, KCYCLIM * GMRES[0], nullptr);
     *iters = st.iters;
     *outer = st.cycles;
-  } else if (method == 1) {
-    MixedParams p = mixed_params(GMRES, tol, 1, op_prec, op_recon, inner_tol);
+  } else if (method == 1 || method == 2) {
+    MixedParams p = mixed_params(GMRES, tol, method == 1 ? 1 : 2, op_prec, op_recon, inner_tol);
     MixedStats ms = solve_mixed(c, kappa, src_e.s, x_e.s, p, g_defl_mixed);
     *iters = ms.inner.iters;
     *outer = ms.outer;
   } else {
-    throw std::runtime_error("bench_solve: method must be 0 (FP64 GMRES-DR) or 1 (mixed)");
+    throw std::runtime_error("bench_solve: method must be 0 (FP64 GMRES-DR), 1 (mixed) or 2 (mixed, FP16 basis)");
   }
   reconstruct_odd(c, kappa, b_o.s, x_e.s, x_o.s);
   QQCD_HIP(hipDeviceSynchronize());

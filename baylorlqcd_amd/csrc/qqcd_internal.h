@@ -139,6 +139,7 @@ void ref_to_spinor(const Geom &g, const double *dref, Spinor &out, int par, hipS
 void spinor_to_ref(const Geom &g, const Spinor &in, double *dref, int par, hipStream_t s);
 void ref_to_gauge(const Geom &g, const double *dref, Gauge &U, hipStream_t s);
 void canon_to_gauge(const Geom &g, const double *dcanon, Gauge &U, hipStream_t s);
-void spinor_copy_convert(const Geom &g, const Spinor &in, Spinor &out, hipStream_t s);
+// out = scale * in across precisions (FP16 fields carry their per-site norm)
+void spinor_copy_convert(const Geom &g, const Spinor &in, Spinor &out, hipStream_t s, double scale = 1.0);
 
 }  // namespace qqcd

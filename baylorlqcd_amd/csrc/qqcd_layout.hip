@@ -214,11 +214,14 @@ __global__ void k_gauge(const LayoutArgs a, int src_ref, int nc, const double *s
 }
 
 template <typename A, typename B>
-__global__ void k_convert(int Vh, long stride, const void *in, const float *ninn, void *out, float *nout) {
+__global__ void k_convert(int Vh, long stride, const void *in, const float *ninn, void *out, float *nout,
+                          double scale) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Vh) return;
   double v[24];
   Put<A>::get(in, ninn, stride, i, v);
+  if (scale != 1.0)
+    for (int k = 0; k < 24; k++) v[k] *= scale;
   Put<B>::put(out, nout, stride, i, v);
 }
 
@@ -283,10 +286,12 @@ static void gauge_any(const Geom &g, int src_ref, const double *src, Gauge &U, h
 void ref_to_gauge(const Geom &g, const double *dref, Gauge &U, hipStream_t s) { gauge_any(g, 1, dref, U, s); }
 void canon_to_gauge(const Geom &g, const double *dcn, Gauge &U, hipStream_t s) { gauge_any(g, 0, dcn, U, s); }
 
-void spinor_copy_convert(const Geom &g, const Spinor &in, Spinor &out, hipStream_t s) {
+void spinor_copy_convert(const Geom &g, const Spinor &in, Spinor &out, hipStream_t s, double scale) {
   const int nb = (int)((g.Vh + 255) / 256);
   const int Vh = (int)g.Vh;
-#define CV(A, B) hipLaunchKernelGGL((k_convert<A, B>), dim3(nb), dim3(256), 0, s, Vh, g.stride, in.data, in.norm, out.data, out.norm)
+#define CV(A, B)                                                                                                  \
+  hipLaunchKernelGGL((k_convert<A, B>), dim3(nb), dim3(256), 0, s, Vh, g.stride, in.data, in.norm, out.data, out.norm, \
+                     scale)
   if (in.prec == PREC_DOUBLE) {
     if (out.prec == PREC_DOUBLE) CV(double, double);
     else if (out.prec == PREC_SINGLE) CV(double, float);

@@ -173,6 +173,14 @@ static double pair_grad(Context &c, NNModel &M, const NNLinks &U, const Pair &p,
   return g2 * p.e2;
 }
 
+LinOp nn_precond(Context &c) {
+  NNModel &M = nn_model(c);
+  LinOp pre;
+  pre.prec = PREC_SINGLE;
+  pre.apply = [&c, &M](const Spinor &a, Spinor &z) { nn_apply(c, M, a, z); };
+  return pre;
+}
+
 // loss only (evaluation)
 static double pair_loss(Context &c, NNModel &M, const NNLinks &U, const Pair &p, double *d_scr) {
   const int L = (int)M.layers.size();

@@ -59,20 +59,32 @@ static double residual(Context &c, LinOp &A, const Spinor &b, const Spinor &x, S
 // ||w||^2, i.e. when cancellation may have spoiled orthogonality).  The
 // first pass computes <V_i, w> and <w, w> in one sweep; the Hessenberg
 // column goes to h[0..j+1].  Returns the new norm.
+// With a separate work vector (a basis kept in a lower storage precision),
+// w = A v_j is formed and orthogonalised there and V[j+1] = w / ||w|| is
+// written once, by the site-wise conversion.
 static double arnoldi_step(Context &c, LinOp &A, std::vector<Spinor> &V, int j, zd *h, int reorth,
-                           LinOp *M = nullptr, std::vector<Spinor> *Z = nullptr) {
+                           LinOp *M = nullptr, std::vector<Spinor> *Z = nullptr, Spinor *work = nullptr) {
+  Spinor &ws = work ? *work : V[j + 1];
   if (M) {
     // flexible right preconditioning: z_j = M v_j, w = A z_j
     M->apply(V[j], (*Z)[j]);
-    A.apply((*Z)[j], V[j + 1]);
+    A.apply((*Z)[j], ws);
   } else {
-    A.apply(V[j], V[j + 1]);
+    A.apply(V[j], ws);
   }
-  std::vector<Vec> Vb = vecs(V, j + 2);  // V[0..j] and w itself
-  Vec w = vv(V[j + 1]);
+  Vec w = vv(ws);
+  std::vector<Vec> Vb = vecs(V, j + 1);
   std::vector<zd> h1(j + 2), h2(j + 1);
-  c.blas->multi_dot(j + 2, Vb.data(), w, h1.data());
-  const double w2 = h1[j + 1].real();
+  double w2;
+  if (!work) {
+    Vb.push_back(w);  // <w, w> in the same pass
+    c.blas->multi_dot(j + 2, Vb.data(), w, h1.data());
+    Vb.pop_back();
+    w2 = h1[j + 1].real();
+  } else {
+    c.blas->multi_dot(j + 1, Vb.data(), w, h1.data());
+    w2 = reorth ? c.blas->norm2(w) : 0.0;
+  }
   double n2 = c.blas->multi_axpy(j + 1, h1.data(), Vb.data(), w, true);
   for (int i = 0; i <= j; i++) h[i] = h1[i];
   if (reorth && n2 < 0.5 * w2) {
@@ -82,8 +94,19 @@ static double arnoldi_step(Context &c, LinOp &A, std::vector<Spinor> &V, int j, 
   }
   const double hn = std::sqrt(n2);
   h[j + 1] = hn;
-  if (hn > 0) c.blas->axpby(1.0 / hn, w, 0.0, w, false);
+  if (work) spinor_copy_convert(c.g, ws, V[j + 1], c.stream, hn > 0 ? 1.0 / hn : 0.0);
+  else if (hn > 0) c.blas->axpby(1.0 / hn, w, 0.0, w, false);
   return hn;
+}
+
+// dst = s * src across precisions (an FP16 destination through the conversion)
+static void scaled_copy(Context &c, const Spinor &src, Spinor &dst, double s) {
+  if (dst.prec == PREC_HALF || src.prec == PREC_HALF) {
+    spinor_copy_convert(c.g, src, dst, c.stream, s);
+  } else {
+    Vec d = vv(dst);
+    c.blas->axpby(s, vv(src), 0.0, d, false);
+  }
 }
 
 static void alloc_basis(Context &c, std::vector<Spinor> &V, int n, Prec p) {
@@ -152,18 +175,25 @@ static bool deflate(Context &c, std::vector<Spinor> &V, std::vector<Spinor> &W, 
     cnew[p] = s;
   }
   if (Zout) *Zout = Z;
-  // V[0..k] <- V[0..j] Z  (through W, then swap)
+  // V[0..k] <- V[0..j] Z  (formed in W)
   std::vector<Vec> Vin = vecs(V, rows), Wout = vecs(W, k + 1);
   c.blas->combine(rows, Vin.data(), k + 1, Z.data(), rows, Wout.data(), false);
-  for (int q = 0; q <= k; q++) std::swap(V[q], W[q]);
   // re-orthogonalise the last vector against the first k (the reference
   // does the same, inverters.f90:3491-3508)
-  std::vector<Vec> Vk = vecs(V, k);
-  Vec vk = vv(V[k]);
+  std::vector<Vec> Wk = vecs(W, k);
+  Vec wk = vv(W[k]);
   std::vector<zd> h(k);
-  c.blas->multi_dot(k, Vk.data(), vk, h.data());
-  const double n2 = c.blas->multi_axpy(k, h.data(), Vk.data(), vk, true);
-  c.blas->axpby(1.0 / std::sqrt(n2), vk, 0.0, vk, false);
+  c.blas->multi_dot(k, Wk.data(), wk, h.data());
+  const double n2 = c.blas->multi_axpy(k, h.data(), Wk.data(), wk, true);
+  const double s = 1.0 / std::sqrt(n2);
+  if (V[0].prec == W[0].prec) {
+    for (int q = 0; q <= k; q++) std::swap(V[q], W[q]);
+    c.blas->axpby(s, wk, 0.0, wk, false);  // wk now lives in V[k]
+  } else {
+    // basis kept in a lower storage precision: convert the new vectors in
+    for (int q = 0; q < k; q++) spinor_copy_convert(c.g, W[q], V[q], c.stream);
+    spinor_copy_convert(c.g, W[k], V[k], c.stream, s);
+  }
   return true;
 }
 
@@ -187,15 +217,18 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
   if (m < 2 || k < 0 || k >= m) throw std::runtime_error("gmresdr: need 0 <= k < m and m >= 2");
   if (M && defl) throw std::runtime_error("gmresdr: no deflation space is kept with a preconditioner");
   SolveStats st;
-  const Prec p = A.prec;
+  const Prec p = A.prec, vp = A.basis();
+  if (M && vp != p) throw std::runtime_error("gmresdr: the flexible variant keeps its basis in the work precision");
   std::vector<Spinor> V, W, Z, ZW;
   BasisGuard g1{V}, g2{W}, g3{Z}, g4{ZW};
-  alloc_basis(c, V, m + 1, p);
+  alloc_basis(c, V, m + 1, vp);
   alloc_basis(c, W, k + 1, p);
   if (M) {
     alloc_basis(c, Z, m, p);
     alloc_basis(c, ZW, k, p);
   }
+  std::unique_ptr<SpinorBuf> work;
+  if (vp != p) work.reset(new SpinorBuf(c.g, p));
   std::vector<zd> Zmat;
   SpinorBuf r(c.g, p);
   double rn = residual(c, A, b, x, r.s, st);
@@ -208,10 +241,7 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
   }
   const int ldh = m + 1;
   std::vector<zd> H((size_t)ldh * m, 0.0), cv(m + 1, 0.0), y(m), col(m + 2);
-  {
-    Vec v0 = vv(V[0]);
-    c.blas->axpby(1.0 / rn, vv(r.s), 0.0, v0, false);
-  }
+  scaled_copy(c, r.s, V[0], 1.0 / rn);
   cv[0] = rn;
   int j = 0;
   bool have_defl = false;
@@ -219,7 +249,7 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
   for (;;) {
     // Arnoldi
     while (j < m) {
-      const double hn = arnoldi_step(c, A, V, j, col.data(), reorth, M, M ? &Z : nullptr);
+      const double hn = arnoldi_step(c, A, V, j, col.data(), reorth, M, M ? &Z : nullptr, work ? &work->s : nullptr);
       st.matvecs++;
       st.iters++;
       for (int i = 0; i <= j + 1; i++) H[i + (size_t)ldh * j] = col[i];
@@ -243,7 +273,7 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
       st.converged = rn < tol * r0;
       // keep the harmonic Ritz space of the final cycle for later right-hand sides
       if (defl && j > k && k > 0 && deflate(c, V, W, j, k, H, ldh, cv, y, Hnew, cnew)) {
-        store_deflation(c, defl, V, k, Hnew, p);
+        store_deflation(c, defl, V, k, Hnew, vp);
         have_defl = true;
       }
       if (defl && !have_defl) defl->clear();
@@ -263,13 +293,12 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
       for (int i = 0; i <= k; i++) cv[i] = cnew[i];
       j = k;
       if (defl) {
-        store_deflation(c, defl, V, k, Hnew, p);
+        store_deflation(c, defl, V, k, Hnew, vp);
         have_defl = true;
       }
     } else {
       // plain restart from the true residual
-      Vec v0 = vv(V[0]);
-      c.blas->axpby(1.0 / rn, vv(r.s), 0.0, v0, false);
+      scaled_copy(c, r.s, V[0], 1.0 / rn);
       std::fill(H.begin(), H.end(), zd(0.0));
       std::fill(cv.begin(), cv.end(), zd(0.0));
       cv[0] = rn;
@@ -281,12 +310,14 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
 SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, const Deflation &defl, double tol,
                      int itermin, int maxiter, int reorth) {
   SolveStats st;
-  const Prec p = A.prec;
+  const Prec p = A.prec, vp = A.basis();
   const int k = defl.k;
   if (m < 1) throw std::runtime_error("gmresproj: m must be >= 1");
   std::vector<Spinor> V;
   BasisGuard g1{V};
-  alloc_basis(c, V, m + 1, p);
+  alloc_basis(c, V, m + 1, vp);
+  std::unique_ptr<SpinorBuf> work;
+  if (vp != p) work.reset(new SpinorBuf(c.g, p));
   SpinorBuf r(c.g, p);
   double rn = residual(c, A, b, x, r.s, st);
   const double r0 = rn;
@@ -312,14 +343,11 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
     // GMRES(m) cycle from r
     std::fill(H.begin(), H.end(), zd(0.0));
     std::fill(cv.begin(), cv.end(), zd(0.0));
-    {
-      Vec v0 = vv(V[0]);
-      c.blas->axpby(1.0 / rn, vv(r.s), 0.0, v0, false);
-    }
+    scaled_copy(c, r.s, V[0], 1.0 / rn);
     cv[0] = rn;
     int j = 0;
     while (j < m) {
-      const double hn = arnoldi_step(c, A, V, j, col.data(), reorth);
+      const double hn = arnoldi_step(c, A, V, j, col.data(), reorth, nullptr, nullptr, work ? &work->s : nullptr);
       st.matvecs++;
       st.iters++;
       for (int i = 0; i <= j + 1; i++) H[i + (size_t)ldh * j] = col[i];
@@ -396,7 +424,6 @@ namespace qqcd {
 
 MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spinor &x, const MixedParams &p,
                        Deflation *defl) {
-  if (p.basis == PREC_HALF) throw std::runtime_error("solve_mixed: the Krylov basis is FP64 or FP32");
   MixedStats ms;
   SchurOp A64(c, PREC_DOUBLE, 18, kappa);
   LinOp op64 = A64.op();
@@ -405,25 +432,33 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
   Prec op_prec = p.op_prec;
   std::unique_ptr<SchurOp> Alo(new SchurOp(c, op_prec, p.op_recon, kappa));
   std::unique_ptr<SpinorBuf> in_lo, out_lo;
+  // inner solves work in FP32 (or FP64); an FP16 basis is kept in storage
+  // only (inputs of the operator may then be FP16 basis vectors)
+  const Prec work = p.basis == PREC_HALF ? PREC_SINGLE : p.basis;
   auto make_inner = [&]() {
     LinOp o;
-    o.prec = p.basis;
-    if (op_prec == p.basis) {
-      o.apply = [&](const Spinor &a, Spinor &r) { (*Alo)(a, r); };
-    } else {
-      in_lo.reset(new SpinorBuf(c.g, op_prec));
-      out_lo.reset(new SpinorBuf(c.g, op_prec));
-      o.apply = [&](const Spinor &a, Spinor &r) {
+    o.prec = work;
+    if (p.basis == PREC_HALF) o.store = PREC_HALF;
+    in_lo.reset(new SpinorBuf(c.g, op_prec));
+    out_lo.reset(new SpinorBuf(c.g, op_prec));
+    o.apply = [&](const Spinor &a, Spinor &r) {
+      const Spinor *in = &a;
+      if (a.prec != op_prec) {
         spinor_copy_convert(c.g, a, in_lo->s, c.stream);
-        (*Alo)(in_lo->s, out_lo->s);
+        in = &in_lo->s;
+      }
+      if (r.prec == op_prec) {
+        (*Alo)(*in, r);
+      } else {
+        (*Alo)(*in, out_lo->s);
         spinor_copy_convert(c.g, out_lo->s, r, c.stream);
-      };
-    }
+      }
+    };
     return o;
   };
   LinOp inner = make_inner();
   double inner_tol = p.inner_tol;
-  SpinorBuf r(c.g, PREC_DOUBLE), rl(c.g, p.basis), el(c.g, p.basis);
+  SpinorBuf r(c.g, PREC_DOUBLE), rl(c.g, work), el(c.g, work);
   SolveStats dummy;
   const double bn = std::sqrt(c.blas->norm2(as_vec(b)));
   double rn = residual(c, op64, b, x, r.s, dummy);

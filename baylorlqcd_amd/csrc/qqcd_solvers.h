@@ -8,10 +8,17 @@
 
 namespace qqcd {
 
-// A linear operator on checkerboard-0 spinors of one precision.
+// A linear operator on checkerboard-0 spinors of precision prec.  store:
+// the precision in which the Krylov solvers keep their basis (-1: prec);
+// PREC_HALF with prec PREC_SINGLE keeps the basis and deflation vectors as
+// FP16 fields (half the bytes of every orthogonalisation pass) while every
+// new vector is formed and orthogonalised in FP32.  apply then also takes
+// FP16 input.
 struct LinOp {
   Prec prec = PREC_DOUBLE;
+  int store = -1;
   std::function<void(const Spinor &, Spinor &)> apply;
+  Prec basis() const { return store < 0 ? prec : (Prec)store; }
 };
 
 // The even-odd Schur operator (Hdbletm semantics) with storage (prec, recon).
@@ -133,6 +140,10 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
 // MixedParams from the C-ABI arguments (GMRES = (m, k); basis 0 FP64 / 1 FP32;
 // op_prec 0/1/2; inner_tol <= 0 picks 1e-2 (FP16), 1e-6 (FP32), 1e-10 (FP64))
 MixedParams mixed_params(const int GMRES[2], double tol, int basis, int op_prec, int op_recon, double inner_tol);
+
+// the library's trained network as a flexible right preconditioner (FP32
+// spinors, even checkerboard) on the context's links (qqcd_learned.cpp)
+LinOp nn_precond(Context &c);
 
 // unit source at 1-based global site src[4], Dirac d, colour col (both checkerboards zeroed first by the caller)
 void point_source_dev(Context &c, Spinor &b_e, Spinor &b_o, const int src[4], int d, int col);

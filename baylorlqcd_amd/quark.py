@@ -29,3 +29,34 @@ def fermprop(rwdir, be, bo, nkappa, kappa, cSW, inverter, coact, bc, resmax, ite
         _lib.ints(bc).ctypes.data_as(_lib._ip), float(resmax), int(itermin), ngm.ctypes.data_as(_lib._ip),
         int(vectype), _lib.dp(u), _lib.dp(xe), _lib.dp(xo), int(isignal), it.ctypes.data_as(_lib._ip)))
     return int(it[0])
+
+
+def propagator(nrhs, src, kappa, GMRES=(40, 16), tol=1e-10, op_prec=2, op_recon=12, method=0, solutions=False):
+    """The point-source propagator on the resident links: nrhs (<= 12)
+    spin-colour sources at the 1-based site src, the first by the mixed solver
+    (kept deflation space), the rest batched (method 0), one by one over the
+    deflation space (1) or one by one with the learned preconditioner (2).
+    Returns a dict with iters, res_full, times (first, rest, total), stats and,
+    with solutions=True, lists xe, xo of reference-layout solutions."""
+    from . import lattice
+    it = np.zeros(nrhs, np.int32)
+    res = np.zeros(nrhs)
+    times = np.zeros(3)
+    stats = np.zeros(8, np.int32)
+    xe = xo = None
+    pe = po = None
+    if solutions:
+        xe = [lattice.new_spinor().ravel(order="F") for _ in range(nrhs)]
+        xo = [lattice.new_spinor().ravel(order="F") for _ in range(nrhs)]
+        pe, _ke = _lib.dpp(xe)
+        po, _ko = _lib.dpp(xo)
+    _lib.check(_lib.lib().qqcd_propagator(
+        int(nrhs), _lib.ints(src).ctypes.data_as(_lib._ip), _lib.dp(np.ascontiguousarray(kappa, dtype=np.float64)),
+        _lib.ints(GMRES).ctypes.data_as(_lib._ip), float(tol), int(op_prec), int(op_recon), int(method), pe, po,
+        it.ctypes.data_as(_lib._ip), _lib.dp(res), _lib.dp(times), stats.ctypes.data_as(_lib._ip)))
+    keys = ("outer", "cycles", "batched_matvecs", "graph_replays", "fallback", "nb", "precond_dropped",
+            "first_outer")
+    out = {"iters": it, "res_full": res, "times": times, "stats": dict(zip(keys, (int(s) for s in stats)))}
+    if solutions:
+        out["xe"], out["xo"] = xe, xo
+    return out

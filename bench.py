@@ -37,7 +37,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from baylorlqcd_amd import _lib, diracops, disconloops, gauge, lattice, learned  # noqa: E402
+from baylorlqcd_amd import _lib, diracops, disconloops, gauge, lattice, learned, quark  # noqa: E402
 
 N = (32, 32, 32, 64)
 BC = (1, 1, 1, -1)
@@ -127,17 +127,7 @@ def solve_section(train_steps=1500):
     learned preconditioner trained on 16^3x32 configurations (Wilson kappa in
     [0.150, 0.157], pairs from plain GMRES-DR iterates), at Wilson kappa
     0.1565; plus the twisted-mass point kappa 0.1565, mu 0.005."""
-    lattice.finalize()
-    lattice.init((16, 16, 16, 32), BC)
-    t0 = time.perf_counter()
-    tr = learned.train(nconf=10, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=2, m=40, k=16, harvest_tol=1e-8,
-                       max_pairs=800, channels=(1, 8, 8, 8, 1), steps=train_steps, batch=8, lr=1e-3, seed=11)
-    train = {"lattice": [16, 16, 16, 32], "configs": 10, "pairs": tr["pairs"], "adam_steps": train_steps,
-             "batch": 8, "lr": 1e-3, "loss_before": round(tr["loss_before"], 4),
-             "loss_after": round(tr["loss_after"], 4), "harvest_s": round(tr["harvest_s"], 2),
-             "train_s": round(tr["train_s"], 2), "wall_s": round(time.perf_counter() - t0, 2)}
-    print(f"[bench] training {train}", file=sys.stderr, flush=True)
-    lattice.finalize()
+    train = train_network(train_steps)
     lattice.init(N, BC)
     co, plaq = heatbath_field()
     nn_ms = learned.bench_nn(2, 10)
@@ -149,6 +139,8 @@ def solve_section(train_steps=1500):
     print(f"[bench] Wilson mixed FP32: {mixed}", file=sys.stderr, flush=True)
     mixed16 = learned.bench(1, wil, GMRES=(40, 16), op_prec=2, op_recon=12)
     print(f"[bench] Wilson mixed FP16 operator: {mixed16}", file=sys.stderr, flush=True)
+    mixed16h = bench_solve(2, wil, 2, 12)
+    print(f"[bench] Wilson mixed FP16 operator, FP16 basis: {mixed16h}", file=sys.stderr, flush=True)
     lrn = learned.bench(2, wil, GMRES=(40, 16), op_prec=1, op_recon=12)
     print(f"[bench] Wilson mixed FP32 + learned: {lrn}", file=sys.stderr, flush=True)
     tm = (0.1565, 0.005)
@@ -156,12 +148,13 @@ def solve_section(train_steps=1500):
     print(f"[bench] twisted plain FP64: {tplain}", file=sys.stderr, flush=True)
     tmixed = bench_solve(1, tm, 1, 12)
     print(f"[bench] twisted mixed FP32: {tmixed}", file=sys.stderr, flush=True)
-    best = min((mixed, mixed16, lrn), key=lambda r: r["seconds"])
+    best = min((mixed, mixed16, mixed16h, lrn), key=lambda r: r["seconds"])
     return {"lattice": list(N), "beta": 6.0, "heatbath_sweeps": 200, "plaquette": round(plaq, 5),
             "source": "point, origin, spin 1 colour 1",
             "target": "||b - M x|| / ||b|| < 1e-10 (FP64, full system)",
             "wilson_kappa_0.1565": {"plain_fp64_gmresdr_40_16": plain, "mixed_fp32": mixed,
-                                    "mixed_fp16_operator": mixed16, "mixed_fp32_learned": lrn,
+                                    "mixed_fp16_operator": mixed16, "mixed_fp16_operator_fp16_basis": mixed16h,
+                                    "mixed_fp32_learned": lrn,
                                     "speedup_best_vs_plain": round(plain["seconds"] / best["seconds"], 2),
                                     "speedup_learned_vs_plain": round(plain["seconds"] / lrn["seconds"], 2),
                                     "iteration_ratio_learned_vs_mixed": round(lrn["iterations"] /
@@ -172,6 +165,60 @@ def solve_section(train_steps=1500):
             "learned_preconditioner": {"training": train, "network_ms_per_application": round(nn_ms, 3),
                                        "layers": "4 parallel-transport convolutions, channels 1-8-8-8-1, "
                                                  "BF16 MFMA, FP32 accumulation"}}
+
+
+def train_network(train_steps):
+    """The learned preconditioner trained on ten 16^3x32 beta=6.0 heatbath
+    configurations (Wilson kappa uniform in [0.150, 0.157], pairs from plain
+    GMRES-DR iterates); returns the training record."""
+    lattice.finalize()
+    lattice.init((16, 16, 16, 32), BC)
+    t0 = time.perf_counter()
+    tr = learned.train(nconf=10, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=2, m=40, k=16, harvest_tol=1e-8,
+                       max_pairs=800, channels=(1, 8, 8, 8, 1), steps=train_steps, batch=8, lr=1e-3, seed=11)
+    train = {"lattice": [16, 16, 16, 32], "configs": 10, "pairs": tr["pairs"], "adam_steps": train_steps,
+             "batch": 8, "lr": 1e-3, "loss_before": round(tr["loss_before"], 4),
+             "loss_after": round(tr["loss_after"], 4), "harvest_s": round(tr["harvest_s"], 2),
+             "train_s": round(tr["train_s"], 2), "wall_s": round(time.perf_counter() - t0, 2)}
+    print(f"[bench] training {train}", file=sys.stderr, flush=True)
+    lattice.finalize()
+    return train
+
+
+def propagator_section(train_steps=1500, n5=(48, 48, 48, 96), nlearned=2):
+    """The full propagator at scale: a 48^3x96 beta=6.0 heatbath field,
+    twisted mass kappa 0.1563, mu 0.0015, the 12 spin-colour point sources at
+    the origin to a 1e-10 true residual each.  The first by the mixed solver
+    (FP16 storage, 12-real links, FP32 basis, GMRES-DR(40,16) inner solves,
+    FP64 reliable updates), the other 11 batched through GMRES-Proj(24) over
+    its deflation space; against plain FP64 GMRES-DR(40,16) on the first
+    source; and the learned preconditioner on the first `nlearned` sources."""
+    train = train_network(train_steps)
+    lattice.init(n5, BC)
+    co, plaq = heatbath_field()
+    tm = (0.1563, 0.0015)
+    plain = learned.bench(0, tm, GMRES=(40, 16))
+    print(f"[bench] propagator: plain FP64 GMRES-DR(40,16), source 1: {plain}", file=sys.stderr, flush=True)
+    b = quark.propagator(12, (1, 1, 1, 1), tm, GMRES=(40, 16), tol=1e-10, op_prec=2, op_recon=12, method=0)
+    print(f"[bench] propagator batched: iters {b['iters']} res {b['res_full'].max():.2e} times {b['times']} "
+          f"stats {b['stats']}", file=sys.stderr, flush=True)
+    lr = quark.propagator(nlearned, (1, 1, 1, 1), tm, GMRES=(40, 16), tol=1e-10, op_prec=2, op_recon=12, method=2)
+    print(f"[bench] propagator learned: iters {lr['iters']} res {lr['res_full'].max():.2e} times {lr['times']} "
+          f"stats {lr['stats']}", file=sys.stderr, flush=True)
+    return {"lattice": list(n5), "beta": 6.0, "plaquette": round(plaq, 5), "kappa": tm[0], "mu": tm[1],
+            "sources": "12 spin-colour point sources at the origin", "target": "1e-10 true residual each",
+            "plain_fp64_gmresdr_40_16_source1": plain,
+            "mixed_fp16_12_batched": {"iterations": [int(v) for v in b["iters"]],
+                                      "max_true_residual": float(b["res_full"].max()),
+                                      "seconds_first": round(float(b["times"][0]), 3),
+                                      "seconds_other_11": round(float(b["times"][1]), 3),
+                                      "seconds_total": round(float(b["times"][2]), 3), "stats": b["stats"]},
+            "learned_first_sources": {"iterations": [int(v) for v in lr["iters"]],
+                                      "max_true_residual": float(lr["res_full"].max()),
+                                      "seconds": [round(float(v), 3) for v in lr["times"]], "stats": lr["stats"]},
+            "plain_seconds_x12_estimate": round(12 * plain["seconds"], 1),
+            "speedup_vs_plain_x12": round(12 * plain["seconds"] / float(b["times"][2]), 2),
+            "training": train}
 
 
 def noise_section(nrhs=64, seq=3):
@@ -222,6 +269,8 @@ def main():
     ap.add_argument("--noise-rhs", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true", help="batched solver without graph replays (profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="also time a 1 GiB plain copy kernel")
+    ap.add_argument("--propagator", action="store_true",
+                    help="also run the 48^3x96 twisted-mass 12-source propagator (several minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
@@ -284,6 +333,9 @@ def main():
         if args.no_graph:
             disconloops.batch_set(use_graph=0)
         noise = noise_section(args.noise_rhs)
+    propagator = None
+    if args.propagator:
+        propagator = propagator_section(args.train_steps)
 
     line = {
         "metric": "Dslash HBM GB/s (algorithmic bytes / kernel time), 32^3x64 Wilson hopping term",
@@ -309,6 +361,8 @@ def main():
     }
     if copy:
         line["copy_probe"] = copy
+    if propagator:
+        line["propagator"] = propagator
     print(json.dumps(line))
     lattice.finalize()
 

@@ -160,7 +160,9 @@ int qqcd_fermprop(const double *be, const double *bo, const double kappa[2], int
 /* Mixed-precision solve of the even-odd system phi (gblclr=1 array) to
  * ||phi - M x|| < tol ||phi||: FP64 solution, true residuals and reliable
  * updates; inner GMRES-DR(GMRES[0], GMRES[1]) and then GMRES-Proj over its
- * deflation space on a basis of basis_prec (0 FP64, 1 FP32) with the
+ * deflation space on a basis of basis_prec (0 FP64, 1 FP32, 2 FP16: the
+ * basis and deflation vectors stored as FP16 fields with per-site norms,
+ * every new vector formed and orthogonalised in FP32) with the
  * operator in (op_prec, op_recon) storage (0 FP64, 1 FP32, 2 FP16; 18 or 12
  * link reals).  inner_tol <= 0 picks a default per op_prec.  keep_deflation
  * = 1 reuses the deflation space of the previous mixed solve (same gauge
@@ -252,6 +254,28 @@ int qqcd_loopops(int ksub, const double *xe, const double *xo, const double *u, 
  * :3079-3143).  Arguments otherwise as qqcd_loopops. */
 int qqcd_loopsum(int ksub, const double *xe, const double *xo, const double *u, const double *be, const double *bo,
                  const int *bc, const int *io, double cur_scale, double *Jtime);
+
+/* A full propagator on the resident links: nrhs (1..12) point sources at the
+ * 1-based site src (spin r/3 + 1, colour r%3 + 1 as pointsource numbers them,
+ * qqcd-test/cfgsprops/quark/quark.f90:46-108), each solved to
+ * ||b - M x|| < tol ||b|| on the whole lattice (Wilson or twisted mass,
+ * kappa = {kappa, mu}).  The first by the mixed solver (inner GMRES-DR(m, k)
+ * with op_prec 1 (FP32) / 2 (FP16) storage and op_recon 12 / 18 links, FP32
+ * basis, FP64 reliable updates), keeping its deflation space; the rest
+ * (method 0) together through the batched GMRES-Proj(m - k) over it, (1) one
+ * by one through the mixed solver with GMRES-Proj inner solves over it, or
+ * (2) one by one with the learned network as flexible right preconditioner
+ * (the unpreconditioned solver takes over when an outer step stalls).
+ * Replaces the loop over fermprop (quark.f90:458-842) of a propagator run;
+ * the reference solves each right-hand side on its own.  xe/xo: optional
+ * arrays of nrhs host pointers (reference layout) for the solutions, may be
+ * NULL.  iters[r], res_full[r] per source; times = {first, rest, total}
+ * seconds; stats[8] = {outer steps, GMRES cycles, batched operator
+ * applications, graph replays, fallbacks, RHS block, preconditioner drops,
+ * outer steps of the first solve}. */
+int qqcd_propagator(int nrhs, const int src[4], const double kappa[2], const int GMRES[2], double tol, int op_prec,
+                    int op_recon, int method, double *const *xe, double *const *xo, int *iters, double *res_full,
+                    double *times, int *stats);
 
 /* The 16 local trace estimators of one source: eta and x on both
  * checkerboards (reference layout); traces as in qqcd_noise_loop for
@@ -352,7 +376,8 @@ int qqcd_schur_canonical(const double *psi, double *out, const double kappa[2], 
 
 /* Time one device-resident solve of M x = point source (1-based site src,
  * Dirac spin, colour) on the resident gauge field: method 0 plain FP64
- * GMRES-DR(m,k), method 1 the mixed solver of qqcd_solve_mixed.  *seconds
+ * GMRES-DR(m,k), method 1 the mixed solver of qqcd_solve_mixed (FP32
+ * basis), method 2 the same with the FP16-stored basis.  *seconds
  * covers source preparation, solve and reconstruction; *true_res is the
  * FP64 ||b - M x|| / ||b|| of the full system. */
 int qqcd_bench_solve(int method, const int src[4], int spin, int colour, const double kappa[2], const int GMRES[2],
@@ -378,6 +403,13 @@ int qqcd_autotune_multi(int prec, int recon, int nrhs, int *nb, int *order, doub
 /* Time a plain device-to-device copy kernel (read + write `bytes` each
  * launch): the calibration point for hardware byte counters. */
 int qqcd_bench_copy(double bytes, int iters, double *ms);
+
+/* Kernel time (ms, mean of niter) of one fused Krylov pass over k (1..16)
+ * checkerboard vectors of the current lattice in prec 0 (FP64) / 1 (FP32):
+ * op 0 the multi-dot <v_i, w>, op 1 the multi-axpy w -= sum h_i v_i with
+ * ||w||^2; grid workgroups of 256 threads.  Tuning aid; no reference
+ * counterpart. */
+int qqcd_bench_blas(int prec, int k, int op, int grid, int niter, double *ms);
 
 /* Toggle the XCD-aware workgroup order of the Dslash (default on). */
 int qqcd_set_xcd_remap(int on);

@@ -63,8 +63,9 @@ def test_heatbath_strong_coupling():
         lattice.finalize()
 
 
-@pytest.mark.parametrize("op_prec,recon", [(1, 12), (1, 18), (2, 12)])
-def test_mixed_precision_config1(op_prec, recon):
+@pytest.mark.parametrize("basis,op_prec,recon", [(1, 1, 12), (1, 1, 18), (1, 2, 12), (2, 2, 12), (2, 1, 18)])
+def test_mixed_precision_config1(basis, op_prec, recon):
+    # basis 2: the Krylov basis and deflation vectors stored as FP16 fields
     from oracle import solvers
     n, bc = (8, 8, 8, 16), (1, 1, 1, -1)
     g = np.load(os.path.join(GOLD, "golden_8x8x8x16.npz"))
@@ -85,7 +86,7 @@ def test_mixed_precision_config1(op_prec, recon):
         x = np.zeros_like(phi)
         it = np.zeros(1, np.int32)
         _lib.check(_lib.lib().qqcd_solve_mixed(
-            _lib.dp(phi), _lib.dp(x), _lib.ints((40, 10)).ctypes.data_as(_lib._ip), 1e-11, 1, op_prec, recon, 0.0,
+            _lib.dp(phi), _lib.dp(x), _lib.ints((40, 10)).ctypes.data_as(_lib._ip), 1e-11, basis, op_prec, recon, 0.0,
             0, _lib.dp(u), _lib.dp(co), _lib.dp(np.array(kap)), _lib.ints(bc).ctypes.data_as(_lib._ip),
             it.ctypes.data_as(_lib._ip)))
         st = inverters.last_solve_stats()
@@ -93,7 +94,8 @@ def test_mixed_precision_config1(op_prec, recon):
         xf = solvers.reconstruct(n, U, bc, kap, b, xe.ravel())
         r = b - O.full_m(n, U, bc, [1] * 4, [1] * 4, kap, xf)
         res = np.linalg.norm(r) / np.linalg.norm(b)
-        print(f"mixed op=({op_prec},{recon}): {int(it[0])} inner iterations, {st['cycles']} outer, residual {res:.2e}")
+        print(f"mixed basis={basis} op=({op_prec},{recon}): {int(it[0])} inner iterations, {st['cycles']} outer, "
+              f"residual {res:.2e}")
         assert res < 1e-10
         assert float(np.max(np.abs(xe - g["gmresdr_k150_xe"])) / np.max(np.abs(g["gmresdr_k150_xe"]))) < 1e-8
     finally:
@@ -101,14 +103,14 @@ def test_mixed_precision_config1(op_prec, recon):
 
 
 def test_bench_solve_entry_point():
-    """Device-resident timing entry point: both methods reach 1e-10 on a
+    """Device-resident timing entry point: every method reaches 1e-10 on a
     16^3x32 heatbath field (beta 6, twisted kappa 0.150, mu 0.01)."""
     n = (16, 16, 16, 32)
     lattice.init(n, (1, 1, 1, -1))
     try:
         co = O.coact_array()
         diracops.gauge_heatbath(6.0, 100, 5, 1, co)
-        for method, prec in ((0, 0), (1, 1), (1, 2)):
+        for method, prec in ((0, 0), (1, 1), (1, 2), (2, 2)):
             sec, res = np.zeros(1), np.zeros(1)
             its, outer = np.zeros(1, np.int32), np.zeros(1, np.int32)
             _lib.check(_lib.lib().qqcd_bench_solve(

@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from baylorlqcd_amd import diracops, disconloops, gauge, lattice
+from baylorlqcd_amd import diracops, disconloops, gauge, lattice, learned, quark
 
 pytestmark = pytest.mark.gpu
 
@@ -259,3 +259,30 @@ def test_noise_loop_traces_and_graph(cfg1):
     # the reference's operators as nsaverage forms them (currents times kappa)
     assert rel(a["loops"][:, 0, :], wloop) < 1e-8
     assert np.all(a["loops"][:, 1:] == 0)
+
+
+@pytest.mark.parametrize("method,kappa", [(0, (0.150, 0.0)), (1, (0.156, 0.005)), (2, (0.150, 0.0))])
+def test_propagator(cfg1, method, kappa):
+    # all twelve spin-colour sources at the origin; the first is the golden
+    # point source of the reference's GMRES-DR solve (Wilson kappa 0.150)
+    n, bc, U, u, co, g = cfg1
+    if method == 2:
+        # untrained network: kappa^2 times the identity (point Jacobi) as the preconditioner
+        learned.init((1, 8, 8, 8, 1), scale=kappa[0] ** 2, noise=0.0, seed=1)
+    print(f"[multi] propagator method={method} kappa={kappa}", flush=True)
+    out = quark.propagator(12, (1, 1, 1, 1), kappa, GMRES=(40, 16), tol=1e-10, op_prec=2, op_recon=12,
+                           method=method, solutions=True)
+    print("[multi] propagator", out["iters"], out["times"], out["stats"], flush=True)
+    assert np.all(out["res_full"] < 1e-10)
+    odd = O.parity_mask(n)
+    V = int(np.prod(n))
+    for r in (0, 4, 11):
+        b = np.zeros((V, 4, 3), dtype=np.complex128)
+        b[0, r // 3, r % 3] = 1.0
+        x = O.spinor_ref_to_canon(n, out["xe"][r], out["xo"][r])
+        res = full_rel_res(n, bc, U, kappa, b, x)
+        assert res < 1e-10, (r, res)
+        assert abs(res - out["res_full"][r]) < 1e-11
+        if r == 0 and kappa == (0.150, 0.0):
+            assert rel(x[~odd], g["gmresdr_k150_xe"]) < 1e-8
+            assert rel(x[odd], g["gmresdr_k150_xo"]) < 1e-8
