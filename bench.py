@@ -167,6 +167,21 @@ def solve_section(train_steps=1500):
                                                  "BF16 MFMA, FP32 accumulation"}}
 
 
+PROFILE_SOLVES = {"plain": (0, 0), "fp32": (1, 1), "fp16": (1, 2), "fp16basis": (2, 2)}
+
+
+def profile_solve_section(which):
+    """One solve of the solve section alone (no training), for kernel traces:
+    the 32^3x64 heatbath field, Wilson kappa 0.1565, point source, 1e-10."""
+    lattice.finalize()
+    lattice.init(N, BC)
+    heatbath_field()
+    method, op_prec = PROFILE_SOLVES[which]
+    r = bench_solve(method, (0.1565, 0.0), op_prec, 12 if op_prec else 18)
+    print(f"[bench] profile solve {which}: {r}", file=sys.stderr, flush=True)
+    return {"which": which, **r}
+
+
 def train_network(train_steps):
     """The learned preconditioner trained on ten 16^3x32 beta=6.0 heatbath
     configurations (Wilson kappa uniform in [0.150, 0.157], pairs from plain
@@ -269,6 +284,8 @@ def main():
     ap.add_argument("--noise-rhs", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true", help="batched solver without graph replays (profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="also time a 1 GiB plain copy kernel")
+    ap.add_argument("--profile-solve", choices=sorted(PROFILE_SOLVES),
+                    help="also run one 32^3x64 solve alone (for kernel traces)")
     ap.add_argument("--propagator", action="store_true",
                     help="also run the 48^3x96 twisted-mass 12-source propagator (several minutes)")
     args = ap.parse_args()
@@ -333,6 +350,9 @@ def main():
         if args.no_graph:
             disconloops.batch_set(use_graph=0)
         noise = noise_section(args.noise_rhs)
+    psolve = None
+    if args.profile_solve:
+        psolve = profile_solve_section(args.profile_solve)
     propagator = None
     if args.propagator:
         propagator = propagator_section(args.train_steps)
@@ -363,6 +383,8 @@ def main():
         line["copy_probe"] = copy
     if propagator:
         line["propagator"] = propagator
+    if psolve:
+        line["profile_solve"] = psolve
     print(json.dumps(line))
     lattice.finalize()
 
