@@ -40,6 +40,12 @@ class Blas {
   double norm2(const Vec &a);
   // w -= sum_i h_i v_i ; returns ||w||^2 when want_norm
   double multi_axpy(int k, const std::complex<double> *h, const Vec *v, Vec &w, bool want_norm);
+  // Device-side variants (no host synchronisation): dout[2i], dout[2i+1] =
+  // <v_i, w>; w -= sum_i h_i v_i with h_i = dcoef[2i] + i dcoef[2i+1] read
+  // from device memory, ||w||^2 into *dnorm2 (if given); w *= 1/sqrt(*dnorm2).
+  void multi_dot_dev(int k, const Vec *v, const Vec &w, double *dout);
+  void multi_axpy_dev(int k, const double *dcoef, const Vec *v, Vec &w, double *dnorm2);
+  void rscale_dev(Vec &w, const double *dnorm2);
   // y = a x + b y ; returns ||y||^2 when want_norm
   double axpby(std::complex<double> a, const Vec &x, std::complex<double> b, Vec &y, bool want_norm = false);
   void copy(const Vec &x, Vec &y);

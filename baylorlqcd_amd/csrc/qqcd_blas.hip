@@ -182,6 +182,52 @@ __global__ void __launch_bounds__(RB) k_maxpy(long nu, PtrPack v, CoefPack h, vo
   if (NORM) block_sum<1>(acc, part + blockIdx.x);
 }
 
+// the same with the coefficients read from device memory, h_k = coef[2k] + i coef[2k+1]
+// (the multi-dot's results), so that no host round trip separates the two passes
+template <typename TV, typename TW, int K, bool NORM>
+__global__ void __launch_bounds__(RB) k_maxpy_d(long nu, PtrPack v, const double *coef, void *w, unsigned stride,
+                                               double *part) {
+  constexpr int U = UnitOf<TV>::U;
+  double hr[K], hi[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    hr[k] = coef[2 * k];
+    hi[k] = coef[2 * k + 1];
+  }
+  double acc[1] = {0};
+  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
+    double2 b[U];
+    C2<TW>::template ld<U>(w, nullptr, stride, i, b);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double2 a[U];
+      C2<TV>::template ld<U>(v.p[k], v.n[k], stride, i, a);
+#pragma unroll
+      for (int u = 0; u < U; u++) b[u] = csub_mul(b[u], hr[k], hi[k], a[u]);
+    }
+    C2<TW>::template st<U>(w, i, b);
+    if (NORM)
+#pragma unroll
+      for (int u = 0; u < U; u++) acc[0] += b[u].x * b[u].x + b[u].y * b[u].y;
+  }
+  if (NORM) block_sum<1>(acc, part + blockIdx.x);
+}
+
+// w *= 1 / sqrt(*n2) (0 when *n2 == 0), the norm read from device memory
+template <typename T>
+__global__ void __launch_bounds__(RB) k_rscale(long nu, void *w, const double *n2, unsigned stride) {
+  constexpr int U = UnitOf<T>::U;
+  const double q = *n2;
+  const double s = q > 0 ? 1.0 / sqrt(q) : 0.0;
+  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
+    double2 b[U];
+    C2<T>::template ld<U>(w, nullptr, stride, i, b);
+#pragma unroll
+    for (int u = 0; u < U; u++) b[u] = make_double2(s * b[u].x, s * b[u].y);
+    C2<T>::template st<U>(w, i, b);
+  }
+}
+
 // ---- y = a x + b y (+ norm of y) ----
 __device__ __forceinline__ double2 caxpby(double ar, double ai, double2 a, double br, double bi, double2 b) {
   return make_double2(ar * a.x - ai * a.y + br * b.x - bi * b.y, ar * a.y + ai * a.x + br * b.y + bi * b.x);
@@ -409,6 +455,75 @@ double Blas::multi_axpy(int k, const std::complex<double> *h, const Vec *v, Vec 
   }
   if (want_norm && k == 0) nrm = norm2(w);
   return nrm;
+}
+
+void Blas::multi_dot_dev(int k, const Vec *v, const Vec &w, double *dout) {
+  if (k > kmax) throw std::runtime_error("multi_dot: too many vectors");
+  long off = 0;
+  for (int k0 = 0; k0 < k; k0 += KMAXP) {
+    const int kk = std::min(KMAXP, k - k0);
+    PtrPack pp{};
+    for (int q = 0; q < kk; q++) {
+      if (v[k0 + q].n != w.n || v[k0 + q].prec != v[k0].prec) throw std::runtime_error("multi_dot: mixed vectors");
+      check_half(v[k0 + q]);
+      pp.p[q] = v[k0 + q].data;
+      pp.n[q] = v[k0 + q].norm;
+    }
+    double *pt = part + off;
+    tsw(v[k0].prec, w.prec, [&](auto ta, auto tb) {
+      TYPES_OF(ta, tb)
+      with_k(kk, [&](auto K) { launch_mdot<decltype(K)::value, TA, TB>(stream, w, pp, pt); });
+    });
+    QQCD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_final, dim3(2 * kk), dim3(RB), 0, stream, RG, 2 * kk, pt, dout + 2 * k0);
+    QQCD_HIP(hipGetLastError());
+    off += (long)RG * 2 * kk;
+  }
+}
+
+void Blas::multi_axpy_dev(int k, const double *dcoef, const Vec *v, Vec &w, double *dnorm2) {
+  for (int k0 = 0; k0 < k; k0 += KMAXP) {
+    const int kk = std::min(KMAXP, k - k0);
+    const bool last = k0 + KMAXP >= k;
+    PtrPack pp{};
+    for (int q = 0; q < kk; q++) {
+      if (v[k0 + q].n != w.n || v[k0 + q].prec != v[k0].prec) throw std::runtime_error("multi_axpy: mixed vectors");
+      check_half(v[k0 + q]);
+      pp.p[q] = v[k0 + q].data;
+      pp.n[q] = v[k0 + q].norm;
+    }
+    const bool nm = dnorm2 && last;
+    const double *cf = dcoef + 2 * k0;
+    tsw(v[k0].prec, w.prec, [&](auto ta, auto tb) {
+      TYPES_OF(ta, tb)
+      with_k(kk, [&](auto K) {
+        constexpr int KK = decltype(K)::value;
+        if (nm)
+          hipLaunchKernelGGL((k_maxpy_d<TA, TB, KK, true>), dim3(RG), dim3(RB), 0, stream, units<TA>(w.n), pp, cf,
+                             w.data, stride_of(w), part);
+        else
+          hipLaunchKernelGGL((k_maxpy_d<TA, TB, KK, false>), dim3(RG), dim3(RB), 0, stream, units<TA>(w.n), pp, cf,
+                             w.data, stride_of(w), part);
+      });
+    });
+    QQCD_HIP(hipGetLastError());
+    if (nm) {
+      hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, stream, RG, 1, part, dnorm2);
+      QQCD_HIP(hipGetLastError());
+    }
+  }
+}
+
+void Blas::rscale_dev(Vec &w, const double *dnorm2) {
+  if (w.prec == PREC_DOUBLE)
+    hipLaunchKernelGGL(k_rscale<double>, dim3(RG), dim3(RB), 0, stream, units<double>(w.n), w.data, dnorm2,
+                       stride_of(w));
+  else if (w.prec == PREC_SINGLE)
+    hipLaunchKernelGGL(k_rscale<float>, dim3(RG), dim3(RB), 0, stream, units<float>(w.n), w.data, dnorm2,
+                       stride_of(w));
+  else
+    throw std::runtime_error("rscale: FP64 or FP32 vector expected");
+  QQCD_HIP(hipGetLastError());
 }
 
 double Blas::axpby(std::complex<double> a, const Vec &x, std::complex<double> b, Vec &y, bool want_norm) {

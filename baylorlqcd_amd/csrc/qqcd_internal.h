@@ -141,5 +141,8 @@ void ref_to_gauge(const Geom &g, const double *dref, Gauge &U, hipStream_t s);
 void canon_to_gauge(const Geom &g, const double *dcanon, Gauge &U, hipStream_t s);
 // out = scale * in across precisions (FP16 fields carry their per-site norm)
 void spinor_copy_convert(const Geom &g, const Spinor &in, Spinor &out, hipStream_t s, double scale = 1.0);
+// out = scale * in / sqrt(*dn2) (0 when *dn2 == 0), dn2 in device memory
+void spinor_convert_rnorm(const Geom &g, const Spinor &in, Spinor &out, const double *dn2, hipStream_t s,
+                          double scale = 1.0);
 
 }  // namespace qqcd

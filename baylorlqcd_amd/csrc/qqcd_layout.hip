@@ -215,11 +215,16 @@ __global__ void k_gauge(const LayoutArgs a, int src_ref, int nc, const double *s
 
 template <typename A, typename B>
 __global__ void k_convert(int Vh, long stride, const void *in, const float *ninn, void *out, float *nout,
-                          double scale) {
+                          double scale, const double *dn2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Vh) return;
   double v[24];
   Put<A>::get(in, ninn, stride, i, v);
+  if (dn2) {
+    // scale by 1 / sqrt(*dn2), a norm left in device memory by a reduction
+    const double q = *dn2;
+    scale *= q > 0 ? 1.0 / sqrt(q) : 0.0;
+  }
   if (scale != 1.0)
     for (int k = 0; k < 24; k++) v[k] *= scale;
   Put<B>::put(out, nout, stride, i, v);
@@ -287,11 +292,16 @@ void ref_to_gauge(const Geom &g, const double *dref, Gauge &U, hipStream_t s) { 
 void canon_to_gauge(const Geom &g, const double *dcn, Gauge &U, hipStream_t s) { gauge_any(g, 0, dcn, U, s); }
 
 void spinor_copy_convert(const Geom &g, const Spinor &in, Spinor &out, hipStream_t s, double scale) {
+  spinor_convert_rnorm(g, in, out, nullptr, s, scale);
+}
+
+void spinor_convert_rnorm(const Geom &g, const Spinor &in, Spinor &out, const double *dn2, hipStream_t s,
+                          double scale) {
   const int nb = (int)((g.Vh + 255) / 256);
   const int Vh = (int)g.Vh;
 #define CV(A, B)                                                                                                  \
   hipLaunchKernelGGL((k_convert<A, B>), dim3(nb), dim3(256), 0, s, Vh, g.stride, in.data, in.norm, out.data, out.norm, \
-                     scale)
+                     scale, dn2)
   if (in.prec == PREC_DOUBLE) {
     if (out.prec == PREC_DOUBLE) CV(double, double);
     else if (out.prec == PREC_SINGLE) CV(double, float);

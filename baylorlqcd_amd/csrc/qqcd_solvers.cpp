@@ -99,6 +99,67 @@ static double arnoldi_step(Context &c, LinOp &A, std::vector<Spinor> &V, int j, 
   return hn;
 }
 
+// ---- synchronisation-free Arnoldi steps ----------------------------------
+// Without the DGKS test (one Gram-Schmidt pass), an Arnoldi step needs no
+// host round trip: the multi-dot leaves <V_i, w> in device memory, the
+// multi-axpy reads its coefficients from there and leaves ||w||^2, and the
+// normalisation reads that.  The host collects the Hessenberg columns every
+// few steps (and at the end of a cycle) for the least-squares estimate, so
+// the device runs ahead of the host by up to kCheck steps; a cycle may take
+// up to kCheck - 1 steps more than it needed.
+static constexpr int kCheck = 4;
+
+struct DevHess {
+  int m, ld;              // ld = 2 (m + 1) doubles per column
+  double *d = nullptr;    // m columns of complex <V_i, w>, then m squared norms
+  double *h = nullptr;    // pinned host copy
+  explicit DevHess(int m_) : m(m_), ld(2 * (m_ + 1)) {
+    const size_t n = (size_t)(ld + 1) * m;
+    QQCD_HIP(hipMalloc(&d, sizeof(double) * n));
+    QQCD_HIP(hipHostMalloc(&h, sizeof(double) * n, hipHostMallocDefault));
+  }
+  ~DevHess() {
+    if (d) (void)hipFree(d);
+    if (h) (void)hipHostFree(h);
+  }
+  DevHess(const DevHess &) = delete;
+  DevHess &operator=(const DevHess &) = delete;
+  double *col(int j) { return d + (size_t)ld * j; }
+  double *n2(int j) { return d + (size_t)ld * m + j; }
+  // Hessenberg columns [j0, j1) into H; returns the first column whose new
+  // vector vanished (breakdown), or -1
+  int fetch(hipStream_t s, int j0, int j1, std::vector<zd> &H, int ldh) {
+    const size_t n = (size_t)(ld + 1) * m;
+    QQCD_HIP(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    QQCD_HIP(hipStreamSynchronize(s));
+    int jb = -1;
+    for (int j = j0; j < j1; j++) {
+      for (int i = 0; i <= j; i++) H[i + (size_t)ldh * j] = zd(h[(size_t)ld * j + 2 * i], h[(size_t)ld * j + 2 * i + 1]);
+      const double q = h[(size_t)ld * m + j];
+      H[j + 1 + (size_t)ldh * j] = q > 0 ? std::sqrt(q) : 0.0;
+      if (!(q > 0) && jb < 0) jb = j;
+    }
+    return jb;
+  }
+};
+
+static void arnoldi_step_dev(Context &c, LinOp &A, std::vector<Spinor> &V, int j, DevHess &dh, LinOp *M,
+                             std::vector<Spinor> *Z, Spinor *work) {
+  Spinor &ws = work ? *work : V[j + 1];
+  if (M) {
+    M->apply(V[j], (*Z)[j]);
+    A.apply((*Z)[j], ws);
+  } else {
+    A.apply(V[j], ws);
+  }
+  Vec w = vv(ws);
+  std::vector<Vec> Vb = vecs(V, j + 1);
+  c.blas->multi_dot_dev(j + 1, Vb.data(), w, dh.col(j));
+  c.blas->multi_axpy_dev(j + 1, dh.col(j), Vb.data(), w, dh.n2(j));
+  if (work) spinor_convert_rnorm(c.g, ws, V[j + 1], dh.n2(j), c.stream);
+  else c.blas->rscale_dev(w, dh.n2(j));
+}
+
 // dst = s * src across precisions (an FP16 destination through the conversion)
 static void scaled_copy(Context &c, const Spinor &src, Spinor &dst, double s) {
   if (dst.prec == PREC_HALF || src.prec == PREC_HALF) {
@@ -229,6 +290,8 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
   }
   std::unique_ptr<SpinorBuf> work;
   if (vp != p) work.reset(new SpinorBuf(c.g, p));
+  std::unique_ptr<DevHess> dh;
+  if (!reorth) dh.reset(new DevHess(m));
   std::vector<zd> Zmat;
   SpinorBuf r(c.g, p);
   double rn = residual(c, A, b, x, r.s, st);
@@ -248,7 +311,24 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
   std::vector<zd> Hnew, cnew;
   for (;;) {
     // Arnoldi
+    int jchk = j;
     while (j < m) {
+      if (dh) {
+        arnoldi_step_dev(c, A, V, j, *dh, M, M ? &Z : nullptr, work ? &work->s : nullptr);
+        st.matvecs++;
+        st.iters++;
+        j++;
+        if (j < m && st.iters < maxiter && j - jchk < kCheck) continue;
+        const int jb = dh->fetch(c.stream, jchk, j, H, ldh);
+        jchk = j;
+        if (jb >= 0) {
+          j = jb + 1;  // the basis ended at column jb
+          break;
+        }
+        const double est = dense::lsq(j + 1, j, H.data(), ldh, cv.data(), y.data());
+        if ((est < tol * r0 && st.iters >= itermin) || st.iters >= maxiter) break;
+        continue;
+      }
       const double hn = arnoldi_step(c, A, V, j, col.data(), reorth, M, M ? &Z : nullptr, work ? &work->s : nullptr);
       st.matvecs++;
       st.iters++;
@@ -318,6 +398,8 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
   alloc_basis(c, V, m + 1, vp);
   std::unique_ptr<SpinorBuf> work;
   if (vp != p) work.reset(new SpinorBuf(c.g, p));
+  std::unique_ptr<DevHess> dh;
+  if (!reorth) dh.reset(new DevHess(m));
   SpinorBuf r(c.g, p);
   double rn = residual(c, A, b, x, r.s, st);
   const double r0 = rn;
@@ -345,8 +427,24 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
     std::fill(cv.begin(), cv.end(), zd(0.0));
     scaled_copy(c, r.s, V[0], 1.0 / rn);
     cv[0] = rn;
-    int j = 0;
+    int j = 0, jchk = 0;
     while (j < m) {
+      if (dh) {
+        arnoldi_step_dev(c, A, V, j, *dh, nullptr, nullptr, work ? &work->s : nullptr);
+        st.matvecs++;
+        st.iters++;
+        j++;
+        if (j < m && st.iters < maxiter && j - jchk < kCheck) continue;
+        const int jb = dh->fetch(c.stream, jchk, j, H, ldh);
+        jchk = j;
+        if (jb >= 0) {
+          j = jb + 1;
+          break;
+        }
+        const double est = dense::lsq(j + 1, j, H.data(), ldh, cv.data(), y.data());
+        if ((est < tol * r0 && st.iters >= itermin) || st.iters >= maxiter) break;
+        continue;
+      }
       const double hn = arnoldi_step(c, A, V, j, col.data(), reorth, nullptr, nullptr, work ? &work->s : nullptr);
       st.matvecs++;
       st.iters++;
