@@ -70,6 +70,11 @@ SIGNATURES = {
                         ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _ip, _dp,
                         _ip],
     "qqcd_loopops": [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp],
+    "qqcd_gfwrite": [_ip, _dp, ctypes.c_char_p, _ip, ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int],
+    "qqcd_gfread": [_ip, _dp, ctypes.c_char_p, _ip, _dp, _dp, _dp, _ip, _ip],
+    "qqcd_fpwrite": [_ip, _dp, _dp, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qqcd_fpread": [_ip, _dp, _dp, ctypes.c_char_p, _dp, _dp, _dp, _ip, _ip, _ip, _ip],
     "qqcd_propagator": [ctypes.c_int, _ip, _dp, _ip, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                         ctypes.c_void_p, ctypes.c_void_p, _ip, _dp, _dp, _ip],
     "qqcd_loopsum": [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, ctypes.c_double, _dp],

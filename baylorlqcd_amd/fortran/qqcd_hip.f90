@@ -9,6 +9,8 @@
 !   pointsource, fermprop                 qqcd-test/cfgsprops/quark/quark.f90
 !   z4source, z2source, loopops, nsaverage
 !                                         qqcd-test/cfgsprops/quark/disconloops.f90
+!   GFwrite, GFread                       qqcd-test/cfgsprops/glue/gaugetools.f90
+!   FPwrite, FPread                       qqcd-test/cfgsprops/quark/quark.f90
 ! and forward to the HIP library through its C ABI (include/qqcd_hip.h).  A
 ! host program replaces `use diracops` / `use inverters` (and the two quark
 ! procedures) with `use qqcd_hip`; everything else in the host is unchanged.
@@ -33,7 +35,8 @@ module qqcd_hip
 
    public :: Hsingle, Hdouble, Hdbletm, gammamult, gam5x, vecdot, bicgstab, gmresdr, psc, &
              gmresproject, pointsource, fermprop, qqcd_gauge_changed, qqcd_shutdown, &
-             qqcd_solve_mixed_f, z4source, z2source, qqcd_z4_seed, loopops, loopops_all, nsaverage
+             qqcd_solve_mixed_f, z4source, z2source, qqcd_z4_seed, loopops, loopops_all, nsaverage, &
+             GFwrite, GFread, FPwrite, FPread
 
    logical, save :: started = .false.
    ! the Z4 stream: seed and the index of the next source (the reference's
@@ -157,6 +160,43 @@ module qqcd_hip
          integer(c_int), value :: ksub
          type(c_ptr), value :: xe, xo, u, be, bo, bc, io, Jtime
          real(c_double), value :: cur
+      end function
+      integer(c_int) function c_gfwrite(n, u, filename, gaction, beta, alat, tad, icfgsave, myid) &
+            bind(C, name="qqcd_gfwrite")
+         import :: c_int, c_ptr, c_double, c_char
+         integer(c_int), intent(in) :: n(4), gaction(4)
+         type(c_ptr), value :: u
+         character(kind=c_char), intent(in) :: filename(*)
+         real(c_double), value :: beta
+         real(c_double), intent(in) :: alat(4), tad(4)
+         integer(c_int), value :: icfgsave, myid
+      end function
+      integer(c_int) function c_gfread(n, u, filename, gaction, beta, alat, tad, icfgsave, myidin) &
+            bind(C, name="qqcd_gfread")
+         import :: c_int, c_ptr, c_double, c_char
+         integer(c_int), intent(in) :: n(4)
+         type(c_ptr), value :: u
+         character(kind=c_char), intent(in) :: filename(*)
+         integer(c_int), intent(out) :: gaction(4), icfgsave, myidin
+         real(c_double), intent(out) :: beta, alat(4), tad(4)
+      end function
+      integer(c_int) function c_fpwrite(n, xe, xo, filename, kappa, tmqcd, cSW, icsrc, idsrc, icfg, myid) &
+            bind(C, name="qqcd_fpwrite")
+         import :: c_int, c_ptr, c_double, c_char
+         integer(c_int), intent(in) :: n(4)
+         type(c_ptr), value :: xe, xo
+         character(kind=c_char), intent(in) :: filename(*)
+         real(c_double), value :: kappa, tmqcd, cSW
+         integer(c_int), value :: icsrc, idsrc, icfg, myid
+      end function
+      integer(c_int) function c_fpread(n, xe, xo, filename, kappa, tmqcd, cSW, icsrc, idsrc, icfgin, myidin) &
+            bind(C, name="qqcd_fpread")
+         import :: c_int, c_ptr, c_double, c_char
+         integer(c_int), intent(in) :: n(4)
+         type(c_ptr), value :: xe, xo
+         character(kind=c_char), intent(in) :: filename(*)
+         real(c_double), intent(out) :: kappa, tmqcd, cSW
+         integer(c_int), intent(out) :: icsrc, idsrc, icfgin, myidin
       end function
    end interface
 
@@ -551,5 +591,109 @@ contains
                                     itermin, c_loc(ngm), vectype, c_loc(u), c_loc(xe), c_loc(xo), isignal, c_loc(it)))
       if (inverter == 0 .and. vectype == 1) vectype = 0
    end subroutine fermprop
+
+   ! ---- the reference's files (host only: no device, usable before the first operator call) ----
+
+   ! the single-process array extents the files are laid out in: (nvhalf, ntotal)
+   subroutine file_dims(where, nsites, ok)
+      character(len=*), intent(in) :: where
+      integer, intent(in) :: nsites
+      logical, intent(in) :: ok
+      integer :: nvh
+      nvh = nx*ny*nz*nt/32
+      if (.not. ok .or. nsites /= nvh + max(2*nvh/nx, 2*nvh/ny, 2*nvh/nz, 2*nvh/nt)) &
+         call fail(where, -1_c_int)
+   end subroutine file_dims
+
+   ! GFwrite(u,filename,gaction,beta,alat,tad,icfgsave,myid), gaugetools.f90:1260: fixsu3 on u in
+   ! place, rows 1-2 of every link as unformatted records, then the trailer; the file must be new.
+   ! After it the host calls qqcd_gauge_changed (u was re-unitarised).
+   subroutine GFwrite(u,filename,gaction,beta,alat,tad,icfgsave,myid)
+      real(kind=8), intent(inout), target, contiguous :: u(:,:,:,:,:)
+      character(len=*), intent(in) :: filename
+      real(kind=8), intent(in) :: beta, alat(:), tad(:)
+      integer, intent(in) :: gaction(:), icfgsave, myid
+      integer(c_int) :: n(4), ga(4)
+      real(c_double) :: al(4), ta(4)
+      call file_dims("GAUGETOOLS", size(u, 2), size(u, 1) == 18 .and. size(u, 3) == 4 .and. &
+                     size(u, 4) == 2 .and. size(u, 5) == 16)
+      n = [nx, ny, nz, nt]
+      ga = gaction(1:4)
+      al = alat(1:4)
+      ta = tad(1:4)
+      call fail("GAUGETOOLS", c_gfwrite(n, c_loc(u), trim(filename)//c_null_char, ga, beta, al, ta, &
+                                        int(icfgsave, c_int), int(myid, c_int)))
+   end subroutine GFwrite
+
+   ! GFread(u,filename,gaction,beta,alat,tad,icfgsave,myidin), gaugetools.f90:1298: rows 1-2 from
+   ! the file, row 3 rebuilt by fixsu3.  After it the host calls qqcd_gauge_changed.
+   subroutine GFread(u,filename,gaction,beta,alat,tad,icfgsave,myidin)
+      real(kind=8), intent(inout), target, contiguous :: u(:,:,:,:,:)
+      character(len=*), intent(in) :: filename
+      real(kind=8), intent(out) :: beta, alat(:), tad(:)
+      integer, intent(out) :: gaction(:), icfgsave, myidin
+      integer(c_int) :: n(4), ga(4), ic, my
+      real(c_double) :: be, al(4), ta(4)
+      call file_dims("GAUGETOOLS", size(u, 2), size(u, 1) == 18 .and. size(u, 3) == 4 .and. &
+                     size(u, 4) == 2 .and. size(u, 5) == 16)
+      n = [nx, ny, nz, nt]
+      call fail("GAUGETOOLS", c_gfread(n, c_loc(u), trim(filename)//c_null_char, ga, be, al, ta, ic, my))
+      gaction(1:4) = ga
+      beta = be
+      alat(1:4) = al
+      tad(1:4) = ta
+      icfgsave = ic
+      myidin = my
+   end subroutine GFread
+
+   ! FPwrite(xe,xo,vecblinv,filename,kappa,tmqcd,cSW,icsrc,idsrc,icfg,myid), quark.f90:3379: the
+   ! propagator by pairs of timeslices, then the trailer; an existing file is replaced.
+   subroutine FPwrite(xe,xo,vecblinv,filename,kappa,tmqcd,cSW,icsrc,idsrc,icfg,myid)
+      real(kind=8), intent(in), target, contiguous :: xe(:,:,:,:,:), xo(:,:,:,:,:)
+      integer, intent(in) :: vecblinv(:,:)
+      character(len=*), intent(in) :: filename
+      real(kind=8), intent(in) :: kappa, tmqcd, cSW
+      integer, intent(in) :: icsrc, idsrc, icfg, myid
+      integer(c_int) :: n(4)
+      call file_dims("QUARK", size(xe, 2), size(xe, 1) == 6 .and. size(xe, 3) == 4 .and. size(xe, 4) == 2 &
+                     .and. size(xe, 5) == 8 .and. all(shape(xo) == shape(xe)) .and. vecblinv_ok(vecblinv))
+      n = [nx, ny, nz, nt]
+      call fail("QUARK", c_fpwrite(n, c_loc(xe), c_loc(xo), trim(filename)//c_null_char, kappa, tmqcd, cSW, &
+                                   int(icsrc, c_int), int(idsrc, c_int), int(icfg, c_int), int(myid, c_int)))
+   end subroutine FPwrite
+
+   ! FPread(xe,xo,vecblinv,filename,kappa,tmqcd,cSW,icsrc,idsrc,icfgin,myidin), quark.f90:3438.
+   subroutine FPread(xe,xo,vecblinv,filename,kappa,tmqcd,cSW,icsrc,idsrc,icfgin,myidin)
+      real(kind=8), intent(inout), target, contiguous :: xe(:,:,:,:,:), xo(:,:,:,:,:)
+      integer, intent(in) :: vecblinv(:,:)
+      character(len=*), intent(in) :: filename
+      real(kind=8), intent(out) :: kappa, tmqcd, cSW
+      integer, intent(out) :: icsrc, idsrc, icfgin, myidin
+      integer(c_int) :: n(4), ic, id, icf, my
+      real(c_double) :: k, t, c
+      call file_dims("QUARK", size(xe, 2), size(xe, 1) == 6 .and. size(xe, 3) == 4 .and. size(xe, 4) == 2 &
+                     .and. size(xe, 5) == 8 .and. all(shape(xo) == shape(xe)) .and. vecblinv_ok(vecblinv))
+      n = [nx, ny, nz, nt]
+      call fail("QUARK", c_fpread(n, c_loc(xe), c_loc(xo), trim(filename)//c_null_char, k, t, c, ic, id, icf, my))
+      kappa = k
+      tmqcd = t
+      cSW = c
+      icsrc = ic
+      idsrc = id
+      icfgin = icf
+      myidin = my
+   end subroutine FPread
+
+   ! vecblinv of one process (the files' block order assumes it)
+   logical function vecblinv_ok(vecblinv)
+      integer, intent(in) :: vecblinv(:,:)
+      integer :: ibl
+      vecblinv_ok = size(vecblinv, 1) >= 2 .and. size(vecblinv, 2) >= 16
+      if (.not. vecblinv_ok) return
+      do ibl = 1, 16
+         if (vecblinv(1,ibl) /= 1 + mod(popcnt(ibl - 1), 2) .or. vecblinv(2,ibl) /= (ibl + 1)/2) &
+            vecblinv_ok = .false.
+      end do
+   end function vecblinv_ok
 
 end module qqcd_hip

@@ -102,10 +102,35 @@ program test_shim
    call nsaverage(jtime, u, be, bo, g, h, 0.15d0, coact, bc, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, &
                   iblv, 0, 1)
    call writebin("nsaverage.bin", jtime, size(jtime))
+   ! the reference's files: the links (re-unitarised in place by GFwrite) and the solution
+   ! of the first solve, read back into fresh arrays
+   call shim_files()
    call qqcd_shutdown()
    print *, "test_shim done"
 
 contains
+
+   subroutine shim_files()
+      real(kind=8), allocatable :: uw(:,:,:,:,:), ur(:,:,:,:,:), fe(:,:,:,:,:), fo(:,:,:,:,:)
+      real(kind=8) :: beta, alat(4), tad(4), fk, ft, fc, meta(13)
+      integer :: gaction(4), icfg, myidin, icsrc, idsrc
+      allocate(uw(18,ntotal,4,2,16), ur(18,ntotal,4,2,16), fe(6,ntotal,4,2,8), fo(6,ntotal,4,2,8))
+      uw = u
+      ur = 0.0d0
+      call GFwrite(uw, "shim.gf", [1, 1, 1, 2], 6.0d0, [1.0d0, 1.0d0, 1.0d0, 0.5d0], [0.9d0, 0.9d0, 0.9d0, 1.0d0], &
+                   7, 0)
+      call GFread(ur, "shim.gf", gaction, beta, alat, tad, icfg, myidin)
+      call writebin("gfread_u.bin", ur, size(ur))
+      call writebin("gfwrite_u.bin", uw, size(uw))
+      call FPwrite(xe(:,:,:,:,:,1), xo(:,:,:,:,:,1), vecblinv, "shim.fp", 0.156d0, 0.005d0, 0.0d0, 2, 3, 7, 0)
+      fe = 0.0d0
+      fo = 0.0d0
+      call FPread(fe, fo, vecblinv, "shim.fp", fk, ft, fc, icsrc, idsrc, icfg, myidin)
+      call writebin("fpread_e.bin", fe, size(fe))
+      call writebin("fpread_o.bin", fo, size(fo))
+      meta = [dble(gaction), beta, alat(4), tad(1), fk, ft, fc, dble(icsrc), dble(idsrc), dble(icfg)]
+      call writebin("files_meta.bin", meta, size(meta))
+   end subroutine shim_files
 
    subroutine readbin(name, a, n)
       character(len=*), intent(in) :: name

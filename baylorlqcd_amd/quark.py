@@ -60,3 +60,23 @@ def propagator(nrhs, src, kappa, GMRES=(40, 16), tol=1e-10, op_prec=2, op_recon=
     if solutions:
         out["xe"], out["xo"] = xe, xo
     return out
+
+
+def fpwrite(n, xe, xo, filename, kappa, tmqcd=0.0, cSW=0.0, icsrc=1, idsrc=1, icfg=1, myid=0):
+    """FPwrite (qqcd-test/cfgsprops/quark/quark.f90:3379-3429) of reference-layout
+    xe / xo (flat float64); host-only."""
+    _lib.check(_lib.lib().qqcd_fpwrite(_lib.ints(n).ctypes.data_as(_lib._ip), _lib.dp(xe), _lib.dp(xo),
+                                       str(filename).encode(), float(kappa), float(tmqcd), float(cSW), int(icsrc),
+                                       int(idsrc), int(icfg), int(myid)))
+
+
+def fpread(n, xe, xo, filename):
+    """FPread (quark.f90:3433-3481) into xe / xo; returns the trailer as a dict."""
+    d = np.zeros(3)
+    i = np.zeros(4, np.int32)
+    ip = [i[k:k + 1].ctypes.data_as(_lib._ip) for k in range(4)]
+    _lib.check(_lib.lib().qqcd_fpread(_lib.ints(n).ctypes.data_as(_lib._ip), _lib.dp(xe), _lib.dp(xo),
+                                      str(filename).encode(), _lib.dp(d[0:1]), _lib.dp(d[1:2]), _lib.dp(d[2:3]),
+                                      *ip))
+    return {"kappa": float(d[0]), "tmqcd": float(d[1]), "cSW": float(d[2]), "icsrc": int(i[0]), "idsrc": int(i[1]),
+            "icfg": int(i[2]), "myid": int(i[3])}

@@ -277,6 +277,32 @@ int qqcd_propagator(int nrhs, const int src[4], const double kappa[2], const int
                     int op_recon, int method, double *const *xe, double *const *xo, int *iters, double *res_full,
                     double *times, int *stats);
 
+/* ---- the reference's files (host only: n = lattice extents, no device) --- */
+
+/* GFwrite (qqcd-test/cfgsprops/glue/gaugetools.f90:1259-1292): fixsu3 is
+ * applied to u in place (rows 1-2 re-orthonormalised, row 3 rebuilt), then
+ * rows 1-2 of every link are written as Fortran unformatted records
+ * (icri, isite, ibl order, mu and ieo inside), then the trailer gaction(4),
+ * beta, alat(4), tad(4), icfgsave, myid.  The file must not exist
+ * (status="new").  u: (18, ntotal, 4, 2, 16). */
+int qqcd_gfwrite(const int n[4], double *u, const char *filename, const int gaction[4], double beta,
+                 const double alat[4], const double tad[4], int icfgsave, int myid);
+
+/* GFread (gaugetools.f90:1296-1334): rows 1-2 from the file, the links
+ * completed by fixsu3; the trailer into the outputs. */
+int qqcd_gfread(const int n[4], double *u, const char *filename, int gaction[4], double *beta, double alat[4],
+                double tad[4], int *icfgsave, int *myidin);
+
+/* FPwrite (qqcd-test/cfgsprops/quark/quark.f90:3379-3429): the propagator
+ * xe / xo (6, ntotal, 4, 2, 8) by site (pairs of timeslices) and block, then
+ * kappa, tmqcd, cSW, icsrc, idsrc, icfg, myid.  Replaces an existing file. */
+int qqcd_fpwrite(const int n[4], const double *xe, const double *xo, const char *filename, double kappa,
+                 double tmqcd, double cSW, int icsrc, int idsrc, int icfg, int myid);
+
+/* FPread (quark.f90:3433-3481). */
+int qqcd_fpread(const int n[4], double *xe, double *xo, const char *filename, double *kappa, double *tmqcd,
+                double *cSW, int *icsrc, int *idsrc, int *icfgin, int *myidin);
+
 /* The 16 local trace estimators of one source: eta and x on both
  * checkerboards (reference layout); traces as in qqcd_noise_loop for
  * nrhs = 1, scale = 1. */

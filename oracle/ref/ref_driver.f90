@@ -241,6 +241,77 @@ contains
       plaq = p
    end subroutine ref_aveplaq
 
+   ! Gauge-field and propagator files of the reference (GFwrite / GFread,
+   ! qqcd-test/cfgsprops/glue/gaugetools.f90:1259-1334; FPwrite / FPread,
+   ! qqcd-test/cfgsprops/quark/quark.f90:3379-3481).
+   subroutine ref_gfwrite(u, fname, nf, gaction, beta, alat, tad, icfg) bind(C, name="ref_gfwrite")
+      real(c_double), intent(inout) :: u(18,ntotal,4,2,16)
+      integer(c_int), value         :: nf, icfg
+      character(kind=c_char), intent(in) :: fname(nf)
+      integer(c_int), intent(in)    :: gaction(4)
+      real(c_double), value         :: beta
+      real(c_double), intent(in)    :: alat(4), tad(4)
+      character(len=512) :: f
+      integer(kind=KI) :: ga(4), k
+      f = ' '
+      do k = 1, nf
+         f(k:k) = fname(k)
+      end do
+      ga = gaction
+      call GFwrite(u, f, ga, beta, alat, tad, icfg, myid)
+   end subroutine ref_gfwrite
+
+   subroutine ref_gfread(u, fname, nf, gaction, beta, alat, tad, icfg, myidin) bind(C, name="ref_gfread")
+      real(c_double), intent(inout) :: u(18,ntotal,4,2,16)
+      integer(c_int), value         :: nf
+      character(kind=c_char), intent(in) :: fname(nf)
+      integer(c_int), intent(out)   :: gaction(4), icfg, myidin
+      real(c_double), intent(out)   :: beta, alat(4), tad(4)
+      character(len=512) :: f
+      integer(kind=KI) :: ga(4), ic, mi, k
+      f = ' '
+      do k = 1, nf
+         f(k:k) = fname(k)
+      end do
+      call GFread(u, f, ga, beta, alat, tad, ic, mi)
+      gaction = ga
+      icfg = ic
+      myidin = mi
+   end subroutine ref_gfread
+
+   subroutine ref_fpwrite(xe, xo, fname, nf, kappa, tmqcd, csw, icsrc, idsrc, icfg) bind(C, name="ref_fpwrite")
+      real(c_double), intent(inout) :: xe(6,ntotal,4,2,8), xo(6,ntotal,4,2,8)
+      integer(c_int), value         :: nf, icsrc, idsrc, icfg
+      character(kind=c_char), intent(in) :: fname(nf)
+      real(c_double), value         :: kappa, tmqcd, csw
+      character(len=512) :: f
+      integer(kind=KI) :: k
+      f = ' '
+      do k = 1, nf
+         f(k:k) = fname(k)
+      end do
+      call FPwrite(xe, xo, vecblinv, f, kappa, tmqcd, csw, icsrc, idsrc, icfg, myid)
+   end subroutine ref_fpwrite
+
+   subroutine ref_fpread(xe, xo, fname, nf, kappa, tmqcd, csw, icsrc, idsrc, icfg, myidin) bind(C, name="ref_fpread")
+      real(c_double), intent(inout) :: xe(6,ntotal,4,2,8), xo(6,ntotal,4,2,8)
+      integer(c_int), value         :: nf
+      character(kind=c_char), intent(in) :: fname(nf)
+      real(c_double), intent(out)   :: kappa, tmqcd, csw
+      integer(c_int), intent(out)   :: icsrc, idsrc, icfg, myidin
+      character(len=512) :: f
+      integer(kind=KI) :: a, b, cc, d, k
+      f = ' '
+      do k = 1, nf
+         f(k:k) = fname(k)
+      end do
+      call FPread(xe, xo, vecblinv, f, kappa, tmqcd, csw, a, b, cc, d)
+      icsrc = a
+      idsrc = b
+      icfg = cc
+      myidin = d
+   end subroutine ref_fpread
+
    ! The loop operators of one (sink x, source b) pair on every (ieo, ibleo)
    ! slice: Je/Jo(:,:,:,ieo,ibleo) is loopops' output for that slice
    ! (qqcd-test/cfgsprops/quark/disconloops.f90:7769-7910).

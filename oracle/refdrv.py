@@ -227,6 +227,45 @@ class RefLattice:
              _p(np.ascontiguousarray(coact)))
         return out
 
+    def gfwrite(self, u, path, gaction=(1, 1, 1, 1), beta=6.0, alat=(1, 1, 1, 1), tad=(1, 1, 1, 1), icfg=1):
+        """GFwrite (fixsu3 is applied to a copy of u, as the reference does in place)."""
+        uu = np.array(u, dtype=np.float64, copy=True)
+        f = path.encode()
+        _run(self.L.ref_gfwrite, _p(uu), ctypes.c_char_p(f), ctypes.c_int(len(f)),
+             _ip_(np.ascontiguousarray(gaction, dtype=np.int32)), ctypes.c_double(beta),
+             _p(np.ascontiguousarray(alat, dtype=np.float64)), _p(np.ascontiguousarray(tad, dtype=np.float64)),
+             ctypes.c_int(icfg))
+        return uu
+
+    def gfread(self, path):
+        u = np.zeros(self.ulen)
+        ga = np.zeros(4, np.int32)
+        beta = ctypes.c_double()
+        alat, tad = np.zeros(4), np.zeros(4)
+        icfg, mi = ctypes.c_int(), ctypes.c_int()
+        f = path.encode()
+        _run(self.L.ref_gfread, _p(u), ctypes.c_char_p(f), ctypes.c_int(len(f)), _ip_(ga), ctypes.byref(beta),
+             _p(alat), _p(tad), ctypes.byref(icfg), ctypes.byref(mi))
+        return u, dict(gaction=ga, beta=beta.value, alat=alat, tad=tad, icfg=icfg.value, myid=mi.value)
+
+    def fpwrite(self, xe, xo, path, kappa=0.15, tmqcd=0.0, csw=0.0, icsrc=1, idsrc=1, icfg=1):
+        a = np.array(xe, dtype=np.float64, copy=True)
+        b = np.array(xo, dtype=np.float64, copy=True)
+        f = path.encode()
+        _run(self.L.ref_fpwrite, _p(a), _p(b), ctypes.c_char_p(f), ctypes.c_int(len(f)), ctypes.c_double(kappa),
+             ctypes.c_double(tmqcd), ctypes.c_double(csw), ctypes.c_int(icsrc), ctypes.c_int(idsrc),
+             ctypes.c_int(icfg))
+
+    def fpread(self, path):
+        xe, xo = np.zeros(self.slen), np.zeros(self.slen)
+        k, t, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        ints = [ctypes.c_int() for _ in range(4)]
+        f = path.encode()
+        _run(self.L.ref_fpread, _p(xe), _p(xo), ctypes.c_char_p(f), ctypes.c_int(len(f)), ctypes.byref(k),
+             ctypes.byref(t), ctypes.byref(c), *[ctypes.byref(v) for v in ints])
+        return xe, xo, dict(kappa=k.value, tmqcd=t.value, cSW=c.value, icsrc=ints[0].value, idsrc=ints[1].value,
+                            icfg=ints[2].value, myid=ints[3].value)
+
     def fixaction(self, beta):
         co = np.zeros(64)
         _run(self.L.ref_fixaction, ctypes.c_double(beta), _p(co))

@@ -4,7 +4,9 @@ Runs only where /root/reference is present (this script builds it with
 oracle/ref/Makefile, then calls it through oracle/refdrv.py).  Output:
 tests/golden/golden_4x4x4x4.npz and tests/golden/golden_8x8x8x16.npz;
 with the argument ``loops`` only tests/golden/golden_loops_8x8x8x16.npz
-(the loop operators loopops / spacesum / nsaverage of the noise loop).
+(the loop operators loopops / spacesum / nsaverage of the noise loop); with
+``io`` only tests/golden/golden_io_8x8x8x16.json (SHA-256 digests of the
+reference's GFwrite / FPwrite files and of what GFread / FPread return).
 
 Inputs are regenerated from seeds by the tests (spinors: numpy
 default_rng(seed) Gaussians; links: baylorlqcd_amd.gauge.haar), except the
@@ -196,8 +198,50 @@ def loops():
     print("wrote golden_loops_8x8x8x16.npz", len(out), "arrays")
 
 
+IO_CASE = dict(links_seed=17, spinor_seed=5, gaction=(1, 1, 1, 2), beta=6.0, alat=(1.0, 1.0, 1.0, 0.5),
+               tad=(0.9, 0.9, 0.9, 1.0), icfg=7, kappa=0.156, tmqcd=0.005, cSW=0.0, icsrc=2, idsrc=3)
+
+
+def io_inputs(n):
+    u = O.links_canon_to_ref(n, gauge.haar(n, IO_CASE["links_seed"]))
+    rng = np.random.default_rng(IO_CASE["spinor_seed"])
+    V = int(np.prod(n))
+    psi = rng.standard_normal((V, 4, 3)) + 1j * rng.standard_normal((V, 4, 3))
+    xe, xo = O.spinor_canon_to_ref(n, psi)
+    return u, xe, xo
+
+
+def io():
+    import hashlib
+    import json
+    import tempfile
+    n = (8, 8, 8, 16)
+    refdrv.build(n, solvers=True)
+    R = refdrv.RefLattice(n, bc=(1, 1, 1, -1))
+    u, xe, xo = io_inputs(n)
+    d = tempfile.mkdtemp()
+    c = IO_CASE
+    gf = os.path.join(d, "cfg.gf")
+    uw = R.gfwrite(u, gf, c["gaction"], c["beta"], c["alat"], c["tad"], c["icfg"])
+    ur, _ = R.gfread(gf)
+    fp = os.path.join(d, "prop.fp")
+    R.fpwrite(xe, xo, fp, c["kappa"], c["tmqcd"], c["cSW"], c["icsrc"], c["idsrc"], c["icfg"])
+    ye, yo, _ = R.fpread(fp)
+    sha = lambda b: hashlib.sha256(b).hexdigest()  # noqa: E731
+    out = {"lattice": list(n), "case": {k: list(v) if isinstance(v, tuple) else v for k, v in c.items()},
+           "gf_file_sha256": sha(open(gf, "rb").read()), "gf_file_bytes": os.path.getsize(gf),
+           "gfwrite_u_sha256": sha(uw.tobytes()), "gfread_u_sha256": sha(ur.tobytes()),
+           "fp_file_sha256": sha(open(fp, "rb").read()), "fp_file_bytes": os.path.getsize(fp),
+           "fpread_xe_sha256": sha(ye.tobytes()), "fpread_xo_sha256": sha(yo.tobytes())}
+    with open(os.path.join(HERE, "golden_io_8x8x8x16.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote golden_io_8x8x8x16.json")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["loops"]:
         loops()
+    elif sys.argv[1:] == ["io"]:
+        io()
     else:
         main()

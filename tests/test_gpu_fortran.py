@@ -5,7 +5,8 @@ The host program (baylorlqcd_amd/fortran/test_shim.f90, built by
 __graft_entry__.build()) reads the config-1 links and a seeded spinor pair,
 calls Hsingle, Hdbletm, Hdouble(idag=1), gam5x, vecdot, pointsource +
 fermprop (GMRES-DR, then psc with isignal=2), the mixed-precision solve and
-z4source, z2source, loopops (one slice) and nsaverage;
+z4source, z2source, loopops (one slice), nsaverage, and GFwrite / GFread /
+FPwrite / FPread;
 every output is compared with the oracle or the reference's golden solution.
 """
 import os
@@ -15,7 +16,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from baylorlqcd_amd import gauge
+from baylorlqcd_amd import gauge, gaugetools, quark
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -103,3 +104,18 @@ def test_fortran_host(tmp_path):
     wt = O.spacesum(N, O.loopops(N, U, BC, psi, z, 0), nmom=1)[:, 0, :]
     wt[:, :4] *= 0.15
     assert rel(Jt[:, 0, :], wt) < 1e-12 and np.all(Jt[:, 1:, :] == 0)
+    # GFwrite / GFread / FPwrite / FPread through the drop-ins: the same bytes as the library's
+    # own host calls (pinned against the reference's files in test_io_formats.py)
+    gw = tmp_path / "lib.gf"
+    uw = u.copy()
+    gaugetools.gfwrite(N, uw, gw, (1, 1, 1, 2), 6.0, (1, 1, 1, 0.5), (0.9, 0.9, 0.9, 1.0), 7, 0)
+    assert (tmp_path / "shim.gf").read_bytes() == gw.read_bytes()
+    assert np.array_equal(load("gfwrite_u.bin"), uw)
+    ur = np.zeros_like(u)
+    gaugetools.gfread(N, ur, gw)
+    assert np.array_equal(load("gfread_u.bin"), ur)
+    fw = tmp_path / "lib.fp"
+    quark.fpwrite(N, load("xe.bin"), load("xo.bin"), fw, 0.156, 0.005, 0.0, 2, 3, 7, 0)
+    assert (tmp_path / "shim.fp").read_bytes() == fw.read_bytes()
+    assert np.array_equal(load("fpread_e.bin"), load("xe.bin")) and np.array_equal(load("fpread_o.bin"), load("xo.bin"))
+    assert np.array_equal(load("files_meta.bin"), [1, 1, 1, 2, 6.0, 0.5, 0.9, 0.156, 0.005, 0.0, 2, 3, 7])
