@@ -70,6 +70,10 @@ SIGNATURES = {
                         ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _ip, _dp,
                         _ip],
     "qqcd_loopops": [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp],
+    "qqcd_lagfib_init": [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip],
+    "qqcd_rnds": [ctypes.c_long, _dp, _ip, _ip, _ip],
+    "qqcd_z4source_lagfib": [_dp, _dp, _ip, _ip, _ip],
+    "qqcd_z2source_lagfib": [_dp, _dp, _ip, _ip, _ip],
     "qqcd_gfwrite": [_ip, _dp, ctypes.c_char_p, _ip, ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int],
     "qqcd_gfread": [_ip, _dp, ctypes.c_char_p, _ip, _dp, _dp, _dp, _ip, _ip],
     "qqcd_fpwrite": [_ip, _dp, _dp, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,

@@ -8,6 +8,7 @@
 
 #include "../../include/qqcd_hip.h"
 #include "qqcd_context.h"
+#include "qqcd_lagfib.h"
 
 namespace qqcd {
 static thread_local std::string g_err;
@@ -69,6 +70,7 @@ extern "C" int qqcd_finalize(void) {
   API_BEGIN
   qqcd_deflation_clear();
   nn_release_device();
+  lagfib_release_device();
   delete g_ctx;
   g_ctx = nullptr;
   API_END

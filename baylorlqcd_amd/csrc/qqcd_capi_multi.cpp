@@ -14,6 +14,7 @@
 #include "qqcd_solvers.h"
 #include "qqcd_loops.h"
 #include "qqcd_noise.h"
+#include "qqcd_lagfib.h"
 
 using namespace qqcd;
 
@@ -146,6 +147,87 @@ extern "C" int qqcd_z2source(double *be, double *bo, unsigned long long seed, lo
     to_host(c, s.s, outs[par], par);
   }
   API_END
+}
+
+// ---- the reference's lagged-Fibonacci stream -------------------------------
+
+namespace {
+
+LagState lag_in(const int *iw, const int *jr, const int *kr) {
+  if (!iw || !jr || !kr) throw std::runtime_error("lagfib: null state");
+  LagState st;
+  for (int k = 0; k < 521; k++) st.w[k] = (uint32_t)iw[k];
+  st.jr = *jr;
+  st.kr = *kr;
+  lagfib_check(st);
+  return st;
+}
+
+void lag_out(const LagState &st, int *iw, int *jr, int *kr) {
+  for (int k = 0; k < 521; k++) iw[k] = (int)st.w[k];
+  *jr = st.jr;
+  *kr = st.kr;
+}
+
+struct DevU32 {
+  uint32_t *p = nullptr;
+  explicit DevU32(size_t n) { QQCD_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(uint32_t))); }
+  ~DevU32() {
+    if (p) hipFree(p);
+  }
+};
+
+int lag_source(double *be, double *bo, int *iw, int *jr, int *kr, int kind) {
+  API_BEGIN
+  Context &c = ctx();
+  LagState st = lag_in(iw, jr, kr);
+  const long nvh = c.g.V / 32;
+  DevU32 y(384 * nvh);
+  lagfib_generate(st, 384 * nvh, y.p, c.stream);
+  SpinorBuf s(c.g, PREC_DOUBLE);
+  double *outs[2] = {be, bo};
+  for (int par = 0; par < 2; par++) {
+    lag_noise_source(c.g, s.s, par, y.p + par * 192 * nvh, kind, 1.0, c.stream);
+    if (outs[par]) to_host(c, s.s, outs[par], par);
+  }
+  lag_out(st, iw, jr, kr);
+  API_END
+}
+
+}  // namespace
+
+extern "C" int qqcd_lagfib_init(int myid, int iseed, int *iw, int *jr, int *kr) {
+  API_BEGIN
+  if (!iw || !jr || !kr) throw std::runtime_error("qqcd_lagfib_init: null output");
+  LagState st;
+  lagfib_init(myid, iseed, st);
+  lag_out(st, iw, jr, kr);
+  API_END
+}
+
+extern "C" int qqcd_rnds(long m, double *rn, int *iw, int *jr, int *kr) {
+  API_BEGIN
+  Context &c = ctx();
+  if (m < 0 || (m > 0 && !rn)) throw std::runtime_error("qqcd_rnds: m >= 0 and an output array expected");
+  LagState st = lag_in(iw, jr, kr);
+  DevU32 y(m);
+  lagfib_generate(st, m, y.p, c.stream);
+  if (m > 0) {
+    DevArr d(nullptr, (size_t)m, c.stream);
+    lagfib_to_double(y.p, m, d.p, c.stream);
+    QQCD_HIP(hipMemcpyAsync(rn, d.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+    QQCD_HIP(hipStreamSynchronize(c.stream));
+  }
+  lag_out(st, iw, jr, kr);
+  API_END
+}
+
+extern "C" int qqcd_z4source_lagfib(double *be, double *bo, int *iw, int *jr, int *kr) {
+  return lag_source(be, bo, iw, jr, kr, 0);
+}
+
+extern "C" int qqcd_z2source_lagfib(double *be, double *bo, int *iw, int *jr, int *kr) {
+  return lag_source(be, bo, iw, jr, kr, 1);
 }
 
 extern "C" int qqcd_hop_multi_canonical(int nrhs, int nact, int nb, int prec, int recon, int idag, int par_out,

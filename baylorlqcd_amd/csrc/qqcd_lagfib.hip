@@ -1,0 +1,405 @@
+// qqcd_lagfib.hip -- the reference's lagged-Fibonacci random numbers on the device.
+//
+// The reference's generator (qqcd-test/cfgsprops/glue/lagfib.f90) keeps 521
+// words w and two cursors jr, kr = jr - 32 (mod 521).  A draw sets
+// w[jr] ^= w[kr] -- again while the result is zero -- returns w[jr] times
+// fnorm and advances both cursors.  As a sequence of draws,
+//   y_t = y_{t-521} ^ y_{t-32},   y_{-521+k} = w[(jr + k) mod 521],
+// bitwise linear over GF(2) with characteristic polynomial
+// P(x) = x^521 + x^489 + 1: y_t = XOR_k a_k y_{k-521} with
+// sum_k a_k x^k = x^(t+521) mod P.
+//
+// Device generation of m draws: chunks of L draws; the 521 words before
+// chunk c are the jump-ahead of the base window by the rows x^(cL+j) mod P,
+// j = 0..520 (a 521 x 521 bit matrix per chunk, built once per L on the
+// host and kept on the device); then one half-wave runs the recurrence 32
+// draws at a time through an LDS ring.  A zero draw (probability 2^-31) is
+// the one non-linear step: the reference XORs again, so y_t becomes
+// y_{t-32}.  The first zero is found with an atomic minimum, corrected, and
+// the stream after it generated again from the corrected window.
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qqcd_lagfib.h"
+
+namespace qqcd {
+
+static constexpr int LP = 521, LQ = 32, LW = 17, RING = 1024;
+static constexpr int KMAX = 2048;  // chunks per launch
+typedef std::array<uint32_t, LW> Poly;
+
+// ---- initlagfib (lagfib.f90:80-216) ----------------------------------------
+
+void lagfib_init(int myid, int iseed, LagState &st) {
+  // initbirep: one bit per word from a multiplicative congruential sequence,
+  // then 32 steps of the bit recurrence per word of the common start
+  int ib[LP];
+  uint32_t ix = 1;
+  for (int i = 0; i < LP; i++) {
+    ix *= 69069u;
+    ib[i] = (int)(ix >> 31);
+  }
+  int jr = 0, kr = LP - LQ;
+  uint32_t w0[LP];
+  for (int j = 0; j < LP; j++) {
+    uint32_t work = 0;
+    for (int i = 0; i < 32; i++) {
+      work = 2u * work + (uint32_t)ib[jr];
+      ib[jr] ^= ib[kr];
+      jr = jr + 1 == LP ? 0 : jr + 1;
+      kr = kr + 1 == LP ? 0 : kr + 1;
+    }
+    w0[j] = work >> 1;
+  }
+  // bigdelay(myid + iseed): the common start advanced by a polynomial in x
+  // built from the binary digits of ndelay (40 squarings first)
+  const int mu = 40;
+  int m = myid + iseed;
+  if (m <= 0) throw std::runtime_error("lagfib_init: myid + iseed must be positive");
+  std::vector<uint32_t> wk(2 * LP - 1);
+  for (int i = 0; i < LP; i++) wk[i] = w0[i];
+  for (int i = LP; i < 2 * LP - 1; i++) wk[i] = wk[i - LP] ^ wk[i - LQ];
+  int bits[LP + 32] = {0};
+  int nb = mu - 1;
+  while (m > LP - 1) {
+    bits[++nb] = m % 2;
+    m /= 2;
+  }
+  std::vector<int> c(2 * LP, 0);
+  c[m] = 1;
+  for (int j = nb; j >= 0; j--) {
+    for (int i = LP - 1; i >= 0; i--) {
+      c[2 * i + bits[j]] = c[i];
+      c[2 * i + 1 - bits[j]] = 0;
+    }
+    for (int i = 2 * LP - 1; i >= LP; i--) {
+      c[i - LP] ^= c[i];
+      c[i - LQ] ^= c[i];
+    }
+  }
+  for (int j = 0; j < LP; j++) {
+    uint32_t v = 0;
+    for (int i = 0; i < LP; i++)
+      if (c[i]) v ^= wk[j + i];
+    st.w[j] = v;
+  }
+  st.jr = jr;
+  st.kr = kr;
+}
+
+void lagfib_check(const LagState &st) {
+  if (st.jr < 0 || st.jr >= LP || st.kr != (st.jr + LP - LQ) % LP)
+    throw std::runtime_error("lagfib: cursors must satisfy 0 <= jr < 521 and kr = jr - 32 mod 521 (got jr=" +
+                             std::to_string(st.jr) + ", kr=" + std::to_string(st.kr) + ")");
+}
+
+// ---- jump-ahead tables -----------------------------------------------------
+
+static void mulx(Poly &a) {
+  uint32_t carry = 0;
+  for (int w = 0; w < LW; w++) {
+    const uint32_t nc = a[w] >> 31;
+    a[w] = (a[w] << 1) | carry;
+    carry = nc;
+  }
+  if (a[LP / 32] & (1u << (LP % 32))) {  // x^521 = x^489 + 1
+    a[LP / 32] &= ~(1u << (LP % 32));
+    a[0] ^= 1u;
+    a[(LP - LQ) / 32] ^= 1u << ((LP - LQ) % 32);
+  }
+}
+
+static Poly mulmod(const Poly &a, const Poly &b) {
+  Poly r{};
+  for (int i = LP - 1; i >= 0; i--) {
+    mulx(r);
+    if ((a[i / 32] >> (i % 32)) & 1u)
+      for (int w = 0; w < LW; w++) r[w] ^= b[w];
+  }
+  return r;
+}
+
+static Poly xpow(long e) {
+  Poly r{}, b{};
+  r[0] = 1;
+  b[0] = 2;  // x
+  while (e) {
+    if (e & 1) r = mulmod(r, b);
+    b = mulmod(b, b);
+    e >>= 1;
+  }
+  return r;
+}
+
+struct LagTable {
+  long L = 0;
+  int K = 0;
+  uint32_t *d = nullptr;
+  std::vector<uint32_t> h;
+};
+static std::vector<LagTable> g_tables;
+
+static const uint32_t *table_for(long L, int K, hipStream_t s) {
+  LagTable *t = nullptr;
+  for (auto &e : g_tables)
+    if (e.L == L) t = &e;
+  if (!t) {
+    g_tables.push_back(LagTable{});
+    t = &g_tables.back();
+    t->L = L;
+  }
+  if (t->K >= K) return t->d;
+  const int K0 = t->K;
+  t->h.resize((size_t)K * LP * LW);
+  const Poly xL = xpow(L);
+  Poly p = xpow((long)K0 * L);
+  for (int c = K0; c < K; c++) {
+    Poly r = p;
+    for (int j = 0; j < LP; j++) {
+      std::memcpy(&t->h[((size_t)c * LP + j) * LW], r.data(), sizeof(uint32_t) * LW);
+      mulx(r);
+    }
+    p = mulmod(p, xL);
+  }
+  if (t->d) {
+    QQCD_HIP(hipStreamSynchronize(s));
+    QQCD_HIP(hipFree(t->d));
+    t->d = nullptr;
+  }
+  QQCD_HIP(hipMalloc(&t->d, sizeof(uint32_t) * t->h.size()));
+  QQCD_HIP(hipMemcpyAsync(t->d, t->h.data(), sizeof(uint32_t) * t->h.size(), hipMemcpyHostToDevice, s));
+  QQCD_HIP(hipStreamSynchronize(s));
+  t->K = K;
+  return t->d;
+}
+
+void lagfib_release_device() {
+  for (auto &e : g_tables)
+    if (e.d) (void)hipFree(e.d);
+  g_tables.clear();
+}
+
+// ---- device generation -----------------------------------------------------
+
+// Workgroup of one wave: lanes 0-31 run chunk 2b, lanes 32-63 chunk 2b+1.
+// Ring slot of draw t of chunk c: (t - cL + 521) mod 1024; slots 0..520 hold
+// the window y_{cL-521} .. y_{cL-1}.
+__global__ void __launch_bounds__(64) k_lag_gen(const uint32_t *base, const uint32_t *rows, long L, long m, int K,
+                                                uint32_t *out, unsigned long long *first_zero) {
+  __shared__ uint32_t sb[LW * 32];
+  __shared__ uint32_t ring[2][RING];
+  const int lane = threadIdx.x, h = lane >> 5, j = lane & 31;
+  const int c = 2 * blockIdx.x + h;
+  for (int k = lane; k < LW * 32; k += 64) sb[k] = k < LP ? base[k] : 0u;
+  __syncthreads();
+  if (c < K) {
+    for (int r = j; r < LP; r += 32) {
+      const uint32_t *row = rows + ((long)c * LP + r) * LW;
+      uint32_t v = 0;
+      for (int w = 0; w < LW; w++) {
+        uint32_t bits = row[w];
+        while (bits) {
+          const int b = __builtin_ctz(bits);
+          bits &= bits - 1;
+          v ^= sb[32 * w + b];
+        }
+      }
+      ring[h][r] = v;
+    }
+  }
+  __syncthreads();
+  // both halves run the same number of steps (the barriers are shared)
+  long len[2];
+  for (int q = 0; q < 2; q++) {
+    const long cq = 2L * blockIdx.x + q;
+    len[q] = cq < K ? std::min(L, m - cq * L) : 0;
+  }
+  const long steps = (std::max(len[0], len[1]) + 31) / 32;
+  const long t0 = (long)c * L;
+  for (long st = 0; st < steps; st++) {
+    const long u = LP + 32 * st + j;
+    if (32 * st + j < len[h]) {
+      const uint32_t y = ring[h][(u - LP) & (RING - 1)] ^ ring[h][(u - LQ) & (RING - 1)];
+      ring[h][u & (RING - 1)] = y;
+      out[t0 + 32 * st + j] = y;
+      if (y == 0u) atomicMin(first_zero, (unsigned long long)(t0 + 32 * st + j));
+    }
+    __syncthreads();
+  }
+}
+
+// nb[k] = y_{tend-521+k} (y_t = out[t] for t >= 0, base[521+t] below); with
+// fix >= 0 the draw at t = fix (== tend-1) becomes y_{fix-32}, in out too,
+// as the reference's zero skip makes it; bad[0] = 1 if that is zero as well.
+__global__ void k_lag_rebase(const uint32_t *base, uint32_t *out, long tend, long fix, uint32_t *nb, int *bad) {
+  __shared__ uint32_t fixed;
+  if (threadIdx.x == 0) {
+    if (fix >= 0) {
+      const long t = fix - LQ;
+      const uint32_t v = t >= 0 ? out[t] : base[LP + t];
+      fixed = v;
+      out[fix] = v;
+      if (v == 0u) bad[0] = 1;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < LP; k += blockDim.x) {
+    const long t = tend - LP + k;
+    uint32_t v = t >= 0 ? out[t] : base[LP + t];
+    if (t == fix) v = fixed;
+    nb[k] = v;
+  }
+}
+
+void lagfib_generate(LagState &st, long m, uint32_t *d_out, hipStream_t s) {
+  lagfib_check(st);
+  if (m < 0) throw std::runtime_error("lagfib_generate: negative count");
+  uint32_t hb[LP];
+  for (int k = 0; k < LP; k++) hb[k] = st.w[(st.jr + k) % LP];
+  uint32_t *dbase = nullptr;
+  unsigned long long *dz = nullptr;
+  int *dbad = nullptr;
+  QQCD_HIP(hipMalloc(&dbase, sizeof(uint32_t) * 2 * LP));
+  QQCD_HIP(hipMalloc(&dz, sizeof(unsigned long long)));
+  QQCD_HIP(hipMalloc(&dbad, sizeof(int)));
+  try {
+    QQCD_HIP(hipMemcpyAsync(dbase, hb, sizeof(hb), hipMemcpyHostToDevice, s));
+    QQCD_HIP(hipMemsetAsync(dbad, 0, sizeof(int), s));
+    uint32_t *cur = dbase, *nxt = dbase + LP;
+    uint32_t *o = d_out;
+    long left = m;
+    while (left > 0) {
+      long L = 8192;
+      while ((left + L - 1) / L > KMAX && L < (1L << 21)) L *= 4;
+      const long piece = std::min(left, (long)KMAX * L);
+      const int K = (int)((piece + L - 1) / L);
+      const uint32_t *rows = table_for(L, K, s);
+      QQCD_HIP(hipMemsetAsync(dz, 0xff, sizeof(unsigned long long), s));
+      hipLaunchKernelGGL(k_lag_gen, dim3((K + 1) / 2), dim3(64), 0, s, cur, rows, L, piece, K, o, dz);
+      QQCD_HIP(hipGetLastError());
+      unsigned long long z = 0;
+      QQCD_HIP(hipMemcpyAsync(&z, dz, sizeof(z), hipMemcpyDeviceToHost, s));
+      QQCD_HIP(hipStreamSynchronize(s));
+      long used = piece;
+      long fix = -1;
+      if (z < (unsigned long long)piece) {
+        used = (long)z + 1;
+        fix = (long)z;
+      }
+      if (used < left) {
+        hipLaunchKernelGGL(k_lag_rebase, dim3(1), dim3(256), 0, s, cur, o, used, fix, nxt, dbad);
+        QQCD_HIP(hipGetLastError());
+        std::swap(cur, nxt);
+      } else if (fix >= 0) {  // the last draw was the zero: correct it in place
+        hipLaunchKernelGGL(k_lag_rebase, dim3(1), dim3(256), 0, s, cur, o, used, fix, nxt, dbad);
+        QQCD_HIP(hipGetLastError());
+      }
+      // a rebase reads the old base, so keep it until the kernel has run
+      int bad = 0;
+      QQCD_HIP(hipMemcpyAsync(&bad, dbad, sizeof(int), hipMemcpyDeviceToHost, s));
+      QQCD_HIP(hipStreamSynchronize(s));
+      if (bad) throw std::runtime_error("lagfib: degenerate state (the zero skip of rnds would not end)");
+      o += used;
+      left -= used;
+    }
+    // the new state: w'[(jr + m + k) mod 521] = y_{m-521+k}
+    const long nlast = std::min(m, (long)LP);
+    std::vector<uint32_t> tail(nlast);
+    if (nlast) {
+      QQCD_HIP(hipMemcpyAsync(tail.data(), d_out + (m - nlast), sizeof(uint32_t) * nlast, hipMemcpyDeviceToHost, s));
+      QQCD_HIP(hipStreamSynchronize(s));
+    }
+    LagState ns;
+    const int njr = (int)((st.jr + m) % LP);
+    for (int k = 0; k < LP; k++) {
+      const long t = m - LP + k;
+      ns.w[(njr + k) % LP] = t >= 0 ? tail[t - (m - nlast)] : hb[LP + t];
+    }
+    ns.jr = njr;
+    ns.kr = (njr + LP - LQ) % LP;
+    st = ns;
+  } catch (...) {
+    (void)hipFree(dbase);
+    (void)hipFree(dz);
+    (void)hipFree(dbad);
+    throw;
+  }
+  QQCD_HIP(hipFree(dbase));
+  QQCD_HIP(hipFree(dz));
+  QQCD_HIP(hipFree(dbad));
+}
+
+__global__ void k_lag_double(const uint32_t *y, long m, double *rn) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) rn[t] = (double)(int)y[t] * LAG_FNORM;
+}
+
+void lagfib_to_double(const uint32_t *d_y, long m, double *d_rn, hipStream_t s) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_lag_double, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, d_y, m, d_rn);
+  QQCD_HIP(hipGetLastError());
+}
+
+// ---- z4source / z2source ---------------------------------------------------
+
+// Site i of checkerboard par; the reference slot (isite, ieo, ibleo) of the
+// site gives the draw of (colour ic, spin id): z4source fills each
+// checkerboard in the order ic, isite, id, ieo, ibleo (ibleo fastest),
+// disconloops.f90:4518-4569 (Z4: 1 for u > 0.75, -1 for 0.5 < u < 0.75, i for
+// 0.25 < u < 0.5, else -i) and :4583-4649 (Z2: 1 for u > 0.5, else -1).
+__global__ void k_lag_noise(int n0, int n1, int n2, int n3, int Vh, long nvh, int par, long stride,
+                            const uint32_t *y, int kind, double scale, double2 *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Vh) return;
+  int x[4];
+  {
+    const int r = 2 * i;
+    const int x0 = r % n0;
+    int q = r / n0;
+    x[1] = q % n1;
+    q /= n1;
+    x[2] = q % n2;
+    x[3] = q / n2;
+    x[0] = x0 + ((par + x[1] + x[2] + x[3]) & 1);
+  }
+  const int ns[4] = {n0 / 2, n1 / 2, n2 / 2, n3 / 2};
+  int b[4], sv[4];
+  for (int m = 0; m < 4; m++) {
+    b[m] = x[m] & 1;
+    sv[m] = x[m] >> 1;
+  }
+  const long L = sv[0] + (long)ns[0] * (sv[1] + (long)ns[1] * (sv[2] + (long)ns[2] * sv[3]));
+  const long isite = L / 2;
+  const int ieo = (sv[0] + sv[1] + sv[2] + sv[3]) & 1;
+  const int ibleo = (b[0] + 2 * b[1] + 4 * b[2] + 8 * b[3]) >> 1;
+  for (int d = 0; d < 4; d++)
+    for (int ic = 0; ic < 3; ic++) {
+      const long e = (long)ic * 64 * nvh + isite * 64 + d * 16 + ieo * 8 + ibleo;
+      const double u = (double)(int)y[e] * LAG_FNORM;
+      double re = 0, im = 0;
+      if (kind == 1) re = u > 0.5 ? 1.0 : -1.0;
+      else if (u > 0.75) re = 1;
+      else if (0.5 < u && u < 0.75) re = -1;
+      else if (0.25 < u && u < 0.5) im = 1;
+      else im = -1;
+      out[(long)(3 * d + ic) * stride + i] = make_double2(scale * re, scale * im);
+    }
+}
+
+void lag_noise_source(const Geom &g, Spinor &out, int par, const uint32_t *d_draws, int kind, double scale,
+                      hipStream_t s) {
+  if (out.prec != PREC_DOUBLE) throw std::runtime_error("lag_noise_source: FP64 spinor expected");
+  if (kind != 0 && kind != 1) throw std::runtime_error("lag_noise_source: kind 0 (Z4) or 1 (Z2)");
+  for (int m = 0; m < 4; m++)
+    if (g.n[m] % 2) throw std::runtime_error("lag_noise_source: even extents expected");
+  const int nb = (int)((g.Vh + 255) / 256);
+  hipLaunchKernelGGL(k_lag_noise, dim3(nb), dim3(256), 0, s, g.n[0], g.n[1], g.n[2], g.n[3], (int)g.Vh, g.V / 32,
+                     par, out.stride, d_draws, kind, scale, (double2 *)out.data);
+  QQCD_HIP(hipGetLastError());
+}
+
+}  // namespace qqcd

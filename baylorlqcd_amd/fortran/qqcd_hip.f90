@@ -16,7 +16,8 @@
 ! procedures) with `use qqcd_hip`; everything else in the host is unchanged.
 !
 ! The lattice extents come from the host's own latdims module (nx, ny, nz,
-! nt), the library starts on the first call.  Only one process is
+! nt) and the random stream from its lagfib module (iwlagfib, jrlagfib,
+! krlagfib); the library starts on the first call.  Only one process is
 ! supported: the geometry arguments (vecbl, vecblinv, myid, nn, ldiv, nms,
 ! lvbc, ib, lbd, iblv) are checked against the single-process tables and
 ! otherwise unused.  Errors are reported as the reference reports them: a
@@ -30,19 +31,16 @@
 module qqcd_hip
    use iso_c_binding
    use latdims, only: nx, ny, nz, nt
+   use lagfib, only: iwlagfib, jrlagfib, krlagfib
    implicit none
    private
 
    public :: Hsingle, Hdouble, Hdbletm, gammamult, gam5x, vecdot, bicgstab, gmresdr, psc, &
              gmresproject, pointsource, fermprop, qqcd_gauge_changed, qqcd_shutdown, &
-             qqcd_solve_mixed_f, z4source, z2source, qqcd_z4_seed, loopops, loopops_all, nsaverage, &
+             qqcd_solve_mixed_f, z4source, z2source, loopops, loopops_all, nsaverage, &
              GFwrite, GFread, FPwrite, FPread
 
    logical, save :: started = .false.
-   ! the Z4 stream: seed and the index of the next source (the reference's
-   ! z4source advances its lagged-Fibonacci stream on every call)
-   integer(c_long_long), save :: z4_seed = 1_c_long_long
-   integer(c_long), save :: z4_next = 0_c_long
 
    interface
       integer(c_int) function c_init(n, bc, device) bind(C, name="qqcd_init")
@@ -136,19 +134,15 @@ module qqcd_hip
          real(c_double), value :: tol, itol
          integer(c_int), value :: basis, opp, recon, keep
       end function
-      integer(c_int) function c_z4source(be, bo, seed, rhs, scale) bind(C, name="qqcd_z4source")
-         import :: c_int, c_ptr, c_double, c_long_long, c_long
+      integer(c_int) function c_z4source_lagfib(be, bo, iw, jr, kr) bind(C, name="qqcd_z4source_lagfib")
+         import :: c_int, c_ptr
          type(c_ptr), value :: be, bo
-         integer(c_long_long), value :: seed
-         integer(c_long), value :: rhs
-         real(c_double), value :: scale
+         integer(c_int), intent(inout) :: iw(521), jr, kr
       end function
-      integer(c_int) function c_z2source(be, bo, seed, rhs, scale) bind(C, name="qqcd_z2source")
-         import :: c_int, c_ptr, c_double, c_long_long, c_long
+      integer(c_int) function c_z2source_lagfib(be, bo, iw, jr, kr) bind(C, name="qqcd_z2source_lagfib")
+         import :: c_int, c_ptr
          type(c_ptr), value :: be, bo
-         integer(c_long_long), value :: seed
-         integer(c_long), value :: rhs
-         real(c_double), value :: scale
+         integer(c_int), intent(inout) :: iw(521), jr, kr
       end function
       integer(c_int) function c_loopops(ksub, xe, xo, u, be, bo, bc, Je, Jo) bind(C, name="qqcd_loopops")
          import :: c_int, c_ptr
@@ -476,35 +470,27 @@ contains
    end subroutine pointsource
 
    ! z4source(be,bo,nvhalf,myid), qqcd-test/cfgsprops/quark/disconloops.f90:4359: Z4 noise
-   ! {1, -1, i, -i} on every (site, spin, colour) of both checkerboards; successive calls give
-   ! successive sources of the stream set by qqcd_z4_seed.
+   ! {1, -1, i, -i} on every (site, spin, colour) of both checkerboards from the host's own
+   ! lagged-Fibonacci stream (module lagfib, qqcd-test/cfgsprops/glue/lagfib.f90): the draws are
+   ! made on the device and iwlagfib, jrlagfib, krlagfib advanced past them, so the source and
+   ! the stream afterwards are the reference's, bit for bit.
    subroutine z4source(be,bo,nvhalf,myid)
       real(kind=8), intent(out), target, contiguous :: be(:,:,:,:,:), bo(:,:,:,:,:)
       integer, intent(in) :: nvhalf, myid
       if (.not. started) call fail("DISCONLOOPS", -1_c_int)
-      if (myid /= 0 .or. size(be, 2) < nvhalf) call fail("DISCONLOOPS", -1_c_int)
-      call fail("DISCONLOOPS", c_z4source(c_loc(be), c_loc(bo), z4_seed, z4_next, 1.0d0))
-      z4_next = z4_next + 1
+      if (size(be, 2) < nvhalf .or. nvhalf /= nx*ny*nz*nt/32) call fail("DISCONLOOPS", -1_c_int)
+      call fail("DISCONLOOPS", c_z4source_lagfib(c_loc(be), c_loc(bo), iwlagfib, jrlagfib, krlagfib))
    end subroutine z4source
 
    ! z2source(be,bo,nvhalf), qqcd-test/cfgsprops/quark/disconloops.f90:4583: Z2 noise {1, -1}
-   ! in the real part, from the same stream as z4source
+   ! in the real part, from the same stream
    subroutine z2source(be,bo,nvhalf)
       real(kind=8), intent(out), target, contiguous :: be(:,:,:,:,:), bo(:,:,:,:,:)
       integer, intent(in) :: nvhalf
       if (.not. started) call fail("DISCONLOOPS", -1_c_int)
-      if (size(be, 2) < nvhalf) call fail("DISCONLOOPS", -1_c_int)
-      call fail("DISCONLOOPS", c_z2source(c_loc(be), c_loc(bo), z4_seed, z4_next, 1.0d0))
-      z4_next = z4_next + 1
+      if (size(be, 2) < nvhalf .or. nvhalf /= nx*ny*nz*nt/32) call fail("DISCONLOOPS", -1_c_int)
+      call fail("DISCONLOOPS", c_z2source_lagfib(c_loc(be), c_loc(bo), iwlagfib, jrlagfib, krlagfib))
    end subroutine z2source
-
-   ! restart the noise stream: the next z4source / z2source call gives source `first` of seed `seed`
-   subroutine qqcd_z4_seed(seed, first)
-      integer(kind=8), intent(in) :: seed
-      integer, intent(in) :: first
-      z4_seed = int(seed, c_long_long)
-      z4_next = int(first, c_long)
-   end subroutine qqcd_z4_seed
 
    ! loopops(ksub,xe,xo,u,be,bo,Je,Jo,ieo,ibleo,bc,...), disconloops.f90:7769: the nine loop
    ! operators (point-split currents J_1..J_4, local scalar, local vector currents) of sink x and

@@ -1,12 +1,19 @@
 ! test_shim.f90 -- a Fortran host calling the drop-in with the reference's
 ! argument lists (8^3x16, one process).  Reads u.bin, be.bin, bo.bin (raw
-! float64 in the reference order) from the working directory, writes the
-! outputs next to them; tests/test_gpu_fortran.py compares them with the
-! oracle.
+! float64 in the reference order; sink_e.bin, sink_o.bin for the loop
+! operators) from the working directory, writes the outputs next to them;
+! tests/test_gpu_fortran.py and tests/test_fortran_files.py compare them with
+! the oracle and the reference's files.
+!
+! The first argument selects sections (default: all): "ops" (operators,
+! solves, loop operators), "noise" (z4source / z2source from the lagfib
+! stream; needs "ops" first), "files" (GFwrite / GFread / FPwrite / FPread of
+! the inputs, host only: no device is touched).
 program test_shim
    use qqcd_hip
    use gmresrhs, only: isignal
    use latdims, only: nx, ny, nz, nt
+   use lagfib, only: initlagfib
    implicit none
    integer, parameter :: nvhalf = nx*ny*nz*nt/32
    integer, parameter :: nbmax = max(2*nvhalf/nx, 2*nvhalf/ny, 2*nvhalf/nz, 2*nvhalf/nt)
@@ -19,6 +26,8 @@ program test_shim
    integer, allocatable :: lvbc(:,:,:), ib(:,:,:,:)
    logical :: ldiv(4), lbd(16,4)
    character(len=8) :: rwdir(1)
+   character(len=32) :: sel
+   logical :: do_ops, do_noise, do_files
    integer :: src(4)
 
    allocate(u(18,ntotal,4,2,16), be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8), g(6,ntotal,4,2,8), h(6,ntotal,4,2,8))
@@ -27,6 +36,10 @@ program test_shim
    call readbin("u.bin", u, size(u))
    call readbin("be.bin", be, size(be))
    call readbin("bo.bin", bo, size(bo))
+   call get_command_argument(1, sel)
+   do_ops = len_trim(sel) == 0 .or. index(sel, "ops") > 0
+   do_noise = len_trim(sel) == 0 .or. index(sel, "noise") > 0
+   do_files = len_trim(sel) == 0 .or. index(sel, "files") > 0
    ! single-process geometry tables (setblk / gblchker of the reference)
    vecbl(1,:) = [1, 4, 6, 7, 10, 11, 13, 16]
    vecbl(2,:) = [2, 3, 5, 8, 9, 12, 14, 15]
@@ -50,6 +63,24 @@ program test_shim
    coact(3,2,:) = 1.0d0
    gdummy = 0.0d0
 
+   if (do_ops) call ops()
+   if (do_noise) call noise()
+   ! the reference's files: the links (re-unitarised in place by GFwrite) and the input spinor
+   ! pair as a propagator, read back into fresh arrays
+   if (do_files) call shim_files()
+   if (do_ops .or. do_noise) call qqcd_shutdown()
+   print *, "test_shim done"
+
+contains
+
+   subroutine step(name)
+      character(len=*), intent(in) :: name
+      print *, "step ", name
+      flush(6)
+   end subroutine step
+
+   subroutine ops()
+   call step("operators")
    call Hsingle(g, u, be, 0, coact, bc, 2, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0)
    call writebin("hsingle.bin", g, size(g))
    kappa = [0.156d0, 0.005d0]
@@ -63,6 +94,7 @@ program test_shim
    call vecdot(be, bo, adb, 0)
    call writebin("vecdot.bin", adb, 2)
 
+   call step("fermprop")
    ! config-1 solve: point source at the origin, GMRES-DR(40,10), Wilson kappa 0.150
    src = [1, 1, 1, 1]
    call pointsource(ps_e, ps_o, src, 1, 1, 0, iblv, vecblinv)
@@ -79,21 +111,15 @@ program test_shim
    call fermprop(rwdir, ps_e, ps_o, -1, kappa, 0.0d0, 6, coact, bc, 1.0d-10, 0, 1.0d0, u, vectype, 10, xe, xo, &
                  0, nn, ldiv, nms, lvbc, lvbc, ib, lbd, iblv, vecbl, vecblinv, 0, 0, 0, 0)
    call writebin("xe_proj.bin", xe, size(xe))
+   call step("mixed")
    ! mixed precision on the even-odd right-hand side of the first solve
    g = 0.0d0
    call qqcd_solve_mixed_f(be, g, [40, 16], 1.0d-10, 2, 12, 0, u, kappa, coact, bc, itercount)
    call writebin("x_mixed.bin", g, size(g))
-   ! two Z4 sources of seed 5: the second one is written out
-   call qqcd_z4_seed(5_8, 0)
-   call z4source(g, h, nvhalf, 0)
-   call z4source(g, h, nvhalf, 0)
-   call writebin("z4_e.bin", g, size(g))
-   call writebin("z4_o.bin", h, size(h))
-   ! the next source of the stream as Z2 noise (index 2)
-   call z2source(g, h, nvhalf)
-   call writebin("z2_e.bin", g, size(g))
-   call writebin("z2_o.bin", h, size(h))
-   ! loop operators of sink g/h (the Z2 source above) and source be/bo: slice (ieo, ibleo) = (2, 3)
+   call step("loops")
+   call readbin("sink_e.bin", g, size(g))
+   call readbin("sink_o.bin", h, size(h))
+   ! loop operators of sink g/h and source be/bo: slice (ieo, ibleo) = (2, 3)
    ! of loopops with ksub = 1, and nsaverage at kappa 0.15
    allocate(jse(2,nvhalf,9), jso(2,nvhalf,9), jtime(2,nt,5,9))
    call loopops(1, g, h, u, be, bo, jse, jso, 2, 3, bc, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0)
@@ -102,18 +128,27 @@ program test_shim
    call nsaverage(jtime, u, be, bo, g, h, 0.15d0, coact, bc, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, &
                   iblv, 0, 1)
    call writebin("nsaverage.bin", jtime, size(jtime))
-   ! the reference's files: the links (re-unitarised in place by GFwrite) and the solution
-   ! of the first solve, read back into fresh arrays
-   call shim_files()
-   call qqcd_shutdown()
-   print *, "test_shim done"
+   end subroutine ops
 
-contains
+   subroutine noise()
+   call step("noise")
+   ! two Z4 sources of the stream of initlagfib(0, 5): the second one is written out
+   call initlagfib(0, 5)
+   call z4source(g, h, nvhalf, 0)
+   call z4source(g, h, nvhalf, 0)
+   call writebin("z4_e.bin", g, size(g))
+   call writebin("z4_o.bin", h, size(h))
+   ! the next source of the stream as Z2 noise
+   call z2source(g, h, nvhalf)
+   call writebin("z2_e.bin", g, size(g))
+   call writebin("z2_o.bin", h, size(h))
+   end subroutine noise
 
    subroutine shim_files()
       real(kind=8), allocatable :: uw(:,:,:,:,:), ur(:,:,:,:,:), fe(:,:,:,:,:), fo(:,:,:,:,:)
       real(kind=8) :: beta, alat(4), tad(4), fk, ft, fc, meta(13)
       integer :: gaction(4), icfg, myidin, icsrc, idsrc
+      call step("files")
       allocate(uw(18,ntotal,4,2,16), ur(18,ntotal,4,2,16), fe(6,ntotal,4,2,8), fo(6,ntotal,4,2,8))
       uw = u
       ur = 0.0d0
@@ -122,7 +157,7 @@ contains
       call GFread(ur, "shim.gf", gaction, beta, alat, tad, icfg, myidin)
       call writebin("gfread_u.bin", ur, size(ur))
       call writebin("gfwrite_u.bin", uw, size(uw))
-      call FPwrite(xe(:,:,:,:,:,1), xo(:,:,:,:,:,1), vecblinv, "shim.fp", 0.156d0, 0.005d0, 0.0d0, 2, 3, 7, 0)
+      call FPwrite(be, bo, vecblinv, "shim.fp", 0.156d0, 0.005d0, 0.0d0, 2, 3, 7, 0)
       fe = 0.0d0
       fo = 0.0d0
       call FPread(fe, fo, vecblinv, "shim.fp", fk, ft, fc, icsrc, idsrc, icfg, myidin)

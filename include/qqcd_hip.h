@@ -277,6 +277,29 @@ int qqcd_propagator(int nrhs, const int src[4], const double kappa[2], const int
                     int op_recon, int method, double *const *xe, double *const *xo, int *iters, double *res_full,
                     double *times, int *stats);
 
+/* ---- the reference's lagged-Fibonacci stream ---------------------------
+ * State: iw[521], *jr, *kr = iwlagfib(0:520), jrlagfib, krlagfib of module
+ * lagfib (qqcd-test/cfgsprops/glue/lagfib.f90:58-61); kr must equal jr - 32
+ * modulo 521.  Every call below reads the state and writes the advanced state
+ * back, so a host can pass its module variables and keep calling its own
+ * rnds afterwards. */
+
+/* initlagfib (lagfib.f90:80-92, with initbirep :96-138 and bigdelay
+ * :142-216): the state of process myid for seed iseed.  Host only. */
+int qqcd_lagfib_init(int myid, int iseed, int *iw, int *jr, int *kr);
+
+/* rnds (lagfib.f90:219-259): m numbers into rn (host), generated on the
+ * device (jump-ahead over chunks), bit-identical to the reference's,
+ * including its skip of zero draws. */
+int qqcd_rnds(long m, double *rn, int *iw, int *jr, int *kr);
+
+/* z4source / z2source (qqcd-test/cfgsprops/quark/disconloops.f90:4359-4649)
+ * from the reference's stream: be, bo (6, ntotal, 4, 2, 8), 384 nvhalf draws;
+ * bit-identical to the reference's sources.  Either output may be NULL (its
+ * draws are still consumed). */
+int qqcd_z4source_lagfib(double *be, double *bo, int *iw, int *jr, int *kr);
+int qqcd_z2source_lagfib(double *be, double *bo, int *iw, int *jr, int *kr);
+
 /* ---- the reference's files (host only: n = lattice extents, no device) --- */
 
 /* GFwrite (qqcd-test/cfgsprops/glue/gaugetools.f90:1259-1292): fixsu3 is

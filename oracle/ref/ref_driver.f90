@@ -25,7 +25,7 @@ module refdrv
    use gaugetools
    use heatbath
    use lagfib
-   use disconloops, only: loopops, momfacs, spacesum, nsaverage
+   use disconloops, only: loopops, momfacs, spacesum, nsaverage, z4source, z2source
 #endif
    implicit none
    include 'mpif.h'
@@ -229,6 +229,75 @@ contains
          end do
       end do
    end subroutine ref_heatbath
+
+   ! The lagged-Fibonacci state (lagfib module variables): initlagfib(myid, iseed), and
+   ! get / set of iwlagfib(0:520), jrlagfib, krlagfib.
+   subroutine ref_lagfib_init(myidin, iseed, iw, jr, kr) bind(C, name="ref_lagfib_init")
+      integer(c_int), value :: myidin, iseed
+      integer(c_int), intent(out) :: iw(0:520), jr, kr
+      call initlagfib(myidin, iseed)
+      iw = iwlagfib
+      jr = jrlagfib
+      kr = krlagfib
+   end subroutine ref_lagfib_init
+
+   subroutine ref_lagfib_get(iw, jr, kr) bind(C, name="ref_lagfib_get")
+      integer(c_int), intent(out) :: iw(0:520), jr, kr
+      iw = iwlagfib
+      jr = jrlagfib
+      kr = krlagfib
+   end subroutine ref_lagfib_get
+
+   subroutine ref_lagfib_set(iw, jr, kr) bind(C, name="ref_lagfib_set")
+      integer(c_int), intent(in) :: iw(0:520)
+      integer(c_int), value :: jr, kr
+      iwlagfib = iw
+      jrlagfib = jr
+      krlagfib = kr
+   end subroutine ref_lagfib_set
+
+   ! rnds(m, rn) from the current state
+   subroutine ref_rnds(m, rn) bind(C, name="ref_rnds")
+      integer(c_int), value :: m
+      real(c_double), intent(out) :: rn(m)
+      call rnds(m, rn)
+   end subroutine ref_rnds
+
+   ! z4source / z2source from the current state
+   subroutine ref_z4source(be, bo) bind(C, name="ref_z4source")
+      real(c_double), intent(out) :: be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8)
+      call z4source(be, bo, nvhalf, myid)
+   end subroutine ref_z4source
+
+   subroutine ref_z2source(be, bo) bind(C, name="ref_z2source")
+      real(c_double), intent(out) :: be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8)
+      call z2source(be, bo, nvhalf)
+   end subroutine ref_z2source
+
+   ! nsweep sweeps (the loop of sweep, cfgsprops.f90:1973-2003) continuing the current
+   ! stream, with the given gaction
+   subroutine ref_sweeps(u, coact, gaction, nsweep, nhit) bind(C, name="ref_sweeps")
+      real(c_double), intent(inout) :: u(18,ntotal,4,2,16)
+      real(c_double), intent(in)    :: coact(4,4,4)
+      integer(c_int), intent(in)    :: gaction(4)
+      integer(c_int), value         :: nsweep, nhit
+      integer(kind=KI) :: isw, ieo, ibl, mu, MRT
+      integer(kind=KI) :: ga(4)
+      real(kind=KR)    :: c(18,nvhalf)
+      MRT = MPI_DOUBLE_PRECISION
+      ga = gaction
+      do isw = 1, nsweep
+         do ieo = 1, 2
+            do ibl = 1, 16
+               do mu = 1, 4
+                  call staple(mu, ieo, ibl, coact, ga, c, u, myid, nn, ldiv, nms, lv, ib, &
+                              lbd, iblv, MRT)
+                  call phbup(mu, ieo, ibl, c, u, nhit)
+               end do
+            end do
+         end do
+      end do
+   end subroutine ref_sweeps
 
    ! average plaquette, aveplaq (qqcd-test/cfgsprops/glue/gaugetools.f90:1436-1474)
    subroutine ref_aveplaq(u, plaq) bind(C, name="ref_aveplaq")

@@ -197,6 +197,42 @@ class RefLattice:
         _run(self.L.ref_heatbath, _p(uu), _p(np.ascontiguousarray(coact)), int(nsweep), int(nhit), int(iseed))
         return uu
 
+    def lagfib_init(self, myid, iseed):
+        iw = np.zeros(521, dtype=np.int32)
+        jr, kr = ctypes.c_int(), ctypes.c_int()
+        _run(self.L.ref_lagfib_init, int(myid), int(iseed), _ip_(iw), ctypes.byref(jr), ctypes.byref(kr))
+        return iw, jr.value, kr.value
+
+    def lagfib_get(self):
+        iw = np.zeros(521, dtype=np.int32)
+        jr, kr = ctypes.c_int(), ctypes.c_int()
+        _run(self.L.ref_lagfib_get, _ip_(iw), ctypes.byref(jr), ctypes.byref(kr))
+        return iw, jr.value, kr.value
+
+    def lagfib_set(self, iw, jr, kr):
+        _run(self.L.ref_lagfib_set, _ip_(np.ascontiguousarray(iw, dtype=np.int32)), int(jr), int(kr))
+
+    def rnds(self, m):
+        rn = np.zeros(int(m))
+        _run(self.L.ref_rnds, int(m), _p(rn))
+        return rn
+
+    def z4source(self):
+        be, bo = np.zeros(self.slen), np.zeros(self.slen)
+        _run(self.L.ref_z4source, _p(be), _p(bo))
+        return be, bo
+
+    def z2source(self):
+        be, bo = np.zeros(self.slen), np.zeros(self.slen)
+        _run(self.L.ref_z2source, _p(be), _p(bo))
+        return be, bo
+
+    def sweeps(self, u, coact, nsweep, gaction=(1, 1, 1, 1), nhit=1):
+        uu = np.array(u, dtype=np.float64, copy=True)
+        _run(self.L.ref_sweeps, _p(uu), _p(np.ascontiguousarray(coact)),
+             _ip_(np.ascontiguousarray(gaction, dtype=np.int32)), int(nsweep), int(nhit))
+        return uu
+
     def aveplaq(self, u):
         uu = np.array(u, dtype=np.float64, copy=True)
         p = ctypes.c_double()

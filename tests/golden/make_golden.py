@@ -6,7 +6,9 @@ tests/golden/golden_4x4x4x4.npz and tests/golden/golden_8x8x8x16.npz;
 with the argument ``loops`` only tests/golden/golden_loops_8x8x8x16.npz
 (the loop operators loopops / spacesum / nsaverage of the noise loop); with
 ``io`` only tests/golden/golden_io_8x8x8x16.json (SHA-256 digests of the
-reference's GFwrite / FPwrite files and of what GFread / FPread return).
+reference's GFwrite / FPwrite files and of what GFread / FPread return); with
+``lagfib`` only tests/golden/golden_lagfib_8x8x8x16.npz (initlagfib states,
+rnds draws including the zero skip, z4source / z2source digests).
 
 Inputs are regenerated from seeds by the tests (spinors: numpy
 default_rng(seed) Gaussians; links: baylorlqcd_amd.gauge.haar), except the
@@ -238,10 +240,54 @@ def io():
     print("wrote golden_io_8x8x8x16.json")
 
 
+LAG_INIT = [(0, 1), (0, 2), (3, 1), (0, 1234567), (5, 99)]
+
+
+def lagfib():
+    import hashlib
+    n = (8, 8, 8, 16)
+    refdrv.build(n, solvers=True)
+    R = refdrv.RefLattice(n, bc=(1, 1, 1, -1))
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()  # noqa: E731
+    out = {}
+    states = [R.lagfib_init(my, sd) for my, sd in LAG_INIT]
+    out["init_cases"] = np.array(LAG_INIT, dtype=np.int32)
+    out["init_iw"] = np.stack([s[0] for s in states])
+    out["init_jrkr"] = np.array([s[1:] for s in states], dtype=np.int32)
+    # from initlagfib(0, 1): 1000 draws, then 200003 more (digest), then 100 from a state whose
+    # next draw is zero
+    R.lagfib_init(0, 1)
+    out["rn_1000"] = R.rnds(1000)
+    iw, jr, kr = R.lagfib_get()
+    out["after_1000_iw"], out["after_1000_jrkr"] = iw, np.array([jr, kr], np.int32)
+    out["rn_200003_sha256"] = np.array(sha(R.rnds(200003)))
+    iw, jr, kr = R.lagfib_get()
+    out["after_200003_iw"], out["after_200003_jrkr"] = iw, np.array([jr, kr], np.int32)
+    iw = iw.copy()
+    iw[kr] = iw[jr]
+    R.lagfib_set(iw, jr, kr)
+    out["zero_in_iw"], out["zero_in_jrkr"] = iw, np.array([jr, kr], np.int32)
+    out["rn_zero_100"] = R.rnds(100)
+    iw, jr, kr = R.lagfib_get()
+    out["zero_out_iw"], out["zero_out_jrkr"] = iw, np.array([jr, kr], np.int32)
+    # z4source then z2source from initlagfib(0, 5)
+    R.lagfib_init(0, 5)
+    be, bo = R.z4source()
+    out["z4_be_sha256"], out["z4_bo_sha256"] = np.array(sha(be)), np.array(sha(bo))
+    be, bo = R.z2source()
+    out["z2_be_sha256"], out["z2_bo_sha256"] = np.array(sha(be)), np.array(sha(bo))
+    iw, jr, kr = R.lagfib_get()
+    out["after_z_iw"], out["after_z_jrkr"] = iw, np.array([jr, kr], np.int32)
+    np.savez_compressed(os.path.join(HERE, "golden_lagfib_8x8x8x16.npz"), **out)
+    print("wrote golden_lagfib_8x8x8x16.npz")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["loops"]:
         loops()
     elif sys.argv[1:] == ["io"]:
         io()
+    elif sys.argv[1:] == ["lagfib"]:
+        lagfib()
     else:
         main()
