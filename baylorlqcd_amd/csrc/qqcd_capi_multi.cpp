@@ -230,6 +230,24 @@ extern "C" int qqcd_z2source_lagfib(double *be, double *bo, int *iw, int *jr, in
   return lag_source(be, bo, iw, jr, kr, 1);
 }
 
+namespace qqcd {
+void heatbath_reference(Context &c, const int gaction[4], const double *coact, int nhit, int nsweep, double *u_host,
+                        LagState &st);
+}
+
+extern "C" int qqcd_sweep(const int *gaction, const double *coact, int nhit, int nsweep, double *u, int *iw,
+                          int *jr, int *kr) {
+  API_BEGIN
+  Context &c = ctx();
+  if (!gaction || !coact || !u) throw std::runtime_error("qqcd_sweep: null argument");
+  LagState st = lag_in(iw, jr, kr);
+  heatbath_reference(c, gaction, coact, nhit, nsweep, u, st);
+  lag_out(st, iw, jr, kr);
+  // the links changed in place: the next call that passes u uploads them again
+  c.u_host = nullptr;
+  API_END
+}
+
 extern "C" int qqcd_hop_multi_canonical(int nrhs, int nact, int nb, int prec, int recon, int idag, int par_out,
                                         int order, const double *in, double *out) {
   API_BEGIN

@@ -1,0 +1,501 @@
+// qqcd_heatbath_ref.hip -- the reference's pseudo-heatbath sweep, step for
+// step, on the device, drawing from the reference's lagged-Fibonacci stream.
+//
+// sweep (qqcd-test/cfgsprops/cfgsprops.f90:1973-2003) visits the 128 link
+// sets (ieo, ibl, mu) -- ieo outermost, mu innermost -- each holding the
+// links U_mu(x) of the nvhalf sites x of block ibl and block-internal parity
+// ieo.  No link of a set appears in the staples of another link of the same
+// set, so a set is one data-parallel step; its sites are taken in the
+// reference's site order i = 1..nvhalf (one lane per site) because that order
+// decides which random numbers each site receives:
+//   staple (qqcd-test/cfgsprops/glue/gaugetools.f90:39-345, plaquette terms):
+//     c = sum_{nu != mu} coact(1,mu,nu) [mmd(mm(U_nu(x), U_mu(x+nu)), U_nu(x+mu))
+//                                      + mm(mdm(U_nu(x-nu), U_mu(x-nu)), U_nu(x-nu+mu))]
+//   phbup (qqcd-test/cfgsprops/glue/heatbath.f90:33-76), for the SU(2)
+//   subgroups (1,2), (2,3), (1,3): phbsub1-3 project rows (p,q) of u c^dagger
+//   to z (4 reals) (:80-235); su2up (:239-323) runs Kennedy-Pendleton passes:
+//   a pass draws 4 numbers for each site not yet accepted, in site order
+//   (4 inotup draws), until every site has accepted; then 2 numbers per site
+//   (2 nvhalf draws) place (a1, a2, a3) on the sphere; sprd1 (:327-467)
+//   multiplies rows p, q of u by the new SU(2) element.
+// The draw of a site in a pass is its rank among the sites not yet accepted
+// (a block scan), so the stream is consumed exactly as the reference
+// consumes it and the stream state after a sweep is the reference's.
+// Every sum and product is evaluated in the reference's written order
+// without fused multiply-adds (mm, mmd, mdm: gaugetools.f90:741-817,
+// 908-984, 1074-1151), so staples and projections are bit-identical for
+// identical links; cos, sin and log come from the device math library and
+// may differ from the host's in the last place.
+//
+// Links live on the device in the reference's own layout
+// u(18, ntotal, 4, 2, 16) for the duration of the call.  Only gaction = 1
+// in every direction (plaquette staples) is implemented; other actions are
+// rejected.
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qqcd_context.h"
+#include "qqcd_lagfib.h"
+
+#pragma clang fp contract(off)
+
+namespace qqcd {
+
+static constexpr int HB_TB = 256;
+static constexpr int HB_PB = 6;  // Kennedy-Pendleton passes per batch between host checks
+// su2up's twoPi: a single-precision literal held in a double
+static constexpr double HB_TWOPI = (double)6.283185307179586f;
+
+struct HbG {
+  int n[4];
+  long nvh, ntot;
+};
+
+// coordinates of reference site i (0-based) of block ibl, block-internal parity ieo
+__device__ __forceinline__ void hb_site(const HbG &G, long i, int ieo, int ibl, int x[4]) {
+  const int ns0 = G.n[0] / 2, ns1 = G.n[1] / 2, ns2 = G.n[2] / 2;
+  const long L = 2 * i;
+  int s[4];
+  s[0] = (int)(L % ns0);
+  long q = L / ns0;
+  s[1] = (int)(q % ns1);
+  q /= ns1;
+  s[2] = (int)(q % ns2);
+  s[3] = (int)(q / ns2);
+  if (((s[0] + s[1] + s[2] + s[3]) & 1) != ieo) s[0] += 1;  // ns0 is even: no carry
+  for (int m = 0; m < 4; m++) x[m] = 2 * s[m] + ((ibl >> m) & 1);
+}
+
+// offset of u(1, i, mu, ieo, ibl) for the link U_mu(x)
+__device__ __forceinline__ long hb_link(const HbG &G, const int x[4], int mu) {
+  int s[4], b = 0;
+  for (int m = 0; m < 4; m++) {
+    s[m] = x[m] >> 1;
+    b |= (x[m] & 1) << m;
+  }
+  const long L = s[0] + (long)(G.n[0] / 2) * (s[1] + (long)(G.n[1] / 2) * (s[2] + (long)(G.n[2] / 2) * s[3]));
+  const int ieo = (s[0] + s[1] + s[2] + s[3]) & 1;
+  return 18L * (L / 2 + G.ntot * (mu + 4L * (ieo + 2L * b)));
+}
+
+__device__ __forceinline__ void hb_load(const double *u, long off, double a[18]) {
+#pragma unroll
+  for (int k = 0; k < 18; k++) a[k] = u[off + k];
+}
+
+// w = y v (mm)
+__device__ __forceinline__ void hb_mm(const double y[18], const double v[18], double w[18]) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double *a = y + 6 * r;
+      w[6 * r + 2 * c] = a[0] * v[2 * c] - a[1] * v[2 * c + 1] + a[2] * v[6 + 2 * c] - a[3] * v[7 + 2 * c] +
+                         a[4] * v[12 + 2 * c] - a[5] * v[13 + 2 * c];
+      w[6 * r + 2 * c + 1] = a[0] * v[2 * c + 1] + a[1] * v[2 * c] + a[2] * v[7 + 2 * c] + a[3] * v[6 + 2 * c] +
+                             a[4] * v[13 + 2 * c] + a[5] * v[12 + 2 * c];
+    }
+}
+
+// w = y v^dagger (mmd)
+__device__ __forceinline__ void hb_mmd(const double y[18], const double v[18], double w[18]) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double *a = y + 6 * r, *b = v + 6 * c;
+      w[6 * r + 2 * c] = a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3] + a[4] * b[4] + a[5] * b[5];
+      w[6 * r + 2 * c + 1] = -(a[0] * b[1]) + a[1] * b[0] - a[2] * b[3] + a[3] * b[2] - a[4] * b[5] + a[5] * b[4];
+    }
+}
+
+// w = y^dagger v (mdm)
+__device__ __forceinline__ void hb_mdm(const double y[18], const double v[18], double w[18]) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      w[6 * r + 2 * c] = y[2 * r] * v[2 * c] + y[2 * r + 1] * v[2 * c + 1] + y[6 + 2 * r] * v[6 + 2 * c] +
+                         y[7 + 2 * r] * v[7 + 2 * c] + y[12 + 2 * r] * v[12 + 2 * c] + y[13 + 2 * r] * v[13 + 2 * c];
+      w[6 * r + 2 * c + 1] = y[2 * r] * v[2 * c + 1] - y[2 * r + 1] * v[2 * c] + y[6 + 2 * r] * v[7 + 2 * c] -
+                             y[7 + 2 * r] * v[6 + 2 * c] + y[12 + 2 * r] * v[13 + 2 * c] -
+                             y[13 + 2 * r] * v[12 + 2 * c];
+    }
+}
+
+// staple sum of the links U_mu(x), x in set (ieo, ibl): C[k*nvh + i]
+__global__ void __launch_bounds__(HB_TB) k_hb_staple(HbG G, const double *u, int ieo, int ibl, int mu, double w0,
+                                                     double w1, double w2, double w3, double *C) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.nvh) return;
+  const double wcoef[4] = {w0, w1, w2, w3};
+  int x[4];
+  hb_site(G, i, ieo, ibl, x);
+  double c[18];
+#pragma unroll
+  for (int k = 0; k < 18; k++) c[k] = 0.0;
+  for (int nu = 0; nu < 4; nu++) {
+    if (nu == mu) continue;
+    int xpn[4], xpm[4], xmn[4], xmnpm[4];
+    for (int m = 0; m < 4; m++) xpn[m] = xpm[m] = xmn[m] = x[m];
+    xpn[nu] = (x[nu] + 1) % G.n[nu];
+    xpm[mu] = (x[mu] + 1) % G.n[mu];
+    xmn[nu] = (x[nu] + G.n[nu] - 1) % G.n[nu];
+    for (int m = 0; m < 4; m++) xmnpm[m] = xmn[m];
+    xmnpm[mu] = (xmn[mu] + 1) % G.n[mu];
+    double a[18], b[18], t[18], w[18];
+    // staple 1: U_nu(x) U_mu(x+nu) U_nu(x+mu)^dagger
+    hb_load(u, hb_link(G, x, nu), a);
+    hb_load(u, hb_link(G, xpn, mu), b);
+    hb_mm(a, b, t);
+    hb_load(u, hb_link(G, xpm, nu), a);
+    hb_mmd(t, a, w);
+#pragma unroll
+    for (int k = 0; k < 18; k++) c[k] = c[k] + wcoef[nu] * w[k];
+    // staple 2: U_nu(x-nu)^dagger U_mu(x-nu) U_nu(x-nu+mu)
+    hb_load(u, hb_link(G, xmn, nu), a);
+    hb_load(u, hb_link(G, xmn, mu), b);
+    hb_mdm(a, b, t);
+    hb_load(u, hb_link(G, xmnpm, nu), a);
+    hb_mm(t, a, w);
+#pragma unroll
+    for (int k = 0; k < 18; k++) c[k] = c[k] + wcoef[nu] * w[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 18; k++) C[(long)k * G.nvh + i] = c[k];
+}
+
+// rows (p, q) of subgroup isu2 as offsets into the 18 reals; subgroup 2 flips z3
+__device__ __forceinline__ void hb_rows(int isu2, int &P, int &Q, double &s3) {
+  P = isu2 == 2 ? 6 : 0;
+  Q = isu2 == 1 ? 6 : 12;
+  s3 = isu2 == 2 ? -1.0 : 1.0;
+}
+
+// phbsub1-3: z of the link's rows (p, q) against the staple; dk = 1/|z|; the
+// site starts the Kennedy-Pendleton passes not accepted
+__global__ void __launch_bounds__(HB_TB) k_hb_proj(HbG G, const double *u, const double *C, int ieo, int ibl,
+                                                   int mu, int isu2, double *Z, double *DK, int *ID, double *A0) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.nvh) return;
+  int P, Q;
+  double s3;
+  hb_rows(isu2, P, Q, s3);
+  const double *up = u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl)));
+  double a[18], c[18];
+#pragma unroll
+  for (int k = 0; k < 18; k++) {
+    a[k] = up[k];
+    c[k] = C[(long)k * G.nvh + i];
+  }
+  double z1 = a[P] * c[P];
+  for (int k = 1; k < 6; k++) z1 = z1 + a[P + k] * c[P + k];
+  for (int k = 0; k < 6; k++) z1 = z1 + a[Q + k] * c[Q + k];
+  double z2 = -(a[P] * c[Q + 1]);
+  z2 = z2 + a[P + 1] * c[Q];
+  for (int m = 1; m < 3; m++) {
+    z2 = z2 - a[P + 2 * m] * c[Q + 2 * m + 1];
+    z2 = z2 + a[P + 2 * m + 1] * c[Q + 2 * m];
+  }
+  for (int m = 0; m < 3; m++) {
+    z2 = z2 - a[Q + 2 * m] * c[P + 2 * m + 1];
+    z2 = z2 + a[Q + 2 * m + 1] * c[P + 2 * m];
+  }
+  double z3 = a[P] * c[Q];
+  for (int k = 1; k < 6; k++) z3 = z3 + a[P + k] * c[Q + k];
+  for (int k = 0; k < 6; k++) z3 = z3 - a[Q + k] * c[P + k];
+  z3 = s3 * z3;  // phbsub2's sum is this one with every sign flipped: the same value negated
+  double z4 = -(a[P] * c[P + 1]);
+  z4 = z4 + a[P + 1] * c[P];
+  for (int m = 1; m < 3; m++) {
+    z4 = z4 - a[P + 2 * m] * c[P + 2 * m + 1];
+    z4 = z4 + a[P + 2 * m + 1] * c[P + 2 * m];
+  }
+  for (int m = 0; m < 3; m++) {
+    z4 = z4 + a[Q + 2 * m] * c[Q + 2 * m + 1];
+    z4 = z4 - a[Q + 2 * m + 1] * c[Q + 2 * m];
+  }
+  Z[i] = z1;
+  Z[G.nvh + i] = z2;
+  Z[2 * G.nvh + i] = z3;
+  Z[3 * G.nvh + i] = z4;
+  DK[i] = 1.0 / sqrt(z1 * z1 + z2 * z2 + z3 * z3 + z4 * z4);
+  ID[i] = 0;
+  A0[i] = 0.0;
+}
+
+// sites not yet accepted, per workgroup
+__global__ void __launch_bounds__(HB_TB) k_hb_count(long nvh, const int *ID, int *cnt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = (i < nvh && ID[i] == 0) ? 1 : 0;
+  const unsigned long long bal = __ballot(f);
+  __shared__ int ws[HB_TB / 64];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = __popcll(bal);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < HB_TB / 64; w++) s += ws[w];
+    cnt[blockIdx.x] = s;
+  }
+}
+
+// one workgroup: exclusive scan of the counts; the pass draws 4 * total
+// numbers from base (the host's position for the first pass of a batch, the
+// device's running position afterwards)
+__global__ void __launch_bounds__(1024) k_hb_scan(int nblk, const int *cnt, int *off, long *dpos, long base_host,
+                                                 int use_host, long *dbase, long *dtot, int pass) {
+  __shared__ long part[1024];
+  const int t = threadIdx.x;
+  const int per = (nblk + 1023) / 1024;
+  const int b0 = t * per, b1 = min(nblk, b0 + per);
+  long s = 0;
+  for (int b = b0; b < b1; b++) s += cnt[b];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const long v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long run = part[t] - s;  // exclusive
+  for (int b = b0; b < b1; b++) {
+    off[b] = (int)run;
+    run += cnt[b];
+  }
+  if (t == 1023) {
+    const long total = part[1023];
+    const long base = use_host ? base_host : *dpos;
+    dbase[pass] = base;
+    dtot[pass] = total;
+    *dpos = base + 4 * total;
+  }
+}
+
+__device__ __forceinline__ double hb_rn(const uint32_t *y, long j) { return (double)(int)y[j] * LAG_FNORM; }
+
+// one Kennedy-Pendleton pass (su2up :270-297) over the sites not yet accepted
+__global__ void __launch_bounds__(HB_TB) k_hb_try(long nvh, const uint32_t *y, int *ID, const double *DK,
+                                                  double *A0, const int *off, const long *dbase, int pass,
+                                                  int *dacc) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = (i < nvh && ID[i] == 0) ? 1 : 0;
+  const unsigned long long bal = __ballot(f);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ int ws[HB_TB / 64];
+  __shared__ int acc_s;
+  if (lane == 0) ws[wv] = __popcll(bal);
+  if (threadIdx.x == 0) acc_s = 0;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wv; w++) before += ws[w];
+  const unsigned long long lower = lane ? (bal & ((~0ull) >> (64 - lane))) : 0ull;
+  const long rank = (long)off[blockIdx.x] + before + __popcll(lower);
+  int acc = 0;
+  if (f) {
+    const long j = dbase[pass] + 4 * rank;
+    const double r0 = hb_rn(y, j), r1 = hb_rn(y, j + 1), r2 = hb_rn(y, j + 2), r3 = hb_rn(y, j + 3);
+    const double tt = cos(HB_TWOPI * r2);
+    const double d2 = -(log(r0) + log(r1) * tt * tt) * DK[i] * 3.0;
+    if (r3 * r3 <= (1.0 - 0.5 * d2)) {
+      ID[i] = 1;
+      A0[i] = 1.0 - d2;
+      acc = 1;
+    }
+  }
+  if (acc) atomicAdd(&acc_s, 1);
+  __syncthreads();
+  if (threadIdx.x == 0 && acc_s) atomicAdd(&dacc[pass], acc_s);
+}
+
+// su2up's sphere (:300-321) and sprd1 (:327-467): rows (p, q) of the link
+// become R (rows p, q) with R = [[z1 + i z4, s3 z3 + i z2], [-s3 z3 + i z2, z1 - i z4]]
+__global__ void __launch_bounds__(HB_TB) k_hb_hyper(HbG G, double *u, int ieo, int ibl, int mu, int isu2,
+                                                    const uint32_t *y, long pos, const double *Z, const double *DK,
+                                                    const double *A0) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.nvh) return;
+  int P, Q;
+  double s3;
+  hb_rows(isu2, P, Q, s3);
+  const double r0 = hb_rn(y, pos + 2 * i), r1 = hb_rn(y, pos + 2 * i + 1);
+  const double a0 = A0[i], dk = DK[i];
+  const double rad = 1.0 - a0 * a0;
+  const double a3 = sqrt(rad) * (2.0 * r0 - 1.0);
+  const double rad2 = sqrt(fabs(rad - a3 * a3));
+  const double theta = HB_TWOPI * r1;
+  const double a1 = rad2 * cos(theta);
+  const double a2 = rad2 * sin(theta);
+  const double z1 = Z[i], z2 = Z[G.nvh + i], z3 = Z[2 * G.nvh + i], z4 = Z[3 * G.nvh + i];
+  const double n1 = dk * (a0 * z1 + a1 * z2 + a2 * z3 + a3 * z4);
+  const double n2 = dk * (-(a0 * z2) + a1 * z1 + a2 * z4 - a3 * z3);
+  const double n3 = dk * (-(a0 * z3) - a1 * z4 + a2 * z1 + a3 * z2);
+  const double n4 = dk * (-(a0 * z4) + a1 * z3 - a2 * z2 + a3 * z1);
+  // sprd1 with (z1, z2, z3, z4) = (n1, n2, s3 n3, n4): subgroup 2 uses -z3 where 1 and 3 use z3
+  const double t3 = s3 * n3;
+  double *up = u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl)));
+  for (int c = 0; c < 3; c++) {
+    const double pr = up[P + 2 * c], pi = up[P + 2 * c + 1], qr = up[Q + 2 * c], qi = up[Q + 2 * c + 1];
+    const double yp_r = n1 * pr - n4 * pi + t3 * qr - n2 * qi;
+    const double yp_i = n1 * pi + n4 * pr + t3 * qi + n2 * qr;
+    const double yq_r = -(t3 * pr) - n2 * pi + n1 * qr + n4 * qi;
+    const double yq_i = -(t3 * pi) + n2 * pr + n1 * qi - n4 * qr;
+    up[P + 2 * c] = yp_r;
+    up[P + 2 * c + 1] = yp_i;
+    up[Q + 2 * c] = yq_r;
+    up[Q + 2 * c + 1] = yq_i;
+  }
+}
+
+namespace {
+
+template <typename T> struct DBuf {
+  T *p = nullptr;
+  explicit DBuf(size_t n) { QQCD_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T))); }
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+};
+
+// the draws of the stream from the start state on, kept in a device window
+// [origin, gen_end) that always holds the 521 draws before the consumed count
+struct HbStream {
+  LagState gen;  // the generator after gen_end draws
+  uint32_t start_w[521];
+  int start_jr = 0;
+  long origin = 0, gen_end = 0, cap = 0;
+  uint32_t *buf = nullptr, *alt = nullptr;
+  hipStream_t s;
+
+  HbStream(const LagState &st, long cap_, hipStream_t s_) : gen(st), cap(cap_), s(s_) {
+    for (int k = 0; k < 521; k++) start_w[k] = st.w[k];
+    start_jr = st.jr;
+    QQCD_HIP(hipMalloc(&buf, sizeof(uint32_t) * cap));
+    QQCD_HIP(hipMalloc(&alt, sizeof(uint32_t) * cap));
+  }
+  ~HbStream() {
+    (void)hipFree(buf);
+    (void)hipFree(alt);
+  }
+  // make draws [pos, pos + need) resident
+  void ensure(long pos, long need, long chunk) {
+    if (gen_end >= pos + need) return;
+    const long amount = std::max(need - (gen_end - pos), chunk);
+    if (gen_end - origin + amount > cap) {
+      const long keep0 = std::max(origin, pos - 521);
+      const long len = gen_end - keep0;
+      if (len + amount > cap) throw std::runtime_error("heatbath: stream window too small");
+      if (len > 0)
+        QQCD_HIP(hipMemcpyAsync(alt, buf + (keep0 - origin), sizeof(uint32_t) * len, hipMemcpyDeviceToDevice, s));
+      std::swap(buf, alt);
+      origin = keep0;
+    }
+    lagfib_generate(gen, amount, buf + (gen_end - origin), s);
+    gen_end += amount;
+  }
+  // the rnds state after `pos` draws from the start
+  LagState state_at(long pos) {
+    LagState out;
+    const int jr = (int)((start_jr + pos) % 521);
+    std::vector<uint32_t> tail;
+    const long n = std::min(pos, 521L);
+    if (n > 0) {
+      if (pos - n < origin) throw std::runtime_error("heatbath: draws before the window were dropped");
+      tail.resize(n);
+      QQCD_HIP(hipMemcpyAsync(tail.data(), buf + (pos - n - origin), sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+      QQCD_HIP(hipStreamSynchronize(s));
+    }
+    for (int k = 0; k < 521; k++) {
+      const long t = pos - 521 + k;
+      out.w[(jr + k) % 521] = t >= 0 ? tail[t - (pos - n)] : start_w[(start_jr + 521 + t) % 521];
+    }
+    out.jr = jr;
+    out.kr = (jr + 521 - 32) % 521;
+    return out;
+  }
+};
+
+}  // namespace
+
+// nsweep sweeps of the reference-layout links u_host (in place), drawing from st (advanced)
+void heatbath_reference(Context &c, const int gaction[4], const double *coact, int nhit, int nsweep, double *u_host,
+                        LagState &st) {
+  const Geom &g = c.g;
+  for (int m = 0; m < 4; m++) {
+    if (gaction[m] != 1)
+      throw std::runtime_error("sweep: only gaction = 1 in every direction (plaquette staples) is implemented");
+    if (g.n[m] % 2) throw std::runtime_error("sweep: even extents expected");
+  }
+  if (g.n[0] % 4) throw std::runtime_error("sweep: nx must be divisible by 4");
+  if (nhit < 1) throw std::runtime_error("sweep: nhit >= 1 expected");
+  if (nsweep < 0) throw std::runtime_error("sweep: nsweep >= 0 expected");
+  lagfib_check(st);
+  HbG G{};
+  for (int m = 0; m < 4; m++) G.n[m] = g.n[m];
+  G.nvh = ref_nvhalf(g);
+  G.ntot = ref_ntotal(g);
+  const size_t ulen = 18 * (size_t)G.ntot * 128;
+  const int nblk = (int)((G.nvh + HB_TB - 1) / HB_TB);
+  DBuf<double> du(ulen), C(18 * G.nvh), Z(4 * G.nvh), DK(G.nvh), A0(G.nvh);
+  DBuf<int> ID(G.nvh), cnt(nblk), off(nblk), dacc(HB_PB);
+  DBuf<long> dpos(1), dbase(HB_PB), dtot(HB_PB);
+  QQCD_HIP(hipMemcpyAsync(du.p, u_host, sizeof(double) * ulen, hipMemcpyHostToDevice, c.stream));
+  const long need_pass = 4 * G.nvh * HB_PB, chunk = std::max(8 * need_pass, 1L << 20);
+  HbStream S(st, 521 + chunk + 2 * need_pass + 2 * G.nvh, c.stream);
+  long pos = 0;
+  std::vector<long> htot(HB_PB);
+  std::vector<int> hacc(HB_PB);
+  const dim3 grid(nblk), blk(HB_TB);
+  for (int sw = 0; sw < nsweep; sw++)
+    for (int ieo = 0; ieo < 2; ieo++)
+      for (int ibl = 0; ibl < 16; ibl++)
+        for (int mu = 0; mu < 4; mu++) {
+          double w[4];
+          for (int nu = 0; nu < 4; nu++) w[nu] = coact[4 * mu + 16 * nu];  // coact(1, mu, nu)
+          hipLaunchKernelGGL(k_hb_staple, grid, blk, 0, c.stream, G, du.p, ieo, ibl, mu, w[0], w[1], w[2], w[3],
+                             C.p);
+          QQCD_HIP(hipGetLastError());
+          for (int isu2 = 1; isu2 <= 3; isu2++) {
+            hipLaunchKernelGGL(k_hb_proj, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, isu2, Z.p, DK.p,
+                               ID.p, A0.p);
+            QQCD_HIP(hipGetLastError());
+            for (int batch = 0;; batch++) {
+              // a site whose staple projects to zero never accepts (the reference would loop forever)
+              if (batch == 200) throw std::runtime_error("sweep: Kennedy-Pendleton passes did not end (zero staple?)");
+              S.ensure(pos, need_pass, chunk);
+              QQCD_HIP(hipMemsetAsync(dacc.p, 0, sizeof(int) * HB_PB, c.stream));
+              QQCD_HIP(hipMemsetAsync(dtot.p, 0, sizeof(long) * HB_PB, c.stream));
+              for (int p = 0; p < HB_PB; p++) {
+                hipLaunchKernelGGL(k_hb_count, grid, blk, 0, c.stream, G.nvh, ID.p, cnt.p);
+                hipLaunchKernelGGL(k_hb_scan, dim3(1), dim3(1024), 0, c.stream, nblk, cnt.p, off.p, dpos.p,
+                                   pos - S.origin, p == 0 ? 1 : 0, dbase.p, dtot.p, p);
+                hipLaunchKernelGGL(k_hb_try, grid, blk, 0, c.stream, G.nvh, S.buf, ID.p, DK.p, A0.p, off.p,
+                                   dbase.p, p, dacc.p);
+                QQCD_HIP(hipGetLastError());
+              }
+              long rel = 0;
+              QQCD_HIP(hipMemcpyAsync(htot.data(), dtot.p, sizeof(long) * HB_PB, hipMemcpyDeviceToHost, c.stream));
+              QQCD_HIP(hipMemcpyAsync(hacc.data(), dacc.p, sizeof(int) * HB_PB, hipMemcpyDeviceToHost, c.stream));
+              QQCD_HIP(hipMemcpyAsync(&rel, dpos.p, sizeof(long), hipMemcpyDeviceToHost, c.stream));
+              QQCD_HIP(hipStreamSynchronize(c.stream));
+              pos = S.origin + rel;
+              const long rem = htot[HB_PB - 1] > 0 ? htot[HB_PB - 1] - hacc[HB_PB - 1] : 0;
+              if (rem == 0) break;
+            }
+            S.ensure(pos, 2 * G.nvh, chunk);
+            hipLaunchKernelGGL(k_hb_hyper, grid, blk, 0, c.stream, G, du.p, ieo, ibl, mu, isu2, S.buf,
+                               pos - S.origin, Z.p, DK.p, A0.p);
+            QQCD_HIP(hipGetLastError());
+            pos += 2 * G.nvh;
+          }
+        }
+  QQCD_HIP(hipMemcpyAsync(u_host, du.p, sizeof(double) * ulen, hipMemcpyDeviceToHost, c.stream));
+  QQCD_HIP(hipStreamSynchronize(c.stream));
+  st = S.state_at(pos);
+}
+
+}  // namespace qqcd

@@ -10,6 +10,7 @@
 !   z4source, z2source, loopops, nsaverage
 !                                         qqcd-test/cfgsprops/quark/disconloops.f90
 !   GFwrite, GFread                       qqcd-test/cfgsprops/glue/gaugetools.f90
+!   sweep                                 qqcd-test/cfgsprops/cfgsprops.f90
 !   FPwrite, FPread                       qqcd-test/cfgsprops/quark/quark.f90
 ! and forward to the HIP library through its C ABI (include/qqcd_hip.h).  A
 ! host program replaces `use diracops` / `use inverters` (and the two quark
@@ -38,7 +39,7 @@ module qqcd_hip
    public :: Hsingle, Hdouble, Hdbletm, gammamult, gam5x, vecdot, bicgstab, gmresdr, psc, &
              gmresproject, pointsource, fermprop, qqcd_gauge_changed, qqcd_shutdown, &
              qqcd_solve_mixed_f, z4source, z2source, loopops, loopops_all, nsaverage, &
-             GFwrite, GFread, FPwrite, FPread
+             GFwrite, GFread, FPwrite, FPread, sweep
 
    logical, save :: started = .false.
 
@@ -154,6 +155,13 @@ module qqcd_hip
          integer(c_int), value :: ksub
          type(c_ptr), value :: xe, xo, u, be, bo, bc, io, Jtime
          real(c_double), value :: cur
+      end function
+      integer(c_int) function c_sweep(gaction, coact, nhit, nsweep, u, iw, jr, kr) bind(C, name="qqcd_sweep")
+         import :: c_int, c_ptr
+         integer(c_int), intent(in) :: gaction(4)
+         type(c_ptr), value :: coact, u
+         integer(c_int), value :: nhit, nsweep
+         integer(c_int), intent(inout) :: iw(521), jr, kr
       end function
       integer(c_int) function c_gfwrite(n, u, filename, gaction, beta, alat, tad, icfgsave, myid) &
             bind(C, name="qqcd_gfwrite")
@@ -577,6 +585,26 @@ contains
                                     itermin, c_loc(ngm), vectype, c_loc(u), c_loc(xe), c_loc(xo), isignal, c_loc(it)))
       if (inverter == 0 .and. vectype == 1) vectype = 0
    end subroutine fermprop
+
+   ! ---- gauge fields ----------------------------------------------------
+
+   ! sweep(gaction,coact,nhit,u,myid,nn,ldiv,nms,lv,ib,lbd,iblv,MRT), cfgsprops.f90:1973: one
+   ! pseudo-heatbath sweep of u (in place) on the device, drawing from the host's lagfib stream
+   ! exactly as the reference's sweep does (staples of the plaquette action: gaction = 1).  The
+   ! operators pick up the new links on their next call.
+   subroutine sweep(gaction,coact,nhit,u,myid,nn,ldiv,nms,lv,ib,lbd,iblv,MRT)
+      integer, intent(in) :: gaction(:), nhit, myid, MRT
+      real(kind=8), intent(in), target, contiguous :: coact(:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: u(:,:,:,:,:)
+      integer, intent(in) :: nn(:,:), iblv(:,:), nms(:), lv(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int) :: ga(4)
+      if (.not. started) call ensure_started([1, 1, 1, 1])
+      call check_geometry("CFGSPROPS", myid, ldiv, iblv=iblv, lbd=lbd)
+      ga = gaction(1:4)
+      call fail("CFGSPROPS", c_sweep(ga, c_loc(coact), int(nhit, c_int), 1_c_int, c_loc(u), iwlagfib, jrlagfib, &
+                                     krlagfib))
+   end subroutine sweep
 
    ! ---- the reference's files (host only: no device, usable before the first operator call) ----
 

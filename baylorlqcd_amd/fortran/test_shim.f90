@@ -8,7 +8,9 @@
 ! The first argument selects sections (default: all): "ops" (operators,
 ! solves, loop operators), "noise" (z4source / z2source from the lagfib
 ! stream; needs "ops" first), "files" (GFwrite / GFread / FPwrite / FPread of
-! the inputs, host only: no device is touched).
+! the inputs, host only: no device is touched), "sweep" (two pseudo-heatbath
+! sweeps of the links from the stream of initlagfib(0, 3), coact read from
+! coact_hb.bin).
 program test_shim
    use qqcd_hip
    use gmresrhs, only: isignal
@@ -27,7 +29,7 @@ program test_shim
    logical :: ldiv(4), lbd(16,4)
    character(len=8) :: rwdir(1)
    character(len=32) :: sel
-   logical :: do_ops, do_noise, do_files
+   logical :: do_ops, do_noise, do_files, do_sweep
    integer :: src(4)
 
    allocate(u(18,ntotal,4,2,16), be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8), g(6,ntotal,4,2,8), h(6,ntotal,4,2,8))
@@ -40,6 +42,7 @@ program test_shim
    do_ops = len_trim(sel) == 0 .or. index(sel, "ops") > 0
    do_noise = len_trim(sel) == 0 .or. index(sel, "noise") > 0
    do_files = len_trim(sel) == 0 .or. index(sel, "files") > 0
+   do_sweep = len_trim(sel) == 0 .or. index(sel, "sweep") > 0
    ! single-process geometry tables (setblk / gblchker of the reference)
    vecbl(1,:) = [1, 4, 6, 7, 10, 11, 13, 16]
    vecbl(2,:) = [2, 3, 5, 8, 9, 12, 14, 15]
@@ -68,7 +71,8 @@ program test_shim
    ! the reference's files: the links (re-unitarised in place by GFwrite) and the input spinor
    ! pair as a propagator, read back into fresh arrays
    if (do_files) call shim_files()
-   if (do_ops .or. do_noise) call qqcd_shutdown()
+   if (do_sweep) call shim_sweep()
+   if (do_ops .or. do_noise .or. do_sweep) call qqcd_shutdown()
    print *, "test_shim done"
 
 contains
@@ -143,6 +147,26 @@ contains
    call writebin("z2_e.bin", g, size(g))
    call writebin("z2_o.bin", h, size(h))
    end subroutine noise
+
+   subroutine shim_sweep()
+      real(kind=8) :: cohb(4,4,4)
+      call step("sweep")
+      call readbin("coact_hb.bin", cohb, size(cohb))
+      call initlagfib(0, 3)
+      call sweep([1, 1, 1, 1], cohb, 1, u, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0)
+      call sweep([1, 1, 1, 1], cohb, 1, u, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0)
+      call writebin("u_swept.bin", u, size(u))
+      call write_state("lagfib_after.bin")
+   end subroutine shim_sweep
+
+   subroutine write_state(name)
+      use lagfib, only: iwlagfib, jrlagfib, krlagfib
+      character(len=*), intent(in) :: name
+      integer :: unit
+      open(newunit=unit, file=name, access="stream", form="unformatted", status="replace", action="write")
+      write(unit) iwlagfib, jrlagfib, krlagfib
+      close(unit)
+   end subroutine write_state
 
    subroutine shim_files()
       real(kind=8), allocatable :: uw(:,:,:,:,:), ur(:,:,:,:,:), fe(:,:,:,:,:), fo(:,:,:,:,:)

@@ -2,7 +2,8 @@
 (qqcd-test/cfgsprops/glue/lagfib.f90): ``initlagfib`` (:80) and ``rnds``
 (:219), plus the reference's Z4 / Z2 sources drawn from that stream
 (``z4source`` / ``z2source``, qqcd-test/cfgsprops/quark/disconloops.f90:4359,
-:4583).  A ``LagFib`` holds what the module holds -- iwlagfib(0:520),
+:4583) and its pseudo-heatbath ``sweep`` (qqcd-test/cfgsprops/cfgsprops.f90:
+1973-2003).  A ``LagFib`` holds what the module holds -- iwlagfib(0:520),
 jrlagfib, krlagfib -- and every call advances it exactly as the reference's
 calls would, so draws, sources and states are bit-identical to the
 reference's.  Draws are generated on the GPU.
@@ -57,3 +58,10 @@ class LagFib:
     def z2source(self, be, bo):
         """z2source(be, bo, nvhalf): Z2 noise into the reference-layout checkerboards."""
         _lib.check(_lib.lib().qqcd_z2source_lagfib(_lib.dp(be), _lib.dp(bo), *self._args()))
+
+    def sweep(self, u, coact, nsweep=1, gaction=(1, 1, 1, 1), nhit=1):
+        """sweep(gaction, coact, nhit, u, ...) nsweep times: the reference-layout links u
+        (18, ntotal, 4, 2, 16) updated in place from this stream."""
+        ga, _ga = _lib.ip(gaction)
+        co = np.ascontiguousarray(coact, dtype=np.float64)
+        _lib.check(_lib.lib().qqcd_sweep(ga, _lib.dp(co), int(nhit), int(nsweep), _lib.dp(u), *self._args()))

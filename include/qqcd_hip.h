@@ -300,6 +300,19 @@ int qqcd_rnds(long m, double *rn, int *iw, int *jr, int *kr);
 int qqcd_z4source_lagfib(double *be, double *bo, int *iw, int *jr, int *kr);
 int qqcd_z2source_lagfib(double *be, double *bo, int *iw, int *jr, int *kr);
 
+/* sweep (qqcd-test/cfgsprops/cfgsprops.f90:1973-2003): nsweep pseudo-heatbath
+ * sweeps of u (18, ntotal, 4, 2, 16), in place, as the reference runs them:
+ * link sets (ieo, ibl, mu) in its order, staples (gaugetools.f90:39-345;
+ * plaquette terms, gaction = 1 in every direction -- other actions are
+ * rejected) weighted by coact(1, mu, nu) of coact (4, 4, 4), phbup
+ * (heatbath.f90:33-76: phbsub1-3, su2up, sprd1) drawing from the stream
+ * iw/jr/kr, which is advanced past exactly the draws the reference makes.
+ * Staples and projections are evaluated in the reference's order of
+ * operations; cos/sin/log may differ from the host library's in the last
+ * place.  The resident links of the operators are refreshed on the next
+ * call that passes u. */
+int qqcd_sweep(const int *gaction, const double *coact, int nhit, int nsweep, double *u, int *iw, int *jr, int *kr);
+
 /* ---- the reference's files (host only: n = lattice extents, no device) --- */
 
 /* GFwrite (qqcd-test/cfgsprops/glue/gaugetools.f90:1259-1292): fixsu3 is

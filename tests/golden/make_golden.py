@@ -8,7 +8,10 @@ with the argument ``loops`` only tests/golden/golden_loops_8x8x8x16.npz
 ``io`` only tests/golden/golden_io_8x8x8x16.json (SHA-256 digests of the
 reference's GFwrite / FPwrite files and of what GFread / FPread return); with
 ``lagfib`` only tests/golden/golden_lagfib_8x8x8x16.npz (initlagfib states,
-rnds draws including the zero skip, z4source / z2source digests).
+rnds draws including the zero skip, z4source / z2source digests); with
+``sweep`` only tests/golden/golden_sweep_8x8x8x16.npz (the reference's
+pseudo-heatbath sweeps from the config-1 field: stream states, per-set link
+sums and the links of block 1 after each sweep).
 
 Inputs are regenerated from seeds by the tests (spinors: numpy
 default_rng(seed) Gaussians; links: baylorlqcd_amd.gauge.haar), except the
@@ -282,6 +285,51 @@ def lagfib():
     print("wrote golden_lagfib_8x8x8x16.npz")
 
 
+def sweep_coact_aniso():
+    """coact with coact(1, mu, nu) different from coact(1, nu, mu) (pins the index order)."""
+    co = np.zeros((4, 4, 4), order="F")
+    for mu in range(4):
+        for nu in range(4):
+            if mu != nu:
+                co[0, mu, nu] = 2.0 + 0.1 * mu + 0.03 * nu
+    return np.asfortranarray(co).ravel(order="F")
+
+
+def sweep_summary(u, n):
+    nvh = int(np.prod(n)) // 32
+    ntot = nvh + max(2 * nvh // m for m in n)
+    U = u.reshape(16, 2, 4, ntot, 18)[:, :, :, :nvh]
+    return U.sum(axis=3), U[0].copy()
+
+
+def sweep_cases():
+    """(name, coact builder, lagfib seed, sweeps)."""
+    return [("iso", None, 3, 2), ("aniso", "aniso", 4, 1)]
+
+
+def sweep():
+    n = (8, 8, 8, 16)
+    refdrv.build(n, solvers=True)
+    R = refdrv.RefLattice(n, bc=(1, 1, 1, 1))
+    g = np.load(os.path.join(HERE, "golden_8x8x8x16.npz"))
+    u0 = O.links_canon_to_ref(n, gauge.reconstruct12(g["links12"]))
+    out = {}
+    for name, cob, seed, nsw in sweep_cases():
+        co = R.fixaction(6.0) if cob is None else sweep_coact_aniso()
+        out[f"{name}_coact"] = np.asarray(co)
+        iw, jr, kr = R.lagfib_init(0, seed)
+        u = u0
+        for k in range(1, nsw + 1):
+            u = R.sweeps(u, co, 1)
+            iw, jr, kr = R.lagfib_get()
+            sums, blk = sweep_summary(u, n)
+            out[f"{name}_{k}_iw"], out[f"{name}_{k}_jrkr"] = iw, np.array([jr, kr], np.int32)
+            out[f"{name}_{k}_sums"], out[f"{name}_{k}_block1"] = sums, blk
+            out[f"{name}_{k}_plaq"] = np.array(R.aveplaq(u))
+    np.savez_compressed(os.path.join(HERE, "golden_sweep_8x8x8x16.npz"), **out)
+    print("wrote golden_sweep_8x8x8x16.npz")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["loops"]:
         loops()
@@ -289,5 +337,7 @@ if __name__ == "__main__":
         io()
     elif sys.argv[1:] == ["lagfib"]:
         lagfib()
+    elif sys.argv[1:] == ["sweep"]:
+        sweep()
     else:
         main()

@@ -1,0 +1,129 @@
+"""The reference's pseudo-heatbath sweep on the GPU (qqcd_sweep,
+qqcd_heatbath_ref.hip) against the reference's own sweeps
+(golden_sweep_8x8x8x16.npz) and, on lattices the fixture does not cover,
+against the oracle's restatement (oracle/heatbath.py, pinned on the same
+fixture).  Bar: the stream state after every sweep identical (every
+accept decision and draw the same), links within 1e-12 (cos / sin / log of
+the device library against the host's)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from baylorlqcd_amd import diracops, gauge, lattice
+from baylorlqcd_amd.lagfib import LagFib
+from oracle import heatbath as HB
+from oracle import lagfib as LF
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+N = (8, 8, 8, 16)
+
+
+def dims(n):
+    nvh = int(np.prod(n)) // 32
+    return nvh, nvh + max(2 * nvh // m for m in n)
+
+
+def summary(u, n):
+    nvh, ntot = dims(n)
+    U = u.reshape(16, 2, 4, ntot, 18)[:, :, :, :nvh]
+    return U.sum(axis=3), U[0]
+
+
+def same_state(L, iw, jrkr):
+    s = L.state
+    return np.array_equal(s[0], iw) and (s[1], s[2]) == tuple(int(v) for v in jrkr)
+
+
+@pytest.fixture(scope="module")
+def gold():
+    lattice.init(N, (1, 1, 1, 1))
+    g = np.load(os.path.join(ROOT, "tests", "golden", "golden_sweep_8x8x8x16.npz"))
+    c = np.load(os.path.join(ROOT, "tests", "golden", "golden_8x8x8x16.npz"))
+    yield g, O.links_canon_to_ref(N, gauge.reconstruct12(c["links12"]))
+    lattice.finalize()
+
+
+def test_sweeps_reference_iso(gold):
+    print("[sweep] two sweeps, one call each, against the reference", flush=True)
+    g, u0 = gold
+    u = u0.copy()
+    L = LagFib(0, 3)
+    for k in (1, 2):
+        L.sweep(u, g["iso_coact"], 1)
+        assert same_state(L, g[f"iso_{k}_iw"], g[f"iso_{k}_jrkr"])
+        sums, blk = summary(u, N)
+        assert np.max(np.abs(sums - g[f"iso_{k}_sums"])) < 1e-11
+        assert np.max(np.abs(blk - g[f"iso_{k}_block1"])) < 1e-12
+
+
+def test_sweeps_reference_one_call(gold):
+    print("[sweep] two sweeps in one call", flush=True)
+    g, u0 = gold
+    u = u0.copy()
+    L = LagFib(0, 3)
+    L.sweep(u, g["iso_coact"], 2)
+    assert same_state(L, g["iso_2_iw"], g["iso_2_jrkr"])
+    assert np.max(np.abs(summary(u, N)[1] - g["iso_2_block1"])) < 1e-12
+
+
+def test_sweep_reference_aniso(gold):
+    print("[sweep] anisotropic coact", flush=True)
+    g, u0 = gold
+    u = u0.copy()
+    L = LagFib(0, 4)
+    L.sweep(u, g["aniso_coact"], 1)
+    assert same_state(L, g["aniso_1_iw"], g["aniso_1_jrkr"])
+    sums, blk = summary(u, N)
+    assert np.max(np.abs(sums - g["aniso_1_sums"])) < 1e-11
+    assert np.max(np.abs(blk - g["aniso_1_block1"])) < 1e-12
+
+
+def test_operators_see_swept_links(gold):
+    # the operators reload links the sweep changed in place
+    print("[sweep] operators after a sweep", flush=True)
+    g, u0 = gold
+    u = u0.copy()
+    co = O.coact_array()
+    diracops.gauge_load(u, co)
+    LagFib(0, 5).sweep(u, g["iso_coact"], 1)
+    rng = np.random.default_rng(7)
+    V = int(np.prod(N))
+    psi = rng.standard_normal((V, 4, 3)) + 1j * rng.standard_normal((V, 4, 3))
+    be, bo = O.spinor_canon_to_ref(N, psi)
+    out = np.zeros_like(be)
+    diracops.Hsingle(out, u, bo, 0, co, (1, 1, 1, 1), 1)
+    want = O.hsingle_ref(N, u, co, (1, 1, 1, 1), 1, 0, bo)
+    assert np.max(np.abs(out - want)) < 1e-12 * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("n", [(16, 16, 16, 32), (12, 8, 8, 16)])
+def test_sweep_against_oracle(n, gold):
+    # many workgroups per link set (the rank scan across workgroups), other shapes
+    print(f"[sweep] {n} against the oracle", flush=True)
+    g, _ = gold
+    lattice.init(n, (1, 1, 1, 1))
+    try:
+        u0 = O.links_canon_to_ref(n, gauge.haar(n, 91))
+        co = g["iso_coact"]
+        st = LF.init(0, 6)
+        uo, so = HB.sweep(n, u0, co, st, 1)
+        u = u0.copy()
+        L = LagFib.from_state(*st)
+        L.sweep(u, co, 1)
+        assert same_state(L, so[0], so[1:])
+        nvh, ntot = dims(n)
+        a = u.reshape(16, 2, 4, ntot, 18)[:, :, :, :nvh]
+        b = uo.reshape(16, 2, 4, ntot, 18)[:, :, :, :nvh]
+        assert np.max(np.abs(a - b)) < 1e-12
+    finally:
+        lattice.init(N, (1, 1, 1, 1))
+
+
+def test_sweep_rejects_improved_action(gold):
+    g, u0 = gold
+    with pytest.raises(Exception):
+        LagFib(0, 3).sweep(u0.copy(), g["iso_coact"], 1, gaction=(1, 1, 1, 2))
