@@ -75,6 +75,9 @@ SIGNATURES = {
     "qqcd_z4source_lagfib": [_dp, _dp, _ip, _ip, _ip],
     "qqcd_z2source_lagfib": [_dp, _dp, _ip, _ip, _ip],
     "qqcd_sweep": [_ip, _dp, ctypes.c_int, ctypes.c_int, _dp, _ip, _ip, _ip],
+    "qqcd_smear": [_dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip],
+    "qqcd_twmshift": [_dp, _dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip],
+    "qqcd_gamma5mult": [_dp, _dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip],
     "qqcd_gfwrite": [_ip, _dp, ctypes.c_char_p, _ip, ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int],
     "qqcd_gfread": [_ip, _dp, ctypes.c_char_p, _ip, _dp, _dp, _dp, _ip, _ip],
     "qqcd_fpwrite": [_ip, _dp, _dp, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,

@@ -170,6 +170,96 @@ extern "C" int qqcd_hdbletm(double *he, const double *u, const double *ge, int i
   API_END
 }
 
+namespace qqcd {
+void smear(Context &c, Spinor h[2], Spinor t[2], double asmear, int nsmear);
+}
+
+extern "C" int qqcd_smear(double *he, double *ho, const double *u, const double *ge, const double *go, double asmear,
+                          int nsmear, const int *bc) {
+  API_BEGIN
+  Context &c = ctx();
+  if (!he || !ho || !ge || !go) throw std::runtime_error("smear: null field");
+  set_bc(c, bc);
+  ensure_gauge(c, u, nullptr);
+  SpinorBuf h0(c.g, PREC_DOUBLE), h1(c.g, PREC_DOUBLE), t0(c.g, PREC_DOUBLE), t1(c.g, PREC_DOUBLE);
+  upload_ref(c, ge, c.dref[0]);
+  ref_to_spinor(c.g, c.dref[0], h0.s, 0, c.stream);
+  upload_ref(c, go, c.dref[1]);
+  ref_to_spinor(c.g, c.dref[1], h1.s, 1, c.stream);
+  Spinor h[2] = {h0.s, h1.s}, t[2] = {t0.s, t1.s};
+  smear(c, h, t, asmear, nsmear);
+  spinor_to_ref(c.g, h[0], c.dref[0], 0, c.stream);
+  download_ref(c, c.dref[0], he);
+  spinor_to_ref(c.g, h[1], c.dref[1], 1, c.stream);
+  download_ref(c, c.dref[1], ho);
+  API_END
+}
+
+// twMshift / gamma5mult share the Wilson-twisted checks of the reference (iflag = -1 only)
+static void shift_checks(int iflag, int idag, const double kappa[2]) {
+  if (iflag != -1) throw std::runtime_error("multiple shifts need the Wilson operator (iflag = -1), no clover term");
+  if (idag != 0 && idag != 1) throw std::runtime_error("idag must be 0 or 1");
+  if (!kappa || kappa[0] == 0.0 || kappa[1] == 0.0)
+    throw std::runtime_error("twMshift / gamma5mult divide by 2 kappa mu: kappa(1) and kappa(2) must be nonzero");
+}
+
+extern "C" int qqcd_twmshift(double *hwhole, const double *u, const double *gwhole, int idag, const double *coact,
+                             const double kappa[2], int iflag, const int *bc) {
+  API_BEGIN
+  Context &c = ctx();
+  shift_checks(iflag, idag, kappa);
+  set_bc(c, bc);
+  ensure_gauge(c, u, coact);
+  const size_t half = 6 * (size_t)ref_ntotal(c.g) * 64;
+  SpinorBuf g0(c.g, PREC_DOUBLE), g1(c.g, PREC_DOUBLE), out(c.g, PREC_DOUBLE);
+  upload_ref(c, gwhole, c.dref[0]);
+  ref_to_spinor(c.g, c.dref[0], g0.s, 0, c.stream);
+  upload_ref(c, gwhole + half, c.dref[1]);
+  ref_to_spinor(c.g, c.dref[1], g1.s, 1, c.stream);
+  const Spinor *g[2] = {&g0.s, &g1.s};
+  // h_p = g_p + fac2 (-i g5) g_p + fac1 (-i g5) H g_other (diracops.f90:1925-1987)
+  Epilogue e;
+  e.a0 = 1.0;
+  e.a1 = -1.0 / (2.0 * kappa[0] * kappa[1]);
+  e.b0 = 0.0;
+  e.b1 = -1.0 / (2.0 * kappa[1]);
+  for (int p = 0; p < 2; p++) {
+    dslash(c.g, c.U64, *g[1 - p], out.s, p, idag, e, g[p], c.stream);
+    spinor_to_ref(c.g, out.s, c.dref[2], p, c.stream);
+    download_ref(c, c.dref[2], hwhole + p * half);
+  }
+  API_END
+}
+
+extern "C" int qqcd_gamma5mult(double *he, const double *u, const double *ge, int idag, const double *coact,
+                               const double kappa[2], int iflag, const int *bc) {
+  API_BEGIN
+  Context &c = ctx();
+  shift_checks(iflag, idag, kappa);
+  set_bc(c, bc);
+  ensure_gauge(c, u, coact);
+  SpinorBuf g0(c.g, PREC_DOUBLE), t(c.g, PREC_DOUBLE), out(c.g, PREC_DOUBLE);
+  upload_ref(c, ge, c.dref[0]);
+  ref_to_spinor(c.g, c.dref[0], g0.s, 0, c.stream);
+  const double k = kappa[0], mu = kappa[1], a = 2.0 * k * mu;
+  // htmp = (1 + 2 kappa mu (-i g5)) H_oe g (diracops.f90:1800-1812)
+  Epilogue e1;
+  e1.b0 = 1.0;
+  e1.b1 = -a;
+  dslash(c.g, c.U64, g0.s, t.s, 1, idag, e1, nullptr, c.stream);
+  // he = fac1 g + fac4 fac1 (-i g5) g - fac4 fac3 (-i g5) H_eo htmp (:1815-1836)
+  const double fac1 = 1.0 / (k * k), fac3 = 1.0 / (1.0 + a * a), fac4 = 1.0 / a;
+  Epilogue e2;
+  e2.a0 = fac1;
+  e2.a1 = -fac4 * fac1;
+  e2.b0 = 0.0;
+  e2.b1 = fac4 * fac3;
+  dslash(c.g, c.U64, t.s, out.s, 0, idag, e2, &g0.s, c.stream);
+  spinor_to_ref(c.g, out.s, c.dref[1], 0, c.stream);
+  download_ref(c, c.dref[1], he);
+  API_END
+}
+
 extern "C" int qqcd_gammamult(const double *even_in, const double *odd_in, double *even_out, double *odd_out,
                               int mu, int ieo, int ibleo) {
   API_BEGIN

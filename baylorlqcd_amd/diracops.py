@@ -1,7 +1,8 @@
 """Python mirror of the reference's ``diracops`` module on the HIP library.
 
 Same procedure names, argument order and meaning as
-qqcd-test/cfgsprops/quark/diracops.f90 (Hsingle :554, Hdouble :264,
+qqcd-test/cfgsprops/quark/diracops.f90 (smear :712, gamma5mult :1750, twMshift :1863,
+Hsingle :554, Hdouble :264,
 Hdbletm :435, gammamult :105) and basics.f90 (gam5x :408); arrays are numpy
 float64 arrays in the reference order (Fortran (6, ntotal, 4, 2, 8) spinors,
 (18, ntotal, 4, 2, 16) links), results are written into the output argument
@@ -48,6 +49,39 @@ def Hdbletm(he, u, GeeGooinv, ge, idag, coact, kappa, iflag, bc, vecbl=None, vec
     _lib.check(L.qqcd_hdbletm(_lib.dp(_f64(he, "he")), _lib.dp(_f64(u, "u")), _lib.dp(_f64(ge, "ge")), int(idag),
                               _lib.dp(np.ascontiguousarray(coact, dtype=np.float64).ravel(order="F")), _lib.dp(k),
                               int(iflag), _lib.ints(bc).ctypes.data_as(_lib._ip)))
+
+
+def smear(he, ho, u, ge, go, asmear, nsmear, bc, vecbl=None, vecblinv=None, myid=None, nn=None, ldiv=None,
+          nms=None, lvbc=None, ib=None, lbd=None, iblv=None, MRT=None):
+    """Gaussian smearing (diracops.f90:712): he, ho = (1 + a sum_{mu<=3} H_mu)^nsmear (ge, go)."""
+    check_geometry(vecbl, vecblinv, myid, ldiv, lbd, iblv)
+    L = _lib.lib()
+    _lib.check(L.qqcd_smear(_lib.dp(_f64(he, "he")), _lib.dp(_f64(ho, "ho")), _lib.dp(_f64(u, "u")),
+                            _lib.dp(_f64(ge, "ge")), _lib.dp(_f64(go, "go")), float(asmear), int(nsmear),
+                            _lib.ints(bc).ctypes.data_as(_lib._ip)))
+
+
+def twMshift(hwhole, u, GeeGooinv, gwhole, idag, coact, kappa, iflag, bc, vecbl=None, vecblinv=None, myid=None,
+             nn=None, ldiv=None, nms=None, lvbc=None, ib=None, lbd=None, iblv=None, MRT=None):
+    """twMshift (diracops.f90:1863): the full-lattice shifted twisted operator; hwhole, gwhole hold
+    (6, ntotal, 4, 2, 8, 2) (both checkerboards)."""
+    check_geometry(vecbl, vecblinv, myid, ldiv, lbd, iblv)
+    L = _lib.lib()
+    _lib.check(L.qqcd_twmshift(_lib.dp(_f64(hwhole, "hwhole")), _lib.dp(_f64(u, "u")), _lib.dp(_f64(gwhole, "gwhole")),
+                               int(idag), _lib.dp(np.ascontiguousarray(coact, dtype=np.float64).ravel(order="F")),
+                               _lib.dp(np.ascontiguousarray(kappa, dtype=np.float64)), int(iflag),
+                               _lib.ints(bc).ctypes.data_as(_lib._ip)))
+
+
+def gamma5mult(he, u, GeeGooinv, ge, idag, coact, kappa, iflag, bc, vecbl=None, vecblinv=None, myid=None, nn=None,
+               ldiv=None, nms=None, lvbc=None, ib=None, lbd=None, iblv=None, MRT=None):
+    """gamma5mult (diracops.f90:1750): (-i g5 / (2 kappa mu)) times the twisted Schur operator."""
+    check_geometry(vecbl, vecblinv, myid, ldiv, lbd, iblv)
+    L = _lib.lib()
+    _lib.check(L.qqcd_gamma5mult(_lib.dp(_f64(he, "he")), _lib.dp(_f64(u, "u")), _lib.dp(_f64(ge, "ge")), int(idag),
+                                 _lib.dp(np.ascontiguousarray(coact, dtype=np.float64).ravel(order="F")),
+                                 _lib.dp(np.ascontiguousarray(kappa, dtype=np.float64)), int(iflag),
+                                 _lib.ints(bc).ctypes.data_as(_lib._ip)))
 
 
 def gammamult(evenIN, oddIN, evenOUT, oddOUT, mu, ieo, ibleo):

@@ -2,7 +2,8 @@
 !
 ! The procedures below carry the names and dummy-argument lists of the
 ! reference's module procedures (BaylorLQCD "qqcd"):
-!   Hsingle, Hdouble, Hdbletm, gammamult  qqcd-test/cfgsprops/quark/diracops.f90
+!   Hsingle, Hdouble, Hdbletm, gammamult, smear, twMshift, gamma5mult
+!                                         qqcd-test/cfgsprops/quark/diracops.f90
 !   gam5x                                 qqcd-test/cfgsprops/basics.f90
 !   bicgstab, gmresdr, psc, gmresproject, vecdot
 !                                         qqcd-test/cfgsprops/quark/inverters.f90
@@ -39,7 +40,7 @@ module qqcd_hip
    public :: Hsingle, Hdouble, Hdbletm, gammamult, gam5x, vecdot, bicgstab, gmresdr, psc, &
              gmresproject, pointsource, fermprop, qqcd_gauge_changed, qqcd_shutdown, &
              qqcd_solve_mixed_f, z4source, z2source, loopops, loopops_all, nsaverage, &
-             GFwrite, GFread, FPwrite, FPread, sweep
+             GFwrite, GFread, FPwrite, FPread, sweep, smear, twMshift, gamma5mult
 
    logical, save :: started = .false.
 
@@ -155,6 +156,22 @@ module qqcd_hip
          integer(c_int), value :: ksub
          type(c_ptr), value :: xe, xo, u, be, bo, bc, io, Jtime
          real(c_double), value :: cur
+      end function
+      integer(c_int) function c_smear(he, ho, u, ge, go, asmear, nsmear, bc) bind(C, name="qqcd_smear")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: he, ho, u, ge, go, bc
+         real(c_double), value :: asmear
+         integer(c_int), value :: nsmear
+      end function
+      integer(c_int) function c_twmshift(hwhole, u, gwhole, idag, coact, kappa, iflag, bc) bind(C, name="qqcd_twmshift")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: hwhole, u, gwhole, coact, kappa, bc
+         integer(c_int), value :: idag, iflag
+      end function
+      integer(c_int) function c_gamma5mult(he, u, ge, idag, coact, kappa, iflag, bc) bind(C, name="qqcd_gamma5mult")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: he, u, ge, coact, kappa, bc
+         integer(c_int), value :: idag, iflag
       end function
       integer(c_int) function c_sweep(gaction, coact, nhit, nsweep, u, iw, jr, kr) bind(C, name="qqcd_sweep")
          import :: c_int, c_ptr
@@ -339,6 +356,59 @@ contains
       if (.not. started) call fail("DIRACOPS", -1_c_int)
       call fail("DIRACOPS", c_gammamult(c_loc(evenIN), c_loc(oddIN), c_loc(evenOUT), c_loc(oddOUT), mu, ieo, ibleo))
    end subroutine gammamult
+
+   ! smear(he,ho,u,ge,go,asmear,nsmear,bc,...), diracops.f90:712: Gaussian smearing of (ge, go)
+   subroutine smear(he,ho,u,ge,go,asmear,nsmear,bc,vecbl,vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd,iblv,MRT)
+      real(kind=8), intent(out), target, contiguous :: he(:,:,:,:,:), ho(:,:,:,:,:)
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:), ge(:,:,:,:,:), go(:,:,:,:,:)
+      real(kind=8), intent(in) :: asmear
+      integer, intent(in) :: nsmear, myid, MRT
+      integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int), target :: b(4)
+      call check_geometry("DIRACOPS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      call ensure_started(bc)
+      b = bc(1:4)
+      call fail("DIRACOPS", c_smear(c_loc(he), c_loc(ho), c_loc(u), c_loc(ge), c_loc(go), asmear, &
+                                    int(nsmear, c_int), c_loc(b)))
+   end subroutine smear
+
+   ! twMshift(hwhole,u,GeeGooinv,gwhole,idag,coact,kappa,iflag,bc,...), diracops.f90:1863:
+   ! hwhole, gwhole(6,ntotal,4,2,8,2); iflag = -1 only, as the reference
+   subroutine twMshift(hwhole,u,GeeGooinv,gwhole,idag,coact,kappa,iflag,bc,vecbl,vecblinv,myid,nn,ldiv,nms, &
+                       lvbc,ib,lbd,iblv,MRT)
+      real(kind=8), intent(out), target, contiguous :: hwhole(:,:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: gwhole(:,:,:,:,:,:)
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:), coact(:,:,:), kappa(:)
+      real(kind=8), intent(in) :: GeeGooinv(:,:,:,:,:)
+      integer, intent(in) :: idag, iflag, myid, MRT
+      integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int), target :: b(4)
+      call check_geometry("DIRACOPS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      if (size(hwhole, 6) /= 2 .or. size(gwhole, 6) /= 2) call fail("DIRACOPS", -1_c_int)
+      call ensure_started(bc)
+      b = bc(1:4)
+      call fail("DIRACOPS", c_twmshift(c_loc(hwhole), c_loc(u), c_loc(gwhole), int(idag, c_int), c_loc(coact), &
+                                       c_loc(kappa), int(iflag, c_int), c_loc(b)))
+   end subroutine twMshift
+
+   ! gamma5mult(he,u,GeeGooinv,ge,idag,coact,kappa,iflag,bc,...), diracops.f90:1750
+   subroutine gamma5mult(he,u,GeeGooinv,ge,idag,coact,kappa,iflag,bc,vecbl,vecblinv,myid,nn,ldiv,nms,lvbc,ib, &
+                         lbd,iblv,MRT)
+      real(kind=8), intent(out), target, contiguous :: he(:,:,:,:,:)
+      real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:), ge(:,:,:,:,:), coact(:,:,:), kappa(:)
+      real(kind=8), intent(in) :: GeeGooinv(:,:,:,:,:)
+      integer, intent(in) :: idag, iflag, myid, MRT
+      integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      integer(c_int), target :: b(4)
+      call check_geometry("DIRACOPS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
+      call ensure_started(bc)
+      b = bc(1:4)
+      call fail("DIRACOPS", c_gamma5mult(c_loc(he), c_loc(u), c_loc(ge), int(idag, c_int), c_loc(coact), &
+                                         c_loc(kappa), int(iflag, c_int), c_loc(b)))
+   end subroutine gamma5mult
 
    subroutine gam5x(a)
       real(kind=8), intent(inout), target, contiguous :: a(:,:,:,:,:)

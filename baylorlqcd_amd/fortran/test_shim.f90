@@ -10,7 +10,8 @@
 ! stream; needs "ops" first), "files" (GFwrite / GFread / FPwrite / FPread of
 ! the inputs, host only: no device is touched), "sweep" (two pseudo-heatbath
 ! sweeps of the links from the stream of initlagfib(0, 3), coact read from
-! coact_hb.bin).
+! coact_hb.bin), "smear" (smear of be/bo with asmear 0.25, nsmear 3; twMshift
+! and gamma5mult at kappa = (0.15, 0.02)).
 program test_shim
    use qqcd_hip
    use gmresrhs, only: isignal
@@ -29,7 +30,7 @@ program test_shim
    logical :: ldiv(4), lbd(16,4)
    character(len=8) :: rwdir(1)
    character(len=32) :: sel
-   logical :: do_ops, do_noise, do_files, do_sweep
+   logical :: do_ops, do_noise, do_files, do_sweep, do_smear
    integer :: src(4)
 
    allocate(u(18,ntotal,4,2,16), be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8), g(6,ntotal,4,2,8), h(6,ntotal,4,2,8))
@@ -43,6 +44,7 @@ program test_shim
    do_noise = len_trim(sel) == 0 .or. index(sel, "noise") > 0
    do_files = len_trim(sel) == 0 .or. index(sel, "files") > 0
    do_sweep = len_trim(sel) == 0 .or. index(sel, "sweep") > 0
+   do_smear = len_trim(sel) == 0 .or. index(sel, "smear") > 0
    ! single-process geometry tables (setblk / gblchker of the reference)
    vecbl(1,:) = [1, 4, 6, 7, 10, 11, 13, 16]
    vecbl(2,:) = [2, 3, 5, 8, 9, 12, 14, 15]
@@ -72,7 +74,8 @@ program test_shim
    ! pair as a propagator, read back into fresh arrays
    if (do_files) call shim_files()
    if (do_sweep) call shim_sweep()
-   if (do_ops .or. do_noise .or. do_sweep) call qqcd_shutdown()
+   if (do_smear) call shim_smear()
+   if (do_ops .or. do_noise .or. do_sweep .or. do_smear) call qqcd_shutdown()
    print *, "test_shim done"
 
 contains
@@ -158,6 +161,24 @@ contains
       call writebin("u_swept.bin", u, size(u))
       call write_state("lagfib_after.bin")
    end subroutine shim_sweep
+
+   subroutine shim_smear()
+      real(kind=8), allocatable :: gw(:,:,:,:,:,:), hw(:,:,:,:,:,:)
+      call step("smear")
+      call smear(g, h, u, be, bo, 0.25d0, 3, bc, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0)
+      call writebin("smear_e.bin", g, size(g))
+      call writebin("smear_o.bin", h, size(h))
+      allocate(gw(6,ntotal,4,2,8,2), hw(6,ntotal,4,2,8,2))
+      gw(:,:,:,:,:,1) = be
+      gw(:,:,:,:,:,2) = bo
+      kappa = [0.15d0, 0.02d0]
+      call twMshift(hw, u, gdummy, gw, 0, coact, kappa, -1, bc, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, &
+                    iblv, 0)
+      call writebin("twm.bin", hw, size(hw))
+      call gamma5mult(g, u, gdummy, be, 0, coact, kappa, -1, bc, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, &
+                      iblv, 0)
+      call writebin("g5m.bin", g, size(g))
+   end subroutine shim_smear
 
    subroutine write_state(name)
       use lagfib, only: iwlagfib, jrlagfib, krlagfib

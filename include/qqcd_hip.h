@@ -94,6 +94,24 @@ int qqcd_hdouble(double *he, const double *u, const double *ge, int idag, const 
 int qqcd_hdbletm(double *he, const double *u, const double *ge, int idag, const double *coact,
                  const double kappa[2], int iflag, const int *bc);
 
+/* smear (qqcd-test/cfgsprops/quark/diracops.f90:712-856): Gaussian smearing,
+ * nsmear times h <- (1 - 6 asmear) h + asmear sum_{mu=1..3} (U_mu(x) h(x+mu)
+ * + U_mu(x-mu)^dag h(x-mu)) on both checkerboards (ge, go in; he, ho out),
+ * bc(mu) on hops across the spatial edge. */
+int qqcd_smear(double *he, double *ho, const double *u, const double *ge, const double *go, double asmear,
+               int nsmear, const int *bc);
+
+/* twMshift (diracops.f90:1863-1995), iflag = -1: on both checkerboards
+ * (hwhole, gwhole = (6, ntotal, 4, 2, 8, 2)) h = g + (-i g5)/(2 kappa mu)
+ * (g + kappa H g_other), kappa = (kappa, mu). */
+int qqcd_twmshift(double *hwhole, const double *u, const double *gwhole, int idag, const double *coact,
+                  const double kappa[2], int iflag, const int *bc);
+
+/* gamma5mult (diracops.f90:1750-1860), iflag = -1: he = (-i g5)/(2 kappa mu)
+ * times the Wilson-twisted Schur operator of Hdbletm on ge. */
+int qqcd_gamma5mult(double *he, const double *u, const double *ge, int idag, const double *coact,
+                    const double kappa[2], int iflag, const int *bc);
+
 /* gammamult(evenIN,oddIN,evenOUT,oddOUT,mu,ieo,ibleo), diracops.f90:105-195:
  * gamma_mu (mu = 1..5) on rows 1..nvhalf of slice (ieo, ibleo) of both
  * arrays; the rest of the outputs is left unchanged. */

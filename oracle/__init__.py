@@ -395,3 +395,26 @@ def loops_ref_to_canon(n, Je, Jo):
         sel = par if odd else ~par
         out[sel] = psi[sel].reshape(-1, 12)[:, :9]
     return out
+
+
+def smear(n, U, bc, psi, asmear, nsmear):
+    """Gaussian smearing (smear, qqcd-test/cfgsprops/quark/diracops.f90:712-856):
+    nsmear times h <- (1 - 6a) h + a sum_{mu=1..3} [U_mu(x) h(x+mu) + U_mu(x-mu)^dag h(x-mu)],
+    both terms times bc(mu) across the edge, spin-diagonal; psi (V, 4, 3)."""
+    h = np.array(psi, dtype=np.complex128, copy=True)
+    fw = [forward(n, bc, mu) for mu in range(3)]
+    bw = []
+    for mu in range(3):
+        idx, f = fw[mu]
+        inv = np.empty_like(idx)
+        inv[idx] = np.arange(idx.size)
+        bw.append((inv, f[inv]))
+    for _ in range(int(nsmear)):
+        acc = np.zeros_like(h)
+        for mu in range(3):
+            i, f = fw[mu]
+            acc = acc + f[:, None, None] * np.einsum("vab,vdb->vda", U[:, mu], h[i])
+            j, g = bw[mu]
+            acc = acc + g[:, None, None] * np.einsum("vba,vdb->vda", np.conj(U[j, mu]), h[j])
+        h = (1.0 - 6.0 * asmear) * h + asmear * acc
+    return h

@@ -120,6 +120,44 @@ contains
       call gammamult(ein, oin, eout, oout, mu, ieo, ibleo)
    end subroutine ref_gammamult
 
+   ! smear (diracops.f90:712-856)
+   subroutine ref_smear(he, ho, u, ge, go, asmear, nsmear) bind(C, name="ref_smear")
+      real(c_double), intent(inout) :: he(6,ntotal,4,2,8), ho(6,ntotal,4,2,8)
+      real(c_double), intent(in)    :: u(18,ntotal,4,2,16), ge(6,ntotal,4,2,8), go(6,ntotal,4,2,8)
+      real(c_double), value         :: asmear
+      integer(c_int), value         :: nsmear
+      integer(kind=KI) :: MRT
+      MRT = MPI_DOUBLE_PRECISION
+      call smear(he, ho, u, ge, go, asmear, nsmear, gbc, vecbl, vecblinv, myid, nn, ldiv, nms, lvbc, ib, lbd, &
+                 iblv, MRT)
+   end subroutine ref_smear
+
+   ! twMshift (diracops.f90:1863-1995), iflag = -1; hwhole, gwhole (6,ntotal,4,2,8,2)
+   subroutine ref_twmshift(hwhole, u, gwhole, idag, coact, kappa) bind(C, name="ref_twmshift")
+      real(c_double), intent(inout) :: hwhole(6,ntotal,4,2,8,2), gwhole(6,ntotal,4,2,8,2)
+      real(c_double), intent(in)    :: u(18,ntotal,4,2,16), coact(4,4,4), kappa(2)
+      integer(c_int), value         :: idag
+      real(kind=KR) :: gdummy(18,1,1,1,1)
+      integer(kind=KI) :: MRT
+      MRT = MPI_DOUBLE_PRECISION
+      gdummy = 0.0_KR
+      call twMshift(hwhole, u, gdummy, gwhole, idag, coact, kappa, -1, gbc, vecbl, vecblinv, myid, nn, ldiv, &
+                    nms, lvbc, ib, lbd, iblv, MRT)
+   end subroutine ref_twmshift
+
+   ! gamma5mult (diracops.f90:1750-1860), iflag = -1
+   subroutine ref_gamma5mult(he, u, ge, idag, coact, kappa) bind(C, name="ref_gamma5mult")
+      real(c_double), intent(inout) :: he(6,ntotal,4,2,8), ge(6,ntotal,4,2,8)
+      real(c_double), intent(in)    :: u(18,ntotal,4,2,16), coact(4,4,4), kappa(2)
+      integer(c_int), value         :: idag
+      real(kind=KR) :: gdummy(18,1,1,1,1)
+      integer(kind=KI) :: MRT
+      MRT = MPI_DOUBLE_PRECISION
+      gdummy = 0.0_KR
+      call gamma5mult(he, u, gdummy, ge, idag, coact, kappa, -1, gbc, vecbl, vecblinv, myid, nn, ldiv, &
+                      nms, lvbc, ib, lbd, iblv, MRT)
+   end subroutine ref_gamma5mult
+
    subroutine ref_gam5x(a) bind(C, name="ref_gam5x")
       real(c_double), intent(inout) :: a(6,ntotal,4,2,8)
       call gam5x(a)

@@ -141,6 +141,26 @@ class RefLattice:
                            _p(np.ascontiguousarray(coact)), _p(k), int(iflag))
         return he
 
+    def smear(self, u, ge, go, asmear, nsmear):
+        he, ho = np.zeros(self.slen), np.zeros(self.slen)
+        _run(self.L.ref_smear, _p(he), _p(ho), _p(np.ascontiguousarray(u)), _p(np.ascontiguousarray(ge)),
+             _p(np.ascontiguousarray(go)), ctypes.c_double(asmear), int(nsmear))
+        return he, ho
+
+    def twmshift(self, u, ge, go, coact, kappa, idag=0):
+        g = np.concatenate([np.asarray(ge, dtype=np.float64), np.asarray(go, dtype=np.float64)])
+        h = np.zeros(2 * self.slen)
+        _run(self.L.ref_twmshift, _p(h), _p(np.ascontiguousarray(u)), _p(g), int(idag),
+             _p(np.ascontiguousarray(coact)), _p(np.ascontiguousarray(kappa, dtype=np.float64)))
+        return h[:self.slen], h[self.slen:]
+
+    def gamma5mult(self, u, ge, coact, kappa, idag=0):
+        he = np.zeros(self.slen)
+        g = np.array(ge, dtype=np.float64, copy=True)
+        _run(self.L.ref_gamma5mult, _p(he), _p(np.ascontiguousarray(u)), _p(g), int(idag),
+             _p(np.ascontiguousarray(coact)), _p(np.ascontiguousarray(kappa, dtype=np.float64)))
+        return he
+
     def gammamult(self, ein, oin, mu, ieo, ibleo, eout=None, oout=None):
         eout = np.zeros(self.slen) if eout is None else np.array(eout, copy=True)
         oout = np.zeros(self.slen) if oout is None else np.array(oout, copy=True)

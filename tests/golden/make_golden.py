@@ -11,7 +11,10 @@ reference's GFwrite / FPwrite files and of what GFread / FPread return); with
 rnds draws including the zero skip, z4source / z2source digests); with
 ``sweep`` only tests/golden/golden_sweep_8x8x8x16.npz (the reference's
 pseudo-heatbath sweeps from the config-1 field: stream states, per-set link
-sums and the links of block 1 after each sweep).
+sums and the links of block 1 after each sweep); with ``smear`` only
+tests/golden/golden_smear_8x8x8x16.npz (the reference's smear, twMshift and
+gamma5mult: timeslices 0 and nt-1 of each output, per-timeslice norms and
+sums).
 
 Inputs are regenerated from seeds by the tests (spinors: numpy
 default_rng(seed) Gaussians; links: baylorlqcd_amd.gauge.haar), except the
@@ -330,6 +333,50 @@ def sweep():
     print("wrote golden_sweep_8x8x8x16.npz")
 
 
+SMEAR_CASES = [("a", (1, 1, 1, -1), 0.25, 3), ("b", (-1, 1, 0, 1), 0.1, 2), ("c", (1, 1, 1, 1), 0.3, 0)]
+SHIFT_KAPPA = (0.15, 0.02)
+
+
+def slice_summary(n, psi):
+    """canonical field -> (timeslices 0 and nt-1, per-timeslice |.|^2, per-timeslice sums)."""
+    ls = n[0] * n[1] * n[2]
+    t = psi.reshape(n[3], ls, 4, 3)
+    return (np.concatenate([t[0], t[-1]]), np.sum(np.abs(t) ** 2, axis=(1, 2, 3)), t.sum(axis=(1, 2, 3)))
+
+
+def smear_inputs(n):
+    U = gauge.haar(n, 3)
+    rng = np.random.default_rng(1)
+    V = int(np.prod(n))
+    psi = rng.standard_normal((V, 4, 3)) + 1j * rng.standard_normal((V, 4, 3))
+    return U, psi
+
+
+def smear():
+    n = (8, 8, 8, 16)
+    refdrv.build(n, solvers=True)
+    U, psi = smear_inputs(n)
+    u = O.links_canon_to_ref(n, U)
+    ge, go = O.spinor_canon_to_ref(n, psi)
+    out = {}
+    for name, bc, a, ns in SMEAR_CASES:
+        R = refdrv.RefLattice(n, bc=bc)
+        he, ho = R.smear(u, ge, go, a, ns)
+        sl, nrm, sm = slice_summary(n, O.spinor_ref_to_canon(n, he, ho))
+        out[f"smear_{name}_slices"], out[f"smear_{name}_norms"], out[f"smear_{name}_sums"] = sl, nrm, sm
+    R = refdrv.RefLattice(n, bc=(1, 1, 1, -1))
+    co = O.coact_array()
+    for idag in (0, 1):
+        h1, h2 = R.twmshift(u, ge, go, co, SHIFT_KAPPA, idag)
+        sl, nrm, sm = slice_summary(n, O.spinor_ref_to_canon(n, h1, h2))
+        out[f"twm{idag}_slices"], out[f"twm{idag}_norms"], out[f"twm{idag}_sums"] = sl, nrm, sm
+        g5 = R.gamma5mult(u, ge, co, SHIFT_KAPPA, idag)
+        sl, nrm, sm = slice_summary(n, O.spinor_ref_to_canon(n, g5, None))
+        out[f"g5m{idag}_slices"], out[f"g5m{idag}_norms"], out[f"g5m{idag}_sums"] = sl, nrm, sm
+    np.savez_compressed(os.path.join(HERE, "golden_smear_8x8x8x16.npz"), **out)
+    print("wrote golden_smear_8x8x8x16.npz")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["loops"]:
         loops()
@@ -339,5 +386,7 @@ if __name__ == "__main__":
         lagfib()
     elif sys.argv[1:] == ["sweep"]:
         sweep()
+    elif sys.argv[1:] == ["smear"]:
+        smear()
     else:
         main()
