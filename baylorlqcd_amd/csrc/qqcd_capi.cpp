@@ -470,6 +470,24 @@ extern "C" int qqcd_gauge_get_canonical(double *U) {
 }
 
 namespace qqcd {
+void gauge_to_ref(const Geom &g, const Gauge &U, double *dref, hipStream_t s);
+}
+
+extern "C" int qqcd_gauge_get_ref(double *u) {
+  API_BEGIN
+  Context &c = ctx();
+  if (!c.have_gauge) throw std::runtime_error("no gauge field loaded");
+  const size_t n = (size_t)18 * ref_ntotal(c.g) * 128;
+  double *d = nullptr;
+  QQCD_HIP(hipMalloc(&d, n * sizeof(double)));
+  gauge_to_ref(c.g, c.U64, d, c.stream);
+  QQCD_HIP(hipMemcpyAsync(u, d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  QQCD_HIP(hipStreamSynchronize(c.stream));
+  hipFree(d);
+  API_END
+}
+
+namespace qqcd {
 void heatbath(Context &c, double beta, int nsweep, uint64_t seed, int hot_start);
 double plaquette(Context &c);
 }  // namespace qqcd
@@ -501,6 +519,25 @@ extern "C" int qqcd_bench_blas(int prec, int k, int op, int grid, int niter, dou
   API_BEGIN
   Context &c = ctx();
   *ms = bench_blas(12 * c.g.stride, (Prec)prec, k, op, grid, niter, c.stream);
+  API_END
+}
+
+namespace qqcd {
+double bench_smear(Context &c, int nwarm, int niter);
+}
+
+extern "C" int qqcd_bench_smear(int nwarm, int niter, double *ms) {
+  API_BEGIN
+  Context &c = ctx();
+  if (!c.have_gauge) throw std::runtime_error("no gauge field loaded");
+  *ms = bench_smear(c, nwarm, niter);
+  API_END
+}
+
+extern "C" int qqcd_bench_lagfib(long m, int niter, double *ms) {
+  API_BEGIN
+  Context &c = ctx();
+  *ms = bench_lagfib(m, niter, c.stream);
   API_END
 }
 

@@ -333,6 +333,43 @@ void lagfib_generate(LagState &st, long m, uint32_t *d_out, hipStream_t s) {
   QQCD_HIP(hipFree(dbad));
 }
 
+double bench_lagfib(long m, int niter, hipStream_t s) {
+  if (m <= 0 || niter <= 0) throw std::runtime_error("bench_lagfib: m, niter > 0");
+  LagState st;
+  lagfib_init(0, 1, st);
+  uint32_t hb[LP];
+  for (int k = 0; k < LP; k++) hb[k] = st.w[(st.jr + k) % LP];
+  long L = 8192;
+  while ((m + L - 1) / L > KMAX && L < (1L << 21)) L *= 4;
+  m = std::min(m, (long)KMAX * L);
+  const int K = (int)((m + L - 1) / L);
+  const uint32_t *rows = table_for(L, K, s);
+  uint32_t *dbase = nullptr, *out = nullptr;
+  unsigned long long *dz = nullptr;
+  QQCD_HIP(hipMalloc(&dbase, sizeof(hb)));
+  QQCD_HIP(hipMalloc(&dz, sizeof(unsigned long long)));
+  QQCD_HIP(hipMalloc(&out, sizeof(uint32_t) * m));
+  QQCD_HIP(hipMemcpyAsync(dbase, hb, sizeof(hb), hipMemcpyHostToDevice, s));
+  QQCD_HIP(hipMemsetAsync(dz, 0xff, sizeof(unsigned long long), s));
+  hipEvent_t e0, e1;
+  QQCD_HIP(hipEventCreate(&e0));
+  QQCD_HIP(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_lag_gen, dim3((K + 1) / 2), dim3(64), 0, s, dbase, rows, L, m, K, out, dz);
+  QQCD_HIP(hipEventRecord(e0, s));
+  for (int it = 0; it < niter; it++)
+    hipLaunchKernelGGL(k_lag_gen, dim3((K + 1) / 2), dim3(64), 0, s, dbase, rows, L, m, K, out, dz);
+  QQCD_HIP(hipEventRecord(e1, s));
+  QQCD_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QQCD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(dbase);
+  (void)hipFree(dz);
+  (void)hipFree(out);
+  return ms / niter;
+}
+
 __global__ void k_lag_double(const uint32_t *y, long m, double *rn) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m) rn[t] = (double)(int)y[t] * LAG_FNORM;

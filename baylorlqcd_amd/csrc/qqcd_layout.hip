@@ -272,6 +272,36 @@ void spinor_to_canon(const Geom &g, const Spinor &in, double *dcn, int par, hipS
   DISPATCH_S(in.prec, k_spinor_to_canon, largs(g, par), in.data, in.norm, dcn);
 }
 
+// the FP64/18 device field back to the reference layout u(18, ntotal, 4, 2, 16)
+__global__ void k_gauge_to_ref(const LayoutArgs a, const double2 *U, double *dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mu = blockIdx.y;
+  if (i >= a.Vh) return;
+  const Coord c = coords_of(a.n, a.par, i);
+  long ri;
+  int ieo, ibl, ibleo;
+  ref_slot(a.n, c, ri, ieo, ibl, ibleo);
+  const long base = (long)((a.par * 4 + mu) * 9) * a.stride + i;
+  double *p = dst + 18 * (ri + a.ntotal * (mu + 4 * (ieo + 2 * (long)ibl)));
+  for (int k = 0; k < 9; k++) {
+    const double2 v = U[base + (long)k * a.stride];
+    p[2 * k] = v.x;
+    p[2 * k + 1] = v.y;
+  }
+}
+
+void gauge_to_ref(const Geom &g, const Gauge &U, double *dref, hipStream_t s) {
+  if (U.prec != PREC_DOUBLE || U.recon != 18) throw std::runtime_error("gauge_to_ref: master must be FP64/18");
+  QQCD_HIP(hipMemsetAsync(dref, 0, sizeof(double) * 18 * ref_ntotal(g) * 128, s));
+  const int nb = (int)((g.Vh + 255) / 256);
+  for (int par = 0; par < 2; par++) {
+    LayoutArgs a = largs(g, par);
+    a.stride = U.stride;
+    hipLaunchKernelGGL(k_gauge_to_ref, dim3(nb, 4), dim3(256), 0, s, a, (const double2 *)U.data, dref);
+    QQCD_HIP(hipGetLastError());
+  }
+}
+
 static void gauge_any(const Geom &g, int src_ref, const double *src, Gauge &U, hipStream_t s) {
   const int nc = U.recon == 18 ? 9 : 6;
   const int nb = (int)((g.Vh + 255) / 256);

@@ -128,4 +128,32 @@ void smear(Context &c, Spinor h[2], Spinor t[2], double asmear, int nsmear) {
       QQCD_HIP(hipMemcpyAsync(h[p].data, cur[p].data, h[p].bytes(), hipMemcpyDeviceToDevice, c.stream));
 }
 
+// kernel time (ms) of one smearing iteration (both checkerboards) on the
+// resident links, averaged over niter iterations after nwarm
+double bench_smear(Context &c, int nwarm, int niter) {
+  Spinor h[2], t[2];
+  for (int p = 0; p < 2; p++) {
+    h[p] = alloc_spinor(c.g, PREC_DOUBLE);
+    t[p] = alloc_spinor(c.g, PREC_DOUBLE);
+    QQCD_HIP(hipMemsetAsync(h[p].data, 0, h[p].bytes(), c.stream));
+  }
+  hipEvent_t e0, e1;
+  QQCD_HIP(hipEventCreate(&e0));
+  QQCD_HIP(hipEventCreate(&e1));
+  smear(c, h, t, 0.25, 2 * nwarm);
+  QQCD_HIP(hipEventRecord(e0, c.stream));
+  smear(c, h, t, 0.25, 2 * niter);  // an even count: no copy back
+  QQCD_HIP(hipEventRecord(e1, c.stream));
+  QQCD_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QQCD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int p = 0; p < 2; p++) {
+    free_spinor(h[p]);
+    free_spinor(t[p]);
+  }
+  return ms / (2.0 * niter);
+}
+
 }  // namespace qqcd

@@ -137,6 +137,14 @@ def gauge_get_canonical(n):
     return U
 
 
+def gauge_get_ref(n=None):
+    """The resident links in the reference layout (flat Fortran order of u(18, ntotal, 4, 2, 16))."""
+    from . import lattice
+    u = np.zeros(int(np.prod(lattice.link_shape(n))))
+    _lib.check(_lib.lib().qqcd_gauge_get_ref(_lib.dp(u)))
+    return u
+
+
 def gauge_heatbath(beta, nsweep, seed, start, coact):
     """Device pseudo-heatbath (start: 0 cold, 1 hot, -1 continue)."""
     _lib.check(_lib.lib().qqcd_gauge_heatbath(float(beta), int(nsweep), int(seed), int(start),

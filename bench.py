@@ -38,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from baylorlqcd_amd import _lib, diracops, disconloops, gauge, lattice, learned, quark  # noqa: E402
+from baylorlqcd_amd.lagfib import LagFib  # noqa: E402
 
 N = (32, 32, 32, 64)
 BC = (1, 1, 1, -1)
@@ -95,6 +96,72 @@ def cpu_baseline(U, vh):
     dt = (time.perf_counter() - t0) / reps
     return {"value": b / dt / 1e9, "unit": "GB/s", "cores": nth, "kind": "port",
             "sample": f"{reps} applications of the C oracle hopping term on 32^3x64 ({dt*1e3:.0f} ms each)"}
+
+
+def cpu_sweep_baseline():
+    """The reference's own sweep (oracle/_ref, 8^3x16 build, one core): link updates per second."""
+    sys.path.insert(0, ROOT)
+    import oracle as O
+    from oracle import refdrv
+    n = (8, 8, 8, 16)
+    if not refdrv.available(n, solvers=True):
+        return None
+    R = refdrv.RefLattice(n, bc=(1, 1, 1, 1))
+    u = O.links_canon_to_ref(n, gauge.haar(n, 7))
+    co = np.zeros(64)
+    for mu in range(4):
+        for nu in range(4):
+            if mu != nu:
+                co[4 * mu + 16 * nu] = 6.0
+    R.lagfib_init(0, 1)
+    u = R.sweeps(u, co, 1)
+    reps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 3.0:
+        u = R.sweeps(u, co, 1)
+        reps += 1
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": round(4 * int(np.prod(n)) / dt / 1e6, 2), "unit": "Mlink updates/s", "cores": 1,
+            "kind": "reference", "sample": f"{reps} sweeps of the reference Fortran on 8^3x16 ({dt*1e3:.1f} ms each)"}
+
+
+def rows_section(no_cpu=False):
+    """The other rows of the path on 32^3x64: Gaussian smearing, the reference's lagged-Fibonacci
+    stream and the reference's heatbath sweep drawing from it."""
+    V = int(np.prod(N))
+    L = _lib.lib()
+    ms = np.zeros(1)
+    _lib.check(L.qqcd_bench_smear(3, 20, _lib.dp(ms)))
+    g = 2400 * V / (ms[0] * 1e-3) / 1e9
+    smear = {"us_per_iteration": round(ms[0] * 1e3, 1), "bytes_per_site": 2400, "GB/s": round(g, 1),
+             "frac": round(g / HBM_PEAK_GBS, 3)}
+    print(f"[bench] smear {smear}", file=sys.stderr, flush=True)
+    m = 2048 * 32768
+    _lib.check(L.qqcd_bench_lagfib(m, 10, _lib.dp(ms)))
+    lag = {"draws": m, "us": round(ms[0] * 1e3, 1), "Gdraws/s": round(m / (ms[0] * 1e-3) / 1e9, 2),
+           "GB/s_written": round(4 * m / (ms[0] * 1e-3) / 1e9, 1), "frac": round(4 * m / (ms[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
+    print(f"[bench] lagfib {lag}", file=sys.stderr, flush=True)
+    u = diracops.gauge_get_ref()
+    co = np.zeros(64)
+    for mu in range(4):
+        for nu in range(4):
+            if mu != nu:
+                co[4 * mu + 16 * nu] = 6.0
+    st = LagFib(0, 1)
+    t = []
+    for nsw in (1, 3):
+        t0 = time.perf_counter()
+        st.sweep(u, co, nsw)
+        t.append(time.perf_counter() - t0)
+    per = (t[1] - t[0]) / 2
+    sweep = {"s_per_sweep": round(per, 3), "Mlink_updates/s": round(4 * V / per / 1e6, 1),
+             "call_with_transfers_s": round(t[0], 3), "stream": "the reference's lagfib, initlagfib(0, 1)"}
+    if not no_cpu:
+        try:
+            sweep["cpu_baseline"] = cpu_sweep_baseline()
+        except Exception as e:  # reported, never required
+            sweep["cpu_baseline"] = {"value": None, "sample": f"unavailable: {e}"}
+    print(f"[bench] sweep {sweep}", file=sys.stderr, flush=True)
+    return {"smear": smear, "lagfib": lag, "sweep_reference_order": sweep}
 
 
 def bench_solve(method, kappa, op_prec=2, recon=12, mk=(40, 16), tol=1e-10):
@@ -279,6 +346,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-rows", action="store_true", help="skip smearing, the lagfib stream and the reference sweep")
     ap.add_argument("--no-noise", action="store_true", help="skip the 24^3x48 batched noise-source section")
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--noise-rhs", type=int, default=64)
@@ -342,6 +410,10 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             cpu = {"value": None, "unit": "GB/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
 
+    rows = None
+    if not args.no_rows:
+        rows = rows_section(args.no_cpu)
+
     solve = None
     if not args.no_solve:
         solve = solve_section(args.train_steps)
@@ -378,6 +450,7 @@ def main():
         "sweep": sweep,
         "solve": solve,
         "noise_batch": noise,
+        "rows": rows,
     }
     if copy:
         line["copy_probe"] = copy

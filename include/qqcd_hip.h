@@ -72,6 +72,11 @@ int qqcd_gauge_plaquette(double *plaq);
 /* Copy the resident FP64 field back in canonical order (see above). */
 int qqcd_gauge_get_canonical(double *U);
 
+/* The resident FP64 links in the reference layout u(18, ntotal, 4, 2, 16)
+ * (rows nvhalf+1..ntotal zero): hands a device-generated field (heatbath,
+ * random) back to a reference host. */
+int qqcd_gauge_get_ref(double *u);
+
 /* ---- operators (qqcd-test/cfgsprops/quark/diracops.f90) ---------------- */
 
 /* Hsingle(gout,u,gin,idag,coact,bc,gblclr,...), diracops.f90:554-599:
@@ -479,6 +484,11 @@ int qqcd_bench_dslash_multi(int prec, int recon, int nrhs, int nact, int nb, int
 /* The autotuner's choice of RHS block and tile order for nrhs right-hand
  * sides on the resident gauge field, and its time per launch. */
 int qqcd_autotune_multi(int prec, int recon, int nrhs, int *nb, int *order, double *ms);
+
+/* Kernel time (ms) of one smearing iteration (both checkerboards, FP64) on
+ * the resident links; of generating m draws of the lagged-Fibonacci stream. */
+int qqcd_bench_smear(int nwarm, int niter, double *ms);
+int qqcd_bench_lagfib(long m, int niter, double *ms);
 
 /* Time a plain device-to-device copy kernel (read + write `bytes` each
  * launch): the calibration point for hardware byte counters. */

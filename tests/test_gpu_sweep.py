@@ -127,3 +127,14 @@ def test_sweep_rejects_improved_action(gold):
     g, u0 = gold
     with pytest.raises(Exception):
         LagFib(0, 3).sweep(u0.copy(), g["iso_coact"], 1, gaction=(1, 1, 1, 2))
+
+
+def test_gauge_get_ref_round_trip(gold):
+    # the resident links handed back in the reference layout: bit-identical, halo rows zero
+    g, u0 = gold
+    diracops.gauge_load(u0, O.coact_array())
+    back = diracops.gauge_get_ref(N)
+    nvh, ntot = dims(N)
+    a = back.reshape(16, 2, 4, ntot, 18)
+    assert np.array_equal(a[:, :, :, :nvh], u0.reshape(16, 2, 4, ntot, 18)[:, :, :, :nvh])
+    assert not np.any(a[:, :, :, nvh:])
