@@ -8,9 +8,11 @@
 // set, so a set is one data-parallel step; its sites are taken in the
 // reference's site order i = 1..nvhalf (one lane per site) because that order
 // decides which random numbers each site receives:
-//   staple (qqcd-test/cfgsprops/glue/gaugetools.f90:39-345, plaquette terms):
-//     c = sum_{nu != mu} coact(1,mu,nu) [mmd(mm(U_nu(x), U_mu(x+nu)), U_nu(x+mu))
-//                                      + mm(mdm(U_nu(x-nu), U_mu(x-nu)), U_nu(x-nu+mu))]
+//   staple (qqcd-test/cfgsprops/glue/gaugetools.f90:39-345): per nu != mu the
+//     plaquette paths times coact(1,mu,nu),
+//       mmd(mm(U_nu(x), U_mu(x+nu)), U_nu(x+mu)) + mm(mdm(U_nu(x-nu), U_mu(x-nu)), U_nu(x-nu+mu)),
+//     and for the improved actions (gaction = 2) the six 2x1 rectangles
+//     (k_hb_staple lists their product chains)
 //   phbup (qqcd-test/cfgsprops/glue/heatbath.f90:33-76), for the SU(2)
 //   subgroups (1,2), (2,3), (1,3): phbsub1-3 project rows (p,q) of u c^dagger
 //   to z (4 reals) (:80-235); su2up (:239-323) runs Kennedy-Pendleton passes:
@@ -28,9 +30,7 @@
 // may differ from the host's in the last place.
 //
 // Links live on the device in the reference's own layout
-// u(18, ntotal, 4, 2, 16) for the duration of the call.  Only gaction = 1
-// in every direction (plaquette staples) is implemented; other actions are
-// rejected.
+// u(18, ntotal, 4, 2, 16) for the duration of the call.
 #include <algorithm>
 #include <stdexcept>
 #include <string>
@@ -125,12 +125,40 @@ __device__ __forceinline__ void hb_mdm(const double y[18], const double v[18], d
     }
 }
 
-// staple sum of the links U_mu(x), x in set (ieo, ibl): C[k*nvh + i]
-__global__ void __launch_bounds__(HB_TB) k_hb_staple(HbG G, const double *u, int ieo, int ibl, int mu, double w0,
-                                                     double w1, double w2, double w3, double *C) {
+// coact(1, mu, nu), coact(2, mu, nu), coact(2, nu, mu) for the link direction mu, and gaction
+struct HbCoef {
+  double c1[4], c2[4], c2t[4];
+  int ga[4];
+};
+
+__device__ __forceinline__ void hb_at(const HbG &G, const int x[4], int d1, int k1, int d2, int k2, int y[4]) {
+  for (int m = 0; m < 4; m++) y[m] = x[m];
+  y[d1] = (y[d1] + k1 + 2 * G.n[d1]) % G.n[d1];
+  y[d2] = (y[d2] + k2 + 2 * G.n[d2]) % G.n[d2];
+}
+
+// the link U_d at x + k1 e_d1 + k2 e_d2
+__device__ __forceinline__ void hb_ld(const HbG &G, const double *u, const int x[4], int d, int d1, int k1, int d2,
+                                      int k2, double a[18]) {
+  int y[4];
+  hb_at(G, x, d1, k1, d2, k2, y);
+  hb_load(u, hb_link(G, y, d), a);
+}
+
+__device__ __forceinline__ void hb_acc(double c[18], double w, const double s[18]) {
+#pragma unroll
+  for (int k = 0; k < 18; k++) c[k] = c[k] + w * s[k];
+}
+
+// staple sum of the links U_mu(x), x in set (ieo, ibl): C[k*nvh + i].  Every
+// term is a path from x to x+mu, in the reference's order of products:
+// plaquettes 1, 2 (coact(1,mu,nu)); for gaction(mu) = 2 the rectangles 3-6,
+// long in mu (coact(2,mu,nu)); for gaction(nu) = 2 the rectangles 7-8, long
+// in nu (coact(2,nu,mu)).
+__global__ void __launch_bounds__(HB_TB) k_hb_staple(HbG G, const double *u, int ieo, int ibl, int mu, HbCoef K,
+                                                     double *C) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= G.nvh) return;
-  const double wcoef[4] = {w0, w1, w2, w3};
   int x[4];
   hb_site(G, i, ieo, ibl, x);
   double c[18];
@@ -138,30 +166,91 @@ __global__ void __launch_bounds__(HB_TB) k_hb_staple(HbG G, const double *u, int
   for (int k = 0; k < 18; k++) c[k] = 0.0;
   for (int nu = 0; nu < 4; nu++) {
     if (nu == mu) continue;
-    int xpn[4], xpm[4], xmn[4], xmnpm[4];
-    for (int m = 0; m < 4; m++) xpn[m] = xpm[m] = xmn[m] = x[m];
-    xpn[nu] = (x[nu] + 1) % G.n[nu];
-    xpm[mu] = (x[mu] + 1) % G.n[mu];
-    xmn[nu] = (x[nu] + G.n[nu] - 1) % G.n[nu];
-    for (int m = 0; m < 4; m++) xmnpm[m] = xmn[m];
-    xmnpm[mu] = (xmn[mu] + 1) % G.n[mu];
-    double a[18], b[18], t[18], w[18];
-    // staple 1: U_nu(x) U_mu(x+nu) U_nu(x+mu)^dagger
-    hb_load(u, hb_link(G, x, nu), a);
-    hb_load(u, hb_link(G, xpn, mu), b);
+    double a[18], b[18], t[18], w[18], v[18];
+    // 1: U_nu(x) U_mu(x+nu) U_nu(x+mu)^dag
+    hb_ld(G, u, x, nu, nu, 0, mu, 0, a);
+    hb_ld(G, u, x, mu, nu, 1, mu, 0, b);
     hb_mm(a, b, t);
-    hb_load(u, hb_link(G, xpm, nu), a);
+    hb_ld(G, u, x, nu, mu, 1, nu, 0, a);
     hb_mmd(t, a, w);
-#pragma unroll
-    for (int k = 0; k < 18; k++) c[k] = c[k] + wcoef[nu] * w[k];
-    // staple 2: U_nu(x-nu)^dagger U_mu(x-nu) U_nu(x-nu+mu)
-    hb_load(u, hb_link(G, xmn, nu), a);
-    hb_load(u, hb_link(G, xmn, mu), b);
+    hb_acc(c, K.c1[nu], w);
+    // 2: U_nu(x-nu)^dag U_mu(x-nu) U_nu(x-nu+mu)
+    hb_ld(G, u, x, nu, nu, -1, mu, 0, a);
+    hb_ld(G, u, x, mu, nu, -1, mu, 0, b);
     hb_mdm(a, b, t);
-    hb_load(u, hb_link(G, xmnpm, nu), a);
+    hb_ld(G, u, x, nu, nu, -1, mu, 1, a);
     hb_mm(t, a, w);
-#pragma unroll
-    for (int k = 0; k < 18; k++) c[k] = c[k] + wcoef[nu] * w[k];
+    hb_acc(c, K.c1[nu], w);
+    if (K.ga[mu] != 1) {
+      // 3: [U_nu(x) U_mu(x+nu)] [U_mu(x+mu) U_nu(x+2mu) U_mu(x+mu+nu)^dag]^dag
+      hb_ld(G, u, x, mu, mu, 1, nu, 0, a);
+      hb_ld(G, u, x, nu, mu, 2, nu, 0, b);
+      hb_mm(a, b, t);
+      hb_ld(G, u, x, mu, mu, 1, nu, 1, a);
+      hb_mmd(t, a, v);
+      hb_ld(G, u, x, nu, nu, 0, mu, 0, a);
+      hb_ld(G, u, x, mu, nu, 1, mu, 0, b);
+      hb_mm(a, b, t);
+      hb_mmd(t, v, w);
+      hb_acc(c, K.c2[nu], w);
+      // 4: [U_nu(x-nu)^dag U_mu(x-nu)] [U_mu(x-nu+mu) U_nu(x-nu+2mu) U_mu(x+mu)^dag]
+      hb_ld(G, u, x, mu, nu, -1, mu, 1, a);
+      hb_ld(G, u, x, nu, nu, -1, mu, 2, b);
+      hb_mm(a, b, t);
+      hb_ld(G, u, x, mu, mu, 1, nu, 0, a);
+      hb_mmd(t, a, v);
+      hb_ld(G, u, x, nu, nu, -1, mu, 0, a);
+      hb_ld(G, u, x, mu, nu, -1, mu, 0, b);
+      hb_mdm(a, b, t);
+      hb_mm(t, v, w);
+      hb_acc(c, K.c2[nu], w);
+      // 5: U_mu(x-mu)^dag U_nu(x-mu) U_mu(x-mu+nu) U_mu(x+nu) U_nu(x+mu)^dag
+      hb_ld(G, u, x, mu, mu, -1, nu, 0, a);
+      hb_ld(G, u, x, nu, mu, -1, nu, 0, b);
+      hb_mdm(a, b, t);
+      hb_ld(G, u, x, mu, mu, -1, nu, 1, a);
+      hb_mm(t, a, v);
+      hb_ld(G, u, x, mu, nu, 1, mu, 0, a);
+      hb_mm(v, a, t);
+      hb_ld(G, u, x, nu, mu, 1, nu, 0, a);
+      hb_mmd(t, a, w);
+      hb_acc(c, K.c2[nu], w);
+      // 6: [[U_mu(z)^dag U_nu(z) U_mu(z+nu)]^dag U_mu(x-nu)] U_nu(x-nu+mu), z = x-nu-mu
+      hb_ld(G, u, x, mu, nu, -1, mu, -1, a);
+      hb_ld(G, u, x, nu, nu, -1, mu, -1, b);
+      hb_mdm(a, b, t);
+      hb_ld(G, u, x, mu, mu, -1, nu, 0, a);
+      hb_mm(t, a, v);
+      hb_ld(G, u, x, mu, nu, -1, mu, 0, a);
+      hb_mdm(v, a, t);
+      hb_ld(G, u, x, nu, nu, -1, mu, 1, a);
+      hb_mm(t, a, w);
+      hb_acc(c, K.c2[nu], w);
+    }
+    if (K.ga[nu] != 1) {
+      // 7: [U_nu(x) [U_nu(x+nu) U_mu(x+2nu) U_nu(x+nu+mu)^dag]] U_nu(x+mu)^dag
+      hb_ld(G, u, x, nu, nu, 1, mu, 0, a);
+      hb_ld(G, u, x, mu, nu, 2, mu, 0, b);
+      hb_mm(a, b, t);
+      hb_ld(G, u, x, nu, nu, 1, mu, 1, a);
+      hb_mmd(t, a, v);
+      hb_ld(G, u, x, nu, nu, 0, mu, 0, a);
+      hb_mm(a, v, t);
+      hb_ld(G, u, x, nu, mu, 1, nu, 0, a);
+      hb_mmd(t, a, w);
+      hb_acc(c, K.c2t[nu], w);
+      // 8: [U_nu(x-nu)^dag [U_nu(x-2nu)^dag U_mu(x-2nu) U_nu(x-2nu+mu)]] U_nu(x-nu+mu)
+      hb_ld(G, u, x, nu, nu, -2, mu, 0, a);
+      hb_ld(G, u, x, mu, nu, -2, mu, 0, b);
+      hb_mdm(a, b, t);
+      hb_ld(G, u, x, nu, nu, -2, mu, 1, a);
+      hb_mm(t, a, v);
+      hb_ld(G, u, x, nu, nu, -1, mu, 0, a);
+      hb_mdm(a, v, t);
+      hb_ld(G, u, x, nu, nu, -1, mu, 1, a);
+      hb_mm(t, a, w);
+      hb_acc(c, K.c2t[nu], w);
+    }
   }
 #pragma unroll
   for (int k = 0; k < 18; k++) C[(long)k * G.nvh + i] = c[k];
@@ -426,8 +515,8 @@ void heatbath_reference(Context &c, const int gaction[4], const double *coact, i
                         LagState &st) {
   const Geom &g = c.g;
   for (int m = 0; m < 4; m++) {
-    if (gaction[m] != 1)
-      throw std::runtime_error("sweep: only gaction = 1 in every direction (plaquette staples) is implemented");
+    if (gaction[m] != 1 && gaction[m] != 2)
+      throw std::runtime_error("sweep: gaction(mu) is 1 (nearest neighbour) or 2 (next-nearest)");
     if (g.n[m] % 2) throw std::runtime_error("sweep: even extents expected");
   }
   if (g.n[0] % 4) throw std::runtime_error("sweep: nx must be divisible by 4");
@@ -454,10 +543,14 @@ void heatbath_reference(Context &c, const int gaction[4], const double *coact, i
     for (int ieo = 0; ieo < 2; ieo++)
       for (int ibl = 0; ibl < 16; ibl++)
         for (int mu = 0; mu < 4; mu++) {
-          double w[4];
-          for (int nu = 0; nu < 4; nu++) w[nu] = coact[4 * mu + 16 * nu];  // coact(1, mu, nu)
-          hipLaunchKernelGGL(k_hb_staple, grid, blk, 0, c.stream, G, du.p, ieo, ibl, mu, w[0], w[1], w[2], w[3],
-                             C.p);
+          HbCoef K{};
+          for (int nu = 0; nu < 4; nu++) {
+            K.c1[nu] = coact[4 * mu + 16 * nu];       // coact(1, mu, nu)
+            K.c2[nu] = coact[1 + 4 * mu + 16 * nu];   // coact(2, mu, nu)
+            K.c2t[nu] = coact[1 + 4 * nu + 16 * mu];  // coact(2, nu, mu)
+            K.ga[nu] = gaction[nu];
+          }
+          hipLaunchKernelGGL(k_hb_staple, grid, blk, 0, c.stream, G, du.p, ieo, ibl, mu, K, C.p);
           QQCD_HIP(hipGetLastError());
           for (int isu2 = 1; isu2 <= 3; isu2++) {
             hipLaunchKernelGGL(k_hb_proj, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, isu2, Z.p, DK.p,

@@ -103,10 +103,11 @@ class _Stream:
         return rn
 
 
-def sweep(n, u, coact, state, nsweep=1, nhit=1):
+def sweep(n, u, coact, state, nsweep=1, nhit=1, gaction=(1, 1, 1, 1)):
     """nsweep sweeps; returns (u, state).  u flat Fortran (18, ntotal, 4, 2, 16)."""
     if nhit < 1:
         raise ValueError("nhit >= 1")
+    ga = [int(v) for v in gaction]
     nvh, ntot = _dims(n)
     U = np.array(u, dtype=np.float64, copy=True).reshape(16, 2, 4, ntot, 18)
     co = np.asarray(coact, dtype=np.float64).reshape(4, 4, 4, order="F")  # co[i, mu, nu]
@@ -117,7 +118,7 @@ def sweep(n, u, coact, state, nsweep=1, nhit=1):
             for ibl in range(16):
                 x = _sites(n, ieo, ibl)
                 for mu in range(4):
-                    c = _staple(n, U, x, mu, co, nn)
+                    c = _staple(n, U, x, mu, co, nn, ga)
                     for isu2 in (1, 2, 3):
                         _phbup_sub(U, ieo, ibl, mu, isu2, c, rs, nvh)
     return U.reshape(-1), rs.state
@@ -128,8 +129,11 @@ def _link(n, U, x, mu):
     return U[b, e, mu, i, :]
 
 
-def _staple(n, U, x, mu, co, nn):
+def _staple(n, U, x, mu, co, nn, ga):
+    """plaquette staples (1, 2) and, for gaction = 2, the rectangles: 3-6 long in mu (coact(2,mu,nu)),
+    7-8 long in nu (coact(2,nu,mu)), each a path from x to x+mu in the reference's product order."""
     c = np.zeros((x.shape[0], 18))
+    L = lambda y, d: _link(n, U, y % nn, d)  # noqa: E731
     for nu in range(4):
         if nu == mu:
             continue
@@ -137,11 +141,35 @@ def _staple(n, U, x, mu, co, nn):
         dn[nu] = 1
         dm = np.zeros(4, dtype=np.int64)
         dm[mu] = 1
-        xpn, xpm, xmn = (x + dn) % nn, (x + dm) % nn, (x - dn) % nn
-        w = _mmd(_mm(_link(n, U, x, nu), _link(n, U, xpn, mu)), _link(n, U, xpm, nu))
+        w = _mmd(_mm(L(x, nu), L(x + dn, mu)), L(x + dm, nu))
         c = c + co[0, mu, nu] * w
-        w = _mm(_mdm(_link(n, U, xmn, nu), _link(n, U, xmn, mu)), _link(n, U, (xmn + dm) % nn, nu))
+        w = _mm(_mdm(L(x - dn, nu), L(x - dn, mu)), L(x - dn + dm, nu))
         c = c + co[0, mu, nu] * w
+        if ga[mu] != 1:
+            c2 = co[1, mu, nu]
+            # 3: x, x+nu, x+nu+mu, x+nu+2mu, x+2mu, x+mu
+            w = _mmd(_mm(L(x, nu), L(x + dn, mu)), _mmd(_mm(L(x + dm, mu), L(x + 2 * dm, nu)), L(x + dm + dn, mu)))
+            c = c + c2 * w
+            # 4: x, x-nu, x-nu+mu, x-nu+2mu, x+2mu, x+mu
+            w = _mm(_mdm(L(x - dn, nu), L(x - dn, mu)),
+                    _mmd(_mm(L(x - dn + dm, mu), L(x - dn + 2 * dm, nu)), L(x + dm, mu)))
+            c = c + c2 * w
+            # 5: x, x-mu, x-mu+nu, x+nu, x+nu+mu, x+mu
+            w = _mmd(_mm(_mm(_mdm(L(x - dm, mu), L(x - dm, nu)), L(x - dm + dn, mu)), L(x + dn, mu)), L(x + dm, nu))
+            c = c + c2 * w
+            # 6: x, x-mu, x-mu-nu, x-nu, x-nu+mu, x+mu
+            z = x - dn - dm
+            w = _mm(_mdm(_mm(_mdm(L(z, mu), L(z, nu)), L(z + dn, mu)), L(x - dn, mu)), L(x - dn + dm, nu))
+            c = c + c2 * w
+        if ga[nu] != 1:
+            c2 = co[1, nu, mu]
+            # 7: x, x+nu, x+2nu, x+2nu+mu, x+nu+mu, x+mu
+            w = _mmd(_mm(L(x, nu), _mmd(_mm(L(x + dn, nu), L(x + 2 * dn, mu)), L(x + dn + dm, nu))), L(x + dm, nu))
+            c = c + c2 * w
+            # 8: x, x-nu, x-2nu, x-2nu+mu, x-nu+mu, x+mu
+            w = _mm(_mdm(L(x - dn, nu), _mm(_mdm(L(x - 2 * dn, nu), L(x - 2 * dn, mu)), L(x - 2 * dn + dm, nu))),
+                    L(x - dn + dm, nu))
+            c = c + c2 * w
     return c
 
 

@@ -268,6 +268,19 @@ contains
       end do
    end subroutine ref_heatbath
 
+   ! staple(mu,ieo,ibl,coact,gaction,c,u,...) (gaugetools.f90:39-345) of one link set
+   subroutine ref_staple(mu, ieo, ibl, coact, gaction, c, u) bind(C, name="ref_staple")
+      integer(c_int), value         :: mu, ieo, ibl
+      real(c_double), intent(in)    :: coact(4,4,4)
+      integer(c_int), intent(in)    :: gaction(4)
+      real(c_double), intent(out)   :: c(18,nvhalf)
+      real(c_double), intent(inout) :: u(18,ntotal,4,2,16)
+      integer(kind=KI) :: MRT, ga(4)
+      MRT = MPI_DOUBLE_PRECISION
+      ga = gaction
+      call staple(mu, ieo, ibl, coact, ga, c, u, myid, nn, ldiv, nms, lv, ib, lbd, iblv, MRT)
+   end subroutine ref_staple
+
    ! The lagged-Fibonacci state (lagfib module variables): initlagfib(myid, iseed), and
    ! get / set of iwlagfib(0:520), jrlagfib, krlagfib.
    subroutine ref_lagfib_init(myidin, iseed, iw, jr, kr) bind(C, name="ref_lagfib_init")

@@ -247,6 +247,14 @@ class RefLattice:
         _run(self.L.ref_z2source, _p(be), _p(bo))
         return be, bo
 
+    def staple(self, u, coact, mu, ieo, ibl, gaction=(1, 1, 1, 1)):
+        """c(18, nvhalf) of link set (mu, ieo, ibl) (1-based), flat Fortran order."""
+        c = np.zeros(18 * self.nvhalf)
+        uu = np.array(u, dtype=np.float64, copy=True)
+        _run(self.L.ref_staple, int(mu), int(ieo), int(ibl), _p(np.ascontiguousarray(coact)),
+             _ip_(np.ascontiguousarray(gaction, dtype=np.int32)), _p(c), _p(uu))
+        return c
+
     def sweeps(self, u, coact, nsweep, gaction=(1, 1, 1, 1), nhit=1):
         uu = np.array(u, dtype=np.float64, copy=True)
         _run(self.L.ref_sweeps, _p(uu), _p(np.ascontiguousarray(coact)),

@@ -305,9 +305,21 @@ def sweep_summary(u, n):
     return U.sum(axis=3), U[0].copy()
 
 
+def sweep_coact_rect():
+    """an improved-action coact: plaquette and rectangle coefficients, both asymmetric in (mu, nu)."""
+    co = np.zeros((4, 4, 4), order="F")
+    for mu in range(4):
+        for nu in range(4):
+            if mu != nu:
+                co[0, mu, nu] = 2.0 + 0.1 * mu + 0.03 * nu
+                co[1, mu, nu] = -0.1 * (1 + 0.1 * mu + 0.03 * nu)
+    return np.asfortranarray(co).ravel(order="F")
+
+
 def sweep_cases():
-    """(name, coact builder, lagfib seed, sweeps)."""
-    return [("iso", None, 3, 2), ("aniso", "aniso", 4, 1)]
+    """(name, coact builder, lagfib seed, sweeps, gaction)."""
+    return [("iso", None, 3, 2, (1, 1, 1, 1)), ("aniso", "aniso", 4, 1, (1, 1, 1, 1)),
+            ("rect", "rect", 8, 1, (2, 2, 2, 2)), ("mixed", "rect", 9, 1, (1, 1, 1, 2))]
 
 
 def sweep():
@@ -317,18 +329,25 @@ def sweep():
     g = np.load(os.path.join(HERE, "golden_8x8x8x16.npz"))
     u0 = O.links_canon_to_ref(n, gauge.reconstruct12(g["links12"]))
     out = {}
-    for name, cob, seed, nsw in sweep_cases():
-        co = R.fixaction(6.0) if cob is None else sweep_coact_aniso()
+    for name, cob, seed, nsw, ga in sweep_cases():
+        co = {None: lambda: R.fixaction(6.0), "aniso": sweep_coact_aniso, "rect": sweep_coact_rect}[cob]()
         out[f"{name}_coact"] = np.asarray(co)
+        out[f"{name}_gaction"] = np.array(ga, np.int32)
         iw, jr, kr = R.lagfib_init(0, seed)
         u = u0
         for k in range(1, nsw + 1):
-            u = R.sweeps(u, co, 1)
+            u = R.sweeps(u, co, 1, gaction=ga)
             iw, jr, kr = R.lagfib_get()
             sums, blk = sweep_summary(u, n)
             out[f"{name}_{k}_iw"], out[f"{name}_{k}_jrkr"] = iw, np.array([jr, kr], np.int32)
             out[f"{name}_{k}_sums"], out[f"{name}_{k}_block1"] = sums, blk
             out[f"{name}_{k}_plaq"] = np.array(R.aveplaq(u))
+    # the staples of every link set of a Haar field with rectangles: bit-exact digest
+    import hashlib
+    uh = O.links_canon_to_ref(n, gauge.haar(n, 4))
+    st = np.concatenate([R.staple(uh, sweep_coact_rect(), mu, ieo, ibl, (2, 2, 2, 2))
+                         for ieo in (1, 2) for ibl in range(1, 17) for mu in range(1, 5)])
+    out["staples_rect_sha256"] = np.array(hashlib.sha256(st.tobytes()).hexdigest())
     np.savez_compressed(os.path.join(HERE, "golden_sweep_8x8x8x16.npz"), **out)
     print("wrote golden_sweep_8x8x8x16.npz")
 
