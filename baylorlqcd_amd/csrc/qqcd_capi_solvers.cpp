@@ -1,6 +1,8 @@
 // qqcd_capi_solvers.cpp -- extern "C" solver entry points on reference-order
 // host arrays: gmresdr, gmresproject, psc, bicgstab, fermprop.
+#include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -60,12 +62,41 @@ void check_mk(int m, int k) {
   if (m < 2 || k < 0 || k >= m || m > 120) throw std::runtime_error("GMRES sizes need 0 <= k < m <= 120");
 }
 
+// The lines the reference's solvers append to CFGSPROPS.LOG, format
+// (a12,i9,es17.10): GMRES-DR at every true residual "gmresdr-norm", matrix-
+// vector products, ||r|| / ||r0|| (inverters.f90:3285-3291); BiCGStab at
+// every iteration "bicgstab", iterations, ||r||^2 / ||r0|| as written there
+// (:476-480).  Cleared by every solve; read back with qqcd_solve_log.
+struct LogLine {
+  char label[13];
+  int count;
+  double value;
+};
+std::vector<LogLine> g_log;
+
+void log_line(const char *label, int count, double value) {
+  LogLine l{};
+  std::memset(l.label, ' ', 12);
+  std::memcpy(l.label, label, std::min<size_t>(12, std::strlen(label)));
+  l.count = count;
+  l.value = value;
+  g_log.push_back(l);
+}
+
+const IterHook &bicgstab_log() {
+  static const IterHook h = [](int it, double rn, double r0) { log_line("bicgstab", it, rn * rn / r0); };
+  return h;
+}
+
 SolveStats run_gmresdr(Context &c, const Spinor &b, Spinor &x, const int GMRES[2], double resmax, int itermin,
                        const double kappa[2]) {
   check_mk(GMRES[0], GMRES[1]);
   SchurOp A(c, PREC_DOUBLE, 18, kappa);
   LinOp op = A.op();
-  return gmresdr(c, op, b, x, GMRES[0], GMRES[1], resmax, itermin, KCYCLIM * GMRES[0], &defl());
+  const CycleHook hook = [](const Spinor &, const Spinor &, const SolveStats &st) {
+    log_line("gmresdr-norm", st.matvecs, st.rel_res);
+  };
+  return gmresdr(c, op, b, x, GMRES[0], GMRES[1], resmax, itermin, KCYCLIM * GMRES[0], &defl(), 1, nullptr, &hook);
 }
 
 SolveStats run_gmresproj(Context &c, const Spinor &b, Spinor &x, int m, int k, double resmax, int itermin,
@@ -86,6 +117,7 @@ extern "C" int qqcd_gmresdr(const double *phi, double *x, const int GMRES[2], do
                             int *itercount, const double *u, const double *coact, const double kappa[2], int iflag,
                             const int *bc) {
   API_BEGIN
+  g_log.clear();
   Context &c = ctx();
   prepare(c, u, coact, bc, iflag);
   SpinorBuf b(c.g, PREC_DOUBLE), X(c.g, PREC_DOUBLE);
@@ -101,6 +133,7 @@ extern "C" int qqcd_gmresproject(const double *bh, double *x, const int GMRES[2]
                                  int *itercount, const double *u, const double *coact, const double kappa[2],
                                  int iflag, const int *bc) {
   API_BEGIN
+  g_log.clear();
   Context &c = ctx();
   prepare(c, u, coact, bc, iflag);
   SpinorBuf b(c.g, PREC_DOUBLE), X(c.g, PREC_DOUBLE);
@@ -115,6 +148,7 @@ extern "C" int qqcd_psc(const double *bh, double *x, const int GMRES[2], double 
                         int *itercount, const double *u, const double *coact, const double kappa[2], int iflag,
                         const int *bc, int isignal, double resmax_proj) {
   API_BEGIN
+  g_log.clear();
   Context &c = ctx();
   prepare(c, u, coact, bc, iflag);
   check_mk(GMRES[0], GMRES[1]);
@@ -136,6 +170,7 @@ extern "C" int qqcd_psc(const double *bh, double *x, const int GMRES[2], double 
 extern "C" int qqcd_bicgstab(const double *phi, double *x, double resmax, int itermin, int *itercount,
                              const double *u, const double *coact, const double kappa[2], int iflag, const int *bc) {
   API_BEGIN
+  g_log.clear();
   Context &c = ctx();
   prepare(c, u, coact, bc, iflag);
   SpinorBuf b(c.g, PREC_DOUBLE), X(c.g, PREC_DOUBLE);
@@ -143,7 +178,7 @@ extern "C" int qqcd_bicgstab(const double *phi, double *x, double resmax, int it
   to_device(c, x, X.s, 0);
   SchurOp A(c, PREC_DOUBLE, 18, kappa);
   LinOp op = A.op();
-  g_last = bicgstab(c, op, b.s, X.s, resmax, itermin, 1000000);
+  g_last = bicgstab(c, op, b.s, X.s, resmax, itermin, 1000000, &bicgstab_log());
   to_host(c, X.s, x, 0);
   *itercount = g_last.iters;
   API_END
@@ -157,6 +192,7 @@ extern "C" int qqcd_fermprop(const double *be, const double *bo, const double ka
                              const double *coact, const int *bc, double resmax, int itermin, const int nGMRES[2],
                              int vectype, const double *u, double *xe, double *xo, int isignal, int *itercount) {
   API_BEGIN
+  g_log.clear();
   Context &c = ctx();
   prepare(c, u, coact, bc, -1);
   const double k = kappa[0], a = 2.0 * kappa[0] * kappa[1];
@@ -183,7 +219,7 @@ extern "C" int qqcd_fermprop(const double *be, const double *bo, const double ka
   if (inverter == 0) {
     SchurOp A(c, PREC_DOUBLE, 18, kappa);
     LinOp op = A.op();
-    g_last = bicgstab(c, op, src.s, x_e.s, resmax, itermin, 1000000);
+    g_last = bicgstab(c, op, src.s, x_e.s, resmax, itermin, 1000000, &bicgstab_log());
   } else if (inverter == 5) {
     g_last = run_gmresdr(c, src.s, x_e.s, nGMRES, resmax, itermin, kappa);
   } else if (inverter == 6) {
@@ -369,5 +405,17 @@ extern "C" int qqcd_bench_solve(int method, const int src[4], int spin, int colo
   auto t1 = std::chrono::steady_clock::now();
   *seconds = std::chrono::duration<double>(t1 - t0).count();
   *true_res = full_residual(c, kappa, b_e.s, b_o.s, x_e.s, x_o.s);
+  API_END
+}
+
+extern "C" int qqcd_solve_log(int cap, int *n, char *labels, int *counts, double *values) {
+  API_BEGIN
+  if (!n) throw std::runtime_error("qqcd_solve_log: null count");
+  *n = (int)g_log.size();
+  for (int i = 0; i < std::min(cap, *n); i++) {
+    if (labels) std::memcpy(labels + 12 * i, g_log[i].label, 12);
+    if (counts) counts[i] = g_log[i].count;
+    if (values) values[i] = g_log[i].value;
+  }
   API_END
 }

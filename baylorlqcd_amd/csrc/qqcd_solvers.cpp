@@ -468,7 +468,8 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
   return st;
 }
 
-SolveStats bicgstab(Context &c, LinOp &A, const Spinor &b, Spinor &x, double tol, int itermin, int maxiter) {
+SolveStats bicgstab(Context &c, LinOp &A, const Spinor &b, Spinor &x, double tol, int itermin, int maxiter,
+                    const IterHook *hook) {
   SolveStats st;
   const Prec p = A.prec;
   SpinorBuf r(c.g, p), rhat(c.g, p), pp(c.g, p), v(c.g, p), t(c.g, p);
@@ -505,6 +506,7 @@ SolveStats bicgstab(Context &c, LinOp &A, const Spinor &b, Spinor &x, double tol
       c.blas->axpby(alpha, P, 1.0, X, false);
       rr = std::sqrt(c.blas->axpby(-omega, T, 1.0, R, true));
       st.iters++;
+      if (hook && *hook) (*hook)(st.iters, rr, r0);
     }
     // confirm on the true residual
     rn = residual(c, A, b, x, r.s, st);

@@ -84,7 +84,11 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
 // BiCGStab from x (Frommer et al.; the reference's bicgstab).  Convergence
 // on the recursive residual, then confirmed on the true residual (restarting
 // from it if the two disagree).
-SolveStats bicgstab(Context &c, LinOp &A, const Spinor &b, Spinor &x, double tol, int itermin, int maxiter);
+// Called after every BiCGStab iteration with the iteration count, the
+// recursive residual norm and the initial residual norm.
+typedef std::function<void(int iters, double rnorm, double r0)> IterHook;
+SolveStats bicgstab(Context &c, LinOp &A, const Spinor &b, Spinor &x, double tol, int itermin, int maxiter,
+                    const IterHook *hook = nullptr);
 
 // out = a0 in + a1 (i g5) in, FP64, on one checkerboard
 void local_spin(const Geom &g, const Spinor &in, Spinor &out, double a0, double a1, hipStream_t s);

@@ -173,6 +173,13 @@ module qqcd_hip
          type(c_ptr), value :: he, u, ge, coact, kappa, bc
          integer(c_int), value :: idag, iflag
       end function
+      integer(c_int) function c_solve_log(cap, n, labels, counts, values) bind(C, name="qqcd_solve_log")
+         import :: c_int, c_double, c_char
+         integer(c_int), value :: cap
+         integer(c_int), intent(out) :: n, counts(*)
+         character(kind=c_char), intent(out) :: labels(*)
+         real(c_double), intent(out) :: values(*)
+      end function
       integer(c_int) function c_sweep(gaction, coact, nhit, nsweep, u, iw, jr, kr) bind(C, name="qqcd_sweep")
          import :: c_int, c_ptr
          integer(c_int), intent(in) :: gaction(4)
@@ -281,6 +288,34 @@ contains
          stop
       end if
    end subroutine check_geometry
+
+   ! Append the last solve's log lines to <rwdir>CFGSPROPS.LOG as the reference's solvers
+   ! write them (format (a12,i9,es17.10); qqcd-test/cfgsprops/quark/inverters.f90:3285-3291,
+   ! :476-480), from process 0.
+   subroutine append_log(rwdir, myid)
+      character(len=*), intent(in) :: rwdir(:)
+      integer, intent(in) :: myid
+      integer, parameter :: cap = 100000
+      integer(c_int) :: n, k
+      integer(c_int), allocatable :: counts(:)
+      real(c_double), allocatable :: values(:)
+      character(kind=c_char), allocatable :: labels(:)
+      character(len=12) :: lab
+      integer :: unit, j
+      if (myid /= 0) return
+      allocate(counts(cap), values(cap), labels(12*cap))
+      call fail("INVERTERS", c_solve_log(int(cap, c_int), n, labels, counts, values))
+      if (n == 0) return
+      open(newunit=unit, file=trim(rwdir(myid+1))//"CFGSPROPS.LOG", action="write", form="formatted", &
+           status="unknown", position="append")
+      do k = 1, min(n, cap)
+         do j = 1, 12
+            lab(j:j) = labels(12*(k-1)+j)
+         end do
+         write(unit=unit, fmt="(a12,i9,es17.10)") trim(lab), counts(k), values(k)
+      end do
+      close(unit, status="keep")
+   end subroutine append_log
 
    ! Refresh the resident links after the host changed u in place.
    subroutine qqcd_gauge_changed(u, coact, bc)
@@ -444,6 +479,7 @@ contains
       b = bc(1:4)
       call fail("INVERTERS", c_bicgstab(c_loc(phi), c_loc(x), resmax, itermin, c_loc(itercount), c_loc(u), &
                                         c_loc(coact), c_loc(kappa), iflag, c_loc(b)))
+      call append_log(rwdir, myid)
    end subroutine bicgstab
 
    subroutine gmresdr(rwdir,phi,x,GMRES,resmax,itermin,itercount,u,GeeGooinv,iflag,kappa,coact,bc,vecbl, &
@@ -466,6 +502,7 @@ contains
       gm = GMRES(1:2)
       call fail("INVERTERS", c_gmresdr(c_loc(phi), c_loc(x), c_loc(gm), resmax, itermin, c_loc(itercount), &
                                        c_loc(u), c_loc(coact), c_loc(kappa), iflag, c_loc(b)))
+      call append_log(rwdir, myid)
    end subroutine gmresdr
 
    subroutine psc(rwdir,b,x,GMRES,resmax,itermin,itercount,u,GeeGooinv,iflag,kappa,coact,bc,vecbl, &
@@ -489,6 +526,7 @@ contains
       gm = GMRES(1:2)
       call fail("INVERTERS", c_psc(c_loc(b), c_loc(x), c_loc(gm), resmax, itermin, c_loc(itercount), c_loc(u), &
                                    c_loc(coact), c_loc(kappa), iflag, c_loc(bb), isignal, 0.0_c_double))
+      call append_log(rwdir, myid)
    end subroutine psc
 
    subroutine gmresproject(rwdir,b,xshift,GMRES,resmax,itermin,itercount,u,GeeGooinv,iflag,kappa,coact,bc, &
@@ -654,6 +692,7 @@ contains
       call fail("QUARK", c_fermprop(c_loc(be), c_loc(bo), c_loc(kappa), inverter, c_loc(coact), c_loc(b), resmax, &
                                     itermin, c_loc(ngm), vectype, c_loc(u), c_loc(xe), c_loc(xo), isignal, c_loc(it)))
       if (inverter == 0 .and. vectype == 1) vectype = 0
+      call append_log(rwdir, myid)
    end subroutine fermprop
 
    ! ---- gauge fields ----------------------------------------------------

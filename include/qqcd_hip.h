@@ -195,6 +195,14 @@ int qqcd_solve_mixed(const double *phi, double *x, const int GMRES[2], double to
                      int op_recon, double inner_tol, int keep_deflation, const double *u, const double *coact,
                      const double kappa[2], const int *bc, int *itercount);
 
+/* The log lines of the last gmresdr / psc / bicgstab / fermprop call, as the
+ * reference's solvers append them to CFGSPROPS.LOG with format
+ * (a12,i9,es17.10) (inverters.f90:3285-3291 "gmresdr-norm", matrix-vector
+ * products, ||r||/||r0|| per true residual; :476-480 "bicgstab", iteration,
+ * ||r||^2/||r0||): *n lines, the first cap of them as 12-character blank-
+ * padded labels (not terminated), counts and values. */
+int qqcd_solve_log(int cap, int *n, char *labels, int *counts, double *values);
+
 /* ---- many right-hand sides on one gauge field (disconnected loops) ------ */
 
 /* Z4 noise source {1, -1, i, -i} * scale on every (site, spin, colour) of be

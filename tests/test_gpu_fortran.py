@@ -98,6 +98,12 @@ def test_fortran_host_ops(tmp_path):
     assert rel(x[odd], g["gmresdr_k150_xo"]) < 1e-8
     xp = O.spinor_ref_to_canon(N, load("xe_proj.bin"), None)
     assert rel(xp[~odd], g["gmresdr_k150_xe"]) < 1e-6
+    # CFGSPROPS.LOG: the reference's per-cycle lines of the GMRES-DR solve, (a12,i9,es17.10)
+    log = (tmp_path / "CFGSPROPS.LOG").read_text().splitlines()
+    assert log and all(len(line) == 38 and line[:12].strip() == "gmresdr-norm" for line in log)
+    vals = [float(line[21:]) for line in log]
+    mvps = [int(line[12:21]) for line in log]
+    assert vals[-1] < 1e-10 and all(b > a for a, b in zip(mvps, mvps[1:]))
     # mixed precision on the even-odd system M_hat x = be
     xm = O.spinor_ref_to_canon(N, load("x_mixed.bin"), None)
     bb = O.spinor_ref_to_canon(N, be, None)
