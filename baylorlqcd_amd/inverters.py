@@ -8,6 +8,8 @@ arguments are accepted and ignored (Wilson/twisted, one process).
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -84,3 +86,18 @@ def last_solve_stats():
 
 def deflation_clear():
     _lib.check(_lib.lib().qqcd_deflation_clear())
+
+
+def solve_log():
+    """The last solve's CFGSPROPS.LOG lines as the reference writes them (format (a12,i9,es17.10)):
+    [(label, count, value)] -- "gmresdr-norm" per true residual, "bicgstab" per iteration."""
+    L = _lib.lib()
+    n = ctypes.c_int()
+    _lib.check(L.qqcd_solve_log(0, ctypes.byref(n), None, None, None))
+    cap = n.value
+    lab = ctypes.create_string_buffer(12 * max(cap, 1))
+    cnt = np.zeros(max(cap, 1), dtype=np.int32)
+    val = np.zeros(max(cap, 1))
+    _lib.check(L.qqcd_solve_log(cap, ctypes.byref(n), lab, cnt.ctypes.data_as(_lib._ip), _lib.dp(val)))
+    raw = lab.raw
+    return [(raw[12 * i:12 * i + 12].decode().strip(), int(cnt[i]), float(val[i])) for i in range(cap)]

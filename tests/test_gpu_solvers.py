@@ -187,3 +187,27 @@ def test_solve_16x16x16x32():
         assert res < 1e-10
     finally:
         lattice.finalize()
+
+
+def test_solve_log_lines():
+    # the CFGSPROPS.LOG records of a GMRES-DR solve and of a BiCGStab solve
+    n, bc = (8, 8, 8, 16), (1, 1, 1, -1)
+    lattice.init(n, bc)
+    try:
+        U = gauge.haar(n, 12)
+        u = O.links_canon_to_ref(n, U)
+        co = O.coact_array()
+        rng = np.random.default_rng(4)
+        psi = rng.standard_normal((int(np.prod(n)), 4, 3)) + 1j * rng.standard_normal((int(np.prod(n)), 4, 3))
+        phi, _ = O.spinor_canon_to_ref(n, psi)
+        x = np.zeros_like(phi)
+        it = inverters.gmresdr(["./"], phi, x, (30, 8), 1e-10, 0, u, None, -1, (0.12, 0.0), co, bc)
+        log = inverters.solve_log()
+        assert log and all(l[0] == "gmresdr-norm" for l in log) and log[-1][2] < 1e-10
+        x = np.zeros_like(phi)
+        inverters.bicgstab(["./"], phi, x, 1e-8, 0, u, None, -1, (0.12, 0.0), co, bc)
+        log = inverters.solve_log()
+        assert log and all(l[0] == "bicgstab" for l in log)
+        assert [l[1] for l in log] == list(range(1, len(log) + 1))
+    finally:
+        lattice.finalize()
