@@ -27,7 +27,7 @@ void lag_noise_source(const Geom &g, Spinor &out, int par, const uint32_t *d_dra
                       hipStream_t s);
 // the rnds scale factor: the single-precision literal 0.465661e-9 held in a double
 constexpr double LAG_FNORM = (double)0.465661e-9f;
-// kernel time (ms) of generating m draws (at most 2048 chunks of one launch)
+// kernel time (ms) of generating m draws (at most 4096 chunks of one launch)
 double bench_lagfib(long m, int niter, hipStream_t s);
 // drop the cached jump-ahead tables (device memory)
 void lagfib_release_device();

@@ -9,11 +9,15 @@
 // P(x) = x^521 + x^489 + 1: y_t = XOR_k a_k y_{k-521} with
 // sum_k a_k x^k = x^(t+521) mod P.
 //
-// Device generation of m draws: chunks of L draws; the 521 words before
-// chunk c are the jump-ahead of the base window by the rows x^(cL+j) mod P,
-// j = 0..520 (a 521 x 521 bit matrix per chunk, built once per L on the
-// host and kept on the device); then one half-wave runs the recurrence 32
-// draws at a time through an LDS ring.  A zero draw (probability 2^-31) is
+// Device generation of m draws: chunks of L draws (8192 2^k, at most 2048
+// per launch).  Because every shift of the sequence obeys the recurrence,
+// x^s mod P = sum_k a_k x^k gives y_{s+t} = XOR_k a_k y_{k+t} for every t, so
+// the 521 words before chunk c (s = cL) are one polynomial (17 words, built
+// once per L on the host) applied to the first 1041 terms of the stream (the
+// base window and the next 520 terms, extended in LDS); then one wave per
+// chunk runs the recurrence 64 draws a step through an LDS ring (lanes 32-63
+// use y_t = y_{t-521} ^ y_{t-553} ^ y_{t-64}, so every operand comes from an
+// earlier step).  A zero draw (probability 2^-31) is
 // the one non-linear step: the reference XORs again, so y_t becomes
 // y_{t-32}.  The first zero is found with an atomic minimum, corrected, and
 // the stream after it generated again from the corrected window.
@@ -143,6 +147,7 @@ struct LagTable {
 };
 static std::vector<LagTable> g_tables;
 
+// x^(cL) mod P for c < K, 17 words each, on the device
 static const uint32_t *table_for(long L, int K, hipStream_t s) {
   LagTable *t = nullptr;
   for (auto &e : g_tables)
@@ -154,15 +159,11 @@ static const uint32_t *table_for(long L, int K, hipStream_t s) {
   }
   if (t->K >= K) return t->d;
   const int K0 = t->K;
-  t->h.resize((size_t)K * LP * LW);
+  t->h.resize((size_t)K * LW);
   const Poly xL = xpow(L);
   Poly p = xpow((long)K0 * L);
   for (int c = K0; c < K; c++) {
-    Poly r = p;
-    for (int j = 0; j < LP; j++) {
-      std::memcpy(&t->h[((size_t)c * LP + j) * LW], r.data(), sizeof(uint32_t) * LW);
-      mulx(r);
-    }
+    std::memcpy(&t->h[(size_t)c * LW], p.data(), sizeof(uint32_t) * LW);
     p = mulmod(p, xL);
   }
   if (t->d) {
@@ -185,50 +186,67 @@ void lagfib_release_device() {
 
 // ---- device generation -----------------------------------------------------
 
-// Workgroup of one wave: lanes 0-31 run chunk 2b, lanes 32-63 chunk 2b+1.
-// Ring slot of draw t of chunk c: (t - cL + 521) mod 1024; slots 0..520 hold
-// the window y_{cL-521} .. y_{cL-1}.
-__global__ void __launch_bounds__(64) k_lag_gen(const uint32_t *base, const uint32_t *rows, long L, long m, int K,
+// Workgroup of one wave per chunk c = blockIdx.x.  Ring slot of draw t:
+// (t - cL + 521) mod 1024; slots 0..520 hold the window y_{cL-521} ..
+// y_{cL-1}.  LDS ordering between steps needs only wavefront scope: the
+// wave's LDS operations complete in order.
+__device__ __forceinline__ void lag_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(64) k_lag_gen(const uint32_t *base, const uint32_t *polys, long L, long m, int K,
                                                 uint32_t *out, unsigned long long *first_zero) {
-  __shared__ uint32_t sb[LW * 32];
-  __shared__ uint32_t ring[2][RING];
-  const int lane = threadIdx.x, h = lane >> 5, j = lane & 31;
-  const int c = 2 * blockIdx.x + h;
-  for (int k = lane; k < LW * 32; k += 64) sb[k] = k < LP ? base[k] : 0u;
+  // y[k] = y_{k-521}, k = 0..1040: the base window, then 520 terms by the recurrence
+  __shared__ uint32_t y[2 * LP + 31];
+  __shared__ uint32_t ring[RING];
+  const int lane = threadIdx.x;
+  const int c = blockIdx.x;
+  for (int k = lane; k < LP; k += 64) y[k] = base[k];
   __syncthreads();
-  if (c < K) {
-    for (int r = j; r < LP; r += 32) {
-      const uint32_t *row = rows + ((long)c * LP + r) * LW;
-      uint32_t v = 0;
-      for (int w = 0; w < LW; w++) {
-        uint32_t bits = row[w];
-        while (bits) {
-          const int b = __builtin_ctz(bits);
-          bits &= bits - 1;
-          v ^= sb[32 * w + b];
+  for (int k0 = LP; k0 < 2 * LP - 1; k0 += 32) {
+    const int k = k0 + lane;
+    if (lane < 32 && k < 2 * LP - 1) y[k] = y[k - LP] ^ y[k - LQ];
+    lag_wave_sync();
+  }
+  // window: w_j = XOR over the set bits k of x^(cL) mod P of y[k + j]
+  {
+    const uint32_t *poly = polys + (long)c * LW;
+    uint32_t v[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) v[q] = 0;
+    for (int w = 0; w < LW; w++) {
+      uint32_t bits = poly[w];  // the same word for the whole wave
+      while (bits) {
+        const int k = 32 * w + __builtin_ctz(bits);
+        bits &= bits - 1;
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+          const int jj = lane + 64 * q;
+          if (jj < LP) v[q] ^= y[k + jj];
         }
       }
-      ring[h][r] = v;
     }
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+      if (lane + 64 * q < LP) ring[lane + 64 * q] = v[q];
   }
   __syncthreads();
-  // both halves run the same number of steps (the barriers are shared)
-  long len[2];
-  for (int q = 0; q < 2; q++) {
-    const long cq = 2L * blockIdx.x + q;
-    len[q] = cq < K ? std::min(L, m - cq * L) : 0;
-  }
-  const long steps = (std::max(len[0], len[1]) + 31) / 32;
+  const long len = std::min(L, m - (long)c * L);
   const long t0 = (long)c * L;
+  const long steps = (len + 63) / 64;
   for (long st = 0; st < steps; st++) {
-    const long u = LP + 32 * st + j;
-    if (32 * st + j < len[h]) {
-      const uint32_t y = ring[h][(u - LP) & (RING - 1)] ^ ring[h][(u - LQ) & (RING - 1)];
-      ring[h][u & (RING - 1)] = y;
-      out[t0 + 32 * st + j] = y;
-      if (y == 0u) atomicMin(first_zero, (unsigned long long)(t0 + 32 * st + j));
+    const long u = LP + 64 * st + lane;
+    if (64 * st + lane < len) {
+      uint32_t yy = ring[(u - LP) & (RING - 1)];
+      if (lane < 32) yy ^= ring[(u - LQ) & (RING - 1)];
+      else yy ^= ring[(u - LP - LQ) & (RING - 1)] ^ ring[(u - 2 * LQ) & (RING - 1)];
+      ring[u & (RING - 1)] = yy;
+      out[t0 + 64 * st + lane] = yy;
+      if (yy == 0u) atomicMin(first_zero, (unsigned long long)(t0 + 64 * st + lane));
     }
-    __syncthreads();
+    lag_wave_sync();
   }
 }
 
@@ -274,12 +292,12 @@ void lagfib_generate(LagState &st, long m, uint32_t *d_out, hipStream_t s) {
     long left = m;
     while (left > 0) {
       long L = 8192;
-      while ((left + L - 1) / L > KMAX && L < (1L << 21)) L *= 4;
+      while ((left + L - 1) / L > KMAX && L < (1L << 21)) L *= 2;
       const long piece = std::min(left, (long)KMAX * L);
       const int K = (int)((piece + L - 1) / L);
       const uint32_t *rows = table_for(L, K, s);
       QQCD_HIP(hipMemsetAsync(dz, 0xff, sizeof(unsigned long long), s));
-      hipLaunchKernelGGL(k_lag_gen, dim3((K + 1) / 2), dim3(64), 0, s, cur, rows, L, piece, K, o, dz);
+      hipLaunchKernelGGL(k_lag_gen, dim3(K), dim3(64), 0, s, cur, rows, L, piece, K, o, dz);
       QQCD_HIP(hipGetLastError());
       unsigned long long z = 0;
       QQCD_HIP(hipMemcpyAsync(&z, dz, sizeof(z), hipMemcpyDeviceToHost, s));
@@ -340,7 +358,7 @@ double bench_lagfib(long m, int niter, hipStream_t s) {
   uint32_t hb[LP];
   for (int k = 0; k < LP; k++) hb[k] = st.w[(st.jr + k) % LP];
   long L = 8192;
-  while ((m + L - 1) / L > KMAX && L < (1L << 21)) L *= 4;
+  while ((m + L - 1) / L > KMAX && L < (1L << 21)) L *= 2;
   m = std::min(m, (long)KMAX * L);
   const int K = (int)((m + L - 1) / L);
   const uint32_t *rows = table_for(L, K, s);
@@ -354,10 +372,10 @@ double bench_lagfib(long m, int niter, hipStream_t s) {
   hipEvent_t e0, e1;
   QQCD_HIP(hipEventCreate(&e0));
   QQCD_HIP(hipEventCreate(&e1));
-  hipLaunchKernelGGL(k_lag_gen, dim3((K + 1) / 2), dim3(64), 0, s, dbase, rows, L, m, K, out, dz);
+  hipLaunchKernelGGL(k_lag_gen, dim3(K), dim3(64), 0, s, dbase, rows, L, m, K, out, dz);
   QQCD_HIP(hipEventRecord(e0, s));
   for (int it = 0; it < niter; it++)
-    hipLaunchKernelGGL(k_lag_gen, dim3((K + 1) / 2), dim3(64), 0, s, dbase, rows, L, m, K, out, dz);
+    hipLaunchKernelGGL(k_lag_gen, dim3(K), dim3(64), 0, s, dbase, rows, L, m, K, out, dz);
   QQCD_HIP(hipEventRecord(e1, s));
   QQCD_HIP(hipEventSynchronize(e1));
   float ms = 0;
