@@ -266,8 +266,11 @@ __device__ __forceinline__ void hb_rows(int isu2, int &P, int &Q, double &s3) {
 // phbsub1-3: z of the link's rows (p, q) against the staple; dk = 1/|z|; the
 // site starts the Kennedy-Pendleton passes not accepted
 __global__ void __launch_bounds__(HB_TB) k_hb_proj(HbG G, const double *u, const double *C, int ieo, int ibl,
-                                                   int mu, int isu2, double *Z, double *DK, int *ID, double *A0) {
+                                                   int mu, int isu2, double *Z, double *DK, int *ID, double *A0,
+                                                   int *cnt) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // the first pass draws for every site of the workgroup
+  if (threadIdx.x == 0) cnt[blockIdx.x] = (int)min((long)HB_TB, G.nvh - (long)blockIdx.x * HB_TB);
   if (i >= G.nvh) return;
   int P, Q;
   double s3;
@@ -315,26 +318,19 @@ __global__ void __launch_bounds__(HB_TB) k_hb_proj(HbG G, const double *u, const
   A0[i] = 0.0;
 }
 
-// sites not yet accepted, per workgroup
-__global__ void __launch_bounds__(HB_TB) k_hb_count(long nvh, const int *ID, int *cnt) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int f = (i < nvh && ID[i] == 0) ? 1 : 0;
-  const unsigned long long bal = __ballot(f);
-  __shared__ int ws[HB_TB / 64];
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = __popcll(bal);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int s = 0;
-    for (int w = 0; w < HB_TB / 64; w++) s += ws[w];
-    cnt[blockIdx.x] = s;
-  }
-}
+// status of a batch of passes, read back with one copy: the running stream
+// position and, per pass, the sites that drew and the sites that accepted
+struct HbStatus {
+  long pos;
+  long tot[HB_PB];
+  int acc[HB_PB];
+};
 
 // one workgroup: exclusive scan of the counts; the pass draws 4 * total
 // numbers from base (the host's position for the first pass of a batch, the
 // device's running position afterwards)
-__global__ void __launch_bounds__(1024) k_hb_scan(int nblk, const int *cnt, int *off, long *dpos, long base_host,
-                                                 int use_host, long *dbase, long *dtot, int pass) {
+__global__ void __launch_bounds__(1024) k_hb_scan(int nblk, const int *cnt, int *off, HbStatus *stt, long base_host,
+                                                 int use_host, long *dbase, int pass) {
   __shared__ long part[1024];
   const int t = threadIdx.x;
   const int per = (nblk + 1023) / 1024;
@@ -356,10 +352,11 @@ __global__ void __launch_bounds__(1024) k_hb_scan(int nblk, const int *cnt, int 
   }
   if (t == 1023) {
     const long total = part[1023];
-    const long base = use_host ? base_host : *dpos;
+    const long base = use_host ? base_host : stt->pos;
     dbase[pass] = base;
-    dtot[pass] = total;
-    *dpos = base + 4 * total;
+    stt->tot[pass] = total;
+    stt->acc[pass] = 0;
+    stt->pos = base + 4 * total;
   }
 }
 
@@ -368,7 +365,7 @@ __device__ __forceinline__ double hb_rn(const uint32_t *y, long j) { return (dou
 // one Kennedy-Pendleton pass (su2up :270-297) over the sites not yet accepted
 __global__ void __launch_bounds__(HB_TB) k_hb_try(long nvh, const uint32_t *y, int *ID, const double *DK,
                                                   double *A0, const int *off, const long *dbase, int pass,
-                                                  int *dacc) {
+                                                  HbStatus *stt, int *cnt) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int f = (i < nvh && ID[i] == 0) ? 1 : 0;
   const unsigned long long bal = __ballot(f);
@@ -396,7 +393,12 @@ __global__ void __launch_bounds__(HB_TB) k_hb_try(long nvh, const uint32_t *y, i
   }
   if (acc) atomicAdd(&acc_s, 1);
   __syncthreads();
-  if (threadIdx.x == 0 && acc_s) atomicAdd(&dacc[pass], acc_s);
+  if (threadIdx.x == 0) {
+    int drew = 0;
+    for (int w = 0; w < HB_TB / 64; w++) drew += ws[w];
+    cnt[blockIdx.x] = drew - acc_s;  // the next pass draws for the sites still not accepted
+    if (acc_s) atomicAdd(&stt->acc[pass], acc_s);
+  }
 }
 
 // su2up's sphere (:300-321) and sprd1 (:327-467): rows (p, q) of the link
@@ -530,14 +532,19 @@ void heatbath_reference(Context &c, const int gaction[4], const double *coact, i
   const size_t ulen = 18 * (size_t)G.ntot * 128;
   const int nblk = (int)((G.nvh + HB_TB - 1) / HB_TB);
   DBuf<double> du(ulen), C(18 * G.nvh), Z(4 * G.nvh), DK(G.nvh), A0(G.nvh);
-  DBuf<int> ID(G.nvh), cnt(nblk), off(nblk), dacc(HB_PB);
-  DBuf<long> dpos(1), dbase(HB_PB), dtot(HB_PB);
+  DBuf<int> ID(G.nvh), cnt(nblk), off(nblk);
+  DBuf<long> dbase(HB_PB);
+  DBuf<HbStatus> dst(1);
+  HbStatus *hst = nullptr;  // pinned: the one status copy per batch
+  QQCD_HIP(hipHostMalloc((void **)&hst, sizeof(HbStatus)));
+  struct PinnedFree {
+    HbStatus *p;
+    ~PinnedFree() { (void)hipHostFree(p); }
+  } pinned_free{hst};
   QQCD_HIP(hipMemcpyAsync(du.p, u_host, sizeof(double) * ulen, hipMemcpyHostToDevice, c.stream));
   const long need_pass = 4 * G.nvh * HB_PB, chunk = std::max(8 * need_pass, 1L << 20);
   HbStream S(st, 521 + chunk + 2 * need_pass + 2 * G.nvh, c.stream);
   long pos = 0;
-  std::vector<long> htot(HB_PB);
-  std::vector<int> hacc(HB_PB);
   const dim3 grid(nblk), blk(HB_TB);
   for (int sw = 0; sw < nsweep; sw++)
     for (int ieo = 0; ieo < 2; ieo++)
@@ -554,29 +561,23 @@ void heatbath_reference(Context &c, const int gaction[4], const double *coact, i
           QQCD_HIP(hipGetLastError());
           for (int isu2 = 1; isu2 <= 3; isu2++) {
             hipLaunchKernelGGL(k_hb_proj, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, isu2, Z.p, DK.p,
-                               ID.p, A0.p);
+                               ID.p, A0.p, cnt.p);
             QQCD_HIP(hipGetLastError());
             for (int batch = 0;; batch++) {
               // a site whose staple projects to zero never accepts (the reference would loop forever)
               if (batch == 200) throw std::runtime_error("sweep: Kennedy-Pendleton passes did not end (zero staple?)");
               S.ensure(pos, need_pass, chunk);
-              QQCD_HIP(hipMemsetAsync(dacc.p, 0, sizeof(int) * HB_PB, c.stream));
-              QQCD_HIP(hipMemsetAsync(dtot.p, 0, sizeof(long) * HB_PB, c.stream));
               for (int p = 0; p < HB_PB; p++) {
-                hipLaunchKernelGGL(k_hb_count, grid, blk, 0, c.stream, G.nvh, ID.p, cnt.p);
-                hipLaunchKernelGGL(k_hb_scan, dim3(1), dim3(1024), 0, c.stream, nblk, cnt.p, off.p, dpos.p,
-                                   pos - S.origin, p == 0 ? 1 : 0, dbase.p, dtot.p, p);
+                hipLaunchKernelGGL(k_hb_scan, dim3(1), dim3(1024), 0, c.stream, nblk, cnt.p, off.p, dst.p,
+                                   pos - S.origin, p == 0 ? 1 : 0, dbase.p, p);
                 hipLaunchKernelGGL(k_hb_try, grid, blk, 0, c.stream, G.nvh, S.buf, ID.p, DK.p, A0.p, off.p,
-                                   dbase.p, p, dacc.p);
+                                   dbase.p, p, dst.p, cnt.p);
                 QQCD_HIP(hipGetLastError());
               }
-              long rel = 0;
-              QQCD_HIP(hipMemcpyAsync(htot.data(), dtot.p, sizeof(long) * HB_PB, hipMemcpyDeviceToHost, c.stream));
-              QQCD_HIP(hipMemcpyAsync(hacc.data(), dacc.p, sizeof(int) * HB_PB, hipMemcpyDeviceToHost, c.stream));
-              QQCD_HIP(hipMemcpyAsync(&rel, dpos.p, sizeof(long), hipMemcpyDeviceToHost, c.stream));
+              QQCD_HIP(hipMemcpyAsync(hst, dst.p, sizeof(HbStatus), hipMemcpyDeviceToHost, c.stream));
               QQCD_HIP(hipStreamSynchronize(c.stream));
-              pos = S.origin + rel;
-              const long rem = htot[HB_PB - 1] > 0 ? htot[HB_PB - 1] - hacc[HB_PB - 1] : 0;
+              pos = S.origin + hst->pos;
+              const long rem = hst->tot[HB_PB - 1] > 0 ? hst->tot[HB_PB - 1] - hst->acc[HB_PB - 1] : 0;
               if (rem == 0) break;
             }
             S.ensure(pos, 2 * G.nvh, chunk);
