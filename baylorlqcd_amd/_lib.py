@@ -69,6 +69,9 @@ SIGNATURES = {
     "qqcd_noise_loop": [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, _dp, _dp, _ip, _dp, _ip,
                         ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _ip, _dp,
                         _ip],
+    "qqcd_noise_loop_lagfib": [ctypes.c_int, _ip, _ip, _ip, ctypes.c_double, _dp, _dp, _ip, _dp, _ip,
+                               ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _ip,
+                               _dp, _ip],
     "qqcd_loopops": [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp],
     "qqcd_lagfib_init": [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip],
     "qqcd_rnds": [ctypes.c_long, _dp, _ip, _ip, _ip],

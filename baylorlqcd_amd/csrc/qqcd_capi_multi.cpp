@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -437,12 +438,76 @@ extern "C" int qqcd_fermprop_multi(int nrhs, const double *const *be, const doub
 // the Krylov steps spent on source r, times = {first solve, batched solves,
 // whole call} in seconds; stats = {outer, cycles, batched matvecs, graph
 // replays, fallback, nb, order, first-solve outer steps}.
+// qqcd_noise_loop_lagfib: the same loop with source r the Z4 noise z4source
+// draws on its r-th call from the lagged-Fibonacci state (iw, jr, kr)
+// (384 nvhalf draws per source, in order), the state advanced past all
+// nrhs sources.
+namespace {
+
+// source r of the loop into (even, odd) checkerboards
+typedef std::function<void(long r, Spinor &e, Spinor &o)> SourceGen;
+
+void noise_loop_impl(Context &c, int nrhs, const SourceGen &gen, double scale, const double kappa[2],
+                     const double *coact, const int *bc, const double *u, const int GMRES[2], double tol, int nb,
+                     int method, double *traces, int ksub, const int *io, double *loops, double *res_full,
+                     int *iters, double *times, int *stats);
+
+}  // namespace
+
 extern "C" int qqcd_noise_loop(int nrhs, unsigned long long seed, double scale, const double kappa[2],
                                const double *coact, const int *bc, const double *u, const int GMRES[2], double tol,
                                int nb, int method, double *traces, int ksub, const int *io, double *loops,
                                double *res_full, int *iters, double *times, int *stats) {
   API_BEGIN
   Context &c = ctx();
+  const SourceGen gen = [&](long r, Spinor &e, Spinor &o) {
+    z4_source(c.g, e, 0, seed, r, scale, c.stream);
+    z4_source(c.g, o, 1, seed, r, scale, c.stream);
+  };
+  noise_loop_impl(c, nrhs, gen, scale, kappa, coact, bc, u, GMRES, tol, nb, method, traces, ksub, io, loops,
+                  res_full, iters, times, stats);
+  API_END
+}
+
+extern "C" int qqcd_noise_loop_lagfib(int nrhs, int *iw, int *jr, int *kr, double scale, const double kappa[2],
+                                      const double *coact, const int *bc, const double *u, const int GMRES[2],
+                                      double tol, int nb, int method, double *traces, int ksub, const int *io,
+                                      double *loops, double *res_full, int *iters, double *times, int *stats) {
+  API_BEGIN
+  Context &c = ctx();
+  // source r = the draws z4source would make on its r-th call from this state:
+  // the stream state at the start of every source is kept, a source is drawn
+  // again from its state when the loop needs it a second time
+  LagState cur = lag_in(iw, jr, kr);
+  const long nvh = c.g.V / 32;
+  DevU32 y(384 * nvh);
+  std::vector<LagState> at;
+  const SourceGen gen = [&](long r, Spinor &e, Spinor &o) {
+    if (r == (long)at.size()) {
+      at.push_back(cur);
+      lagfib_generate(cur, 384 * nvh, y.p, c.stream);
+    } else if (r < (long)at.size()) {
+      LagState t = at[r];
+      lagfib_generate(t, 384 * nvh, y.p, c.stream);
+    } else {
+      throw std::runtime_error("noise_loop_lagfib: sources must first be drawn in order");
+    }
+    lag_noise_source(c.g, e, 0, y.p, 0, scale, c.stream);
+    lag_noise_source(c.g, o, 1, y.p + 192 * nvh, 0, scale, c.stream);
+  };
+  noise_loop_impl(c, nrhs, gen, scale, kappa, coact, bc, u, GMRES, tol, nb, method, traces, ksub, io, loops,
+                  res_full, iters, times, stats);
+  if ((long)at.size() != nrhs) throw std::runtime_error("noise_loop_lagfib: not every source was drawn");
+  lag_out(cur, iw, jr, kr);
+  API_END
+}
+
+namespace {
+
+void noise_loop_impl(Context &c, int nrhs, const SourceGen &gen, double scale, const double kappa[2],
+                     const double *coact, const int *bc, const double *u, const int GMRES[2], double tol, int nb,
+                     int method, double *traces, int ksub, const int *io, double *loops, double *res_full,
+                     int *iters, double *times, int *stats) {
   if (nrhs < 1 || nrhs > 257) throw std::runtime_error("noise_loop: 1..257 right-hand sides");
   if (GMRES[1] < 1 || GMRES[1] >= GMRES[0] || GMRES[0] > 120) throw std::runtime_error("GMRES sizes need 0 < k < m <= 120");
   if (!(scale > 0)) throw std::runtime_error("noise_loop: scale must be positive");
@@ -464,10 +529,7 @@ extern "C" int qqcd_noise_loop(int nrhs, unsigned long long seed, double scale, 
     lo.reset(new DevArr(nullptr, (size_t)nt * 90, c.stream));
     lpart.reset(new DevArr(nullptr, (size_t)2 * nt * loopsum_gx(c.g) * 58, c.stream));
   }
-  auto noise = [&](long r) {
-    z4_source(c.g, e_e.s, 0, seed, r, scale, c.stream);
-    z4_source(c.g, e_o.s, 1, seed, r, scale, c.stream);
-  };
+  auto noise = [&](long r) { gen(r, e_e.s, e_o.s); };
   auto finish = [&](long r, int first) {
     // x_o, the whole-lattice residual, and the traces of source r
     reconstruct_odd(c, kappa, e_o.s, x_e.s, x_o.s);
@@ -564,8 +626,9 @@ extern "C" int qqcd_noise_loop(int nrhs, unsigned long long seed, double scale, 
   times[0] = std::chrono::duration<double>(t1 - t0).count();
   times[1] = std::chrono::duration<double>(t2 - t1).count();
   times[2] = std::chrono::duration<double>(t2 - t0).count();
-  API_END
 }
+
+}  // namespace
 
 // The full propagator of one source site: nrhs (<= 12) point sources
 // (spin r/3 + 1, colour r%3 + 1, pointsource's numbering) solved on the

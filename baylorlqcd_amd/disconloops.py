@@ -96,13 +96,16 @@ def fermprop_multi(be, bo, kappa, coact, bc, resmax, GMRES, u):
 
 
 def noise_loop(nrhs, seed, kappa, GMRES, tol=1e-10, scale=1.0, coact=None, bc=(1, 1, 1, -1), u=None, nb=0,
-               method=0, loops=True, ksub=0, io=None):
+               method=0, loops=True, ksub=0, io=None, lagfib=None):
     """The device noise loop.  Returns a dict with traces ((nt, 16) complex),
     loops ((nt, 5, 9) complex: the reference's operators averaged over the
     sources, currents times kappa; None when loops is False), res_full, iters,
     times (first, rest, total) and stats.  method 0: batched; 1: sequential
     plain FP64 (GMRES-DR then GMRES-Proj per source); 2: sequential
-    single-RHS mixed precision."""
+    single-RHS mixed precision.  lagfib (a LagFib state): the sources are the
+    ones its z4source would return on nrhs successive calls (seed unused), and
+    the state is advanced past them, as the reference's per-source loop
+    (disconloops.f90:3032-3053) leaves it."""
     L = _lib.lib()
     nt = _nt()
     tr = np.zeros(nt * 32)
@@ -113,11 +116,14 @@ def noise_loop(nrhs, seed, kappa, GMRES, tol=1e-10, scale=1.0, coact=None, bc=(1
     times = np.zeros(3)
     stats = np.zeros(8, np.int32)
     co = None if coact is None else np.ascontiguousarray(coact, dtype=np.float64).ravel(order="F")
-    _lib.check(L.qqcd_noise_loop(
-        int(nrhs), int(seed), float(scale), _lib.dp(np.ascontiguousarray(kappa, dtype=np.float64)), _lib.dp(co),
-        _lib.ints(bc).ctypes.data_as(_lib._ip), _lib.dp(u), _lib.ints(GMRES).ctypes.data_as(_lib._ip), float(tol),
-        int(nb), int(method), _lib.dp(tr), int(ksub), iop, _lib.dp(lo), _lib.dp(res), it.ctypes.data_as(_lib._ip),
-        _lib.dp(times), stats.ctypes.data_as(_lib._ip)))
+    rest = (_lib.dp(np.ascontiguousarray(kappa, dtype=np.float64)), _lib.dp(co),
+            _lib.ints(bc).ctypes.data_as(_lib._ip), _lib.dp(u), _lib.ints(GMRES).ctypes.data_as(_lib._ip),
+            float(tol), int(nb), int(method), _lib.dp(tr), int(ksub), iop, _lib.dp(lo), _lib.dp(res),
+            it.ctypes.data_as(_lib._ip), _lib.dp(times), stats.ctypes.data_as(_lib._ip))
+    if lagfib is None:
+        _lib.check(L.qqcd_noise_loop(int(nrhs), int(seed), float(scale), *rest))
+    else:
+        _lib.check(L.qqcd_noise_loop_lagfib(int(nrhs), *lagfib._args(), float(scale), *rest))
     keys = ("outer", "cycles", "batched_matvecs", "graph_replays", "fallback", "nb", "order", "first_outer")
     return {"traces": tr.reshape(nt, 16, 2) @ np.array([1.0, 1j]),
             "loops": None if lo is None else lo.reshape(nt, 5, 9, 2) @ np.array([1.0, 1j]),

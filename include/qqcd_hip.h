@@ -263,6 +263,19 @@ int qqcd_noise_loop(int nrhs, unsigned long long seed, double scale, const doubl
                     double *traces, int ksub, const int *io, double *loops, double *res_full, int *iters,
                     double *times, int *stats);
 
+/* qqcd_noise_loop with the sources of the reference's stream, replacing the
+ * per-source loop "call z4source; call fermprop" of nsaverage
+ * (qqcd-test/cfgsprops/quark/disconloops.f90:3032-3053): source r is the Z4
+ * noise z4source (disconloops.f90:4359-4571, drawing through rnds,
+ * lagfib.f90:219-259) makes on its r-th call from the state (iw[521], jr,
+ * kr) (384 nvhalf draws each, even checkerboard first), times scale.  The
+ * state is advanced past the nrhs sources.  Other arguments and outputs as
+ * qqcd_noise_loop. */
+int qqcd_noise_loop_lagfib(int nrhs, int *iw, int *jr, int *kr, double scale, const double kappa[2],
+                           const double *coact, const int *bc, const double *u, const int GMRES[2], double tol,
+                           int nb, int method, double *traces, int ksub, const int *io, double *loops,
+                           double *res_full, int *iters, double *times, int *stats);
+
 /* The reference's loop operators of one (sink x, source b) pair, replacing
  * loopops (qqcd-test/cfgsprops/quark/disconloops.f90:7769-7910) called over
  * every (ieo, ibleo) slice: xe/xo the sink, be/bo the source (reference
