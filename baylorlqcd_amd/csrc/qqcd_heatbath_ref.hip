@@ -21,7 +21,7 @@
 //   (2 nvhalf draws) place (a1, a2, a3) on the sphere; sprd1 (:327-467)
 //   multiplies rows p, q of u by the new SU(2) element.
 // The draw of a site in a pass is its rank among the sites not yet accepted
-// (a block scan), so the stream is consumed exactly as the reference
+// (each workgroup adds the counts of the workgroups before it), so the stream is consumed exactly as the reference
 // consumes it and the stream state after a sweep is the reference's.
 // Every sum and product is evaluated in the reference's written order
 // without fused multiply-adds (mm, mmd, mdm: gaugetools.f90:741-817,
@@ -44,7 +44,7 @@
 namespace qqcd {
 
 static constexpr int HB_TB = 256;
-static constexpr int HB_PB = 6;  // Kennedy-Pendleton passes per batch between host checks
+static constexpr int HB_PB = 6;  // Kennedy-Pendleton passes per batch between host checks (even)
 // su2up's twoPi: a single-precision literal held in a double
 static constexpr double HB_TWOPI = (double)6.283185307179586f;
 
@@ -319,69 +319,65 @@ __global__ void __launch_bounds__(HB_TB) k_hb_proj(HbG G, const double *u, const
 }
 
 // status of a batch of passes, read back with one copy: the running stream
-// position and, per pass, the sites that drew and the sites that accepted
+// position, per pass the sites that drew, and the sites that accepted in the
+// batch's last pass
 struct HbStatus {
   long pos;
   long tot[HB_PB];
-  int acc[HB_PB];
+  int acc_last;
 };
-
-// one workgroup: exclusive scan of the counts; the pass draws 4 * total
-// numbers from base (the host's position for the first pass of a batch, the
-// device's running position afterwards)
-__global__ void __launch_bounds__(1024) k_hb_scan(int nblk, const int *cnt, int *off, HbStatus *stt, long base_host,
-                                                 int use_host, long *dbase, int pass) {
-  __shared__ long part[1024];
-  const int t = threadIdx.x;
-  const int per = (nblk + 1023) / 1024;
-  const int b0 = t * per, b1 = min(nblk, b0 + per);
-  long s = 0;
-  for (int b = b0; b < b1; b++) s += cnt[b];
-  part[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const long v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  long run = part[t] - s;  // exclusive
-  for (int b = b0; b < b1; b++) {
-    off[b] = (int)run;
-    run += cnt[b];
-  }
-  if (t == 1023) {
-    const long total = part[1023];
-    const long base = use_host ? base_host : stt->pos;
-    dbase[pass] = base;
-    stt->tot[pass] = total;
-    stt->acc[pass] = 0;
-    stt->pos = base + 4 * total;
-  }
-}
+static_assert(HB_PB > 1 && HB_PB % 2 == 0, "a batch starts on the counts the previous one left in cnt[0, nblk)");
 
 __device__ __forceinline__ double hb_rn(const uint32_t *y, long j) { return (double)(int)y[j] * LAG_FNORM; }
 
-// one Kennedy-Pendleton pass (su2up :270-297) over the sites not yet accepted
-__global__ void __launch_bounds__(HB_TB) k_hb_try(long nvh, const uint32_t *y, int *ID, const double *DK,
-                                                  double *A0, const int *off, const long *dbase, int pass,
-                                                  HbStatus *stt, int *cnt) {
+// one Kennedy-Pendleton pass (su2up :270-297) over the sites not yet
+// accepted.  cnt_in[b]: the sites of workgroup b that draw in this pass.
+// The pass draws 4 numbers per such site, in site order, from base (the
+// host's position for the first pass of a batch, where the previous pass
+// ended afterwards, dpos[pass & 1]); each workgroup sums the counts of the
+// workgroups before it itself (nblk reads from L2), so no scan launch and
+// no wait on other workgroups.  cnt_out: the next pass's counts; workgroup
+// 0 records where this pass ends in dpos[(pass + 1) & 1] and the status.
+__global__ void __launch_bounds__(HB_TB) k_hb_try(long nvh, int nblk, const uint32_t *y, int *ID, const double *DK,
+                                                  double *A0, const int *cnt_in, int *cnt_out, long *dpos,
+                                                  long base_host, int pass, HbStatus *stt) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ int red_lo[HB_TB / 64], red_all[HB_TB / 64], ws[HB_TB / 64];
+  __shared__ int acc_s;
+  int lo = 0, all = 0;
+  for (int b = threadIdx.x; b < nblk; b += HB_TB) {
+    const int v = cnt_in[b];
+    all += v;
+    lo += b < (int)blockIdx.x ? v : 0;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo += __shfl_xor(lo, o);
+    all += __shfl_xor(all, o);
+  }
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int f = (i < nvh && ID[i] == 0) ? 1 : 0;
   const unsigned long long bal = __ballot(f);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  __shared__ int ws[HB_TB / 64];
-  __shared__ int acc_s;
-  if (lane == 0) ws[wv] = __popcll(bal);
+  if (lane == 0) {
+    red_lo[wv] = lo;
+    red_all[wv] = all;
+    ws[wv] = __popcll(bal);
+  }
   if (threadIdx.x == 0) acc_s = 0;
   __syncthreads();
+  long off = 0, total = 0;
   int before = 0;
-  for (int w = 0; w < wv; w++) before += ws[w];
+  for (int w = 0; w < HB_TB / 64; w++) {
+    off += red_lo[w];
+    total += red_all[w];
+    before += w < wv ? ws[w] : 0;
+  }
+  const long base = pass == 0 ? base_host : dpos[pass & 1];
   const unsigned long long lower = lane ? (bal & ((~0ull) >> (64 - lane))) : 0ull;
-  const long rank = (long)off[blockIdx.x] + before + __popcll(lower);
+  const long rank = off + before + __popcll(lower);
   int acc = 0;
   if (f) {
-    const long j = dbase[pass] + 4 * rank;
+    const long j = base + 4 * rank;
     const double r0 = hb_rn(y, j), r1 = hb_rn(y, j + 1), r2 = hb_rn(y, j + 2), r3 = hb_rn(y, j + 3);
     const double tt = cos(HB_TWOPI * r2);
     const double d2 = -(log(r0) + log(r1) * tt * tt) * DK[i] * 3.0;
@@ -396,8 +392,14 @@ __global__ void __launch_bounds__(HB_TB) k_hb_try(long nvh, const uint32_t *y, i
   if (threadIdx.x == 0) {
     int drew = 0;
     for (int w = 0; w < HB_TB / 64; w++) drew += ws[w];
-    cnt[blockIdx.x] = drew - acc_s;  // the next pass draws for the sites still not accepted
-    if (acc_s) atomicAdd(&stt->acc[pass], acc_s);
+    cnt_out[blockIdx.x] = drew - acc_s;  // the next pass draws for the sites still not accepted
+    if (pass == HB_PB - 1 && acc_s) atomicAdd(&stt->acc_last, acc_s);
+    if (blockIdx.x == 0) {
+      dpos[(pass + 1) & 1] = base + 4 * total;
+      stt->tot[pass] = total;
+      stt->pos = base + 4 * total;
+      if (pass == 0) stt->acc_last = 0;  // no workgroup of this pass adds to it
+    }
   }
 }
 
@@ -532,8 +534,8 @@ void heatbath_reference(Context &c, const int gaction[4], const double *coact, i
   const size_t ulen = 18 * (size_t)G.ntot * 128;
   const int nblk = (int)((G.nvh + HB_TB - 1) / HB_TB);
   DBuf<double> du(ulen), C(18 * G.nvh), Z(4 * G.nvh), DK(G.nvh), A0(G.nvh);
-  DBuf<int> ID(G.nvh), cnt(nblk), off(nblk);
-  DBuf<long> dbase(HB_PB);
+  DBuf<int> ID(G.nvh), cnt(2 * (size_t)nblk);  // the counts of a pass and of the next
+  DBuf<long> dpos(2);
   DBuf<HbStatus> dst(1);
   HbStatus *hst = nullptr;  // pinned: the one status copy per batch
   QQCD_HIP(hipHostMalloc((void **)&hst, sizeof(HbStatus)));
@@ -567,17 +569,17 @@ void heatbath_reference(Context &c, const int gaction[4], const double *coact, i
               // a site whose staple projects to zero never accepts (the reference would loop forever)
               if (batch == 200) throw std::runtime_error("sweep: Kennedy-Pendleton passes did not end (zero staple?)");
               S.ensure(pos, need_pass, chunk);
+              // HB_PB is even: every batch starts on the counts in cnt[0, nblk)
               for (int p = 0; p < HB_PB; p++) {
-                hipLaunchKernelGGL(k_hb_scan, dim3(1), dim3(1024), 0, c.stream, nblk, cnt.p, off.p, dst.p,
-                                   pos - S.origin, p == 0 ? 1 : 0, dbase.p, p);
-                hipLaunchKernelGGL(k_hb_try, grid, blk, 0, c.stream, G.nvh, S.buf, ID.p, DK.p, A0.p, off.p,
-                                   dbase.p, p, dst.p, cnt.p);
+                hipLaunchKernelGGL(k_hb_try, grid, blk, 0, c.stream, G.nvh, nblk, S.buf, ID.p, DK.p, A0.p,
+                                   cnt.p + (p & 1) * nblk, cnt.p + ((p + 1) & 1) * nblk, dpos.p, pos - S.origin,
+                                   p, dst.p);
                 QQCD_HIP(hipGetLastError());
               }
               QQCD_HIP(hipMemcpyAsync(hst, dst.p, sizeof(HbStatus), hipMemcpyDeviceToHost, c.stream));
               QQCD_HIP(hipStreamSynchronize(c.stream));
               pos = S.origin + hst->pos;
-              const long rem = hst->tot[HB_PB - 1] > 0 ? hst->tot[HB_PB - 1] - hst->acc[HB_PB - 1] : 0;
+              const long rem = hst->tot[HB_PB - 1] > 0 ? hst->tot[HB_PB - 1] - hst->acc_last : 0;
               if (rem == 0) break;
             }
             S.ensure(pos, 2 * G.nvh, chunk);
