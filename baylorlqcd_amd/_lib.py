@@ -82,7 +82,7 @@ SIGNATURES = {
     "qqcd_bench_smear": [ctypes.c_int, ctypes.c_int, _dp],
     "qqcd_gauge_get_ref": [_dp],
     "qqcd_solve_log": [ctypes.c_int, _ip, ctypes.c_char_p, _ip, _dp],
-    "qqcd_bench_lagfib": [ctypes.c_long, ctypes.c_int, _dp],
+    "qqcd_bench_lagfib": [ctypes.c_long, ctypes.c_int, ctypes.c_long, _dp],
     "qqcd_twmshift": [_dp, _dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip],
     "qqcd_gamma5mult": [_dp, _dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip],
     "qqcd_gfwrite": [_ip, _dp, ctypes.c_char_p, _ip, ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int],

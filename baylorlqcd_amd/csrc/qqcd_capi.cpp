@@ -534,10 +534,10 @@ extern "C" int qqcd_bench_smear(int nwarm, int niter, double *ms) {
   API_END
 }
 
-extern "C" int qqcd_bench_lagfib(long m, int niter, double *ms) {
+extern "C" int qqcd_bench_lagfib(long m, int niter, long chunk, double *ms) {
   API_BEGIN
   Context &c = ctx();
-  *ms = bench_lagfib(m, niter, c.stream);
+  *ms = bench_lagfib(m, niter, chunk, c.stream);
   API_END
 }
 

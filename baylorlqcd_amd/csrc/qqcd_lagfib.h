@@ -27,8 +27,9 @@ void lag_noise_source(const Geom &g, Spinor &out, int par, const uint32_t *d_dra
                       hipStream_t s);
 // the rnds scale factor: the single-precision literal 0.465661e-9 held in a double
 constexpr double LAG_FNORM = (double)0.465661e-9f;
-// kernel time (ms) of generating m draws (at most 4096 chunks of one launch)
-double bench_lagfib(long m, int niter, hipStream_t s);
+// kernel time (ms) of generating m draws in one launch of chunks of `chunk`
+// draws (0: the length lagfib_generate picks); m is cut to 2048 chunks
+double bench_lagfib(long m, int niter, long chunk, hipStream_t s);
 // drop the cached jump-ahead tables (device memory)
 void lagfib_release_device();
 

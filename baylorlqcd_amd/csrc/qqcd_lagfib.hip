@@ -15,9 +15,11 @@
 // the 521 words before chunk c (s = cL) are one polynomial (17 words, built
 // once per L on the host) applied to the first 1041 terms of the stream (the
 // base window and the next 520 terms, extended in LDS); then one wave per
-// chunk runs the recurrence 64 draws a step through an LDS ring (lanes 32-63
-// use y_t = y_{t-521} ^ y_{t-553} ^ y_{t-64}, so every operand comes from an
-// earlier step).  A zero draw (probability 2^-31) is
+// chunk runs the recurrence through an LDS ring: the first 1600 draws 64 a
+// step (lanes 32-63 use y_t = y_{t-521} ^ y_{t-553} ^ y_{t-64}, so every
+// operand comes from an earlier step), the rest 128 a step by the
+// recurrence of P(x)^4 = P(x^4), y_t = y_{t-128} ^ y_{t-2084}, which holds
+// for the same sequence (it is P(x) times P(x)^3).  A zero draw (probability 2^-31) is
 // the one non-linear step: the reference XORs again, so y_t becomes
 // y_{t-32}.  The first zero is found with an atomic minimum, corrected, and
 // the stream after it generated again from the corrected window.
@@ -32,7 +34,16 @@
 
 namespace qqcd {
 
-static constexpr int LP = 521, LQ = 32, LW = 17, RING = 1024;
+static constexpr int LP = 521, LQ = 32, LW = 17, RING = 4096;
+// draws of a chunk made 64 a step before the ring holds the 2084 draws the
+// fourth-power recurrence reaches back to (521 + B1 >= 2084, B1 % 64 == 0)
+static constexpr int B1 = 1600;
+// ring slot of draw r of a chunk (r >= -521): (r + S0) mod RING, S0 a
+// multiple of 64 at least 521
+static constexpr int S0 = 576;
+static_assert(LP + B1 >= 4 * LP && B1 % 64 == 0, "bootstrap too short");
+static_assert(S0 >= LP && S0 % 64 == 0 && (B1 + S0) % 64 == 0, "slot blocks must not wrap");
+static_assert(4 * LP + 128 <= RING && LP - 1 + 575 < RING, "ring too small");
 static constexpr int KMAX = 2048;  // chunks per launch
 typedef std::array<uint32_t, LW> Poly;
 
@@ -186,67 +197,123 @@ void lagfib_release_device() {
 
 // ---- device generation -----------------------------------------------------
 
-// Workgroup of one wave per chunk c = blockIdx.x.  Ring slot of draw t:
-// (t - cL + 521) mod 1024; slots 0..520 hold the window y_{cL-521} ..
-// y_{cL-1}.  LDS ordering between steps needs only wavefront scope: the
-// wave's LDS operations complete in order.
+// Workgroup of one wave per chunk c = blockIdx.x.  Slots 55..575 of the
+// ring hold the window y_{cL-521} .. y_{cL-1} at the start.  LDS ordering
+// between steps needs only wavefront scope: the wave's LDS operations
+// complete in order.
+__device__ __forceinline__ int lag_slot(int r) { return (r + S0) & (RING - 1); }
 __device__ __forceinline__ void lag_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// one step of the fourth-power recurrence: draws r0 + lane and r0 + 64 + lane
+// (r0 + S0 a multiple of 64, so the blocks of 64 slots at r0 - 128, r0 - 64,
+// r0 and r0 + 64 never wrap; the taps 2084 back are wrapped per lane)
+template <bool TAIL>
+__device__ __forceinline__ void lag_step4(uint32_t *ring, uint32_t *o, int r0, int lane, int len, uint32_t &vmin) {
+  const int s1 = (r0 + S0 - 4 * LQ) & (RING - 1), s2 = (r0 + S0 - 2 * LQ) & (RING - 1);
+  const int sw = (r0 + S0) & (RING - 1), sw2 = (r0 + S0 + 64) & (RING - 1);
+  const int sf = r0 + S0 - 4 * LP;  // >= 0 for r0 >= B1
+  const uint32_t ya = ring[s1 + lane] ^ ring[(sf + lane) & (RING - 1)];
+  const uint32_t yb = ring[s2 + lane] ^ ring[(sf + 64 + lane) & (RING - 1)];
+  if (!TAIL || r0 + lane < len) {
+    ring[sw + lane] = ya;
+    o[r0 + lane] = ya;
+    vmin = min(vmin, ya);
+  }
+  if (!TAIL || r0 + 64 + lane < len) {
+    ring[sw2 + lane] = yb;
+    o[r0 + 64 + lane] = yb;
+    vmin = min(vmin, yb);
+  }
+}
+
 __global__ void __launch_bounds__(64) k_lag_gen(const uint32_t *base, const uint32_t *polys, long L, long m, int K,
                                                 uint32_t *out, unsigned long long *first_zero) {
-  // y[k] = y_{k-521}, k = 0..1040: the base window, then 520 terms by the recurrence
-  __shared__ uint32_t y[2 * LP + 31];
+  // ring[k] = y_{k-521}, k = 0..1040, while the window is formed: the base
+  // window, then 520 terms by the recurrence; afterwards the ring of draws
   __shared__ uint32_t ring[RING];
   const int lane = threadIdx.x;
   const int c = blockIdx.x;
-  for (int k = lane; k < LP; k += 64) y[k] = base[k];
-  __syncthreads();
+  // the chunk's polynomial x^(cL) mod P, word w in lane w, and the base
+  // window, all loads in flight together
+  const uint32_t pw = lane < LW ? polys[(long)c * LW + lane] : 0u;
+  uint32_t v[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) v[q] = lane + 64 * q < LP ? base[lane + 64 * q] : 0u;
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+    if (lane + 64 * q < LP) ring[lane + 64 * q] = v[q];
+  lag_wave_sync();
   for (int k0 = LP; k0 < 2 * LP - 1; k0 += 32) {
     const int k = k0 + lane;
-    if (lane < 32 && k < 2 * LP - 1) y[k] = y[k - LP] ^ y[k - LQ];
+    if (lane < 32 && k < 2 * LP - 1) ring[k] = ring[k - LP] ^ ring[k - LQ];
     lag_wave_sync();
   }
-  // window: w_j = XOR over the set bits k of x^(cL) mod P of y[k + j]
-  {
-    const uint32_t *poly = polys + (long)c * LW;
-    uint32_t v[9];
+  // window: w_j = XOR over the set bits k of the polynomial of y[k + j], two
+  // bits a round, every read unconditional (k + j < 1096 stays in the ring;
+  // lanes with j >= 521 read words they never keep)
 #pragma unroll
-    for (int q = 0; q < 9; q++) v[q] = 0;
-    for (int w = 0; w < LW; w++) {
-      uint32_t bits = poly[w];  // the same word for the whole wave
-      while (bits) {
-        const int k = 32 * w + __builtin_ctz(bits);
+  for (int q = 0; q < 9; q++) v[q] = 0u;
+  for (int w = 0; w < LW; w++) {
+    uint32_t bits = (uint32_t)__builtin_amdgcn_readlane((int)pw, w);
+    while (bits) {
+      const int k1 = 32 * w + __builtin_ctz(bits);
+      bits &= bits - 1;
+      int k2 = k1;
+      uint32_t m2 = 0u;
+      if (bits) {
+        k2 = 32 * w + __builtin_ctz(bits);
         bits &= bits - 1;
+        m2 = ~0u;
+      }
 #pragma unroll
-        for (int q = 0; q < 9; q++) {
-          const int jj = lane + 64 * q;
-          if (jj < LP) v[q] ^= y[k + jj];
-        }
+      for (int q = 0; q < 9; q++) v[q] ^= ring[k1 + lane + 64 * q] ^ (ring[k2 + lane + 64 * q] & m2);
+    }
+  }
+  __syncthreads();  // every read of the start terms is done before the window replaces them
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+    if (lane + 64 * q < LP) ring[lag_slot(lane + 64 * q - LP)] = v[q];
+  __syncthreads();
+  const long t0 = (long)c * L;
+  const int len = (int)std::min(L, m - t0);
+  uint32_t *o = out + t0;
+  uint32_t vmin = ~0u;  // a zero draw shows as a zero minimum
+  // draws 0 .. B1-1 of the chunk, 64 a step
+  const int n1 = std::min(len, B1);
+  for (int r0 = 0; r0 < n1; r0 += 64) {
+    const int r = r0 + lane;
+    uint32_t yy = ring[lag_slot(r - LP)];
+    if (lane < 32) yy ^= ring[lag_slot(r - LQ)];
+    else yy ^= ring[lag_slot(r - LP - LQ)] ^ ring[lag_slot(r - 2 * LQ)];
+    if (r < n1) {
+      ring[lag_slot(r)] = yy;
+      o[r] = yy;
+      vmin = min(vmin, yy);
+    }
+    lag_wave_sync();
+  }
+  // the rest 128 a step by y_t = y_{t-128} ^ y_{t-2084} (P(x)^4 = P(x^4)):
+  // every tap lies at least 64 draws back, in an earlier step
+  int r0 = B1;
+  for (; r0 + 128 <= len; r0 += 128) {
+    lag_step4<false>(ring, o, r0, lane, len, vmin);
+    lag_wave_sync();
+  }
+  if (r0 < len) lag_step4<true>(ring, o, r0, lane, len, vmin);
+  // the first zero of the chunk (probability about L 2^-32): found again
+  // from the chunk's own output, each lane reading back what it wrote
+  if (__ballot(vmin == 0u)) {
+    for (int q0 = 0; q0 < len; q0 += 64) {
+      const unsigned long long z = __ballot(q0 + lane < len && o[q0 + lane] == 0u);
+      if (z) {
+        if (lane == 0) atomicMin(first_zero, (unsigned long long)(t0 + q0 + __ffsll((long long)z) - 1));
+        break;
       }
     }
-#pragma unroll
-    for (int q = 0; q < 9; q++)
-      if (lane + 64 * q < LP) ring[lane + 64 * q] = v[q];
-  }
-  __syncthreads();
-  const long len = std::min(L, m - (long)c * L);
-  const long t0 = (long)c * L;
-  const long steps = (len + 63) / 64;
-  for (long st = 0; st < steps; st++) {
-    const long u = LP + 64 * st + lane;
-    if (64 * st + lane < len) {
-      uint32_t yy = ring[(u - LP) & (RING - 1)];
-      if (lane < 32) yy ^= ring[(u - LQ) & (RING - 1)];
-      else yy ^= ring[(u - LP - LQ) & (RING - 1)] ^ ring[(u - 2 * LQ) & (RING - 1)];
-      ring[u & (RING - 1)] = yy;
-      out[t0 + 64 * st + lane] = yy;
-      if (yy == 0u) atomicMin(first_zero, (unsigned long long)(t0 + 64 * st + lane));
-    }
-    lag_wave_sync();
   }
 }
 
@@ -273,6 +340,13 @@ __global__ void k_lag_rebase(const uint32_t *base, uint32_t *out, long tend, lon
   }
 }
 
+// chunk length for n draws: 8192 2^k, the least that needs at most KMAX chunks
+static long chunk_len(long n) {
+  long L = 8192;
+  while ((n + L - 1) / L > KMAX && L < (1L << 21)) L *= 2;
+  return L;
+}
+
 void lagfib_generate(LagState &st, long m, uint32_t *d_out, hipStream_t s) {
   lagfib_check(st);
   if (m < 0) throw std::runtime_error("lagfib_generate: negative count");
@@ -291,8 +365,7 @@ void lagfib_generate(LagState &st, long m, uint32_t *d_out, hipStream_t s) {
     uint32_t *o = d_out;
     long left = m;
     while (left > 0) {
-      long L = 8192;
-      while ((left + L - 1) / L > KMAX && L < (1L << 21)) L *= 2;
+      const long L = chunk_len(left);
       const long piece = std::min(left, (long)KMAX * L);
       const int K = (int)((piece + L - 1) / L);
       const uint32_t *rows = table_for(L, K, s);
@@ -351,14 +424,14 @@ void lagfib_generate(LagState &st, long m, uint32_t *d_out, hipStream_t s) {
   QQCD_HIP(hipFree(dbad));
 }
 
-double bench_lagfib(long m, int niter, hipStream_t s) {
+double bench_lagfib(long m, int niter, long chunk, hipStream_t s) {
   if (m <= 0 || niter <= 0) throw std::runtime_error("bench_lagfib: m, niter > 0");
+  if (chunk < 0 || chunk > (1L << 21)) throw std::runtime_error("bench_lagfib: chunk in 0 (default) .. 2^21");
   LagState st;
   lagfib_init(0, 1, st);
   uint32_t hb[LP];
   for (int k = 0; k < LP; k++) hb[k] = st.w[(st.jr + k) % LP];
-  long L = 8192;
-  while ((m + L - 1) / L > KMAX && L < (1L << 21)) L *= 2;
+  const long L = chunk ? chunk : chunk_len(m);
   m = std::min(m, (long)KMAX * L);
   const int K = (int)((m + L - 1) / L);
   const uint32_t *rows = table_for(L, K, s);

@@ -594,7 +594,9 @@ contains
       real(kind=8), intent(out), target, contiguous :: be(:,:,:,:,:), bo(:,:,:,:,:)
       integer, intent(in) :: nvhalf, myid
       if (.not. started) call fail("DISCONLOOPS", -1_c_int)
-      if (size(be, 2) < nvhalf .or. nvhalf /= nx*ny*nz*nt/32) call fail("DISCONLOOPS", -1_c_int)
+      if (nvhalf /= nx*ny*nz*nt/32) call fail("DISCONLOOPS", -1_c_int)
+      call file_dims("DISCONLOOPS", size(be, 2), spinor_shape(be) .and. spinor_shape(bo) .and. &
+                     size(bo, 2) == size(be, 2))
       call fail("DISCONLOOPS", c_z4source_lagfib(c_loc(be), c_loc(bo), iwlagfib, jrlagfib, krlagfib))
    end subroutine z4source
 
@@ -604,7 +606,9 @@ contains
       real(kind=8), intent(out), target, contiguous :: be(:,:,:,:,:), bo(:,:,:,:,:)
       integer, intent(in) :: nvhalf
       if (.not. started) call fail("DISCONLOOPS", -1_c_int)
-      if (size(be, 2) < nvhalf .or. nvhalf /= nx*ny*nz*nt/32) call fail("DISCONLOOPS", -1_c_int)
+      if (nvhalf /= nx*ny*nz*nt/32) call fail("DISCONLOOPS", -1_c_int)
+      call file_dims("DISCONLOOPS", size(be, 2), spinor_shape(be) .and. spinor_shape(bo) .and. &
+                     size(bo, 2) == size(be, 2))
       call fail("DISCONLOOPS", c_z2source_lagfib(c_loc(be), c_loc(bo), iwlagfib, jrlagfib, krlagfib))
    end subroutine z2source
 
@@ -716,6 +720,12 @@ contains
    end subroutine sweep
 
    ! ---- the reference's files (host only: no device, usable before the first operator call) ----
+
+   ! a spinor array the library fills whole: (6, ntotal, 4, 2, 8), both halves of the source alike
+   logical function spinor_shape(a)
+      real(kind=8), intent(in) :: a(:,:,:,:,:)
+      spinor_shape = size(a, 1) == 6 .and. size(a, 3) == 4 .and. size(a, 4) == 2 .and. size(a, 5) == 8
+   end function spinor_shape
 
    ! the single-process array extents the files are laid out in: (nvhalf, ntotal)
    subroutine file_dims(where, nsites, ok)

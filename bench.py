@@ -136,7 +136,7 @@ def rows_section(no_cpu=False):
              "frac": round(g / HBM_PEAK_GBS, 3)}
     print(f"[bench] smear {smear}", file=sys.stderr, flush=True)
     m = 2048 * 32768
-    _lib.check(L.qqcd_bench_lagfib(m, 10, _lib.dp(ms)))
+    _lib.check(L.qqcd_bench_lagfib(m, 10, 0, _lib.dp(ms)))
     lag = {"draws": m, "us": round(ms[0] * 1e3, 1), "Gdraws/s": round(m / (ms[0] * 1e-3) / 1e9, 2),
            "GB/s_written": round(4 * m / (ms[0] * 1e-3) / 1e9, 1), "frac": round(4 * m / (ms[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
     print(f"[bench] lagfib {lag}", file=sys.stderr, flush=True)

@@ -507,9 +507,11 @@ int qqcd_bench_dslash_multi(int prec, int recon, int nrhs, int nact, int nb, int
 int qqcd_autotune_multi(int prec, int recon, int nrhs, int *nb, int *order, double *ms);
 
 /* Kernel time (ms) of one smearing iteration (both checkerboards, FP64) on
- * the resident links; of generating m draws of the lagged-Fibonacci stream. */
+ * the resident links; of generating m draws of the lagged-Fibonacci stream
+ * in one launch of chunks of `chunk` draws (0: the generator's own choice;
+ * m is cut to 2048 chunks). */
 int qqcd_bench_smear(int nwarm, int niter, double *ms);
-int qqcd_bench_lagfib(long m, int niter, double *ms);
+int qqcd_bench_lagfib(long m, int niter, long chunk, double *ms);
 
 /* Time a plain device-to-device copy kernel (read + write `bytes` each
  * launch): the calibration point for hardware byte counters. */
