@@ -95,7 +95,7 @@ SIGNATURES = {
     "qqcd_loopsum": [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, ctypes.c_double, _dp],
     "qqcd_trace16": [_dp, _dp, _dp, _dp, _dp],
     "qqcd_batch_set": [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                       ctypes.c_int],
+                       ctypes.c_int, ctypes.c_int],
     "qqcd_batch_stats": [_ip, _ip, _lp, _lp, _ip, _ip],
     "qqcd_hop_multi_canonical": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp],

@@ -88,6 +88,10 @@ class Batch {
   MultiBuf R64, Bin, E, R;
   std::vector<MultiBuf *> basis_bufs;
   std::vector<MultiSpinor> V;
+  // FP16 basis only: the FP32 vector being orthogonalised and the newest
+  // basis vector in FP32 (the operator's input)
+  std::unique_ptr<MultiBuf> Wf, Qf;
+  bool half() const { return p.basis == PREC_HALF; }
   // device state: ints [0] n_outer, [1] n_inner, [2] any, [3, 3+cap) flags
   DevBuf<int> di;
   DevBuf<double> h1, h2, n2a, n2b, nf, H, coef, coef2, n2;
@@ -164,10 +168,16 @@ Batch::Batch(Context &c_, const double kappa[2], MultiSpinor &B_, MultiSpinor &X
       throw std::runtime_error("batched_solve: the deflation space must hold k+1 FP32 vectors");
     if (k > m) throw std::runtime_error("batched_solve: k > m");
   }
+  if (p.basis != PREC_SINGLE && p.basis != PREC_HALF)
+    throw std::runtime_error("batched_solve: the Krylov basis is FP32 or FP16");
   basis_bufs.reserve(m + 1);
   for (int i = 0; i <= m; i++) {
-    basis_bufs.push_back(new MultiBuf(c.g, PREC_SINGLE, cap, nb));
+    basis_bufs.push_back(new MultiBuf(c.g, p.basis, cap, nb, false));
     V.push_back(basis_bufs.back()->m);
+  }
+  if (half()) {
+    Wf.reset(new MultiBuf(c.g, PREC_SINGLE, cap, nb));
+    Qf.reset(new MultiBuf(c.g, PREC_SINGLE, cap, nb));
   }
   mb.reserve(mb.need());
   clear_flags(d_flag(), d_any(), cap, c.stream);
@@ -194,16 +204,24 @@ void Batch::swap_slots(int a, int b, bool inner_fields) {
 // one Arnoldi step of every active inner slot; device-only (capturable)
 void Batch::arnoldi_step(int j) {
   int *nin = d_nin();
-  A32(V[j], V[j + 1], nin);
-  mb.dot(j + 2, V.data(), V[j + 1], nin, h1.p);
-  mb.axpy(j + 1, V.data(), h1.p, 0, -1.0, V[j + 1], nin, n2a.p);
+  // w: the new vector, orthogonalised in place (FP32; FP16 basis: in Wf)
+  MultiSpinor &w = half() ? Wf->m : V[j + 1];
+  A32(half() ? Qf->m : V[j], w, nin);
+  if (half()) {
+    mb.dot(j + 1, V.data(), w, nin, h1.p);
+    mb.dot(1, &w, w, nin, h1.p + (size_t)2 * (j + 1) * cap);
+  } else {
+    mb.dot(j + 2, V.data(), w, nin, h1.p);
+  }
+  mb.axpy(j + 1, V.data(), h1.p, 0, -1.0, w, nin, n2a.p);
   if (p.reorth > 0) {
     arnoldi_flags(n2a.p, h1.p + (size_t)2 * (j + 1) * cap, d_flag(), d_any(), nin, cap, p.reorth == 2, c.stream);
-    mb.dot(j + 1, V.data(), V[j + 1], nin, h2.p, d_any());
-    mb.axpy(j + 1, V.data(), h2.p, 0, -1.0, V[j + 1], nin, n2b.p, d_flag(), d_any());
+    mb.dot(j + 1, V.data(), w, nin, h2.p, d_any());
+    mb.axpy(j + 1, V.data(), h2.p, 0, -1.0, w, nin, n2b.p, d_flag(), d_any());
   }
   arnoldi_column(H.p, m + 1, j, h1.p, h2.p, d_flag(), n2a.p, n2b.p, nf.p, nin, cap, c.stream);
-  mb.scale_inv_norm(V[j + 1], V[j + 1], nf.p, nin);
+  if (half()) mb.scale_inv_norm2(w, Qf->m, V[j + 1], nf.p, nin);
+  else mb.scale_inv_norm(w, w, nf.p, nin);
 }
 
 void Batch::arnoldi_cycle() {
@@ -279,7 +297,8 @@ void Batch::inner() {
       mb.norm2(R.m, d_nin(), n2.p);
     }
     // GMRES(m) from r: V_0 = r / ||r||
-    mb.scale_inv_norm(R.m, V[0], n2.p, d_nin());
+    if (half()) mb.scale_inv_norm2(R.m, Qf->m, V[0], n2.p, d_nin());
+    else mb.scale_inv_norm(R.m, V[0], n2.p, d_nin());
     download(hn2.p, n2.p, cap);
     std::vector<double> beta(n_in);
     for (int s = 0; s < n_in; s++) beta[s] = std::sqrt(hn2[s]);

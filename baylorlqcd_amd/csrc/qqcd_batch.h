@@ -43,6 +43,9 @@ struct BatchParams {
   int use_graph = 1;      // replay the Arnoldi cycle from a captured graph
   Prec op_prec = PREC_SINGLE;
   int op_recon = 12;
+  // Krylov basis storage of the inner GMRES: FP32, or FP16 scaled unit
+  // vectors (the operator then reads an FP32 copy of the newest one)
+  Prec basis = PREC_HALF;
   MultiTune tune;
 };
 

@@ -354,8 +354,9 @@ extern "C" int qqcd_loopsum(int ksub, const double *xe, const double *xo, const 
 }
 
 extern "C" int qqcd_batch_set(int reorth, int use_graph, double inner_tol, int inner_cycles, int max_outer,
-                              int op_prec, int op_recon) {
+                              int op_prec, int op_recon, int basis_prec) {
   API_BEGIN
+  if (basis_prec != PREC_SINGLE && basis_prec != PREC_HALF) throw std::runtime_error("basis_prec is 1 (FP32) or 2 (FP16)");
   if (reorth < 0 || reorth > 2) throw std::runtime_error("reorth must be 0, 1 or 2");
   if (inner_cycles < 1 || max_outer < 1) throw std::runtime_error("cycle limits must be >= 1");
   if (op_prec != PREC_SINGLE && op_prec != PREC_HALF) throw std::runtime_error("inner operator is FP32 or FP16 links");
@@ -368,6 +369,7 @@ extern "C" int qqcd_batch_set(int reorth, int use_graph, double inner_tol, int i
   // FP32 spinors always; op_prec picks the link storage of the inner operator
   g_bp.op_prec = PREC_SINGLE;
   g_bp.op_recon = op_recon;
+  g_bp.basis = (Prec)basis_prec;
   (void)op_prec;
   API_END
 }

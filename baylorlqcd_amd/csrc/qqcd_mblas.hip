@@ -10,6 +10,7 @@
 // Coefficients are read from device arrays [q * cap + slot] and the active
 // count from device memory, so the kernels can be replayed from a graph.
 #include <algorithm>
+#include <cmath>
 #include <functional>
 #include <stdexcept>
 #include <type_traits>
@@ -41,13 +42,32 @@ template <> struct E2<float> {
   }
 };
 
+// FP16 Krylov basis (no per-site norms): unit vectors stored times
+// hs = sqrt(12 stride), so their elements are of order one; E2<f16> moves the
+// stored values, and the passes that read such vectors scale by 1/hs
+template <> struct E2<f16> {
+  typedef f16 h2 __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ double2 ld(const void *p, long i) {
+    const h2 v = ((const h2 *)p)[i];
+    return make_double2((double)(float)v[0], (double)(float)v[1]);
+  }
+  static __device__ __forceinline__ double2 st(void *p, long i, double re, double im) {
+    h2 v;
+    v[0] = (f16)(float)re;
+    v[1] = (f16)(float)im;
+    ((h2 *)p)[i] = v;
+    return make_double2((double)(float)v[0], (double)(float)v[1]);
+  }
+};
+
 struct MShape {
   long rows;       // 12 * stride
   long blk_elems;  // rows * nb
   const int *nact;
   int lognb;
   int cap;
-  int gx;  // workgroups per RHS block
+  int gx;         // workgroups per RHS block
+  double hs, hinv;  // FP16 basis scale and its inverse
 };
 
 struct MPtr {
@@ -104,6 +124,10 @@ __global__ void __launch_bounds__(MB) k_mdot(MShape a, MPtr v, const void *w, do
       }
     }
   }
+  if (std::is_same<TV, f16>::value) {
+#pragma unroll
+    for (int k = 0; k < 2 * K; k++) acc[k] *= a.hinv;
+  }
   slot_sum<2 * K>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
 }
 
@@ -139,10 +163,11 @@ __global__ void __launch_bounds__(MB) k_maxpy(MShape a, MPtr v, const double *co
   if (act && flag) act = flag[slot] != 0;
   if (act) {
     double cr[K], ci[K];
+    const double f = std::is_same<TV, f16>::value ? sign * a.hinv : sign;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      cr[k] = sign * coef[2 * ((long)(q0 + k) * a.cap + slot)];
-      ci[k] = sign * coef[2 * ((long)(q0 + k) * a.cap + slot) + 1];
+      cr[k] = f * coef[2 * ((long)(q0 + k) * a.cap + slot)];
+      ci[k] = f * coef[2 * ((long)(q0 + k) * a.cap + slot) + 1];
     }
     const int per = MB >> a.lognb;
     const long base = (long)blk * a.blk_elems;
@@ -218,6 +243,26 @@ __global__ void __launch_bounds__(MB) k_mscale(MShape a, const void *x, void *w,
   }
 }
 
+// Q(r) = X(r) / sqrt(n2[r]) and H(r) = hs Q(r) in FP16 (a new basis vector:
+// Q feeds the operator, H the Gram-Schmidt passes)
+template <typename TX, typename TQ>
+__global__ void __launch_bounds__(MB) k_mscale2(MShape a, const void *x, void *q, void *h, const double *n2) {
+  const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
+  if ((blk << a.lognb) >= nact) return;
+  const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
+  if (slot >= nact) return;
+  const double n = n2[slot];
+  const double s = n > 0.0 ? 1.0 / sqrt(n) : 0.0;
+  const int per = MB >> a.lognb;
+  const long base = (long)blk * a.blk_elems;
+  for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
+    const long e = base + (p << a.lognb) + rr;
+    const double2 u = E2<TX>::ld(x, e);
+    const double2 v = E2<TQ>::st(q, e, s * u.x, s * u.y);
+    E2<f16>::st(h, e, a.hs * v.x, a.hs * v.y);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(MB) k_mswap(long rows, int lognb, long blk_elems, void *w, int sa, int sb) {
   const long p = blockIdx.x * (long)MB + threadIdx.x;
@@ -290,7 +335,7 @@ __global__ void k_arnoldi_column(double *H, int ldh, int j, const double *h1, co
 
 static size_t esize(Prec p) { return p == PREC_DOUBLE ? 16 : p == PREC_SINGLE ? 8 : 4; }
 
-MultiSpinor alloc_multi(const Geom &g, Prec p, int cap, int nb) {
+MultiSpinor alloc_multi(const Geom &g, Prec p, int cap, int nb, bool norms) {
   if (nb < 1 || nb > 64 || (nb & (nb - 1))) throw std::runtime_error("alloc_multi: RHS block must be 1..64, a power of two");
   if (cap < 1 || cap > 256) throw std::runtime_error("alloc_multi: 1..256 right-hand sides");
   MultiSpinor m;
@@ -302,7 +347,7 @@ MultiSpinor alloc_multi(const Geom &g, Prec p, int cap, int nb) {
   m.stride = g.stride;
   QQCD_HIP(hipMalloc(&m.data, (size_t)m.elems() * esize(p)));
   QQCD_HIP(hipMemset(m.data, 0, (size_t)m.elems() * esize(p)));
-  if (p == PREC_HALF) {
+  if (p == PREC_HALF && norms) {
     const size_t nn = (size_t)m.stride * nb * m.nblk();
     QQCD_HIP(hipMalloc(&m.norm, nn * sizeof(float)));
     QQCD_HIP(hipMemset(m.norm, 0, nn * sizeof(float)));
@@ -350,8 +395,13 @@ static MShape shape(const MultiSpinor &W, const int *nact) {
   a.lognb = W.lognb;
   a.cap = W.cap;
   a.gx = gx_of(a.rows, W.nb, W.nblk());
+  a.hs = std::sqrt(12.0 * (double)W.stride);
+  a.hinv = 1.0 / a.hs;
   return a;
 }
+
+// a scaled FP16 basis field: PREC_HALF without per-site norms
+static bool basis16(const MultiSpinor &V) { return V.prec == PREC_HALF && !V.norm; }
 
 static void need_fp(const MultiSpinor &W, const char *who) {
   if (W.prec == PREC_HALF) throw std::runtime_error(std::string(who) + ": FP16 multi-RHS fields are hopping-only");
@@ -363,6 +413,13 @@ template <typename F> static void two_types(Prec a, Prec b, F &&f) {
   else if (a == PREC_SINGLE && b == PREC_DOUBLE) f((float *)0, (double *)0);
   else if (a == PREC_SINGLE && b == PREC_SINGLE) f((float *)0, (float *)0);
   else throw std::runtime_error("multi BLAS: FP64 / FP32 fields only");
+}
+
+// the Gram-Schmidt passes also read scaled FP16 basis vectors (a = PREC_HALF)
+template <typename F> static void basis_types(Prec a, Prec b, F &&f) {
+  if (a == PREC_HALF && b == PREC_SINGLE) f((f16 *)0, (float *)0);
+  else if (a == PREC_HALF && b == PREC_DOUBLE) f((f16 *)0, (double *)0);
+  else two_types(a, b, f);
 }
 
 #define T_OF(p) typename std::remove_pointer<decltype(p)>::type
@@ -377,7 +434,7 @@ static void dot_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, const
     const int K = std::min(8, nq - q0);
     MPtr pk{};
     for (int k = 0; k < K; k++) pk.p[k] = vp[q0 + k];
-    two_types(vprec, W.prec, [&](auto *tv, auto *tw) {
+    basis_types(vprec, W.prec, [&](auto *tv, auto *tw) {
       typedef T_OF(tv) TV;
       typedef T_OF(tw) TW;
       dim3 grid(a.gx, nblk);
@@ -405,8 +462,9 @@ void MBlas::dot(int nq, const MultiSpinor *V, const MultiSpinor &W, const int *n
   need_fp(W, "MBlas::dot");
   std::vector<const void *> vp(nq);
   for (int q = 0; q < nq; q++) {
-    if (V[q].prec != V[0].prec || V[q].nb != W.nb || V[q].cap != W.cap)
+    if (V[q].prec != V[0].prec || V[q].nb != W.nb || V[q].cap != W.cap || V[q].stride != W.stride)
       throw std::runtime_error("MBlas::dot: layouts differ");
+    if (V[q].prec == PREC_HALF && !basis16(V[q])) throw std::runtime_error("MBlas::dot: FP16 fields with norms");
     vp[q] = V[q].data;
   }
   if (nq == 0) return;
@@ -435,7 +493,7 @@ static void axpy_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, cons
     const bool last = q0 + K >= nq;
     MPtr pk{};
     for (int k = 0; k < K; k++) pk.p[k] = vp[q0 + k];
-    two_types(vprec, W.prec, [&](auto *tv, auto *tw) {
+    basis_types(vprec, W.prec, [&](auto *tv, auto *tw) {
       typedef T_OF(tv) TV;
       typedef T_OF(tw) TW;
       dim3 grid(a.gx, nblk);
@@ -474,8 +532,9 @@ void MBlas::axpy(int nq, const MultiSpinor *V, const double *coef, int q0, doubl
   need_fp(W, "MBlas::axpy");
   std::vector<const void *> vp(nq);
   for (int q = 0; q < nq; q++) {
-    if (V[q].prec != V[0].prec || V[q].nb != W.nb || V[q].cap != W.cap)
+    if (V[q].prec != V[0].prec || V[q].nb != W.nb || V[q].cap != W.cap || V[q].stride != W.stride)
       throw std::runtime_error("MBlas::axpy: layouts differ");
+    if (V[q].prec == PREC_HALF && !basis16(V[q])) throw std::runtime_error("MBlas::axpy: FP16 fields with norms");
     vp[q] = V[q].data;
   }
   if (nq == 0) {
@@ -546,6 +605,23 @@ void MBlas::scale_inv_norm(const MultiSpinor &X, MultiSpinor &W, const double *n
     typedef T_OF(tx) TX;
     typedef T_OF(tw) TW;
     hipLaunchKernelGGL((k_mscale<TX, TW>), dim3(a.gx, W.nblk()), dim3(MB), 0, stream, a, X.data, W.data, n2);
+  });
+  QQCD_HIP(hipGetLastError());
+}
+
+void MBlas::scale_inv_norm2(const MultiSpinor &X, MultiSpinor &Q, MultiSpinor &H, const double *n2,
+                            const int *nact) {
+  need_fp(Q, "MBlas::scale_inv_norm2");
+  if (!basis16(H)) throw std::runtime_error("MBlas::scale_inv_norm2: H must be an FP16 basis field");
+  if (X.nb != Q.nb || X.cap != Q.cap || X.stride != Q.stride || H.nb != Q.nb || H.cap != Q.cap ||
+      H.stride != Q.stride)
+    throw std::runtime_error("MBlas::scale_inv_norm2: layouts differ");
+  const MShape a = shape(Q, nact);
+  two_types(X.prec, Q.prec, [&](auto *tx, auto *tq) {
+    typedef T_OF(tx) TX;
+    typedef T_OF(tq) TQ;
+    hipLaunchKernelGGL((k_mscale2<TX, TQ>), dim3(a.gx, Q.nblk()), dim3(MB), 0, stream, a, X.data, Q.data, H.data,
+                       n2);
   });
   QQCD_HIP(hipGetLastError());
 }

@@ -15,12 +15,14 @@
 
 namespace qqcd {
 
-MultiSpinor alloc_multi(const Geom &g, Prec p, int cap, int nb);
+// norms = false for PREC_HALF: a Krylov basis field, unit vectors stored
+// times sqrt(12 stride) without per-site norms (read by dot and axpy only)
+MultiSpinor alloc_multi(const Geom &g, Prec p, int cap, int nb, bool norms = true);
 void free_multi(MultiSpinor &m);
 
 struct MultiBuf {
   MultiSpinor m;
-  MultiBuf(const Geom &g, Prec p, int cap, int nb) : m(alloc_multi(g, p, cap, nb)) {}
+  MultiBuf(const Geom &g, Prec p, int cap, int nb, bool norms = true) : m(alloc_multi(g, p, cap, nb, norms)) {}
   ~MultiBuf() { free_multi(m); }
   MultiBuf(const MultiBuf &) = delete;
   MultiBuf &operator=(const MultiBuf &) = delete;
@@ -34,8 +36,8 @@ class MBlas {
   MBlas(const MBlas &) = delete;
   MBlas &operator=(const MBlas &) = delete;
 
-  // res[q*cap + r] = <V_q(r), W(r)> for q < nq (V_q multi) -- device result;
-  // with `any` given, the pass is skipped when *any == 0
+  // res[q*cap + r] = <V_q(r), W(r)> for q < nq (V_q multi, FP64, FP32 or an
+  // FP16 basis) -- device result; with `any` given, the pass is skipped when *any == 0
   void dot(int nq, const MultiSpinor *V, const MultiSpinor &W, const int *nact, double *res,
            const int *any = nullptr);
   // res[q*cap + r] = <D_q, W(r)> for single-RHS vectors D_q shared by all slots
@@ -53,6 +55,8 @@ class MBlas {
   void axpby(double a, const MultiSpinor &X, double b, MultiSpinor &Y, const int *nact, double *n2);
   // W(r) = X(r) * s[r] where s[r] = 1/sqrt(n2[r]) (0 when n2[r] == 0); X may be W
   void scale_inv_norm(const MultiSpinor &X, MultiSpinor &W, const double *n2, const int *nact);
+  // the same into Q and into the FP16 basis field H (X may be Q)
+  void scale_inv_norm2(const MultiSpinor &X, MultiSpinor &Q, MultiSpinor &H, const double *n2, const int *nact);
   // swap slots a and b of W
   void swap(MultiSpinor &W, int a, int b);
   // single <-> slot, with precision conversion

@@ -130,9 +130,12 @@ def noise_loop(nrhs, seed, kappa, GMRES, tol=1e-10, scale=1.0, coact=None, bc=(1
             "res_full": res, "iters": it, "times": times, "stats": dict(zip(keys, (int(s) for s in stats)))}
 
 
-def batch_set(reorth=0, use_graph=1, inner_tol=1e-5, inner_cycles=4, max_outer=30, op_prec=1, op_recon=12):
+def batch_set(reorth=0, use_graph=1, inner_tol=1e-5, inner_cycles=4, max_outer=30, op_prec=1, op_recon=12,
+              basis_prec=2):
+    """Knobs of the batched solver; basis_prec 2 (FP16, the default) or 1
+    (FP32) for the inner GMRES's Krylov basis."""
     _lib.check(_lib.lib().qqcd_batch_set(int(reorth), int(use_graph), float(inner_tol), int(inner_cycles),
-                                         int(max_outer), int(op_prec), int(op_recon)))
+                                         int(max_outer), int(op_prec), int(op_recon), int(basis_prec)))
 
 
 def batch_stats():

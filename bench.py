@@ -303,20 +303,27 @@ def propagator_section(train_steps=1500, n5=(48, 48, 48, 96), nlearned=2):
             "training": train}
 
 
-def noise_section(nrhs=64, seq=3):
+def noise_section(nrhs=64, seq=3, graph=1):
     """The disconnected-loop workload: 24^3x48 beta=6.0 heatbath field,
     Wilson kappa 0.1565, nrhs Z4 sources, GMRES-DR(60,20) on the first and
-    the batched GMRES-Proj(40) over its deflation space on the rest, every
-    source to a 1e-10 true residual, 16 local trace estimators; against the
-    same sources one after the other in plain FP64 (psc sequence), timed on
-    `seq` sources and scaled to nrhs - 1."""
+    the batched GMRES-Proj(40) over its deflation space on the rest (Krylov
+    basis in FP16, then again in FP32), every source to a 1e-10 true
+    residual, 16 local trace estimators; against the same sources one after
+    the other in plain FP64 (psc sequence), timed on `seq` sources and
+    scaled to nrhs - 1."""
     lattice.finalize()
     n3 = (24, 24, 24, 48)
     lattice.init(n3, BC)
     co, plaq = heatbath_field()
     kappa = (0.1565, 0.0)
+    disconloops.batch_set(use_graph=graph)
     r = disconloops.noise_loop(nrhs, 99, kappa, (60, 20), tol=1e-10, coact=co, bc=BC)
     print(f"[bench] batched noise loop: times {r['times']} stats {r['stats']}", file=sys.stderr, flush=True)
+    disconloops.batch_set(use_graph=graph, basis_prec=1)
+    h = disconloops.noise_loop(nrhs, 99, kappa, (60, 20), tol=1e-10, coact=co, bc=BC)
+    disconloops.batch_set(use_graph=graph)
+    print(f"[bench] batched noise loop, FP32 basis: times {h['times']} stats {h['stats']}", file=sys.stderr,
+          flush=True)
     s = disconloops.noise_loop(seq, 99, kappa, (60, 20), tol=1e-10, coact=co, bc=BC, method=1)
     per = s["times"][1] / max(1, seq - 1)
     print(f"[bench] sequential plain FP64: times {s['times']}", file=sys.stderr, flush=True)
@@ -330,6 +337,13 @@ def noise_section(nrhs=64, seq=3):
                         "total_s": round(r["times"][2], 3), "max_rel_residual": float(np.max(r["res_full"])),
                         "iterations_per_source": int(np.median(r["iters"][1:])) if nrhs > 1 else None,
                         "stats": r["stats"]},
+            "batched_fp32_basis": {"batched_s": round(h["times"][1], 3), "total_s": round(h["times"][2], 3),
+                                   "max_rel_residual": float(np.max(h["res_full"])),
+                                   "iterations_per_source": int(np.median(h["iters"][1:])) if nrhs > 1 else None,
+                                   "traces_max_rel_diff_vs_fp16_basis":
+                                       float(np.max(np.abs(h["traces"] - r["traces"])) / np.max(np.abs(r["traces"]))),
+                                   "speedup_vs_sequential_mixed": round(perm * (nrhs - 1) / h["times"][1], 2),
+                                   "stats": h["stats"]},
             "autotune": {"rhs_block": nb, "tile_order": order, "ms_per_hop": round(tms, 4)},
             "sequential_plain_fp64_s_per_source": round(per, 3),
             "sequential_mixed_s_per_source": round(perm, 3),
@@ -419,9 +433,7 @@ def main():
         solve = solve_section(args.train_steps)
     noise = None
     if not args.no_noise:
-        if args.no_graph:
-            disconloops.batch_set(use_graph=0)
-        noise = noise_section(args.noise_rhs)
+        noise = noise_section(args.noise_rhs, graph=0 if args.no_graph else 1)
     psolve = None
     if args.profile_solve:
         psolve = profile_solve_section(args.profile_solve)

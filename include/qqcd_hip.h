@@ -394,9 +394,11 @@ int qqcd_trace16(const double *eta_e, const double *eta_o, const double *x_e, co
  * tolerance of each FP32 inner solve; inner_cycles GMRES-Proj cycles per
  * inner solve; max_outer FP64 reliable updates before the single-RHS
  * fallback; op_prec 1 (FP32 links) / op_recon 12 or 18 for the inner
- * operator. */
+ * operator; basis_prec 2 (FP16, the default: unit vectors scaled by
+ * sqrt(12 stride), the operator reading an FP32 copy of the newest one) or
+ * 1 (FP32) for the inner GMRES's Krylov basis. */
 int qqcd_batch_set(int reorth, int use_graph, double inner_tol, int inner_cycles, int max_outer, int op_prec,
-                   int op_recon);
+                   int op_recon, int basis_prec);
 /* Statistics of the last batched solve. */
 int qqcd_batch_stats(int *outer, int *cycles, long *slot_matvecs, long *batched_matvecs, int *graph_replays,
                      int *fallback);

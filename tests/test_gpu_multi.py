@@ -179,11 +179,13 @@ def full_rel_res(n, bc, U, kappa, b, x):
     return np.linalg.norm(r) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("reorth,graph", [(1, 1), (0, 0), (2, 1)])
-def test_fermprop_multi(cfg1, reorth, graph):
+@pytest.mark.parametrize("reorth,graph,basis", [(1, 1, 1), (0, 0, 1), (2, 1, 1), (1, 1, 2), (0, 0, 2)])
+def test_fermprop_multi(cfg1, reorth, graph, basis):
+    # basis 2: the inner GMRES keeps its Krylov basis in FP16; the FP64
+    # reliable updates still bring every source to the same true residual
     n, bc, U, u, co, g = cfg1
     kappa = (0.150, 0.0)
-    disconloops.batch_set(reorth=reorth, use_graph=graph)
+    disconloops.batch_set(reorth=reorth, use_graph=graph, basis_prec=basis)
     # sources: the config-1 point source (origin, spin 1, colour 1; its reference
     # solution is a golden fixture) first and again as the fourth, Z4 noise otherwise
     srcs = []
@@ -199,7 +201,7 @@ def test_fermprop_multi(cfg1, reorth, graph):
             bo = lattice.new_spinor(n).ravel(order="F")
             disconloops.z4source(be, bo, 77, r, 1.0)
             srcs.append((be, bo))
-    print(f"[multi] fermprop_multi reorth={reorth} graph={graph}", flush=True)
+    print(f"[multi] fermprop_multi reorth={reorth} graph={graph} basis={basis}", flush=True)
     xe, xo, it, rr = disconloops.fermprop_multi([s[0] for s in srcs], [s[1] for s in srcs], kappa, co, bc, 1e-10,
                                                 (40, 10), u)
     st = disconloops.batch_stats()
@@ -259,6 +261,27 @@ def test_noise_loop_traces_and_graph(cfg1):
     # the reference's operators as nsaverage forms them (currents times kappa)
     assert rel(a["loops"][:, 0, :], wloop) < 1e-8
     assert np.all(a["loops"][:, 1:] == 0)
+
+
+def test_noise_loop_fp16_basis(cfg1):
+    # the batched loop with an FP16 Krylov basis against the FP32 one: the same
+    # sources solved to the same true residual, so traces agree to that order
+    n, bc, U, u, co, g = cfg1
+    kappa = (0.150, 0.0)
+    runs = {}
+    for basis in (1, 2):
+        disconloops.batch_set(basis_prec=basis)
+        print(f"[multi] noise_loop basis={basis}", flush=True)
+        runs[basis] = disconloops.noise_loop(6, 41, kappa, (40, 10), tol=1e-10, coact=co, bc=bc)
+    disconloops.batch_set()
+    a, b = runs[2], runs[1]
+    print("[multi] FP16 basis", a["times"], a["stats"], a["iters"], "FP32", b["times"], b["iters"], flush=True)
+    assert np.all(a["res_full"] < 1e-10)
+    assert rel(a["traces"], b["traces"]) < 1e-8
+    assert rel(a["loops"], b["loops"]) < 1e-8
+    with pytest.raises(RuntimeError):
+        disconloops.batch_set(basis_prec=0)
+    disconloops.batch_set()
 
 
 @pytest.mark.parametrize("method,scale", [(0, 1.0), (1, 2.0)])
