@@ -124,6 +124,42 @@ def cpu_sweep_baseline():
             "kind": "reference", "sample": f"{reps} sweeps of the reference Fortran on 8^3x16 ({dt*1e3:.1f} ms each)"}
 
 
+def cpu_rows_baselines():
+    """The reference's own smear and rnds (oracle/_ref, 8^3x16 build, one core), about two seconds each:
+    smearing in GB/s algorithmic (2400 B per site and iteration, as the GPU row counts it), draws per second."""
+    sys.path.insert(0, ROOT)
+    import oracle as O
+    from oracle import refdrv
+    n = (8, 8, 8, 16)
+    if not refdrv.available(n, solvers=True):
+        return None, None
+    R = refdrv.RefLattice(n, bc=(1, 1, 1, -1))
+    V = int(np.prod(n))
+    u = O.links_canon_to_ref(n, gauge.haar(n, 7))
+    rng = np.random.default_rng(5)
+    ge, go = rng.standard_normal(R.slen), rng.standard_normal(R.slen)
+    nsm = 20
+    R.smear(u, ge, go, 0.25, nsm)
+    reps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 2.0:
+        R.smear(u, ge, go, 0.25, nsm)
+        reps += 1
+    dt = (time.perf_counter() - t0) / (reps * nsm)
+    smear = {"value": round(2400 * V / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "reference",
+             "sample": f"{reps * nsm} smearing iterations of the reference Fortran on 8^3x16 ({dt*1e3:.2f} ms each)"}
+    R.lagfib_init(0, 1)
+    m = 1 << 22
+    R.rnds(m)
+    reps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 2.0:
+        R.rnds(m)
+        reps += 1
+    dt = (time.perf_counter() - t0) / reps
+    lag = {"value": round(m / dt / 1e9, 4), "unit": "Gdraws/s", "cores": 1, "kind": "reference",
+           "sample": f"{reps} calls of the reference's rnds for 2^22 numbers ({dt*1e3:.1f} ms each)"}
+    return smear, lag
+
+
 def rows_section(no_cpu=False):
     """The other rows of the path on 32^3x64: Gaussian smearing, the reference's lagged-Fibonacci
     stream and the reference's heatbath sweep drawing from it."""
@@ -160,6 +196,12 @@ def rows_section(no_cpu=False):
             sweep["cpu_baseline"] = cpu_sweep_baseline()
         except Exception as e:  # reported, never required
             sweep["cpu_baseline"] = {"value": None, "sample": f"unavailable: {e}"}
+        try:
+            smear["cpu_baseline"], lag["cpu_baseline"] = cpu_rows_baselines()
+        except Exception as e:  # reported, never required
+            smear["cpu_baseline"] = lag["cpu_baseline"] = {"value": None, "sample": f"unavailable: {e}"}
+        print(f"[bench] smear / lagfib cpu baselines {smear['cpu_baseline']} {lag['cpu_baseline']}",
+              file=sys.stderr, flush=True)
     print(f"[bench] sweep {sweep}", file=sys.stderr, flush=True)
     return {"smear": smear, "lagfib": lag, "sweep_reference_order": sweep}
 
