@@ -230,7 +230,45 @@ def heatbath_field(seed=2024, sweeps=200):
     return co, plaq
 
 
-def solve_section(train_steps=1500):
+def cpu_solve_baseline():
+    """The reference's own gmresdr (oracle/_ref, 8^3x16 build, one core) on the config-1 field (the golden
+    fixture), Wilson kappa 0.150, point source, GMRES-DR(40,16) to 1e-10, repeated for about three seconds:
+    even-site Schur iterations x sites per second, the rate the GPU rows are compared with.  Run in a child
+    process: the reference's gmresdr writes its own progress lines to standard output."""
+    import subprocess
+    code = "import json, sys, bench; sys.stderr.write('CPU_SOLVE ' + json.dumps(bench._cpu_solve_child()) + '\\n')"
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    lines = [ln for ln in out.stderr.splitlines() if ln.startswith("CPU_SOLVE ")]
+    if out.returncode != 0 or not lines:
+        raise RuntimeError(out.stderr.strip().splitlines()[-1] if out.stderr.strip() else "child failed")
+    return json.loads(lines[-1][len("CPU_SOLVE "):])
+
+
+def _cpu_solve_child():
+    sys.path.insert(0, ROOT)
+    import oracle as O
+    from oracle import refdrv
+    n = (8, 8, 8, 16)
+    if not refdrv.available(n, solvers=True):
+        return None
+    R = refdrv.RefLattice(n, bc=(1, 1, 1, -1))
+    g = np.load(os.path.join(ROOT, "tests", "golden", "golden_8x8x8x16.npz"))
+    u = O.links_canon_to_ref(n, gauge.reconstruct12(g["links12"]))
+    co = O.coact_array()
+    be, _ = R.pointsource((1, 1, 1, 1), 1, 1)
+    its, reps, t0 = 0, 0, time.perf_counter()
+    while time.perf_counter() - t0 < 3.0 or reps == 0:
+        _, it = R.solve(5, be, u, co, (0.150, 0.0), nk=(40, 16), resmax=1e-10)
+        its += it
+        reps += 1
+    dt = time.perf_counter() - t0
+    vh = int(np.prod(n)) // 2
+    return {"value": round(vh * its / dt / 1e6, 3), "unit": "M site-iterations/s", "cores": 1, "kind": "reference",
+            "sample": f"{reps} GMRES-DR(40,16) solves of the reference Fortran on 8^3x16, {its // reps} iterations "
+                      f"each ({dt / reps:.2f} s each)"}
+
+
+def solve_section(train_steps=1500, no_cpu=False):
     """Time to a 1e-10 true residual on a 32^3x64 beta=6.0 heatbath field:
     plain FP64 GMRES-DR, the mixed solver, and the mixed solver with the
     learned preconditioner trained on 16^3x32 configurations (Wilson kappa in
@@ -258,7 +296,18 @@ def solve_section(train_steps=1500):
     tmixed = bench_solve(1, tm, 1, 12)
     print(f"[bench] twisted mixed FP32: {tmixed}", file=sys.stderr, flush=True)
     best = min((mixed, mixed16, mixed16h, lrn), key=lambda r: r["seconds"])
+    vh = int(np.prod(N)) // 2
+    rates = {"plain_fp64": round(vh * plain["iterations"] / plain["seconds"] / 1e6, 1),
+             "mixed_fp16_operator_fp16_basis": round(vh * mixed16h["iterations"] / mixed16h["seconds"] / 1e6, 1)}
+    cpu = None
+    if not no_cpu:
+        try:
+            cpu = cpu_solve_baseline()
+        except Exception as e:  # reported, never required
+            cpu = {"value": None, "sample": f"unavailable: {e}"}
+        print(f"[bench] solve cpu baseline {cpu}", file=sys.stderr, flush=True)
     return {"lattice": list(N), "beta": 6.0, "heatbath_sweeps": 200, "plaquette": round(plaq, 5),
+            "M_site_iterations_per_s": rates, "cpu_baseline": cpu,
             "source": "point, origin, spin 1 colour 1",
             "target": "||b - M x|| / ||b|| < 1e-10 (FP64, full system)",
             "wilson_kappa_0.1565": {"plain_fp64_gmresdr_40_16": plain, "mixed_fp32": mixed,
@@ -472,7 +521,7 @@ def main():
 
     solve = None
     if not args.no_solve:
-        solve = solve_section(args.train_steps)
+        solve = solve_section(args.train_steps, args.no_cpu)
     noise = None
     if not args.no_noise:
         noise = noise_section(args.noise_rhs, graph=0 if args.no_graph else 1)
