@@ -14,7 +14,10 @@ pseudo-heatbath sweeps from the config-1 field: stream states, per-set link
 sums and the links of block 1 after each sweep); with ``smear`` only
 tests/golden/golden_smear_8x8x8x16.npz (the reference's smear, twMshift and
 gamma5mult: timeslices 0 and nt-1 of each output, per-timeslice norms and
-sums).
+sums); with ``noiseloop`` only tests/golden/golden_noiseloop_8x8x8x16.npz
+(the reference's own per-source loop z4source -> fermprop -> nsaverage from
+initlagfib(0, 5) on the config-1 field: the mean of nsaverage's momentum-1
+operators over the sources and the lagfib state after the loop).
 
 Inputs are regenerated from seeds by the tests (spinors: numpy
 default_rng(seed) Gaussians; links: baylorlqcd_amd.gauge.haar), except the
@@ -396,9 +399,40 @@ def smear():
     print("wrote golden_smear_8x8x8x16.npz")
 
 
+NOISE_CASE = dict(iseed=5, nrhs=3, kappa=0.150, nk=(40, 10), resmax=1e-12)
+
+
+def noiseloop():
+    """The reference's own noise loop on the config-1 field: initlagfib(0, iseed), then per
+    source z4source, fermprop (GMRES-DR) and nsaverage, averaged over the sources (what a
+    testFUNC-style loop accumulates), and the lagfib state after the loop."""
+    n = (8, 8, 8, 16)
+    refdrv.build(n, solvers=True)
+    g = np.load(os.path.join(HERE, "golden_8x8x8x16.npz"))
+    u = O.links_canon_to_ref(n, gauge.reconstruct12(g["links12"]))
+    co = O.coact_array()
+    cs = NOISE_CASE
+    R = refdrv.RefLattice(n, bc=(1, 1, 1, -1))
+    R.lagfib_init(0, cs["iseed"])
+    J = np.zeros((n[3], 5, 9), dtype=np.complex128)
+    for r in range(cs["nrhs"]):
+        be, bo = R.z4source()
+        xe, xo = R.fermprop(be, bo, u, co, (cs["kappa"], 0.0), 5, nk=cs["nk"], resmax=cs["resmax"])
+        Jn = R.nsaverage(u, be, bo, xe, xo, cs["kappa"], co).reshape(9, 5, n[3], 2)
+        J += (Jn[..., 0] + 1j * Jn[..., 1]).transpose(2, 1, 0)
+    J /= cs["nrhs"]
+    iw, jr, kr = R.lagfib_get()
+    np.savez_compressed(os.path.join(HERE, "golden_noiseloop_8x8x8x16.npz"), nsaverage_mean=J[:, 0, :],
+                        lagfib_after_iw=iw, lagfib_after_jrkr=np.array([jr, kr], dtype=np.int32),
+                        case=np.array([cs["iseed"], cs["nrhs"]], dtype=np.int32), kappa=np.float64(cs["kappa"]))
+    print("wrote golden_noiseloop_8x8x8x16.npz", np.abs(J[:, 0, :]).max())
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["loops"]:
         loops()
+    elif sys.argv[1:] == ["noiseloop"]:
+        noiseloop()
     elif sys.argv[1:] == ["io"]:
         io()
     elif sys.argv[1:] == ["lagfib"]:

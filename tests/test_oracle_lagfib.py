@@ -71,3 +71,17 @@ def test_recurrence():
     w = np.roll(LF.init(0, 3)[0].astype(np.int64), 0)
     y = np.concatenate([w, raw])
     assert np.array_equal(y[521:], y[:-521] ^ y[521 - 32:-32])
+
+
+def test_noiseloop_fixture_stream():
+    # the reference's noise-loop fixture consumed exactly nrhs z4source draws of
+    # the stream of initlagfib(0, iseed): the oracle lands on the state it recorded
+    gl = np.load(os.path.join(ROOT, "tests", "golden", "golden_noiseloop_8x8x8x16.npz"))
+    iseed, nrhs = (int(v) for v in gl["case"])
+    st = LF.init(0, iseed)
+    for _ in range(nrhs):
+        _, _, st = LF.source(N, st, 0)
+    iw, jr, kr = st
+    assert np.array_equal(np.asarray(iw, dtype=np.int64) & 0xFFFFFFFF,
+                          np.asarray(gl["lagfib_after_iw"], dtype=np.int64) & 0xFFFFFFFF)
+    assert (int(jr), int(kr)) == tuple(int(v) for v in gl["lagfib_after_jrkr"])
