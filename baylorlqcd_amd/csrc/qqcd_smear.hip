@@ -9,7 +9,8 @@
 // across the spatial edge.  The increment of a direction is summed as the
 // reference sums it: (acc + forward) + backward, mu = 1, 2, 3.
 //
-// One lane per site of the output checkerboard; FP64 throughout.  Per site
+// One lane per site of the output checkerboard, each XCD on one contiguous
+// range of sites; FP64 throughout.  Per site
 // and iteration: the spinor at x and at its 6 spatial neighbours (7 x 192 B),
 // 6 links (6 x 144 B) and the result (192 B): 2400 B.
 #include <stdexcept>
@@ -51,7 +52,14 @@ __device__ __forceinline__ void sm_mv(const double2 *U, long us, int par, int mu
 
 __global__ void __launch_bounds__(128) k_smear(SmArgs a, const double2 *U, const double2 *same,
                                                const double2 *other, double2 *out) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+  // contiguous run of sites so the neighbours (up to n0 n1 / 2 sites away)
+  // are fetched into its own L2
+  const int nb = gridDim.x, per = (nb + 7) / 8;
+  const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+  const int full = nb - 8 * (per - 1);  // XCDs that get `per` workgroups
+  const int blk = xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
+  const long i = (long)blk * blockDim.x + threadIdx.x;
   if (i >= a.Vh) return;
   const int *n = a.n;
   int x[4];
