@@ -165,16 +165,24 @@ def rows_section(no_cpu=False):
     stream and the reference's heatbath sweep drawing from it."""
     V = int(np.prod(N))
     L = _lib.lib()
+    # HBM bytes per launch from the hardware counters (profiles/traffic_smear_lagfib_32x32x32x64.md)
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "traffic_smear_lagfib_32x32x32x64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)
     ms = np.zeros(1)
     _lib.check(L.qqcd_bench_smear(3, 20, _lib.dp(ms)))
     g = 2400 * V / (ms[0] * 1e-3) / 1e9
     smear = {"us_per_iteration": round(ms[0] * 1e3, 1), "bytes_per_site": 2400, "GB/s": round(g, 1),
-             "frac": round(g / HBM_PEAK_GBS, 3)}
+             "frac": round(g / HBM_PEAK_GBS, 3),
+             "traffic_per_launch": traffic.get("k_smear", {}).get("bytes_per_launch")}
     print(f"[bench] smear {smear}", file=sys.stderr, flush=True)
     m = 2048 * 32768
     _lib.check(L.qqcd_bench_lagfib(m, 10, 0, _lib.dp(ms)))
     lag = {"draws": m, "us": round(ms[0] * 1e3, 1), "Gdraws/s": round(m / (ms[0] * 1e-3) / 1e9, 2),
-           "GB/s_written": round(4 * m / (ms[0] * 1e-3) / 1e9, 1), "frac": round(4 * m / (ms[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
+           "GB/s_written": round(4 * m / (ms[0] * 1e-3) / 1e9, 1), "frac": round(4 * m / (ms[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
+           "traffic_per_launch": traffic.get("k_lag_gen", {}).get("bytes_per_launch")}
     print(f"[bench] lagfib {lag}", file=sys.stderr, flush=True)
     u = diracops.gauge_get_ref()
     co = np.zeros(64)
