@@ -44,7 +44,11 @@
 namespace qqcd {
 
 static constexpr int HB_TB = 256;
-static constexpr int HB_PB = 6;  // Kennedy-Pendleton passes per batch between host checks (even)
+// Kennedy-Pendleton passes per batch between host checks (even).  At beta = 6
+// a pass accepts ~97% of the sites that draw, so nvhalf = 65536 sites need
+// about four passes and a second batch is rare; passes with nothing left to
+// draw still cost a launch.
+static constexpr int HB_PB = 4;
 // su2up's twoPi: a single-precision literal held in a double
 static constexpr double HB_TWOPI = (double)6.283185307179586f;
 
