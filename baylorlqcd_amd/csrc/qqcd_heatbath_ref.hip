@@ -260,32 +260,20 @@ __global__ void __launch_bounds__(HB_TB) k_hb_staple(HbG G, const double *u, int
   for (int k = 0; k < 18; k++) C[(long)k * G.nvh + i] = c[k];
 }
 
-// rows (p, q) of subgroup isu2 as offsets into the 18 reals; subgroup 2 flips z3
-__device__ __forceinline__ void hb_rows(int isu2, int &P, int &Q, double &s3) {
-  P = isu2 == 2 ? 6 : 0;
-  Q = isu2 == 1 ? 6 : 12;
-  s3 = isu2 == 2 ? -1.0 : 1.0;
-}
 
 // phbsub1-3: z of the link's rows (p, q) against the staple; dk = 1/|z|; the
 // site starts the Kennedy-Pendleton passes not accepted
-__global__ void __launch_bounds__(HB_TB) k_hb_proj(HbG G, const double *u, const double *C, int ieo, int ibl,
-                                                   int mu, int isu2, double *Z, double *DK, int *ID, double *A0,
-                                                   int *cnt) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  // the first pass draws for every site of the workgroup
-  if (threadIdx.x == 0) cnt[blockIdx.x] = (int)min((long)HB_TB, G.nvh - (long)blockIdx.x * HB_TB);
-  if (i >= G.nvh) return;
-  int P, Q;
-  double s3;
-  hb_rows(isu2, P, Q, s3);
-  const double *up = u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl)));
-  double a[18], c[18];
+// phbsub of site i for subgroup ISU2 from its link a (18 reals) and staple C.
+// Rows (p, q) of the subgroup are the offsets P, Q into the 18 reals; subgroup
+// 2 flips z3
+template <int ISU2>
+__device__ __forceinline__ void hb_proj_site(const HbG &G, const double a[18], const double *C, long i, double *Z,
+                                             double *DK, int *ID, double *A0) {
+  constexpr int P = ISU2 == 2 ? 6 : 0, Q = ISU2 == 1 ? 6 : 12;
+  constexpr double s3 = ISU2 == 2 ? -1.0 : 1.0;
+  double c[18];
 #pragma unroll
-  for (int k = 0; k < 18; k++) {
-    a[k] = up[k];
-    c[k] = C[(long)k * G.nvh + i];
-  }
+  for (int k = 0; k < 18; k++) c[k] = C[(long)k * G.nvh + i];
   double z1 = a[P] * c[P];
   for (int k = 1; k < 6; k++) z1 = z1 + a[P + k] * c[P + k];
   for (int k = 0; k < 6; k++) z1 = z1 + a[Q + k] * c[Q + k];
@@ -320,6 +308,21 @@ __global__ void __launch_bounds__(HB_TB) k_hb_proj(HbG G, const double *u, const
   DK[i] = 1.0 / sqrt(z1 * z1 + z2 * z2 + z3 * z3 + z4 * z4);
   ID[i] = 0;
   A0[i] = 0.0;
+}
+
+
+template <int ISU2>
+__global__ void __launch_bounds__(HB_TB) k_hb_proj(HbG G, const double *u, const double *C, int ieo, int ibl,
+                                                   int mu, double *Z, double *DK, int *ID, double *A0, int *cnt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // the first pass draws for every site of the workgroup
+  if (threadIdx.x == 0) cnt[blockIdx.x] = (int)min((long)HB_TB, G.nvh - (long)blockIdx.x * HB_TB);
+  if (i >= G.nvh) return;
+  const double *up = u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl)));
+  double a[18];
+#pragma unroll
+  for (int k = 0; k < 18; k++) a[k] = up[k];
+  hb_proj_site<ISU2>(G, a, C, i, Z, DK, ID, A0);
 }
 
 // status of a batch of passes, read back with one copy: the running stream
@@ -407,16 +410,13 @@ __global__ void __launch_bounds__(HB_TB) k_hb_try(long nvh, int nblk, const uint
   }
 }
 
-// su2up's sphere (:300-321) and sprd1 (:327-467): rows (p, q) of the link
-// become R (rows p, q) with R = [[z1 + i z4, s3 z3 + i z2], [-s3 z3 + i z2, z1 - i z4]]
-__global__ void __launch_bounds__(HB_TB) k_hb_hyper(HbG G, double *u, int ieo, int ibl, int mu, int isu2,
-                                                    const uint32_t *y, long pos, const double *Z, const double *DK,
-                                                    const double *A0) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= G.nvh) return;
-  int P, Q;
-  double s3;
-  hb_rows(isu2, P, Q, s3);
+// su2up's sphere and sprd1 for site i: rows (p, q) of its link (in `up`,
+// real k at up[k]) become R times those rows
+template <int ISU2>
+__device__ __forceinline__ void hb_hyper_site(const HbG &G, double *up, long i, const uint32_t *y, long pos,
+                                              const double *Z, const double *DK, const double *A0) {
+  constexpr int P = ISU2 == 2 ? 6 : 0, Q = ISU2 == 1 ? 6 : 12;
+  constexpr double s3 = ISU2 == 2 ? -1.0 : 1.0;
   const double r0 = hb_rn(y, pos + 2 * i), r1 = hb_rn(y, pos + 2 * i + 1);
   const double a0 = A0[i], dk = DK[i];
   const double rad = 1.0 - a0 * a0;
@@ -432,7 +432,6 @@ __global__ void __launch_bounds__(HB_TB) k_hb_hyper(HbG G, double *u, int ieo, i
   const double n4 = dk * (-(a0 * z4) + a1 * z3 - a2 * z2 + a3 * z1);
   // sprd1 with (z1, z2, z3, z4) = (n1, n2, s3 n3, n4): subgroup 2 uses -z3 where 1 and 3 use z3
   const double t3 = s3 * n3;
-  double *up = u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl)));
   for (int c = 0; c < 3; c++) {
     const double pr = up[P + 2 * c], pi = up[P + 2 * c + 1], qr = up[Q + 2 * c], qi = up[Q + 2 * c + 1];
     const double yp_r = n1 * pr - n4 * pi + t3 * qr - n2 * qi;
@@ -444,6 +443,40 @@ __global__ void __launch_bounds__(HB_TB) k_hb_hyper(HbG G, double *u, int ieo, i
     up[Q + 2 * c] = yq_r;
     up[Q + 2 * c + 1] = yq_i;
   }
+}
+
+// su2up's sphere (:300-321) and sprd1 (:327-467): rows (p, q) of the link
+// become R (rows p, q) with R = [[z1 + i z4, s3 z3 + i z2], [-s3 z3 + i z2, z1 - i z4]]
+template <int ISU2>
+__global__ void __launch_bounds__(HB_TB) k_hb_hyper(HbG G, double *u, int ieo, int ibl, int mu, const uint32_t *y,
+                                                    long pos, const double *Z, const double *DK, const double *A0) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.nvh) return;
+  hb_hyper_site<ISU2>(G, u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl))), i, y, pos, Z, DK, A0);
+}
+
+// the update of subgroup isu2 and, on the updated link, the projection of
+// subgroup isu2 + 1 (one launch instead of two; the same arithmetic in the
+// same order, the link held in registers between them)
+template <int ISU2>
+__global__ void __launch_bounds__(HB_TB) k_hb_hyper_proj(HbG G, double *u, const double *C, int ieo, int ibl,
+                                                         int mu, const uint32_t *y, long pos, double *Z, double *DK,
+                                                         int *ID, double *A0, int *cnt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (threadIdx.x == 0) cnt[blockIdx.x] = (int)min((long)HB_TB, G.nvh - (long)blockIdx.x * HB_TB);
+  if (i >= G.nvh) return;
+  double *up = u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl)));
+  double a[18];
+#pragma unroll
+  for (int k = 0; k < 18; k++) a[k] = up[k];
+  hb_hyper_site<ISU2>(G, a, i, y, pos, Z, DK, A0);
+  constexpr int P = ISU2 == 2 ? 6 : 0, Q = ISU2 == 1 ? 6 : 12;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    up[P + k] = a[P + k];
+    up[Q + k] = a[Q + k];
+  }
+  hb_proj_site<ISU2 + 1>(G, a, C, i, Z, DK, ID, A0);
 }
 
 namespace {
@@ -566,9 +599,12 @@ void heatbath_reference(Context &c, const int gaction[4], const double *coact, i
           hipLaunchKernelGGL(k_hb_staple, grid, blk, 0, c.stream, G, du.p, ieo, ibl, mu, K, C.p);
           QQCD_HIP(hipGetLastError());
           for (int isu2 = 1; isu2 <= 3; isu2++) {
-            hipLaunchKernelGGL(k_hb_proj, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, isu2, Z.p, DK.p,
-                               ID.p, A0.p, cnt.p);
-            QQCD_HIP(hipGetLastError());
+            // subgroups 2 and 3 were projected by the update kernel of the one before
+            if (isu2 == 1) {
+              hipLaunchKernelGGL(k_hb_proj<1>, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, Z.p, DK.p, ID.p,
+                                 A0.p, cnt.p);
+              QQCD_HIP(hipGetLastError());
+            }
             for (int batch = 0;; batch++) {
               // a site whose staple projects to zero never accepts (the reference would loop forever)
               if (batch == 200) throw std::runtime_error("sweep: Kennedy-Pendleton passes did not end (zero staple?)");
@@ -587,8 +623,16 @@ void heatbath_reference(Context &c, const int gaction[4], const double *coact, i
               if (rem == 0) break;
             }
             S.ensure(pos, 2 * G.nvh, chunk);
-            hipLaunchKernelGGL(k_hb_hyper, grid, blk, 0, c.stream, G, du.p, ieo, ibl, mu, isu2, S.buf,
-                               pos - S.origin, Z.p, DK.p, A0.p);
+            const long ypos = pos - S.origin;
+            if (isu2 == 1)
+              hipLaunchKernelGGL(k_hb_hyper_proj<1>, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, S.buf, ypos,
+                                 Z.p, DK.p, ID.p, A0.p, cnt.p);
+            else if (isu2 == 2)
+              hipLaunchKernelGGL(k_hb_hyper_proj<2>, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, S.buf, ypos,
+                                 Z.p, DK.p, ID.p, A0.p, cnt.p);
+            else
+              hipLaunchKernelGGL(k_hb_hyper<3>, grid, blk, 0, c.stream, G, du.p, ieo, ibl, mu, S.buf, ypos, Z.p,
+                                 DK.p, A0.p);
             QQCD_HIP(hipGetLastError());
             pos += 2 * G.nvh;
           }
