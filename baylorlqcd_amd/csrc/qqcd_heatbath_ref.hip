@@ -267,8 +267,8 @@ __global__ void __launch_bounds__(HB_TB) k_hb_staple(HbG G, const double *u, int
 // Rows (p, q) of the subgroup are the offsets P, Q into the 18 reals; subgroup
 // 2 flips z3
 template <int ISU2>
-__device__ __forceinline__ void hb_proj_site(const HbG &G, const double a[18], const double *C, long i, double *Z,
-                                             double *DK, int *ID, double *A0) {
+__device__ __forceinline__ double hb_proj_site(const HbG &G, const double a[18], const double *C, long i, double *Z,
+                                               double *DK, int *ID, double *A0) {
   constexpr int P = ISU2 == 2 ? 6 : 0, Q = ISU2 == 1 ? 6 : 12;
   constexpr double s3 = ISU2 == 2 ? -1.0 : 1.0;
   double c[18];
@@ -305,25 +305,14 @@ __device__ __forceinline__ void hb_proj_site(const HbG &G, const double a[18], c
   Z[G.nvh + i] = z2;
   Z[2 * G.nvh + i] = z3;
   Z[3 * G.nvh + i] = z4;
-  DK[i] = 1.0 / sqrt(z1 * z1 + z2 * z2 + z3 * z3 + z4 * z4);
+  const double dk = 1.0 / sqrt(z1 * z1 + z2 * z2 + z3 * z3 + z4 * z4);
+  DK[i] = dk;
   ID[i] = 0;
   A0[i] = 0.0;
+  return dk;
 }
 
 
-template <int ISU2>
-__global__ void __launch_bounds__(HB_TB) k_hb_proj(HbG G, const double *u, const double *C, int ieo, int ibl,
-                                                   int mu, double *Z, double *DK, int *ID, double *A0, int *cnt) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  // the first pass draws for every site of the workgroup
-  if (threadIdx.x == 0) cnt[blockIdx.x] = (int)min((long)HB_TB, G.nvh - (long)blockIdx.x * HB_TB);
-  if (i >= G.nvh) return;
-  const double *up = u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl)));
-  double a[18];
-#pragma unroll
-  for (int k = 0; k < 18; k++) a[k] = up[k];
-  hb_proj_site<ISU2>(G, a, C, i, Z, DK, ID, A0);
-}
 
 // status of a batch of passes, read back with one copy: the running stream
 // position, per pass the sites that drew, and the sites that accepted in the
@@ -336,6 +325,63 @@ struct HbStatus {
 static_assert(HB_PB > 1 && HB_PB % 2 == 0, "a batch starts on the counts the previous one left in cnt[0, nblk)");
 
 __device__ __forceinline__ double hb_rn(const uint32_t *y, long j) { return (double)(int)y[j] * LAG_FNORM; }
+
+// the first Kennedy-Pendleton pass of site i: every site draws, in site
+// order, so its 4 draws start at base + 4 i; the test and a0 exactly as
+// k_hb_try forms them.  1 if the site accepts.
+__device__ __forceinline__ int hb_first_pass(const uint32_t *y, long base, long i, double dk, int *ID, double *A0) {
+  const long j = base + 4 * i;
+  const double r0 = hb_rn(y, j), r1 = hb_rn(y, j + 1), r2 = hb_rn(y, j + 2), r3 = hb_rn(y, j + 3);
+  const double tt = cos(HB_TWOPI * r2);
+  const double d2 = -(log(r0) + log(r1) * tt * tt) * dk * 3.0;
+  if (r3 * r3 <= (1.0 - 0.5 * d2)) {
+    ID[i] = 1;
+    A0[i] = 1.0 - d2;
+    return 1;
+  }
+  return 0;
+}
+
+// after the first pass: the workgroup's sites still drawing (the second
+// pass's counts, cnt slot 1) and, from workgroup 0, where the second pass
+// starts (dpos[1]) and the batch status; every thread of the workgroup calls it
+__device__ __forceinline__ void hb_first_pass_counts(long nvh, int acc, int *cnt_next, long *dpos, long base,
+                                                     HbStatus *stt) {
+  __shared__ int acc_s;
+  if (threadIdx.x == 0) acc_s = 0;
+  __syncthreads();
+  if (acc) atomicAdd(&acc_s, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int drew = (int)min((long)HB_TB, nvh - (long)blockIdx.x * HB_TB);
+    cnt_next[blockIdx.x] = drew - acc_s;
+    if (blockIdx.x == 0) {
+      dpos[1] = base + 4 * nvh;
+      stt->tot[0] = nvh;
+      stt->pos = base + 4 * nvh;
+      stt->acc_last = 0;
+    }
+  }
+}
+
+template <int ISU2>
+__global__ void __launch_bounds__(HB_TB) k_hb_proj(HbG G, const double *u, const double *C, int ieo, int ibl,
+                                                   int mu, double *Z, double *DK, int *ID, double *A0,
+                                                   const uint32_t *y, long base0, int *cnt_next, long *dpos,
+                                                   HbStatus *stt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int acc = 0;
+  if (i < G.nvh) {
+    const double *up = u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl)));
+    double a[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) a[k] = up[k];
+    const double dk = hb_proj_site<ISU2>(G, a, C, i, Z, DK, ID, A0);
+    acc = hb_first_pass(y, base0, i, dk, ID, A0);
+  }
+  hb_first_pass_counts(G.nvh, acc, cnt_next, dpos, base0, stt);
+}
+
 
 // one Kennedy-Pendleton pass (su2up :270-297) over the sites not yet
 // accepted.  cnt_in[b]: the sites of workgroup b that draw in this pass.
@@ -455,28 +501,33 @@ __global__ void __launch_bounds__(HB_TB) k_hb_hyper(HbG G, double *u, int ieo, i
   hb_hyper_site<ISU2>(G, u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl))), i, y, pos, Z, DK, A0);
 }
 
-// the update of subgroup isu2 and, on the updated link, the projection of
-// subgroup isu2 + 1 (one launch instead of two; the same arithmetic in the
-// same order, the link held in registers between them)
+// the update of subgroup ISU2 and, on the updated link, the projection and
+// first Kennedy-Pendleton pass of subgroup ISU2 + 1 (one launch instead of
+// three; the same arithmetic in the same order, the link held in registers).
+// The update draws 2 per site from pos, the pass 4 per site from base0.
 template <int ISU2>
 __global__ void __launch_bounds__(HB_TB) k_hb_hyper_proj(HbG G, double *u, const double *C, int ieo, int ibl,
                                                          int mu, const uint32_t *y, long pos, double *Z, double *DK,
-                                                         int *ID, double *A0, int *cnt) {
+                                                         int *ID, double *A0, long base0, int *cnt_next, long *dpos,
+                                                         HbStatus *stt) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (threadIdx.x == 0) cnt[blockIdx.x] = (int)min((long)HB_TB, G.nvh - (long)blockIdx.x * HB_TB);
-  if (i >= G.nvh) return;
-  double *up = u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl)));
-  double a[18];
+  int acc = 0;
+  if (i < G.nvh) {
+    double *up = u + 18L * (i + G.ntot * (mu + 4L * (ieo + 2L * ibl)));
+    double a[18];
 #pragma unroll
-  for (int k = 0; k < 18; k++) a[k] = up[k];
-  hb_hyper_site<ISU2>(G, a, i, y, pos, Z, DK, A0);
-  constexpr int P = ISU2 == 2 ? 6 : 0, Q = ISU2 == 1 ? 6 : 12;
+    for (int k = 0; k < 18; k++) a[k] = up[k];
+    hb_hyper_site<ISU2>(G, a, i, y, pos, Z, DK, A0);
+    constexpr int P = ISU2 == 2 ? 6 : 0, Q = ISU2 == 1 ? 6 : 12;
 #pragma unroll
-  for (int k = 0; k < 6; k++) {
-    up[P + k] = a[P + k];
-    up[Q + k] = a[Q + k];
+    for (int k = 0; k < 6; k++) {
+      up[P + k] = a[P + k];
+      up[Q + k] = a[Q + k];
+    }
+    const double dk = hb_proj_site<ISU2 + 1>(G, a, C, i, Z, DK, ID, A0);
+    acc = hb_first_pass(y, base0, i, dk, ID, A0);
   }
-  hb_proj_site<ISU2 + 1>(G, a, C, i, Z, DK, ID, A0);
+  hb_first_pass_counts(G.nvh, acc, cnt_next, dpos, base0, stt);
 }
 
 namespace {
@@ -599,18 +650,25 @@ void heatbath_reference(Context &c, const int gaction[4], const double *coact, i
           hipLaunchKernelGGL(k_hb_staple, grid, blk, 0, c.stream, G, du.p, ieo, ibl, mu, K, C.p);
           QQCD_HIP(hipGetLastError());
           for (int isu2 = 1; isu2 <= 3; isu2++) {
-            // subgroups 2 and 3 were projected by the update kernel of the one before
+            // the projection and first Kennedy-Pendleton pass of subgroup 1 (those
+            // of subgroups 2 and 3 run in the update kernel of the subgroup
+            // before); the window is made to cover that pass and the first batch,
+            // so it does not move before the passes that start from dpos
             if (isu2 == 1) {
+              S.ensure(pos, 4 * G.nvh + need_pass, chunk);
               hipLaunchKernelGGL(k_hb_proj<1>, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, Z.p, DK.p, ID.p,
-                                 A0.p, cnt.p);
+                                 A0.p, S.buf, pos - S.origin, cnt.p + nblk, dpos.p, dst.p);
               QQCD_HIP(hipGetLastError());
+              pos += 4 * G.nvh;
             }
             for (int batch = 0;; batch++) {
               // a site whose staple projects to zero never accepts (the reference would loop forever)
               if (batch == 200) throw std::runtime_error("sweep: Kennedy-Pendleton passes did not end (zero staple?)");
-              S.ensure(pos, need_pass, chunk);
-              // HB_PB is even: every batch starts on the counts in cnt[0, nblk)
-              for (int p = 0; p < HB_PB; p++) {
+              S.ensure(pos, need_pass, chunk);  // already resident for the first batch
+              // the first batch goes on after the first pass (its counts in
+              // cnt[nblk, 2 nblk)); HB_PB is even, so every later batch starts on
+              // the counts in cnt[0, nblk)
+              for (int p = batch == 0 ? 1 : 0; p < HB_PB; p++) {
                 hipLaunchKernelGGL(k_hb_try, grid, blk, 0, c.stream, G.nvh, nblk, S.buf, ID.p, DK.p, A0.p,
                                    cnt.p + (p & 1) * nblk, cnt.p + ((p + 1) & 1) * nblk, dpos.p, pos - S.origin,
                                    p, dst.p);
@@ -622,19 +680,25 @@ void heatbath_reference(Context &c, const int gaction[4], const double *coact, i
               const long rem = hst->tot[HB_PB - 1] > 0 ? hst->tot[HB_PB - 1] - hst->acc_last : 0;
               if (rem == 0) break;
             }
-            S.ensure(pos, 2 * G.nvh, chunk);
-            const long ypos = pos - S.origin;
-            if (isu2 == 1)
-              hipLaunchKernelGGL(k_hb_hyper_proj<1>, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, S.buf, ypos,
-                                 Z.p, DK.p, ID.p, A0.p, cnt.p);
-            else if (isu2 == 2)
-              hipLaunchKernelGGL(k_hb_hyper_proj<2>, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, S.buf, ypos,
-                                 Z.p, DK.p, ID.p, A0.p, cnt.p);
-            else
-              hipLaunchKernelGGL(k_hb_hyper<3>, grid, blk, 0, c.stream, G, du.p, ieo, ibl, mu, S.buf, ypos, Z.p,
-                                 DK.p, A0.p);
-            QQCD_HIP(hipGetLastError());
-            pos += 2 * G.nvh;
+            if (isu2 < 3) {
+              // the update (2 draws per site) and the next subgroup's first pass (4 per site)
+              S.ensure(pos, 6 * G.nvh + need_pass, chunk);
+              const long ypos = pos - S.origin;
+              if (isu2 == 1)
+                hipLaunchKernelGGL(k_hb_hyper_proj<1>, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, S.buf,
+                                   ypos, Z.p, DK.p, ID.p, A0.p, ypos + 2 * G.nvh, cnt.p + nblk, dpos.p, dst.p);
+              else
+                hipLaunchKernelGGL(k_hb_hyper_proj<2>, grid, blk, 0, c.stream, G, du.p, C.p, ieo, ibl, mu, S.buf,
+                                   ypos, Z.p, DK.p, ID.p, A0.p, ypos + 2 * G.nvh, cnt.p + nblk, dpos.p, dst.p);
+              QQCD_HIP(hipGetLastError());
+              pos += 6 * G.nvh;
+            } else {
+              S.ensure(pos, 2 * G.nvh, chunk);
+              hipLaunchKernelGGL(k_hb_hyper<3>, grid, blk, 0, c.stream, G, du.p, ieo, ibl, mu, S.buf, pos - S.origin,
+                                 Z.p, DK.p, A0.p);
+              QQCD_HIP(hipGetLastError());
+              pos += 2 * G.nvh;
+            }
           }
         }
   QQCD_HIP(hipMemcpyAsync(u_host, du.p, sizeof(double) * ulen, hipMemcpyDeviceToHost, c.stream));
