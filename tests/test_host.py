@@ -150,3 +150,13 @@ def test_geometry_check_rejects_multi_process():
         lattice.check_geometry(ldiv=[True, False, False, False])
     with pytest.raises(_lib.QQCDError):
         lattice.check_geometry(vecbl=vb[::-1])
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the bench measures one GPU; a request for more exits with a message (status 2)
+    import subprocess
+    import sys
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], cwd=ROOT,
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 2
+    assert "one GPU" in out.stderr and out.stdout.strip() == ""
