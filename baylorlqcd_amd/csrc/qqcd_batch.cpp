@@ -208,8 +208,7 @@ void Batch::arnoldi_step(int j) {
   MultiSpinor &w = half() ? Wf->m : V[j + 1];
   A32(half() ? Qf->m : V[j], w, nin);
   if (half()) {
-    mb.dot(j + 1, V.data(), w, nin, h1.p);
-    mb.dot(1, &w, w, nin, h1.p + (size_t)2 * (j + 1) * cap);
+    mb.dot(j + 1, V.data(), w, nin, h1.p, nullptr, true);  // <V_q, w> and <w, w> in one pass
   } else {
     mb.dot(j + 2, V.data(), w, nin, h1.p);
   }

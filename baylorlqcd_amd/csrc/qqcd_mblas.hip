@@ -100,16 +100,18 @@ __device__ __forceinline__ void slot_sum(double (&v)[NV], int lognb, double *par
   }
 }
 
-// partials: part[((k * cap + slot) * gx + bx] for workgroup bx of the slot's block
-template <typename TV, typename TW, int K, bool SHARED>
+// partials: part[((k * cap + slot) * gx + bx] for workgroup bx of the slot's block;
+// SELF: two more components, <W, W> (real, imaginary 0) after the K dots
+template <typename TV, typename TW, int K, bool SHARED, bool SELF>
 __global__ void __launch_bounds__(MB) k_mdot(MShape a, MPtr v, const void *w, double *part, const int *any) {
   if (any && !*any) return;
   const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
   if ((blk << a.lognb) >= nact) return;
   const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
-  double acc[2 * K];
+  constexpr int NV = 2 * K + (SELF ? 2 : 0);
+  double acc[NV];
 #pragma unroll
-  for (int k = 0; k < 2 * K; k++) acc[k] = 0;
+  for (int k = 0; k < NV; k++) acc[k] = 0;
   if (slot < nact) {
     const int per = MB >> a.lognb;
     const long base = (long)blk * a.blk_elems;
@@ -122,13 +124,14 @@ __global__ void __launch_bounds__(MB) k_mdot(MShape a, MPtr v, const void *w, do
         acc[2 * k] += x.x * b.x + x.y * b.y;
         acc[2 * k + 1] += x.x * b.y - x.y * b.x;
       }
+      if (SELF) acc[2 * K] += b.x * b.x + b.y * b.y;
     }
   }
   if (std::is_same<TV, f16>::value) {
 #pragma unroll
     for (int k = 0; k < 2 * K; k++) acc[k] *= a.hinv;
   }
-  slot_sum<2 * K>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
+  slot_sum<NV>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
 }
 
 // One wavefront per (component c, slot): the sum over bx of the partials
@@ -384,8 +387,8 @@ static int gx_of(long rows, int nb, int nblk) {
   return (int)std::max(1L, std::min(groups, (long)std::max(16, 2048 / nblk)));
 }
 
-// gx <= 2048 and at most 16 partial values per slot
-size_t MBlas::need() const { return (size_t)2048 * 16 * cap + 64; }
+// gx <= 2048 and at most 18 partial values per slot (8 complex dots and a norm)
+size_t MBlas::need() const { return (size_t)2048 * 18 * cap + 64; }
 
 static MShape shape(const MultiSpinor &W, const int *nact) {
   MShape a;
@@ -426,19 +429,28 @@ template <typename F> static void basis_types(Prec a, Prec b, F &&f) {
 
 template <bool SHARED>
 static void dot_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, const MultiSpinor &W, const int *nact,
-                     double *res, double *&part, const int *any, std::function<void(size_t)> reserve) {
+                     double *res, double *&part, const int *any, std::function<void(size_t)> reserve,
+                     bool self = false) {
   const MShape a = shape(W, nact);
   const int nblk = W.nblk();
-  reserve((size_t)a.gx * 16 * W.cap);
+  reserve((size_t)a.gx * 18 * W.cap);
   for (int q0 = 0; q0 < nq; q0 += 8) {
     const int K = std::min(8, nq - q0);
+    // the last chunk also forms <W, W> when asked (res entry nq)
+    const bool sf = self && q0 + K == nq;
     MPtr pk{};
     for (int k = 0; k < K; k++) pk.p[k] = vp[q0 + k];
     basis_types(vprec, W.prec, [&](auto *tv, auto *tw) {
       typedef T_OF(tv) TV;
       typedef T_OF(tw) TW;
       dim3 grid(a.gx, nblk);
-#define L_(KK) hipLaunchKernelGGL((k_mdot<TV, TW, KK, SHARED>), grid, dim3(MB), 0, mb.stream, a, pk, W.data, part, any)
+#define L_(KK)                                                                                                    \
+  do {                                                                                                            \
+    if (sf)                                                                                                       \
+      hipLaunchKernelGGL((k_mdot<TV, TW, KK, SHARED, true>), grid, dim3(MB), 0, mb.stream, a, pk, W.data, part, any); \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_mdot<TV, TW, KK, SHARED, false>), grid, dim3(MB), 0, mb.stream, a, pk, W.data, part, any); \
+  } while (0)
       switch (K) {
         case 1: L_(1); break;
         case 2: L_(2); break;
@@ -452,13 +464,15 @@ static void dot_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, const
 #undef L_
     });
     QQCD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_mfinal, dim3(W.cap, 2 * K), dim3(64), 0, mb.stream, 2 * K, W.cap, a.gx, nact,
+    const int nv = 2 * K + (sf ? 2 : 0);
+    hipLaunchKernelGGL(k_mfinal, dim3(W.cap, nv), dim3(64), 0, mb.stream, nv, W.cap, a.gx, nact,
                        (const double *)part, res, q0, 1);
     QQCD_HIP(hipGetLastError());
   }
 }
 
-void MBlas::dot(int nq, const MultiSpinor *V, const MultiSpinor &W, const int *nact, double *res, const int *any) {
+void MBlas::dot(int nq, const MultiSpinor *V, const MultiSpinor &W, const int *nact, double *res, const int *any,
+                bool self) {
   need_fp(W, "MBlas::dot");
   std::vector<const void *> vp(nq);
   for (int q = 0; q < nq; q++) {
@@ -468,7 +482,7 @@ void MBlas::dot(int nq, const MultiSpinor *V, const MultiSpinor &W, const int *n
     vp[q] = V[q].data;
   }
   if (nq == 0) return;
-  dot_impl<false>(*this, nq, vp.data(), V[0].prec, W, nact, res, part, any, [this](size_t n) { reserve(n); });
+  dot_impl<false>(*this, nq, vp.data(), V[0].prec, W, nact, res, part, any, [this](size_t n) { reserve(n); }, self);
 }
 
 void MBlas::dot_shared(int nq, const Spinor *D, const MultiSpinor &W, const int *nact, double *res) {

@@ -37,9 +37,10 @@ class MBlas {
   MBlas &operator=(const MBlas &) = delete;
 
   // res[q*cap + r] = <V_q(r), W(r)> for q < nq (V_q multi, FP64, FP32 or an
-  // FP16 basis) -- device result; with `any` given, the pass is skipped when *any == 0
+  // FP16 basis) -- device result; with `any` given, the pass is skipped when
+  // *any == 0; self: also res[nq*cap + r] = <W(r), W(r)> from the same pass
   void dot(int nq, const MultiSpinor *V, const MultiSpinor &W, const int *nact, double *res,
-           const int *any = nullptr);
+           const int *any = nullptr, bool self = false);
   // res[q*cap + r] = <D_q, W(r)> for single-RHS vectors D_q shared by all slots
   void dot_shared(int nq, const Spinor *D, const MultiSpinor &W, const int *nact, double *res);
   // W(r) += sign * sum_q coef[(q0+q)*cap + r] V_q(r); if n2 != null, n2[r] = ||W(r)||^2.
