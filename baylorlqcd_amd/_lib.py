@@ -36,7 +36,8 @@ SIGNATURES = {
     "qqcd_vecdot": [_dp, _dp, _dp],
     "qqcd_pointsource": [_dp, _dp, _ip, ctypes.c_int, ctypes.c_int],
     "qqcd_gmresdr": [_dp, _dp, _ip, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int, _ip],
-    "qqcd_gmresproject": [_dp, _dp, _ip, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int, _ip],
+    "qqcd_gmresproject": [_dp, _dp, ctypes.c_int, _ip, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp,
+                          ctypes.c_int, _ip, ctypes.c_int, _ip, _dp, ctypes.c_int],
     "qqcd_psc": [_dp, _dp, _ip, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int, _ip,
                  ctypes.c_int, ctypes.c_double],
     "qqcd_bicgstab": [_dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int, _ip],
@@ -46,6 +47,7 @@ SIGNATURES = {
     "qqcd_hop_canonical": [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qqcd_schur_canonical": [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qqcd_set_xcd_remap": [ctypes.c_int],
+    "qqcd_set_dslash_tune": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qqcd_gauge_random": [ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp],
     "qqcd_gauge_get_canonical": [_dp],
     "qqcd_gauge_heatbath": [ctypes.c_double, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_int, _dp],
@@ -97,6 +99,9 @@ SIGNATURES = {
     "qqcd_batch_set": [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                        ctypes.c_int, ctypes.c_int],
     "qqcd_batch_stats": [_ip, _ip, _lp, _lp, _ip, _ip],
+    "qqcd_batch_rejected": [_ip],
+    "qqcd_mixed_stats": [_ip, _ip, _ip, _ip, _ip, _dp],
+    "qqcd_set_fault": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qqcd_hop_multi_canonical": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp],
     "qqcd_schur_multi_canonical": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,

@@ -382,6 +382,27 @@ BatchStats Batch::run(int n) {
     // x += e over every slot of this outer step
     n_in = 0;
     set_counts();
+    const FaultPlan &fault = fault_plan();
+    if (st.outer < fault.corrupt_outer && fault.batch_rhs >= 0) {
+      for (int s = 0; s < n_out; s++)
+        if (id[s] == fault.batch_rhs) {
+          SpinorBuf bad(c.g, PREC_SINGLE);
+          fill_spinor(c.g, bad.s, std::nan(""), std::nan(""), c.stream);
+          mb.put(bad.s, E.m, s);
+          QQCD_HIP(hipStreamSynchronize(c.stream));
+        }
+    }
+    // a slot whose correction holds a NaN or infinity keeps its x (the next
+    // outer step, or the single-RHS solver, works from its true residual)
+    mb.norm2(E.m, d_nout(), n2.p);
+    download(hn2.p, n2.p, cap);
+    for (int s = 0; s < n_out; s++)
+      if (!std::isfinite(hn2[s])) {
+        SpinorBuf zero(c.g, PREC_SINGLE);
+        mb.put(zero.s, E.m, s);
+        QQCD_HIP(hipStreamSynchronize(c.stream));
+        st.rejected++;
+      }
     mb.axpby(1.0, E.m, 1.0, X, d_nout(), nullptr);
     outer_residual();
     st.outer++;

@@ -56,6 +56,7 @@ struct BatchStats {
   long batched_matvecs = 0;  // multi-RHS operator launches (pairs of hopping kernels)
   int graph_replays = 0;
   int fallback = 0;          // right-hand sides finished by the single-RHS solver
+  int rejected = 0;          // per-slot inner corrections with a NaN or infinite entry, not added to x
   std::vector<int> iters;    // per right-hand side: Arnoldi steps while active
   std::vector<double> rel_res;
   int converged = 0;

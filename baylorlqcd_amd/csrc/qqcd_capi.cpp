@@ -64,7 +64,12 @@ extern "C" int qqcd_init(const int n[4], const int bc[4], int device) {
 
 namespace qqcd {
 void nn_release_device();
+
+void forget_host_gauge(const void *u) {
+  if (g_ctx && g_ctx->u_host == u) g_ctx->u_host = nullptr;
 }
+
+}  // namespace qqcd
 
 extern "C" int qqcd_finalize(void) {
   API_BEGIN
@@ -336,6 +341,17 @@ extern "C" int qqcd_pointsource(double *be, double *bo, const int src[4], int ic
 extern "C" int qqcd_set_xcd_remap(int on) {
   API_BEGIN
   set_xcd_remap(on ? 1 : 0);
+  API_END
+}
+
+extern "C" int qqcd_set_dslash_tune(int order, int bz, int depth, int nt) {
+  API_BEGIN
+  DslashTune t;
+  t.order = order;
+  t.bz = bz;
+  t.depth = depth;
+  t.nt = nt;
+  set_dslash_tune(t);
   API_END
 }
 

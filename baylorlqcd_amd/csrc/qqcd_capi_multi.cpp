@@ -749,6 +749,12 @@ extern "C" int qqcd_batch_stats(int *outer, int *cycles, long *slot_matvecs, lon
   API_END
 }
 
+extern "C" int qqcd_batch_rejected(int *rejected) {
+  API_BEGIN
+  *rejected = g_bst.rejected;
+  API_END
+}
+
 extern "C" int qqcd_autotune_multi(int prec, int recon, int nrhs, int *nb, int *order, double *ms) {
   API_BEGIN
   Context &c = ctx();

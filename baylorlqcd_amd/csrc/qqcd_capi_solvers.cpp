@@ -100,7 +100,7 @@ SolveStats run_gmresdr(Context &c, const Spinor &b, Spinor &x, const int GMRES[2
 }
 
 SolveStats run_gmresproj(Context &c, const Spinor &b, Spinor &x, int m, int k, double resmax, int itermin,
-                         const double kappa[2]) {
+                         const double kappa[2], ResHistory *hist = nullptr) {
   Deflation &d = defl();
   if (d.k == 0 || (int)d.V.size() != d.k + 1)
     throw std::runtime_error("GMRES-Proj needs the deflation space of an earlier GMRES-DR solve");
@@ -108,7 +108,24 @@ SolveStats run_gmresproj(Context &c, const Spinor &b, Spinor &x, int m, int k, d
   if (m < 1 || m > 120) throw std::runtime_error("GMRES-Proj cycle length must be 1..120");
   SchurOp A(c, PREC_DOUBLE, 18, kappa);
   LinOp op = A.op();
-  return gmresproj(c, op, b, x, m, d, resmax, itermin, KCYCLIM * m);
+  return gmresproj(c, op, b, x, m, d, resmax, itermin, KCYCLIM * m, 1, hist);
+}
+
+// The right-hand side gmresproject solves when called with isignal = 1: the
+// reference's module variable beg (shift.f90), which psc sets after its
+// GMRES-DR to the last vector of the deflation space, vtemp(:,:,:,:,:,k+1)
+// (inverters.f90:4437), and gmresproject with isignal /= 1 sets to its
+// initial residual b - M 0 = b (:7438-7443).
+Spinor g_beg;
+
+void set_beg(Context &c, const Spinor &v) {
+  if (!g_beg.data) g_beg = alloc_spinor(c.g, PREC_DOUBLE);
+  spinor_copy_convert(c.g, v, g_beg, c.stream);
+}
+
+void beg_after_gmresdr(Context &c) {
+  Deflation &d = defl();
+  if (d.k > 0 && (int)d.V.size() == d.k + 1) set_beg(c, d.V[d.k]);
 }
 
 }  // namespace
@@ -129,18 +146,35 @@ extern "C" int qqcd_gmresdr(const double *phi, double *x, const int GMRES[2], do
   API_END
 }
 
-extern "C" int qqcd_gmresproject(const double *bh, double *x, const int GMRES[2], double resmax, int itermin,
-                                 int *itercount, const double *u, const double *coact, const double kappa[2],
-                                 int iflag, const int *bc) {
+extern "C" int qqcd_gmresproject(const double *bh, double *xshift, int nshift, const int GMRES[2], double resmax,
+                                 int itermin, int *itercount, const double *u, const double *coact,
+                                 const double kappa[2], int iflag, const int *bc, int isignal, int *mvp, double *gdr,
+                                 int ldgdr) {
   API_BEGIN
   g_log.clear();
   Context &c = ctx();
+  if (nshift < 1) throw std::runtime_error("gmresproject: xshift needs at least one shift");
+  if (!mvp || (ldgdr > 0 && !gdr)) throw std::runtime_error("gmresproject: mvp and gdr are outputs");
   prepare(c, u, coact, bc, iflag);
   SpinorBuf b(c.g, PREC_DOUBLE), X(c.g, PREC_DOUBLE);
-  to_device(c, bh, b.s, 0);
-  g_last = run_gmresproj(c, b.s, X.s, GMRES[0], GMRES[1], resmax, itermin, kappa);
-  to_host(c, X.s, x, 0);
+  if (isignal == 1) {
+    // inverters.f90:7438-7440: the right-hand side is the resident beg
+    if (!g_beg.data) throw std::runtime_error("gmresproject with isignal = 1 needs a beg from an earlier psc");
+    spinor_copy_convert(c.g, g_beg, b.s, c.stream);
+  } else {
+    to_device(c, bh, b.s, 0);
+    set_beg(c, b.s);
+  }
+  ResHistory hist;
+  g_last = run_gmresproj(c, b.s, X.s, GMRES[0], GMRES[1], resmax, itermin, kappa, &hist);
+  // every shift of xshift is zeroed (the reference's xshift = 0), the solution is shift 1
+  const size_t slen = (size_t)6 * ref_ntotal(c.g) * 64;
+  std::memset(xshift, 0, slen * nshift * sizeof(double));
+  to_host(c, X.s, xshift, 0);
   *itercount = g_last.iters;
+  *mvp = g_last.matvecs;
+  for (const auto &h : hist)
+    if (h.first >= 1 && h.first <= ldgdr) gdr[h.first - 1] = h.second;
   API_END
 }
 
@@ -157,6 +191,7 @@ extern "C" int qqcd_psc(const double *bh, double *x, const int GMRES[2], double 
   if (isignal == 1) {
     to_device(c, x, X.s, 0);
     g_last = run_gmresdr(c, b.s, X.s, GMRES, resmax, itermin, kappa);
+    beg_after_gmresdr(c);
   } else {
     // inverters.f90:4439-4458: GMRES-Proj(m-k, k), tolerance 1e-8 hard-coded
     const double tol = resmax_proj > 0 ? resmax_proj : 1e-8;
@@ -225,6 +260,7 @@ extern "C" int qqcd_fermprop(const double *be, const double *bo, const double ka
   } else if (inverter == 6) {
     if (isignal == 1) {
       g_last = run_gmresdr(c, src.s, x_e.s, nGMRES, resmax, itermin, kappa);
+      beg_after_gmresdr(c);
     } else {
       check_mk(nGMRES[0], nGMRES[1]);
       g_last = run_gmresproj(c, src.s, x_e.s, nGMRES[0] - nGMRES[1], nGMRES[1], 1e-8, itermin, kappa);
@@ -266,6 +302,7 @@ extern "C" int qqcd_deflation_clear(void) {
     delete g_defl_mixed;
     g_defl_mixed = nullptr;
   }
+  free_spinor(g_beg);
   API_END
 }
 
@@ -405,6 +442,32 @@ extern "C" int qqcd_bench_solve(int method, const int src[4], int spin, int colo
   auto t1 = std::chrono::steady_clock::now();
   *seconds = std::chrono::duration<double>(t1 - t0).count();
   *true_res = full_residual(c, kappa, b_e.s, b_o.s, x_e.s, x_o.s);
+  API_END
+}
+
+extern "C" int qqcd_set_fault(int precond, int corrupt_outer, int batch_rhs, int maxiter) {
+  API_BEGIN
+  if (precond < 0 || precond > 3 || corrupt_outer < 0 || maxiter < 0)
+    throw std::runtime_error("qqcd_set_fault: precond 0..3, corrupt_outer >= 0, maxiter >= 0");
+  FaultPlan f;
+  f.precond = precond;
+  f.corrupt_outer = corrupt_outer;
+  f.batch_rhs = batch_rhs;
+  f.maxiter = maxiter;
+  fault_plan() = f;
+  API_END
+}
+
+extern "C" int qqcd_mixed_stats(int *outer, int *fallback, int *precond_dropped, int *rejected, int *converged,
+                                double *rel_res) {
+  API_BEGIN
+  const MixedStats &m = last_mixed_stats();
+  if (outer) *outer = m.outer;
+  if (fallback) *fallback = m.fallback;
+  if (precond_dropped) *precond_dropped = m.precond_dropped;
+  if (rejected) *rejected = m.rejected;
+  if (converged) *converged = m.converged;
+  if (rel_res) *rel_res = m.rel_res;
   API_END
 }
 

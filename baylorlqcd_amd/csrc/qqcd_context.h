@@ -74,8 +74,11 @@ void spinor_to_canon(const Geom &g, const Spinor &in, double *dcn, int par, hipS
 void gamma_ref(const Geom &g, const double *in, double *out, int mu, int nslices, const int *slices,
                hipStream_t s);
 void random_spinor(const Geom &g, Spinor &s, uint64_t seed, hipStream_t st);
+// every component of the lattice sites set to re + i im, any precision (padding left as it is)
+void fill_spinor(const Geom &g, Spinor &s, double re, double im, hipStream_t st);
 // gamma_mu (mu = 1..5) of the reference basis as a dense 4x4 complex matrix
 void gamma_dense(int mu, double re[4][4], double im[4][4]);
 void set_xcd_remap(int v);
+
 
 }  // namespace qqcd

@@ -204,6 +204,14 @@ struct DslashArgs {
   double a0, a1, b0, b1;
   int has_x;
   int xcd_remap;
+  // traversal of the sites (single right-hand side): order 0 walks the
+  // checkerboard index; order 1 walks blocks of bz z-planes, all t of a block
+  // before the next block (z fastest inside the block), so the t and z
+  // neighbours of a wavefront were read a few thousand sites earlier by the
+  // same XCD and are still in its L2
+  int order;
+  int bz;
+  int plane;  // sites of one (z, t) plane of a checkerboard: nx*ny/2
 };
 
 // Pointers and strides a hop reads (scalars only, so that nothing of the
@@ -400,6 +408,354 @@ __global__ void __launch_bounds__(256) k_dslash(const DslashArgs a) {
   dslash_site<S, L, RECON, SIG, false>(a, h, a.out, a.out_norm, a.x, a.x_norm, i, 0, 0);
 }
 
+// ---- pipelined hops (projector form) -----------------------------------
+//
+// The eight hops of a site each load one link and one (half) neighbour
+// spinor.  Computed one after the other, every hop waits for its own loads,
+// so a wavefront makes eight serial trips to memory.  Here the loads of hop
+// k + D are issued into raw registers before hop k is computed, so D hops'
+// loads are in flight per wavefront.  NT = 1 loads the links and stores the
+// output non-temporally: both are touched once per launch, and streaming
+// them leaves the L2 to the neighbour spinors, which are read eight times.
+
+template <typename T> struct V2;
+template <> struct V2<double> { typedef double __attribute__((ext_vector_type(2))) T; };
+template <> struct V2<float> { typedef float __attribute__((ext_vector_type(2))) T; };
+template <> struct V2<f16> { typedef f16 __attribute__((ext_vector_type(2))) T; };
+
+template <typename S, typename L, int RECON> struct HopRaw {
+  typename V2<L>::T u[RECON == 18 ? 9 : 6];
+  typename V2<S>::T p[12];
+  float nrm;
+  float edge;
+};
+
+// neighbour of hop K (mu = K/2, forward for even K): checkerboard index j and
+// the boundary factor of the hop
+template <int K>
+__device__ __forceinline__ void hop_nbr(const DslashArgs &a, int x, int y, int z, int t, int &j, float &edge) {
+  constexpr int MU = K >> 1;
+  constexpr bool FWD = !(K & 1);
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2], nt = a.n[3];
+  int c[4] = {x, y, z, t};
+  const int n = a.n[MU];
+  const int v = c[MU];
+  edge = 1.f;
+  if (FWD) {
+    c[MU] = v + 1 == n ? 0 : v + 1;
+    if (v + 1 == n) edge = a.bcf[MU];
+  } else {
+    c[MU] = v == 0 ? n - 1 : v - 1;
+    if (v == 0) edge = a.bcf[MU];
+  }
+  (void)nt;
+  j = (c[0] + nx * (c[1] + ny * (c[2] + nz * c[3]))) >> 1;
+}
+
+// Buffer access: a descriptor over a field (wavefront-uniform, in scalar
+// registers), a per-lane byte offset and a per-component offset in a scalar
+// register, so no load or store needs per-lane address arithmetic.  A
+// descriptor covers less than 4 GiB (checked on the host).  AUX 2 = the
+// non-temporal hint.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+template <typename X, int AUX>
+__device__ __forceinline__ X bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(X) == 4) return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
+  else if constexpr (sizeof(X) == 8)
+    return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
+  else return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, AUX));
+}
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <typename X, int AUX>
+__device__ __forceinline__ void bst(X v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(X) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, AUX);
+  else if constexpr (sizeof(X) == 8) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, AUX);
+  else __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, voff, soff, AUX);
+}
+
+template <typename S, typename L, int RECON, int SIG, bool NT, bool MULTI, int K>
+__device__ __forceinline__ void hop_fetch(const DslashArgs &a, const HopIn &h, int i, int x, int y, int z, int t,
+                                          int rr, int lognb, HopRaw<S, L, RECON> &r) {
+  constexpr int MU = K >> 1;
+  constexpr bool FWD = !(K & 1);
+  constexpr int NC = RECON == 18 ? 9 : 6;
+  constexpr int AUXL = NT ? 2 : 0;
+  typedef typename V2<L>::T LE;
+  typedef typename V2<S>::T SE;
+  int j;
+  hop_nbr<K>(a, x, y, z, t, j, r.edge);
+  const int lpar = FWD ? a.par : (a.par ^ 1);
+  const uint32_t lo = (uint32_t)(FWD ? i : j) * (uint32_t)sizeof(LE);
+  const uint32_t js = MULTI ? ((uint32_t)j << lognb) + rr : (uint32_t)j;
+  const uint32_t so = js * (uint32_t)sizeof(SE);
+  const uint32_t lrun = (uint32_t)h.lstride * (uint32_t)sizeof(LE), srun = (uint32_t)h.sstride * (uint32_t)sizeof(SE);
+  // the link block of (lpar, MU): NC runs
+  const __amdgpu_buffer_rsrc_t lr =
+      make_rsrc((const char *)h.links + (size_t)((lpar * 4 + MU) * NC) * lrun, NC * lrun);
+#pragma unroll
+  for (int k = 0; k < NC; k++) r.u[k] = bld<LE, AUXL>(lr, lo, k * lrun);
+  // (1 +/- g_4) keeps the upper or the lower spin pair of the neighbour
+  constexpr int s0 = MU == 3 ? (((FWD ? SIG : -SIG) > 0) ? 0 : 6) : 0;
+  constexpr int ns = MU == 3 ? 6 : 12;
+  const __amdgpu_buffer_rsrc_t sr = make_rsrc(h.in, 12 * srun);
+#pragma unroll
+  for (int k = 0; k < ns; k++) r.p[k] = bld<SE, 0>(sr, so, (s0 + k) * srun);
+  if constexpr (std::is_same<S, f16>::value) r.nrm = bld<float, 0>(make_rsrc(h.in_norm, 4u * (uint32_t)h.sstride), js * 4u, 0);
+  else r.nrm = 1.f;
+}
+
+template <typename R> __device__ __forceinline__ void recon_row2(cx<R> u[9]) {
+  u[6] = mk<R>(u[1].re * u[5].re - u[1].im * u[5].im - (u[2].re * u[4].re - u[2].im * u[4].im),
+               -(u[1].re * u[5].im + u[1].im * u[5].re - (u[2].re * u[4].im + u[2].im * u[4].re)));
+  u[7] = mk<R>(u[2].re * u[3].re - u[2].im * u[3].im - (u[0].re * u[5].re - u[0].im * u[5].im),
+               -(u[2].re * u[3].im + u[2].im * u[3].re - (u[0].re * u[5].im + u[0].im * u[5].re)));
+  u[8] = mk<R>(u[0].re * u[4].re - u[0].im * u[4].im - (u[1].re * u[3].re - u[1].im * u[3].im),
+               -(u[0].re * u[4].im + u[0].im * u[4].re - (u[1].re * u[3].im + u[1].im * u[3].re)));
+}
+
+// acc += f w and acc += f (i w) for a real f, as two fused multiply-adds each
+template <typename R> __device__ __forceinline__ void fma_acc(cx<R> &acc, R f, cx<R> w) {
+  acc.re = fma(f, w.re, acc.re);
+  acc.im = fma(f, w.im, acc.im);
+}
+template <typename R> __device__ __forceinline__ void fma_acc_i(cx<R> &acc, R f, cx<R> w) {
+  acc.re = fma(-f, w.im, acc.re);
+  acc.im = fma(f, w.re, acc.im);
+}
+
+// The hop is linear in the neighbour spinor, so its scale factors (c1, the
+// boundary factor, the FP16 per-site norm, the 2 of the t projector) are
+// one scalar applied in the accumulation, not to the loaded values.
+template <typename S, typename L, int RECON, int SIG, int K, typename R>
+__device__ __forceinline__ void hop_consume(const HopRaw<S, L, RECON> &r, cx<R> acc[12], R sc) {
+  constexpr int MU = K >> 1;
+  constexpr bool FWD = !(K & 1);
+  constexpr int NC = RECON == 18 ? 9 : 6;
+  cx<R> u[9];
+#pragma unroll
+  for (int k = 0; k < NC; k++) u[k] = mk<R>((R)r.u[k].x, (R)r.u[k].y);
+  if constexpr (RECON == 12) recon_row2(u);
+  R f = sc * (R)r.edge;
+  if constexpr (std::is_same<S, f16>::value) f *= (R)r.nrm;
+  cx<R> w[6];
+  if constexpr (MU == 3) {
+    constexpr int s0 = ((FWD ? SIG : -SIG) > 0) ? 0 : 6;
+    cx<R> hh[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) hh[c] = mk<R>((R)r.p[c].x, (R)r.p[c].y);
+    if (FWD) {
+      mv(u, hh, w);
+      mv(u, hh + 3, w + 3);
+    } else {
+      mdv(u, hh, w);
+      mdv(u, hh + 3, w + 3);
+    }
+    const R f2 = R(2) * f;
+#pragma unroll
+    for (int c = 0; c < 6; c++) fma_acc(acc[s0 + c], f2, w[c]);
+  } else {
+    constexpr int sg = FWD ? SIG : -SIG;
+    const R sig = R(sg);
+    cx<R> p[12], hh[6];
+#pragma unroll
+    for (int k = 0; k < 12; k++) p[k] = mk<R>((R)r.p[k].x, (R)r.p[k].y);
+    project<MU>(p, sig, hh);
+    if (FWD) {
+      mv(u, hh, w);
+      mv(u, hh + 3, w + 3);
+    } else {
+      mdv(u, hh, w);
+      mdv(u, hh + 3, w + 3);
+    }
+    const R fs = sg > 0 ? f : -f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      fma_acc(acc[c], f, w[c]);
+      fma_acc(acc[3 + c], f, w[3 + c]);
+      if constexpr (MU == 0) {
+        fma_acc(acc[6 + c], fs, w[3 + c]);
+        fma_acc(acc[9 + c], fs, w[c]);
+      } else if constexpr (MU == 1) {
+        fma_acc_i(acc[6 + c], -fs, w[3 + c]);
+        fma_acc_i(acc[9 + c], fs, w[c]);
+      } else {
+        fma_acc(acc[6 + c], fs, w[c]);
+        fma_acc(acc[9 + c], -fs, w[3 + c]);
+      }
+    }
+  }
+}
+
+// Stage boundary of the pipeline: the accumulators pass through an empty
+// asm statement (no instruction is emitted), so the arithmetic of the hops
+// before it is finished there, and the memory clobber keeps the loads of
+// the stages after it from being hoisted above it.  Without it the compiler
+// issues every load first and keeps all raw buffers live at once.
+template <typename R> __device__ __forceinline__ void stage_fence(cx<R> acc[12]) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) asm volatile("" : "+v"(acc[k].re), "+v"(acc[k].im));
+  asm volatile("" ::: "memory");
+}
+
+// The other side of a stage: hop K's raw registers pass through an empty
+// asm statement right before hop K is computed, so its arithmetic cannot be
+// scheduled above the loads issued for later hops (the wait for hop K's
+// loads lands here, with the later hops' loads already in flight).
+template <typename S, typename L, int RECON> __device__ __forceinline__ void stage_pin(HopRaw<S, L, RECON> &r) {
+  constexpr int NC = RECON == 18 ? 9 : 6;
+  asm volatile("" ::: "memory");
+#pragma unroll
+  for (int k = 0; k < NC; k++) asm volatile("" : "+v"(r.u[k]));
+#pragma unroll
+  for (int k = 0; k < 12; k++) asm volatile("" : "+v"(r.p[k]));
+  asm volatile("" : "+v"(r.nrm), "+v"(r.edge));
+}
+
+// hops K .. 7: compute hop K from its buffer, refill the buffer with hop
+// K + D; once no hop is left to issue, the epilogue's local spinor is loaded
+// in its place
+template <typename S, typename L, int RECON, int SIG, bool NT, bool MULTI, int D, int K, typename R>
+__device__ __forceinline__ void hop_pipe(const DslashArgs &a, const HopIn &h, int i, int x, int y, int z, int t,
+                                         int rr, int lognb, HopRaw<S, L, RECON> *buf, cx<R> acc[12],
+                                         const void *xin, const float *x_norm, int io, typename V2<S>::T *xr,
+                                         float &xn) {
+  if constexpr (K < 8) {
+    stage_pin(buf[K % D]);
+    hop_consume<S, L, RECON, SIG, K>(buf[K % D], acc, (R)a.scl[K >> 1]);
+    stage_fence(acc);
+    if constexpr (K + D < 8) {
+      hop_fetch<S, L, RECON, SIG, NT, MULTI, K + D>(a, h, i, x, y, z, t, rr, lognb, buf[K % D]);
+    } else if constexpr (K + D == 8) {
+      if (a.has_x) {
+        typedef typename V2<S>::T SE;
+        const uint32_t srun = (uint32_t)h.sstride * (uint32_t)sizeof(SE);
+        const __amdgpu_buffer_rsrc_t xrs = make_rsrc(xin, 12 * srun);
+#pragma unroll
+        for (int k = 0; k < 12; k++) xr[k] = bld<SE, 0>(xrs, (uint32_t)io * (uint32_t)sizeof(SE), k * srun);
+        if constexpr (std::is_same<S, f16>::value)
+          xn = bld<float, 0>(make_rsrc(x_norm, 4u * (uint32_t)h.sstride), (uint32_t)io * 4u, 0);
+      }
+    }
+    hop_pipe<S, L, RECON, SIG, NT, MULTI, D, K + 1>(a, h, i, x, y, z, t, rr, lognb, buf, acc, xin, x_norm, io, xr,
+                                                     xn);
+  }
+}
+
+template <typename S, typename L, int RECON, int SIG, bool NT, bool MULTI, int D, int K>
+__device__ __forceinline__ void hop_prefetch(const DslashArgs &a, const HopIn &h, int i, int x, int y, int z, int t,
+                                             int rr, int lognb, HopRaw<S, L, RECON> *buf) {
+  if constexpr (K < D) {
+    hop_fetch<S, L, RECON, SIG, NT, MULTI, K>(a, h, i, x, y, z, t, rr, lognb, buf[K]);
+    asm volatile("" ::: "memory");
+    hop_prefetch<S, L, RECON, SIG, NT, MULTI, D, K + 1>(a, h, i, x, y, z, t, rr, lognb, buf);
+  }
+}
+
+template <typename S, typename L, int RECON, int SIG, bool NT, bool MULTI, int D>
+__device__ __forceinline__ void dslash_site_pipe(const DslashArgs &a, const HopIn &h, void *out, float *out_norm,
+                                                 const void *xin, const float *x_norm, int i, int rr, int lognb) {
+  typedef typename SIO<S>::R R;
+  typedef typename V2<S>::T SE;
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+  const int r2 = 2 * i;
+  const int x0 = r2 % nx;
+  int q = r2 / nx;
+  const int y = q % ny;
+  q /= ny;
+  const int z = q % nz;
+  const int t = q / nz;
+  const int x = x0 + ((a.par + y + z + t) & 1);
+
+  HopRaw<S, L, RECON> buf[D];
+  hop_prefetch<S, L, RECON, SIG, NT, MULTI, D, 0>(a, h, i, x, y, z, t, rr, lognb, buf);
+  const int io = MULTI ? (i << lognb) + rr : i;
+  SE xr[12];
+  float xn = 1.f;
+  cx<R> acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) acc[k] = mk<R>(0, 0);
+  hop_pipe<S, L, RECON, SIG, NT, MULTI, D, 0>(a, h, i, x, y, z, t, rr, lognb, buf, acc, xin, x_norm, io, xr, xn);
+
+  // epilogue: out = a0 x + a1 (i g5 x) + b0 D + b1 (i g5 D);  i g5 v = (v2, v3, -v0, -v1)
+  const R b0 = (R)a.b0, b1 = (R)a.b1;
+  cx<R> o[12];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    o[c] = b0 * acc[c] + b1 * acc[6 + c];
+    o[3 + c] = b0 * acc[3 + c] + b1 * acc[9 + c];
+    o[6 + c] = b0 * acc[6 + c] - b1 * acc[c];
+    o[9 + c] = b0 * acc[9 + c] - b1 * acc[3 + c];
+  }
+  if (a.has_x) {
+    const R a0 = (R)a.a0, a1 = (R)a.a1;
+    cx<R> xv[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) xv[k] = mk<R>((R)xn * (R)xr[k].x, (R)xn * (R)xr[k].y);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      o[c] = o[c] + a0 * xv[c] + a1 * xv[6 + c];
+      o[3 + c] = o[3 + c] + a0 * xv[3 + c] + a1 * xv[9 + c];
+      o[6 + c] = o[6 + c] + a0 * xv[6 + c] - a1 * xv[c];
+      o[9 + c] = o[9 + c] + a0 * xv[9 + c] - a1 * xv[3 + c];
+    }
+  }
+  constexpr int AUXS = NT ? 2 : 0;
+  const uint32_t srun = (uint32_t)h.sstride * (uint32_t)sizeof(SE);
+  const uint32_t oo = (uint32_t)io * (uint32_t)sizeof(SE);
+  const __amdgpu_buffer_rsrc_t ors = make_rsrc(out, 12 * srun);
+  if constexpr (std::is_same<S, f16>::value) {
+    float m = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf(o[k].re), fabsf(o[k].im)));
+    bst<float, AUXS>(m, make_rsrc(out_norm, 4u * (uint32_t)h.sstride), (uint32_t)io * 4u, 0);
+    const float inv = m > 0.f ? 1.f / m : 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      SE w;
+      w.x = (f16)(o[k].re * inv);
+      w.y = (f16)(o[k].im * inv);
+      bst<SE, AUXS>(w, ors, oo, k * srun);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      SE w;
+      w.x = o[k].re;
+      w.y = o[k].im;
+      bst<SE, AUXS>(w, ors, oo, k * srun);
+    }
+  }
+}
+
+// logical lane index -> checkerboard site (see DslashArgs::order)
+__device__ __forceinline__ int site_of(const DslashArgs &a, int lidx) {
+  if (a.order == 0) return lidx;
+  const int p = lidx / a.plane, o = lidx - p * a.plane;
+  const int per = a.bz * a.n[3];
+  const int zb = p / per, r = p - zb * per;
+  const int t = r / a.bz, z = zb * a.bz + (r - t * a.bz);
+  return (z + a.n[2] * t) * a.plane + o;
+}
+
+template <typename S, typename L, int RECON, int SIG, bool NT, int D>
+__global__ void __launch_bounds__(256) k_dslash_pipe(const DslashArgs a) {
+  int blk = blockIdx.x;
+  if (a.xcd_remap) {
+    const int nb = gridDim.x, per = (nb + 7) / 8;
+    const int xcd = blk & 7, k = blk >> 3;
+    const int full = nb - 8 * (per - 1);
+    blk = xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
+  }
+  const int l = blk * blockDim.x + threadIdx.x;
+  if (l >= a.Vh) return;
+  const int i = site_of(a, l);
+  const HopIn h{a.in, a.in_norm, a.links, a.sstride, a.lstride};
+  dslash_site_pipe<S, L, RECON, SIG, NT, false, D>(a, h, a.out, a.out_norm, a.x, a.x_norm, i, 0, 0);
+}
+
 // Several right-hand sides, RHS-innermost in blocks of nb = 2^lognb
 // (element (rhs r, component k, site j) of a field at
 // ((r/nb * 12 + k) * stride + j) * nb + r%nb).  A work tile is 256 lanes of
@@ -467,15 +823,55 @@ HopCoef make_hopcoef(const Geom &g, int idag) {
 static int g_xcd_remap = 1;
 void set_xcd_remap(int v) { g_xcd_remap = v; }
 
+static DslashTune g_tune;
+void set_dslash_tune(const DslashTune &t) {
+  if (t.order < 0 || t.order > 1 || t.bz < 0 || (t.depth != 0 && t.depth != 1 && t.depth != 2 && t.depth != 4) ||
+      t.nt < 0 || t.nt > 1)
+    throw std::runtime_error("dslash tuning: order 0/1, bz >= 0, depth 0/1/2/4, nt 0/1");
+  g_tune = t;
+}
+DslashTune get_dslash_tune() { return g_tune; }
+
+template <typename S, typename L, int RECON, int SIG, bool NT>
+static void launch_pipe_nt(const DslashArgs &a, int depth, int nb, hipStream_t s) {
+  switch (depth) {
+    case 1: hipLaunchKernelGGL((k_dslash_pipe<S, L, RECON, SIG, NT, 1>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_dslash_pipe<S, L, RECON, SIG, NT, 2>), dim3(nb), dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_dslash_pipe<S, L, RECON, SIG, NT, 4>), dim3(nb), dim3(256), 0, s, a); break;
+  }
+}
+
+template <typename S, typename L, int RECON, int SIG>
+static void launch_pipe(const DslashArgs &a, int depth, int nt, int nb, hipStream_t s) {
+  if (nt) launch_pipe_nt<S, L, RECON, SIG, true>(a, depth, nb, s);
+  else launch_pipe_nt<S, L, RECON, SIG, false>(a, depth, nb, s);
+}
+
 template <typename S, typename L, int RECON>
-static void launch_t(const DslashArgs &a, int sig, hipStream_t s) {
-  const int nb = (a.Vh + 255) / 256;
-  if (sig > 0)
-    hipLaunchKernelGGL((k_dslash<S, L, RECON, 1>), dim3(nb), dim3(256), 0, s, a);
+static void launch_t(const DslashArgs &a0, int sig, hipStream_t s) {
+  const int nb = (a0.Vh + 255) / 256;
+  const DslashTune &t = g_tune;
+  // the buffer descriptors of the pipelined kernel cover < 4 GiB each: a
+  // spinor field (12 runs) and one (parity, mu) block of links
+  const bool fits = 12.0 * a0.sstride * sizeof(typename V2<S>::T) < 4294967295.0 &&
+                    (RECON == 18 ? 9.0 : 6.0) * a0.lstride * sizeof(typename V2<L>::T) < 4294967295.0;
+  if (sig != 0 && t.depth > 0 && fits) {
+    DslashArgs a = a0;
+    a.plane = a.n[0] * a.n[1] / 2;
+    int bz = t.bz;
+    if (bz == 0)  // the largest of 16, 8, 4, 2 that divides nz
+      for (bz = 16; bz > 1 && a.n[2] % bz; bz /= 2) {
+      }
+    a.order = (t.order == 1 && bz > 1 && a.n[2] % bz == 0) ? 1 : 0;
+    a.bz = a.order ? bz : 1;
+    if (sig > 0) launch_pipe<S, L, RECON, 1>(a, t.depth, t.nt, nb, s);
+    else launch_pipe<S, L, RECON, -1>(a, t.depth, t.nt, nb, s);
+  } else if (sig > 0)
+    hipLaunchKernelGGL((k_dslash<S, L, RECON, 1>), dim3(nb), dim3(256), 0, s, a0);
   else if (sig < 0)
-    hipLaunchKernelGGL((k_dslash<S, L, RECON, -1>), dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_dslash<S, L, RECON, -1>), dim3(nb), dim3(256), 0, s, a0);
   else
-    hipLaunchKernelGGL((k_dslash<S, L, RECON, 0>), dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_dslash<S, L, RECON, 0>), dim3(nb), dim3(256), 0, s, a0);
   QQCD_HIP(hipGetLastError());
 }
 

@@ -93,6 +93,22 @@ struct MultiTune {
   int grid = 0;   // persistent workgroups (0: occupancy x CUs)
 };
 
+// Launch shape of the single-right-hand-side hopping kernel (projector form):
+// depth 0 = hops computed one after the other; 1/2/4 = the pipelined kernel
+// with that many hops' loads in flight; order/bz = site traversal (0: the
+// checkerboard index, 1: blocks of bz z-planes swept in t; bz 0 picks the
+// largest of 16, 8, 4, 2 dividing nz); nt = links and output streamed past
+// the L2.  The default is the fastest shape measured at 32^3x64 in all six
+// storage formats.
+struct DslashTune {
+  int order = 1;
+  int bz = 0;
+  int depth = 2;
+  int nt = 1;
+};
+void set_dslash_tune(const DslashTune &t);
+DslashTune get_dslash_tune();
+
 // Coefficients of the local spin matrix applied in a Dslash epilogue:
 //   out = a0*x + a1*(i g5 x) + b0*D + b1*(i g5 D)
 // with D the hopping sum and x a spinor on the output checkerboard.

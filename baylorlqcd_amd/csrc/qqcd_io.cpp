@@ -190,10 +190,17 @@ template <typename T> T get_bytes(const char *&p) {
 
 }  // namespace
 
+namespace qqcd {
+// the library's device copy of the links no longer matches the host array u
+// (defined with the context, qqcd_capi.cpp)
+void forget_host_gauge(const void *u);
+}  // namespace qqcd
+
 extern "C" int qqcd_gfwrite(const int n[4], double *u, const char *filename, const int gaction[4], double beta,
                             const double alat[4], const double tad[4], int icfgsave, int myid) {
   API_BEGIN
   const Dims d = dims_of(n);
+  qqcd::forget_host_gauge(u);  // fixsu3 re-unitarises u in place
   fixsu3(d, u);
   RecWriter w(filename, true);
   double rec[8];
@@ -223,6 +230,7 @@ extern "C" int qqcd_gfread(const int n[4], double *u, const char *filename, int 
   API_BEGIN
   const Dims d = dims_of(n);
   RecReader r(filename);
+  qqcd::forget_host_gauge(u);  // u is overwritten in place
   for (int c = 0; c < 12; c++)
     for (long i = 0; i < d.nvhalf; i++)
       for (int ibl = 0; ibl < 16; ibl++) {

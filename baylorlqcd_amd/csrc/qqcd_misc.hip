@@ -171,6 +171,37 @@ void random_spinor(const Geom &g, Spinor &s, uint64_t seed, hipStream_t st) {
   QQCD_HIP(hipGetLastError());
 }
 
+// every component of the Vh sites set to re + i im (FP16: norm 1); padding untouched
+template <typename S> __global__ void k_fill(int Vh, long stride, double re, double im, void *out, float *nrm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Vh) return;
+  if (sizeof(S) == 8) {
+    double2 *p = (double2 *)out;
+    for (int k = 0; k < 12; k++) p[(long)k * stride + i] = make_double2(re, im);
+  } else if (sizeof(S) == 4) {
+    float2 *p = (float2 *)out;
+    for (int k = 0; k < 12; k++) p[(long)k * stride + i] = make_float2((float)re, (float)im);
+  } else {
+    nrm[i] = 1.f;
+    h2 *p = (h2 *)out;
+    h2 w;
+    w.x = (f16)(float)re;
+    w.y = (f16)(float)im;
+    for (int k = 0; k < 12; k++) p[(long)k * stride + i] = w;
+  }
+}
+
+void fill_spinor(const Geom &g, Spinor &s, double re, double im, hipStream_t st) {
+  const int nb = (int)((g.Vh + 255) / 256);
+  if (s.prec == PREC_DOUBLE)
+    hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, st, (int)g.Vh, s.stride, re, im, s.data, s.norm);
+  else if (s.prec == PREC_SINGLE)
+    hipLaunchKernelGGL(k_fill<float>, dim3(nb), dim3(256), 0, st, (int)g.Vh, s.stride, re, im, s.data, s.norm);
+  else
+    hipLaunchKernelGGL(k_fill<f16>, dim3(nb), dim3(256), 0, st, (int)g.Vh, s.stride, re, im, s.data, s.norm);
+  QQCD_HIP(hipGetLastError());
+}
+
 }  // namespace qqcd
 
 namespace qqcd {
@@ -275,19 +306,6 @@ void local_spin(const Geom &g, const Spinor &in, Spinor &out, double a0, double 
   const int nb = (int)((g.Vh + 255) / 256);
   hipLaunchKernelGGL(k_local_spin, dim3(nb), dim3(256), 0, s, (int)g.Vh, g.stride, (const double2 *)in.data,
                      (double2 *)out.data, a0, a1);
-  QQCD_HIP(hipGetLastError());
-}
-
-__global__ void k_fill(int Vh, long stride, double2 *x, double re, double im) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= Vh) return;
-  for (int k = 0; k < 12; k++) x[(long)k * stride + i] = make_double2(re, im);
-}
-
-void fill_spinor(const Geom &g, Spinor &x, double re, double im, hipStream_t s) {
-  if (x.prec != PREC_DOUBLE) throw std::runtime_error("fill_spinor: FP64 only");
-  const int nb = (int)((g.Vh + 255) / 256);
-  hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, s, (int)g.Vh, g.stride, (double2 *)x.data, re, im);
   QQCD_HIP(hipGetLastError());
 }
 

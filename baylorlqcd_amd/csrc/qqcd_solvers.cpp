@@ -347,6 +347,12 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
     rn = residual(c, A, b, x, r.s, st);
     st.cycles++;
     st.rel_res = rn / r0;
+    if (!std::isfinite(rn)) {
+      // a NaN or infinity reached x: no restart can remove it
+      st.nonfinite = 1;
+      if (defl) defl->clear();
+      return st;
+    }
     if (hook && *hook) (*hook)(x, r.s, st);
     const bool done = (rn < tol * r0 && st.iters >= itermin) || st.iters >= maxiter;
     if (done) {
@@ -388,7 +394,7 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
 }
 
 SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, const Deflation &defl, double tol,
-                     int itermin, int maxiter, int reorth) {
+                     int itermin, int maxiter, int reorth, ResHistory *hist) {
   SolveStats st;
   const Prec p = A.prec, vp = A.basis();
   const int k = defl.k;
@@ -420,6 +426,10 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
       c.blas->combine(k, Dv.data(), 1, d.data(), k, &xv, true);
       rn = residual(c, A, b, x, r.s, st);
       st.rel_res = rn / r0;
+      if (!std::isfinite(rn)) {
+        st.nonfinite = 1;
+        break;
+      }
       if ((rn < tol * r0 && st.iters >= itermin) || st.iters >= maxiter) break;
     }
     // GMRES(m) cycle from r
@@ -436,6 +446,10 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
         j++;
         if (j < m && st.iters < maxiter && j - jchk < kCheck) continue;
         const int jb = dh->fetch(c.stream, jchk, j, H, ldh);
+        const int jlast = jb >= 0 ? jb + 1 : j;
+        if (hist)  // one entry per Arnoldi step of this fetch
+          for (int jj = jchk + 1; jj <= jlast; jj++)
+            hist->emplace_back(st.matvecs - (j - jj), dense::lsq(jj + 1, jj, H.data(), ldh, cv.data(), y.data()));
         jchk = j;
         if (jb >= 0) {
           j = jb + 1;
@@ -451,6 +465,7 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
       for (int i = 0; i <= j + 1; i++) H[i + (size_t)ldh * j] = col[i];
       j++;
       const double est = dense::lsq(j + 1, j, H.data(), ldh, cv.data(), y.data());
+      if (hist) hist->emplace_back(st.matvecs, est);
       if ((est < tol * r0 && st.iters >= itermin) || hn == 0.0 || st.iters >= maxiter) break;
     }
     dense::lsq(j + 1, j, H.data(), ldh, cv.data(), y.data());
@@ -462,6 +477,10 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
     rn = residual(c, A, b, x, r.s, st);
     st.cycles++;
     st.rel_res = rn / r0;
+    if (!std::isfinite(rn)) {
+      st.nonfinite = 1;
+      break;
+    }
     if ((rn < tol * r0 && st.iters >= itermin) || st.iters >= maxiter) break;
   }
   st.converged = st.rel_res < tol;
@@ -512,6 +531,10 @@ SolveStats bicgstab(Context &c, LinOp &A, const Spinor &b, Spinor &x, double tol
     rn = residual(c, A, b, x, r.s, st);
     st.cycles++;
     st.rel_res = rn / r0;
+    if (!std::isfinite(rn)) {
+      st.nonfinite = 1;
+      break;
+    }
     if ((rn < tol * r0 && st.iters >= itermin) || st.iters >= maxiter) break;
   }
   st.converged = st.rel_res < tol;
@@ -522,9 +545,33 @@ SolveStats bicgstab(Context &c, LinOp &A, const Spinor &b, Spinor &x, double tol
 
 namespace qqcd {
 
-MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spinor &x, const MixedParams &p,
+FaultPlan &fault_plan() {
+  static FaultPlan f;
+  return f;
+}
+
+MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spinor &x, const MixedParams &p0,
                        Deflation *defl) {
   MixedStats ms;
+  const FaultPlan fault = fault_plan();
+  MixedParams p = p0;
+  if (fault.maxiter > 0) p.maxiter = std::min(p.maxiter, fault.maxiter);
+  // the preconditioner as the inner solves see it (its output replaced when a fault is planned)
+  LinOp pre_f;
+  if (p.precond && fault.precond) {
+    LinOp *orig = p.precond;
+    pre_f.prec = orig->prec;
+    pre_f.store = orig->store;
+    const int mode = fault.precond;
+    uint64_t seed = 977;
+    pre_f.apply = [&c, orig, mode, seed](const Spinor &a, Spinor &z) mutable {
+      orig->apply(a, z);
+      if (mode == 1) fill_spinor(c.g, z, std::nan(""), std::nan(""), c.stream);
+      else if (mode == 2) fill_spinor(c.g, z, 0.0, 0.0, c.stream);
+      else random_spinor(c.g, z, seed++, c.stream);
+    };
+    p.precond = &pre_f;
+  }
   SchurOp A64(c, PREC_DOUBLE, 18, kappa);
   LinOp op64 = A64.op();
   // operator levels: the requested storage, then FP32, then FP64 (fallback
@@ -563,12 +610,20 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
   const double bn = std::sqrt(c.blas->norm2(as_vec(b)));
   double rn = residual(c, op64, b, x, r.s, dummy);
   ms.fp64_matvecs++;
+  if (!std::isfinite(rn) && std::isfinite(bn)) {
+    // a starting guess with a NaN or infinite entry: start from zero
+    Vec xv = as_vec(x);
+    c.blas->zero(xv);
+    rn = residual(c, op64, b, x, r.s, dummy);
+    ms.fp64_matvecs++;
+  }
   Deflation local;
   Deflation &D = defl ? *defl : local;
   bool use_pre = p.precond != nullptr;
   if (D.k > 0 && (D.prec != p.basis || D.k != p.k)) D.clear();
   while (rn >= p.tol * bn && ms.outer < p.max_outer && ms.inner.iters < p.maxiter) {
-    if (op_prec == PREC_DOUBLE && p.basis == PREC_DOUBLE && ms.outer > 0 && ms.stalled) {
+    if (op_prec == PREC_DOUBLE && p.basis == PREC_DOUBLE && ms.outer > 0 && ms.stalled &&
+        p.maxiter > ms.inner.iters) {
       // last resort: plain FP64 GMRES-DR on the true residual equation
       SolveStats st = gmresdr(c, op64, b, x, p.m, p.k, p.tol * bn / rn * 0.5, 0, p.maxiter - ms.inner.iters, nullptr);
       ms.inner.iters += st.iters;
@@ -597,12 +652,22 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
     ms.inner.iters += st.iters;
     ms.inner.matvecs += st.matvecs;
     ms.inner.cycles += st.cycles;
-    c.blas->axpby(1.0, elv, 1.0, xv, false);
+    if (fault.batch_rhs < 0 && ms.outer < fault.corrupt_outer) fill_spinor(c.g, el.s, std::nan(""), std::nan(""), c.stream);
+    // the correction is added only when every entry is finite: a NaN from the
+    // preconditioner or a low-precision overflow must not reach x (x + e with
+    // finite x and e cannot overflow the FP64 residual, ||e|| < 1e154)
+    const double en2 = c.blas->norm2(elv);
     const double rn_old = rn;
-    rn = residual(c, op64, b, x, r.s, dummy);
-    ms.fp64_matvecs++;
+    if (std::isfinite(en2)) {
+      c.blas->axpby(1.0, elv, 1.0, xv, false);
+      rn = residual(c, op64, b, x, r.s, dummy);
+      ms.fp64_matvecs++;
+    } else {
+      ms.rejected++;
+    }
     ms.outer++;
-    ms.stalled = rn > p.stall * rn_old;
+    // written so that a NaN residual counts as a stall
+    ms.stalled = !(rn <= p.stall * rn_old);
     if (ms.stalled && use_pre) {
       // the unpreconditioned solver takes over
       use_pre = false;
@@ -620,6 +685,7 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
       }
       if (op_prec == PREC_DOUBLE && p.basis != PREC_DOUBLE) {
         // the FP32 basis cannot carry an FP64 operator: finish with plain FP64 GMRES-DR
+        if (p.maxiter <= ms.inner.iters) break;
         SolveStats st2 = gmresdr(c, op64, b, x, p.m, p.k, 0.5 * p.tol * bn / rn, 0, p.maxiter - ms.inner.iters,
                                  nullptr);
         ms.inner.iters += st2.iters;
@@ -635,7 +701,13 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
   }
   ms.rel_res = bn > 0 ? rn / bn : 0.0;
   ms.converged = rn < p.tol * bn;
+  last_mixed_stats() = ms;
   return ms;
+}
+
+MixedStats &last_mixed_stats() {
+  static MixedStats m;
+  return m;
 }
 
 void prep_source(Context &c, const double kappa[2], const Spinor &b_e, const Spinor &b_o, Spinor &src) {

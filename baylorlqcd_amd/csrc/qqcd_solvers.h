@@ -1,6 +1,7 @@
 // qqcd_solvers.h -- Krylov solvers on device-resident spinors.
 #pragma once
 #include <functional>
+#include <utility>
 #include <vector>
 
 #include "qqcd_context.h"
@@ -48,6 +49,7 @@ struct SolveStats {
   int cycles = 0;
   double rel_res = 0;  // final ||b - A x|| / ||r0|| with the true residual
   int converged = 0;
+  int nonfinite = 0;   // the true residual became NaN or infinite: the solve stopped there
 };
 
 // Deflation space carried between solves: A V_k = V_{k+1} Hbar_k.
@@ -76,10 +78,15 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
                    int maxiter, Deflation *defl, int reorth = 1, LinOp *M = nullptr,
                    const CycleHook *hook = nullptr);
 
+// Residual history of a solve: (number of operator applications so far, the
+// least-squares residual norm ||c - Hbar y|| after that Arnoldi step).
+typedef std::vector<std::pair<int, double>> ResHistory;
+
 // GMRES-Proj(m, k) with deflation space defl: alternates a projection over
-// (V_{k+1}, Hbar_k) with a GMRES(m) cycle.  x holds x0 on entry.
+// (V_{k+1}, Hbar_k) with a GMRES(m) cycle.  x holds x0 on entry.  hist, if
+// given, receives one entry per Arnoldi step.
 SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, const Deflation &defl, double tol,
-                     int itermin, int maxiter, int reorth = 1);
+                     int itermin, int maxiter, int reorth = 1, ResHistory *hist = nullptr);
 
 // BiCGStab from x (Frommer et al.; the reference's bicgstab).  Convergence
 // on the recursive residual, then confirmed on the true residual (restarting
@@ -92,8 +99,6 @@ SolveStats bicgstab(Context &c, LinOp &A, const Spinor &b, Spinor &x, double tol
 
 // out = a0 in + a1 (i g5) in, FP64, on one checkerboard
 void local_spin(const Geom &g, const Spinor &in, Spinor &out, double a0, double a1, hipStream_t s);
-// x = (re, im) in every complex component of every site (padding untouched)
-void fill_spinor(const Geom &g, Spinor &x, double re, double im, hipStream_t s);
 
 }  // namespace qqcd
 
@@ -136,7 +141,22 @@ struct MixedStats {
   int fallback = 0;  // 1: operator raised to FP32, 2: finished in plain FP64
   int precond_dropped = 0;  // the preconditioner stalled and was switched off
   int precond_steps = 0;    // Krylov steps taken with the preconditioner
+  int rejected = 0;         // inner corrections with a NaN or infinite entry, not added to x
 };
+
+// Failures forced from the host, for tests of the fallbacks: every kernel
+// still runs to completion, only values are replaced.  Process-wide; reset
+// with fault_plan() = FaultPlan().
+struct FaultPlan {
+  int precond = 0;        // output of the learned preconditioner: 1 NaN, 2 zero, 3 random
+  int corrupt_outer = 0;  // mixed solver: the inner corrections of the first n outer steps set to NaN
+  int batch_rhs = -1;     // batched solver: this right-hand side's correction set to NaN in the
+                          // first corrupt_outer outer steps
+  int maxiter = 0;        // > 0: cap on the Krylov steps of a mixed solve
+};
+FaultPlan &fault_plan();
+// statistics of the most recent solve_mixed call
+MixedStats &last_mixed_stats();
 
 MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spinor &x, const MixedParams &p,
                        Deflation *defl);

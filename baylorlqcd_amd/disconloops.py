@@ -148,6 +148,14 @@ def batch_stats():
             "graph_replays": g.value, "fallback": f.value}
 
 
+def batch_rejected():
+    """Per-slot corrections of the last batched solve that were not added to x (NaN or infinite)."""
+    import ctypes
+    r = ctypes.c_int()
+    _lib.check(_lib.lib().qqcd_batch_rejected(ctypes.byref(r)))
+    return r.value
+
+
 def hop_multi_canonical(psi, out, par_out, idag=0, prec=0, recon=18, nact=None, nb=0, order=1):
     """Multi-RHS hop on canonical arrays psi/out of shape (nrhs, V, 4, 3) complex."""
     nrhs = psi.shape[0]

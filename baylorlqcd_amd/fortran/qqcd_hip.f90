@@ -26,9 +26,12 @@
 ! <MODULE>.ERROR file with the message, then stop
 ! (qqcd-test/cfgsprops/quark/diracops.f90:1077-1082).
 !
-! After the host changes the link array in place (heatbath sweep, GFread),
-! it calls qqcd_gauge_changed(u, coact) so the resident copy is refreshed;
-! an array at a new address is picked up without it.
+! The link array is uploaded when an operator call first passes it and kept
+! on the device; the drop-ins that change it in place (sweep, GFread,
+! GFwrite's fixsu3) make the library forget that copy, so the next call
+! uploads it again.  A host that changes u in place by its own code calls
+! qqcd_gauge_changed(u, coact, bc); an array at a new address is picked up
+! without it.
 
 module qqcd_hip
    use iso_c_binding
@@ -38,7 +41,7 @@ module qqcd_hip
    private
 
    public :: Hsingle, Hdouble, Hdbletm, gammamult, gam5x, vecdot, bicgstab, gmresdr, psc, &
-             gmresproject, pointsource, fermprop, qqcd_gauge_changed, qqcd_shutdown, &
+             gmresproject, pointsource, fermprop, fermpropR, qqcd_gauge_changed, qqcd_shutdown, &
              qqcd_solve_mixed_f, z4source, z2source, loopops, loopops_all, nsaverage, &
              GFwrite, GFread, FPwrite, FPread, sweep, smear, twMshift, gamma5mult
 
@@ -101,12 +104,12 @@ module qqcd_hip
          real(c_double), value :: resmax
          integer(c_int), value :: itermin, iflag
       end function
-      integer(c_int) function c_gmresproject(b, x, gm, resmax, itermin, itercount, u, coact, kappa, iflag, bc) &
-            bind(C, name="qqcd_gmresproject")
+      integer(c_int) function c_gmresproject(b, xshift, nshift, gm, resmax, itermin, itercount, u, coact, kappa, &
+                                             iflag, bc, isignal, mvp, gdr, ldgdr) bind(C, name="qqcd_gmresproject")
          import :: c_int, c_ptr, c_double
-         type(c_ptr), value :: b, x, gm, itercount, u, coact, kappa, bc
+         type(c_ptr), value :: b, xshift, gm, itercount, u, coact, kappa, bc, mvp, gdr
          real(c_double), value :: resmax
-         integer(c_int), value :: itermin, iflag
+         integer(c_int), value :: nshift, itermin, iflag, isignal, ldgdr
       end function
       integer(c_int) function c_psc(b, x, gm, resmax, itermin, itercount, u, coact, kappa, iflag, bc, &
                                     isignal, resproj) bind(C, name="qqcd_psc")
@@ -529,15 +532,24 @@ contains
       call append_log(rwdir, myid)
    end subroutine psc
 
+   ! gmresproject(rwdir,b,xshift,GMRES,resmax,itermin,itercount,u,GeeGooinv,iflag,kappa,coact,bc,
+   ! vecbl,vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd,iblv,MRT,MRT2,isignal,mvp,gdr), inverters.f90:
+   ! 7183-7271: GMRES-Proj over the resident deflation space.  Every shift of xshift is zeroed
+   ! and the solution is xshift(:,:,:,:,:,1); isignal = 1 solves for the resident beg instead
+   ! of b (the last deflation vector after psc's GMRES-DR); mvp = operator applications;
+   ! gdr(mvp_j,1) = the least-squares residual norm after the Arnoldi step that was operator
+   ! application mvp_j, other entries of gdr as they were.
    subroutine gmresproject(rwdir,b,xshift,GMRES,resmax,itermin,itercount,u,GeeGooinv,iflag,kappa,coact,bc, &
-                           vecbl,vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd,iblv,MRT,MRT2)
+                           vecbl,vecblinv,myid,nn,ldiv,nms,lvbc,ib,lbd,iblv,MRT,MRT2,isignal,mvp,gdr)
       character(len=*), intent(in) :: rwdir(:)
       real(kind=8), intent(in), target, contiguous :: b(:,:,:,:,:)
-      real(kind=8), intent(inout), target, contiguous :: xshift(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: xshift(:,:,:,:,:,:)
       integer, intent(in), target :: GMRES(:)
       real(kind=8), intent(in) :: resmax
-      integer, intent(in) :: itermin, iflag, myid, MRT, MRT2
+      integer, intent(in) :: itermin, iflag, myid, MRT, MRT2, isignal
       integer, intent(out), target :: itercount
+      integer, intent(inout), target :: mvp
+      real(kind=8), intent(inout), target, contiguous :: gdr(:,:)
       real(kind=8), intent(in), target, contiguous :: u(:,:,:,:,:), kappa(:), coact(:,:,:)
       real(kind=8), intent(in) :: GeeGooinv(:,:,:,:,:)
       integer, intent(in) :: bc(:), nms(:), vecbl(:,:), vecblinv(:,:), nn(:,:), iblv(:,:), lvbc(:,:,:), ib(:,:,:,:)
@@ -545,10 +557,15 @@ contains
       integer(c_int), target :: bb(4), gm(2)
       call check_geometry("INVERTERS", myid, ldiv, vecbl, vecblinv, iblv, lbd)
       call ensure_started(bc)
+      call file_dims("INVERTERS", size(xshift, 2), size(xshift, 1) == 6 .and. size(xshift, 3) == 4 .and. &
+                     size(xshift, 4) == 2 .and. size(xshift, 5) == 8 .and. spinor_shape(b) .and. &
+                     size(b, 2) == size(xshift, 2))
       bb = bc(1:4)
       gm = GMRES(1:2)
-      call fail("INVERTERS", c_gmresproject(c_loc(b), c_loc(xshift), c_loc(gm), resmax, itermin, &
-                                            c_loc(itercount), c_loc(u), c_loc(coact), c_loc(kappa), iflag, c_loc(bb)))
+      call fail("INVERTERS", c_gmresproject(c_loc(b), c_loc(xshift), int(size(xshift, 6), c_int), c_loc(gm), &
+                                            resmax, itermin, c_loc(itercount), c_loc(u), c_loc(coact), c_loc(kappa), &
+                                            iflag, c_loc(bb), isignal, c_loc(mvp), c_loc(gdr), &
+                                            int(size(gdr, 1), c_int)))
    end subroutine gmresproject
 
    ! Mixed-precision solve of the even-odd system (no counterpart in the
@@ -699,6 +716,31 @@ contains
       call append_log(rwdir, myid)
    end subroutine fermprop
 
+   ! fermpropR(rwdir,be,bo,nkappa,...,MRT,MRT2,ir), quark.f90:846-931: fermprop with the Wilson
+   ! r coefficient ir; ir = -1 negates coact(3,1,:) (:921-926), any other value leaves it.
+   subroutine fermpropR(rwdir,be,bo,nkappa,kappa,cSW,inverter,coact,bc,resmax,itermin,omegaM3R,u,vectype, &
+                        kGMRES,xe,xo,myid,nn,ldiv,nms,lvbc,lvcl,ib,lbd,iblv,vecbl,vecblinv,ntmqcd,hoption,MRT, &
+                        MRT2,ir)
+      character(len=*), intent(in) :: rwdir(:)
+      real(kind=8), intent(in), target, contiguous :: be(:,:,:,:,:)
+      real(kind=8), intent(inout), target, contiguous :: bo(:,:,:,:,:)
+      integer, intent(in) :: nkappa, hoption, inverter, kGMRES, ntmqcd, itermin, myid, MRT, MRT2, ir
+      real(kind=8), intent(in), target, contiguous :: kappa(:)
+      real(kind=8), intent(in) :: cSW, resmax, omegaM3R
+      real(kind=8), intent(in) :: coact(:,:,:)
+      integer, intent(in) :: bc(:), nms(:)
+      real(kind=8), intent(inout), target, contiguous :: u(:,:,:,:,:)
+      integer, intent(inout) :: vectype
+      real(kind=8), intent(inout), target, contiguous :: xe(:,:,:,:,:,:), xo(:,:,:,:,:,:)
+      integer, intent(in) :: nn(:,:), iblv(:,:), vecbl(:,:), vecblinv(:,:), lvbc(:,:,:), lvcl(:,:,:), ib(:,:,:,:)
+      logical, intent(in) :: ldiv(:), lbd(:,:)
+      real(kind=8), target :: coactr(4,4,4)
+      coactr = coact(1:4,1:4,1:4)
+      if (ir == -1) coactr(3,1,:) = -coact(3,1,1:4)
+      call fermprop(rwdir,be,bo,nkappa,kappa,cSW,inverter,coactr,bc,resmax,itermin,omegaM3R,u,vectype, &
+                    kGMRES,xe,xo,myid,nn,ldiv,nms,lvbc,lvcl,ib,lbd,iblv,vecbl,vecblinv,ntmqcd,hoption,MRT,MRT2)
+   end subroutine fermpropR
+
    ! ---- gauge fields ----------------------------------------------------
 
    ! sweep(gaction,coact,nhit,u,myid,nn,ldiv,nms,lv,ib,lbd,iblv,MRT), cfgsprops.f90:1973: one
@@ -740,7 +782,8 @@ contains
 
    ! GFwrite(u,filename,gaction,beta,alat,tad,icfgsave,myid), gaugetools.f90:1260: fixsu3 on u in
    ! place, rows 1-2 of every link as unformatted records, then the trailer; the file must be new.
-   ! After it the host calls qqcd_gauge_changed (u was re-unitarised).
+   ! The library forgets its device copy of u (re-unitarised in place); the next operator call
+   ! that passes u uploads it again.
    subroutine GFwrite(u,filename,gaction,beta,alat,tad,icfgsave,myid)
       real(kind=8), intent(inout), target, contiguous :: u(:,:,:,:,:)
       character(len=*), intent(in) :: filename
@@ -759,7 +802,7 @@ contains
    end subroutine GFwrite
 
    ! GFread(u,filename,gaction,beta,alat,tad,icfgsave,myidin), gaugetools.f90:1298: rows 1-2 from
-   ! the file, row 3 rebuilt by fixsu3.  After it the host calls qqcd_gauge_changed.
+   ! the file, row 3 rebuilt by fixsu3.  The library forgets its device copy of u, as after GFwrite.
    subroutine GFread(u,filename,gaction,beta,alat,tad,icfgsave,myidin)
       real(kind=8), intent(inout), target, contiguous :: u(:,:,:,:,:)
       character(len=*), intent(in) :: filename

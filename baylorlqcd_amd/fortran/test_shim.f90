@@ -30,7 +30,7 @@ program test_shim
    logical :: ldiv(4), lbd(16,4)
    character(len=8) :: rwdir(1)
    character(len=32) :: sel
-   logical :: do_ops, do_noise, do_files, do_sweep, do_smear
+   logical :: do_ops, do_noise, do_files, do_sweep, do_smear, do_bnd
    integer :: src(4)
 
    allocate(u(18,ntotal,4,2,16), be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8), g(6,ntotal,4,2,8), h(6,ntotal,4,2,8))
@@ -45,6 +45,7 @@ program test_shim
    do_files = len_trim(sel) == 0 .or. index(sel, "files") > 0
    do_sweep = len_trim(sel) == 0 .or. index(sel, "sweep") > 0
    do_smear = len_trim(sel) == 0 .or. index(sel, "smear") > 0
+   do_bnd = len_trim(sel) == 0 .or. index(sel, "boundary") > 0
    ! single-process geometry tables (setblk / gblchker of the reference)
    vecbl(1,:) = [1, 4, 6, 7, 10, 11, 13, 16]
    vecbl(2,:) = [2, 3, 5, 8, 9, 12, 14, 15]
@@ -75,15 +76,22 @@ program test_shim
    if (do_files) call shim_files()
    if (do_sweep) call shim_sweep()
    if (do_smear) call shim_smear()
-   if (do_ops .or. do_noise .or. do_sweep .or. do_smear) call qqcd_shutdown()
+   if (do_bnd) call shim_boundary()
+   if (do_ops .or. do_noise .or. do_sweep .or. do_smear .or. do_bnd) call qqcd_shutdown()
    print *, "test_shim done"
 
 contains
 
+   ! each step is printed and appended to steps.log (opened, written and closed
+   ! every time), so a run that stops anywhere names its last step
    subroutine step(name)
       character(len=*), intent(in) :: name
+      integer :: unit
       print *, "step ", name
       flush(6)
+      open(newunit=unit, file="steps.log", position="append", action="write", form="formatted")
+      write(unit, "(a)") name
+      close(unit)
    end subroutine step
 
    subroutine ops()
@@ -179,6 +187,62 @@ contains
                       iblv, 0)
       call writebin("g5m.bin", g, size(g))
    end subroutine shim_smear
+
+   ! The reference's argument lists of psc, gmresproject (with xshift, isignal, mvp, gdr) and
+   ! fermpropR, and GFread of a second configuration into the array the operators used before.
+   subroutine shim_boundary()
+      real(kind=8), allocatable :: xs(:,:,:,:,:,:), gdr(:,:), p1e(:,:,:,:,:), p1o(:,:,:,:,:), p2e(:,:,:,:,:), &
+                                   p2o(:,:,:,:,:), x1(:,:,:,:,:)
+      real(kind=8) :: beta, alat(4), tad(4), meta(3)
+      integer :: mvp, it1, it2, gaction(4), icfg, myidin
+      call step("boundary")
+      allocate(xs(6,ntotal,4,2,8,2), gdr(2000,1), p1e(6,ntotal,4,2,8), p1o(6,ntotal,4,2,8), &
+               p2e(6,ntotal,4,2,8), p2o(6,ntotal,4,2,8), x1(6,ntotal,4,2,8))
+      call pointsource(p1e, p1o, [1, 1, 1, 1], 1, 1, 0, iblv, vecblinv)
+      call pointsource(p2e, p2o, [3, 2, 4, 5], 3, 2, 0, iblv, vecblinv)
+      kappa = [0.150d0, 0.0d0]
+      ! first right-hand side: psc with isignal = 1 (GMRES-DR(40,10)), the deflation space stays
+      isignal = 1
+      x1 = 0.0d0
+      call psc(rwdir, p1e, x1, [40, 10], 1.0d-11, 0, it1, u, gdummy, -1, kappa, coact, bc, vecbl, vecblinv, 0, &
+               nn, ldiv, nms, lvbc, ib, lbd, iblv, 0, 0)
+      ! a second one through gmresproject directly, GMRES-Proj(30,10), two shifts
+      call step("gmresproject")
+      xs = 7.0d0
+      gdr = -1.0d0
+      mvp = -5
+      call gmresproject(rwdir, p2e, xs, [30, 10], 1.0d-10, 0, it2, u, gdummy, -1, kappa, coact, bc, vecbl, &
+                        vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0, 0, 2, mvp, gdr)
+      call writebin("proj_x.bin", xs, size(xs))
+      call writebin("proj_gdr.bin", gdr, size(gdr))
+      meta = [dble(it1), dble(it2), dble(mvp)]
+      call writebin("proj_meta.bin", meta, 3)
+      ! isignal = 1: the right-hand side is the resident beg (b of the previous gmresproject)
+      xs = 7.0d0
+      mvp = -5
+      call gmresproject(rwdir, p1e, xs, [30, 10], 1.0d-10, 0, it2, u, gdummy, -1, kappa, coact, bc, vecbl, &
+                        vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0, 0, 1, mvp, gdr)
+      call writebin("proj_beg_x.bin", xs(:,:,:,:,:,1), size(xs(:,:,:,:,:,1)))
+      ! fermpropR with Wilson r = -1 on the config-1 point source, GMRES-DR(40,10)
+      call step("fermpropR")
+      vectype = 40
+      isignal = 1
+      xe = 0.0d0
+      xo = 0.0d0
+      call fermpropR(rwdir, p1e, p1o, -1, kappa, 0.0d0, 5, coact, bc, 1.0d-11, 0, 1.0d0, u, vectype, 10, xe, xo, &
+                     0, nn, ldiv, nms, lvbc, lvbc, ib, lbd, iblv, vecbl, vecblinv, 0, 0, 0, 0, -1)
+      call writebin("fermpropR_xe.bin", xe, size(xe))
+      call writebin("fermpropR_xo.bin", xo, size(xo))
+      ! a second configuration (cfg2.gf, in the reference's file format) read into u, which the
+      ! operators above used and the library holds on the device: Hsingle must see the new links
+      call step("gfread")
+      g = 0.0d0
+      call Hsingle(g, u, be, 0, coact, bc, 2, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0)
+      call GFread(u, "cfg2.gf", gaction, beta, alat, tad, icfg, myidin)
+      call writebin("gfread_u2.bin", u, size(u))
+      call Hsingle(g, u, be, 0, coact, bc, 2, vecbl, vecblinv, 0, nn, ldiv, nms, lvbc, ib, lbd, iblv, 0)
+      call writebin("hsingle_cfg2.bin", g, size(g))
+   end subroutine shim_boundary
 
    subroutine write_state(name)
       use lagfib, only: iwlagfib, jrlagfib, krlagfib

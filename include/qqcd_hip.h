@@ -152,11 +152,21 @@ int qqcd_pointsource(double *be, double *bo, const int src[4], int icsrc, int id
 int qqcd_gmresdr(const double *phi, double *x, const int GMRES[2], double resmax, int itermin, int *itercount,
                  const double *u, const double *coact, const double kappa[2], int iflag, const int *bc);
 
-/* gmresproject(...), inverters.f90:7183-8304: GMRES-Proj(m - k, k) with
- * the resident deflation space; x0 = 0 (:7347). */
-int qqcd_gmresproject(const double *b, double *x, const int GMRES[2], double resmax, int itermin,
+/* gmresproject(rwdir,b,xshift,GMRES,resmax,itermin,itercount,u,GeeGooinv,
+ * iflag,kappa,coact,bc,...,MRT,MRT2,isignal,mvp,gdr), inverters.f90:
+ * 7183-7271 (body to :8304): GMRES-Proj with cycle length GMRES[0] over the
+ * resident deflation space of k = GMRES[1] vectors, x0 = 0.
+ * xshift: nshift reference-order spinors, all zeroed (:7347), the solution
+ * in the first.  isignal == 1: the right-hand side is the resident beg
+ * (the last deflation vector after psc's GMRES-DR, :4437; b after a
+ * gmresproject with isignal != 1, :7438-7443), b is not read.
+ * mvp: operator applications of this call (reset to 0 first, :7385).
+ * gdr: column 1 of the reference's gdr(:,:), ldgdr entries: gdr[mvp_j - 1]
+ * = the least-squares residual norm after the Arnoldi step that was
+ * operator application mvp_j (:7981); other entries are left as they are. */
+int qqcd_gmresproject(const double *b, double *xshift, int nshift, const int GMRES[2], double resmax, int itermin,
                       int *itercount, const double *u, const double *coact, const double kappa[2], int iflag,
-                      const int *bc);
+                      const int *bc, int isignal, int *mvp, double *gdr, int ldgdr);
 
 /* psc(...), inverters.f90:4310-4465: isignal = 1 runs GMRES-DR(m, k) and
  * keeps its deflation space; otherwise GMRES-Proj(m-k, k) with the
@@ -402,6 +412,9 @@ int qqcd_batch_set(int reorth, int use_graph, double inner_tol, int inner_cycles
 /* Statistics of the last batched solve. */
 int qqcd_batch_stats(int *outer, int *cycles, long *slot_matvecs, long *batched_matvecs, int *graph_replays,
                      int *fallback);
+/* Per-slot corrections of the last batched solve that held a NaN or an
+ * infinity and were not added to x. */
+int qqcd_batch_rejected(int *rejected);
 
 /* ---- learned preconditioner (no counterpart in the reference) ---------
  * A gauge-equivariant network of parallel-transport convolution layers:
@@ -461,6 +474,26 @@ int qqcd_bench_nn(int nwarm, int niter, double *ms);
 int qqcd_last_solve_stats(double *rel_res, int *iters, int *matvecs, int *cycles, int *converged);
 /* Drop the resident deflation space (the next psc must use isignal = 1). */
 int qqcd_deflation_clear(void);
+
+/* Statistics of the last mixed-precision solve (any entry that runs one):
+ * outer FP64 steps; fallback 0 none, 1 operator raised to FP32, 2 finished
+ * by plain FP64 GMRES-DR; precond_dropped 1 when the learned preconditioner
+ * stalled and was switched off; rejected = inner corrections with a NaN or
+ * infinite entry that were not added to x; converged; ||b - A x|| / ||b||.
+ * Any pointer may be NULL.  No reference counterpart: the reference stops
+ * with INVERTERS.ERROR instead (qqcd-test/cfgsprops/quark/inverters.f90:
+ * 13128-13133). */
+int qqcd_mixed_stats(int *outer, int *fallback, int *precond_dropped, int *rejected, int *converged,
+                     double *rel_res);
+
+/* Failures forced from the host for tests of the fallbacks (process-wide,
+ * all zero / -1 to switch off).  precond: the learned preconditioner's
+ * output replaced by 1 NaN, 2 zero, 3 random values; corrupt_outer: the
+ * inner corrections of the first n outer steps of a mixed solve set to NaN
+ * (in the batched solver: the correction of right-hand side batch_rhs);
+ * maxiter > 0 caps the Krylov steps of a mixed solve.  Every kernel still
+ * runs to completion; only values are replaced.  No reference counterpart. */
+int qqcd_set_fault(int precond, int corrupt_outer, int batch_rhs, int maxiter);
 
 /* ---- native (device-resident) interface: benchmarks and tests ---------- */
 
@@ -528,6 +561,14 @@ int qqcd_bench_blas(int prec, int k, int op, int grid, int niter, double *ms);
 
 /* Toggle the XCD-aware workgroup order of the Dslash (default on). */
 int qqcd_set_xcd_remap(int on);
+
+/* Launch shape of the single-right-hand-side Dslash (projector form): depth
+ * 0 computes the eight hops one after the other, 1/2/4 issues that many
+ * hops' loads ahead; order 1 sweeps blocks of bz z-planes in t (bz 0: the
+ * largest of 16, 8, 4, 2 dividing nz; order 0: the checkerboard index); nt 1
+ * streams links and output past the L2.  Default (1, 0, 2, 1).  Tuning aid;
+ * no reference counterpart. */
+int qqcd_set_dslash_tune(int order, int bz, int depth, int nt);
 
 /* ---- host dense linear algebra of the solvers (verification hooks) ------
  * Column-major complex matrices as interleaved (re, im) doubles.  They

@@ -205,6 +205,61 @@ contains
       end select
    end subroutine ref_solve
 
+   ! gmresproject(rwdir,b,xshift,GMRES,resmax,itermin,itercount,u,GeeGooinv,iflag,kappa,coact,bc,
+   ! ...,MRT,MRT2,isignal,mvp,gdr) called directly (inverters.f90:7183-7271), over the
+   ! deflation space a previous gmresdr (through psc with isignal=1) left in module shift:
+   ! phi -> xshift(:,:,:,:,:,1..nshift), itercount, mvp and gdr(1:ngdr,1:ncol).
+   subroutine ref_gmresproject(phi, xshift, nshift, nk, resmax, itermin, itercount, u, coact, kappa, &
+                               isig, mvp, gdr, ngdr, ncol) bind(C, name="ref_gmresproject")
+      integer(c_int), value         :: nshift, itermin, isig, ngdr, ncol
+      real(c_double), intent(inout) :: phi(6,ntotal,4,2,8), xshift(6,ntotal,4,2,8,nshift)
+      integer(c_int), intent(in)    :: nk(2)
+      real(c_double), value         :: resmax
+      integer(c_int), intent(out)   :: itercount
+      integer(c_int), intent(inout) :: mvp
+      real(c_double), intent(inout) :: gdr(ngdr,ncol)
+      real(c_double), intent(in)    :: u(18,ntotal,4,2,16), coact(4,4,4), kappa(2)
+      real(kind=KR) :: gdummy(18,1,1,1,1)
+      integer(kind=KI) :: MRT, MRT2, iflag, gm(2), isg
+      MRT = MPI_DOUBLE_PRECISION
+      MRT2 = MPI_DOUBLE_PRECISION
+      iflag = -1
+      gm = nk
+      isg = isig
+      gdummy = 0.0_KR
+      call gmresproject(rwdir, phi, xshift, gm, resmax, itermin, itercount, u, gdummy, iflag, kappa, &
+                        coact, gbc, vecbl, vecblinv, myid, nn, ldiv, nms, lvbc, ib, lbd, iblv, MRT, MRT2, &
+                        isg, mvp, gdr)
+   end subroutine ref_gmresproject
+
+   ! fermpropR(...,ir) (quark.f90:846-931): fermprop with coact(3,1,:) negated for ir = -1.
+   subroutine ref_fermpropr(be, bo, kappa, inverter, coact, resmax, itermin, nk, u, xe, xo, &
+                            isig, ir) bind(C, name="ref_fermpropr")
+      real(c_double), intent(inout) :: be(6,ntotal,4,2,8), bo(6,ntotal,4,2,8)
+      real(c_double), intent(inout) :: xe(6,ntotal,4,2,8,1), xo(6,ntotal,4,2,8,1)
+      real(c_double), intent(in)    :: kappa(2), coact(4,4,4)
+      real(c_double), intent(inout) :: u(18,ntotal,4,2,16)
+      integer(c_int), value         :: inverter, itermin, isig, ir
+      real(c_double), value         :: resmax
+      integer(c_int), intent(in)    :: nk(2)
+      integer(kind=KI) :: MRT, MRT2, vectype, kg, ntm, hopt, irr
+      real(kind=KR2)   :: om
+      real(kind=KR)    :: csw
+      MRT = MPI_DOUBLE_PRECISION
+      MRT2 = MPI_DOUBLE_PRECISION
+      vectype = nk(1)
+      kg = nk(2)
+      ntm = 0
+      hopt = 0
+      om = 1.0_KR2
+      csw = 0.0_KR
+      irr = ir
+      isignal = isig
+      call fermpropR(rwdir, be, bo, -1, kappa, csw, inverter, coact, gbc, resmax, itermin, om, &
+                     u, vectype, kg, xe, xo, myid, nn, ldiv, nms, lvbc, lv, ib, lbd, iblv, vecbl, &
+                     vecblinv, ntm, hopt, MRT, MRT2, irr)
+   end subroutine ref_fermpropr
+
    ! fermprop for twisted/Wilson (nkappa=-1): be, bo -> xe, xo.
    subroutine ref_fermprop(be, bo, kappa, inverter, coact, resmax, itermin, nk, u, xe, xo, &
                            isig) bind(C, name="ref_fermprop")

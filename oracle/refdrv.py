@@ -192,6 +192,33 @@ class RefLattice:
                          _p(np.ascontiguousarray(kappa, dtype=np.float64)), int(isignal))
         return x, it.value
 
+    def gmresproject(self, phi, u, coact, kappa, nk, resmax, itermin=0, isignal=2, nshift=1, ngdr=2000, ncol=1):
+        """The reference's gmresproject called directly (after psc with isignal=1 built
+        its deflation space): returns xshift (slen, nshift), itercount, mvp, gdr (ngdr, ncol)."""
+        xs = np.full(self.slen * nshift, 7.0)  # the reference zeroes every shift
+        ph = np.array(phi, dtype=np.float64, copy=True)
+        it = ctypes.c_int()
+        mvp = ctypes.c_int(-5)
+        gdr = np.zeros((ngdr, ncol), order="F")
+        nkk = np.ascontiguousarray(nk, dtype=np.int32)
+        _run(self.L.ref_gmresproject, _p(ph), _p(xs), int(nshift), _ip_(nkk), ctypes.c_double(resmax),
+             int(itermin), ctypes.byref(it), _p(np.ascontiguousarray(u)), _p(np.ascontiguousarray(coact)),
+             _p(np.ascontiguousarray(kappa, dtype=np.float64)), int(isignal), ctypes.byref(mvp), _p(gdr),
+             int(ngdr), int(ncol))
+        return xs.reshape(nshift, self.slen), it.value, mvp.value, gdr
+
+    def fermpropr(self, be, bo, u, coact, kappa, inverter, ir, nk=(40, 10), resmax=1e-10, itermin=0, isignal=1):
+        xe = np.zeros(self.slen)
+        xo = np.zeros(self.slen)
+        b_e = np.array(be, dtype=np.float64, copy=True)
+        b_o = np.array(bo, dtype=np.float64, copy=True)
+        uu = np.array(u, dtype=np.float64, copy=True)
+        nkk = np.ascontiguousarray(nk, dtype=np.int32)
+        _run(self.L.ref_fermpropr, _p(b_e), _p(b_o), _p(np.ascontiguousarray(kappa, dtype=np.float64)),
+             int(inverter), _p(np.ascontiguousarray(coact)), ctypes.c_double(resmax), int(itermin), _ip_(nkk),
+             _p(uu), _p(xe), _p(xo), int(isignal), int(ir))
+        return xe, xo
+
     def fermprop(self, be, bo, u, coact, kappa, inverter, nk=(40, 10), resmax=1e-10, itermin=0,
                  isignal=1, xe0=None):
         xe = np.zeros(self.slen) if xe0 is None else np.array(xe0, dtype=np.float64, copy=True)

@@ -428,8 +428,49 @@ def noiseloop():
     print("wrote golden_noiseloop_8x8x8x16.npz", np.abs(J[:, 0, :]).max())
 
 
+BOUNDARY_CASE = dict(kappa=0.150, nk=(40, 10), resmax_dr=1e-11, resmax_proj=1e-10)
+
+
+def boundary():
+    """The reference's gmresproject called directly with its whole argument list
+    (inverters.f90:7183-7271) after psc (isignal=1) built the deflation space, and
+    fermpropR with ir = -1 (quark.f90:846-931), on the config-1 field."""
+    n = (8, 8, 8, 16)
+    refdrv.build(n, solvers=True)
+    g = np.load(os.path.join(HERE, "golden_8x8x8x16.npz"))
+    u = O.links_canon_to_ref(n, gauge.reconstruct12(g["links12"]))
+    co = O.coact_array()
+    cs = BOUNDARY_CASE
+    kap = (cs["kappa"], 0.0)
+    R = refdrv.RefLattice(n, bc=(1, 1, 1, -1))
+    out = {}
+    # even-odd right-hand sides: the even halves of two point sources
+    p1e, _ = R.pointsource((1, 1, 1, 1), 1, 1)
+    p2e, _ = R.pointsource((3, 2, 4, 5), 2, 3)
+    x1, it1 = R.solve(6, p1e, u, co, kap, nk=cs["nk"], resmax=cs["resmax_dr"], isignal=1)
+    m, k = cs["nk"]
+    xs, it, mvp, gdr = R.gmresproject(p2e, u, co, kap, (m - k, k), cs["resmax_proj"], isignal=2, nshift=2)
+    out["proj_phi_e"] = canon_even(n, p2e)
+    out["proj_x"] = canon_even(n, xs[0])
+    out["proj_x_shift2_absmax"] = np.float64(np.abs(xs[1]).max())
+    out["proj_itercount"] = np.int32(it)
+    out["proj_mvp"] = np.int32(mvp)
+    out["proj_gdr"] = gdr[:max(mvp, 1), 0].copy()
+    out["case"] = np.array([cs["kappa"], m, k, cs["resmax_dr"], cs["resmax_proj"]])
+    # fermpropR, Wilson r = -1, GMRES-DR(40,10): the config-1 point source
+    ps_e, ps_o = R.pointsource((1, 1, 1, 1), 1, 1)
+    xe, xo = R.fermpropr(ps_e, ps_o, u, co, kap, 5, -1, nk=cs["nk"], resmax=cs["resmax_dr"], isignal=1)
+    out["fermpropR_xe"] = canon_even(n, xe)
+    out["fermpropR_xo"] = canon_odd(n, xo)
+    np.savez_compressed(os.path.join(HERE, "golden_boundary_8x8x8x16.npz"), **out)
+    print("wrote golden_boundary_8x8x8x16.npz: gmresproject itercount", it, "mvp", mvp, "gdr[-1]",
+          out["proj_gdr"][-1], "rhs1 iters", it1)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["loops"]:
+    if sys.argv[1:] == ["boundary"]:
+        boundary()
+    elif sys.argv[1:] == ["loops"]:
         loops()
     elif sys.argv[1:] == ["noiseloop"]:
         noiseloop()

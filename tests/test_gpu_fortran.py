@@ -62,15 +62,29 @@ def _inputs(tmp_path):
     return g, U, u, psi, be, bo, O.spinor_ref_to_canon(N, ze, zo)
 
 
-def _run(tmp_path, section):
+def _run(tmp_path, section, timeout=120):
+    """Run the Fortran host in its own session; on a time-out the whole group
+    is killed and the steps it reached (steps.log, written and closed step by
+    step) are reported."""
     if not os.path.exists(EXE):
         pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
     print(f"[fortran] running test_shim {section}", flush=True)
-    out = subprocess.run([EXE, section], cwd=tmp_path, capture_output=True, text=True, timeout=120)
-    print(out.stdout[-2000:], out.stderr[-2000:])
+    proc = subprocess.Popen([EXE, section], cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                            start_new_session=True)
+    try:
+        stdout, stderr = proc.communicate(timeout=timeout)
+    except subprocess.TimeoutExpired:
+        os.killpg(proc.pid, 9)
+        try:
+            proc.communicate(timeout=30)
+        except subprocess.TimeoutExpired:
+            pass
+        steps = (tmp_path / "steps.log").read_text() if (tmp_path / "steps.log").exists() else ""
+        pytest.fail(f"test_shim {section} did not finish in {timeout} s; steps reached: {steps.split()}")
+    print(stdout[-2000:], stderr[-2000:])
     for f in tmp_path.glob("*.ERROR"):
         pytest.fail(f.read_text())
-    assert out.returncode == 0 and "test_shim done" in out.stdout
+    assert proc.returncode == 0 and "test_shim done" in stdout
 
 
 def test_fortran_host_ops(tmp_path):
@@ -129,6 +143,52 @@ def test_fortran_host_ops(tmp_path):
     wt = O.spacesum(N, O.loopops(N, U, BC, psi, z, 0), nmom=1)[:, 0, :]
     wt[:, :4] *= 0.15
     assert rel(Jt[:, 0, :], wt) < 1e-12 and np.all(Jt[:, 1:, :] == 0)
+
+
+def test_fortran_host_boundary(tmp_path):
+    """psc, gmresproject with the reference's whole dummy list (rank-6 xshift,
+    isignal, mvp, gdr), fermpropR (ir = -1) and GFread into the array the
+    operators already hold on the device, through the Fortran drop-in; against
+    the reference's own gmresproject and fermpropR (golden_boundary_8x8x8x16.npz)."""
+    g, U, u, psi, be, bo, z = _inputs(tmp_path)
+    gb = np.load(os.path.join(ROOT, "tests", "golden", "golden_boundary_8x8x8x16.npz"))
+    # the second configuration, in the reference's file format
+    u2 = O.links_canon_to_ref(N, gauge.haar(N, 424242)).ravel(order="F").copy()
+    gaugetools.gfwrite(N, u2, tmp_path / "cfg2.gf", (1, 1, 1, 1), 6.0, (1, 1, 1, 1), (1, 1, 1, 1), 2, 0)
+    _run(tmp_path, "boundary")
+    steps = (tmp_path / "steps.log").read_text().split()
+    assert steps == ["boundary", "gmresproject", "fermpropR", "gfread"]
+    odd = O.parity_mask(N)
+    nvh, ntot = O.ref_sizes(N)
+    slen = 6 * ntot * 64
+    xs = np.fromfile(tmp_path / "proj_x.bin").reshape(2, slen)
+    x1 = O.spinor_ref_to_canon(N, xs[0], None)[~odd]
+    assert np.all(xs[1] == 0)
+    assert rel(x1, gb["proj_x"]) < 1e-8
+    it1, it2, mvp = (int(v) for v in np.fromfile(tmp_path / "proj_meta.bin"))
+    gdr = np.fromfile(tmp_path / "proj_gdr.bin")
+    print(f"[fortran] gmresproject: itercount {it2} mvp {mvp} (reference {int(gb['proj_itercount'])}, "
+          f"{int(gb['proj_mvp'])}), psc first rhs {it1}", flush=True)
+    assert 0 < it2 < mvp <= 2 * int(gb["proj_mvp"])
+    written = gdr != -1.0
+    assert written.sum() == it2 and not written[mvp:].any()
+    # the same quantity as the reference's gdr: decreasing within a cycle, below resmax at the end
+    assert gdr[written][-1] < 1e-10 and np.all(gdr[written] > 0)
+    assert abs(gdr[written][0] - gb["proj_gdr"][gb["proj_gdr"] > 0][0]) < 1e-3 * gb["proj_gdr"].max()
+    # isignal = 1 solves for the resident beg, here the previous call's b
+    xb = np.fromfile(tmp_path / "proj_beg_x.bin")
+    assert rel(O.spinor_ref_to_canon(N, xb, None)[~odd], x1) < 1e-12
+    # fermpropR (Wilson r = -1)
+    xe = O.spinor_ref_to_canon(N, np.fromfile(tmp_path / "fermpropR_xe.bin"), None)
+    xo = O.spinor_ref_to_canon(N, None, np.fromfile(tmp_path / "fermpropR_xo.bin"))
+    assert rel(xe[~odd], gb["fermpropR_xe"]) < 1e-8
+    assert rel(xo[odd], gb["fermpropR_xo"]) < 1e-8
+    # GFread into u: the operator after it works on the new links
+    u2r = np.fromfile(tmp_path / "gfread_u2.bin")
+    assert np.max(np.abs(u2r.reshape(u2.shape) - u2)) < 1e-14
+    co = O.coact_array()
+    want = O.hsingle_ref(N, u2r, co, BC, 2, 0, be)
+    assert rel(rows(np.fromfile(tmp_path / "hsingle_cfg2.bin")), rows(want)) < 1e-13
 
 
 def test_fortran_host_sweep(tmp_path):

@@ -144,13 +144,24 @@ def test_gmresdr_and_proj_direct(cfg1):
     r = psi - O.schur(N8, U, BC, [1] * 4, [1] * 4, 0, kap, xe)
     even = ~O.parity_mask(N8)
     assert np.linalg.norm(r[even]) / np.linalg.norm(psi[even]) < 1e-11
-    x2 = np.zeros_like(phi)
-    it2 = inverters.gmresproject(None, phi, x2, (22, 8), 1e-11, 0, u, None, -1, kap, co, BC)
-    xe2 = O.spinor_ref_to_canon(N8, x2, None)
+    # rank-6 xshift with two shifts (both zeroed, the solution in the first), mvp and gdr
+    x2 = np.full(phi.shape + (2,), 3.0, order="F")
+    mvp = np.zeros(1, np.int32)
+    gdr = np.full((400, 2), -1.0, order="F")
+    it2 = inverters.gmresproject(None, phi, x2, (22, 8), 1e-11, 0, u, None, -1, kap, co, BC, isignal=2, mvp=mvp,
+                                 gdr=gdr)
+    xe2 = O.spinor_ref_to_canon(N8, x2[..., 0], None)
     r2 = psi - O.schur(N8, U, BC, [1] * 4, [1] * 4, 0, kap, xe2)
-    print(f"direct: GMRES-DR(30,8) {it} its, GMRES-Proj(22,8) on the same system {it2} its")
+    print(f"direct: GMRES-DR(30,8) {it} its, GMRES-Proj(22,8) on the same system {it2} its, mvp {mvp[0]}")
     assert np.linalg.norm(r2[even]) / np.linalg.norm(psi[even]) < 1e-11
     assert it2 < it
+    assert np.all(x2[..., 1] == 0)
+    # the history: one entry per Arnoldi step at its operator-application index, the rest untouched
+    assert it2 < mvp[0] <= it2 + 40
+    h = gdr[:mvp[0], 0]
+    written = h != -1.0
+    assert written.sum() == it2 and np.all(gdr[mvp[0]:, 0] == -1.0) and np.all(gdr[:, 1] == -1.0)
+    assert np.all(h[written] > 0) and h[written][-1] < 1e-10 * np.linalg.norm(psi[even])
 
 
 def test_errors_are_reported(cfg1):
