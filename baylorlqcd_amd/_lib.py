@@ -44,6 +44,7 @@ SIGNATURES = {
     "qqcd_fermprop": [_dp, _dp, _dp, ctypes.c_int, _dp, _ip, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int,
                       _dp, _dp, _dp, ctypes.c_int, _ip],
     "qqcd_bench_dslash": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp],
+    "qqcd_bench_dslash_rot": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
     "qqcd_hop_canonical": [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qqcd_schur_canonical": [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qqcd_set_xcd_remap": [ctypes.c_int],

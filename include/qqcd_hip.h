@@ -506,6 +506,11 @@ int qqcd_set_fault(int precond, int corrupt_outer, int batch_rhs, int maxiter);
  * same input in *relerr.  which = 0: one hop H_oe; 1: the full Schur
  * operator (two hops + epilogue). */
 int qqcd_bench_dslash(int prec, int recon, int which, int nwarm, int niter, double *ms, double *relerr);
+/* The hop H_oe (average ms per launch) cycled over nbuf (input, output)
+ * spinor pairs, so that at nbuf >= 2 on 32^3x64 a launch's fields are not
+ * left in the last-level cache by the launch before.  Tuning aid; no
+ * reference counterpart. */
+int qqcd_bench_dslash_rot(int prec, int recon, int nbuf, int nwarm, int niter, double *ms);
 
 /* Apply the hop H (par_out = 0: H_eo, 1: H_oe) to a canonical-order spinor
  * psi[((s*4 + d)*3 + c)] complex, at storage precision (prec, recon);
