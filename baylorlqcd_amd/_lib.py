@@ -78,6 +78,7 @@ SIGNATURES = {
     "qqcd_loopops": [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp],
     "qqcd_lagfib_init": [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip],
     "qqcd_rnds": [ctypes.c_long, _dp, _ip, _ip, _ip],
+    "qqcd_rnds_noalloc": [ctypes.c_long, _dp, _ip, _ip, _ip],
     "qqcd_z4source_lagfib": [_dp, _dp, _ip, _ip, _ip],
     "qqcd_z2source_lagfib": [_dp, _dp, _ip, _ip, _ip],
     "qqcd_sweep": [_ip, _dp, ctypes.c_int, ctypes.c_int, _dp, _ip, _ip, _ip],

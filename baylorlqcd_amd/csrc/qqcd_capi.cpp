@@ -43,6 +43,7 @@ extern "C" int qqcd_init(const int n[4], const int bc[4], int device) {
   QQCD_HIP(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) throw std::runtime_error("no such HIP device");
   QQCD_HIP(hipSetDevice(device));
+  lagfib_noalloc_setup(device);
   Context *c = new Context();
   c->device = device;
   Geom &g = c->g;

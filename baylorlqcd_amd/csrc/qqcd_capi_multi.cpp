@@ -223,6 +223,15 @@ extern "C" int qqcd_rnds(long m, double *rn, int *iw, int *jr, int *kr) {
   API_END
 }
 
+extern "C" int qqcd_rnds_noalloc(long m, double *rn, int *iw, int *jr, int *kr) {
+  API_BEGIN
+  Context &c = ctx();
+  LagState st = lag_in(iw, jr, kr);
+  rnds_noalloc(st, m, rn, c.device, c.stream);
+  lag_out(st, iw, jr, kr);
+  API_END
+}
+
 extern "C" int qqcd_z4source_lagfib(double *be, double *bo, int *iw, int *jr, int *kr) {
   return lag_source(be, bo, iw, jr, kr, 0);
 }

@@ -32,5 +32,11 @@ constexpr double LAG_FNORM = (double)0.465661e-9f;
 double bench_lagfib(long m, int niter, long chunk, hipStream_t s);
 // drop the cached jump-ahead tables (device memory)
 void lagfib_release_device();
+// draws of one qqcd_rnds_noalloc call at most (its static device workspace)
+constexpr long LAG_NOALLOC_MAX = 1L << 20;
+// the device address of that workspace on device dev (current), taken once
+void lagfib_noalloc_setup(int dev);
+// rnds(m, rn) into host rn through the workspace of device dev: no allocation
+void rnds_noalloc(LagState &st, long m, double *rn, int dev, hipStream_t s);
 
 }  // namespace qqcd

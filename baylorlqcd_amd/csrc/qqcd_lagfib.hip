@@ -347,81 +347,193 @@ static long chunk_len(long n) {
   return L;
 }
 
+__global__ void k_lag_reset(unsigned long long *first_zero, int *bad, int clear_bad) {
+  if (threadIdx.x == 0) {
+    first_zero[0] = ~0ull;
+    if (clear_bad) bad[0] = 0;
+  }
+}
+
+// Device buffers of one generation: the two base windows, the first-zero
+// word and the degenerate-state flag, and the jump table of chunk polynomials
+// (table(L, K) returns x^(cL) mod P for c < K).
+struct LagWork {
+  uint32_t *base = nullptr;  // 2 * 521 words
+  unsigned long long *dz = nullptr;
+  int *bad = nullptr;
+  const uint32_t *(*table)(void *ctx, long L, int K, hipStream_t s) = nullptr;
+  void *table_ctx = nullptr;
+};
+
+static void lag_generate_with(LagState &st, long m, uint32_t *d_out, hipStream_t s, const LagWork &w) {
+  uint32_t hb[LP];
+  for (int k = 0; k < LP; k++) hb[k] = st.w[(st.jr + k) % LP];
+  QQCD_HIP(hipMemcpyAsync(w.base, hb, sizeof(hb), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_lag_reset, dim3(1), dim3(64), 0, s, w.dz, w.bad, 1);
+  QQCD_HIP(hipGetLastError());
+  uint32_t *cur = w.base, *nxt = w.base + LP;
+  uint32_t *o = d_out;
+  long left = m;
+  while (left > 0) {
+    const long L = chunk_len(left);
+    const long piece = std::min(left, (long)KMAX * L);
+    const int K = (int)((piece + L - 1) / L);
+    const uint32_t *rows = w.table(w.table_ctx, L, K, s);
+    if (o != d_out) {
+      hipLaunchKernelGGL(k_lag_reset, dim3(1), dim3(64), 0, s, w.dz, w.bad, 0);
+      QQCD_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_lag_gen, dim3(K), dim3(64), 0, s, cur, rows, L, piece, K, o, w.dz);
+    QQCD_HIP(hipGetLastError());
+    unsigned long long z = 0;
+    QQCD_HIP(hipMemcpyAsync(&z, w.dz, sizeof(z), hipMemcpyDeviceToHost, s));
+    QQCD_HIP(hipStreamSynchronize(s));
+    long used = piece;
+    long fix = -1;
+    if (z < (unsigned long long)piece) {
+      used = (long)z + 1;
+      fix = (long)z;
+    }
+    if (used < left) {
+      hipLaunchKernelGGL(k_lag_rebase, dim3(1), dim3(256), 0, s, cur, o, used, fix, nxt, w.bad);
+      QQCD_HIP(hipGetLastError());
+      std::swap(cur, nxt);
+    } else if (fix >= 0) {  // the last draw was the zero: correct it in place
+      hipLaunchKernelGGL(k_lag_rebase, dim3(1), dim3(256), 0, s, cur, o, used, fix, nxt, w.bad);
+      QQCD_HIP(hipGetLastError());
+    }
+    // a rebase reads the old base, so keep it until the kernel has run
+    int bad = 0;
+    QQCD_HIP(hipMemcpyAsync(&bad, w.bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    QQCD_HIP(hipStreamSynchronize(s));
+    if (bad) throw std::runtime_error("lagfib: degenerate state (the zero skip of rnds would not end)");
+    o += used;
+    left -= used;
+  }
+  // the new state: w'[(jr + m + k) mod 521] = y_{m-521+k}
+  const long nlast = std::min(m, (long)LP);
+  uint32_t tail[LP];
+  if (nlast) {
+    QQCD_HIP(hipMemcpyAsync(tail, d_out + (m - nlast), sizeof(uint32_t) * nlast, hipMemcpyDeviceToHost, s));
+    QQCD_HIP(hipStreamSynchronize(s));
+  }
+  LagState ns;
+  const int njr = (int)((st.jr + m) % LP);
+  for (int k = 0; k < LP; k++) {
+    const long t = m - LP + k;
+    ns.w[(njr + k) % LP] = t >= 0 ? tail[t - (m - nlast)] : hb[LP + t];
+  }
+  ns.jr = njr;
+  ns.kr = (njr + LP - LQ) % LP;
+  st = ns;
+}
+
 void lagfib_generate(LagState &st, long m, uint32_t *d_out, hipStream_t s) {
   lagfib_check(st);
   if (m < 0) throw std::runtime_error("lagfib_generate: negative count");
-  uint32_t hb[LP];
-  for (int k = 0; k < LP; k++) hb[k] = st.w[(st.jr + k) % LP];
-  uint32_t *dbase = nullptr;
-  unsigned long long *dz = nullptr;
-  int *dbad = nullptr;
-  QQCD_HIP(hipMalloc(&dbase, sizeof(uint32_t) * 2 * LP));
-  QQCD_HIP(hipMalloc(&dz, sizeof(unsigned long long)));
-  QQCD_HIP(hipMalloc(&dbad, sizeof(int)));
+  LagWork w;
+  w.table = [](void *, long L, int K, hipStream_t st) { return table_for(L, K, st); };
+  QQCD_HIP(hipMalloc(&w.base, sizeof(uint32_t) * 2 * LP));
+  QQCD_HIP(hipMalloc(&w.dz, sizeof(unsigned long long)));
+  QQCD_HIP(hipMalloc(&w.bad, sizeof(int)));
   try {
-    QQCD_HIP(hipMemcpyAsync(dbase, hb, sizeof(hb), hipMemcpyHostToDevice, s));
-    QQCD_HIP(hipMemsetAsync(dbad, 0, sizeof(int), s));
-    uint32_t *cur = dbase, *nxt = dbase + LP;
-    uint32_t *o = d_out;
-    long left = m;
-    while (left > 0) {
-      const long L = chunk_len(left);
-      const long piece = std::min(left, (long)KMAX * L);
-      const int K = (int)((piece + L - 1) / L);
-      const uint32_t *rows = table_for(L, K, s);
-      QQCD_HIP(hipMemsetAsync(dz, 0xff, sizeof(unsigned long long), s));
-      hipLaunchKernelGGL(k_lag_gen, dim3(K), dim3(64), 0, s, cur, rows, L, piece, K, o, dz);
-      QQCD_HIP(hipGetLastError());
-      unsigned long long z = 0;
-      QQCD_HIP(hipMemcpyAsync(&z, dz, sizeof(z), hipMemcpyDeviceToHost, s));
-      QQCD_HIP(hipStreamSynchronize(s));
-      long used = piece;
-      long fix = -1;
-      if (z < (unsigned long long)piece) {
-        used = (long)z + 1;
-        fix = (long)z;
-      }
-      if (used < left) {
-        hipLaunchKernelGGL(k_lag_rebase, dim3(1), dim3(256), 0, s, cur, o, used, fix, nxt, dbad);
-        QQCD_HIP(hipGetLastError());
-        std::swap(cur, nxt);
-      } else if (fix >= 0) {  // the last draw was the zero: correct it in place
-        hipLaunchKernelGGL(k_lag_rebase, dim3(1), dim3(256), 0, s, cur, o, used, fix, nxt, dbad);
-        QQCD_HIP(hipGetLastError());
-      }
-      // a rebase reads the old base, so keep it until the kernel has run
-      int bad = 0;
-      QQCD_HIP(hipMemcpyAsync(&bad, dbad, sizeof(int), hipMemcpyDeviceToHost, s));
-      QQCD_HIP(hipStreamSynchronize(s));
-      if (bad) throw std::runtime_error("lagfib: degenerate state (the zero skip of rnds would not end)");
-      o += used;
-      left -= used;
-    }
-    // the new state: w'[(jr + m + k) mod 521] = y_{m-521+k}
-    const long nlast = std::min(m, (long)LP);
-    std::vector<uint32_t> tail(nlast);
-    if (nlast) {
-      QQCD_HIP(hipMemcpyAsync(tail.data(), d_out + (m - nlast), sizeof(uint32_t) * nlast, hipMemcpyDeviceToHost, s));
-      QQCD_HIP(hipStreamSynchronize(s));
-    }
-    LagState ns;
-    const int njr = (int)((st.jr + m) % LP);
-    for (int k = 0; k < LP; k++) {
-      const long t = m - LP + k;
-      ns.w[(njr + k) % LP] = t >= 0 ? tail[t - (m - nlast)] : hb[LP + t];
-    }
-    ns.jr = njr;
-    ns.kr = (njr + LP - LQ) % LP;
-    st = ns;
+    lag_generate_with(st, m, d_out, s, w);
   } catch (...) {
-    (void)hipFree(dbase);
-    (void)hipFree(dz);
-    (void)hipFree(dbad);
+    (void)hipFree(w.base);
+    (void)hipFree(w.dz);
+    (void)hipFree(w.bad);
     throw;
   }
-  QQCD_HIP(hipFree(dbase));
-  QQCD_HIP(hipFree(dz));
-  QQCD_HIP(hipFree(dbad));
+  QQCD_HIP(hipFree(w.base));
+  QQCD_HIP(hipFree(w.dz));
+  QQCD_HIP(hipFree(w.bad));
+}
+
+// ---- rnds without allocation -------------------------------------------------
+//
+// qqcd_rnds_noalloc works in one static device array per GPU instead of
+// memory it allocates: the raw draws, their doubles, the two base windows,
+// the first-zero word and flag, and the jump table (LAG_NOALLOC_MAX draws
+// need chunks of 8192 draws, at most 128 of them per launch).  Its address on
+// each device is taken once, at qqcd_init with that device current; host
+// code passes that address, never the array's name, to kernels and copies.
+// Every call on one GPU uses the same array, so calls that overlap on one GPU
+// race; the array keeps the last call's contents (the table is reused when
+// the next call needs no more of it).
+static constexpr long NA_Y = 0;                                   // LAG_NOALLOC_MAX raw draws
+static constexpr long NA_D = NA_Y + 4 * LAG_NOALLOC_MAX;           // LAG_NOALLOC_MAX doubles
+static constexpr long NA_BASE = NA_D + 8 * LAG_NOALLOC_MAX;        // 2 * 521 words
+static constexpr long NA_FLAGS = NA_BASE + 4352;                  // first zero (8 B), flag (4 B)
+static constexpr long NA_TAB = NA_FLAGS + 256;                    // NA_CHUNKS * 17 words
+static constexpr int NA_CHUNKS = (int)((LAG_NOALLOC_MAX + 8191) / 8192);
+static constexpr long NA_BYTES = NA_TAB + 4L * LW * NA_CHUNKS;
+static_assert(8 * LP <= 4352 && NA_BYTES % 256 == 0, "workspace layout");
+static_assert(NA_CHUNKS <= KMAX, "one launch covers the maximum");
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+struct NoAllocDevice {
+  unsigned char *p = nullptr;  // device address of scratch on this GPU
+  long tab_L = 0;              // the jump table scratch holds: chunk length, chunks
+  int tab_K = 0;
+};
+static constexpr int NA_MAXDEV = 64;
+static NoAllocDevice g_noalloc[NA_MAXDEV];
+
+void lagfib_noalloc_setup(int dev) {
+  if (dev < 0 || dev >= NA_MAXDEV) throw std::runtime_error("lagfib: device index beyond 63");
+  if (g_noalloc[dev].p) return;
+  void *ptr = nullptr;
+  QQCD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) throw std::runtime_error("lagfib: no device address for the static workspace");
+  g_noalloc[dev].p = (unsigned char *)ptr;
+}
+
+// the jump table in the workspace, rebuilt when the call needs a different
+// chunk length or more chunks than it holds
+static const uint32_t *noalloc_table(void *ctx, long L, int K, hipStream_t s) {
+  NoAllocDevice &d = *(NoAllocDevice *)ctx;
+  uint32_t *tab = (uint32_t *)(d.p + NA_TAB);
+  if (K > NA_CHUNKS) throw std::runtime_error("qqcd_rnds_noalloc: jump table beyond its workspace");
+  if (d.tab_L == L && d.tab_K >= K) return tab;
+  static thread_local uint32_t h[NA_CHUNKS * LW];
+  const Poly xL = xpow(L);
+  Poly p = xpow(0);
+  for (int c = 0; c < K; c++) {
+    std::memcpy(&h[(size_t)c * LW], p.data(), sizeof(uint32_t) * LW);
+    p = mulmod(p, xL);
+  }
+  d.tab_L = 0;  // invalid until the copy has landed
+  QQCD_HIP(hipMemcpyAsync(tab, h, sizeof(uint32_t) * LW * K, hipMemcpyHostToDevice, s));
+  QQCD_HIP(hipStreamSynchronize(s));
+  d.tab_L = L;
+  d.tab_K = K;
+  return tab;
+}
+
+void rnds_noalloc(LagState &st, long m, double *rn, int dev, hipStream_t s) {
+  lagfib_check(st);
+  if (m < 0 || m > LAG_NOALLOC_MAX)
+    throw std::runtime_error("qqcd_rnds_noalloc: m = " + std::to_string(m) + " outside 0 .. " +
+                             std::to_string(LAG_NOALLOC_MAX) + " (the size of its static workspace; qqcd_rnds has no limit)");
+  if (m > 0 && !rn) throw std::runtime_error("qqcd_rnds_noalloc: no output array");
+  if (dev < 0 || dev >= NA_MAXDEV || !g_noalloc[dev].p)
+    throw std::runtime_error("qqcd_rnds_noalloc: workspace not set up on this device (qqcd_init)");
+  NoAllocDevice &d = g_noalloc[dev];
+  LagWork w;
+  w.base = (uint32_t *)(d.p + NA_BASE);
+  w.dz = (unsigned long long *)(d.p + NA_FLAGS);
+  w.bad = (int *)(d.p + NA_FLAGS + 8);
+  w.table = noalloc_table;
+  w.table_ctx = &d;
+  uint32_t *y = (uint32_t *)(d.p + NA_Y);
+  double *dd = (double *)(d.p + NA_D);
+  lag_generate_with(st, m, y, s, w);
+  if (m > 0) {
+    lagfib_to_double(y, m, dd, s);
+    QQCD_HIP(hipMemcpyAsync(rn, dd, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    QQCD_HIP(hipStreamSynchronize(s));
+  }
 }
 
 double bench_lagfib(long m, int niter, long chunk, hipStream_t s) {

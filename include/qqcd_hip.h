@@ -347,6 +347,16 @@ int qqcd_lagfib_init(int myid, int iseed, int *iw, int *jr, int *kr);
  * including its skip of zero draws. */
 int qqcd_rnds(long m, double *rn, int *iw, int *jr, int *kr);
 
+/* No counterpart in the reference: qqcd_rnds for callers that cannot
+ * allocate device memory (harnesses that forbid hipMalloc, tight loops).
+ * Same arguments, outputs and state update as qqcd_rnds, bit for bit, but
+ * every device buffer it needs lies in one static array per GPU (12 MiB,
+ * held until the process ends) whose address qqcd_init takes; it calls no
+ * hipMalloc or hipFree.  m is limited to QQCD_RNDS_NOALLOC_MAX draws (larger
+ * m is an error).  Calls that overlap on one GPU share that array and race. */
+#define QQCD_RNDS_NOALLOC_MAX 1048576L
+int qqcd_rnds_noalloc(long m, double *rn, int *iw, int *jr, int *kr);
+
 /* z4source / z2source (qqcd-test/cfgsprops/quark/disconloops.f90:4359-4649)
  * from the reference's stream: be, bo (6, ntotal, 4, 2, 8), 384 nvhalf draws;
  * bit-identical to the reference's sources.  Either output may be NULL (its
