@@ -244,6 +244,7 @@ contains
          if (msg(k) == c_null_char) exit
          text(k:k) = msg(k)
       end do
+      if (len_trim(text) == 0) write(text, "(a,i0)") "qqcd_hip: call failed with status ", rc
       open(newunit=unit, file=where//".ERROR", action="write", status="replace", form="formatted")
       write(unit, fmt=*) trim(text)
       close(unit, status="keep")
@@ -391,7 +392,7 @@ contains
       real(kind=8), intent(in), target, contiguous :: evenIN(:,:,:,:,:), oddIN(:,:,:,:,:)
       real(kind=8), intent(inout), target, contiguous :: evenOUT(:,:,:,:,:), oddOUT(:,:,:,:,:)
       integer, intent(in) :: mu, ieo, ibleo
-      if (.not. started) call fail("DIRACOPS", -1_c_int)
+      if (.not. started) call ensure_started([1, 1, 1, 1])
       call fail("DIRACOPS", c_gammamult(c_loc(evenIN), c_loc(oddIN), c_loc(evenOUT), c_loc(oddOUT), mu, ieo, ibleo))
    end subroutine gammamult
 
@@ -450,7 +451,7 @@ contains
 
    subroutine gam5x(a)
       real(kind=8), intent(inout), target, contiguous :: a(:,:,:,:,:)
-      if (.not. started) call fail("BASICS", -1_c_int)
+      if (.not. started) call ensure_started([1, 1, 1, 1])
       call fail("BASICS", c_gam5x(c_loc(a)))
    end subroutine gam5x
 
@@ -460,7 +461,7 @@ contains
       real(kind=8), intent(in), target, contiguous :: a(:,:,:,:,:), b(:,:,:,:,:)
       real(kind=8), intent(out), target :: adb(2)
       integer, intent(in) :: MRT2
-      if (.not. started) call fail("INVERTERS", -1_c_int)
+      if (.not. started) call ensure_started([1, 1, 1, 1])
       call fail("INVERTERS", c_vecdot(c_loc(a), c_loc(b), c_loc(adb)))
    end subroutine vecdot
 
@@ -597,7 +598,7 @@ contains
       integer, intent(in) :: icsrc, idsrc, myid
       integer, intent(in) :: iblv(:,:), vecblinv(:,:)
       integer(c_int), target :: s(4)
-      if (.not. started) call fail("QUARK", -1_c_int)
+      if (.not. started) call ensure_started([1, 1, 1, 1])
       s = src(1:4)
       call fail("QUARK", c_pointsource(c_loc(be), c_loc(bo), c_loc(s), icsrc, idsrc))
    end subroutine pointsource
@@ -610,7 +611,7 @@ contains
    subroutine z4source(be,bo,nvhalf,myid)
       real(kind=8), intent(out), target, contiguous :: be(:,:,:,:,:), bo(:,:,:,:,:)
       integer, intent(in) :: nvhalf, myid
-      if (.not. started) call fail("DISCONLOOPS", -1_c_int)
+      if (.not. started) call ensure_started([1, 1, 1, 1])
       if (nvhalf /= nx*ny*nz*nt/32) call fail("DISCONLOOPS", -1_c_int)
       call file_dims("DISCONLOOPS", size(be, 2), spinor_shape(be) .and. spinor_shape(bo) .and. &
                      size(bo, 2) == size(be, 2))
@@ -622,7 +623,7 @@ contains
    subroutine z2source(be,bo,nvhalf)
       real(kind=8), intent(out), target, contiguous :: be(:,:,:,:,:), bo(:,:,:,:,:)
       integer, intent(in) :: nvhalf
-      if (.not. started) call fail("DISCONLOOPS", -1_c_int)
+      if (.not. started) call ensure_started([1, 1, 1, 1])
       if (nvhalf /= nx*ny*nz*nt/32) call fail("DISCONLOOPS", -1_c_int)
       call file_dims("DISCONLOOPS", size(be, 2), spinor_shape(be) .and. spinor_shape(bo) .and. &
                      size(bo, 2) == size(be, 2))

@@ -199,7 +199,8 @@ contains
       allocate(xs(6,ntotal,4,2,8,2), gdr(2000,1), p1e(6,ntotal,4,2,8), p1o(6,ntotal,4,2,8), &
                p2e(6,ntotal,4,2,8), p2o(6,ntotal,4,2,8), x1(6,ntotal,4,2,8))
       call pointsource(p1e, p1o, [1, 1, 1, 1], 1, 1, 0, iblv, vecblinv)
-      call pointsource(p2e, p2o, [3, 2, 4, 5], 3, 2, 0, iblv, vecblinv)
+      call pointsource(p2e, p2o, [3, 2, 4, 5], 2, 3, 0, iblv, vecblinv)
+      call writebin("proj_phi_e.bin", p2e, size(p2e))
       kappa = [0.150d0, 0.0d0]
       ! first right-hand side: psc with isignal = 1 (GMRES-DR(40,10)), the deflation space stays
       isignal = 1

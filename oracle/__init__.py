@@ -270,7 +270,7 @@ def trace16(n, eta, x):
     V = volume(n)
     t = np.arange(V) // (n[0] * n[1] * n[2])
     # per site and Gamma: sum_{a,b,c} conj(eta_ac) G_ab x_bc
-    per = np.einsum("sac,nab,sbc->sn", np.conj(eta), G, x)
+    per = np.einsum("sac,nab,sbc->sn", np.conj(eta), G, x, optimize=True)
     out = np.zeros((n[3], 16), dtype=np.complex128)
     np.add.at(out, t, per)
     return out
@@ -316,12 +316,12 @@ def loopops(n, U, bc, b, x, ksub=0):
     for mu in range(4):
         nb, f = forward(n, bc, mu)
         Um = U[:, mu]
-        udb = np.einsum("sji,saj->sai", np.conj(Um), b)  # U^dag b(x)
-        t1 = f * np.einsum("sac,ab,sbc->s", np.conj(x[nb]), one - g[mu], udb)
-        ub = np.einsum("sij,saj->sai", Um, b[nb]) * f[:, None, None]  # U b(x+mu)
-        t2 = -np.einsum("sac,ab,sbc->s", np.conj(x), one + g[mu], ub)
+        udb = np.einsum("sji,saj->sai", np.conj(Um), b, optimize=True)  # U^dag b(x)
+        t1 = f * np.einsum("sac,ab,sbc->s", np.conj(x[nb]), one - g[mu], udb, optimize=True)
+        ub = np.einsum("sij,saj->sai", Um, b[nb], optimize=True) * f[:, None, None]  # U b(x+mu)
+        t2 = -np.einsum("sac,ab,sbc->s", np.conj(x), one + g[mu], ub, optimize=True)
         J[:, mu] = np.conj(t1 + t2)
-        J[:, 5 + mu] = np.conj(np.einsum("sac,ab,sbc->s", np.conj(x), g[mu], b))
+        J[:, 5 + mu] = np.conj(np.einsum("sac,ab,sbc->s", np.conj(x), g[mu], b, optimize=True))
     J[:, 4] = np.conj(np.einsum("sac,sac->s", np.conj(x), b)) + 12.0 * ksub
     return J
 

@@ -161,6 +161,9 @@ def test_fortran_host_boundary(tmp_path):
     odd = O.parity_mask(N)
     nvh, ntot = O.ref_sizes(N)
     slen = 6 * ntot * 64
+    # the right-hand side is the reference's: the even half of its point source
+    phi = O.spinor_ref_to_canon(N, np.fromfile(tmp_path / "proj_phi_e.bin"), None)[~odd]
+    assert np.array_equal(phi, gb["proj_phi_e"])
     xs = np.fromfile(tmp_path / "proj_x.bin").reshape(2, slen)
     x1 = O.spinor_ref_to_canon(N, xs[0], None)[~odd]
     assert np.all(xs[1] == 0)
