@@ -537,11 +537,14 @@ def main():
                 "GB/s": round(2 * (1 << 30) / (cms[0] * 1e-3) / 1e9, 1)}
         print(f"[bench] copy {copy}", file=sys.stderr, flush=True)
 
-    prof = None
+    prof, prof_src = None, None
     ppath = os.path.join(ROOT, "profiles", "traffic_dslash_f64_18_32x32x32x64.json")
     if os.path.exists(ppath):
         with open(ppath) as f:
-            prof = json.load(f).get("bytes_per_launch")
+            pj = json.load(f)
+        prof = pj.get("bytes_per_launch")
+        prof_src = (f"rocprofv3 FETCH_SIZE/WRITE_SIZE of {pj.get('kernel')}, measured on commit "
+                    f"{pj.get('measured_on_commit')} (profiles/traffic_dslash_f64_18_32x32x32x64.md)")
 
     cpu = None
     if not args.no_cpu:
@@ -584,7 +587,7 @@ def main():
                    "lattice": list(N), "sites_per_launch": vh, "bytes_per_site": unique_bytes(0, 18),
                    "buffers": "4 input/output pairs in turn"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 3), "traffic": prof,
+                     "frac": round(gbs / HBM_PEAK_GBS, 3), "traffic": prof, "traffic_source": prof_src,
                      "bytes_model": "unique: input and output spinor and the eight links once per output site",
                      "neighbour_counted_GB/s": round(site_bytes(0, 18) * vh / (ms * 1e-3) / 1e9, 1),
                      "us_same_buffers": round(ms_hot * 1e3, 2)},
