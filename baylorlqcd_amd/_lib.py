@@ -49,6 +49,8 @@ SIGNATURES = {
     "qqcd_schur_canonical": [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "qqcd_set_xcd_remap": [ctypes.c_int],
     "qqcd_set_dslash_tune": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qqcd_set_multi_tune": [ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "qqcd_set_step_graphs": [ctypes.c_int],
     "qqcd_gauge_random": [ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp],
     "qqcd_gauge_get_canonical": [_dp],
     "qqcd_gauge_heatbath": [ctypes.c_double, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_int, _dp],

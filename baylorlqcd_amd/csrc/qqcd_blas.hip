@@ -77,6 +77,17 @@ template <typename T> struct UnitOf {
   static constexpr int U = std::is_same<T, f16>::value ? 4 : 2;
 };
 
+// FP16 vectors are walked site-major: a lane takes four consecutive sites
+// (q) and all 12 component runs of them, so each vector's per-site norms are
+// read once per pass (component-major units would read them once per run).
+__device__ __forceinline__ void ld_h4(const void *p, float4 m, long i, double2 (&e)[4]) {
+  const h8 v = ((const h8 *)p)[i];
+  e[0] = make_double2((double)(m.x * (float)v[0]), (double)(m.x * (float)v[1]));
+  e[1] = make_double2((double)(m.y * (float)v[2]), (double)(m.y * (float)v[3]));
+  e[2] = make_double2((double)(m.z * (float)v[4]), (double)(m.z * (float)v[5]));
+  e[3] = make_double2((double)(m.w * (float)v[6]), (double)(m.w * (float)v[7]));
+}
+
 // conj(a) b accumulated into (re, im)
 __device__ __forceinline__ void cdot_acc(double2 a, double2 b, double &re, double &im) {
   re += a.x * b.x + a.y * b.y;
@@ -127,15 +138,36 @@ __global__ void __launch_bounds__(RB) k_mdot(long nu, PtrPack v, const void *w, 
   double acc[2 * K];
 #pragma unroll
   for (int k = 0; k < 2 * K; k++) acc[k] = 0;
-  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
-    double2 b[U];
-    C2<TW>::template ld<U>(w, wn, stride, i, b);
+  if constexpr (std::is_same<TV, f16>::value) {
+    const long nq = nu / 12;  // site quads
+    for (long q = blockIdx.x * (long)RB + threadIdx.x; q < nq; q += (long)gridDim.x * RB) {
+      float4 m[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-      double2 a[U];
-      C2<TV>::template ld<U>(v.p[k], v.n[k], stride, i, a);
+      for (int k = 0; k < K; k++) m[k] = *(const float4 *)(v.n[k] + 4 * q);
+      for (int c = 0; c < 12; c++) {
+        const long i = c * nq + q;
+        double2 b[4];
+        C2<TW>::template ld<4>(w, wn, stride, i, b);
 #pragma unroll
-      for (int u = 0; u < U; u++) cdot_acc(a[u], b[u], acc[2 * k], acc[2 * k + 1]);
+        for (int k = 0; k < K; k++) {
+          double2 a[4];
+          ld_h4(v.p[k], m[k], i, a);
+#pragma unroll
+          for (int u = 0; u < 4; u++) cdot_acc(a[u], b[u], acc[2 * k], acc[2 * k + 1]);
+        }
+      }
+    }
+  } else {
+    for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
+      double2 b[U];
+      C2<TW>::template ld<U>(w, wn, stride, i, b);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        double2 a[U];
+        C2<TV>::template ld<U>(v.p[k], v.n[k], stride, i, a);
+#pragma unroll
+        for (int u = 0; u < U; u++) cdot_acc(a[u], b[u], acc[2 * k], acc[2 * k + 1]);
+      }
     }
   }
   block_sum<2 * K>(acc, part + (long)blockIdx.x * 2 * K);
@@ -164,20 +196,45 @@ __global__ void __launch_bounds__(RB) k_maxpy(long nu, PtrPack v, CoefPack h, vo
                                              double *part) {
   constexpr int U = UnitOf<TV>::U;
   double acc[1] = {0};
-  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
-    double2 b[U];
-    C2<TW>::template ld<U>(w, nullptr, stride, i, b);
+  if constexpr (std::is_same<TV, f16>::value) {
+    const long nq = nu / 12;  // site quads
+    for (long q = blockIdx.x * (long)RB + threadIdx.x; q < nq; q += (long)gridDim.x * RB) {
+      float4 m[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-      double2 a[U];
-      C2<TV>::template ld<U>(v.p[k], v.n[k], stride, i, a);
+      for (int k = 0; k < K; k++) m[k] = *(const float4 *)(v.n[k] + 4 * q);
+      for (int c = 0; c < 12; c++) {
+        const long i = c * nq + q;
+        double2 b[4];
+        C2<TW>::template ld<4>(w, nullptr, stride, i, b);
 #pragma unroll
-      for (int u = 0; u < U; u++) b[u] = csub_mul(b[u], h.re[k], h.im[k], a[u]);
+        for (int k = 0; k < K; k++) {
+          double2 a[4];
+          ld_h4(v.p[k], m[k], i, a);
+#pragma unroll
+          for (int u = 0; u < 4; u++) b[u] = csub_mul(b[u], h.re[k], h.im[k], a[u]);
+        }
+        C2<TW>::template st<4>(w, i, b);
+        if (NORM)
+#pragma unroll
+          for (int u = 0; u < 4; u++) acc[0] += b[u].x * b[u].x + b[u].y * b[u].y;
+      }
     }
-    C2<TW>::template st<U>(w, i, b);
-    if (NORM)
+  } else {
+    for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
+      double2 b[U];
+      C2<TW>::template ld<U>(w, nullptr, stride, i, b);
 #pragma unroll
-      for (int u = 0; u < U; u++) acc[0] += b[u].x * b[u].x + b[u].y * b[u].y;
+      for (int k = 0; k < K; k++) {
+        double2 a[U];
+        C2<TV>::template ld<U>(v.p[k], v.n[k], stride, i, a);
+#pragma unroll
+        for (int u = 0; u < U; u++) b[u] = csub_mul(b[u], h.re[k], h.im[k], a[u]);
+      }
+      C2<TW>::template st<U>(w, i, b);
+      if (NORM)
+#pragma unroll
+        for (int u = 0; u < U; u++) acc[0] += b[u].x * b[u].x + b[u].y * b[u].y;
+    }
   }
   if (NORM) block_sum<1>(acc, part + blockIdx.x);
 }
@@ -195,20 +252,45 @@ __global__ void __launch_bounds__(RB) k_maxpy_d(long nu, PtrPack v, const double
     hi[k] = coef[2 * k + 1];
   }
   double acc[1] = {0};
-  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
-    double2 b[U];
-    C2<TW>::template ld<U>(w, nullptr, stride, i, b);
+  if constexpr (std::is_same<TV, f16>::value) {
+    const long nq = nu / 12;  // site quads
+    for (long q = blockIdx.x * (long)RB + threadIdx.x; q < nq; q += (long)gridDim.x * RB) {
+      float4 m[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-      double2 a[U];
-      C2<TV>::template ld<U>(v.p[k], v.n[k], stride, i, a);
+      for (int k = 0; k < K; k++) m[k] = *(const float4 *)(v.n[k] + 4 * q);
+      for (int c = 0; c < 12; c++) {
+        const long i = c * nq + q;
+        double2 b[4];
+        C2<TW>::template ld<4>(w, nullptr, stride, i, b);
 #pragma unroll
-      for (int u = 0; u < U; u++) b[u] = csub_mul(b[u], hr[k], hi[k], a[u]);
+        for (int k = 0; k < K; k++) {
+          double2 a[4];
+          ld_h4(v.p[k], m[k], i, a);
+#pragma unroll
+          for (int u = 0; u < 4; u++) b[u] = csub_mul(b[u], hr[k], hi[k], a[u]);
+        }
+        C2<TW>::template st<4>(w, i, b);
+        if (NORM)
+#pragma unroll
+          for (int u = 0; u < 4; u++) acc[0] += b[u].x * b[u].x + b[u].y * b[u].y;
+      }
     }
-    C2<TW>::template st<U>(w, i, b);
-    if (NORM)
+  } else {
+    for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
+      double2 b[U];
+      C2<TW>::template ld<U>(w, nullptr, stride, i, b);
 #pragma unroll
-      for (int u = 0; u < U; u++) acc[0] += b[u].x * b[u].x + b[u].y * b[u].y;
+      for (int k = 0; k < K; k++) {
+        double2 a[U];
+        C2<TV>::template ld<U>(v.p[k], v.n[k], stride, i, a);
+#pragma unroll
+        for (int u = 0; u < U; u++) b[u] = csub_mul(b[u], hr[k], hi[k], a[u]);
+      }
+      C2<TW>::template st<U>(w, i, b);
+      if (NORM)
+#pragma unroll
+        for (int u = 0; u < U; u++) acc[0] += b[u].x * b[u].x + b[u].y * b[u].y;
+    }
   }
   if (NORM) block_sum<1>(acc, part + blockIdx.x);
 }

@@ -292,6 +292,12 @@ extern "C" int qqcd_last_solve_stats(double *rel_res, int *iters, int *matvecs, 
   API_END
 }
 
+extern "C" int qqcd_set_step_graphs(int on) {
+  API_BEGIN
+  set_step_graphs(on);
+  API_END
+}
+
 extern "C" int qqcd_deflation_clear(void) {
   API_BEGIN
   if (g_defl) {

@@ -805,6 +805,47 @@ __global__ void __launch_bounds__(256) k_dslash_multi(const DslashArgs a, const 
   }
 }
 
+// Pipelined form of k_dslash_multi.  The tiles (RHS block, 256/nb sites in
+// the z-blocked t-sweep order of site_of) are split into eight contiguous
+// ranges, one per XCD for the whole launch, which its workgroups walk in
+// order: with the RHS block outermost (order 0) and nb * 8 right-hand sides,
+// every XCD sweeps all sites of one block, so the t and z neighbours of its
+// wavefronts were read a few slabs earlier into its own L2 and the links,
+// read by all XCDs at about the same pace, are shared in the last-level
+// cache.  Persistent like k_dslash_multi: the active count is read at launch.
+template <typename S, typename L, int RECON, int SIG, bool NT, int D>
+__global__ void __launch_bounds__(256) k_dslash_multi_pipe(const DslashArgs a, const MultiArgs m) {
+  typedef typename V2<S>::T SE;
+  const int nact = *m.nact;
+  const int nb = 1 << m.lognb;
+  const int nblk = (nact + nb - 1) >> m.lognb;
+  const long ntiles = (long)m.site_tiles * nblk;
+  const int g8 = gridDim.x >> 3;  // workgroups per XCD (the grid is a multiple of 8)
+  const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
+  const long per = (ntiles + 7) >> 3;
+  const long t0 = xcd * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+  const int spt = 256 >> m.lognb;  // sites per tile
+  for (long t = t0 + w; t < t1; t += g8) {
+    int blk, st;
+    if (m.order == 0) {
+      blk = (int)(t / m.site_tiles);
+      st = (int)(t - (long)blk * m.site_tiles);
+    } else {
+      st = (int)(t / nblk);
+      blk = (int)(t - (long)st * nblk);
+    }
+    const int lidx = st * spt + ((int)threadIdx.x >> m.lognb);
+    const int rr = (int)threadIdx.x & (nb - 1);
+    if (lidx >= a.Vh || (blk << m.lognb) + rr >= nact) continue;
+    const int i = site_of(a, lidx);
+    const long eo = (long)blk * m.blk_elems, no = (long)blk * m.blk_norms;
+    const HopIn h{(const SE *)a.in + eo, a.in_norm ? a.in_norm + no : nullptr, a.links, a.sstride, a.lstride};
+    dslash_site_pipe<S, L, RECON, SIG, NT, true, D>(a, h, (SE *)a.out + eo, a.out_norm ? a.out_norm + no : nullptr,
+                                                    a.x ? (const SE *)a.x + eo : nullptr,
+                                                    a.x_norm ? a.x_norm + no : nullptr, i, rr, m.lognb);
+  }
+}
+
 HopCoef make_hopcoef(const Geom &g, int idag) {
   HopCoef h{};
   const double s = idag ? -1.0 : 1.0;
@@ -894,11 +935,42 @@ static void launch_multi_sig(const DslashArgs &a, const MultiArgs &m, int grid, 
   QQCD_HIP(hipGetLastError());
 }
 
+template <typename S, typename L, int RECON, int SIG, bool NT>
+static void launch_multi_pipe(const DslashArgs &a, const MultiArgs &m, int grid, hipStream_t s) {
+  static int g0 = 0;
+  if (!g0) g0 = persistent_grid(k_dslash_multi_pipe<S, L, RECON, SIG, NT, 2>);
+  const int g = grid > 0 ? (grid + 7) / 8 * 8 : g0;
+  hipLaunchKernelGGL((k_dslash_multi_pipe<S, L, RECON, SIG, NT, 2>), dim3(g), dim3(256), 0, s, a, m);
+  QQCD_HIP(hipGetLastError());
+}
+
 template <typename S, typename L, int RECON>
-static void launch_multi(const DslashArgs &a, const MultiArgs &m, int sig, int grid, hipStream_t s) {
-  if (sig > 0) launch_multi_sig<S, L, RECON, 1>(a, m, grid, s);
-  else if (sig < 0) launch_multi_sig<S, L, RECON, -1>(a, m, grid, s);
-  else launch_multi_sig<S, L, RECON, 0>(a, m, grid, s);
+static void launch_multi(const DslashArgs &a0, const MultiArgs &m, int sig, const MultiTune &t, hipStream_t s) {
+  // the pipelined kernel: projector form, buffer descriptors of < 4 GiB (one
+  // RHS block of a field, one (parity, mu) block of links)
+  const bool fits = 12.0 * a0.sstride * sizeof(typename V2<S>::T) < 4294967295.0 &&
+                    (RECON == 18 ? 9.0 : 6.0) * a0.lstride * sizeof(typename V2<L>::T) < 4294967295.0;
+  if (t.kernel == 1 && sig != 0 && fits) {
+    DslashArgs a = a0;
+    a.plane = a.n[0] * a.n[1] / 2;
+    int bz = t.bz;
+    if (bz <= 0 || a.n[2] % bz)  // the largest of 8, 4, 2 that divides nz
+      for (bz = 8; bz > 1 && a.n[2] % bz; bz /= 2) {
+      }
+    a.order = bz > 1 ? 1 : 0;
+    a.bz = bz > 1 ? bz : 1;
+    if (sig > 0) {
+      if (t.nt) launch_multi_pipe<S, L, RECON, 1, true>(a, m, t.grid, s);
+      else launch_multi_pipe<S, L, RECON, 1, false>(a, m, t.grid, s);
+    } else {
+      if (t.nt) launch_multi_pipe<S, L, RECON, -1, true>(a, m, t.grid, s);
+      else launch_multi_pipe<S, L, RECON, -1, false>(a, m, t.grid, s);
+    }
+    return;
+  }
+  if (sig > 0) launch_multi_sig<S, L, RECON, 1>(a0, m, t.grid, s);
+  else if (sig < 0) launch_multi_sig<S, L, RECON, -1>(a0, m, t.grid, s);
+  else launch_multi_sig<S, L, RECON, 0>(a0, m, t.grid, s);
 }
 
 // Kernel arguments shared by the single- and multi-RHS launches; returns
@@ -998,8 +1070,21 @@ size_t MultiSpinor::bytes() const {
   return (size_t)elems() * e + (prec == PREC_HALF ? (size_t)stride * nb * nblk() * sizeof(float) : 0);
 }
 
+static int g_multi_kernel = -1, g_multi_bz = -1, g_multi_nt = -1;
+void set_multi_override(int kernel, int bz, int nt) {
+  if (kernel < -1 || kernel > 1 || bz < -1 || nt < -1 || nt > 1)
+    throw std::runtime_error("multi-RHS tuning: kernel -1/0/1, bz >= -1, nt -1/0/1 (-1: the caller's choice)");
+  g_multi_kernel = kernel;
+  g_multi_bz = bz;
+  g_multi_nt = nt;
+}
+
 void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpinor &out, int par_out, int idag,
-                  const Epilogue &ep, const MultiSpinor *x, const int *d_nact, const MultiTune &t, hipStream_t s) {
+                  const Epilogue &ep, const MultiSpinor *x, const int *d_nact, const MultiTune &t0, hipStream_t s) {
+  MultiTune t = t0;
+  if (g_multi_kernel >= 0) t.kernel = g_multi_kernel;
+  if (g_multi_bz >= 0) t.bz = g_multi_bz;
+  if (g_multi_nt >= 0) t.nt = g_multi_nt;
   if (in.prec != out.prec || (x && x->prec != out.prec))
     throw std::runtime_error("dslash_multi: spinor precisions differ");
   if (in.stride != g.stride || out.stride != g.stride || U.stride != g.stride)
@@ -1031,7 +1116,7 @@ void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpi
   dispatch_prec(in.prec, U, "dslash_multi", [&](auto *sp, auto *lp, auto rc) {
     typedef typename std::remove_pointer<decltype(sp)>::type S;
     typedef typename std::remove_pointer<decltype(lp)>::type L;
-    if constexpr (!std::is_same<S, f16>::value) launch_multi<S, L, decltype(rc)::value>(a, m, proj, t.grid, s);
+    if constexpr (!std::is_same<S, f16>::value) launch_multi<S, L, decltype(rc)::value>(a, m, proj, t, s);
   });
 }
 

@@ -91,6 +91,13 @@ struct MultiSpinor {
 struct MultiTune {
   int order = 1;  // 0: RHS block outermost; 1: innermost
   int grid = 0;   // persistent workgroups (0: occupancy x CUs)
+  // kernel 0: hops one after the other, tiles dealt to the XCDs per sweep of
+  // the grid; 1: pipelined hops (depth 2), each XCD owning one contiguous
+  // range of tiles for the whole launch and walking its sites in blocks of
+  // bz z-planes swept in t (bz 0: the largest of 8, 4, 2 dividing nz)
+  int kernel = 1;
+  int bz = 0;
+  int nt = 1;  // kernel 1: links and output streamed past the L2
 };
 
 // Launch shape of the single-right-hand-side hopping kernel (projector form):
@@ -107,6 +114,8 @@ struct DslashTune {
   int nt = 1;
 };
 void set_dslash_tune(const DslashTune &t);
+// overrides of the multi-RHS launch shape for every call (-1: the caller's)
+void set_multi_override(int kernel, int bz, int nt);
 DslashTune get_dslash_tune();
 
 // Coefficients of the local spin matrix applied in a Dslash epilogue:

@@ -160,6 +160,72 @@ static void arnoldi_step_dev(Context &c, LinOp &A, std::vector<Spinor> &V, int j
   else c.blas->rscale_dev(w, dh.n2(j));
 }
 
+// Groups of synchronisation-free steps replayed from HIP graphs: a group
+// [j0, j0 + n) that already ran on the same vectors within this solve is
+// launched as one graph, which removes the launch gaps between its ~13
+// kernels per step.  Only for operators whose buffers stay put between
+// applications (every LinOp here except a preconditioner's).
+static int g_step_graphs = 1;
+void set_step_graphs(int on) { g_step_graphs = on ? 1 : 0; }
+
+struct StepGraphs {
+  struct Entry {
+    std::vector<const void *> key;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+  };
+  std::vector<Entry> list;
+  int replays = 0;
+  StepGraphs() = default;
+  StepGraphs(const StepGraphs &) = delete;
+  StepGraphs &operator=(const StepGraphs &) = delete;
+  ~StepGraphs() {
+    for (auto &x : list) {
+      if (x.e) (void)hipGraphExecDestroy(x.e);
+      if (x.g) (void)hipGraphDestroy(x.g);
+    }
+  }
+};
+
+static void arnoldi_steps_dev(Context &c, LinOp &A, std::vector<Spinor> &V, int j0, int n, DevHess &dh,
+                              Spinor *work, StepGraphs *sg) {
+  if (!sg) {
+    for (int j = j0; j < j0 + n; j++) arnoldi_step_dev(c, A, V, j, dh, nullptr, nullptr, work);
+    return;
+  }
+  std::vector<const void *> key{(const void *)(intptr_t)j0, (const void *)(intptr_t)n, work ? work->data : nullptr,
+                                dh.d};
+  for (int q = 0; q <= j0 + n; q++) {
+    key.push_back(V[q].data);
+    key.push_back(V[q].norm);
+  }
+  for (auto &x : sg->list)
+    if (x.key == key) {
+      QQCD_HIP(hipGraphLaunch(x.e, c.stream));
+      sg->replays++;
+      return;
+    }
+  // first time: run the steps, then record them for the next time
+  for (int j = j0; j < j0 + n; j++) arnoldi_step_dev(c, A, V, j, dh, nullptr, nullptr, work);
+  QQCD_HIP(hipStreamSynchronize(c.stream));
+  StepGraphs::Entry e;
+  e.key = key;
+  QQCD_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int j = j0; j < j0 + n; j++) arnoldi_step_dev(c, A, V, j, dh, nullptr, nullptr, work);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  QQCD_HIP(hipStreamEndCapture(c.stream, &e.g));
+  QQCD_HIP(hipGraphInstantiate(&e.e, e.g, nullptr, nullptr, 0));
+  sg->list.push_back(std::move(e));
+  e.g = nullptr;
+  e.e = nullptr;
+}
+
 // dst = s * src across precisions (an FP16 destination through the conversion)
 static void scaled_copy(Context &c, const Spinor &src, Spinor &dst, double s) {
   if (dst.prec == PREC_HALF || src.prec == PREC_HALF) {
@@ -274,7 +340,8 @@ static void store_deflation(Context &c, Deflation *defl, const std::vector<Spino
 }
 
 SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int k, double tol, int itermin,
-                   int maxiter, Deflation *defl, int reorth, LinOp *M, const CycleHook *hook) {
+                   int maxiter, Deflation *defl, int reorth, LinOp *M, const CycleHook *hook,
+                   const StepHook *step_hook) {
   if (m < 2 || k < 0 || k >= m) throw std::runtime_error("gmresdr: need 0 <= k < m and m >= 2");
   if (M && defl) throw std::runtime_error("gmresdr: no deflation space is kept with a preconditioner");
   SolveStats st;
@@ -292,6 +359,8 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
   if (vp != p) work.reset(new SpinorBuf(c.g, p));
   std::unique_ptr<DevHess> dh;
   if (!reorth) dh.reset(new DevHess(m));
+  std::unique_ptr<StepGraphs> sg;
+  if (dh && !M && g_step_graphs) sg.reset(new StepGraphs);
   std::vector<zd> Zmat;
   SpinorBuf r(c.g, p);
   double rn = residual(c, A, b, x, r.s, st);
@@ -314,11 +383,19 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
     int jchk = j;
     while (j < m) {
       if (dh) {
-        arnoldi_step_dev(c, A, V, j, *dh, M, M ? &Z : nullptr, work ? &work->s : nullptr);
-        st.matvecs++;
-        st.iters++;
-        j++;
-        if (j < m && st.iters < maxiter && j - jchk < kCheck) continue;
+        if (sg) {
+          const int n = std::min(std::min(kCheck, m - j), std::max(1, maxiter - st.iters));
+          arnoldi_steps_dev(c, A, V, j, n, *dh, work ? &work->s : nullptr, sg.get());
+          st.matvecs += n;
+          st.iters += n;
+          j += n;
+        } else {
+          arnoldi_step_dev(c, A, V, j, *dh, M, M ? &Z : nullptr, work ? &work->s : nullptr);
+          st.matvecs++;
+          st.iters++;
+          j++;
+          if (j < m && st.iters < maxiter && j - jchk < kCheck) continue;
+        }
         const int jb = dh->fetch(c.stream, jchk, j, H, ldh);
         jchk = j;
         if (jb >= 0) {
@@ -335,6 +412,7 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
       for (int i = 0; i <= j + 1; i++) H[i + (size_t)ldh * j] = col[i];
       j++;
       const double est = dense::lsq(j + 1, j, H.data(), ldh, cv.data(), y.data());
+      if (step_hook && *step_hook && !M) (*step_hook)(x, V, j, y.data());
       if ((est < tol * r0 && st.iters >= itermin) || hn == 0.0 || st.iters >= maxiter) break;
     }
     // solution update (x += V y, or Z y when preconditioned) and true residual
@@ -406,6 +484,8 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
   if (vp != p) work.reset(new SpinorBuf(c.g, p));
   std::unique_ptr<DevHess> dh;
   if (!reorth) dh.reset(new DevHess(m));
+  std::unique_ptr<StepGraphs> sg;
+  if (dh && g_step_graphs) sg.reset(new StepGraphs);
   SpinorBuf r(c.g, p);
   double rn = residual(c, A, b, x, r.s, st);
   const double r0 = rn;
@@ -440,11 +520,19 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
     int j = 0, jchk = 0;
     while (j < m) {
       if (dh) {
-        arnoldi_step_dev(c, A, V, j, *dh, nullptr, nullptr, work ? &work->s : nullptr);
-        st.matvecs++;
-        st.iters++;
-        j++;
-        if (j < m && st.iters < maxiter && j - jchk < kCheck) continue;
+        if (sg) {
+          const int n = std::min(std::min(kCheck, m - j), std::max(1, maxiter - st.iters));
+          arnoldi_steps_dev(c, A, V, j, n, *dh, work ? &work->s : nullptr, sg.get());
+          st.matvecs += n;
+          st.iters += n;
+          j += n;
+        } else {
+          arnoldi_step_dev(c, A, V, j, *dh, nullptr, nullptr, work ? &work->s : nullptr);
+          st.matvecs++;
+          st.iters++;
+          j++;
+          if (j < m && st.iters < maxiter && j - jchk < kCheck) continue;
+        }
         const int jb = dh->fetch(c.stream, jchk, j, H, ldh);
         const int jlast = jb >= 0 ? jb + 1 : j;
         if (hist)  // one entry per Arnoldi step of this fetch

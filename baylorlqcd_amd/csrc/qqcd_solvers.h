@@ -74,9 +74,15 @@ typedef std::function<void(const Spinor &x, const Spinor &r, const SolveStats &s
 // M: right preconditioner, applied flexibly (FGMRES-DR: z_j = M v_j is kept,
 // A Z_m = V_{m+1} Hbar_m, x += Z y, and the deflated restart combines Z with
 // the same harmonic Ritz coefficients as V); no deflation space is kept then.
+// Called after every Arnoldi step of the synchronising path (reorth != 0,
+// no preconditioner) with the iterate x of the cycle start, the basis and
+// the step's least-squares solution y[0..j): the step's iterate is
+// x + V[0..j) y (the reference forms it at every step, inverters.f90:3242-3255).
+typedef std::function<void(const Spinor &x, const std::vector<Spinor> &V, int j, const dense::zd *y)> StepHook;
+
 SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int k, double tol, int itermin,
                    int maxiter, Deflation *defl, int reorth = 1, LinOp *M = nullptr,
-                   const CycleHook *hook = nullptr);
+                   const CycleHook *hook = nullptr, const StepHook *step_hook = nullptr);
 
 // Residual history of a solve: (number of operator applications so far, the
 // least-squares residual norm ||c - Hbar y|| after that Arnoldi step).
@@ -113,6 +119,10 @@ namespace qqcd {
 // not halve the true residual raises the operator precision (FP16 -> FP32
 // -> plain FP64 GMRES-DR), so the FP64 solver is the fallback and the FP64
 // true residual the judge of every result.
+// replay repeated groups of synchronisation-free Arnoldi steps from HIP
+// graphs (default on)
+void set_step_graphs(int on);
+
 struct MixedParams {
   int m = 40, k = 16;
   Prec basis = PREC_SINGLE;

@@ -574,6 +574,18 @@ int qqcd_bench_copy(double bytes, int iters, double *ms);
  * counterpart. */
 int qqcd_bench_blas(int prec, int k, int op, int grid, int niter, double *ms);
 
+/* Override the launch shape of every multi-right-hand-side hop (-1 keeps
+ * the caller's or the autotuner's choice): kernel 0 (hops one after the
+ * other) or 1 (pipelined hops, one contiguous tile range per XCD, sites in
+ * blocks of bz z-planes swept in t), bz, nt 0/1 (links and output streamed
+ * past the L2).  Tuning aid; no reference counterpart. */
+int qqcd_set_multi_tune(int kernel, int bz, int nt);
+
+/* Toggle the replay of repeated groups of Krylov steps from HIP graphs in
+ * the single-right-hand-side solvers (default on).  Tuning aid; no
+ * reference counterpart. */
+int qqcd_set_step_graphs(int on);
+
 /* Toggle the XCD-aware workgroup order of the Dslash (default on). */
 int qqcd_set_xcd_remap(int on);
 
