@@ -82,7 +82,7 @@ static void nn_init_identity(const std::vector<int> &ch, double scale, double no
 }
 
 struct Pair {
-  Spinor r, e;  // FP32, even checkerboard, ||r|| = 1
+  Spinor r, e;  // even checkerboard, FP16 with per-site norms, ||r|| = 1
   double e2;    // ||e||^2
   int conf;
 };
@@ -90,6 +90,7 @@ struct Pair {
 struct TrainSet {
   std::vector<NNLinks> U;  // the BF16 links of every configuration
   std::vector<Pair> pairs;
+  std::unique_ptr<SpinorBuf> r32, e32;  // a pair in FP32, as the network reads it
   ~TrainSet() {
     for (auto &u : U) nn_free_links(u);
     for (auto &p : pairs) {
@@ -99,8 +100,12 @@ struct TrainSet {
   }
 };
 
-// pairs (r_i, e_i) from the cycle ends of a plain FP64 GMRES-DR solve:
-// e_i = x* - x_i, r_i = b - A x_i, both divided by ||r_i||
+// Pairs (r_j, e_j) from every Arnoldi step j of plain FP64 GMRES-DR solves
+// of Gaussian sources: the step's iterate x_j = x + V y_j (the reference
+// forms it at every step, inverters.f90:3242-3255), e_j = x* - x_j and
+// r_j = b - A x_j, both divided by ||r_j||, kept in FP16 with per-site norms
+// (6.8 MB per pair at 16^3x32); the starting guess x_0 = 0 gives the first.
+// Sources are solved until max_pairs pairs are held or nsrc sources are done.
 static void harvest(Context &c, const double kappa[2], int nsrc, int m, int k, double tol, uint64_t seed, int conf,
                     int max_pairs, TrainSet &T) {
   SchurOp A(c, PREC_DOUBLE, 18, kappa);
@@ -108,61 +113,73 @@ static void harvest(Context &c, const double kappa[2], int nsrc, int m, int k, d
   for (int src = 0; src < nsrc && (int)T.pairs.size() < max_pairs; src++) {
     SpinorBuf b(c.g, PREC_DOUBLE), x(c.g, PREC_DOUBLE);
     random_spinor(c.g, b.s, seed * 1000003ull + src, c.stream);
-    std::vector<Spinor> xs, rs;
-    CycleHook hook = [&](const Spinor &xi, const Spinor &ri, const SolveStats &) {
-      Spinor a = alloc_spinor(c.g, PREC_DOUBLE), r = alloc_spinor(c.g, PREC_DOUBLE);
-      Vec av = as_vec(a), rv = as_vec(r);
-      c.blas->copy(as_vec(xi), av);
-      c.blas->copy(as_vec(ri), rv);
-      xs.push_back(a);
-      rs.push_back(r);
+    const int room = max_pairs - (int)T.pairs.size() - 1;  // the x_0 = 0 pair comes first
+    std::vector<Spinor> xs;
+    StepHook hook = [&](const Spinor &x0, const std::vector<Spinor> &V, int j, const dense::zd *y) {
+      if ((int)xs.size() >= room) return;
+      xs.push_back(alloc_spinor(c.g, PREC_DOUBLE));
+      Vec av = as_vec(xs.back());
+      std::vector<Vec> Vb(j);
+      for (int q = 0; q < j; q++) Vb[q] = as_vec(V[q]);
+      c.blas->combine(j, Vb.data(), 1, y, j, &av, false);
+      c.blas->axpby(1.0, as_vec(x0), 1.0, av, false);
     };
     try {
-      gmresdr(c, op, b.s, x.s, m, k, tol, 0, 5000 * m, nullptr, 1, nullptr, &hook);
-      // the initial guess x0 = 0 is an iterate too: e_0 = x*, r_0 = b
-      std::vector<const Spinor *> X, R;
-      X.push_back(nullptr);
-      R.push_back(&b.s);
-      for (size_t i = 0; i + 1 < xs.size(); i++) {
-        X.push_back(&xs[i]);
-        R.push_back(&rs[i]);
-      }
-      SpinorBuf e64(c.g, PREC_DOUBLE);
-      for (size_t i = 0; i < X.size() && (int)T.pairs.size() < max_pairs; i++) {
-        Vec ev = as_vec(e64.s);
+      gmresdr(c, op, b.s, x.s, m, k, tol, 0, 5000 * m, nullptr, 1, nullptr, nullptr, &hook);
+      SpinorBuf e64(c.g, PREC_DOUBLE), r64(c.g, PREC_DOUBLE);
+      // iterates whose error is at the rounding level of x* (the last step's
+      // iterate is x* itself) carry no target
+      const double xn2 = c.blas->norm2(as_vec(x.s));
+      for (int i = -1; i < (int)xs.size() && (int)T.pairs.size() < max_pairs; i++) {
+        Vec ev = as_vec(e64.s), rv = as_vec(r64.s);
         c.blas->copy(as_vec(x.s), ev);
-        if (X[i]) c.blas->axpby(-1.0, as_vec(*X[i]), 1.0, ev, false);
-        const double rn = std::sqrt(c.blas->norm2(as_vec(*R[i])));
-        if (!(rn > 0)) continue;
+        if (i < 0) {
+          c.blas->copy(as_vec(b.s), rv);
+        } else {
+          c.blas->axpby(-1.0, as_vec(xs[i]), 1.0, ev, false);
+          op.apply(xs[i], r64.s);
+          c.blas->axpby(1.0, as_vec(b.s), -1.0, rv, false);
+        }
+        const double rn = std::sqrt(c.blas->norm2(rv));
+        const double en2 = c.blas->norm2(ev);
+        if (!(rn > 0) || !(en2 > 1e-24 * xn2)) continue;
         Pair p;
         p.conf = conf;
-        p.r = alloc_spinor(c.g, PREC_SINGLE);
-        p.e = alloc_spinor(c.g, PREC_SINGLE);
-        Vec prv = as_vec(p.r), pev = as_vec(p.e);
-        c.blas->axpby(1.0 / rn, as_vec(*R[i]), 0.0, prv, false);
-        p.e2 = c.blas->axpby(1.0 / rn, ev, 0.0, pev, true);
+        p.e2 = en2 / (rn * rn);
+        p.r = alloc_spinor(c.g, PREC_HALF);
+        p.e = alloc_spinor(c.g, PREC_HALF);
         T.pairs.push_back(p);
+        spinor_copy_convert(c.g, r64.s, T.pairs.back().r, c.stream, 1.0 / rn);
+        spinor_copy_convert(c.g, e64.s, T.pairs.back().e, c.stream, 1.0 / rn);
       }
     } catch (...) {
       for (auto &s : xs) free_spinor(s);
-      for (auto &s : rs) free_spinor(s);
       throw;
     }
     for (auto &s : xs) free_spinor(s);
-    for (auto &s : rs) free_spinor(s);
   }
+}
+
+// the pair's r and e in FP32 scratch spinors
+static void pair_fp32(Context &c, TrainSet &T, const Pair &p) {
+  if (!T.r32) T.r32.reset(new SpinorBuf(c.g, PREC_SINGLE));
+  if (!T.e32) T.e32.reset(new SpinorBuf(c.g, PREC_SINGLE));
+  spinor_copy_convert(c.g, p.r, T.r32->s, c.stream);
+  spinor_copy_convert(c.g, p.e, T.e32->s, c.stream);
 }
 
 // One pass over a pair: forward, loss ||N(r) - e||^2 / ||e||^2, backward
 // into dW (accumulated).  Returns the loss.
-static double pair_grad(Context &c, NNModel &M, const NNLinks &U, const Pair &p, double *d_scr,
+static double pair_grad(Context &c, NNModel &M, TrainSet &T, const Pair &p, double *d_scr,
                         std::vector<std::vector<std::complex<double>>> &dW) {
   const int L = (int)M.layers.size();
-  nn_from_spinor(c.g, p.r, M.acts[0], 1.f, c.stream);
+  const NNLinks &U = T.U[p.conf];
+  pair_fp32(c, T, p);
+  nn_from_spinor(c.g, T.r32->s, M.acts[0], 1.f, c.stream);
   for (int l = 0; l < L; l++) nn_layer(c.g, U, M.dev[l], 0, M.acts[l], M.acts[l + 1], c.stream);
   const float f = (float)(1.0 / p.e2);
   // g = (N(r) - e) / ||e||^2 = dL/dconj(N(r)); loss = ||g||^2 ||e||^2
-  const double g2 = nn_diff(c.g, M.acts[L], p.e, f, f, M.grads[L], d_scr, c.stream);
+  const double g2 = nn_diff(c.g, M.acts[L], T.e32->s, f, f, M.grads[L], d_scr, c.stream);
   std::vector<std::complex<double>> tmp;
   for (int l = L - 1; l >= 0; l--) {
     tmp.resize(dW[l].size());
@@ -182,12 +199,13 @@ LinOp nn_precond(Context &c) {
 }
 
 // loss only (evaluation)
-static double pair_loss(Context &c, NNModel &M, const NNLinks &U, const Pair &p, double *d_scr) {
+static double pair_loss(Context &c, NNModel &M, TrainSet &T, const Pair &p, double *d_scr) {
   const int L = (int)M.layers.size();
-  nn_from_spinor(c.g, p.r, M.acts[0], 1.f, c.stream);
-  for (int l = 0; l < L; l++) nn_layer(c.g, U, M.dev[l], 0, M.acts[l], M.acts[l + 1], c.stream);
+  pair_fp32(c, T, p);
+  nn_from_spinor(c.g, T.r32->s, M.acts[0], 1.f, c.stream);
+  for (int l = 0; l < L; l++) nn_layer(c.g, T.U[p.conf], M.dev[l], 0, M.acts[l], M.acts[l + 1], c.stream);
   const float f = (float)(1.0 / std::sqrt(p.e2));
-  return nn_diff(c.g, M.acts[L], p.e, f, f, M.grads[L], d_scr, c.stream);
+  return nn_diff(c.g, M.acts[L], T.e32->s, f, f, M.grads[L], d_scr, c.stream);
 }
 
 }  // namespace qqcd
@@ -320,8 +338,10 @@ extern "C" int qqcd_nn_wgrad_canonical(int layer, const double *psi, const doubl
 // of nsrc Gaussian sources each (to harvest_tol), then `steps` Adam steps
 // (learning rate lr) on minibatches of `batch` pairs.  The network is
 // (re)initialised near kappa^2 * 1 with the given channels unless keep = 1.
-// loss[steps]: mean minibatch loss per step; stats = {pairs, harvest seconds,
-// training seconds, mean loss over all pairs before, after}.
+// Pairs come from every Arnoldi step (max_pairs in all, spread evenly over the
+// configurations).  loss[steps]: mean minibatch loss per step; stats = {pairs,
+// harvest seconds, training seconds, mean loss over a fixed sample of at most
+// 512 pairs before and after}.
 extern "C" int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo, double kappa_hi, int nsrc, int m,
                              int k, double harvest_tol, int max_pairs, int nlayers, const int *channels, int keep,
                              int steps, int batch, double lr, unsigned long long seed, double *loss, double *stats) {
@@ -361,10 +381,16 @@ extern "C" int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo
   M.alloc_grads(c.g);
   double *d_scr = nullptr;
   QQCD_HIP(hipMalloc(&d_scr, sizeof(double) * ((2 * c.g.Vh + 255) / 256 + 16)));
+  // the mean loss over a fixed sample of at most 512 pairs, evenly spaced
+  // over the harvest (every configuration and every stage of the solves)
+  const size_t nsample = std::min<size_t>(512, T.pairs.size());
   auto mean_loss = [&]() {
     double s = 0;
-    for (auto &p : T.pairs) s += pair_loss(c, M, T.U[p.conf], p, d_scr);
-    return s / T.pairs.size();
+    for (size_t q = 0; q < nsample; q++) {
+      const Pair &p = T.pairs[q * T.pairs.size() / nsample];
+      s += pair_loss(c, M, T, p, d_scr);
+    }
+    return s / nsample;
   };
   try {
     const double before = mean_loss();
@@ -383,7 +409,7 @@ extern "C" int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo
       double ls = 0;
       for (int bi = 0; bi < batch; bi++) {
         const Pair &p = T.pairs[pick(rng)];
-        ls += pair_grad(c, M, T.U[p.conf], p, d_scr, dW);
+        ls += pair_grad(c, M, T, p, d_scr, dW);
       }
       loss[step] = ls / batch;
       const double bc1 = 1.0 - std::pow(b1, step + 1), bc2 = 1.0 - std::pow(b2, step + 1);

@@ -37,7 +37,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from baylorlqcd_amd import _lib, diracops, disconloops, gauge, lattice, learned, quark  # noqa: E402
+from baylorlqcd_amd import _lib, diracops, disconloops, gauge, inverters, lattice, learned, quark  # noqa: E402
 from baylorlqcd_amd.lagfib import LagFib  # noqa: E402
 
 N = (32, 32, 32, 64)
@@ -192,9 +192,12 @@ def rows_section(no_cpu=False):
             traffic = json.load(f)
     ms = np.zeros(1)
     _lib.check(L.qqcd_bench_smear(3, 20, _lib.dp(ms)))
-    g = 2400 * V / (ms[0] * 1e-3) / 1e9
-    smear = {"us_per_iteration": round(ms[0] * 1e3, 1), "bytes_per_site": 2400, "GB/s": round(g, 1),
-             "frac": round(g / HBM_PEAK_GBS, 3),
+    # unique bytes per output site and checkerboard launch: the site's own and the other
+    # checkerboard's input spinor, the six spatial links of its hops and the output once (FP64)
+    g = 1440 * V / (ms[0] * 1e-3) / 1e9
+    smear = {"us_per_iteration": round(ms[0] * 1e3, 1), "bytes_per_site": 1440, "bytes_model": "unique",
+             "GB/s": round(g, 1), "frac": round(g / HBM_PEAK_GBS, 3),
+             "neighbour_counted_GB/s": round(2400 * V / (ms[0] * 1e-3) / 1e9, 1),
              "traffic_per_launch": traffic.get("k_smear", {}).get("bytes_per_launch")}
     print(f"[bench] smear {smear}", file=sys.stderr, flush=True)
     m = 2048 * 32768
@@ -257,18 +260,24 @@ def heatbath_field(seed=2024, sweeps=200):
     return co, plaq
 
 
+_CPU_SOLVE = []
+
+
 def cpu_solve_baseline():
     """The reference's own gmresdr (oracle/_ref, 8^3x16 build, one core) on the config-1 field (the golden
     fixture), Wilson kappa 0.150, point source, GMRES-DR(40,16) to 1e-10, repeated for about three seconds:
     even-site Schur iterations x sites per second, the rate the GPU rows are compared with.  Run in a child
     process: the reference's gmresdr writes its own progress lines to standard output."""
+    if _CPU_SOLVE:
+        return _CPU_SOLVE[0]
     import subprocess
     code = "import json, sys, bench; sys.stderr.write('CPU_SOLVE ' + json.dumps(bench._cpu_solve_child()) + '\\n')"
     out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300)
     lines = [ln for ln in out.stderr.splitlines() if ln.startswith("CPU_SOLVE ")]
     if out.returncode != 0 or not lines:
         raise RuntimeError(out.stderr.strip().splitlines()[-1] if out.stderr.strip() else "child failed")
-    return json.loads(lines[-1][len("CPU_SOLVE "):])
+    _CPU_SOLVE.append(json.loads(lines[-1][len("CPU_SOLVE "):]))
+    return _CPU_SOLVE[0]
 
 
 def _cpu_solve_child():
@@ -290,9 +299,67 @@ def _cpu_solve_child():
         reps += 1
     dt = time.perf_counter() - t0
     vh = int(np.prod(n)) // 2
+    cfg1 = {"cores": 1, "kind": "reference"}
+    for name, which, nk in (("gmresdr_40_10", 5, (40, 10)), ("bicgstab", 0, (40, 10))):
+        t1 = time.perf_counter()
+        _, it1 = R.solve(which, be, u, co, (0.150, 0.0), nk=nk, resmax=1e-10)
+        cfg1[name] = {"seconds": round(time.perf_counter() - t1, 4), "iterations": it1}
     return {"value": round(vh * its / dt / 1e6, 3), "unit": "M site-iterations/s", "cores": 1, "kind": "reference",
             "sample": f"{reps} GMRES-DR(40,16) solves of the reference Fortran on 8^3x16, {its // reps} iterations "
-                      f"each ({dt / reps:.2f} s each)"}
+                      f"each ({dt / reps:.2f} s each)", "config1": cfg1}
+
+
+def config1_section(no_cpu=False):
+    """Config 1 beside the reference: 8^3x16, the golden fixture's beta = 6.0
+    heatbath field (200 of the reference's own sweeps), Wilson kappa 0.150, the
+    even half of the point source at the origin (spin 1, colour 1) as the
+    even-odd right-hand side, GMRES-DR(40,10) and BiCGStab to the reference's
+    relative tolerance 1e-10; the GPU drop-ins timed by the host clock around
+    the call (transfers included, median of 5) against the reference's own
+    gmresdr and bicgstab on one host core (oracle/_ref, child process)."""
+    lattice.finalize()
+    n = (8, 8, 8, 16)
+    lattice.init(n, BC)
+    g = np.load(os.path.join(ROOT, "tests", "golden", "golden_8x8x8x16.npz"))
+    co = np.zeros(64)
+    for mu in range(4):
+        co[2 + 4 * (0 + 4 * mu)] = 1.0
+        co[2 + 4 * (1 + 4 * mu)] = 1.0
+    diracops.gauge_load_canonical(gauge.reconstruct12(g["links12"]), co)
+    u = diracops.gauge_get_ref(n)
+    be = lattice.new_spinor(n).ravel(order="F")
+    bo = lattice.new_spinor(n).ravel(order="F")
+    quark.pointsource(be, bo, (1, 1, 1, 1), 1, 1)
+    kap = (0.150, 0.0)
+    out = {"lattice": list(n), "kappa": kap[0], "source": "point, origin, spin 1 colour 1 (even half)",
+           "tolerance": "1e-10 relative to the initial even-odd residual (the reference's test)"}
+    for name, call in (("gmresdr_40_10", lambda x: inverters.gmresdr(None, be, x, (40, 10), 1e-10, 0, u, None, -1,
+                                                                        kap, co, BC)),
+                       ("bicgstab", lambda x: inverters.bicgstab(None, be, x, 1e-10, 0, u, None, -1, kap, co, BC))):
+        ts, its = [], 0
+        for _ in range(5):
+            x = np.zeros_like(be)
+            _lib.check(_lib.lib().qqcd_deflation_clear())
+            t0 = time.perf_counter()
+            its = call(x)
+            ts.append(time.perf_counter() - t0)
+        st = inverters.last_solve_stats()
+        out[name] = {"gpu_seconds": round(float(np.median(ts)), 4), "iterations": its,
+                     "relative_residual": float(f"{st['rel_res']:.3e}")}
+    if not no_cpu:
+        try:
+            cpu = cpu_solve_baseline()
+            out["cpu_baseline"] = cpu.get("config1") if cpu else None
+        except Exception as e:  # reported, never required
+            out["cpu_baseline"] = {"sample": f"unavailable: {e}"}
+        ref = out.get("cpu_baseline") or {}
+        for name in ("gmresdr_40_10", "bicgstab"):
+            if name in ref and ref[name].get("seconds"):
+                out[name]["reference_cpu_seconds"] = ref[name]["seconds"]
+                out[name]["speedup_vs_reference"] = round(ref[name]["seconds"] / out[name]["gpu_seconds"], 1)
+    print(f"[bench] config 1: {out}", file=sys.stderr, flush=True)
+    lattice.finalize()
+    return out
 
 
 def solve_section(train_steps=1500, no_cpu=False):
@@ -322,6 +389,18 @@ def solve_section(train_steps=1500, no_cpu=False):
     print(f"[bench] twisted plain FP64: {tplain}", file=sys.stderr, flush=True)
     tmixed = bench_solve(1, tm, 1, 12)
     print(f"[bench] twisted mixed FP32: {tmixed}", file=sys.stderr, flush=True)
+    # three more held-out 32^3x64 fields (four with the first): the mixed FP32 solve with and without
+    # the network on each
+    held = [{"seed": 2024, "plaquette": round(plaq, 5), "mixed_fp32": mixed, "mixed_fp32_learned": lrn}]
+    for seed in (2025, 2026, 2027):
+        _, pq = heatbath_field(seed)
+        hm = learned.bench(1, wil, GMRES=(40, 16), op_prec=1, op_recon=12)
+        hl = learned.bench(2, wil, GMRES=(40, 16), op_prec=1, op_recon=12)
+        held.append({"seed": seed, "plaquette": round(pq, 5), "mixed_fp32": hm, "mixed_fp32_learned": hl})
+        print(f"[bench] held-out field {seed}: mixed {hm} learned {hl}", file=sys.stderr, flush=True)
+    for h in held:
+        h["iteration_ratio"] = round(h["mixed_fp32_learned"]["iterations"] / max(1, h["mixed_fp32"]["iterations"]), 3)
+        h["time_ratio"] = round(h["mixed_fp32_learned"]["seconds"] / h["mixed_fp32"]["seconds"], 3)
     best = min((mixed, mixed16, mixed16h, lrn), key=lambda r: r["seconds"])
     vh = int(np.prod(N)) // 2
     rates = {"plain_fp64": round(vh * plain["iterations"] / plain["seconds"] / 1e6, 1),
@@ -348,6 +427,7 @@ def solve_section(train_steps=1500, no_cpu=False):
                                               "speedup_mixed_vs_plain": round(tplain["seconds"] /
                                                                               tmixed["seconds"], 2)},
             "learned_preconditioner": {"training": train, "network_ms_per_application": round(nn_ms, 3),
+                                       "held_out_fields_kappa_0.1565": held,
                                        "layers": "4 parallel-transport convolutions, channels 1-8-8-8-1, "
                                                  "BF16 MFMA, FP32 accumulation"}}
 
@@ -369,13 +449,14 @@ def profile_solve_section(which):
 
 def train_network(train_steps):
     """The learned preconditioner trained on ten 16^3x32 beta=6.0 heatbath
-    configurations (Wilson kappa uniform in [0.150, 0.157], pairs from plain
-    GMRES-DR iterates); returns the training record."""
+    configurations (Wilson kappa uniform in [0.150, 0.157]) on 20,000
+    (residual, error) pairs from every Arnoldi step of plain GMRES-DR(40,16)
+    solves; returns the training record."""
     lattice.finalize()
     lattice.init((16, 16, 16, 32), BC)
     t0 = time.perf_counter()
-    tr = learned.train(nconf=10, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=2, m=40, k=16, harvest_tol=1e-8,
-                       max_pairs=800, channels=(1, 8, 8, 8, 1), steps=train_steps, batch=8, lr=1e-3, seed=11)
+    tr = learned.train(nconf=10, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=16, m=40, k=16, harvest_tol=1e-8,
+                       max_pairs=20000, channels=(1, 8, 8, 8, 1), steps=train_steps, batch=8, lr=1e-3, seed=11)
     train = {"lattice": [16, 16, 16, 32], "configs": 10, "pairs": tr["pairs"], "adam_steps": train_steps,
              "batch": 8, "lr": 1e-3, "loss_before": round(tr["loss_before"], 4),
              "loss_after": round(tr["loss_after"], 4), "harvest_s": round(tr["harvest_s"], 2),
@@ -557,6 +638,9 @@ def main():
     if not args.no_rows:
         rows = rows_section(args.no_cpu)
 
+    config1 = None
+    if not args.no_solve:
+        config1 = config1_section(args.no_cpu)
     solve = None
     if not args.no_solve:
         solve = solve_section(args.train_steps, args.no_cpu)
@@ -593,6 +677,7 @@ def main():
                      "us_same_buffers": round(ms_hot * 1e3, 2)},
         "cpu_baseline": cpu,
         "sweep": sweep,
+        "config1": config1,
         "solve": solve,
         "noise_batch": noise,
         "rows": rows,

@@ -452,14 +452,16 @@ int qqcd_nn_wgrad_canonical(int layer, const double *psi, const double *g, doubl
 /* z = N(r) on the even sites of canonical arrays (the preconditioner as the solver applies it). */
 int qqcd_nn_apply_canonical(const double *r, double *z);
 /* Train on nconf seeded heatbath configurations of the current lattice:
- * (residual, error) pairs from the cycle ends of plain FP64 GMRES-DR(m, k)
- * solves of nsrc Gaussian sources per configuration (kappa uniform in
- * [kappa_lo, kappa_hi]), at most max_pairs; loss ||N(r) - e||^2 / ||e||^2 with
+ * (residual, error) pairs from every Arnoldi step of plain FP64
+ * GMRES-DR(m, k) solves of up to nsrc Gaussian sources per configuration
+ * (kappa uniform in [kappa_lo, kappa_hi]), max_pairs in all, spread evenly
+ * over the configurations and kept in FP16; loss ||N(r) - e||^2 / ||e||^2 with
  * ||r|| = 1; `steps` Adam steps (lr, betas 0.9 / 0.999) on minibatches of
  * `batch` pairs, gradients from the device forward / adjoint / weight-gradient
  * kernels.  keep = 0 starts from the kappa^2-Jacobi initialisation with the
  * given channels.  loss[steps]: minibatch losses; stats[5] = {pairs, harvest
- * seconds, training seconds, mean loss before, mean loss after}.  Leaves the
+ * seconds, training seconds, mean loss before, mean loss after (over a fixed
+ * sample of at most 512 pairs)}.  Leaves the
  * last training configuration resident. */
 int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo, double kappa_hi, int nsrc, int m, int k,
                   double harvest_tol, int max_pairs, int nlayers, const int *channels, int keep, int steps, int batch,
