@@ -88,6 +88,41 @@ __device__ __forceinline__ void ld_h4(const void *p, float4 m, long i, double2 (
   e[3] = make_double2((double)(m.w * (float)v[6]), (double)(m.w * (float)v[7]));
 }
 
+// the same four elements in single precision, and four elements of an FP32
+// or FP64 vector as single-precision pairs (the FP16-basis passes accumulate
+// within a lane in FP32: fewer registers, and the basis carries 11 bits)
+__device__ __forceinline__ void ld_h4f(const void *p, float4 m, long i, float2 (&e)[4]) {
+  const h8 v = ((const h8 *)p)[i];
+  e[0] = make_float2(m.x * (float)v[0], m.x * (float)v[1]);
+  e[1] = make_float2(m.y * (float)v[2], m.y * (float)v[3]);
+  e[2] = make_float2(m.z * (float)v[4], m.z * (float)v[5]);
+  e[3] = make_float2(m.w * (float)v[6], m.w * (float)v[7]);
+}
+template <typename T> __device__ __forceinline__ void ld4f(const void *p, long i, float2 (&e)[4]);
+template <> __device__ __forceinline__ void ld4f<float>(const void *p, long i, float2 (&e)[4]) {
+  const float4 a = ((const float4 *)p)[2 * i], b = ((const float4 *)p)[2 * i + 1];
+  e[0] = make_float2(a.x, a.y);
+  e[1] = make_float2(a.z, a.w);
+  e[2] = make_float2(b.x, b.y);
+  e[3] = make_float2(b.z, b.w);
+}
+template <> __device__ __forceinline__ void ld4f<double>(const void *p, long i, float2 (&e)[4]) {
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const double2 d = ((const double2 *)p)[4 * i + u];
+    e[u] = make_float2((float)d.x, (float)d.y);
+  }
+}
+template <typename T> __device__ __forceinline__ void st4f(void *p, long i, const float2 (&e)[4]);
+template <> __device__ __forceinline__ void st4f<float>(void *p, long i, const float2 (&e)[4]) {
+  ((float4 *)p)[2 * i] = make_float4(e[0].x, e[0].y, e[1].x, e[1].y);
+  ((float4 *)p)[2 * i + 1] = make_float4(e[2].x, e[2].y, e[3].x, e[3].y);
+}
+template <> __device__ __forceinline__ void st4f<double>(void *p, long i, const float2 (&e)[4]) {
+#pragma unroll
+  for (int u = 0; u < 4; u++) ((double2 *)p)[4 * i + u] = make_double2(e[u].x, e[u].y);
+}
+
 // conj(a) b accumulated into (re, im)
 __device__ __forceinline__ void cdot_acc(double2 a, double2 b, double &re, double &im) {
   re += a.x * b.x + a.y * b.y;
@@ -140,23 +175,31 @@ __global__ void __launch_bounds__(RB) k_mdot(long nu, PtrPack v, const void *w, 
   for (int k = 0; k < 2 * K; k++) acc[k] = 0;
   if constexpr (std::is_same<TV, f16>::value) {
     const long nq = nu / 12;  // site quads
+    float af[2 * K];
+#pragma unroll
+    for (int k = 0; k < 2 * K; k++) af[k] = 0.f;
     for (long q = blockIdx.x * (long)RB + threadIdx.x; q < nq; q += (long)gridDim.x * RB) {
       float4 m[K];
 #pragma unroll
       for (int k = 0; k < K; k++) m[k] = *(const float4 *)(v.n[k] + 4 * q);
       for (int c = 0; c < 12; c++) {
         const long i = c * nq + q;
-        double2 b[4];
-        C2<TW>::template ld<4>(w, wn, stride, i, b);
+        float2 b[4];
+        ld4f<TW>(w, i, b);
 #pragma unroll
         for (int k = 0; k < K; k++) {
-          double2 a[4];
-          ld_h4(v.p[k], m[k], i, a);
+          float2 a[4];
+          ld_h4f(v.p[k], m[k], i, a);
 #pragma unroll
-          for (int u = 0; u < 4; u++) cdot_acc(a[u], b[u], acc[2 * k], acc[2 * k + 1]);
+          for (int u = 0; u < 4; u++) {
+            af[2 * k] += a[u].x * b[u].x + a[u].y * b[u].y;
+            af[2 * k + 1] += a[u].x * b[u].y - a[u].y * b[u].x;
+          }
         }
       }
     }
+#pragma unroll
+    for (int k = 0; k < 2 * K; k++) acc[k] = (double)af[k];
   } else {
     for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
       double2 b[U];
@@ -198,27 +241,36 @@ __global__ void __launch_bounds__(RB) k_maxpy(long nu, PtrPack v, CoefPack h, vo
   double acc[1] = {0};
   if constexpr (std::is_same<TV, f16>::value) {
     const long nq = nu / 12;  // site quads
+    float fr[K], fi[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      fr[k] = (float)(h.re[k]);
+      fi[k] = (float)(h.im[k]);
+    }
+    float nf = 0.f;
     for (long q = blockIdx.x * (long)RB + threadIdx.x; q < nq; q += (long)gridDim.x * RB) {
       float4 m[K];
 #pragma unroll
       for (int k = 0; k < K; k++) m[k] = *(const float4 *)(v.n[k] + 4 * q);
       for (int c = 0; c < 12; c++) {
         const long i = c * nq + q;
-        double2 b[4];
-        C2<TW>::template ld<4>(w, nullptr, stride, i, b);
+        float2 b[4];
+        ld4f<TW>(w, i, b);
 #pragma unroll
         for (int k = 0; k < K; k++) {
-          double2 a[4];
-          ld_h4(v.p[k], m[k], i, a);
+          float2 a[4];
+          ld_h4f(v.p[k], m[k], i, a);
 #pragma unroll
-          for (int u = 0; u < 4; u++) b[u] = csub_mul(b[u], h.re[k], h.im[k], a[u]);
+          for (int u = 0; u < 4; u++)
+            b[u] = make_float2(b[u].x - (fr[k] * a[u].x - fi[k] * a[u].y), b[u].y - (fr[k] * a[u].y + fi[k] * a[u].x));
         }
-        C2<TW>::template st<4>(w, i, b);
+        st4f<TW>(w, i, b);
         if (NORM)
 #pragma unroll
-          for (int u = 0; u < 4; u++) acc[0] += b[u].x * b[u].x + b[u].y * b[u].y;
+          for (int u = 0; u < 4; u++) nf += b[u].x * b[u].x + b[u].y * b[u].y;
       }
     }
+    if (NORM) acc[0] = (double)nf;
   } else {
     for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
       double2 b[U];
@@ -254,27 +306,36 @@ __global__ void __launch_bounds__(RB) k_maxpy_d(long nu, PtrPack v, const double
   double acc[1] = {0};
   if constexpr (std::is_same<TV, f16>::value) {
     const long nq = nu / 12;  // site quads
+    float fr[K], fi[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      fr[k] = (float)(hr[k]);
+      fi[k] = (float)(hi[k]);
+    }
+    float nf = 0.f;
     for (long q = blockIdx.x * (long)RB + threadIdx.x; q < nq; q += (long)gridDim.x * RB) {
       float4 m[K];
 #pragma unroll
       for (int k = 0; k < K; k++) m[k] = *(const float4 *)(v.n[k] + 4 * q);
       for (int c = 0; c < 12; c++) {
         const long i = c * nq + q;
-        double2 b[4];
-        C2<TW>::template ld<4>(w, nullptr, stride, i, b);
+        float2 b[4];
+        ld4f<TW>(w, i, b);
 #pragma unroll
         for (int k = 0; k < K; k++) {
-          double2 a[4];
-          ld_h4(v.p[k], m[k], i, a);
+          float2 a[4];
+          ld_h4f(v.p[k], m[k], i, a);
 #pragma unroll
-          for (int u = 0; u < 4; u++) b[u] = csub_mul(b[u], hr[k], hi[k], a[u]);
+          for (int u = 0; u < 4; u++)
+            b[u] = make_float2(b[u].x - (fr[k] * a[u].x - fi[k] * a[u].y), b[u].y - (fr[k] * a[u].y + fi[k] * a[u].x));
         }
-        C2<TW>::template st<4>(w, i, b);
+        st4f<TW>(w, i, b);
         if (NORM)
 #pragma unroll
-          for (int u = 0; u < 4; u++) acc[0] += b[u].x * b[u].x + b[u].y * b[u].y;
+          for (int u = 0; u < 4; u++) nf += b[u].x * b[u].x + b[u].y * b[u].y;
       }
     }
+    if (NORM) acc[0] = (double)nf;
   } else {
     for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
       double2 b[U];
@@ -686,33 +747,44 @@ void Blas::combine(int m, const Vec *V, int k, const std::complex<double> *Z, in
   QQCD_HIP(hipStreamSynchronize(stream));
 }
 
-// Time one fused pass over k lattice vectors of precision p (FP64 or FP32)
-// and grid workgroups: op 0 the multi-dot, 1 the multi-axpy with norm.
+// Time one fused pass over k lattice vectors of precision p (FP64, FP32, or
+// FP16 with an FP32 w) and grid workgroups: op 0 the multi-dot, 1 the
+// multi-axpy with norm.
 // Average kernel time in ms over niter launches (tuning only).
 double bench_blas(long n, Prec p, int k, int op, int grid, int niter, hipStream_t s) {
   if (k < 1 || k > KMAXP || grid < 1 || grid > 8192 || niter < 1) throw std::runtime_error("bench_blas: bad sizes");
-  if (p != PREC_DOUBLE && p != PREC_SINGLE) throw std::runtime_error("bench_blas: FP64 or FP32");
-  const size_t bytes = (size_t)n * (p == PREC_DOUBLE ? 16 : 8);
-  std::vector<void *> v(k + 1, nullptr);
+  if (p != PREC_DOUBLE && p != PREC_SINGLE && p != PREC_HALF)
+    throw std::runtime_error("bench_blas: FP64, FP32 or FP16 (with an FP32 w)");
+  // FP16: the k basis vectors in FP16 with per-site norms, w in FP32
+  const Prec pw = p == PREC_HALF ? PREC_SINGLE : p;
+  const size_t bytes = (size_t)n * (p == PREC_DOUBLE ? 16 : p == PREC_SINGLE ? 8 : 4);
+  const size_t wbytes = (size_t)n * (pw == PREC_DOUBLE ? 16 : 8);
+  std::vector<void *> v(k + 1, nullptr), nv(k, nullptr);
   double *part = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   float ms = 0;
   try {
-    for (auto &q : v) {
-      QQCD_HIP(hipMalloc(&q, bytes));
-      QQCD_HIP(hipMemsetAsync(q, 0, bytes, s));
+    for (int q = 0; q <= k; q++) {
+      const size_t b = q < k ? bytes : wbytes;
+      QQCD_HIP(hipMalloc(&v[q], b));
+      QQCD_HIP(hipMemsetAsync(v[q], 0, b, s));
+      if (q < k && p == PREC_HALF) {
+        QQCD_HIP(hipMalloc(&nv[q], sizeof(float) * (n / 12)));
+        QQCD_HIP(hipMemsetAsync(nv[q], 0, sizeof(float) * (n / 12), s));
+      }
     }
     QQCD_HIP(hipMalloc(&part, sizeof(double) * grid * 2 * KMAXP));
     PtrPack pp{};
     CoefPack cp{};
     for (int q = 0; q < k; q++) {
       pp.p[q] = v[q];
+      pp.n[q] = (const float *)nv[q];
       cp.re[q] = 1e-3;
     }
     QQCD_HIP(hipEventCreate(&e0));
     QQCD_HIP(hipEventCreate(&e1));
     auto run = [&]() {
-      tsw(p, p, [&](auto ta, auto tb) {
+      tsw(p, pw, [&](auto ta, auto tb) {
         TYPES_OF(ta, tb)
         with_k(k, [&](auto K) {
           constexpr int KK = decltype(K)::value;
@@ -735,12 +807,16 @@ double bench_blas(long n, Prec p, int k, int op, int grid, int niter, hipStream_
   } catch (...) {
     for (auto &q : v)
       if (q) (void)hipFree(q);
+    for (auto &q : nv)
+      if (q) (void)hipFree(q);
     if (part) (void)hipFree(part);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     throw;
   }
   for (auto &q : v) (void)hipFree(q);
+  for (auto &q : nv)
+    if (q) (void)hipFree(q);
   (void)hipFree(part);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

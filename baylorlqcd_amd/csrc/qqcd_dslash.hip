@@ -208,7 +208,8 @@ struct DslashArgs {
   // checkerboard index; order 1 walks blocks of bz z-planes, all t of a block
   // before the next block (z fastest inside the block), so the t and z
   // neighbours of a wavefront were read a few thousand sites earlier by the
-  // same XCD and are still in its L2
+  // same XCD and are still in its L2; order 2 does the same with blocks of
+  // bz t-slices swept in z
   int order;
   int bz;
   int plane;  // sites of one (z, t) plane of a checkerboard: nx*ny/2
@@ -734,6 +735,12 @@ __device__ __forceinline__ void dslash_site_pipe(const DslashArgs &a, const HopI
 __device__ __forceinline__ int site_of(const DslashArgs &a, int lidx) {
   if (a.order == 0) return lidx;
   const int p = lidx / a.plane, o = lidx - p * a.plane;
+  if (a.order == 2) {  // blocks of bz t-slices, swept in z
+    const int per = a.bz * a.n[2];
+    const int tb = p / per, r = p - tb * per;
+    const int z = r / a.bz, t = tb * a.bz + (r - z * a.bz);
+    return (z + a.n[2] * t) * a.plane + o;
+  }
   const int per = a.bz * a.n[3];
   const int zb = p / per, r = p - zb * per;
   const int t = r / a.bz, z = zb * a.bz + (r - t * a.bz);
@@ -866,9 +873,9 @@ void set_xcd_remap(int v) { g_xcd_remap = v; }
 
 static DslashTune g_tune;
 void set_dslash_tune(const DslashTune &t) {
-  if (t.order < 0 || t.order > 1 || t.bz < 0 || (t.depth != 0 && t.depth != 1 && t.depth != 2 && t.depth != 4) ||
+  if (t.order < 0 || t.order > 2 || t.bz < 0 || (t.depth != 0 && t.depth != 1 && t.depth != 2 && t.depth != 4) ||
       t.nt < 0 || t.nt > 1)
-    throw std::runtime_error("dslash tuning: order 0/1, bz >= 0, depth 0/1/2/4, nt 0/1");
+    throw std::runtime_error("dslash tuning: order 0/1/2, bz >= 0, depth 0/1/2/4, nt 0/1");
   g_tune = t;
 }
 DslashTune get_dslash_tune() { return g_tune; }
@@ -900,10 +907,14 @@ static void launch_t(const DslashArgs &a0, int sig, hipStream_t s) {
     DslashArgs a = a0;
     a.plane = a.n[0] * a.n[1] / 2;
     int bz = t.bz;
-    if (bz == 0)  // the largest of 16, 8, 4, 2 that divides nz
-      for (bz = 16; bz > 1 && a.n[2] % bz; bz /= 2) {
+    // order 1: blocks of bz z-planes swept in t; order 2: blocks of bz
+    // t-slices swept in z (bz 0: the largest of 16, 8, 4, 2 dividing the
+    // blocked extent)
+    const int ext = t.order == 2 ? a.n[3] : a.n[2];
+    if (bz == 0)
+      for (bz = 16; bz > 1 && ext % bz; bz /= 2) {
       }
-    a.order = (t.order == 1 && bz > 1 && a.n[2] % bz == 0) ? 1 : 0;
+    a.order = ((t.order == 1 || t.order == 2) && bz > 1 && ext % bz == 0) ? t.order : 0;
     a.bz = a.order ? bz : 1;
     if (sig > 0) launch_pipe<S, L, RECON, 1>(a, t.depth, t.nt, nb, s);
     else launch_pipe<S, L, RECON, -1>(a, t.depth, t.nt, nb, s);

@@ -29,6 +29,18 @@ struct SmArgs {
 
 __device__ __forceinline__ double2 sm_ld(const double2 *p, long stride, int k, long i) { return p[k * stride + i]; }
 
+// the links and the output are touched once per launch: loaded and stored
+// non-temporally, so the L2 is left to the neighbour spinors
+__device__ __forceinline__ double2 nt_ld(const double2 *p) {
+  const double *q = (const double *)p;
+  return make_double2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+}
+__device__ __forceinline__ void nt_st(double2 v, double2 *p) {
+  double *q = (double *)p;
+  __builtin_nontemporal_store(v.x, q);
+  __builtin_nontemporal_store(v.y, q + 1);
+}
+
 // w (+)= s * M v (M row-major 3x3 at link (par, mu, i)), or M^dag v
 template <bool DAG>
 __device__ __forceinline__ void sm_mv(const double2 *U, long us, int par, int mu, long i, const double2 v[3],
@@ -36,7 +48,7 @@ __device__ __forceinline__ void sm_mv(const double2 *U, long us, int par, int mu
   const long base = (long)((par * 4 + mu) * 9) * us + i;
   double2 m[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) m[k] = U[base + (long)k * us];
+  for (int k = 0; k < 9; k++) m[k] = nt_ld(U + base + (long)k * us);
 #pragma unroll
   for (int r = 0; r < 3; r++) {
     double re = 0, im = 0;
@@ -102,7 +114,7 @@ __global__ void __launch_bounds__(128) k_smear(SmArgs a, const double2 *U, const
 #pragma unroll
   for (int k = 0; k < 12; k++) {
     const double2 h = sm_ld(same, a.stride, k, i);
-    out[(long)k * a.stride + i] = make_double2(a.b * h.x + a.a * acc[k].x, a.b * h.y + a.a * acc[k].y);
+    nt_st(make_double2(a.b * h.x + a.a * acc[k].x, a.b * h.y + a.a * acc[k].y), out + (long)k * a.stride + i);
   }
 }
 

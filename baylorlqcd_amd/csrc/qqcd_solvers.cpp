@@ -164,8 +164,12 @@ static void arnoldi_step_dev(Context &c, LinOp &A, std::vector<Spinor> &V, int j
 // [j0, j0 + n) that already ran on the same vectors within this solve is
 // launched as one graph, which removes the launch gaps between its ~13
 // kernels per step.  Only for operators whose buffers stay put between
-// applications (every LinOp here except a preconditioner's).
-static int g_step_graphs = 1;
+// applications (every LinOp here except a preconditioner's).  Off by
+// default: the inner solves of the mixed solver allocate a fresh basis per
+// outer step, so each graph is replayed about three times, and the captures
+// cost more than the replays save (32^3x64 mixed FP16 solve: 1.32-1.35 s
+// with graphs, 1.25 s without).
+static int g_step_graphs = 0;
 void set_step_graphs(int on) { g_step_graphs = on ? 1 : 0; }
 
 struct StepGraphs {

@@ -120,7 +120,7 @@ namespace qqcd {
 // -> plain FP64 GMRES-DR), so the FP64 solver is the fallback and the FP64
 // true residual the judge of every result.
 // replay repeated groups of synchronisation-free Arnoldi steps from HIP
-// graphs (default on)
+// graphs (default off)
 void set_step_graphs(int on);
 
 struct MixedParams {

@@ -4,25 +4,35 @@ One step = one application of the hopping term H_oe (V/2 = 1,048,576 output
 sites) on the resident gauge field, FP64 storage and arithmetic with
 18-real links -- the reference's precision.  Inputs are resident in HBM
 before timing starts; the time is the HIP-event average per launch on the
-library's stream.  The other storage precisions (FP32, FP16 with per-site
-norms; 18- and 12-real links) are measured on the same input and reported in
-``sweep`` with their relative error against FP64.
+library's stream, the launches cycling over four input/output pairs so that
+no launch finds its input in the last-level cache.  The other storage
+precisions (FP32, FP16 with per-site norms; 18- and 12-real links) are
+measured on the same input and reported in ``sweep`` with their relative
+error against FP64.
 
-Algorithmic bytes per output site: 8*24*s_psi + 8*L*s_U + 24*s_psi
-(L = 18 or 12 link reals); FP16 spinors add 9*4 bytes of per-site norms.
+Unique bytes per output site (the value's model): the input spinor and the
+output once and the eight links of the site's hops once,
+2*24*s_psi + 8*L*s_U (L = 18 or 12 link reals; FP16 adds 8 bytes of norms);
+the neighbour-counted rate (8*24*s_psi + 8*L*s_U + 24*s_psi) is reported
+beside it.
 
-A second section (``solve``) times the solve path on a seeded beta = 6.0
-heatbath field (device heatbath), point source, to a true full-system
-residual ||b - M x|| / ||b|| < 1e-10: at Wilson kappa = 0.1565 the plain
-FP64 GMRES-DR(40,16), the mixed-precision solver (FP32 Krylov basis, FP32 or
-FP16 12-real operator, FP64 reliable updates) and the mixed solver with the
-learned preconditioner (trained first on 16^3x32 heatbath configurations,
-flexible GMRES-DR); at twisted mass kappa = 0.1565, mu = 0.005 plain and
-mixed.  A third (``noise_batch``) runs the disconnected-loop workload on
+``config1`` times config 1 (8^3x16) beside the reference's own solvers.
+``solve`` times the solve path on a seeded beta = 6.0 heatbath field (device
+heatbath), point source, to a true full-system residual ||b - M x|| / ||b||
+< 1e-10: at Wilson kappa = 0.1565 the plain FP64 GMRES-DR(40,16), the
+mixed-precision solver (FP32 or FP16 12-real operator, FP32 or FP16 Krylov
+basis, FP64 reliable updates) and the mixed solver with the learned
+preconditioner (trained first on 16^3x32 heatbath configurations, flexible
+GMRES-DR; four held-out fields); at twisted mass kappa = 0.1565, mu = 0.005
+plain and mixed.  ``noise_batch`` runs the disconnected-loop workload on
 24^3x48: 64 Z4 sources, the first by GMRES-DR(60,20), the rest together by
 the batched GMRES-Proj, against the sequential plain FP64 and mixed solves.
+``propagator`` solves the twelve spin-colour point sources on 48^3x96
+(twisted mass), the first by the mixed solver and the other eleven batched,
+against plain FP64 GMRES-DR on the first.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-solve] [--no-noise]
+                    [--no-rows] [--no-propagator] [--propagator-learned N]
 """
 from __future__ import annotations
 
@@ -455,7 +465,7 @@ def train_network(train_steps):
     lattice.finalize()
     lattice.init((16, 16, 16, 32), BC)
     t0 = time.perf_counter()
-    tr = learned.train(nconf=10, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=16, m=40, k=16, harvest_tol=1e-8,
+    tr = learned.train(nconf=10, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=24, m=40, k=16, harvest_tol=1e-8,
                        max_pairs=20000, channels=(1, 8, 8, 8, 1), steps=train_steps, batch=8, lr=1e-3, seed=11)
     train = {"lattice": [16, 16, 16, 32], "configs": 10, "pairs": tr["pairs"], "adam_steps": train_steps,
              "batch": 8, "lr": 1e-3, "loss_before": round(tr["loss_before"], 4),
@@ -466,7 +476,7 @@ def train_network(train_steps):
     return train
 
 
-def propagator_section(train_steps=1500, n5=(48, 48, 48, 96), nlearned=2):
+def propagator_section(train_steps=1500, n5=(48, 48, 48, 96), nlearned=0, trained=None):
     """The full propagator at scale: a 48^3x96 beta=6.0 heatbath field,
     twisted mass kappa 0.1563, mu 0.0015, the 12 spin-colour point sources at
     the origin to a 1e-10 true residual each.  The first by the mixed solver
@@ -474,7 +484,9 @@ def propagator_section(train_steps=1500, n5=(48, 48, 48, 96), nlearned=2):
     FP64 reliable updates), the other 11 batched through GMRES-Proj(24) over
     its deflation space; against plain FP64 GMRES-DR(40,16) on the first
     source; and the learned preconditioner on the first `nlearned` sources."""
-    train = train_network(train_steps)
+    # the network the solve section trained stays loaded in the library
+    train = trained if trained is not None or nlearned == 0 else train_network(train_steps)
+    lattice.finalize()
     lattice.init(n5, BC)
     co, plaq = heatbath_field()
     tm = (0.1563, 0.0015)
@@ -483,9 +495,12 @@ def propagator_section(train_steps=1500, n5=(48, 48, 48, 96), nlearned=2):
     b = quark.propagator(12, (1, 1, 1, 1), tm, GMRES=(40, 16), tol=1e-10, op_prec=2, op_recon=12, method=0)
     print(f"[bench] propagator batched: iters {b['iters']} res {b['res_full'].max():.2e} times {b['times']} "
           f"stats {b['stats']}", file=sys.stderr, flush=True)
-    lr = quark.propagator(nlearned, (1, 1, 1, 1), tm, GMRES=(40, 16), tol=1e-10, op_prec=2, op_recon=12, method=2)
-    print(f"[bench] propagator learned: iters {lr['iters']} res {lr['res_full'].max():.2e} times {lr['times']} "
-          f"stats {lr['stats']}", file=sys.stderr, flush=True)
+    lr = None
+    if nlearned > 0:
+        lr = quark.propagator(nlearned, (1, 1, 1, 1), tm, GMRES=(40, 16), tol=1e-10, op_prec=2, op_recon=12,
+                              method=2)
+        print(f"[bench] propagator learned: iters {lr['iters']} res {lr['res_full'].max():.2e} times {lr['times']} "
+              f"stats {lr['stats']}", file=sys.stderr, flush=True)
     return {"lattice": list(n5), "beta": 6.0, "plaquette": round(plaq, 5), "kappa": tm[0], "mu": tm[1],
             "sources": "12 spin-colour point sources at the origin", "target": "1e-10 true residual each",
             "plain_fp64_gmresdr_40_16_source1": plain,
@@ -494,15 +509,15 @@ def propagator_section(train_steps=1500, n5=(48, 48, 48, 96), nlearned=2):
                                       "seconds_first": round(float(b["times"][0]), 3),
                                       "seconds_other_11": round(float(b["times"][1]), 3),
                                       "seconds_total": round(float(b["times"][2]), 3), "stats": b["stats"]},
-            "learned_first_sources": {"iterations": [int(v) for v in lr["iters"]],
-                                      "max_true_residual": float(lr["res_full"].max()),
-                                      "seconds": [round(float(v), 3) for v in lr["times"]], "stats": lr["stats"]},
+            "learned_first_sources": None if lr is None else {
+                "iterations": [int(v) for v in lr["iters"]], "max_true_residual": float(lr["res_full"].max()),
+                "seconds": [round(float(v), 3) for v in lr["times"]], "stats": lr["stats"]},
             "plain_seconds_x12_estimate": round(12 * plain["seconds"], 1),
             "speedup_vs_plain_x12": round(12 * plain["seconds"] / float(b["times"][2]), 2),
             "training": train}
 
 
-def noise_section(nrhs=64, seq=3, graph=1):
+def noise_section(nrhs=64, seq=3, graph=1, fp32_basis=False):
     """The disconnected-loop workload: 24^3x48 beta=6.0 heatbath field,
     Wilson kappa 0.1565, nrhs Z4 sources, GMRES-DR(60,20) on the first and
     the batched GMRES-Proj(40) over its deflation space on the rest (Krylov
@@ -518,11 +533,13 @@ def noise_section(nrhs=64, seq=3, graph=1):
     disconloops.batch_set(use_graph=graph)
     r = disconloops.noise_loop(nrhs, 99, kappa, (60, 20), tol=1e-10, coact=co, bc=BC)
     print(f"[bench] batched noise loop: times {r['times']} stats {r['stats']}", file=sys.stderr, flush=True)
-    disconloops.batch_set(use_graph=graph, basis_prec=1)
-    h = disconloops.noise_loop(nrhs, 99, kappa, (60, 20), tol=1e-10, coact=co, bc=BC)
-    disconloops.batch_set(use_graph=graph)
-    print(f"[bench] batched noise loop, FP32 basis: times {h['times']} stats {h['stats']}", file=sys.stderr,
-          flush=True)
+    h = None
+    if fp32_basis:
+        disconloops.batch_set(use_graph=graph, basis_prec=1)
+        h = disconloops.noise_loop(nrhs, 99, kappa, (60, 20), tol=1e-10, coact=co, bc=BC)
+        disconloops.batch_set(use_graph=graph)
+        print(f"[bench] batched noise loop, FP32 basis: times {h['times']} stats {h['stats']}", file=sys.stderr,
+              flush=True)
     s = disconloops.noise_loop(seq, 99, kappa, (60, 20), tol=1e-10, coact=co, bc=BC, method=1)
     per = s["times"][1] / max(1, seq - 1)
     print(f"[bench] sequential plain FP64: times {s['times']}", file=sys.stderr, flush=True)
@@ -536,13 +553,14 @@ def noise_section(nrhs=64, seq=3, graph=1):
                         "total_s": round(r["times"][2], 3), "max_rel_residual": float(np.max(r["res_full"])),
                         "iterations_per_source": int(np.median(r["iters"][1:])) if nrhs > 1 else None,
                         "stats": r["stats"]},
-            "batched_fp32_basis": {"batched_s": round(h["times"][1], 3), "total_s": round(h["times"][2], 3),
-                                   "max_rel_residual": float(np.max(h["res_full"])),
-                                   "iterations_per_source": int(np.median(h["iters"][1:])) if nrhs > 1 else None,
-                                   "traces_max_rel_diff_vs_fp16_basis":
-                                       float(np.max(np.abs(h["traces"] - r["traces"])) / np.max(np.abs(r["traces"]))),
-                                   "speedup_vs_sequential_mixed": round(perm * (nrhs - 1) / h["times"][1], 2),
-                                   "stats": h["stats"]},
+            "batched_fp32_basis": None if h is None else {
+                "batched_s": round(h["times"][1], 3), "total_s": round(h["times"][2], 3),
+                "max_rel_residual": float(np.max(h["res_full"])),
+                "iterations_per_source": int(np.median(h["iters"][1:])) if nrhs > 1 else None,
+                "traces_max_rel_diff_vs_fp16_basis":
+                    float(np.max(np.abs(h["traces"] - r["traces"])) / np.max(np.abs(r["traces"]))),
+                "speedup_vs_sequential_mixed": round(perm * (nrhs - 1) / h["times"][1], 2),
+                "stats": h["stats"]},
             "autotune": {"rhs_block": nb, "tile_order": order, "ms_per_hop": round(tms, 4)},
             "sequential_plain_fp64_s_per_source": round(per, 3),
             "sequential_mixed_s_per_source": round(perm, 3),
@@ -567,8 +585,12 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="also time a 1 GiB plain copy kernel")
     ap.add_argument("--profile-solve", choices=sorted(PROFILE_SOLVES),
                     help="also run one 32^3x64 solve alone (for kernel traces)")
-    ap.add_argument("--propagator", action="store_true",
-                    help="also run the 48^3x96 twisted-mass 12-source propagator (several minutes)")
+    ap.add_argument("--no-propagator", action="store_true",
+                    help="skip the 48^3x96 twisted-mass 12-source propagator (about 90 s)")
+    ap.add_argument("--propagator-learned", type=int, default=0,
+                    help="also solve the first N propagator sources with the learned preconditioner (about a minute each)")
+    ap.add_argument("--noise-fp32-basis", action="store_true",
+                    help="also run the batched noise loop with an FP32 Krylov basis")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
@@ -646,13 +668,14 @@ def main():
         solve = solve_section(args.train_steps, args.no_cpu)
     noise = None
     if not args.no_noise:
-        noise = noise_section(args.noise_rhs, graph=0 if args.no_graph else 1)
+        noise = noise_section(args.noise_rhs, graph=0 if args.no_graph else 1, fp32_basis=args.noise_fp32_basis)
     psolve = None
     if args.profile_solve:
         psolve = profile_solve_section(args.profile_solve)
     propagator = None
-    if args.propagator:
-        propagator = propagator_section(args.train_steps)
+    if not args.no_propagator:
+        propagator = propagator_section(args.train_steps, nlearned=args.propagator_learned,
+                                        trained=solve["learned_preconditioner"]["training"] if solve else None)
 
     line = {
         "metric": "Dslash HBM GB/s (unique bytes / kernel time), 32^3x64 Wilson hopping term",

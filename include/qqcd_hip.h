@@ -570,7 +570,8 @@ int qqcd_bench_lagfib(long m, int niter, long chunk, double *ms);
 int qqcd_bench_copy(double bytes, int iters, double *ms);
 
 /* Kernel time (ms, mean of niter) of one fused Krylov pass over k (1..16)
- * checkerboard vectors of the current lattice in prec 0 (FP64) / 1 (FP32):
+ * checkerboard vectors of the current lattice in prec 0 (FP64) / 1 (FP32) /
+ * 2 (FP16 vectors with per-site norms, an FP32 w):
  * op 0 the multi-dot <v_i, w>, op 1 the multi-axpy w -= sum h_i v_i with
  * ||w||^2; grid workgroups of 256 threads.  Tuning aid; no reference
  * counterpart. */
@@ -584,7 +585,8 @@ int qqcd_bench_blas(int prec, int k, int op, int grid, int niter, double *ms);
 int qqcd_set_multi_tune(int kernel, int bz, int nt);
 
 /* Toggle the replay of repeated groups of Krylov steps from HIP graphs in
- * the single-right-hand-side solvers (default on).  Tuning aid; no
+ * the single-right-hand-side solvers (default off: with a fresh basis per
+ * inner solve the captures cost more than the replays save).  Tuning aid; no
  * reference counterpart. */
 int qqcd_set_step_graphs(int on);
 
@@ -593,8 +595,9 @@ int qqcd_set_xcd_remap(int on);
 
 /* Launch shape of the single-right-hand-side Dslash (projector form): depth
  * 0 computes the eight hops one after the other, 1/2/4 issues that many
- * hops' loads ahead; order 1 sweeps blocks of bz z-planes in t (bz 0: the
- * largest of 16, 8, 4, 2 dividing nz; order 0: the checkerboard index); nt 1
+ * hops' loads ahead; order 1 sweeps blocks of bz z-planes in t, order 2
+ * blocks of bz t-slices in z (bz 0: the largest of 16, 8, 4, 2 dividing the
+ * blocked extent; order 0: the checkerboard index); nt 1
  * streams links and output past the L2.  Default (1, 0, 2, 1).  Tuning aid;
  * no reference counterpart. */
 int qqcd_set_dslash_tune(int order, int bz, int depth, int nt);

@@ -222,3 +222,36 @@ def test_solve_log_lines():
         assert [l[1] for l in log] == list(range(1, len(log) + 1))
     finally:
         lattice.finalize()
+
+
+def test_step_graphs_replay_same_solve():
+    """The mixed solver (FP16 operator and basis, one Gram-Schmidt pass) with
+    its groups of Arnoldi steps replayed from HIP graphs gives the eager
+    launches' solution bit for bit and the same step count, twice in a row
+    (graphs captured in the first solve are not carried into the second)."""
+    from baylorlqcd_amd import _lib
+    g = np.load(os.path.join(GOLD, "golden_8x8x8x16.npz"))
+    lattice.init(N8, BC)
+    U = gauge.reconstruct12(g["links12"])
+    u = O.links_canon_to_ref(N8, U)
+    co = O.coact_array()
+    kappa = (0.150, 0.0)
+    be, bo = lattice.new_spinor(N8), lattice.new_spinor(N8)
+    quark.pointsource(be, bo, (1, 1, 1, 1), 1, 1)
+    runs = []
+    try:
+        for graphs in (0, 1, 1):
+            _lib.check(_lib.lib().qqcd_set_step_graphs(graphs))
+            x = np.zeros_like(be)
+            it = inverters.solve_mixed(be, x, (40, 10), 1e-10, 2, 2, 12, 0.0, 0, u, co, kappa, BC)
+            runs.append((it, x))
+    finally:
+        _lib.check(_lib.lib().qqcd_set_step_graphs(0))
+        lattice.finalize()
+    assert runs[0][0] == runs[1][0] == runs[2][0]
+    assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][1], runs[2][1])
+    bb = O.spinor_ref_to_canon(N8, be.ravel(order="F"), None)
+    xm = O.spinor_ref_to_canon(N8, runs[1][1].ravel(order="F"), None)
+    odd = O.parity_mask(N8)
+    r = bb - O.schur(N8, U, BC, [1] * 4, [1] * 4, 0, kappa, xm)
+    assert np.linalg.norm(r[~odd]) / np.linalg.norm(bb[~odd]) < 1e-10
