@@ -341,7 +341,8 @@ extern "C" int qqcd_nn_wgrad_canonical(int layer, const double *psi, const doubl
 // Pairs come from every Arnoldi step (max_pairs in all, spread evenly over the
 // configurations).  loss[steps]: mean minibatch loss per step; stats = {pairs,
 // harvest seconds, training seconds, mean loss over a fixed sample of at most
-// 512 pairs before and after}.
+// 512 pairs before and after (for the kept weights), validation steps of the
+// starting and of the kept weights}.
 extern "C" int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo, double kappa_hi, int nsrc, int m,
                              int k, double harvest_tol, int max_pairs, int nlayers, const int *channels, int keep,
                              int steps, int batch, double lr, unsigned long long seed, double *loss, double *stats) {
@@ -363,10 +364,12 @@ extern "C" int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo
     coact[2 + 4 * (1 + 4 * mu)] = 1.0;
   }
   const int per_conf = (max_pairs + nconf - 1) / nconf;
+  double last_kappa = kappa_hi;
   for (int conf = 0; conf < nconf; conf++) {
     if (qqcd_gauge_heatbath(beta, nsweep, seed * 7919ull + conf, 1, coact) != 0)
       throw std::runtime_error(std::string("nn_train: heatbath: ") + qqcd_last_error());
     const double kap[2] = {kappa_lo + (kappa_hi - kappa_lo) * uni(rng), 0.0};
+    last_kappa = kap[0];
     const int before = (int)T.pairs.size();
     harvest(c, kap, nsrc, m, k, harvest_tol, seed + 31ull * conf, conf, before + per_conf, T);
     // keep this configuration's links
@@ -392,8 +395,32 @@ extern "C" int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo
     }
     return s / nsample;
   };
+  // Validation by the solver itself: Krylov steps of flexible GMRES-DR(40,16)
+  // with the network as right preconditioner to 1e-6 on a point source (the
+  // origin, spin and colour 0) of the last training configuration, FP32
+  // operator; the weights that need the fewest steps are kept (the starting
+  // weights, near kappa^2 * 1, included), checked every val_every Adam steps.
+  // A lower loss alone does not make a better preconditioner.
+  const int val_every = 150;
+  auto validate = [&]() -> int {
+    const double kv[2] = {last_kappa, 0.0};
+    SchurOp A32(c, PREC_SINGLE, 12, kv);
+    LinOp op = A32.op();
+    LinOp pre = nn_precond(c);
+    SpinorBuf b(c.g, PREC_SINGLE), x(c.g, PREC_SINGLE);
+    Vec bv = as_vec(b.s), xv = as_vec(x.s);
+    c.blas->zero(bv);
+    c.blas->zero(xv);
+    const float one[2] = {1.f, 0.f};  // component 0 of checkerboard site 0, the origin
+    QQCD_HIP(hipMemcpyAsync(b.s.data, one, sizeof(one), hipMemcpyHostToDevice, c.stream));
+    SolveStats st = gmresdr(c, op, b.s, x.s, 40, 16, 1e-6, 0, 2000, nullptr, 0, &pre);
+    return st.converged ? st.iters : 1000000 + st.iters;
+  };
   try {
     const double before = mean_loss();
+    int val_best = validate();
+    const int val_init = val_best;
+    std::vector<NNLayer> best = M.layers;
     // Adam on the real and imaginary parts of every weight
     const int L = (int)M.layers.size();
     std::vector<std::vector<double>> m1(L), m2(L);
@@ -429,7 +456,16 @@ extern "C" int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo
         }
       }
       M.upload(c.stream);
+      if ((step + 1) % val_every == 0 || step + 1 == steps) {
+        const int v = validate();
+        if (v < val_best) {
+          val_best = v;
+          best = M.layers;
+        }
+      }
     }
+    M.layers = best;
+    M.upload(c.stream);
     const double after = mean_loss();
     g_nn.layers = M.layers;
     const auto t2 = std::chrono::steady_clock::now();
@@ -438,6 +474,8 @@ extern "C" int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo
     stats[2] = std::chrono::duration<double>(t2 - t1).count();
     stats[3] = before;
     stats[4] = after;
+    stats[5] = val_init;
+    stats[6] = val_best;
   } catch (...) {
     (void)hipFree(d_scr);
     throw;

@@ -80,14 +80,21 @@ def apply_canonical(r):
 def train(nconf=4, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=1, m=40, k=16, harvest_tol=1e-8,
           max_pairs=256, channels=CHANNELS, keep=0, steps=200, batch=4, lr=1e-3, seed=1):
     loss = np.zeros(max(steps, 1))
-    stats = np.zeros(5)
+    stats = np.zeros(7)
     ch = _ch(channels)
     _lib.check(_lib.lib().qqcd_nn_train(
         int(nconf), float(beta), int(nsweep), float(kappa[0]), float(kappa[1]), int(nsrc), int(m), int(k),
         float(harvest_tol), int(max_pairs), len(channels) - 1, ch.ctypes.data_as(_lib._ip), int(keep), int(steps),
         int(batch), float(lr), int(seed), _lib.dp(loss), _lib.dp(stats)))
     return {"loss": loss[:steps], "pairs": int(stats[0]), "harvest_s": stats[1], "train_s": stats[2],
-            "loss_before": stats[3], "loss_after": stats[4]}
+            "loss_before": stats[3], "loss_after": stats[4],
+            "validation_start": _validation(stats[5]), "validation_kept": _validation(stats[6])}
+
+
+def _validation(v):
+    """Decode a validation solve count: Krylov steps, plus 1,000,000 when it did not converge."""
+    v = int(v)
+    return {"krylov_steps": v % 1000000, "converged": v < 1000000}
 
 
 def bench(method, kappa, GMRES=(40, 16), tol=1e-10, op_prec=1, op_recon=12, src=(1, 1, 1, 1), spin=1, colour=1):

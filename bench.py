@@ -468,9 +468,11 @@ def train_network(train_steps):
     tr = learned.train(nconf=10, beta=6.0, nsweep=200, kappa=(0.150, 0.157), nsrc=24, m=40, k=16, harvest_tol=1e-8,
                        max_pairs=20000, channels=(1, 8, 8, 8, 1), steps=train_steps, batch=8, lr=1e-3, seed=11)
     train = {"lattice": [16, 16, 16, 32], "configs": 10, "pairs": tr["pairs"], "adam_steps": train_steps,
-             "batch": 8, "lr": 1e-3, "loss_before": round(tr["loss_before"], 4),
-             "loss_after": round(tr["loss_after"], 4), "harvest_s": round(tr["harvest_s"], 2),
-             "train_s": round(tr["train_s"], 2), "wall_s": round(time.perf_counter() - t0, 2)}
+             "batch": 8, "lr": 1e-3, "loss_before": round(float(tr["loss_before"]), 4),
+             "loss_after": round(float(tr["loss_after"]), 4),
+             "validation_solve": {"start": tr["validation_start"], "kept": tr["validation_kept"]},
+             "harvest_s": round(float(tr["harvest_s"]), 2),
+             "train_s": round(float(tr["train_s"]), 2), "wall_s": round(time.perf_counter() - t0, 2)}
     print(f"[bench] training {train}", file=sys.stderr, flush=True)
     lattice.finalize()
     return train
