@@ -363,6 +363,12 @@ extern "C" int qqcd_set_multi_tune(int kernel, int bz, int nt) {
   API_END
 }
 
+extern "C" int qqcd_set_multi_lds(int bytes) {
+  API_BEGIN
+  set_multi_lds(bytes);
+  API_END
+}
+
 extern "C" int qqcd_bench_dslash(int prec, int recon, int which, int nwarm, int niter, double *ms, double *relerr) {
   API_BEGIN
   Context &c = ctx();

@@ -529,24 +529,19 @@ template <typename R> __device__ __forceinline__ void fma_acc_i(cx<R> &acc, R f,
 
 // The hop is linear in the neighbour spinor, so its scale factors (c1, the
 // boundary factor, the FP16 per-site norm, the 2 of the t projector) are
-// one scalar applied in the accumulation, not to the loaded values.
-template <typename S, typename L, int RECON, int SIG, int K, typename R>
-__device__ __forceinline__ void hop_consume(const HopRaw<S, L, RECON> &r, cx<R> acc[12], R sc) {
+// one scalar f applied in the accumulation, not to the loaded values.
+// hop_math: acc += f (1 +/- g_mu) U p for the link u of hop K and the loaded
+// neighbour components p (for a t hop, the kept spin pair in p[0..5]).
+template <int SIG, int K, typename R>
+__device__ __forceinline__ void hop_math(const cx<R> u[9], const cx<R> p[12], cx<R> acc[12], R f) {
   constexpr int MU = K >> 1;
   constexpr bool FWD = !(K & 1);
-  constexpr int NC = RECON == 18 ? 9 : 6;
-  cx<R> u[9];
-#pragma unroll
-  for (int k = 0; k < NC; k++) u[k] = mk<R>((R)r.u[k].x, (R)r.u[k].y);
-  if constexpr (RECON == 12) recon_row2(u);
-  R f = sc * (R)r.edge;
-  if constexpr (std::is_same<S, f16>::value) f *= (R)r.nrm;
   cx<R> w[6];
   if constexpr (MU == 3) {
     constexpr int s0 = ((FWD ? SIG : -SIG) > 0) ? 0 : 6;
     cx<R> hh[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) hh[c] = mk<R>((R)r.p[c].x, (R)r.p[c].y);
+    for (int c = 0; c < 6; c++) hh[c] = p[c];
     if (FWD) {
       mv(u, hh, w);
       mv(u, hh + 3, w + 3);
@@ -560,9 +555,7 @@ __device__ __forceinline__ void hop_consume(const HopRaw<S, L, RECON> &r, cx<R> 
   } else {
     constexpr int sg = FWD ? SIG : -SIG;
     const R sig = R(sg);
-    cx<R> p[12], hh[6];
-#pragma unroll
-    for (int k = 0; k < 12; k++) p[k] = mk<R>((R)r.p[k].x, (R)r.p[k].y);
+    cx<R> hh[6];
     project<MU>(p, sig, hh);
     if (FWD) {
       mv(u, hh, w);
@@ -588,6 +581,22 @@ __device__ __forceinline__ void hop_consume(const HopRaw<S, L, RECON> &r, cx<R> 
       }
     }
   }
+}
+
+template <typename S, typename L, int RECON, int SIG, int K, typename R>
+__device__ __forceinline__ void hop_consume(const HopRaw<S, L, RECON> &r, cx<R> acc[12], R sc) {
+  constexpr int NC = RECON == 18 ? 9 : 6;
+  constexpr int NP = (K >> 1) == 3 ? 6 : 12;
+  cx<R> u[9];
+#pragma unroll
+  for (int k = 0; k < NC; k++) u[k] = mk<R>((R)r.u[k].x, (R)r.u[k].y);
+  if constexpr (RECON == 12) recon_row2(u);
+  R f = sc * (R)r.edge;
+  if constexpr (std::is_same<S, f16>::value) f *= (R)r.nrm;
+  cx<R> p[12];
+#pragma unroll
+  for (int k = 0; k < NP; k++) p[k] = mk<R>((R)r.p[k].x, (R)r.p[k].y);
+  hop_math<SIG, K>(u, p, acc, f);
 }
 
 // Stage boundary of the pipeline: the accumulators pass through an empty
@@ -781,6 +790,7 @@ struct MultiArgs {
   int site_tiles;
   long blk_elems;  // elements of one RHS block (12 * stride * nb)
   long blk_norms;  // norms of one RHS block (stride * nb)
+  int per_tile;    // k_dslash_multi_pipe: one workgroup per tile instead of a persistent grid
 };
 
 template <typename S, typename L, int RECON, int SIG>
@@ -819,7 +829,12 @@ __global__ void __launch_bounds__(256) k_dslash_multi(const DslashArgs a, const 
 // every XCD sweeps all sites of one block, so the t and z neighbours of its
 // wavefronts were read a few slabs earlier into its own L2 and the links,
 // read by all XCDs at about the same pace, are shared in the last-level
-// cache.  Persistent like k_dslash_multi: the active count is read at launch.
+// cache.  Persistent like k_dslash_multi (the active count is read at
+// launch), or (per_tile) one workgroup per tile of the full capacity, whose
+// workgroups beyond the active tiles return at once: the hardware then hands
+// each XCD its range in order, so the sites in flight on an XCD stay within a
+// few slabs, where the workgroups of a persistent grid drift apart over the
+// sweep and spread its reads beyond the L2.
 template <typename S, typename L, int RECON, int SIG, bool NT, int D>
 __global__ void __launch_bounds__(256) k_dslash_multi_pipe(const DslashArgs a, const MultiArgs m) {
   typedef typename V2<S>::T SE;
@@ -830,9 +845,11 @@ __global__ void __launch_bounds__(256) k_dslash_multi_pipe(const DslashArgs a, c
   const int g8 = gridDim.x >> 3;  // workgroups per XCD (the grid is a multiple of 8)
   const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
   const long per = (ntiles + 7) >> 3;
-  const long t0 = xcd * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+  const long r0 = xcd * per, r1 = r0 + per < ntiles ? r0 + per : ntiles;
+  // per_tile: the one tile w of the XCD's range (none when w is past it)
+  const long t0 = r0 + w, t1 = m.per_tile ? (t0 + 1 < r1 ? t0 + 1 : r1) : r1, step = m.per_tile ? 1 : g8;
   const int spt = 256 >> m.lognb;  // sites per tile
-  for (long t = t0 + w; t < t1; t += g8) {
+  for (long t = t0; t < t1; t += step) {
     int blk, st;
     if (m.order == 0) {
       blk = (int)(t / m.site_tiles);
@@ -851,6 +868,225 @@ __global__ void __launch_bounds__(256) k_dslash_multi_pipe(const DslashArgs a, c
                                                     a.x ? (const SE *)a.x + eo : nullptr,
                                                     a.x_norm ? a.x_norm + no : nullptr, i, rr, m.lognb);
   }
+}
+
+// ---- two right-hand sides per lane (FP32 spinors) -------------------------
+// Slots 2q and 2q+1 of an RHS block are adjacent in memory (element
+// j*nb + r), so one 16-byte load fetches a component of both; the lane loads
+// and rebuilds the link of each hop once for the two and keeps two
+// accumulators.  Per right-hand side this halves the load instructions and
+// the link and index arithmetic of the one-lane-per-slot kernel, whose
+// issue rate, not its HBM traffic, bounds it.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename L, int RECON> struct HopRaw2 {
+  typename V2<L>::T u[RECON == 18 ? 9 : 6];
+  f32x4 p[12];  // (re, im) of slot 2q, then (re, im) of slot 2q+1
+  float edge;
+};
+
+template <typename L, int RECON, int SIG, bool NT, int K>
+__device__ __forceinline__ void hop_fetch2(const DslashArgs &a, const HopIn &h, int i, int x, int y, int z, int t,
+                                           int q2, int lognb, HopRaw2<L, RECON> &r) {
+  constexpr int MU = K >> 1;
+  constexpr bool FWD = !(K & 1);
+  constexpr int NC = RECON == 18 ? 9 : 6;
+  constexpr int AUXL = NT ? 2 : 0;
+  typedef typename V2<L>::T LE;
+  int j;
+  hop_nbr<K>(a, x, y, z, t, j, r.edge);
+  const int lpar = FWD ? a.par : (a.par ^ 1);
+  const uint32_t lo = (uint32_t)(FWD ? i : j) * (uint32_t)sizeof(LE);
+  const uint32_t so = (((uint32_t)j << lognb) + (uint32_t)q2) * 8u;
+  const uint32_t lrun = (uint32_t)h.lstride * (uint32_t)sizeof(LE), srun = (uint32_t)h.sstride * 8u;
+  const __amdgpu_buffer_rsrc_t lr =
+      make_rsrc((const char *)h.links + (size_t)((lpar * 4 + MU) * NC) * lrun, NC * lrun);
+#pragma unroll
+  for (int k = 0; k < NC; k++) r.u[k] = bld<LE, AUXL>(lr, lo, k * lrun);
+  constexpr int s0 = MU == 3 ? (((FWD ? SIG : -SIG) > 0) ? 0 : 6) : 0;
+  constexpr int ns = MU == 3 ? 6 : 12;
+  const __amdgpu_buffer_rsrc_t sr = make_rsrc(h.in, 12 * srun);
+#pragma unroll
+  for (int k = 0; k < ns; k++) r.p[k] = bld<f32x4, 0>(sr, so, (s0 + k) * srun);
+}
+
+template <typename L, int RECON, int SIG, int K>
+__device__ __forceinline__ void hop_consume2(const HopRaw2<L, RECON> &r, cx<float> acc0[12], cx<float> acc1[12],
+                                             float sc) {
+  constexpr int NC = RECON == 18 ? 9 : 6;
+  constexpr int NP = (K >> 1) == 3 ? 6 : 12;
+  cx<float> u[9];
+#pragma unroll
+  for (int k = 0; k < NC; k++) u[k] = mk<float>((float)r.u[k].x, (float)r.u[k].y);
+  if constexpr (RECON == 12) recon_row2(u);
+  const float f = sc * r.edge;
+  cx<float> p[12];
+#pragma unroll
+  for (int k = 0; k < NP; k++) p[k] = mk<float>(r.p[k].x, r.p[k].y);
+  hop_math<SIG, K>(u, p, acc0, f);
+#pragma unroll
+  for (int k = 0; k < NP; k++) p[k] = mk<float>(r.p[k].z, r.p[k].w);
+  hop_math<SIG, K>(u, p, acc1, f);
+}
+
+template <typename L, int RECON> __device__ __forceinline__ void stage_pin2(HopRaw2<L, RECON> &r) {
+  constexpr int NC = RECON == 18 ? 9 : 6;
+  asm volatile("" ::: "memory");
+#pragma unroll
+  for (int k = 0; k < NC; k++) asm volatile("" : "+v"(r.u[k]));
+#pragma unroll
+  for (int k = 0; k < 12; k++) asm volatile("" : "+v"(r.p[k]));
+  asm volatile("" : "+v"(r.edge));
+}
+
+template <typename L, int RECON, int SIG, bool NT, int D, int K>
+__device__ __forceinline__ void hop_prefetch2(const DslashArgs &a, const HopIn &h, int i, int x, int y, int z, int t,
+                                              int q2, int lognb, HopRaw2<L, RECON> *buf) {
+  if constexpr (K < D) {
+    hop_fetch2<L, RECON, SIG, NT, K>(a, h, i, x, y, z, t, q2, lognb, buf[K]);
+    asm volatile("" ::: "memory");
+    hop_prefetch2<L, RECON, SIG, NT, D, K + 1>(a, h, i, x, y, z, t, q2, lognb, buf);
+  }
+}
+
+// hops K .. 7 of the pair, pipelined as hop_pipe: hop K is computed from its
+// buffer, which is refilled with hop K + D, or with the local pair of the
+// epilogue once no hop is left
+template <typename L, int RECON, int SIG, bool NT, int D, int K>
+__device__ __forceinline__ void hop_pipe2(const DslashArgs &a, const HopIn &h, int i, int x, int y, int z, int t,
+                                          int q2, int lognb, HopRaw2<L, RECON> *buf, cx<float> acc0[12],
+                                          cx<float> acc1[12], const void *xin, int io, f32x4 *xr) {
+  if constexpr (K < 8) {
+    stage_pin2(buf[K % D]);
+    hop_consume2<L, RECON, SIG, K>(buf[K % D], acc0, acc1, (float)a.scl[K >> 1]);
+    stage_fence(acc0);
+    stage_fence(acc1);
+    if constexpr (K + D < 8) {
+      hop_fetch2<L, RECON, SIG, NT, K + D>(a, h, i, x, y, z, t, q2, lognb, buf[K % D]);
+    } else if constexpr (K + D == 8) {
+      if (a.has_x) {
+        const uint32_t srun = (uint32_t)h.sstride * 8u;
+        const __amdgpu_buffer_rsrc_t xrs = make_rsrc(xin, 12 * srun);
+#pragma unroll
+        for (int k = 0; k < 12; k++) xr[k] = bld<f32x4, 0>(xrs, (uint32_t)io * 8u, k * srun);
+      }
+    }
+    hop_pipe2<L, RECON, SIG, NT, D, K + 1>(a, h, i, x, y, z, t, q2, lognb, buf, acc0, acc1, xin, io, xr);
+  }
+}
+
+// out = a0 x + a1 (i g5 x) + b0 D + b1 (i g5 D) for one right-hand side
+__device__ __forceinline__ void epilogue_f(const DslashArgs &a, const cx<float> acc[12], const cx<float> *xv,
+                                           cx<float> o[12]) {
+  const float b0 = (float)a.b0, b1 = (float)a.b1;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    o[c] = b0 * acc[c] + b1 * acc[6 + c];
+    o[3 + c] = b0 * acc[3 + c] + b1 * acc[9 + c];
+    o[6 + c] = b0 * acc[6 + c] - b1 * acc[c];
+    o[9 + c] = b0 * acc[9 + c] - b1 * acc[3 + c];
+  }
+  if (xv) {
+    const float a0 = (float)a.a0, a1 = (float)a.a1;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      o[c] = o[c] + a0 * xv[c] + a1 * xv[6 + c];
+      o[3 + c] = o[3 + c] + a0 * xv[3 + c] + a1 * xv[9 + c];
+      o[6 + c] = o[6 + c] + a0 * xv[6 + c] - a1 * xv[c];
+      o[9 + c] = o[9 + c] + a0 * xv[9 + c] - a1 * xv[3 + c];
+    }
+  }
+}
+
+// Output site i, slots q2 and q2 + 1 of the lane's RHS block (both: whether
+// slot q2 + 1 is active; a retired slot is computed but not written).
+template <typename L, int RECON, int SIG, bool NT, int D>
+__device__ __forceinline__ void dslash_site_pair(const DslashArgs &a, const HopIn &h, void *out, const void *xin,
+                                                 int i, int q2, int lognb, bool both) {
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+  const int r2 = 2 * i;
+  const int x0 = r2 % nx;
+  int q = r2 / nx;
+  const int y = q % ny;
+  q /= ny;
+  const int z = q % nz;
+  const int t = q / nz;
+  const int x = x0 + ((a.par + y + z + t) & 1);
+
+  HopRaw2<L, RECON> buf[D];
+  hop_prefetch2<L, RECON, SIG, NT, D, 0>(a, h, i, x, y, z, t, q2, lognb, buf);
+  const int io = (i << lognb) + q2;
+  f32x4 xr[12];
+  cx<float> acc0[12], acc1[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) acc0[k] = acc1[k] = mk<float>(0, 0);
+  hop_pipe2<L, RECON, SIG, NT, D, 0>(a, h, i, x, y, z, t, q2, lognb, buf, acc0, acc1, xin, io, xr);
+
+  cx<float> o0[12], o1[12], xv[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) xv[k] = mk<float>(xr[k].x, xr[k].y);
+  epilogue_f(a, acc0, a.has_x ? xv : nullptr, o0);
+#pragma unroll
+  for (int k = 0; k < 12; k++) xv[k] = mk<float>(xr[k].z, xr[k].w);
+  epilogue_f(a, acc1, a.has_x ? xv : nullptr, o1);
+
+  constexpr int AUXS = NT ? 2 : 0;
+  const uint32_t srun = (uint32_t)h.sstride * 8u;
+  const uint32_t oo = (uint32_t)io * 8u;
+  const __amdgpu_buffer_rsrc_t ors = make_rsrc(out, 12 * srun);
+  if (both) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      f32x4 w;
+      w.x = o0[k].re;
+      w.y = o0[k].im;
+      w.z = o1[k].re;
+      w.w = o1[k].im;
+      bst<f32x4, AUXS>(w, ors, oo, k * srun);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      typename V2<float>::T w;
+      w.x = o0[k].re;
+      w.y = o0[k].im;
+      bst<typename V2<float>::T, AUXS>(w, ors, oo, k * srun);
+    }
+  }
+}
+
+// One workgroup per tile of 256 lanes = 512/nb sites x nb/2 slot pairs of one
+// RHS block (nb >= 2), tiles split into eight contiguous ranges, one per XCD,
+// as k_dslash_multi_pipe with per_tile; m.site_tiles counts tiles of pairs.
+template <typename L, int RECON, int SIG, bool NT, int D>
+__global__ void __launch_bounds__(256) k_dslash_multi_pair(const DslashArgs a, const MultiArgs m) {
+  const int nact = *m.nact;
+  const int nb = 1 << m.lognb;
+  const int nblk = (nact + nb - 1) >> m.lognb;
+  const long ntiles = (long)m.site_tiles * nblk;
+  const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
+  const long per = (ntiles + 7) >> 3;
+  const long tt = xcd * per + w;
+  if (w >= per || tt >= ntiles) return;
+  int blk, st;
+  if (m.order == 0) {
+    blk = (int)(tt / m.site_tiles);
+    st = (int)(tt - (long)blk * m.site_tiles);
+  } else {
+    st = (int)(tt / nblk);
+    blk = (int)(tt - (long)st * nblk);
+  }
+  const int lp = m.lognb - 1;  // log2 of the pair lanes of a site
+  const int lidx = st * (256 >> lp) + ((int)threadIdx.x >> lp);
+  const int q2 = ((int)threadIdx.x & ((1 << lp) - 1)) << 1;
+  const int r0 = (blk << m.lognb) + q2;
+  if (lidx >= a.Vh || r0 >= nact) return;
+  const int i = site_of(a, lidx);
+  typedef typename V2<float>::T SE;
+  const long eo = (long)blk * m.blk_elems;
+  const HopIn h{(const SE *)a.in + eo, nullptr, a.links, a.sstride, a.lstride};
+  dslash_site_pair<L, RECON, SIG, NT, D>(a, h, (SE *)a.out + eo, a.x ? (const SE *)a.x + eo : nullptr, i, q2,
+                                         m.lognb, r0 + 1 < nact);
 }
 
 HopCoef make_hopcoef(const Geom &g, int idag) {
@@ -947,21 +1183,48 @@ static void launch_multi_sig(const DslashArgs &a, const MultiArgs &m, int grid, 
 }
 
 template <typename S, typename L, int RECON, int SIG, bool NT>
-static void launch_multi_pipe(const DslashArgs &a, const MultiArgs &m, int grid, hipStream_t s) {
+static void launch_multi_pipe(const DslashArgs &a, const MultiArgs &m, int grid, long max_tiles, int lds,
+                              hipStream_t s) {
   static int g0 = 0;
   if (!g0) g0 = persistent_grid(k_dslash_multi_pipe<S, L, RECON, SIG, NT, 2>);
-  const int g = grid > 0 ? (grid + 7) / 8 * 8 : g0;
-  hipLaunchKernelGGL((k_dslash_multi_pipe<S, L, RECON, SIG, NT, 2>), dim3(g), dim3(256), 0, s, a, m);
+  // per tile: eight ranges of the capacity's tiles, each range's workgroups
+  // spaced eight apart
+  const long gt = 8 * ((max_tiles + 7) / 8);
+  if (m.per_tile && gt > 0x7fffffffL) throw std::runtime_error("dslash_multi: too many tiles for one launch");
+  const int g = m.per_tile ? (int)gt : grid > 0 ? (grid + 7) / 8 * 8 : g0;
+  // lds (per tile only): unused dynamic LDS that caps the workgroups resident
+  // on a CU, and with them the sites in flight on an XCD
+  hipLaunchKernelGGL((k_dslash_multi_pipe<S, L, RECON, SIG, NT, 2>), dim3(g), dim3(256), m.per_tile ? lds : 0, s, a,
+                     m);
+  QQCD_HIP(hipGetLastError());
+}
+
+template <typename L, int RECON, int SIG, bool NT>
+static void launch_multi_pair(const DslashArgs &a, const MultiArgs &m0, int depth, long max_blk, hipStream_t s) {
+  MultiArgs m = m0;
+  m.site_tiles = (int)(((long)a.Vh * (1L << (m.lognb - 1)) + 255) / 256);
+  const long gt = 8 * (((long)m.site_tiles * max_blk + 7) / 8);
+  if (gt > 0x7fffffffL) throw std::runtime_error("dslash_multi: too many tiles for one launch");
+  if (depth == 1)
+    hipLaunchKernelGGL((k_dslash_multi_pair<L, RECON, SIG, NT, 1>), dim3((int)gt), dim3(256), 0, s, a, m);
+  else
+    hipLaunchKernelGGL((k_dslash_multi_pair<L, RECON, SIG, NT, 2>), dim3((int)gt), dim3(256), 0, s, a, m);
   QQCD_HIP(hipGetLastError());
 }
 
 template <typename S, typename L, int RECON>
-static void launch_multi(const DslashArgs &a0, const MultiArgs &m, int sig, const MultiTune &t, hipStream_t s) {
+static void launch_multi(const DslashArgs &a0, const MultiArgs &m0, int sig, const MultiTune &t, long max_blk,
+                         hipStream_t s) {
+  const long max_tiles = (long)m0.site_tiles * max_blk;
   // the pipelined kernel: projector form, buffer descriptors of < 4 GiB (one
   // RHS block of a field, one (parity, mu) block of links)
   const bool fits = 12.0 * a0.sstride * sizeof(typename V2<S>::T) < 4294967295.0 &&
                     (RECON == 18 ? 9.0 : 6.0) * a0.lstride * sizeof(typename V2<L>::T) < 4294967295.0;
-  if (t.kernel == 1 && sig != 0 && fits) {
+  if (t.kernel >= 1 && t.kernel <= 4 && sig != 0 && fits) {
+    MultiArgs m = m0;
+    // kernels 3 / 4 with FP64 spinors: the persistent form, the faster one-slot
+    // form there
+    m.per_tile = t.kernel == 2 || (t.kernel >= 3 && std::is_same<S, float>::value);
     DslashArgs a = a0;
     a.plane = a.n[0] * a.n[1] / 2;
     int bz = t.bz;
@@ -970,18 +1233,32 @@ static void launch_multi(const DslashArgs &a0, const MultiArgs &m, int sig, cons
       }
     a.order = bz > 1 ? 1 : 0;
     a.bz = bz > 1 ? bz : 1;
+    // kernels 3 / 4: two right-hand sides per lane, FP32 spinors, nb >= 2
+    if constexpr (std::is_same<S, float>::value) {
+      if (t.kernel >= 3 && m.lognb >= 1) {
+        const int depth = t.kernel == 3 ? 2 : 1;
+        if (sig > 0) {
+          if (t.nt) launch_multi_pair<L, RECON, 1, true>(a, m, depth, max_blk, s);
+          else launch_multi_pair<L, RECON, 1, false>(a, m, depth, max_blk, s);
+        } else {
+          if (t.nt) launch_multi_pair<L, RECON, -1, true>(a, m, depth, max_blk, s);
+          else launch_multi_pair<L, RECON, -1, false>(a, m, depth, max_blk, s);
+        }
+        return;
+      }
+    }
     if (sig > 0) {
-      if (t.nt) launch_multi_pipe<S, L, RECON, 1, true>(a, m, t.grid, s);
-      else launch_multi_pipe<S, L, RECON, 1, false>(a, m, t.grid, s);
+      if (t.nt) launch_multi_pipe<S, L, RECON, 1, true>(a, m, t.grid, max_tiles, t.lds, s);
+      else launch_multi_pipe<S, L, RECON, 1, false>(a, m, t.grid, max_tiles, t.lds, s);
     } else {
-      if (t.nt) launch_multi_pipe<S, L, RECON, -1, true>(a, m, t.grid, s);
-      else launch_multi_pipe<S, L, RECON, -1, false>(a, m, t.grid, s);
+      if (t.nt) launch_multi_pipe<S, L, RECON, -1, true>(a, m, t.grid, max_tiles, t.lds, s);
+      else launch_multi_pipe<S, L, RECON, -1, false>(a, m, t.grid, max_tiles, t.lds, s);
     }
     return;
   }
-  if (sig > 0) launch_multi_sig<S, L, RECON, 1>(a0, m, t.grid, s);
-  else if (sig < 0) launch_multi_sig<S, L, RECON, -1>(a0, m, t.grid, s);
-  else launch_multi_sig<S, L, RECON, 0>(a0, m, t.grid, s);
+  if (sig > 0) launch_multi_sig<S, L, RECON, 1>(a0, m0, t.grid, s);
+  else if (sig < 0) launch_multi_sig<S, L, RECON, -1>(a0, m0, t.grid, s);
+  else launch_multi_sig<S, L, RECON, 0>(a0, m0, t.grid, s);
 }
 
 // Kernel arguments shared by the single- and multi-RHS launches; returns
@@ -1081,13 +1358,21 @@ size_t MultiSpinor::bytes() const {
   return (size_t)elems() * e + (prec == PREC_HALF ? (size_t)stride * nb * nblk() * sizeof(float) : 0);
 }
 
-static int g_multi_kernel = -1, g_multi_bz = -1, g_multi_nt = -1;
+static int g_multi_kernel = -1, g_multi_bz = -1, g_multi_nt = -1, g_multi_lds = -1;
 void set_multi_override(int kernel, int bz, int nt) {
-  if (kernel < -1 || kernel > 1 || bz < -1 || nt < -1 || nt > 1)
-    throw std::runtime_error("multi-RHS tuning: kernel -1/0/1, bz >= -1, nt -1/0/1 (-1: the caller's choice)");
+  if (kernel < -1 || kernel > 4 || bz < -1 || nt < -1 || nt > 1)
+    throw std::runtime_error("multi-RHS tuning: kernel -1..4, bz >= -1, nt -1/0/1 (-1: the caller's choice)");
   g_multi_kernel = kernel;
   g_multi_bz = bz;
   g_multi_nt = nt;
+}
+void set_multi_lds(int bytes) {
+  int dev = 0, most = 0;
+  QQCD_HIP(hipGetDevice(&dev));
+  QQCD_HIP(hipDeviceGetAttribute(&most, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  if (bytes < -1 || bytes > most)
+    throw std::runtime_error("multi-RHS tuning: LDS bytes -1 (the default) or 0.." + std::to_string(most));
+  g_multi_lds = bytes;
 }
 
 void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpinor &out, int par_out, int idag,
@@ -1096,6 +1381,7 @@ void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpi
   if (g_multi_kernel >= 0) t.kernel = g_multi_kernel;
   if (g_multi_bz >= 0) t.bz = g_multi_bz;
   if (g_multi_nt >= 0) t.nt = g_multi_nt;
+  if (g_multi_lds >= 0) t.lds = g_multi_lds;
   if (in.prec != out.prec || (x && x->prec != out.prec))
     throw std::runtime_error("dslash_multi: spinor precisions differ");
   if (in.stride != g.stride || out.stride != g.stride || U.stride != g.stride)
@@ -1127,7 +1413,8 @@ void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpi
   dispatch_prec(in.prec, U, "dslash_multi", [&](auto *sp, auto *lp, auto rc) {
     typedef typename std::remove_pointer<decltype(sp)>::type S;
     typedef typename std::remove_pointer<decltype(lp)>::type L;
-    if constexpr (!std::is_same<S, f16>::value) launch_multi<S, L, decltype(rc)::value>(a, m, proj, t, s);
+    if constexpr (!std::is_same<S, f16>::value)
+      launch_multi<S, L, decltype(rc)::value>(a, m, proj, t, (long)in.nblk(), s);
   });
 }
 

@@ -94,10 +94,14 @@ struct MultiTune {
   // kernel 0: hops one after the other, tiles dealt to the XCDs per sweep of
   // the grid; 1: pipelined hops (depth 2), each XCD owning one contiguous
   // range of tiles for the whole launch and walking its sites in blocks of
-  // bz z-planes swept in t (bz 0: the largest of 8, 4, 2 dividing nz)
-  int kernel = 1;
+  // bz z-planes swept in t (bz 0: the largest of 8, 4, 2 dividing nz); 2: the
+  // same with one workgroup per tile, dispatched in order, instead of a
+  // persistent grid; 3 / 4: FP32 spinors two right-hand sides per lane, one
+  // workgroup per tile, two / one hops' loads in flight (FP64 spinors: as 1)
+  int kernel = 3;
   int bz = 0;
-  int nt = 1;  // kernel 1: links and output streamed past the L2
+  int nt = 0;  // kernel 1: links and output streamed past the L2 (measured slower here)
+  int lds = 0;  // kernel 2: dynamic LDS per workgroup (unused; caps the workgroups per CU)
 };
 
 // Launch shape of the single-right-hand-side hopping kernel (projector form):
@@ -116,6 +120,7 @@ struct DslashTune {
 void set_dslash_tune(const DslashTune &t);
 // overrides of the multi-RHS launch shape for every call (-1: the caller's)
 void set_multi_override(int kernel, int bz, int nt);
+void set_multi_lds(int bytes);
 DslashTune get_dslash_tune();
 
 // Coefficients of the local spin matrix applied in a Dslash epilogue:

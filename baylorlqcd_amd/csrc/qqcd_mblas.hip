@@ -70,8 +70,12 @@ struct MShape {
   double hs, hinv;  // FP16 basis scale and its inverse
 };
 
+// at most MAXK vectors per pass: a pass over q vectors reads (and, for the
+// multi-axpy, writes) the working field once, so chunks of 16 rather than 8
+// cut those re-reads; the slot sums go through LDS in groups of 18 values
+static constexpr int MAXK = 16;
 struct MPtr {
-  const void *p[8];
+  const void *p[MAXK];
 };
 
 // Sum NV per-lane values over the lanes of each slot in the workgroup;
@@ -79,24 +83,30 @@ struct MPtr {
 // part[((k * cap + slot0 + r) * gx + bx].
 template <int NV>
 __device__ __forceinline__ void slot_sum(double (&v)[NV], int lognb, double *part, int cap, int slot0, int gx) {
-  __shared__ double sm[MB / 64][NV][64];
+  constexpr int G = NV < 18 ? NV : 18;  // values per round through LDS
+  __shared__ double sm[MB / 64][G][64];
   const int nb = 1 << lognb;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < NV; k++) {
-    double s = v[k];
-    for (int o = 32; o >= nb; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane < nb) sm[w][k][lane] = s;
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < NV * nb; t += MB) {
-    const int k = t >> lognb, r = t & (nb - 1);
-    // the last block may hold fewer than nb slots
-    if (slot0 + r >= cap) continue;
-    double s = 0;
+  for (int k0 = 0; k0 < NV; k0 += G) {
+    if (k0 > 0) __syncthreads();  // the previous round's reads are done
 #pragma unroll
-    for (int q = 0; q < MB / 64; q++) s += sm[q][k][r];
-    part[((long)k * cap + slot0 + r) * gx + blockIdx.x] = s;
+    for (int k = k0; k < (k0 + G < NV ? k0 + G : NV); k++) {
+      double s = v[k];
+      for (int o = 32; o >= nb; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane < nb) sm[w][k - k0][lane] = s;
+    }
+    __syncthreads();
+    const int ng = k0 + G < NV ? G : NV - k0;
+    for (int t = threadIdx.x; t < ng * nb; t += MB) {
+      const int k = t >> lognb, r = t & (nb - 1);
+      // the last block may hold fewer than nb slots
+      if (slot0 + r >= cap) continue;
+      double s = 0;
+#pragma unroll
+      for (int q = 0; q < MB / 64; q++) s += sm[q][k][r];
+      part[((long)(k0 + k) * cap + slot0 + r) * gx + blockIdx.x] = s;
+    }
   }
 }
 
@@ -427,15 +437,24 @@ template <typename F> static void basis_types(Prec a, Prec b, F &&f) {
 
 #define T_OF(p) typename std::remove_pointer<decltype(p)>::type
 
+// length of the chunk starting at q0 when nq vectors are split into the
+// fewest chunks of at most MAXK, as even as possible
+static int chunk_len(int nq, int q0) {
+  const int n = (nq + MAXK - 1) / MAXK, base = nq / n, extra = nq % n;
+  // chunks 0 .. extra-1 hold base + 1 vectors, the rest base
+  const int big = extra * (base + 1);
+  return q0 < big ? base + 1 : base;
+}
+
 template <bool SHARED>
 static void dot_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, const MultiSpinor &W, const int *nact,
                      double *res, double *&part, const int *any, std::function<void(size_t)> reserve,
                      bool self = false) {
   const MShape a = shape(W, nact);
   const int nblk = W.nblk();
-  reserve((size_t)a.gx * 18 * W.cap);
-  for (int q0 = 0; q0 < nq; q0 += 8) {
-    const int K = std::min(8, nq - q0);
+  reserve((size_t)a.gx * (2 * MAXK + 2) * W.cap);
+  for (int q0 = 0, K; q0 < nq; q0 += K) {
+    K = chunk_len(nq, q0);
     // the last chunk also forms <W, W> when asked (res entry nq)
     const bool sf = self && q0 + K == nq;
     MPtr pk{};
@@ -459,7 +478,15 @@ static void dot_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, const
         case 5: L_(5); break;
         case 6: L_(6); break;
         case 7: L_(7); break;
-        default: L_(8); break;
+        case 8: L_(8); break;
+        case 9: L_(9); break;
+        case 10: L_(10); break;
+        case 11: L_(11); break;
+        case 12: L_(12); break;
+        case 13: L_(13); break;
+        case 14: L_(14); break;
+        case 15: L_(15); break;
+        default: L_(16); break;
       }
 #undef L_
     });
@@ -502,8 +529,8 @@ static void axpy_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, cons
                       double *part) {
   const MShape a = shape(W, nact);
   const int nblk = W.nblk();
-  for (int q0 = 0; q0 < nq; q0 += 8) {
-    const int K = std::min(8, nq - q0);
+  for (int q0 = 0, K; q0 < nq; q0 += K) {
+    K = chunk_len(nq, q0);
     const bool last = q0 + K >= nq;
     MPtr pk{};
     for (int k = 0; k < K; k++) pk.p[k] = vp[q0 + k];
@@ -522,7 +549,15 @@ static void axpy_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, cons
     case 5: L_(5, NR); break; \
     case 6: L_(6, NR); break; \
     case 7: L_(7, NR); break; \
-    default: L_(8, NR); break; \
+    case 8: L_(8, NR); break; \
+    case 9: L_(9, NR); break; \
+    case 10: L_(10, NR); break; \
+    case 11: L_(11, NR); break; \
+    case 12: L_(12, NR); break; \
+    case 13: L_(13, NR); break; \
+    case 14: L_(14, NR); break; \
+    case 15: L_(15, NR); break; \
+    default: L_(16, NR); break; \
   }
       if (n2 && last) {
         LK_(true)

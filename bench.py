@@ -571,6 +571,29 @@ def noise_section(nrhs=64, seq=3, graph=1, fp32_basis=False):
             "traces_timeslice0_gamma5": [float(r["traces"][0, 5].real), float(r["traces"][0, 5].imag)]}
 
 
+def multi_probe(spec):
+    """One 1 GiB copy (the counters' calibration) and four launches of the
+    63-slot multi-RHS hop at 24^3x48 (FP32 spinors, 12-real links, RHS block
+    8, block outermost) in launch form KERNEL with bz BZ."""
+    kernel, bz = (int(v) for v in spec.split(":"))
+    lattice.init((24, 24, 24, 48), BC)
+    co = np.zeros(64)
+    for mu in range(4):
+        co[2 + 4 * (0 + 4 * mu)] = 1.0
+        co[2 + 4 * (1 + 4 * mu)] = 1.0
+    diracops.gauge_random(1, 0.1, 2024, co)
+    L = _lib.lib()
+    ms = np.zeros(1)
+    _lib.check(L.qqcd_bench_copy(float(1 << 30), 3, _lib.dp(ms)))
+    copy_ms = ms[0]
+    _lib.check(L.qqcd_set_multi_tune(kernel, bz, 0))
+    _lib.check(L.qqcd_bench_dslash_multi(1, 12, 63, 63, 8, 0, 1, 3, _lib.dp(ms)))
+    _lib.check(L.qqcd_set_multi_tune(-1, -1, -1))
+    print(json.dumps({"probe": "multi-RHS hop, 24x24x24x48, 63 slots, FP32/12, nb 8", "kernel": kernel, "bz": bz,
+                      "copy_ms": round(copy_ms, 4), "hop_ms": round(ms[0], 4)}), flush=True)
+    lattice.finalize()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -585,6 +608,9 @@ def main():
     ap.add_argument("--noise-rhs", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true", help="batched solver without graph replays (profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="also time a 1 GiB plain copy kernel")
+    ap.add_argument("--multi-probe", metavar="KERNEL:BZ",
+                    help="only: a 1 GiB copy, then the 63-slot multi-RHS hop at 24^3x48 (FP32/12, RHS block 8) "
+                         "in the given launch form (hardware-counter runs)")
     ap.add_argument("--profile-solve", choices=sorted(PROFILE_SOLVES),
                     help="also run one 32^3x64 solve alone (for kernel traces)")
     ap.add_argument("--no-propagator", action="store_true",
@@ -598,6 +624,9 @@ def main():
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
               "(the multi-GPU decomposition is a design note only)", file=sys.stderr)
         sys.exit(2)
+    if args.multi_probe:
+        multi_probe(args.multi_probe)
+        return
 
     t_setup = time.perf_counter()
     lattice.init(N, BC)

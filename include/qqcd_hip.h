@@ -459,9 +459,14 @@ int qqcd_nn_apply_canonical(const double *r, double *z);
  * ||r|| = 1; `steps` Adam steps (lr, betas 0.9 / 0.999) on minibatches of
  * `batch` pairs, gradients from the device forward / adjoint / weight-gradient
  * kernels.  keep = 0 starts from the kappa^2-Jacobi initialisation with the
- * given channels.  loss[steps]: minibatch losses; stats[5] = {pairs, harvest
- * seconds, training seconds, mean loss before, mean loss after (over a fixed
- * sample of at most 512 pairs)}.  Leaves the
+ * given channels.  Every 150 Adam steps the network is checked by the solver
+ * itself (flexible GMRES-DR(40,16) to 1e-6 on a point source of the last
+ * configuration) and the weights needing the fewest Krylov steps, the
+ * starting ones included, are kept.  loss[steps]: minibatch losses; stats[7]
+ * = {pairs, harvest seconds, training seconds, mean loss before, mean loss
+ * after (over a fixed sample of at most 512 pairs), validation steps of the
+ * starting and of the kept weights (plus 1,000,000 for a validation solve
+ * that did not converge in 2000 steps)}.  Leaves the
  * last training configuration resident. */
 int qqcd_nn_train(int nconf, double beta, int nsweep, double kappa_lo, double kappa_hi, int nsrc, int m, int k,
                   double harvest_tol, int max_pairs, int nlayers, const int *channels, int keep, int steps, int batch,
@@ -579,10 +584,19 @@ int qqcd_bench_blas(int prec, int k, int op, int grid, int niter, double *ms);
 
 /* Override the launch shape of every multi-right-hand-side hop (-1 keeps
  * the caller's or the autotuner's choice): kernel 0 (hops one after the
- * other) or 1 (pipelined hops, one contiguous tile range per XCD, sites in
- * blocks of bz z-planes swept in t), bz, nt 0/1 (links and output streamed
- * past the L2).  Tuning aid; no reference counterpart. */
+ * other), 1 (pipelined hops, one contiguous tile range per XCD, sites in
+ * blocks of bz z-planes swept in t, persistent grid), 2 (as 1 with one
+ * workgroup per tile), 3 or 4 (FP32 spinors: two right-hand sides per lane,
+ * one workgroup per tile, two or one hops' loads in flight; FP64: as 1; 3 is
+ * the default), bz, nt 0/1 (links and output streamed past the L2).  Tuning
+ * aid; no reference counterpart. */
 int qqcd_set_multi_tune(int kernel, int bz, int nt);
+
+/* Dynamic LDS per workgroup of the one-workgroup-per-tile multi-RHS hop
+ * (kernel 2; -1 restores the default): unused by the kernel, it caps the
+ * workgroups resident on a CU and so the sites in flight on an XCD.  Tuning
+ * aid; no reference counterpart. */
+int qqcd_set_multi_lds(int bytes);
 
 /* Toggle the replay of repeated groups of Krylov steps from HIP graphs in
  * the single-right-hand-side solvers (default off: with a fresh basis per

@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from baylorlqcd_amd import diracops, disconloops, gauge, lattice, learned, quark
+from baylorlqcd_amd import _lib, diracops, disconloops, gauge, lattice, learned, quark
 
 pytestmark = pytest.mark.gpu
 
@@ -92,6 +92,37 @@ def test_hop_multi_partial_activity(lat):
             assert rel(out[r][odd], want[odd]) < 1e-13, r
         else:
             assert np.array_equal(out[r][odd], sentinel[r][odd]) or np.all(out[r][odd] == 0), r
+
+
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4])
+def test_hop_multi_kernels(lat, kernel):
+    # every launch form of the multi-RHS hop (hops in sequence; pipelined on a
+    # persistent grid; pipelined with one workgroup per tile; two slots per
+    # lane with two or one hops in flight), with retired slots (nact odd, so
+    # one slot pair is half retired), in both storage formats and both tile
+    # orders
+    n, bc, U, u, co = lat
+    nrhs, nact = 37, 29
+    psi = spinors(n, nrhs, 11)
+    one = [1.0] * 4
+    odd = O.parity_mask(n)
+    lib = _lib.lib()
+    _lib.check(lib.qqcd_set_multi_tune(kernel, -1, -1))
+    try:
+        for prec, recon, bar, nb, order in ((0, 18, 1e-13, 8, 0), (1, 12, 2e-6, 8, 0), (1, 18, 2e-6, 4, 1),
+                                            (1, 12, 2e-6, 2, 0)):
+            print(f"[multi] hop kernel={kernel} prec={prec} recon={recon} nb={nb} order={order}", flush=True)
+            sentinel = np.full_like(psi, 3.25 - 1.5j)
+            out = sentinel.copy()
+            disconloops.hop_multi_canonical(psi, out, 1, prec=prec, recon=recon, nact=nact, nb=nb, order=order)
+            for r in range(nrhs):
+                if r < nact:
+                    want = O.hop(n, U, bc, one, one, 0, 1, psi[r])
+                    assert rel(out[r][odd], want[odd]) < bar, (kernel, prec, r)
+                else:
+                    assert np.array_equal(out[r][odd], sentinel[r][odd]) or np.all(out[r][odd] == 0), (kernel, r)
+    finally:
+        _lib.check(lib.qqcd_set_multi_tune(-1, -1, -1))
 
 
 @pytest.mark.parametrize("kappa", [(0.150, 0.0), (0.156, 0.005)])
