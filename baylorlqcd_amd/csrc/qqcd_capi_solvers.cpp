@@ -292,9 +292,9 @@ extern "C" int qqcd_last_solve_stats(double *rel_res, int *iters, int *matvecs, 
   API_END
 }
 
-extern "C" int qqcd_set_step_graphs(int on) {
+extern "C" int qqcd_set_step_graphs(int mode) {
   API_BEGIN
-  set_step_graphs(on);
+  set_step_graphs(mode);
   API_END
 }
 

@@ -71,9 +71,14 @@ struct MShape {
 };
 
 // at most MAXK vectors per pass: a pass over q vectors reads (and, for the
-// multi-axpy, writes) the working field once, so chunks of 16 rather than 8
-// cut those re-reads; the slot sums go through LDS in groups of 18 values
-static constexpr int MAXK = 16;
+// multi-axpy, writes) the working field once, so longer chunks cut those
+// re-reads.  The basis fields are equal-sized allocations whose element e
+// lies at the same address bits in every field, so a pass streams 1 + q
+// lines that fall into the same L2 set; past 16 of them (q >= 15) the sets
+// thrash and the pass runs at half speed (measured: 24^3x48, 63 slots, FP16
+// basis: 15 or 16 vectors 7.5-8.5 ms, 13 or 14 vectors 2.6-3.7 ms).  The
+// slot sums go through LDS in groups of 18 values.
+static constexpr int MAXK = 14;
 struct MPtr {
   const void *p[MAXK];
 };
@@ -484,9 +489,7 @@ static void dot_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, const
         case 11: L_(11); break;
         case 12: L_(12); break;
         case 13: L_(13); break;
-        case 14: L_(14); break;
-        case 15: L_(15); break;
-        default: L_(16); break;
+        default: L_(14); break;
       }
 #undef L_
     });
@@ -555,9 +558,7 @@ static void axpy_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, cons
     case 11: L_(11, NR); break; \
     case 12: L_(12, NR); break; \
     case 13: L_(13, NR); break; \
-    case 14: L_(14, NR); break; \
-    case 15: L_(15, NR); break; \
-    default: L_(16, NR); break; \
+    default: L_(14, NR); break; \
   }
       if (n2 && last) {
         LK_(true)

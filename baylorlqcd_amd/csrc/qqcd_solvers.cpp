@@ -161,16 +161,20 @@ static void arnoldi_step_dev(Context &c, LinOp &A, std::vector<Spinor> &V, int j
 }
 
 // Groups of synchronisation-free steps replayed from HIP graphs: a group
-// [j0, j0 + n) that already ran on the same vectors within this solve is
-// launched as one graph, which removes the launch gaps between its ~13
-// kernels per step.  Only for operators whose buffers stay put between
-// applications (every LinOp here except a preconditioner's).  Off by
-// default: the inner solves of the mixed solver allocate a fresh basis per
-// outer step, so each graph is replayed about three times, and the captures
+// [j0, j0 + n) that already ran on the same vectors is launched as one
+// graph, which removes the launch gaps between its ~13 kernels per step.
+// Only for operators whose buffers stay put between applications (every
+// LinOp here except a preconditioner's).  A graph captured within one call
+// on a fresh basis is replayed about three times, and then the captures
 // cost more than the replays save (32^3x64 mixed FP16 solve: 1.32-1.35 s
-// with graphs, 1.25 s without).
-static int g_step_graphs = 0;
-void set_step_graphs(int on) { g_step_graphs = on ? 1 : 0; }
+// with graphs, 1.25 s without); so by default (mode 2) graphs are kept only
+// with a ProjWork, whose basis outlives the call and whose graphs serve
+// every later outer step of the mixed solver.
+static int g_step_graphs = 2;
+void set_step_graphs(int mode) {
+  if (mode < 0 || mode > 2) throw std::runtime_error("step graphs: mode 0 (never), 1 (always) or 2 (kept buffers)");
+  g_step_graphs = mode;
+}
 
 struct StepGraphs {
   struct Entry {
@@ -253,6 +257,15 @@ struct BasisGuard {
   std::vector<Spinor> &V;
   ~BasisGuard() { free_basis(V); }
 };
+
+ProjWork::~ProjWork() { reset(); }
+void ProjWork::reset() {
+  sg.reset();  // the graphs name the buffers: they go first
+  dh.reset();
+  work.reset();
+  free_basis(V);
+  m = 0;
+}
 
 // Deflated restart of GMRES-DR from a cycle of size j (> k):
 // builds Z = [[P_k; 0], srv_perp], replaces V[0..k] by V[0..j] Z, returns the
@@ -364,7 +377,7 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
   std::unique_ptr<DevHess> dh;
   if (!reorth) dh.reset(new DevHess(m));
   std::unique_ptr<StepGraphs> sg;
-  if (dh && !M && g_step_graphs) sg.reset(new StepGraphs);
+  if (dh && !M && g_step_graphs == 1) sg.reset(new StepGraphs);
   std::vector<zd> Zmat;
   SpinorBuf r(c.g, p);
   double rn = residual(c, A, b, x, r.s, st);
@@ -476,20 +489,48 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
 }
 
 SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, const Deflation &defl, double tol,
-                     int itermin, int maxiter, int reorth, ResHistory *hist) {
+                     int itermin, int maxiter, int reorth, ResHistory *hist, ProjWork *ws) {
   SolveStats st;
   const Prec p = A.prec, vp = A.basis();
   const int k = defl.k;
   if (m < 1) throw std::runtime_error("gmresproj: m must be >= 1");
-  std::vector<Spinor> V;
-  BasisGuard g1{V};
-  alloc_basis(c, V, m + 1, vp);
-  std::unique_ptr<SpinorBuf> work;
-  if (vp != p) work.reset(new SpinorBuf(c.g, p));
-  std::unique_ptr<DevHess> dh;
-  if (!reorth) dh.reset(new DevHess(m));
-  std::unique_ptr<StepGraphs> sg;
-  if (dh && g_step_graphs) sg.reset(new StepGraphs);
+  // the buffers: this call's own, or the workspace's (rebuilt when the shape
+  // or the precisions change)
+  std::vector<Spinor> Vown;
+  BasisGuard g1{Vown};
+  std::unique_ptr<SpinorBuf> work_own;
+  std::unique_ptr<DevHess> dh_own;
+  std::unique_ptr<StepGraphs> sg_own;
+  std::vector<Spinor> *Vp = &Vown;
+  SpinorBuf *work = nullptr;
+  DevHess *dh = nullptr;
+  StepGraphs *sg = nullptr;
+  if (ws) {
+    if (ws->m != m || ws->p != p || ws->vp != vp || (!reorth) != (ws->dh != nullptr)) {
+      ws->reset();
+      alloc_basis(c, ws->V, m + 1, vp);
+      if (vp != p) ws->work.reset(new SpinorBuf(c.g, p));
+      if (!reorth) ws->dh.reset(new DevHess(m));
+      ws->m = m;
+      ws->p = p;
+      ws->vp = vp;
+    }
+    if (ws->dh && g_step_graphs && !ws->sg) ws->sg.reset(new StepGraphs);
+    if (!g_step_graphs) ws->sg.reset();
+    Vp = &ws->V;
+    work = ws->work.get();
+    dh = ws->dh.get();
+    sg = ws->sg.get();
+  } else {
+    alloc_basis(c, Vown, m + 1, vp);
+    if (vp != p) work_own.reset(new SpinorBuf(c.g, p));
+    if (!reorth) dh_own.reset(new DevHess(m));
+    if (dh_own && g_step_graphs == 1) sg_own.reset(new StepGraphs);
+    work = work_own.get();
+    dh = dh_own.get();
+    sg = sg_own.get();
+  }
+  std::vector<Spinor> &V = *Vp;
   SpinorBuf r(c.g, p);
   double rn = residual(c, A, b, x, r.s, st);
   const double r0 = rn;
@@ -526,7 +567,7 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
       if (dh) {
         if (sg) {
           const int n = std::min(std::min(kCheck, m - j), std::max(1, maxiter - st.iters));
-          arnoldi_steps_dev(c, A, V, j, n, *dh, work ? &work->s : nullptr, sg.get());
+          arnoldi_steps_dev(c, A, V, j, n, *dh, work ? &work->s : nullptr, sg);
           st.matvecs += n;
           st.iters += n;
           j += n;
@@ -711,6 +752,8 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
   }
   Deflation local;
   Deflation &D = defl ? *defl : local;
+  // the later outer steps' GMRES-Proj keeps its basis and step graphs here
+  ProjWork pw;
   bool use_pre = p.precond != nullptr;
   if (D.k > 0 && (D.prec != p.basis || D.k != p.k)) D.clear();
   while (rn >= p.tol * bn && ms.outer < p.max_outer && ms.inner.iters < p.maxiter) {
@@ -740,7 +783,7 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
       st = gmresdr(c, inner, rl.s, el.s, p.m, p.k, itol, 0, std::min(budget, p.first_cycles * p.m), &D, p.reorth);
     else
       st = gmresproj(c, inner, rl.s, el.s, p.m - p.k, D, itol, 0, std::min(budget, p.later_cycles * (p.m - p.k)),
-                     p.reorth);
+                     p.reorth, nullptr, &pw);
     ms.inner.iters += st.iters;
     ms.inner.matvecs += st.matvecs;
     ms.inner.cycles += st.cycles;
@@ -786,6 +829,7 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
         ms.outer++;
         break;
       }
+      pw.reset();  // its graphs name the old operator's kernels and buffers
       Alo.reset(new SchurOp(c, op_prec, op_prec == PREC_DOUBLE ? 18 : p.op_recon, kappa));
       inner = make_inner();
       inner_tol = op_prec == PREC_SINGLE ? std::min(inner_tol, 1e-6) : inner_tol;

@@ -1,6 +1,7 @@
 // qqcd_solvers.h -- Krylov solvers on device-resident spinors.
 #pragma once
 #include <functional>
+#include <memory>
 #include <utility>
 #include <vector>
 
@@ -88,11 +89,34 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
 // least-squares residual norm ||c - Hbar y|| after that Arnoldi step).
 typedef std::vector<std::pair<int, double>> ResHistory;
 
+struct DevHess;
+struct StepGraphs;
+
+// Buffers of GMRES-Proj kept from one call to the next (the basis, the
+// device Hessenberg, the operator's work vector) together with the HIP
+// graphs of its groups of Arnoldi steps, which then replay in every later
+// call.  Owned by a caller that solves one system after another with the
+// same operator; reset() whenever that operator is replaced.
+struct ProjWork {
+  std::vector<Spinor> V;
+  std::unique_ptr<SpinorBuf> work;
+  std::unique_ptr<DevHess> dh;
+  std::unique_ptr<StepGraphs> sg;
+  int m = 0;
+  Prec p = PREC_DOUBLE, vp = PREC_DOUBLE;
+  ProjWork() = default;
+  ProjWork(const ProjWork &) = delete;
+  ProjWork &operator=(const ProjWork &) = delete;
+  ~ProjWork();
+  void reset();
+};
+
 // GMRES-Proj(m, k) with deflation space defl: alternates a projection over
 // (V_{k+1}, Hbar_k) with a GMRES(m) cycle.  x holds x0 on entry.  hist, if
-// given, receives one entry per Arnoldi step.
+// given, receives one entry per Arnoldi step.  ws, if given, holds the
+// buffers between calls.
 SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, const Deflation &defl, double tol,
-                     int itermin, int maxiter, int reorth = 1, ResHistory *hist = nullptr);
+                     int itermin, int maxiter, int reorth = 1, ResHistory *hist = nullptr, ProjWork *ws = nullptr);
 
 // BiCGStab from x (Frommer et al.; the reference's bicgstab).  Convergence
 // on the recursive residual, then confirmed on the true residual (restarting
@@ -120,8 +144,9 @@ namespace qqcd {
 // -> plain FP64 GMRES-DR), so the FP64 solver is the fallback and the FP64
 // true residual the judge of every result.
 // replay repeated groups of synchronisation-free Arnoldi steps from HIP
-// graphs (default off)
-void set_step_graphs(int on);
+// graphs: 0 never, 1 in every solve, 2 (the default) only where the basis is
+// kept between calls (ProjWork), so that a capture serves many solves
+void set_step_graphs(int mode);
 
 struct MixedParams {
   int m = 40, k = 16;

@@ -606,7 +606,8 @@ def main():
     ap.add_argument("--no-noise", action="store_true", help="skip the 24^3x48 batched noise-source section")
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--noise-rhs", type=int, default=64)
-    ap.add_argument("--no-graph", action="store_true", help="batched solver without graph replays (profiling)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="no HIP graph replays in the batched and the single-RHS solvers (kernel traces)")
     ap.add_argument("--copy-probe", action="store_true", help="also time a 1 GiB plain copy kernel")
     ap.add_argument("--multi-probe", metavar="KERNEL:BZ",
                     help="only: a 1 GiB copy, then the 63-slot multi-RHS hop at 24^3x48 (FP32/12, RHS block 8) "
@@ -627,6 +628,8 @@ def main():
     if args.multi_probe:
         multi_probe(args.multi_probe)
         return
+    if args.no_graph:
+        _lib.check(_lib.lib().qqcd_set_step_graphs(0))
 
     t_setup = time.perf_counter()
     lattice.init(N, BC)

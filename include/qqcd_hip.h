@@ -598,11 +598,13 @@ int qqcd_set_multi_tune(int kernel, int bz, int nt);
  * aid; no reference counterpart. */
 int qqcd_set_multi_lds(int bytes);
 
-/* Toggle the replay of repeated groups of Krylov steps from HIP graphs in
- * the single-right-hand-side solvers (default off: with a fresh basis per
- * inner solve the captures cost more than the replays save).  Tuning aid; no
- * reference counterpart. */
-int qqcd_set_step_graphs(int on);
+/* Replay of repeated groups of Krylov steps from HIP graphs in the
+ * single-right-hand-side solvers: 0 never, 1 in every solve (a graph on a
+ * fresh basis replays about three times, too few to repay its capture), 2
+ * (the default) only in the mixed solver's later outer steps, whose
+ * GMRES-Proj keeps its basis and graphs from one outer step to the next.
+ * Tuning aid; no reference counterpart. */
+int qqcd_set_step_graphs(int mode);
 
 /* Toggle the XCD-aware workgroup order of the Dslash (default on). */
 int qqcd_set_xcd_remap(int on);
