@@ -37,6 +37,28 @@ void MultiSchur::operator()(const MultiSpinor &in, MultiSpinor &out, const int *
   dslash_multi(c.g, U, tmp.m, out, 0, idag, e2, &in, nact, tune, c.stream);
 }
 
+bool MultiSchur::can_scale() const { return dslash_multi_can_scale(c.g, tmp.m.prec, tmp.m.nb, tune); }
+
+void MultiSchur::scaled(const MultiSpinor &in, MultiSpinor &out, const int *nact, const double *n2,
+                        MultiSpinor *side, double hs) {
+  const double k = kappa[0], mu = kappa[1];
+  Epilogue e1, e2;
+  if (mu < 1e-8) {
+    e2.a0 = 1.0 / (k * k);
+    e2.b0 = -1.0;
+  } else {
+    const double a = 2.0 * k * mu;
+    e1.b1 = -a;
+    e2.a0 = 1.0 / (k * k);
+    e2.a1 = 2.0 * mu / k;
+    e2.b0 = -1.0 / (1.0 + a * a);
+  }
+  // the operator is linear: the first hop runs on the unscaled input, the
+  // second hop's epilogue scales the whole result
+  dslash_multi(c.g, U, in, tmp.m, 1, idag, e1, nullptr, nact, tune, c.stream);
+  dslash_multi(c.g, U, tmp.m, out, 0, idag, e2, &in, nact, tune, c.stream, n2, side, hs);
+}
+
 namespace {
 
 // pinned host mirror of a device array
@@ -92,6 +114,11 @@ class Batch {
   // basis vector in FP32 (the operator's input)
   std::unique_ptr<MultiBuf> Wf, Qf;
   bool half() const { return p.basis == PREC_HALF; }
+  // FP16 basis with the two-slot operator: each step's normalisation is
+  // done by the next operator application (MultiSchur::scaled); hs: the
+  // basis scale
+  bool fused = false;
+  double hs = 0.0;
   // device state: ints [0] n_outer, [1] n_inner, [2] any, [3, 3+cap) flags
   DevBuf<int> di;
   DevBuf<double> h1, h2, n2a, n2b, nf, H, coef, coef2, n2;
@@ -178,6 +205,8 @@ Batch::Batch(Context &c_, const double kappa[2], MultiSpinor &B_, MultiSpinor &X
   if (half()) {
     Wf.reset(new MultiBuf(c.g, PREC_SINGLE, cap, nb));
     Qf.reset(new MultiBuf(c.g, PREC_SINGLE, cap, nb));
+    fused = p.fuse_norm && A32.can_scale();
+    hs = std::sqrt(12.0 * (double)c.g.stride);
   }
   mb.reserve(mb.need());
   clear_flags(d_flag(), d_any(), cap, c.stream);
@@ -204,6 +233,26 @@ void Batch::swap_slots(int a, int b, bool inner_fields) {
 // one Arnoldi step of every active inner slot; device-only (capturable)
 void Batch::arnoldi_step(int j) {
   int *nin = d_nin();
+  if (fused) {
+    // FP16 basis, normalisation folded into the operator: step j's input
+    // is the orthogonalised but unnormalised w of step j - 1 (r / ||r|| in
+    // Qf for step 0), its squared norm nf left by that step's column; the
+    // operator writes V[j] = hs w / ||w|| and A w / ||w||; the two FP32
+    // fields take turns as input and output
+    MultiSpinor &in = (j & 1) ? Wf->m : Qf->m;
+    MultiSpinor &w = (j & 1) ? Qf->m : Wf->m;
+    if (j == 0) A32(in, w, nin);
+    else A32.scaled(in, w, nin, nf.p, &V[j], hs);
+    mb.dot(j + 1, V.data(), w, nin, h1.p, nullptr, true);  // <V_q, w> and <w, w> in one pass
+    mb.axpy(j + 1, V.data(), h1.p, 0, -1.0, w, nin, n2a.p);
+    if (p.reorth > 0) {
+      arnoldi_flags(n2a.p, h1.p + (size_t)2 * (j + 1) * cap, d_flag(), d_any(), nin, cap, p.reorth == 2, c.stream);
+      mb.dot(j + 1, V.data(), w, nin, h2.p, d_any());
+      mb.axpy(j + 1, V.data(), h2.p, 0, -1.0, w, nin, n2b.p, d_flag(), d_any());
+    }
+    arnoldi_column(H.p, m + 1, j, h1.p, h2.p, d_flag(), n2a.p, n2b.p, nf.p, nin, cap, c.stream);
+    return;
+  }
   // w: the new vector, orthogonalised in place (FP32; FP16 basis: in Wf)
   MultiSpinor &w = half() ? Wf->m : V[j + 1];
   A32(half() ? Qf->m : V[j], w, nin);

@@ -18,6 +18,14 @@ struct MultiSchur {
   MultiSchur(Context &c_, Prec p, int recon, const double k[2], int cap, int nb, const MultiTune &t,
              int idag_ = 0);
   void operator()(const MultiSpinor &in, MultiSpinor &out, const int *nact);
+  // out = A in / ||in|| per slot (n2: ||in||^2 per slot in device memory, 0
+  // where it is not positive) and side = hs in / ||in|| as an FP16 field
+  // without norms, both in the second hop's epilogue: the normalisation of
+  // an Arnoldi step folded into the next operator application.  Needs
+  // can_scale().
+  void scaled(const MultiSpinor &in, MultiSpinor &out, const int *nact, const double *n2, MultiSpinor *side,
+              double hs);
+  bool can_scale() const;
 };
 
 // Batched GMRES-Proj with reliable updates.  Every right-hand side b_r is
@@ -46,6 +54,9 @@ struct BatchParams {
   // Krylov basis storage of the inner GMRES: FP32, or FP16 scaled unit
   // vectors (the operator then reads an FP32 copy of the newest one)
   Prec basis = PREC_HALF;
+  // FP16 basis: fold each Arnoldi step's normalisation into the next
+  // operator application where the launch form allows it
+  int fuse_norm = 1;
   MultiTune tune;
 };
 

@@ -213,6 +213,11 @@ struct DslashArgs {
   int order;
   int bz;
   int plane;  // sites of one (z, t) plane of a checkerboard: nx*ny/2
+  // two-slot multi-RHS kernel only: per-slot scale 1/sqrt(slot_n2[slot]) of
+  // the epilogue's result, and the FP16 side output side_hs x / sqrt(slot_n2)
+  const double *slot_n2;
+  void *side;
+  float side_hs;
 };
 
 // Pointers and strides a hop reads (scalars only, so that nothing of the
@@ -1002,7 +1007,7 @@ __device__ __forceinline__ void epilogue_f(const DslashArgs &a, const cx<float> 
 // slot q2 + 1 is active; a retired slot is computed but not written).
 template <typename L, int RECON, int SIG, bool NT, int D>
 __device__ __forceinline__ void dslash_site_pair(const DslashArgs &a, const HopIn &h, void *out, const void *xin,
-                                                 int i, int q2, int lognb, bool both) {
+                                                 void *side, int i, int q2, int lognb, int slot0, bool both) {
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
   const int r2 = 2 * i;
   const int x0 = r2 % nx;
@@ -1033,6 +1038,38 @@ __device__ __forceinline__ void dslash_site_pair(const DslashArgs &a, const HopI
   constexpr int AUXS = NT ? 2 : 0;
   const uint32_t srun = (uint32_t)h.sstride * 8u;
   const uint32_t oo = (uint32_t)io * 8u;
+  if (a.slot_n2) {
+    // the result is A x / ||x|| per slot; side = hs x / ||x|| in FP16
+    const double n0 = a.slot_n2[slot0], n1 = both ? a.slot_n2[slot0 + 1] : 0.0;
+    const float s0 = n0 > 0.0 ? (float)(1.0 / sqrt(n0)) : 0.f, s1 = n1 > 0.0 ? (float)(1.0 / sqrt(n1)) : 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      o0[k] = s0 * o0[k];
+      o1[k] = s1 * o1[k];
+    }
+    if (side) {
+      typedef f16 h4 __attribute__((ext_vector_type(4)));
+      const float f0 = a.side_hs * s0, f1 = a.side_hs * s1;
+      const uint32_t hrun = (uint32_t)h.sstride * 4u;
+      const __amdgpu_buffer_rsrc_t hrs = make_rsrc(side, 12 * hrun);
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        if (both) {
+          h4 v;
+          v.x = (f16)(f0 * xr[k].x);
+          v.y = (f16)(f0 * xr[k].y);
+          v.z = (f16)(f1 * xr[k].z);
+          v.w = (f16)(f1 * xr[k].w);
+          bst<h4, AUXS>(v, hrs, (uint32_t)io * 4u, k * hrun);
+        } else {
+          h2 v;
+          v.x = (f16)(f0 * xr[k].x);
+          v.y = (f16)(f0 * xr[k].y);
+          bst<h2, AUXS>(v, hrs, (uint32_t)io * 4u, k * hrun);
+        }
+      }
+    }
+  }
   const __amdgpu_buffer_rsrc_t ors = make_rsrc(out, 12 * srun);
   if (both) {
 #pragma unroll
@@ -1085,8 +1122,8 @@ __global__ void __launch_bounds__(256) k_dslash_multi_pair(const DslashArgs a, c
   typedef typename V2<float>::T SE;
   const long eo = (long)blk * m.blk_elems;
   const HopIn h{(const SE *)a.in + eo, nullptr, a.links, a.sstride, a.lstride};
-  dslash_site_pair<L, RECON, SIG, NT, D>(a, h, (SE *)a.out + eo, a.x ? (const SE *)a.x + eo : nullptr, i, q2,
-                                         m.lognb, r0 + 1 < nact);
+  dslash_site_pair<L, RECON, SIG, NT, D>(a, h, (SE *)a.out + eo, a.x ? (const SE *)a.x + eo : nullptr,
+                                         a.side ? (h2 *)a.side + eo : nullptr, i, q2, m.lognb, r0, r0 + 1 < nact);
 }
 
 HopCoef make_hopcoef(const Geom &g, int idag) {
@@ -1247,6 +1284,7 @@ static void launch_multi(const DslashArgs &a0, const MultiArgs &m0, int sig, con
         return;
       }
     }
+    if (a0.slot_n2 || a0.side) throw std::runtime_error("dslash_multi: the scaled epilogue needs the two-slot FP32 kernel");
     if (sig > 0) {
       if (t.nt) launch_multi_pipe<S, L, RECON, 1, true>(a, m, t.grid, max_tiles, t.lds, s);
       else launch_multi_pipe<S, L, RECON, 1, false>(a, m, t.grid, max_tiles, t.lds, s);
@@ -1256,6 +1294,7 @@ static void launch_multi(const DslashArgs &a0, const MultiArgs &m0, int sig, con
     }
     return;
   }
+  if (a0.slot_n2 || a0.side) throw std::runtime_error("dslash_multi: the scaled epilogue needs the two-slot FP32 kernel");
   if (sig > 0) launch_multi_sig<S, L, RECON, 1>(a0, m0, t.grid, s);
   else if (sig < 0) launch_multi_sig<S, L, RECON, -1>(a0, m0, t.grid, s);
   else launch_multi_sig<S, L, RECON, 0>(a0, m0, t.grid, s);
@@ -1375,13 +1414,36 @@ void set_multi_lds(int bytes) {
   g_multi_lds = bytes;
 }
 
-void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpinor &out, int par_out, int idag,
-                  const Epilogue &ep, const MultiSpinor *x, const int *d_nact, const MultiTune &t0, hipStream_t s) {
+static MultiTune multi_tune_of(const MultiTune &t0) {
   MultiTune t = t0;
   if (g_multi_kernel >= 0) t.kernel = g_multi_kernel;
   if (g_multi_bz >= 0) t.bz = g_multi_bz;
   if (g_multi_nt >= 0) t.nt = g_multi_nt;
   if (g_multi_lds >= 0) t.lds = g_multi_lds;
+  return t;
+}
+
+bool dslash_multi_can_scale(const Geom &g, Prec p, int nb, const MultiTune &t0) {
+  const MultiTune t = multi_tune_of(t0);
+  // the launch conditions of launch_multi's two-slot branch: projector form,
+  // FP32 spinors, nb >= 2, buffer descriptors of < 4 GiB (links: at most 9
+  // FP32 complex runs per (parity, mu) block)
+  const HopCoef h = make_hopcoef(g, 0);
+  bool proj = h.proj != 0;
+  if (proj) {
+    const bool s0 = h.fw_m[0] == 0.0;
+    for (int mu = 1; mu < 4; mu++)
+      if ((h.fw_m[mu] == 0.0) != s0) proj = false;
+  }
+  const double sst = (double)g.stride * nb;
+  return proj && p == PREC_SINGLE && nb >= 2 && (t.kernel == 3 || t.kernel == 4) && 12.0 * sst * 8 < 4294967295.0 &&
+         9.0 * g.stride * 8 < 4294967295.0;
+}
+
+void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpinor &out, int par_out, int idag,
+                  const Epilogue &ep, const MultiSpinor *x, const int *d_nact, const MultiTune &t0, hipStream_t s,
+                  const double *slot_n2, MultiSpinor *side, double side_hs) {
+  const MultiTune t = multi_tune_of(t0);
   if (in.prec != out.prec || (x && x->prec != out.prec))
     throw std::runtime_error("dslash_multi: spinor precisions differ");
   if (in.stride != g.stride || out.stride != g.stride || U.stride != g.stride)
@@ -1402,6 +1464,17 @@ void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpi
   a.has_x = x ? 1 : 0;
   const int proj = make_args(g, U, par_out, idag, ep, a);
   a.sstride = g.stride * in.nb;
+  a.slot_n2 = slot_n2;
+  a.side = nullptr;
+  a.side_hs = 0.f;
+  if (side) {
+    if (!slot_n2 || !x) throw std::runtime_error("dslash_multi: the FP16 side output needs slot norms and x");
+    if (side->prec != PREC_HALF || side->norm || side->nb != in.nb || side->cap != in.cap ||
+        side->stride != in.stride)
+      throw std::runtime_error("dslash_multi: the side output is an FP16 field without norms in the same layout");
+    a.side = side->data;
+    a.side_hs = (float)side_hs;
+  }
   MultiArgs m{};
   m.nact = d_nact;
   m.lognb = in.lognb;

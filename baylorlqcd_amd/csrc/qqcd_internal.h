@@ -158,9 +158,16 @@ void dslash(const Geom &g, const Gauge &U, const Spinor &in, Spinor &out, int pa
             const Epilogue &ep, const Spinor *x, hipStream_t s);
 
 // Hopping operator on the active slots [0, *d_nact) of a multi-RHS field
-// (d_nact in device memory); same epilogue as dslash.
+// (d_nact in device memory); same epilogue as dslash.  With slot_n2 (device,
+// one value per slot) the epilogue's result is scaled by 1/sqrt(slot_n2[r])
+// (0 where that is not positive), and with side (an FP16 field without
+// norms) the epilogue also writes side = side_hs x / sqrt(slot_n2[r]); both
+// need the two-slot FP32 kernel (dslash_multi_can_scale) and x.
 void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpinor &out, int par_out, int idag,
-                  const Epilogue &ep, const MultiSpinor *x, const int *d_nact, const MultiTune &t, hipStream_t s);
+                  const Epilogue &ep, const MultiSpinor *x, const int *d_nact, const MultiTune &t, hipStream_t s,
+                  const double *slot_n2 = nullptr, MultiSpinor *side = nullptr, double side_hs = 0.0);
+// whether dslash_multi on these fields takes the two-slot FP32 kernel
+bool dslash_multi_can_scale(const Geom &g, Prec p, int nb, const MultiTune &t);
 
 // ---- layout (qqcd_layout.hip) ----
 long ref_nvhalf(const Geom &g);

@@ -402,8 +402,10 @@ static int gx_of(long rows, int nb, int nblk) {
   return (int)std::max(1L, std::min(groups, (long)std::max(16, 2048 / nblk)));
 }
 
-// gx <= 2048 and at most 18 partial values per slot (8 complex dots and a norm)
-size_t MBlas::need() const { return (size_t)2048 * 18 * cap + 64; }
+// gx <= 2048 and at most 2 MAXK + 2 partial values per slot (MAXK complex
+// dots and a norm): sized once, before any capture, so that no later pass
+// moves the buffer a captured graph names
+size_t MBlas::need() const { return (size_t)2048 * (2 * MAXK + 2) * cap + 64; }
 
 static MShape shape(const MultiSpinor &W, const int *nact) {
   MShape a;

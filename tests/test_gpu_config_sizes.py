@@ -6,7 +6,11 @@
   the reunitarised 1 + 0.1 Gaussian field of the device generator.  Bars:
   FP64 1e-13 relative (max norm; 1e-12 with 12-real links, whose third row
   is rebuilt), FP32 2e-6 and FP16 5e-3 relative L2.
-* 24^3x48: the batched noise loop (8 Z4 sources, the first by
+* 24^3x48: the multi-RHS hop in its default launch form (two slots per
+  lane, one workgroup per tile) for 12 right-hand sides in blocks of 8 (the
+  second block half filled) against the oracle per right-hand side, FP32
+  with 12-real links (2e-6 relative L2) and FP64 (1e-13 relative, max norm);
+  the batched noise loop (8 Z4 sources, the first by
   GMRES-DR(60,20), the rest in one multi-RHS GMRES-Proj) on a beta = 6.0
   heatbath field at kappa 0.1565: every source reaches a true residual below
   1e-10 by the oracle's own full-lattice operator, and the 16 local trace
@@ -75,6 +79,33 @@ def test_hdbletm_32x32x32x64_twisted(lat32):
     got = diracops.schur_canonical(psi, kappa, 0, 0, 18)
     even = ~O.parity_mask(n)
     assert relmax(got[even], want[even]) < 1e-13
+
+
+def test_hop_multi_24x24x24x48():
+    n = (24, 24, 24, 48)
+    nrhs = 12
+    print(f"[config sizes] multi-RHS hop {n}, {nrhs} right-hand sides", flush=True)
+    lattice.init(n, BC)
+    try:
+        co = O.coact_array()
+        diracops.gauge_random(1, 0.1, 2024, co)
+        U = diracops.gauge_get_canonical(n)
+        V = int(np.prod(n))
+        rng = np.random.default_rng(4321)
+        psi = (rng.standard_normal((nrhs, V, 4, 3)) + 1j * rng.standard_normal((nrhs, V, 4, 3)))
+        odd = O.parity_mask(n)
+        for prec, recon in ((1, 12), (0, 18)):
+            print(f"[config sizes] multi-RHS hop prec {prec} recon {recon}", flush=True)
+            out = np.zeros_like(psi)
+            disconloops.hop_multi_canonical(psi, out, 1, prec=prec, recon=recon, nb=8, order=0)
+            for r in range(nrhs):
+                want = O.hop(n, U, BC, [1] * 4, [1] * 4, 0, 1, psi[r])[odd]
+                got = out[r][odd]
+                err = rel2(got, want) if prec == 1 else relmax(got, want)
+                assert err < (2e-6 if prec == 1 else 1e-13), (prec, r, err)
+            del out
+    finally:
+        lattice.finalize()
 
 
 def test_noise_loop_24x24x24x48_eight_sources():
