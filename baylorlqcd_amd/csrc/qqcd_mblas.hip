@@ -22,7 +22,12 @@ namespace qqcd {
 static constexpr int MB = 256;  // threads per workgroup
 
 template <typename T> struct E2;
+// ld: (re, im) in FP64; raw / ldr / cvt: the stored pair as loaded, and its
+// conversion, so that a pass can issue all its loads before the arithmetic
 template <> struct E2<double> {
+  typedef double raw __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ raw ldr(const void *p, long i) { return ((const raw *)p)[i]; }
+  static __device__ __forceinline__ double2 cvt(raw v) { return make_double2(v[0], v[1]); }
   static __device__ __forceinline__ double2 ld(const void *p, long i) { return ((const double2 *)p)[i]; }
   // stores (re, im) and returns the stored value
   static __device__ __forceinline__ double2 st(void *p, long i, double re, double im) {
@@ -31,6 +36,9 @@ template <> struct E2<double> {
   }
 };
 template <> struct E2<float> {
+  typedef float raw __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ raw ldr(const void *p, long i) { return ((const raw *)p)[i]; }
+  static __device__ __forceinline__ double2 cvt(raw v) { return make_double2(v[0], v[1]); }
   static __device__ __forceinline__ double2 ld(const void *p, long i) {
     const float2 v = ((const float2 *)p)[i];
     return make_double2(v.x, v.y);
@@ -47,6 +55,9 @@ template <> struct E2<float> {
 // stored values, and the passes that read such vectors scale by 1/hs
 template <> struct E2<f16> {
   typedef f16 h2 __attribute__((ext_vector_type(2)));
+  typedef h2 raw;
+  static __device__ __forceinline__ raw ldr(const void *p, long i) { return ((const h2 *)p)[i]; }
+  static __device__ __forceinline__ double2 cvt(raw v) { return make_double2((double)(float)v[0], (double)(float)v[1]); }
   static __device__ __forceinline__ double2 ld(const void *p, long i) {
     const h2 v = ((const h2 *)p)[i];
     return make_double2((double)(float)v[0], (double)(float)v[1]);
@@ -72,13 +83,8 @@ struct MShape {
 
 // at most MAXK vectors per pass: a pass over q vectors reads (and, for the
 // multi-axpy, writes) the working field once, so longer chunks cut those
-// re-reads.  The basis fields are equal-sized allocations whose element e
-// lies at the same address bits in every field, so a pass streams 1 + q
-// lines that fall into the same L2 set; past 16 of them (q >= 15) the sets
-// thrash and the pass runs at half speed (measured: 24^3x48, 63 slots, FP16
-// basis: 15 or 16 vectors 7.5-8.5 ms, 13 or 14 vectors 2.6-3.7 ms).  The
-// slot sums go through LDS in groups of 18 values.
-static constexpr int MAXK = 14;
+// re-reads.  The slot sums go through LDS in groups of 18 values.
+static constexpr int MAXK = 24;
 struct MPtr {
   const void *p[MAXK];
 };
@@ -115,6 +121,17 @@ __device__ __forceinline__ void slot_sum(double (&v)[NV], int lognb, double *par
   }
 }
 
+// The K basis values of one element pass through an empty asm statement (no
+// instruction is emitted) before they are used, so all K loads are issued
+// back to back and are in flight together: left to itself the compiler
+// interleaved loads and arithmetic for K >= 15 and those passes ran at half
+// speed (24^3x48, 63 slots: 7.2-8.2 ms for 15 or 16 FP16 vectors against
+// 3.6 ms for 14).
+template <typename R, int K> __device__ __forceinline__ void pin_loads(R (&x)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; k++) asm volatile("" : "+v"(x[k]));
+}
+
 // partials: part[((k * cap + slot) * gx + bx] for workgroup bx of the slot's block;
 // SELF: two more components, <W, W> (real, imaginary 0) after the K dots
 template <typename TV, typename TW, int K, bool SHARED, bool SELF>
@@ -133,9 +150,13 @@ __global__ void __launch_bounds__(MB) k_mdot(MShape a, MPtr v, const void *w, do
     for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
       const long e = base + (p << a.lognb) + rr;
       const double2 b = E2<TW>::ld(w, e);
+      typename E2<TV>::raw xr[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) xr[k] = E2<TV>::ldr(v.p[k], SHARED ? p : e);
+      pin_loads(xr);
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        const double2 x = E2<TV>::ld(v.p[k], SHARED ? p : e);
+        const double2 x = E2<TV>::cvt(xr[k]);
         acc[2 * k] += x.x * b.x + x.y * b.y;
         acc[2 * k + 1] += x.x * b.y - x.y * b.x;
       }
@@ -192,9 +213,13 @@ __global__ void __launch_bounds__(MB) k_maxpy(MShape a, MPtr v, const double *co
     for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
       const long e = base + (p << a.lognb) + rr;
       double2 b = E2<TW>::ld(w, e);
+      typename E2<TV>::raw xr[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) xr[k] = E2<TV>::ldr(v.p[k], SHARED ? p : e);
+      pin_loads(xr);
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        const double2 x = E2<TV>::ld(v.p[k], SHARED ? p : e);
+        const double2 x = E2<TV>::cvt(xr[k]);
         b.x += cr[k] * x.x - ci[k] * x.y;
         b.y += cr[k] * x.y + ci[k] * x.x;
       }
@@ -444,6 +469,14 @@ template <typename F> static void basis_types(Prec a, Prec b, F &&f) {
 
 #define T_OF(p) typename std::remove_pointer<decltype(p)>::type
 
+// f(std::integral_constant<int, K>()) for a run-time K in 1..KK
+template <int KK, typename F> static void with_k(int K, F &&f) {
+  if constexpr (KK >= 1) {
+    if (K == KK) f(std::integral_constant<int, KK>());
+    else with_k<KK - 1>(K, f);
+  }
+}
+
 // length of the chunk starting at q0 when nq vectors are split into the
 // fewest chunks of at most MAXK, as even as possible
 static int chunk_len(int nq, int q0) {
@@ -477,22 +510,7 @@ static void dot_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, const
     else                                                                                                          \
       hipLaunchKernelGGL((k_mdot<TV, TW, KK, SHARED, false>), grid, dim3(MB), 0, mb.stream, a, pk, W.data, part, any); \
   } while (0)
-      switch (K) {
-        case 1: L_(1); break;
-        case 2: L_(2); break;
-        case 3: L_(3); break;
-        case 4: L_(4); break;
-        case 5: L_(5); break;
-        case 6: L_(6); break;
-        case 7: L_(7); break;
-        case 8: L_(8); break;
-        case 9: L_(9); break;
-        case 10: L_(10); break;
-        case 11: L_(11); break;
-        case 12: L_(12); break;
-        case 13: L_(13); break;
-        default: L_(14); break;
-      }
+      with_k<MAXK>(K, [&](auto kk) { L_(decltype(kk)::value); });
 #undef L_
     });
     QQCD_HIP(hipGetLastError());
@@ -545,23 +563,7 @@ static void axpy_impl(MBlas &mb, int nq, const void *const *vp, Prec vprec, cons
       dim3 grid(a.gx, nblk);
 #define L_(KK, NR) \
   hipLaunchKernelGGL((k_maxpy<TV, TW, KK, SHARED, NR>), grid, dim3(MB), 0, mb.stream, a, pk, coef, q0g + q0, sign, W.data, part, flag, any)
-#define LK_(NR)            \
-  switch (K) {             \
-    case 1: L_(1, NR); break; \
-    case 2: L_(2, NR); break; \
-    case 3: L_(3, NR); break; \
-    case 4: L_(4, NR); break; \
-    case 5: L_(5, NR); break; \
-    case 6: L_(6, NR); break; \
-    case 7: L_(7, NR); break; \
-    case 8: L_(8, NR); break; \
-    case 9: L_(9, NR); break; \
-    case 10: L_(10, NR); break; \
-    case 11: L_(11, NR); break; \
-    case 12: L_(12, NR); break; \
-    case 13: L_(13, NR); break; \
-    default: L_(14, NR); break; \
-  }
+#define LK_(NR) with_k<MAXK>(K, [&](auto kk) { L_(decltype(kk)::value, NR); });
       if (n2 && last) {
         LK_(true)
       } else {
