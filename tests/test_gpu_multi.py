@@ -108,6 +108,9 @@ def test_hop_multi_kernels(lat, kernel):
     odd = O.parity_mask(n)
     lib = _lib.lib()
     _lib.check(lib.qqcd_set_multi_tune(kernel, -1, -1))
+    # form 2 also with its residency cap (dynamic LDS: two workgroups per CU)
+    if kernel == 2:
+        _lib.check(lib.qqcd_set_multi_lds(57344))
     try:
         for prec, recon, bar, nb, order in ((0, 18, 1e-13, 8, 0), (1, 12, 2e-6, 8, 0), (1, 18, 2e-6, 4, 1),
                                             (1, 12, 2e-6, 2, 0)):
@@ -123,6 +126,18 @@ def test_hop_multi_kernels(lat, kernel):
                     assert np.array_equal(out[r][odd], sentinel[r][odd]) or np.all(out[r][odd] == 0), (kernel, r)
     finally:
         _lib.check(lib.qqcd_set_multi_tune(-1, -1, -1))
+        _lib.check(lib.qqcd_set_multi_lds(-1))
+
+
+def test_tuning_knobs_reject_bad_values(lat):
+    lib = _lib.lib()
+    assert lib.qqcd_set_multi_tune(5, -1, -1) != 0
+    assert lib.qqcd_set_multi_lds(1 << 30) != 0
+    assert lib.qqcd_set_step_graphs(3) != 0
+    # the defaults are left in place
+    _lib.check(lib.qqcd_set_multi_tune(-1, -1, -1))
+    _lib.check(lib.qqcd_set_multi_lds(-1))
+    _lib.check(lib.qqcd_set_step_graphs(2))
 
 
 @pytest.mark.parametrize("kappa", [(0.150, 0.0), (0.156, 0.005)])
