@@ -119,9 +119,13 @@ class Batch {
   // basis scale
   bool fused = false;
   double hs = 0.0;
+  // two-step blocks (BatchParams::sstep): A^2 v_j in U2f, the four scalars
+  // of the update pass and the per-slot coefficients of the next vector
+  bool paired = false;
+  std::unique_ptr<MultiBuf> U2f;
   // device state: ints [0] n_outer, [1] n_inner, [2] any, [3, 3+cap) flags
   DevBuf<int> di;
-  DevBuf<double> h1, h2, n2a, n2b, nf, H, coef, coef2, n2;
+  DevBuf<double> h1, h2, n2a, n2b, nf, H, coef, coef2, n2, ssc, sbc;
   HostBuf<double> hH, hc, hcoef, hcoef2, hn2;
   std::vector<double> bn, rn, tgt;  // per slot: ||b||, current ||r||, inner target
   BatchStats st;
@@ -147,6 +151,7 @@ class Batch {
   }
   void swap_slots(int a, int b, bool inner_fields);
   void arnoldi_step(int j);
+  void arnoldi_block(int j);
   void arnoldi_cycle();
   void inner();
   void outer_residual();
@@ -181,6 +186,8 @@ Batch::Batch(Context &c_, const double kappa[2], MultiSpinor &B_, MultiSpinor &X
       coef((size_t)2 * (p_.m + 1) * B_.cap),
       coef2((size_t)2 * (p_.m + 1) * B_.cap),
       n2(B_.cap),
+      ssc((size_t)4 * B_.cap),
+      sbc((size_t)4 * B_.cap),
       hH((size_t)2 * p_.m * (p_.m + 1) * B_.cap),
       hc((size_t)2 * (p_.m + 1) * B_.cap),
       hcoef((size_t)2 * (p_.m + 1) * B_.cap),
@@ -207,6 +214,8 @@ Batch::Batch(Context &c_, const double kappa[2], MultiSpinor &B_, MultiSpinor &X
     Qf.reset(new MultiBuf(c.g, PREC_SINGLE, cap, nb));
     fused = p.fuse_norm && A32.can_scale();
     hs = std::sqrt(12.0 * (double)c.g.stride);
+    paired = fused && p.sstep == 2 && p.reorth == 0 && m % 2 == 0;
+    if (paired) U2f.reset(new MultiBuf(c.g, PREC_SINGLE, cap, nb));
   }
   mb.reserve(mb.need());
   clear_flags(d_flag(), d_any(), cap, c.stream);
@@ -272,18 +281,44 @@ void Batch::arnoldi_step(int j) {
   else mb.scale_inv_norm(w, w, nf.p, nin);
 }
 
+// Arnoldi steps j and j + 1 (j even) of every active inner slot in one pair:
+// u1 = A v_j (the operator also normalises v_j and writes it to V[j]), u2 =
+// A u1, the dots of both against V_0..V_j in one pass, both updates
+// u1' = u1 - V g1, u2' = u2 - V g2 in another; then v_{j+1} = u1' / nu and
+// the part of A v_{j+1} orthogonal to v_0..v_{j+1}, (u2' - alpha u1') / nu,
+// which the next pair's operator normalises.  Half the passes over the basis
+// of two single steps; device-only (capturable).
+void Batch::arnoldi_block(int j) {
+  int *nin = d_nin();
+  MultiSpinor &rin = Qf->m, &u1 = Wf->m, &u2 = U2f->m;
+  if (j == 0) A32(rin, u1, nin);
+  else A32.scaled(rin, u1, nin, nf.p, &V[j], hs);
+  A32(u1, u2, nin);
+  mb.dot2(j + 1, V.data(), u1, u2, nin, h1.p, h2.p);
+  mb.axpy2(j + 1, V.data(), h1.p, h2.p, u1, u2, nin, ssc.p);
+  sstep_columns(H.p, m + 1, j, h1.p, h2.p, ssc.p, sbc.p, nin, cap, hs, c.stream);
+  mb.next2(u1, u2, rin, V[j + 1], sbc.p, nin, nf.p);
+  set_subdiag(H.p, m + 1, j + 1, nf.p, nin, cap, c.stream);
+}
+
 void Batch::arnoldi_cycle() {
+  auto run_steps = [&]() {
+    if (paired)
+      for (int j = 0; j < m; j += 2) arnoldi_block(j);
+    else
+      for (int j = 0; j < m; j++) arnoldi_step(j);
+  };
   if (p.use_graph && have_graph) {
     QQCD_HIP(hipGraphLaunch(graph.e, c.stream));
     st.graph_replays++;
   } else {
-    for (int j = 0; j < m; j++) arnoldi_step(j);
+    run_steps();
     if (p.use_graph) {
       // every kernel has run once (static launch shapes are set), capture now
       QQCD_HIP(hipStreamSynchronize(c.stream));
       QQCD_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
       try {
-        for (int j = 0; j < m; j++) arnoldi_step(j);
+        run_steps();
       } catch (...) {
         hipGraph_t g = nullptr;
         hipStreamEndCapture(c.stream, &g);

@@ -57,6 +57,12 @@ struct BatchParams {
   // FP16 basis: fold each Arnoldi step's normalisation into the next
   // operator application where the launch form allows it
   int fuse_norm = 1;
+  // Arnoldi steps per Gram-Schmidt pass over the basis: 2 runs the steps in
+  // pairs (u1 = A v_j, u2 = A u1, one pass over V for both sets of dots and
+  // one for both updates, column j + 1 from the Arnoldi relation) where the
+  // FP16 basis, the normalising operator, one Gram-Schmidt pass and an even
+  // m allow it; 1 one step at a time
+  int sstep = 2;
   MultiTune tune;
 };
 

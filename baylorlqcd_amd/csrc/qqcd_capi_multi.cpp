@@ -362,6 +362,13 @@ extern "C" int qqcd_loopsum(int ksub, const double *xe, const double *xo, const 
   API_END
 }
 
+extern "C" int qqcd_batch_set_steps(int steps) {
+  API_BEGIN
+  if (steps != 1 && steps != 2) throw std::runtime_error("Arnoldi steps per Gram-Schmidt pass: 1 or 2");
+  g_bp.sstep = steps;
+  API_END
+}
+
 extern "C" int qqcd_batch_set(int reorth, int use_graph, double inner_tol, int inner_cycles, int max_outer,
                               int op_prec, int op_recon, int basis_prec) {
   API_BEGIN

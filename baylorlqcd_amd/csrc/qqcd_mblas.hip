@@ -85,6 +85,8 @@ struct MShape {
 // multi-axpy, writes) the working field once, so longer chunks cut those
 // re-reads.  The slot sums go through LDS in groups of 18 values.
 static constexpr int MAXK = 24;
+// the two-vector passes of the two-step block: at most MAXK2 basis fields
+static constexpr int MAXK2 = 24;
 struct MPtr {
   const void *p[MAXK];
 };
@@ -92,8 +94,8 @@ struct MPtr {
 // Sum NV per-lane values over the lanes of each slot in the workgroup;
 // thread (k, r) with k < NV, r < nb then writes its total to
 // part[((k * cap + slot0 + r) * gx + bx].
-template <int NV>
-__device__ __forceinline__ void slot_sum(double (&v)[NV], int lognb, double *part, int cap, int slot0, int gx) {
+template <int NV, typename T>
+__device__ __forceinline__ void slot_sum(T (&v)[NV], int lognb, double *part, int cap, int slot0, int gx) {
   constexpr int G = NV < 18 ? NV : 18;  // values per round through LDS
   __shared__ double sm[MB / 64][G][64];
   const int nb = 1 << lognb;
@@ -103,7 +105,7 @@ __device__ __forceinline__ void slot_sum(double (&v)[NV], int lognb, double *par
     if (k0 > 0) __syncthreads();  // the previous round's reads are done
 #pragma unroll
     for (int k = k0; k < (k0 + G < NV ? k0 + G : NV); k++) {
-      double s = v[k];
+      double s = (double)v[k];
       for (int o = 32; o >= nb; o >>= 1) s += __shfl_xor(s, o, 64);
       if (lane < nb) sm[w][k - k0][lane] = s;
     }
@@ -374,6 +376,193 @@ __global__ void k_arnoldi_column(double *H, int ldh, int j, const double *h1, co
   }
 }
 
+// ---- two-step Arnoldi block (FP16 basis V, FP32 W1 = A v_j, W2 = A W1) ----
+// Per-lane sums in FP32 (the basis carries 11 bits), across lanes in FP64.
+
+// part[((c * cap + slot) * gx + bx]: c < 2K <V_k, W1>, then 2K + c <V_k, W2>
+template <int K>
+__global__ void __launch_bounds__(MB) k_mdot2(MShape a, MPtr v, const void *w1, const void *w2, double *part) {
+  const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
+  if ((blk << a.lognb) >= nact) return;
+  const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
+  constexpr int NV = 4 * K;
+  float af[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) af[k] = 0.f;
+  if (slot < nact) {
+    const int per = MB >> a.lognb;
+    const long base = (long)blk * a.blk_elems;
+    for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
+      const long e = base + (p << a.lognb) + rr;
+      const typename E2<float>::raw b1 = E2<float>::ldr(w1, e), b2 = E2<float>::ldr(w2, e);
+      typename E2<f16>::raw xr[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) xr[k] = E2<f16>::ldr(v.p[k], e);
+      pin_loads(xr);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const float xx = (float)xr[k][0], xy = (float)xr[k][1];
+        af[2 * k] += xx * b1[0] + xy * b1[1];
+        af[2 * k + 1] += xx * b1[1] - xy * b1[0];
+        af[2 * K + 2 * k] += xx * b2[0] + xy * b2[1];
+        af[2 * K + 2 * k + 1] += xx * b2[1] - xy * b2[0];
+      }
+    }
+  }
+  const float hinv = (float)a.hinv;
+#pragma unroll
+  for (int k = 0; k < NV; k++) af[k] *= hinv;
+  slot_sum<NV>(af, a.lognb, part, a.cap, blk << a.lognb, a.gx);
+}
+
+// W1 -= sum_k g1_k V_k, W2 -= sum_k g2_k V_k (coefficients g[2 ((q0 + k) cap + slot)]);
+// LAST: partials of ||W1||^2, Re <W1, W2>, Im <W1, W2>, ||W2||^2 of the stored values
+template <int K, bool LAST>
+__global__ void __launch_bounds__(MB) k_maxpy2(MShape a, MPtr v, const double *g1, const double *g2, int q0, void *w1,
+                                               void *w2, double *part) {
+  const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
+  if ((blk << a.lognb) >= nact) return;
+  const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  if (slot < nact) {
+    float c1r[K], c1i[K], c2r[K], c2i[K];
+    const double f = a.hinv;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const long o = 2 * ((long)(q0 + k) * a.cap + slot);
+      c1r[k] = (float)(f * g1[o]);
+      c1i[k] = (float)(f * g1[o + 1]);
+      c2r[k] = (float)(f * g2[o]);
+      c2i[k] = (float)(f * g2[o + 1]);
+    }
+    const int per = MB >> a.lognb;
+    const long base = (long)blk * a.blk_elems;
+    typedef typename E2<float>::raw f2;
+    for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
+      const long e = base + (p << a.lognb) + rr;
+      f2 b1 = E2<float>::ldr(w1, e), b2 = E2<float>::ldr(w2, e);
+      typename E2<f16>::raw xr[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) xr[k] = E2<f16>::ldr(v.p[k], e);
+      pin_loads(xr);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const float xx = (float)xr[k][0], xy = (float)xr[k][1];
+        b1[0] -= c1r[k] * xx - c1i[k] * xy;
+        b1[1] -= c1r[k] * xy + c1i[k] * xx;
+        b2[0] -= c2r[k] * xx - c2i[k] * xy;
+        b2[1] -= c2r[k] * xy + c2i[k] * xx;
+      }
+      ((f2 *)w1)[e] = b1;
+      ((f2 *)w2)[e] = b2;
+      if (LAST) {
+        acc[0] += b1[0] * b1[0] + b1[1] * b1[1];
+        acc[1] += b1[0] * b2[0] + b1[1] * b2[1];
+        acc[2] += b1[0] * b2[1] - b1[1] * b2[0];
+        acc[3] += b2[0] * b2[0] + b2[1] * b2[1];
+      }
+    }
+  }
+  if (LAST) slot_sum<4>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
+}
+
+// R = b0 W2 + (b1 + i b2) W1 in FP32, H = b3 W1 in FP16 (b per slot, 4 doubles),
+// partials of ||R||^2 of the stored values
+__global__ void __launch_bounds__(MB) k_next2(MShape a, const void *w1, const void *w2, void *rout, void *hout,
+                                              const double *b, double *part) {
+  const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
+  if ((blk << a.lognb) >= nact) return;
+  const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
+  double acc[1] = {0.0};
+  if (slot < nact) {
+    const float s2 = (float)b[4 * slot], s1r = (float)b[4 * slot + 1], s1i = (float)b[4 * slot + 2],
+                sv = (float)b[4 * slot + 3];
+    const int per = MB >> a.lognb;
+    const long base = (long)blk * a.blk_elems;
+    typedef typename E2<float>::raw f2;
+    typedef typename E2<f16>::raw h2;
+    float nr = 0.f;
+    for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
+      const long e = base + (p << a.lognb) + rr;
+      const f2 u1 = E2<float>::ldr(w1, e), u2 = E2<float>::ldr(w2, e);
+      f2 r;
+      r[0] = s2 * u2[0] + (s1r * u1[0] - s1i * u1[1]);
+      r[1] = s2 * u2[1] + (s1r * u1[1] + s1i * u1[0]);
+      ((f2 *)rout)[e] = r;
+      h2 hv;
+      hv[0] = (f16)(sv * u1[0]);
+      hv[1] = (f16)(sv * u1[1]);
+      ((h2 *)hout)[e] = hv;
+      nr += r[0] * r[0] + r[1] * r[1];
+    }
+    acc[0] = (double)nr;
+  }
+  slot_sum<1>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
+}
+
+// One thread per slot: column j of H from g1 and ||W1'||, column j+1 from the
+// Arnoldi relation A v_{j+1} = (A^2 v_j - V_{j+1} H_j g1) / nu (nu = ||W1'||),
+// and the coefficients of R = (W2' - alpha W1') / nu, alpha = <W1', W2'> / nu^2,
+// with W1' = W1 - V g1, W2' = W2 - V g2 as axpy2 left them (so R is the part
+// of A v_{j+1} orthogonal to v_0 .. v_{j+1}); H[j+2, j+1] = ||R|| follows
+// from next2 (set_subdiag).
+__global__ void k_sstep_columns(double *H, int ldh, int j, const double *g1, const double *g2, const double *sc,
+                                double *b, const int *nact, int cap, double hs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= cap || r >= *nact) return;
+#define HH(q, c) H[2 * (((long)(c) * ldh + (q)) * cap + r)]
+#define HI(q, c) H[2 * (((long)(c) * ldh + (q)) * cap + r) + 1]
+  const double n1 = sc[r];
+  const double nu = n1 > 0.0 ? sqrt(n1) : 0.0;
+  for (int q = 0; q <= j; q++) {
+    HH(q, j) = g1[2 * ((long)q * cap + r)];
+    HI(q, j) = g1[2 * ((long)q * cap + r) + 1];
+  }
+  HH(j + 1, j) = nu;
+  HI(j + 1, j) = 0.0;
+  if (nu > 0.0) {
+    const double inv = 1.0 / nu;
+    for (int k = 0; k <= j + 1; k++) {
+      // c_k = sum_i H[k, i] g1_i over the columns i >= k - 1 where H[k, i] can be nonzero
+      double cr = 0.0, ci = 0.0;
+      for (int i = k > 0 ? k - 1 : 0; i <= j; i++) {
+        const double hr = HH(k, i), hi = HI(k, i);
+        const double gr = g1[2 * ((long)i * cap + r)], gi = g1[2 * ((long)i * cap + r) + 1];
+        cr += hr * gr - hi * gi;
+        ci += hr * gi + hi * gr;
+      }
+      if (k <= j) {
+        HH(k, j + 1) = (g2[2 * ((long)k * cap + r)] - cr) * inv;
+        HI(k, j + 1) = (g2[2 * ((long)k * cap + r) + 1] - ci) * inv;
+      } else {
+        HH(k, j + 1) = (sc[cap + r] * inv - cr) * inv;
+        HI(k, j + 1) = (sc[2 * cap + r] * inv - ci) * inv;
+      }
+    }
+    const double ar = sc[cap + r] * inv * inv, ai = sc[2 * cap + r] * inv * inv;
+    b[4 * r] = inv;
+    b[4 * r + 1] = -ar * inv;
+    b[4 * r + 2] = -ai * inv;
+    b[4 * r + 3] = hs * inv;
+  } else {
+    for (int k = 0; k <= j + 1; k++) {
+      HH(k, j + 1) = 0.0;
+      HI(k, j + 1) = 0.0;
+    }
+    b[4 * r] = b[4 * r + 1] = b[4 * r + 2] = b[4 * r + 3] = 0.0;
+  }
+#undef HH
+#undef HI
+}
+
+__global__ void k_set_subdiag(double *H, int ldh, int j, const double *n2, const int *nact, int cap) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= cap || r >= *nact) return;
+  const double n = n2[r];
+  H[2 * (((long)j * ldh + j + 1) * cap + r)] = n > 0.0 ? sqrt(n) : 0.0;
+  H[2 * (((long)j * ldh + j + 1) * cap + r) + 1] = 0.0;
+}
+
 // ===================================================================== //
 
 static size_t esize(Prec p) { return p == PREC_DOUBLE ? 16 : p == PREC_SINGLE ? 8 : 4; }
@@ -427,10 +616,11 @@ static int gx_of(long rows, int nb, int nblk) {
   return (int)std::max(1L, std::min(groups, (long)std::max(16, 2048 / nblk)));
 }
 
-// gx <= 2048 and at most 2 MAXK + 2 partial values per slot (MAXK complex
-// dots and a norm): sized once, before any capture, so that no later pass
-// moves the buffer a captured graph names
-size_t MBlas::need() const { return (size_t)2048 * (2 * MAXK + 2) * cap + 64; }
+// gx <= 2048 and at most max(2 MAXK + 2, 4 MAXK2) partial values per slot
+// (MAXK complex dots and a norm, or MAXK2 complex dots with each of two
+// fields): sized once, before any capture, so that no later pass moves the
+// buffer a captured graph names
+size_t MBlas::need() const { return (size_t)2048 * std::max(2 * MAXK + 2, 4 * MAXK2) * cap + 64; }
 
 static MShape shape(const MultiSpinor &W, const int *nact) {
   MShape a;
@@ -479,8 +669,8 @@ template <int KK, typename F> static void with_k(int K, F &&f) {
 
 // length of the chunk starting at q0 when nq vectors are split into the
 // fewest chunks of at most MAXK, as even as possible
-static int chunk_len(int nq, int q0) {
-  const int n = (nq + MAXK - 1) / MAXK, base = nq / n, extra = nq % n;
+static int chunk_len(int nq, int q0, int maxk = MAXK) {
+  const int n = (nq + maxk - 1) / maxk, base = nq / n, extra = nq % n;
   // chunks 0 .. extra-1 hold base + 1 vectors, the rest base
   const int big = extra * (base + 1);
   return q0 < big ? base + 1 : base;
@@ -680,6 +870,96 @@ void MBlas::scale_inv_norm2(const MultiSpinor &X, MultiSpinor &Q, MultiSpinor &H
   QQCD_HIP(hipGetLastError());
 }
 
+// ---- two-step Arnoldi block ----
+static void same_layout2(const MultiSpinor &A, const MultiSpinor &B, const char *who) {
+  if (A.nb != B.nb || A.cap != B.cap || A.stride != B.stride) throw std::runtime_error(std::string(who) + ": layouts differ");
+}
+static void basis_fields(int nq, const MultiSpinor *V, const MultiSpinor &W, std::vector<const void *> &vp,
+                         const char *who) {
+  vp.resize(nq);
+  for (int q = 0; q < nq; q++) {
+    if (!basis16(V[q])) throw std::runtime_error(std::string(who) + ": V must be FP16 basis fields");
+    same_layout2(V[q], W, who);
+    vp[q] = V[q].data;
+  }
+}
+
+void MBlas::dot2(int nq, const MultiSpinor *V, const MultiSpinor &W1, const MultiSpinor &W2, const int *nact,
+                 double *g1, double *g2) {
+  if (W1.prec != PREC_SINGLE || W2.prec != PREC_SINGLE) throw std::runtime_error("MBlas::dot2: FP32 W1, W2");
+  same_layout2(W1, W2, "MBlas::dot2");
+  std::vector<const void *> vp;
+  basis_fields(nq, V, W1, vp, "MBlas::dot2");
+  const MShape a = shape(W1, nact);
+  reserve((size_t)a.gx * 4 * MAXK2 * W1.cap);
+  for (int q0 = 0, K; q0 < nq; q0 += K) {
+    K = chunk_len(nq, q0, MAXK2);
+    MPtr pk{};
+    for (int k = 0; k < K; k++) pk.p[k] = vp[q0 + k];
+    dim3 grid(a.gx, W1.nblk());
+    with_k<MAXK2>(K, [&](auto kk) {
+      hipLaunchKernelGGL((k_mdot2<decltype(kk)::value>), grid, dim3(MB), 0, stream, a, pk, W1.data, W2.data, part);
+    });
+    QQCD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mfinal, dim3(W1.cap, 2 * K), dim3(64), 0, stream, 2 * K, W1.cap, a.gx, nact,
+                       (const double *)part, g1, q0, 1);
+    hipLaunchKernelGGL(k_mfinal, dim3(W1.cap, 2 * K), dim3(64), 0, stream, 2 * K, W1.cap, a.gx, nact,
+                       (const double *)part + (size_t)2 * K * W1.cap * a.gx, g2, q0, 1);
+    QQCD_HIP(hipGetLastError());
+  }
+}
+
+void MBlas::axpy2(int nq, const MultiSpinor *V, const double *g1, const double *g2, MultiSpinor &W1,
+                  MultiSpinor &W2, const int *nact, double *sc) {
+  if (W1.prec != PREC_SINGLE || W2.prec != PREC_SINGLE) throw std::runtime_error("MBlas::axpy2: FP32 W1, W2");
+  if (W1.data == W2.data) throw std::runtime_error("MBlas::axpy2: W1 and W2 alias");
+  same_layout2(W1, W2, "MBlas::axpy2");
+  if (nq < 1) throw std::runtime_error("MBlas::axpy2: at least one basis field");
+  std::vector<const void *> vp;
+  basis_fields(nq, V, W1, vp, "MBlas::axpy2");
+  const MShape a = shape(W1, nact);
+  reserve((size_t)a.gx * 4 * W1.cap);
+  for (int q0 = 0, K; q0 < nq; q0 += K) {
+    K = chunk_len(nq, q0, MAXK2);
+    const bool last = q0 + K >= nq;
+    MPtr pk{};
+    for (int k = 0; k < K; k++) pk.p[k] = vp[q0 + k];
+    dim3 grid(a.gx, W1.nblk());
+    with_k<MAXK2>(K, [&](auto kk) {
+      constexpr int KK = decltype(kk)::value;
+      if (last)
+        hipLaunchKernelGGL((k_maxpy2<KK, true>), grid, dim3(MB), 0, stream, a, pk, g1, g2, q0, W1.data, W2.data,
+                           part);
+      else
+        hipLaunchKernelGGL((k_maxpy2<KK, false>), grid, dim3(MB), 0, stream, a, pk, g1, g2, q0, W1.data, W2.data,
+                           part);
+    });
+    QQCD_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_mfinal, dim3(W1.cap, 4), dim3(64), 0, stream, 4, W1.cap, a.gx, nact, (const double *)part, sc,
+                     0, 0);
+  QQCD_HIP(hipGetLastError());
+}
+
+void MBlas::next2(const MultiSpinor &W1, const MultiSpinor &W2, MultiSpinor &R, MultiSpinor &H, const double *b,
+                  const int *nact, double *nr) {
+  if (W1.prec != PREC_SINGLE || W2.prec != PREC_SINGLE || R.prec != PREC_SINGLE)
+    throw std::runtime_error("MBlas::next2: FP32 W1, W2, R");
+  if (!basis16(H)) throw std::runtime_error("MBlas::next2: H must be an FP16 basis field");
+  same_layout2(W1, W2, "MBlas::next2");
+  same_layout2(W1, R, "MBlas::next2");
+  same_layout2(W1, H, "MBlas::next2");
+  if (R.data == W1.data || R.data == W2.data) throw std::runtime_error("MBlas::next2: R aliases W1 or W2");
+  const MShape a = shape(W1, nact);
+  reserve((size_t)a.gx * W1.cap);
+  hipLaunchKernelGGL(k_next2, dim3(a.gx, W1.nblk()), dim3(MB), 0, stream, a, W1.data, W2.data, R.data, H.data, b,
+                     part);
+  QQCD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mfinal, dim3(W1.cap, 1), dim3(64), 0, stream, 1, W1.cap, a.gx, nact, (const double *)part, nr,
+                     0, 0);
+  QQCD_HIP(hipGetLastError());
+}
+
 void MBlas::swap(MultiSpinor &W, int sa, int sb) {
   need_fp(W, "MBlas::swap");
   if (sa == sb) return;
@@ -733,6 +1013,17 @@ void arnoldi_flags(const double *n2, const double *w2, int *flag, int *any, cons
 void clear_flags(int *flag, int *any, int cap, hipStream_t s) {
   if (cap > MB) throw std::runtime_error("clear_flags: at most 256 slots");
   hipLaunchKernelGGL(k_arnoldi_flags, dim3(1), dim3(MB), 0, s, nullptr, nullptr, flag, any, nullptr, cap, 0);
+  QQCD_HIP(hipGetLastError());
+}
+
+void sstep_columns(double *H, int ldh, int j, const double *g1, const double *g2, const double *sc, double *b,
+                   const int *nact, int cap, double hs, hipStream_t s) {
+  hipLaunchKernelGGL(k_sstep_columns, dim3((cap + 63) / 64), dim3(64), 0, s, H, ldh, j, g1, g2, sc, b, nact, cap, hs);
+  QQCD_HIP(hipGetLastError());
+}
+
+void set_subdiag(double *H, int ldh, int j, const double *n2, const int *nact, int cap, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_subdiag, dim3((cap + 63) / 64), dim3(64), 0, s, H, ldh, j, n2, nact, cap);
   QQCD_HIP(hipGetLastError());
 }
 

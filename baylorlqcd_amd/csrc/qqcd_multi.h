@@ -58,6 +58,20 @@ class MBlas {
   void scale_inv_norm(const MultiSpinor &X, MultiSpinor &W, const double *n2, const int *nact);
   // the same into Q and into the FP16 basis field H (X may be Q)
   void scale_inv_norm2(const MultiSpinor &X, MultiSpinor &Q, MultiSpinor &H, const double *n2, const int *nact);
+  // Two-vector passes of the two-step Arnoldi block (FP32 W1, W2; FP16
+  // basis fields V): g1[q*cap + r] = <V_q(r), W1(r)>, g2[...] = <V_q(r),
+  // W2(r)> in one pass over V (per-lane sums in FP32, across lanes in FP64).
+  void dot2(int nq, const MultiSpinor *V, const MultiSpinor &W1, const MultiSpinor &W2, const int *nact, double *g1,
+            double *g2);
+  // W1(r) -= sum_q g1[q*cap + r] V_q(r), W2(r) -= sum_q g2[...] V_q(r) in one
+  // pass over V; then sc[c*cap + r] = ||W1||^2, Re <W1, W2>, Im <W1, W2>,
+  // ||W2||^2 for c = 0..3
+  void axpy2(int nq, const MultiSpinor *V, const double *g1, const double *g2, MultiSpinor &W1, MultiSpinor &W2,
+             const int *nact, double *sc);
+  // R(r) = b[4r] W2(r) + (b[4r+1] + i b[4r+2]) W1(r) (FP32) and H(r) =
+  // b[4r+3] W1(r) as an FP16 basis field; nr[r] = ||R(r)||^2
+  void next2(const MultiSpinor &W1, const MultiSpinor &W2, MultiSpinor &R, MultiSpinor &H, const double *b,
+             const int *nact, double *nr);
   // swap slots a and b of W
   void swap(MultiSpinor &W, int a, int b);
   // single <-> slot, with precision conversion
@@ -88,6 +102,13 @@ void clear_flags(int *flag, int *any, int cap, hipStream_t s);
 // Hessenberg column j (complex, [(j*ldh + q)*cap + r]): h1 + (flag ? h2 : 0) for
 // q <= j, and sqrt(nf) below the diagonal with nf[r] = flag ? n2b[r] : n2a[r]
 // (the squared norm after the last Gram-Schmidt pass)
+// Columns j and j+1 of the batched Hessenberg matrix for a two-step block
+// from g1 = V_j^H A v_j, g2 = V_j^H A^2 v_j and the four scalars of axpy2;
+// b receives the per-slot coefficients next2 needs.
+void sstep_columns(double *H, int ldh, int j, const double *g1, const double *g2, const double *sc, double *b,
+                   const int *nact, int cap, double hs, hipStream_t s);
+// H[q, j] = sqrt(n2[r]) for one row q of column j (the subdiagonal entry after next2)
+void set_subdiag(double *H, int ldh, int j, const double *n2, const int *nact, int cap, hipStream_t s);
 void arnoldi_column(double *H, int ldh, int j, const double *h1, const double *h2, const int *flag,
                     const double *n2a, const double *n2b, double *nf, const int *nact, int cap, hipStream_t s);
 

@@ -270,10 +270,12 @@ def test_fermprop_multi(cfg1, reorth, graph, basis):
 def test_fermprop_multi_normalisation_in_operator(cfg1):
     """With the FP16 basis and the two-slot operator each Arnoldi step's
     normalisation is done by the next operator application (the second hop
-    scales its result by 1 / ||w|| and writes the FP16 basis vector); with
-    the one-slot launch form (2) a separate pass does it.  Both reach the
-    same true residuals and solutions to the solve's tolerance, with graph
-    replays and eagerly."""
+    scales its result by 1 / ||w|| and writes the FP16 basis vector), and the
+    steps run in pairs (one Gram-Schmidt pass over the basis for the dots of
+    A v_j and A^2 v_j, one for both updates, column j + 1 from the Arnoldi
+    relation) or one at a time; with the one-slot launch form (2) a separate
+    pass normalises.  All reach the same true residuals and solutions to the
+    solve's tolerance, with graph replays and eagerly."""
     n, bc, U, u, co, g = cfg1
     kappa = (0.150, 0.0)
     srcs = []
@@ -285,11 +287,12 @@ def test_fermprop_multi_normalisation_in_operator(cfg1):
     lib = _lib.lib()
     sols = {}
     try:
-        for form in (3, 2):
+        for form, steps in ((3, 2), (3, 1), (2, 1)):
             for graph in (1, 0):
                 _lib.check(lib.qqcd_set_multi_tune(form, -1, -1))
+                _lib.check(lib.qqcd_batch_set_steps(steps))
                 disconloops.batch_set(use_graph=graph, basis_prec=2)
-                print(f"[multi] normalisation in the operator: form {form} graph {graph}", flush=True)
+                print(f"[multi] normalisation in the operator: form {form} steps {steps} graph {graph}", flush=True)
                 xe, xo, it, rr = disconloops.fermprop_multi([s[0] for s in srcs], [s[1] for s in srcs], kappa, co,
                                                             bc, 1e-10, (40, 10), u)
                 xs = []
@@ -298,16 +301,18 @@ def test_fermprop_multi_normalisation_in_operator(cfg1):
                     x = O.spinor_ref_to_canon(n, xe[r], xo[r])
                     assert full_rel_res(n, bc, U, kappa, b, x) < 1e-10, (form, graph, r)
                     xs.append(x)
-                sols[(form, graph)] = xs
+                sols[(form, steps, graph)] = xs
     finally:
         _lib.check(lib.qqcd_set_multi_tune(-1, -1, -1))
+        _lib.check(lib.qqcd_batch_set_steps(2))
         disconloops.batch_set()
     for key, xs in sols.items():
         for r in range(len(srcs)):
-            assert rel(xs[r], sols[(2, 0)][r]) < 1e-8, (key, r)
+            assert rel(xs[r], sols[(2, 1, 0)][r]) < 1e-8, (key, r)
     # graph replays give the eager launches' result bit for bit
     for r in range(len(srcs)):
-        assert np.array_equal(sols[(3, 1)][r], sols[(3, 0)][r]), r
+        assert np.array_equal(sols[(3, 2, 1)][r], sols[(3, 2, 0)][r]), r
+        assert np.array_equal(sols[(3, 1, 1)][r], sols[(3, 1, 0)][r]), r
 
 
 def test_noise_loop_traces_and_graph(cfg1):
