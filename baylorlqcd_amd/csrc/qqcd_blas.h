@@ -27,6 +27,9 @@ inline Vec as_vec(const Spinor &s) {
   return v;
 }
 
+// Column j + 1 of a paired step from the device Hessenberg (see k_pair_columns)
+void pair_columns(double *d, int ld, int m, int j, const double *g2, const double *sc, double *b, hipStream_t s);
+
 class Blas {
  public:
   explicit Blas(hipStream_t s);
@@ -46,6 +49,14 @@ class Blas {
   void multi_dot_dev(int k, const Vec *v, const Vec &w, double *dout);
   void multi_axpy_dev(int k, const double *dcoef, const Vec *v, Vec &w, double *dnorm2);
   void rscale_dev(Vec &w, const double *dnorm2);
+  // Paired Arnoldi steps over an FP16 basis with per-site norms (FP32 w1,
+  // w2, device-side): g1[2i..] = <v_i, w1>, g2[2i..] = <v_i, w2> in one pass;
+  // w1 -= sum g1_i v_i, w2 -= sum g2_i v_i in one pass with sc[0..3] =
+  // ||w1||^2, Re <w1, w2>, Im <w1, w2>, ||w2||^2; w2 = b[0] w2 + (b[1] +
+  // i b[2]) w1 with ||w2||^2 into *dnorm2.
+  void pair_dot_dev(int k, const Vec *v, const Vec &w1, const Vec &w2, double *g1, double *g2);
+  void pair_axpy_dev(int k, const double *g1, const double *g2, const Vec *v, Vec &w1, Vec &w2, double *sc);
+  void pair_comb_dev(const double *b, const Vec &w1, Vec &w2, double *dnorm2);
   // y = a x + b y ; returns ||y||^2 when want_norm
   double axpby(std::complex<double> a, const Vec &x, std::complex<double> b, Vec &y, bool want_norm = false);
   void copy(const Vec &x, Vec &y);

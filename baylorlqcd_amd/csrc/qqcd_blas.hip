@@ -229,6 +229,180 @@ __global__ void __launch_bounds__(RB) k_final(int nb, int nq, const double *part
   if (threadIdx.x == 0) res[q] = o[0];
 }
 
+// res[q] = sum_b part[b*ncol + c0 + q]: columns c0.. of rows ncol wide
+__global__ void __launch_bounds__(RB) k_final_cols(int nb, int ncol, int c0, const double *part, double *res) {
+  const int q = blockIdx.x;
+  double s = 0;
+  for (int b = threadIdx.x; b < nb; b += RB) s += part[(long)b * ncol + c0 + q];
+  double v[1] = {s};
+  __shared__ double o[1];
+  block_sum<1>(v, o);
+  __syncthreads();
+  if (threadIdx.x == 0) res[q] = o[0];
+}
+
+// ---- paired Arnoldi steps over an FP16 basis with per-site norms ----
+// at most KMAXP2 basis vectors per pass: the two work vectors' accumulators
+// or coefficients and the basis norms double the registers of a lane, and a
+// 16-vector pass drops to two waves per SIMD
+static constexpr int KMAXP2 = 8;
+// Two FP32 vectors w1 = A v_j, w2 = A w1 against K basis vectors in one pass:
+// part rows of 4K: the K complex dots with w1, then the K with w2.
+template <int K>
+__global__ void __launch_bounds__(RB) k_mdot2(long nu, PtrPack v, const void *w1, const void *w2, double *part) {
+  const long nq = nu / 12;  // site quads
+  float af[4 * K];
+#pragma unroll
+  for (int k = 0; k < 4 * K; k++) af[k] = 0.f;
+  for (long q = blockIdx.x * (long)RB + threadIdx.x; q < nq; q += (long)gridDim.x * RB) {
+    float4 m[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) m[k] = *(const float4 *)(v.n[k] + 4 * q);
+    for (int c = 0; c < 12; c++) {
+      const long i = c * nq + q;
+      float2 b1[4], b2[4];
+      ld4f<float>(w1, i, b1);
+      ld4f<float>(w2, i, b2);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        float2 a[4];
+        ld_h4f(v.p[k], m[k], i, a);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          af[2 * k] += a[u].x * b1[u].x + a[u].y * b1[u].y;
+          af[2 * k + 1] += a[u].x * b1[u].y - a[u].y * b1[u].x;
+          af[2 * K + 2 * k] += a[u].x * b2[u].x + a[u].y * b2[u].y;
+          af[2 * K + 2 * k + 1] += a[u].x * b2[u].y - a[u].y * b2[u].x;
+        }
+      }
+    }
+  }
+  double acc[4 * K];
+#pragma unroll
+  for (int k = 0; k < 4 * K; k++) acc[k] = (double)af[k];
+  block_sum<4 * K>(acc, part + (long)blockIdx.x * 4 * K);
+}
+
+// w1 -= sum_k g1_k v_k, w2 -= sum_k g2_k v_k (coefficients in device memory);
+// LAST: partials of ||w1||^2, Re <w1, w2>, Im <w1, w2>, ||w2||^2
+template <int K, bool LAST>
+__global__ void __launch_bounds__(RB) k_maxpy2(long nu, PtrPack v, const double *g1, const double *g2, void *w1,
+                                              void *w2, double *part) {
+  const long nq = nu / 12;
+  float c1r[K], c1i[K], c2r[K], c2i[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    c1r[k] = (float)g1[2 * k];
+    c1i[k] = (float)g1[2 * k + 1];
+    c2r[k] = (float)g2[2 * k];
+    c2i[k] = (float)g2[2 * k + 1];
+  }
+  float s4[4] = {0.f, 0.f, 0.f, 0.f};
+  for (long q = blockIdx.x * (long)RB + threadIdx.x; q < nq; q += (long)gridDim.x * RB) {
+    float4 m[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) m[k] = *(const float4 *)(v.n[k] + 4 * q);
+    for (int c = 0; c < 12; c++) {
+      const long i = c * nq + q;
+      float2 b1[4], b2[4];
+      ld4f<float>(w1, i, b1);
+      ld4f<float>(w2, i, b2);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        float2 a[4];
+        ld_h4f(v.p[k], m[k], i, a);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          b1[u] = make_float2(b1[u].x - (c1r[k] * a[u].x - c1i[k] * a[u].y), b1[u].y - (c1r[k] * a[u].y + c1i[k] * a[u].x));
+          b2[u] = make_float2(b2[u].x - (c2r[k] * a[u].x - c2i[k] * a[u].y), b2[u].y - (c2r[k] * a[u].y + c2i[k] * a[u].x));
+        }
+      }
+      st4f<float>(w1, i, b1);
+      st4f<float>(w2, i, b2);
+      if (LAST)
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          s4[0] += b1[u].x * b1[u].x + b1[u].y * b1[u].y;
+          s4[1] += b1[u].x * b2[u].x + b1[u].y * b2[u].y;
+          s4[2] += b1[u].x * b2[u].y - b1[u].y * b2[u].x;
+          s4[3] += b2[u].x * b2[u].x + b2[u].y * b2[u].y;
+        }
+    }
+  }
+  if (LAST) {
+    double acc[4] = {s4[0], s4[1], s4[2], s4[3]};
+    block_sum<4>(acc, part + (long)blockIdx.x * 4);
+  }
+}
+
+// w2 = b[0] w2 + (b[1] + i b[2]) w1 (FP32, in place), partials of ||w2||^2
+__global__ void __launch_bounds__(RB) k_comb2(long nu, const double *b, const void *w1, void *w2, double *part) {
+  const float s2 = (float)b[0], s1r = (float)b[1], s1i = (float)b[2];
+  float nr = 0.f;
+  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < nu; i += (long)gridDim.x * RB) {
+    float2 u1[4], u2[4];
+    ld4f<float>(w1, i, u1);
+    ld4f<float>(w2, i, u2);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      u2[u] = make_float2(s2 * u2[u].x + (s1r * u1[u].x - s1i * u1[u].y), s2 * u2[u].y + (s1r * u1[u].y + s1i * u1[u].x));
+      nr += u2[u].x * u2[u].x + u2[u].y * u2[u].y;
+    }
+    st4f<float>(w2, i, u2);
+  }
+  double acc[1] = {(double)nr};
+  block_sum<1>(acc, part + blockIdx.x);
+}
+
+// Paired steps j, j + 1 on the device Hessenberg of the single-RHS solvers
+// (columns of ld doubles: the complex <v_i, w>, then m squared norms): column
+// j holds g1 = V_j^H A v_j (from pair_dot_dev); with nu^2 = ||u1'||^2 = sc[0],
+// column j + 1 follows from the Arnoldi relation A v_{j+1} = (A^2 v_j -
+// V_{j+1} H_j g1) / nu: H[k, j+1] = (g2[k] - c[k]) / nu for k <= j, (<u1',
+// u2'> / nu - c[j+1]) / nu on the diagonal, c = H_j g1; b = (1/nu, -alpha/nu)
+// with alpha = <u1', u2'> / nu^2, the coefficients of the part of A v_{j+1}
+// orthogonal to v_0..v_{j+1}.  One thread.
+__global__ void k_pair_columns(double *d, int ld, int m, int j, const double *g2, const double *sc, double *b) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double *n2 = d + (long)ld * m;
+  const double *g1 = d + (long)ld * j;
+  double *c1 = d + (long)ld * (j + 1);
+  const double n1 = sc[0];
+  n2[j] = n1;
+  const double nu = n1 > 0.0 ? sqrt(n1) : 0.0;
+  if (!(nu > 0.0)) {
+    for (int k = 0; k <= j + 1; k++) c1[2 * k] = c1[2 * k + 1] = 0.0;
+    b[0] = b[1] = b[2] = 0.0;
+    return;
+  }
+  const double inv = 1.0 / nu;
+  for (int k = 0; k <= j + 1; k++) {
+    double cr = 0.0, ci = 0.0;
+    for (int i = k > 0 ? k - 1 : 0; i <= j; i++) {
+      double hr, hi;
+      if (k <= i) {
+        hr = d[(long)ld * i + 2 * k];
+        hi = d[(long)ld * i + 2 * k + 1];
+      } else {  // k == i + 1: the subdiagonal entry
+        hr = n2[i] > 0.0 ? sqrt(n2[i]) : 0.0;
+        hi = 0.0;
+      }
+      cr += hr * g1[2 * i] - hi * g1[2 * i + 1];
+      ci += hr * g1[2 * i + 1] + hi * g1[2 * i];
+    }
+    if (k <= j) {
+      c1[2 * k] = (g2[2 * k] - cr) * inv;
+      c1[2 * k + 1] = (g2[2 * k + 1] - ci) * inv;
+    } else {
+      c1[2 * k] = (sc[1] * inv - cr) * inv;
+      c1[2 * k + 1] = (sc[2] * inv - ci) * inv;
+    }
+  }
+  b[0] = inv;
+  b[1] = -sc[1] * inv * inv * inv;
+  b[2] = -sc[2] * inv * inv * inv;
+}
+
 // ---- w = w - sum_k h_k v_k, returns ||w||^2 (optional) ----
 struct CoefPack {
   double re[KMAXP], im[KMAXP];
@@ -655,6 +829,74 @@ void Blas::multi_axpy_dev(int k, const double *dcoef, const Vec *v, Vec &w, doub
       QQCD_HIP(hipGetLastError());
     }
   }
+}
+
+void Blas::pair_dot_dev(int k, const Vec *v, const Vec &w1, const Vec &w2, double *g1, double *g2) {
+  if (k > kmax) throw std::runtime_error("pair_dot: too many vectors");
+  if (w1.prec != PREC_SINGLE || w2.prec != PREC_SINGLE || w1.n != w2.n) throw std::runtime_error("pair_dot: FP32 w1, w2");
+  long off = 0;
+  for (int k0 = 0; k0 < k; k0 += KMAXP2) {
+    const int kk = std::min(KMAXP2, k - k0);
+    PtrPack pp{};
+    for (int q = 0; q < kk; q++) {
+      if (v[k0 + q].n != w1.n || v[k0 + q].prec != PREC_HALF) throw std::runtime_error("pair_dot: FP16 basis vectors");
+      check_half(v[k0 + q]);
+      pp.p[q] = v[k0 + q].data;
+      pp.n[q] = v[k0 + q].norm;
+    }
+    double *pt = part + off;
+    with_k(kk, [&](auto K) {
+      hipLaunchKernelGGL((k_mdot2<decltype(K)::value>), dim3(RG), dim3(RB), 0, stream, units<f16>(w1.n), pp, w1.data,
+                         w2.data, pt);
+    });
+    QQCD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_final_cols, dim3(2 * kk), dim3(RB), 0, stream, RG, 4 * kk, 0, pt, g1 + 2 * k0);
+    hipLaunchKernelGGL(k_final_cols, dim3(2 * kk), dim3(RB), 0, stream, RG, 4 * kk, 2 * kk, pt, g2 + 2 * k0);
+    QQCD_HIP(hipGetLastError());
+    off += (long)RG * 4 * kk;
+  }
+}
+
+void Blas::pair_axpy_dev(int k, const double *g1, const double *g2, const Vec *v, Vec &w1, Vec &w2, double *sc) {
+  if (w1.prec != PREC_SINGLE || w2.prec != PREC_SINGLE || w1.n != w2.n || w1.data == w2.data)
+    throw std::runtime_error("pair_axpy: distinct FP32 w1, w2");
+  if (k < 1) throw std::runtime_error("pair_axpy: at least one vector");
+  for (int k0 = 0; k0 < k; k0 += KMAXP2) {
+    const int kk = std::min(KMAXP2, k - k0);
+    const bool last = k0 + KMAXP2 >= k;
+    PtrPack pp{};
+    for (int q = 0; q < kk; q++) {
+      if (v[k0 + q].n != w1.n || v[k0 + q].prec != PREC_HALF) throw std::runtime_error("pair_axpy: FP16 basis vectors");
+      check_half(v[k0 + q]);
+      pp.p[q] = v[k0 + q].data;
+      pp.n[q] = v[k0 + q].norm;
+    }
+    with_k(kk, [&](auto K) {
+      constexpr int KK = decltype(K)::value;
+      if (last)
+        hipLaunchKernelGGL((k_maxpy2<KK, true>), dim3(RG), dim3(RB), 0, stream, units<f16>(w1.n), pp, g1 + 2 * k0,
+                           g2 + 2 * k0, w1.data, w2.data, part);
+      else
+        hipLaunchKernelGGL((k_maxpy2<KK, false>), dim3(RG), dim3(RB), 0, stream, units<f16>(w1.n), pp, g1 + 2 * k0,
+                           g2 + 2 * k0, w1.data, w2.data, part);
+    });
+    QQCD_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_final, dim3(4), dim3(RB), 0, stream, RG, 4, part, sc);
+  QQCD_HIP(hipGetLastError());
+}
+
+void Blas::pair_comb_dev(const double *b, const Vec &w1, Vec &w2, double *dnorm2) {
+  if (w1.prec != PREC_SINGLE || w2.prec != PREC_SINGLE || w1.n != w2.n) throw std::runtime_error("pair_comb: FP32 w1, w2");
+  hipLaunchKernelGGL(k_comb2, dim3(RG), dim3(RB), 0, stream, units<f16>(w1.n), b, w1.data, w2.data, part);
+  QQCD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, stream, RG, 1, part, dnorm2);
+  QQCD_HIP(hipGetLastError());
+}
+
+void pair_columns(double *d, int ld, int m, int j, const double *g2, const double *sc, double *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_columns, dim3(1), dim3(64), 0, s, d, ld, m, j, g2, sc, b);
+  QQCD_HIP(hipGetLastError());
 }
 
 void Blas::rscale_dev(Vec &w, const double *dnorm2) {

@@ -362,10 +362,10 @@ extern "C" int qqcd_loopsum(int ksub, const double *xe, const double *xo, const 
   API_END
 }
 
-extern "C" int qqcd_batch_set_steps(int steps) {
+extern "C" int qqcd_set_arnoldi_steps(int steps) {
   API_BEGIN
-  if (steps != 1 && steps != 2) throw std::runtime_error("Arnoldi steps per Gram-Schmidt pass: 1 or 2");
-  g_bp.sstep = steps;
+  set_arnoldi_steps(steps);  // the single-RHS solvers (checks the value)
+  g_bp.sstep = steps;        // the batched solver
   API_END
 }
 

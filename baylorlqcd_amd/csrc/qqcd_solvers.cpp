@@ -113,15 +113,21 @@ struct DevHess {
   int m, ld;              // ld = 2 (m + 1) doubles per column
   double *d = nullptr;    // m columns of complex <V_i, w>, then m squared norms
   double *h = nullptr;    // pinned host copy
+  double *x = nullptr;    // paired steps: g2 (ld), the four scalars, the three coefficients
   explicit DevHess(int m_) : m(m_), ld(2 * (m_ + 1)) {
     const size_t n = (size_t)(ld + 1) * m;
     QQCD_HIP(hipMalloc(&d, sizeof(double) * n));
+    QQCD_HIP(hipMalloc(&x, sizeof(double) * (ld + 8)));
     QQCD_HIP(hipHostMalloc(&h, sizeof(double) * n, hipHostMallocDefault));
   }
   ~DevHess() {
     if (d) (void)hipFree(d);
+    if (x) (void)hipFree(x);
     if (h) (void)hipHostFree(h);
   }
+  double *g2() { return x; }
+  double *sc() { return x + ld; }
+  double *cb() { return x + ld + 4; }
   DevHess(const DevHess &) = delete;
   DevHess &operator=(const DevHess &) = delete;
   double *col(int j) { return d + (size_t)ld * j; }
@@ -160,6 +166,35 @@ static void arnoldi_step_dev(Context &c, LinOp &A, std::vector<Spinor> &V, int j
   else c.blas->rscale_dev(w, dh.n2(j));
 }
 
+// Arnoldi steps j and j + 1 in one pair (FP16 basis with per-site norms, FP32
+// work vectors w1, w2): u1 = A v_j, u2 = A u1, the dots of both against
+// V_0..V_j in one pass over the basis, both updates in another, column j + 1
+// of H from the Arnoldi relation (pair_columns); then v_{j+1} = u1' / ||u1'||
+// and v_{j+2} from the part of A v_{j+1} orthogonal to v_0..v_{j+1}.  Half
+// the passes over the basis of two single steps.  The derivation of column
+// j + 1 uses the orthonormality of V_j, which the FP16 basis holds to its
+// rounding, as the single-step passes do.
+static int g_arnoldi_steps = 2;
+void set_arnoldi_steps(int steps) {
+  if (steps != 1 && steps != 2) throw std::runtime_error("Arnoldi steps per Gram-Schmidt pass: 1 or 2");
+  g_arnoldi_steps = steps;
+}
+int arnoldi_steps() { return g_arnoldi_steps; }
+
+static void arnoldi_pair_dev(Context &c, LinOp &A, std::vector<Spinor> &V, int j, DevHess &dh, Spinor &w1,
+                             Spinor &w2) {
+  A.apply(V[j], w1);
+  A.apply(w1, w2);
+  Vec a1 = vv(w1), a2 = vv(w2);
+  std::vector<Vec> Vb = vecs(V, j + 1);
+  c.blas->pair_dot_dev(j + 1, Vb.data(), a1, a2, dh.col(j), dh.g2());
+  c.blas->pair_axpy_dev(j + 1, dh.col(j), dh.g2(), Vb.data(), a1, a2, dh.sc());
+  pair_columns(dh.d, dh.ld, dh.m, j, dh.g2(), dh.sc(), dh.cb(), c.stream);
+  c.blas->pair_comb_dev(dh.cb(), a1, a2, dh.n2(j + 1));
+  spinor_convert_rnorm(c.g, w1, V[j + 1], dh.n2(j), c.stream);
+  spinor_convert_rnorm(c.g, w2, V[j + 2], dh.n2(j + 1), c.stream);
+}
+
 // Groups of synchronisation-free steps replayed from HIP graphs: a group
 // [j0, j0 + n) that already ran on the same vectors is launched as one
 // graph, which removes the launch gaps between its ~13 kernels per step.
@@ -196,13 +231,20 @@ struct StepGraphs {
 };
 
 static void arnoldi_steps_dev(Context &c, LinOp &A, std::vector<Spinor> &V, int j0, int n, DevHess &dh,
-                              Spinor *work, StepGraphs *sg) {
+                              Spinor *work, StepGraphs *sg, Spinor *work2 = nullptr) {
+  // in pairs when a second work vector is given and the group is whole pairs
+  auto run = [&]() {
+    if (work && work2 && j0 % 2 == 0 && n % 2 == 0)
+      for (int j = j0; j < j0 + n; j += 2) arnoldi_pair_dev(c, A, V, j, dh, *work, *work2);
+    else
+      for (int j = j0; j < j0 + n; j++) arnoldi_step_dev(c, A, V, j, dh, nullptr, nullptr, work);
+  };
   if (!sg) {
-    for (int j = j0; j < j0 + n; j++) arnoldi_step_dev(c, A, V, j, dh, nullptr, nullptr, work);
+    run();
     return;
   }
   std::vector<const void *> key{(const void *)(intptr_t)j0, (const void *)(intptr_t)n, work ? work->data : nullptr,
-                                dh.d};
+                                work2 ? work2->data : nullptr, dh.d};
   for (int q = 0; q <= j0 + n; q++) {
     key.push_back(V[q].data);
     key.push_back(V[q].norm);
@@ -214,13 +256,13 @@ static void arnoldi_steps_dev(Context &c, LinOp &A, std::vector<Spinor> &V, int 
       return;
     }
   // first time: run the steps, then record them for the next time
-  for (int j = j0; j < j0 + n; j++) arnoldi_step_dev(c, A, V, j, dh, nullptr, nullptr, work);
+  run();
   QQCD_HIP(hipStreamSynchronize(c.stream));
   StepGraphs::Entry e;
   e.key = key;
   QQCD_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
   try {
-    for (int j = j0; j < j0 + n; j++) arnoldi_step_dev(c, A, V, j, dh, nullptr, nullptr, work);
+    run();
   } catch (...) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(c.stream, &g);
@@ -263,6 +305,7 @@ void ProjWork::reset() {
   sg.reset();  // the graphs name the buffers: they go first
   dh.reset();
   work.reset();
+  work2.reset();
   free_basis(V);
   m = 0;
 }
@@ -372,8 +415,10 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
     alloc_basis(c, Z, m, p);
     alloc_basis(c, ZW, k, p);
   }
-  std::unique_ptr<SpinorBuf> work;
+  std::unique_ptr<SpinorBuf> work, work2;
   if (vp != p) work.reset(new SpinorBuf(c.g, p));
+  // paired Arnoldi steps: one Gram-Schmidt pass, FP32 work over an FP16 basis
+  if (g_arnoldi_steps == 2 && !reorth && !M && p == PREC_SINGLE && vp == PREC_HALF) work2.reset(new SpinorBuf(c.g, p));
   std::unique_ptr<DevHess> dh;
   if (!reorth) dh.reset(new DevHess(m));
   std::unique_ptr<StepGraphs> sg;
@@ -400,9 +445,9 @@ SolveStats gmresdr(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, int 
     int jchk = j;
     while (j < m) {
       if (dh) {
-        if (sg) {
+        if (sg || work2) {
           const int n = std::min(std::min(kCheck, m - j), std::max(1, maxiter - st.iters));
-          arnoldi_steps_dev(c, A, V, j, n, *dh, work ? &work->s : nullptr, sg.get());
+          arnoldi_steps_dev(c, A, V, j, n, *dh, work ? &work->s : nullptr, sg.get(), work2 ? &work2->s : nullptr);
           st.matvecs += n;
           st.iters += n;
           j += n;
@@ -498,18 +543,22 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
   // or the precisions change)
   std::vector<Spinor> Vown;
   BasisGuard g1{Vown};
-  std::unique_ptr<SpinorBuf> work_own;
+  std::unique_ptr<SpinorBuf> work_own, work2_own;
   std::unique_ptr<DevHess> dh_own;
   std::unique_ptr<StepGraphs> sg_own;
   std::vector<Spinor> *Vp = &Vown;
-  SpinorBuf *work = nullptr;
+  SpinorBuf *work = nullptr, *work2 = nullptr;
   DevHess *dh = nullptr;
   StepGraphs *sg = nullptr;
+  // paired Arnoldi steps: one Gram-Schmidt pass, FP32 work over an FP16 basis
+  const bool pairs = g_arnoldi_steps == 2 && !reorth && p == PREC_SINGLE && vp == PREC_HALF;
   if (ws) {
-    if (ws->m != m || ws->p != p || ws->vp != vp || (!reorth) != (ws->dh != nullptr)) {
+    if (ws->m != m || ws->p != p || ws->vp != vp || (!reorth) != (ws->dh != nullptr) ||
+        pairs != (ws->work2 != nullptr)) {
       ws->reset();
       alloc_basis(c, ws->V, m + 1, vp);
       if (vp != p) ws->work.reset(new SpinorBuf(c.g, p));
+      if (pairs) ws->work2.reset(new SpinorBuf(c.g, p));
       if (!reorth) ws->dh.reset(new DevHess(m));
       ws->m = m;
       ws->p = p;
@@ -519,14 +568,17 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
     if (!g_step_graphs) ws->sg.reset();
     Vp = &ws->V;
     work = ws->work.get();
+    work2 = ws->work2.get();
     dh = ws->dh.get();
     sg = ws->sg.get();
   } else {
     alloc_basis(c, Vown, m + 1, vp);
     if (vp != p) work_own.reset(new SpinorBuf(c.g, p));
+    if (pairs) work2_own.reset(new SpinorBuf(c.g, p));
     if (!reorth) dh_own.reset(new DevHess(m));
     if (dh_own && g_step_graphs == 1) sg_own.reset(new StepGraphs);
     work = work_own.get();
+    work2 = work2_own.get();
     dh = dh_own.get();
     sg = sg_own.get();
   }
@@ -565,9 +617,9 @@ SolveStats gmresproj(Context &c, LinOp &A, const Spinor &b, Spinor &x, int m, co
     int j = 0, jchk = 0;
     while (j < m) {
       if (dh) {
-        if (sg) {
+        if (sg || work2) {
           const int n = std::min(std::min(kCheck, m - j), std::max(1, maxiter - st.iters));
-          arnoldi_steps_dev(c, A, V, j, n, *dh, work ? &work->s : nullptr, sg);
+          arnoldi_steps_dev(c, A, V, j, n, *dh, work ? &work->s : nullptr, sg, work2 ? &work2->s : nullptr);
           st.matvecs += n;
           st.iters += n;
           j += n;

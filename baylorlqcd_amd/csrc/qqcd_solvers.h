@@ -99,7 +99,7 @@ struct StepGraphs;
 // same operator; reset() whenever that operator is replaced.
 struct ProjWork {
   std::vector<Spinor> V;
-  std::unique_ptr<SpinorBuf> work;
+  std::unique_ptr<SpinorBuf> work, work2;
   std::unique_ptr<DevHess> dh;
   std::unique_ptr<StepGraphs> sg;
   int m = 0;
@@ -147,6 +147,10 @@ namespace qqcd {
 // graphs: 0 never, 1 in every solve, 2 (the default) only where the basis is
 // kept between calls (ProjWork), so that a capture serves many solves
 void set_step_graphs(int mode);
+// Arnoldi steps per Gram-Schmidt pass over an FP16 basis (single-RHS
+// solvers): 2 (the default) in pairs, 1 one at a time
+void set_arnoldi_steps(int steps);
+int arnoldi_steps();
 
 struct MixedParams {
   int m = 40, k = 16;

@@ -420,14 +420,14 @@ int qqcd_trace16(const double *eta_e, const double *eta_o, const double *x_e, co
 int qqcd_batch_set(int reorth, int use_graph, double inner_tol, int inner_cycles, int max_outer, int op_prec,
                    int op_recon, int basis_prec);
 
-/* Arnoldi steps of the batched solver per Gram-Schmidt pass over its basis:
- * 2 (the default) runs the steps in pairs -- u1 = A v_j, u2 = A u1, the dots
- * of both and the updates of both in one pass each over the basis, column
- * j + 1 of the Hessenberg matrix from the Arnoldi relation -- where the FP16
- * basis, the two-slot operator, one Gram-Schmidt pass and an even cycle
- * length allow it; 1 one step at a time.  Tuning aid; no reference
- * counterpart. */
-int qqcd_batch_set_steps(int steps);
+/* Arnoldi steps per Gram-Schmidt pass over an FP16 Krylov basis, in the
+ * batched and the single-right-hand-side solvers: 2 (the default) runs the
+ * steps in pairs -- u1 = A v_j, u2 = A u1, the dots of both and the updates
+ * of both in one pass each over the basis, column j + 1 of the Hessenberg
+ * matrix from the Arnoldi relation -- where one Gram-Schmidt pass (and, in
+ * the batched solver, the two-slot operator and an even cycle length)
+ * allows it; 1 one step at a time.  Tuning aid; no reference counterpart. */
+int qqcd_set_arnoldi_steps(int steps);
 /* Statistics of the last batched solve. */
 int qqcd_batch_stats(int *outer, int *cycles, long *slot_matvecs, long *batched_matvecs, int *graph_replays,
                      int *fallback);

@@ -290,7 +290,7 @@ def test_fermprop_multi_normalisation_in_operator(cfg1):
         for form, steps in ((3, 2), (3, 1), (2, 1)):
             for graph in (1, 0):
                 _lib.check(lib.qqcd_set_multi_tune(form, -1, -1))
-                _lib.check(lib.qqcd_batch_set_steps(steps))
+                _lib.check(lib.qqcd_set_arnoldi_steps(steps))
                 disconloops.batch_set(use_graph=graph, basis_prec=2)
                 print(f"[multi] normalisation in the operator: form {form} steps {steps} graph {graph}", flush=True)
                 xe, xo, it, rr = disconloops.fermprop_multi([s[0] for s in srcs], [s[1] for s in srcs], kappa, co,
@@ -304,7 +304,7 @@ def test_fermprop_multi_normalisation_in_operator(cfg1):
                 sols[(form, steps, graph)] = xs
     finally:
         _lib.check(lib.qqcd_set_multi_tune(-1, -1, -1))
-        _lib.check(lib.qqcd_batch_set_steps(2))
+        _lib.check(lib.qqcd_set_arnoldi_steps(2))
         disconloops.batch_set()
     for key, xs in sols.items():
         for r in range(len(srcs)):

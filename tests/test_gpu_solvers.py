@@ -259,3 +259,45 @@ def test_step_graphs_replay_same_solve():
     odd = O.parity_mask(N8)
     r = bb - O.schur(N8, U, BC, [1] * 4, [1] * 4, 0, kappa, xm)
     assert np.linalg.norm(r[~odd]) / np.linalg.norm(bb[~odd]) < 1e-10
+
+def test_mixed_paired_steps():
+    """The mixed solver over an FP16 basis runs its inner Arnoldi steps in
+    pairs (A v_j and A^2 v_j orthogonalised with one pass over the basis for
+    both sets of dots and one for both updates, column j + 1 from the Arnoldi
+    relation) or one at a time: both reach the true residual, their solutions
+    agree to the solve's tolerance, and the paired solve replays from HIP
+    graphs bit for bit."""
+    from baylorlqcd_amd import _lib
+    g = np.load(os.path.join(GOLD, "golden_8x8x8x16.npz"))
+    lattice.init(N8, BC)
+    U = gauge.reconstruct12(g["links12"])
+    u = O.links_canon_to_ref(N8, U)
+    co = O.coact_array()
+    kappa = (0.150, 0.0)
+    be, bo = lattice.new_spinor(N8), lattice.new_spinor(N8)
+    quark.pointsource(be, bo, (1, 1, 1, 1), 1, 1)
+    runs = {}
+    lib = _lib.lib()
+    try:
+        for steps, graphs in ((1, 0), (2, 0), (2, 1), (2, 2)):
+            _lib.check(lib.qqcd_set_arnoldi_steps(steps))
+            _lib.check(lib.qqcd_set_step_graphs(graphs))
+            x = np.zeros_like(be)
+            it = inverters.solve_mixed(be, x, (40, 10), 1e-10, 2, 2, 12, 0.0, 0, u, co, kappa, BC)
+            runs[(steps, graphs)] = (it, x)
+    finally:
+        _lib.check(lib.qqcd_set_arnoldi_steps(2))
+        _lib.check(lib.qqcd_set_step_graphs(2))
+        lattice.finalize()
+    bb = O.spinor_ref_to_canon(N8, be.ravel(order="F"), None)
+    odd = O.parity_mask(N8)
+    for key, (it, x) in runs.items():
+        xm = O.spinor_ref_to_canon(N8, x.ravel(order="F"), None)
+        r = bb - O.schur(N8, U, BC, [1] * 4, [1] * 4, 0, kappa, xm)
+        assert np.linalg.norm(r[~odd]) / np.linalg.norm(bb[~odd]) < 1e-10, key
+        x1 = O.spinor_ref_to_canon(N8, runs[(1, 0)][1].ravel(order="F"), None)
+        assert np.linalg.norm(xm[~odd] - x1[~odd]) / np.linalg.norm(x1[~odd]) < 1e-8, key
+        assert abs(it - runs[(1, 0)][0]) <= 0.15 * runs[(1, 0)][0], (key, it, runs[(1, 0)][0])
+    for key in ((2, 1), (2, 2)):
+        assert runs[key][0] == runs[(2, 0)][0]
+        assert np.array_equal(runs[key][1], runs[(2, 0)][1]), key
