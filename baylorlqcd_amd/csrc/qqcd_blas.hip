@@ -620,7 +620,8 @@ __global__ void __launch_bounds__(RB) k_combine(long nu, int m, const void *cons
 // host side
 
 Blas::Blas(hipStream_t s) : stream(s) {
-  QQCD_HIP(hipMalloc(&part, sizeof(double) * RG * 2 * kmax));
+  // partials: RG rows of 2 per vector (multi_dot), 4 per vector (pair_dot_dev)
+  QQCD_HIP(hipMalloc(&part, sizeof(double) * RG * 4 * kmax));
   // one 2*KMAXP slab of results per pass of up to KMAXP vectors
   const long nres = 2L * KMAXP * ((kmax + KMAXP - 1) / KMAXP);
   QQCD_HIP(hipMalloc(&res, sizeof(double) * nres));
