@@ -500,56 +500,55 @@ __global__ void __launch_bounds__(MB) k_next2(MShape a, const void *w1, const vo
   slot_sum<1>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
 }
 
-// One thread per slot: column j of H from g1 and ||W1'||, column j+1 from the
-// Arnoldi relation A v_{j+1} = (A^2 v_j - V_{j+1} H_j g1) / nu (nu = ||W1'||),
-// and the coefficients of R = (W2' - alpha W1') / nu, alpha = <W1', W2'> / nu^2,
-// with W1' = W1 - V g1, W2' = W2 - V g2 as axpy2 left them (so R is the part
-// of A v_{j+1} orthogonal to v_0 .. v_{j+1}); H[j+2, j+1] = ||R|| follows
-// from next2 (set_subdiag).
+// One thread per (row k, slot): column j of H from g1 and nu = ||W1'||,
+// column j+1 from the Arnoldi relation A v_{j+1} = (A^2 v_j - V_{j+1} H_j g1)
+// / nu, and (row 0) the coefficients of R = (W2' - alpha W1') / nu, alpha =
+// <W1', W2'> / nu^2, with W1' = W1 - V g1, W2' = W2 - V g2 as axpy2 left them
+// (so R is the part of A v_{j+1} orthogonal to v_0 .. v_{j+1});
+// H[j+2, j+1] = ||R|| follows from next2 (set_subdiag).
 __global__ void k_sstep_columns(double *H, int ldh, int j, const double *g1, const double *g2, const double *sc,
                                 double *b, const int *nact, int cap, double hs) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= cap || r >= *nact) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t / cap, r = t - k * cap;
+  if (k > j + 1 || r >= *nact) return;
 #define HH(q, c) H[2 * (((long)(c) * ldh + (q)) * cap + r)]
 #define HI(q, c) H[2 * (((long)(c) * ldh + (q)) * cap + r) + 1]
   const double n1 = sc[r];
   const double nu = n1 > 0.0 ? sqrt(n1) : 0.0;
-  for (int q = 0; q <= j; q++) {
-    HH(q, j) = g1[2 * ((long)q * cap + r)];
-    HI(q, j) = g1[2 * ((long)q * cap + r) + 1];
+  // row k of column j: (g1, nu)
+  const double hjr = k <= j ? g1[2 * ((long)k * cap + r)] : nu, hji = k <= j ? g1[2 * ((long)k * cap + r) + 1] : 0.0;
+  HH(k, j) = hjr;
+  HI(k, j) = hji;
+  if (!(nu > 0.0)) {
+    HH(k, j + 1) = 0.0;
+    HI(k, j + 1) = 0.0;
+    if (k == 0) b[4 * r] = b[4 * r + 1] = b[4 * r + 2] = b[4 * r + 3] = 0.0;
+    return;
   }
-  HH(j + 1, j) = nu;
-  HI(j + 1, j) = 0.0;
-  if (nu > 0.0) {
-    const double inv = 1.0 / nu;
-    for (int k = 0; k <= j + 1; k++) {
-      // c_k = sum_i H[k, i] g1_i over the columns i >= k - 1 where H[k, i] can be nonzero
-      double cr = 0.0, ci = 0.0;
-      for (int i = k > 0 ? k - 1 : 0; i <= j; i++) {
-        const double hr = HH(k, i), hi = HI(k, i);
-        const double gr = g1[2 * ((long)i * cap + r)], gi = g1[2 * ((long)i * cap + r) + 1];
-        cr += hr * gr - hi * gi;
-        ci += hr * gi + hi * gr;
-      }
-      if (k <= j) {
-        HH(k, j + 1) = (g2[2 * ((long)k * cap + r)] - cr) * inv;
-        HI(k, j + 1) = (g2[2 * ((long)k * cap + r) + 1] - ci) * inv;
-      } else {
-        HH(k, j + 1) = (sc[cap + r] * inv - cr) * inv;
-        HI(k, j + 1) = (sc[2 * cap + r] * inv - ci) * inv;
-      }
-    }
+  const double inv = 1.0 / nu;
+  // c_k = sum_i H[k, i] g1_i over the columns i >= k - 1 where H[k, i] can be
+  // nonzero; column j from the values above (another thread writes it)
+  const double gjr = g1[2 * ((long)j * cap + r)], gji = g1[2 * ((long)j * cap + r) + 1];
+  double cr = hjr * gjr - hji * gji, ci = hjr * gji + hji * gjr;
+  for (int i = k > 0 ? k - 1 : 0; i < j; i++) {
+    const double hr = HH(k, i), hi = HI(k, i);
+    const double gr = g1[2 * ((long)i * cap + r)], gi = g1[2 * ((long)i * cap + r) + 1];
+    cr += hr * gr - hi * gi;
+    ci += hr * gi + hi * gr;
+  }
+  if (k <= j) {
+    HH(k, j + 1) = (g2[2 * ((long)k * cap + r)] - cr) * inv;
+    HI(k, j + 1) = (g2[2 * ((long)k * cap + r) + 1] - ci) * inv;
+  } else {
+    HH(k, j + 1) = (sc[cap + r] * inv - cr) * inv;
+    HI(k, j + 1) = (sc[2 * cap + r] * inv - ci) * inv;
+  }
+  if (k == 0) {
     const double ar = sc[cap + r] * inv * inv, ai = sc[2 * cap + r] * inv * inv;
     b[4 * r] = inv;
     b[4 * r + 1] = -ar * inv;
     b[4 * r + 2] = -ai * inv;
     b[4 * r + 3] = hs * inv;
-  } else {
-    for (int k = 0; k <= j + 1; k++) {
-      HH(k, j + 1) = 0.0;
-      HI(k, j + 1) = 0.0;
-    }
-    b[4 * r] = b[4 * r + 1] = b[4 * r + 2] = b[4 * r + 3] = 0.0;
   }
 #undef HH
 #undef HI
@@ -1018,7 +1017,8 @@ void clear_flags(int *flag, int *any, int cap, hipStream_t s) {
 
 void sstep_columns(double *H, int ldh, int j, const double *g1, const double *g2, const double *sc, double *b,
                    const int *nact, int cap, double hs, hipStream_t s) {
-  hipLaunchKernelGGL(k_sstep_columns, dim3((cap + 63) / 64), dim3(64), 0, s, H, ldh, j, g1, g2, sc, b, nact, cap, hs);
+  const int nt = (j + 2) * cap;
+  hipLaunchKernelGGL(k_sstep_columns, dim3((nt + 63) / 64), dim3(64), 0, s, H, ldh, j, g1, g2, sc, b, nact, cap, hs);
   QQCD_HIP(hipGetLastError());
 }
 

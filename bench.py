@@ -254,6 +254,7 @@ def bench_solve(method, kappa, op_prec=2, recon=12, mk=(40, 16), tol=1e-10):
         _lib.ints(mk).ctypes.data_as(_lib._ip), float(tol), int(op_prec), int(recon), 0.0, 0, _lib.dp(sec),
         its.ctypes.data_as(_lib._ip), outer.ctypes.data_as(_lib._ip), _lib.dp(res)))
     return {"seconds": round(float(sec[0]), 4), "iterations": int(its[0]), "outer": int(outer[0]),
+            "ms_per_iteration": round(1e3 * float(sec[0]) / max(1, int(its[0])), 4),
             "true_residual": float(f"{res[0]:.3e}")}
 
 

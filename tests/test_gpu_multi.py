@@ -315,6 +315,29 @@ def test_fermprop_multi_normalisation_in_operator(cfg1):
         assert np.array_equal(sols[(3, 1, 1)][r], sols[(3, 1, 0)][r]), r
 
 
+@pytest.mark.parametrize("nsrc,gm", [(2, (40, 10)), (6, (41, 10))])
+def test_fermprop_multi_unpaired_shapes(cfg1, nsrc, gm):
+    # shapes where the batched solver cannot pair its Arnoldi steps: one
+    # batched source (RHS block 1: no two-slot operator) and an odd cycle
+    # length (m - k = 31); every source still reaches the true residual
+    n, bc, U, u, co, g = cfg1
+    kappa = (0.150, 0.0)
+    srcs = []
+    for r in range(nsrc):
+        be = lattice.new_spinor(n).ravel(order="F")
+        bo = lattice.new_spinor(n).ravel(order="F")
+        disconloops.z4source(be, bo, 55, r, 1.0)
+        srcs.append((be, bo))
+    disconloops.batch_set(basis_prec=2)
+    print(f"[multi] unpaired shapes: {nsrc} sources, GMRES {gm}", flush=True)
+    xe, xo, it, rr = disconloops.fermprop_multi([s[0] for s in srcs], [s[1] for s in srcs], kappa, co, bc, 1e-10,
+                                                gm, u)
+    for r, (be, bo) in enumerate(srcs):
+        b = O.spinor_ref_to_canon(n, be, bo)
+        x = O.spinor_ref_to_canon(n, xe[r], xo[r])
+        assert full_rel_res(n, bc, U, kappa, b, x) < 1e-10, r
+
+
 def test_noise_loop_traces_and_graph(cfg1):
     n, bc, U, u, co, g = cfg1
     kappa = (0.150, 0.0)
