@@ -67,7 +67,7 @@ template <typename T> struct HostBuf {
   size_t n = 0;
   explicit HostBuf(size_t n_) : n(n_) { QQCD_HIP(hipHostMalloc((void **)&p, n * sizeof(T), hipHostMallocDefault)); }
   ~HostBuf() {
-    if (p) hipHostFree(p);
+    if (p) (void)hipHostFree(p);
   }
   T &operator[](size_t i) { return p[i]; }
 };
@@ -79,7 +79,7 @@ template <typename T> struct DevBuf {
     QQCD_HIP(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
   }
   ~DevBuf() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   }
 };
 
@@ -87,8 +87,8 @@ struct Graph {
   hipGraph_t g = nullptr;
   hipGraphExec_t e = nullptr;
   ~Graph() {
-    if (e) hipGraphExecDestroy(e);
-    if (g) hipGraphDestroy(g);
+    if (e) (void)hipGraphExecDestroy(e);
+    if (g) (void)hipGraphDestroy(g);
   }
 };
 
@@ -125,7 +125,7 @@ class Batch {
   std::unique_ptr<MultiBuf> U2f;
   // device state: ints [0] n_outer, [1] n_inner, [2] any, [3, 3+cap) flags
   DevBuf<int> di;
-  DevBuf<double> h1, h2, n2a, n2b, nf, H, coef, coef2, n2, ssc, sbc;
+  DevBuf<double> h1, h2, n2a, n2b, nf, H, coef, coef2, n2, ssc, sbc, sself;
   HostBuf<double> hH, hc, hcoef, hcoef2, hn2;
   std::vector<double> bn, rn, tgt;  // per slot: ||b||, current ||r||, inner target
   BatchStats st;
@@ -188,6 +188,7 @@ Batch::Batch(Context &c_, const double kappa[2], MultiSpinor &B_, MultiSpinor &X
       n2(B_.cap),
       ssc((size_t)4 * B_.cap),
       sbc((size_t)4 * B_.cap),
+      sself((size_t)4 * B_.cap),
       hH((size_t)2 * p_.m * (p_.m + 1) * B_.cap),
       hc((size_t)2 * (p_.m + 1) * B_.cap),
       hcoef((size_t)2 * (p_.m + 1) * B_.cap),
@@ -294,10 +295,17 @@ void Batch::arnoldi_block(int j) {
   if (j == 0) A32(rin, u1, nin);
   else A32.scaled(rin, u1, nin, nf.p, &V[j], hs);
   A32(u1, u2, nin);
-  mb.dot2(j + 1, V.data(), u1, u2, nin, h1.p, h2.p);
-  mb.axpy2(j + 1, V.data(), h1.p, h2.p, u1, u2, nin, ssc.p);
-  sstep_columns(H.p, m + 1, j, h1.p, h2.p, ssc.p, sbc.p, nin, cap, hs, c.stream);
-  mb.next2(u1, u2, rin, V[j + 1], sbc.p, nin, nf.p);
+  if (p.pair_norms) {
+    mb.dot2(j + 1, V.data(), u1, u2, nin, h1.p, h2.p, sself.p);
+    pythag_scalars(j + 1, h1.p, h2.p, sself.p, ssc.p, nin, cap, c.stream);
+    sstep_columns(H.p, m + 1, j, h1.p, h2.p, ssc.p, sbc.p, nin, cap, hs, c.stream);
+    mb.axpy2_next(j + 1, V.data(), h1.p, h2.p, u1, u2, sbc.p, rin, V[j + 1], nin, nf.p);
+  } else {
+    mb.dot2(j + 1, V.data(), u1, u2, nin, h1.p, h2.p);
+    mb.axpy2(j + 1, V.data(), h1.p, h2.p, u1, u2, nin, ssc.p);
+    sstep_columns(H.p, m + 1, j, h1.p, h2.p, ssc.p, sbc.p, nin, cap, hs, c.stream);
+    mb.next2(u1, u2, rin, V[j + 1], sbc.p, nin, nf.p);
+  }
   set_subdiag(H.p, m + 1, j + 1, nf.p, nin, cap, c.stream);
 }
 
@@ -321,8 +329,8 @@ void Batch::arnoldi_cycle() {
         run_steps();
       } catch (...) {
         hipGraph_t g = nullptr;
-        hipStreamEndCapture(c.stream, &g);
-        if (g) hipGraphDestroy(g);
+        (void)hipStreamEndCapture(c.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
         throw;
       }
       QQCD_HIP(hipStreamEndCapture(c.stream, &graph.g));
@@ -575,14 +583,14 @@ TuneResult autotune_multi(Context &c, Prec p, int recon, int nrhs) {
       }
     }
   } catch (...) {
-    hipFree(d_n);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipFree(d_n);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     throw;
   }
-  hipFree(d_n);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipFree(d_n);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   cache[key] = best;
   return best;
 }

@@ -63,6 +63,11 @@ struct BatchParams {
   // FP16 basis, the normalising operator, one Gram-Schmidt pass and an even
   // m allow it; 1 one step at a time
   int sstep = 2;
+  // paired steps: 1 takes the norm and inner product of the updated vectors
+  // from those of u1, u2 (formed in the pass of dots) less the projections,
+  // so the update pass also writes the next basis vector and operator input;
+  // 0 computes them from the updated vectors, with a third pass for those two
+  int pair_norms = 1;
   MultiTune tune;
 };
 

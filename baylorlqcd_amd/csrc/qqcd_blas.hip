@@ -631,11 +631,11 @@ Blas::Blas(hipStream_t s) : stream(s) {
 }
 
 Blas::~Blas() {
-  hipFree(part);
-  hipFree(res);
-  hipHostFree(hres);
-  hipFree(zbuf);
-  hipFree(vptr);
+  (void)hipFree(part);
+  (void)hipFree(res);
+  (void)hipHostFree(hres);
+  (void)hipFree(zbuf);
+  (void)hipFree(vptr);
 }
 
 void Blas::fetch(int nq) {

@@ -111,7 +111,7 @@ extern "C" int qqcd_gauge_load_canonical(const double *U, const double *coact) {
   if (!c.U64.data) c.U64 = alloc_gauge(c.g, PREC_DOUBLE, 18);
   canon_to_gauge(c.g, d, c.U64, c.stream);
   QQCD_HIP(hipStreamSynchronize(c.stream));
-  hipFree(d);
+  (void)hipFree(d);
   c.u_host = nullptr;
   c.have_gauge = 1;
   c.gauge_gen++;
@@ -395,8 +395,8 @@ extern "C" int qqcd_bench_dslash(int prec, int recon, int which, int nwarm, int 
   QQCD_HIP(hipEventSynchronize(e1));
   float t = 0;
   QQCD_HIP(hipEventElapsedTime(&t, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   *ms = (double)t / niter / (which == 0 ? 1 : 2);
   // accuracy against FP64 / 18 on the same (FP64) input
   apply(c.U64, in64.s, ref.s, tmp64.s);
@@ -437,8 +437,8 @@ extern "C" int qqcd_bench_dslash_rot(int prec, int recon, int nbuf, int nwarm, i
   QQCD_HIP(hipEventSynchronize(e1));
   float t = 0;
   QQCD_HIP(hipEventElapsedTime(&t, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   *ms = (double)t / niter;
   API_END
 }
@@ -465,12 +465,12 @@ extern "C" int qqcd_hop_canonical(const double *psi, double *out, int par_out, i
     QQCD_HIP(hipMemcpyAsync(out, dout, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     QQCD_HIP(hipStreamSynchronize(c.stream));
   } catch (...) {
-    hipFree(dp);
-    hipFree(dout);
+    (void)hipFree(dp);
+    (void)hipFree(dout);
     throw;
   }
-  hipFree(dp);
-  hipFree(dout);
+  (void)hipFree(dp);
+  (void)hipFree(dout);
   API_END
 }
 
@@ -492,12 +492,12 @@ extern "C" int qqcd_schur_canonical(const double *psi, double *out, const double
     QQCD_HIP(hipMemcpyAsync(out, dout, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     QQCD_HIP(hipStreamSynchronize(c.stream));
   } catch (...) {
-    hipFree(dp);
-    hipFree(dout);
+    (void)hipFree(dp);
+    (void)hipFree(dout);
     throw;
   }
-  hipFree(dp);
-  hipFree(dout);
+  (void)hipFree(dp);
+  (void)hipFree(dout);
   API_END
 }
 
@@ -530,7 +530,7 @@ extern "C" int qqcd_gauge_get_canonical(double *U) {
   gauge_to_canon(c.g, c.U64, d, c.stream);
   QQCD_HIP(hipMemcpyAsync(U, d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   QQCD_HIP(hipStreamSynchronize(c.stream));
-  hipFree(d);
+  (void)hipFree(d);
   API_END
 }
 
@@ -548,7 +548,7 @@ extern "C" int qqcd_gauge_get_ref(double *u) {
   gauge_to_ref(c.g, c.U64, d, c.stream);
   QQCD_HIP(hipMemcpyAsync(u, d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   QQCD_HIP(hipStreamSynchronize(c.stream));
-  hipFree(d);
+  (void)hipFree(d);
   API_END
 }
 

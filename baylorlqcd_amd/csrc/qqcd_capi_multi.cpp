@@ -41,7 +41,7 @@ struct DevArr {
     else QQCD_HIP(hipMemsetAsync(p, 0, n * sizeof(double), s));
   }
   ~DevArr() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   }
 };
 
@@ -108,10 +108,10 @@ void multi_canonical(int which, int nrhs, int nact, int nb, int prec, int recon,
     }
     QQCD_HIP(hipStreamSynchronize(c.stream));
   } catch (...) {
-    hipFree(d_n);
+    (void)hipFree(d_n);
     throw;
   }
-  hipFree(d_n);
+  (void)hipFree(d_n);
 }
 
 BatchParams batch_params(const int GMRES[2], double tol, int nb_rhs) {
@@ -174,7 +174,7 @@ struct DevU32 {
   uint32_t *p = nullptr;
   explicit DevU32(size_t n) { QQCD_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(uint32_t))); }
   ~DevU32() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   }
 };
 
@@ -366,6 +366,13 @@ extern "C" int qqcd_set_arnoldi_steps(int steps) {
   API_BEGIN
   set_arnoldi_steps(steps);  // the single-RHS solvers (checks the value)
   g_bp.sstep = steps;        // the batched solver
+  API_END
+}
+
+extern "C" int qqcd_set_pair_norms(int mode) {
+  API_BEGIN
+  if (mode != 0 && mode != 1) throw std::runtime_error("pair norms: 0 (from the updated vectors) or 1 (from the projections)");
+  g_bp.pair_norms = mode;
   API_END
 }
 
@@ -822,13 +829,13 @@ extern "C" int qqcd_bench_dslash_multi(int prec, int recon, int nrhs, int nact, 
     QQCD_HIP(hipEventElapsedTime(&f, e0, e1));
     *ms = f / std::max(niter, 1);
   } catch (...) {
-    hipFree(d_n);
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
+    (void)hipFree(d_n);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     throw;
   }
-  hipFree(d_n);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipFree(d_n);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   API_END
 }

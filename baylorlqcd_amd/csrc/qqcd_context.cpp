@@ -39,8 +39,8 @@ Spinor alloc_spinor(const Geom &g, Prec p) {
 }
 
 void free_spinor(Spinor &s) {
-  if (s.data) hipFree(s.data);
-  if (s.norm) hipFree(s.norm);
+  if (s.data) (void)hipFree(s.data);
+  if (s.norm) (void)hipFree(s.norm);
   s.data = nullptr;
   s.norm = nullptr;
 }
@@ -56,18 +56,18 @@ Gauge alloc_gauge(const Geom &g, Prec p, int recon) {
 }
 
 void free_gauge(Gauge &u) {
-  if (u.data) hipFree(u.data);
+  if (u.data) (void)hipFree(u.data);
   u.data = nullptr;
 }
 
 Context::~Context() {
-  if (stream) hipStreamSynchronize(stream);
+  if (stream) (void)hipStreamSynchronize(stream);
   for (auto &kv : copies) free_gauge(kv.second.U);
   free_gauge(U64);
   for (double *&p : dref)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   blas.reset();
-  if (stream) hipStreamDestroy(stream);
+  if (stream) (void)hipStreamDestroy(stream);
 }
 
 // Lower-precision or 12-real copies of the master field, rebuilt lazily.
@@ -87,7 +87,7 @@ const Gauge &Context::gauge(Prec p, int recon) {
     gauge_to_canon(g, U64, tmp, stream);
     canon_to_gauge(g, tmp, cp.U, stream);
     QQCD_HIP(hipStreamSynchronize(stream));
-    hipFree(tmp);
+    (void)hipFree(tmp);
     cp.gen = gauge_gen;
   }
   return cp.U;
@@ -128,14 +128,14 @@ void ensure_gauge(Context &c, const double *u_ref, const double *coact) {
   if (!c.U64.data) c.U64 = alloc_gauge(c.g, PREC_DOUBLE, 18);
   ref_to_gauge(c.g, d, c.U64, c.stream);
   QQCD_HIP(hipStreamSynchronize(c.stream));
-  hipFree(d);
+  (void)hipFree(d);
   c.u_host = u_ref;
   c.have_gauge = 1;
   c.gauge_gen++;
 }
 
 void schur_apply(Context &c, const Gauge &U, const double kappa[2], int idag, const Spinor &in, Spinor &out,
-                 Spinor &tmp_odd) {
+                 Spinor &tmp_odd, Spinor *out32, bool only32) {
   const double k = kappa[0], mu = kappa[1];
   Epilogue e1, e2;
   if (mu < 1e-8) {
@@ -150,7 +150,7 @@ void schur_apply(Context &c, const Gauge &U, const double kappa[2], int idag, co
     e2.b0 = -1.0 / (1.0 + a * a);
   }
   dslash(c.g, U, in, tmp_odd, 1, idag, e1, nullptr, c.stream);
-  dslash(c.g, U, tmp_odd, out, 0, idag, e2, &in, c.stream);
+  dslash(c.g, U, tmp_odd, out, 0, idag, e2, &in, c.stream, out32, only32);
 }
 
 }  // namespace qqcd

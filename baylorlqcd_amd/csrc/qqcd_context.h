@@ -66,8 +66,10 @@ void ensure_gauge(Context &c, const double *u_ref, const double *coact);
 //   Wilson   (mu < 1e-8): out = in/k^2 - H_eo H_oe in
 //   twisted  (a=2 k mu):  out = (1 + a i g5) in/k^2 - H_eo (1 - a i g5) H_oe in/(1+a^2)
 // (Hdouble / Hdbletm, qqcd-test/cfgsprops/quark/diracops.f90:264-346, 435-550)
+// out32 (FP16 spinors): the result also as FP32, from the same launch where
+// the pipelined kernel runs (dslash's out32 / only32).
 void schur_apply(Context &c, const Gauge &U, const double kappa[2], int idag, const Spinor &in, Spinor &out,
-                 Spinor &tmp_odd);
+                 Spinor &tmp_odd, Spinor *out32 = nullptr, bool only32 = false);
 
 void canon_to_spinor(const Geom &g, const double *dcn, Spinor &out, int par, hipStream_t s);
 void spinor_to_canon(const Geom &g, const Spinor &in, double *dcn, int par, hipStream_t s);

@@ -218,6 +218,10 @@ struct DslashArgs {
   const double *slot_n2;
   void *side;
   float side_hs;
+  // single right-hand side, FP16 spinors: the result also in FP32 (out32, no
+  // norms), and skip16 leaves out / out_norm unwritten
+  void *out32;
+  int skip16;
 };
 
 // Pointers and strides a hop reads (scalars only, so that nothing of the
@@ -722,17 +726,33 @@ __device__ __forceinline__ void dslash_site_pipe(const DslashArgs &a, const HopI
   const uint32_t oo = (uint32_t)io * (uint32_t)sizeof(SE);
   const __amdgpu_buffer_rsrc_t ors = make_rsrc(out, 12 * srun);
   if constexpr (std::is_same<S, f16>::value) {
-    float m = 0.f;
+    if (MULTI || !a.skip16) {
+      float m = 0.f;
 #pragma unroll
-    for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf(o[k].re), fabsf(o[k].im)));
-    bst<float, AUXS>(m, make_rsrc(out_norm, 4u * (uint32_t)h.sstride), (uint32_t)io * 4u, 0);
-    const float inv = m > 0.f ? 1.f / m : 0.f;
+      for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf(o[k].re), fabsf(o[k].im)));
+      bst<float, AUXS>(m, make_rsrc(out_norm, 4u * (uint32_t)h.sstride), (uint32_t)io * 4u, 0);
+      const float inv = m > 0.f ? 1.f / m : 0.f;
 #pragma unroll
-    for (int k = 0; k < 12; k++) {
-      SE w;
-      w.x = (f16)(o[k].re * inv);
-      w.y = (f16)(o[k].im * inv);
-      bst<SE, AUXS>(w, ors, oo, k * srun);
+      for (int k = 0; k < 12; k++) {
+        SE w;
+        w.x = (f16)(o[k].re * inv);
+        w.y = (f16)(o[k].im * inv);
+        bst<SE, AUXS>(w, ors, oo, k * srun);
+      }
+    }
+    if constexpr (!MULTI) {
+      if (a.out32) {
+        typedef typename V2<float>::T FE;
+        const uint32_t frun = (uint32_t)h.sstride * (uint32_t)sizeof(FE);
+        const __amdgpu_buffer_rsrc_t frs = make_rsrc(a.out32, 12 * frun);
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+          FE w;
+          w.x = o[k].re;
+          w.y = o[k].im;
+          bst<FE, AUXS>(w, frs, (uint32_t)io * (uint32_t)sizeof(FE), k * frun);
+        }
+      }
     }
   } else {
 #pragma unroll
@@ -1169,7 +1189,7 @@ static void launch_pipe(const DslashArgs &a, int depth, int nt, int nb, hipStrea
 }
 
 template <typename S, typename L, int RECON>
-static void launch_t(const DslashArgs &a0, int sig, hipStream_t s) {
+static bool launch_t(const DslashArgs &a0, int sig, hipStream_t s) {
   const int nb = (a0.Vh + 255) / 256;
   const DslashTune &t = g_tune;
   // the buffer descriptors of the pipelined kernel cover < 4 GiB each: a
@@ -1191,13 +1211,17 @@ static void launch_t(const DslashArgs &a0, int sig, hipStream_t s) {
     a.bz = a.order ? bz : 1;
     if (sig > 0) launch_pipe<S, L, RECON, 1>(a, t.depth, t.nt, nb, s);
     else launch_pipe<S, L, RECON, -1>(a, t.depth, t.nt, nb, s);
-  } else if (sig > 0)
+    QQCD_HIP(hipGetLastError());
+    return true;
+  }
+  if (sig > 0)
     hipLaunchKernelGGL((k_dslash<S, L, RECON, 1>), dim3(nb), dim3(256), 0, s, a0);
   else if (sig < 0)
     hipLaunchKernelGGL((k_dslash<S, L, RECON, -1>), dim3(nb), dim3(256), 0, s, a0);
   else
     hipLaunchKernelGGL((k_dslash<S, L, RECON, 0>), dim3(nb), dim3(256), 0, s, a0);
   QQCD_HIP(hipGetLastError());
+  return false;
 }
 
 // resident workgroups of a persistent kernel: occupancy x CUs, a multiple of 8
@@ -1370,12 +1394,15 @@ static void dispatch_prec(Prec sp, const Gauge &U, const char *who, F &&f) {
 }
 
 void dslash(const Geom &g, const Gauge &U, const Spinor &in, Spinor &out, int par_out, int idag,
-            const Epilogue &ep, const Spinor *x, hipStream_t s) {
+            const Epilogue &ep, const Spinor *x, hipStream_t s, Spinor *out32, bool only32) {
   if (in.prec != out.prec || (x && x->prec != out.prec))
     throw std::runtime_error("dslash: spinor precisions differ");
   if (in.stride != g.stride || out.stride != g.stride || U.stride != g.stride)
     throw std::runtime_error("dslash: field stride does not match the geometry");
   if (in.data == out.data) throw std::runtime_error("dslash: in and out alias");
+  if (out32 && (out.prec != PREC_HALF || out32->prec != PREC_SINGLE || out32->stride != g.stride ||
+                out32->data == in.data || (x && out32->data == x->data)))
+    throw std::runtime_error("dslash: the FP32 copy of the result is for FP16 spinors, a separate FP32 field");
   DslashArgs a{};
   a.in = in.data;
   a.in_norm = in.norm;
@@ -1384,12 +1411,17 @@ void dslash(const Geom &g, const Gauge &U, const Spinor &in, Spinor &out, int pa
   a.x = x ? x->data : nullptr;
   a.x_norm = x ? x->norm : nullptr;
   a.has_x = x ? 1 : 0;
+  a.out32 = out32 ? out32->data : nullptr;
+  a.skip16 = out32 && only32 ? 1 : 0;
   const int proj = make_args(g, U, par_out, idag, ep, a);
+  bool piped = false;
   dispatch_prec(in.prec, U, "dslash", [&](auto *sp, auto *lp, auto rc) {
     typedef typename std::remove_pointer<decltype(sp)>::type S;
     typedef typename std::remove_pointer<decltype(lp)>::type L;
-    launch_t<S, L, decltype(rc)::value>(a, proj, s);
+    piped = launch_t<S, L, decltype(rc)::value>(a, proj, s);
   });
+  // the other kernels write the FP16 result only
+  if (out32 && !piped) spinor_copy_convert(g, out, *out32, s);
 }
 
 size_t MultiSpinor::bytes() const {

@@ -154,8 +154,11 @@ void set_error(const std::string &msg);
   } while (0)
 
 // ---- kernels (qqcd_dslash.hip) ----
+// out32 (FP16 spinors only): the result also as an FP32 field, in the same
+// launch where the pipelined kernel runs; only32 leaves out unwritten there
+// (out is then scratch for the other kernels).
 void dslash(const Geom &g, const Gauge &U, const Spinor &in, Spinor &out, int par_out, int idag,
-            const Epilogue &ep, const Spinor *x, hipStream_t s);
+            const Epilogue &ep, const Spinor *x, hipStream_t s, Spinor *out32 = nullptr, bool only32 = false);
 
 // Hopping operator on the active slots [0, *d_nact) of a multi-RHS field
 // (d_nact in device memory); same epilogue as dslash.  With slot_n2 (device,

@@ -379,13 +379,14 @@ __global__ void k_arnoldi_column(double *H, int ldh, int j, const double *h1, co
 // ---- two-step Arnoldi block (FP16 basis V, FP32 W1 = A v_j, W2 = A W1) ----
 // Per-lane sums in FP32 (the basis carries 11 bits), across lanes in FP64.
 
-// part[((c * cap + slot) * gx + bx]: c < 2K <V_k, W1>, then 2K + c <V_k, W2>
-template <int K>
+// part[((c * cap + slot) * gx + bx]: c < 2K <V_k, W1>, then 2K + c <V_k, W2>;
+// SELF: then ||W1||^2, Re <W1, W2>, Im <W1, W2>, ||W2||^2
+template <int K, bool SELF>
 __global__ void __launch_bounds__(MB) k_mdot2(MShape a, MPtr v, const void *w1, const void *w2, double *part) {
   const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
   if ((blk << a.lognb) >= nact) return;
   const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
-  constexpr int NV = 4 * K;
+  constexpr int NV = 4 * K + (SELF ? 4 : 0);
   float af[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) af[k] = 0.f;
@@ -407,11 +408,17 @@ __global__ void __launch_bounds__(MB) k_mdot2(MShape a, MPtr v, const void *w1, 
         af[2 * K + 2 * k] += xx * b2[0] + xy * b2[1];
         af[2 * K + 2 * k + 1] += xx * b2[1] - xy * b2[0];
       }
+      if (SELF) {
+        af[4 * K] += b1[0] * b1[0] + b1[1] * b1[1];
+        af[4 * K + 1] += b1[0] * b2[0] + b1[1] * b2[1];
+        af[4 * K + 2] += b1[0] * b2[1] - b1[1] * b2[0];
+        af[4 * K + 3] += b2[0] * b2[0] + b2[1] * b2[1];
+      }
     }
   }
   const float hinv = (float)a.hinv;
 #pragma unroll
-  for (int k = 0; k < NV; k++) af[k] *= hinv;
+  for (int k = 0; k < 4 * K; k++) af[k] *= hinv;
   slot_sum<NV>(af, a.lognb, part, a.cap, blk << a.lognb, a.gx);
 }
 
@@ -464,6 +471,91 @@ __global__ void __launch_bounds__(MB) k_maxpy2(MShape a, MPtr v, const double *g
     }
   }
   if (LAST) slot_sum<4>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
+}
+
+// The last chunk of axpy2 with next2 folded in: W1' = W1 - sum_k g1_k V_k and
+// W2' likewise stay in registers, R = b0 W2' + (b1 + i b2) W1' (FP32) and
+// H = b3 W1' (FP16) are stored, with partials of ||R||^2 of the stored values.
+// The coefficients b come before the pass, from norms and inner products of
+// W1', W2' formed from those of W1, W2 and the projections (pythag_scalars).
+template <int K>
+__global__ void __launch_bounds__(MB) k_maxpy2n(MShape a, MPtr v, const double *g1, const double *g2, int q0,
+                                                const void *w1, const void *w2, void *rout, void *hout,
+                                                const double *b, double *part) {
+  const int nact = *a.nact, nb = 1 << a.lognb, blk = blockIdx.y;
+  if ((blk << a.lognb) >= nact) return;
+  const int rr = threadIdx.x & (nb - 1), slot = (blk << a.lognb) + rr;
+  double acc[1] = {0.0};
+  if (slot < nact) {
+    float c1r[K], c1i[K], c2r[K], c2i[K];
+    const double f = a.hinv;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const long o = 2 * ((long)(q0 + k) * a.cap + slot);
+      c1r[k] = (float)(f * g1[o]);
+      c1i[k] = (float)(f * g1[o + 1]);
+      c2r[k] = (float)(f * g2[o]);
+      c2i[k] = (float)(f * g2[o + 1]);
+    }
+    const float s2 = (float)b[4 * slot], s1r = (float)b[4 * slot + 1], s1i = (float)b[4 * slot + 2],
+                sv = (float)b[4 * slot + 3];
+    const int per = MB >> a.lognb;
+    const long base = (long)blk * a.blk_elems;
+    typedef typename E2<float>::raw f2;
+    typedef typename E2<f16>::raw h2;
+    float nr = 0.f;
+    for (long p = (long)blockIdx.x * per + (threadIdx.x >> a.lognb); p < a.rows; p += (long)a.gx * per) {
+      const long e = base + (p << a.lognb) + rr;
+      f2 b1 = E2<float>::ldr(w1, e), b2 = E2<float>::ldr(w2, e);
+      typename E2<f16>::raw xr[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) xr[k] = E2<f16>::ldr(v.p[k], e);
+      pin_loads(xr);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const float xx = (float)xr[k][0], xy = (float)xr[k][1];
+        b1[0] -= c1r[k] * xx - c1i[k] * xy;
+        b1[1] -= c1r[k] * xy + c1i[k] * xx;
+        b2[0] -= c2r[k] * xx - c2i[k] * xy;
+        b2[1] -= c2r[k] * xy + c2i[k] * xx;
+      }
+      f2 r;
+      r[0] = s2 * b2[0] + (s1r * b1[0] - s1i * b1[1]);
+      r[1] = s2 * b2[1] + (s1r * b1[1] + s1i * b1[0]);
+      ((f2 *)rout)[e] = r;
+      h2 hv;
+      hv[0] = (f16)(sv * b1[0]);
+      hv[1] = (f16)(sv * b1[1]);
+      ((h2 *)hout)[e] = hv;
+      nr += r[0] * r[0] + r[1] * r[1];
+    }
+    acc[0] = (double)nr;
+  }
+  slot_sum<1>(acc, a.lognb, part, a.cap, blk << a.lognb, a.gx);
+}
+
+// One thread per slot: sc = (||W1'||^2, Re <W1', W2'>, Im <W1', W2'>,
+// ||W2'||^2) of W1' = W1 - V g1, W2' = W2 - V g2 from self = the same four of
+// W1, W2 and the projections g = <V_k, W> (V orthonormal: ||W1'||^2 = ||W1||^2
+// - sum |g1_k|^2, <W1', W2'> = <W1, W2> - sum conj(g1_k) g2_k).
+__global__ void k_pythag_scalars(int nq, const double *g1, const double *g2, const double *self, double *sc,
+                                 const int *nact, int cap) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= cap || r >= *nact) return;
+  double n1 = 0.0, cr = 0.0, ci = 0.0, n2 = 0.0;
+  for (int q = 0; q < nq; q++) {
+    const long o = 2 * ((long)q * cap + r);
+    const double ar = g1[o], ai = g1[o + 1], br = g2[o], bi = g2[o + 1];
+    n1 += ar * ar + ai * ai;
+    n2 += br * br + bi * bi;
+    cr += ar * br + ai * bi;
+    ci += ar * bi - ai * br;
+  }
+  const double p1 = self[r] - n1;
+  sc[r] = p1 > 0.0 ? p1 : 0.0;
+  sc[cap + r] = self[cap + r] - cr;
+  sc[2 * cap + r] = self[2 * cap + r] - ci;
+  sc[3 * cap + r] = self[3 * cap + r] - n2;
 }
 
 // R = b0 W2 + (b1 + i b2) W1 in FP32, H = b3 W1 in FP16 (b per slot, 4 doubles),
@@ -587,15 +679,15 @@ MultiSpinor alloc_multi(const Geom &g, Prec p, int cap, int nb, bool norms) {
 }
 
 void free_multi(MultiSpinor &m) {
-  if (m.data) hipFree(m.data);
-  if (m.norm) hipFree(m.norm);
+  if (m.data) (void)hipFree(m.data);
+  if (m.norm) (void)hipFree(m.norm);
   m.data = nullptr;
   m.norm = nullptr;
 }
 
 MBlas::MBlas(hipStream_t s, int cap_) : stream(s), cap(cap_) {}
 MBlas::~MBlas() {
-  if (part) hipFree(part);
+  if (part) (void)hipFree(part);
 }
 
 void MBlas::reserve(size_t n) {
@@ -603,7 +695,7 @@ void MBlas::reserve(size_t n) {
   // only outside graph capture: the first call of each shape sizes the buffer
   if (part) {
     QQCD_HIP(hipStreamSynchronize(stream));
-    hipFree(part);
+    (void)hipFree(part);
   }
   QQCD_HIP(hipMalloc(&part, n * sizeof(double)));
   part_cap = n;
@@ -619,7 +711,7 @@ static int gx_of(long rows, int nb, int nblk) {
 // (MAXK complex dots and a norm, or MAXK2 complex dots with each of two
 // fields): sized once, before any capture, so that no later pass moves the
 // buffer a captured graph names
-size_t MBlas::need() const { return (size_t)2048 * std::max(2 * MAXK + 2, 4 * MAXK2) * cap + 64; }
+size_t MBlas::need() const { return (size_t)2048 * std::max(2 * MAXK + 2, 4 * MAXK2 + 4) * cap + 64; }
 
 static MShape shape(const MultiSpinor &W, const int *nact) {
   MShape a;
@@ -884,28 +976,81 @@ static void basis_fields(int nq, const MultiSpinor *V, const MultiSpinor &W, std
 }
 
 void MBlas::dot2(int nq, const MultiSpinor *V, const MultiSpinor &W1, const MultiSpinor &W2, const int *nact,
-                 double *g1, double *g2) {
+                 double *g1, double *g2, double *self) {
   if (W1.prec != PREC_SINGLE || W2.prec != PREC_SINGLE) throw std::runtime_error("MBlas::dot2: FP32 W1, W2");
   same_layout2(W1, W2, "MBlas::dot2");
   std::vector<const void *> vp;
   basis_fields(nq, V, W1, vp, "MBlas::dot2");
   const MShape a = shape(W1, nact);
-  reserve((size_t)a.gx * 4 * MAXK2 * W1.cap);
+  reserve((size_t)a.gx * (4 * MAXK2 + 4) * W1.cap);
   for (int q0 = 0, K; q0 < nq; q0 += K) {
     K = chunk_len(nq, q0, MAXK2);
+    const bool with_self = self && q0 == 0;
     MPtr pk{};
     for (int k = 0; k < K; k++) pk.p[k] = vp[q0 + k];
     dim3 grid(a.gx, W1.nblk());
     with_k<MAXK2>(K, [&](auto kk) {
-      hipLaunchKernelGGL((k_mdot2<decltype(kk)::value>), grid, dim3(MB), 0, stream, a, pk, W1.data, W2.data, part);
+      constexpr int KK = decltype(kk)::value;
+      if (with_self)
+        hipLaunchKernelGGL((k_mdot2<KK, true>), grid, dim3(MB), 0, stream, a, pk, W1.data, W2.data, part);
+      else
+        hipLaunchKernelGGL((k_mdot2<KK, false>), grid, dim3(MB), 0, stream, a, pk, W1.data, W2.data, part);
     });
     QQCD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_mfinal, dim3(W1.cap, 2 * K), dim3(64), 0, stream, 2 * K, W1.cap, a.gx, nact,
                        (const double *)part, g1, q0, 1);
     hipLaunchKernelGGL(k_mfinal, dim3(W1.cap, 2 * K), dim3(64), 0, stream, 2 * K, W1.cap, a.gx, nact,
                        (const double *)part + (size_t)2 * K * W1.cap * a.gx, g2, q0, 1);
+    if (with_self)
+      hipLaunchKernelGGL(k_mfinal, dim3(W1.cap, 4), dim3(64), 0, stream, 4, W1.cap, a.gx, nact,
+                         (const double *)part + (size_t)4 * K * W1.cap * a.gx, self, 0, 0);
     QQCD_HIP(hipGetLastError());
   }
+}
+
+void MBlas::axpy2_next(int nq, const MultiSpinor *V, const double *g1, const double *g2, MultiSpinor &W1,
+                       MultiSpinor &W2, const double *b, MultiSpinor &R, MultiSpinor &H, const int *nact, double *nr) {
+  if (W1.prec != PREC_SINGLE || W2.prec != PREC_SINGLE || R.prec != PREC_SINGLE)
+    throw std::runtime_error("MBlas::axpy2_next: FP32 W1, W2, R");
+  if (W1.data == W2.data) throw std::runtime_error("MBlas::axpy2_next: W1 and W2 alias");
+  if (!basis16(H)) throw std::runtime_error("MBlas::axpy2_next: H must be an FP16 basis field");
+  same_layout2(W1, W2, "MBlas::axpy2_next");
+  same_layout2(W1, R, "MBlas::axpy2_next");
+  same_layout2(W1, H, "MBlas::axpy2_next");
+  if (R.data == W1.data || R.data == W2.data) throw std::runtime_error("MBlas::axpy2_next: R aliases W1 or W2");
+  if (nq < 1) throw std::runtime_error("MBlas::axpy2_next: at least one basis field");
+  std::vector<const void *> vp;
+  basis_fields(nq, V, W1, vp, "MBlas::axpy2_next");
+  for (int q = 0; q < nq; q++)
+    if (vp[q] == H.data) throw std::runtime_error("MBlas::axpy2_next: H is one of the basis fields read");
+  const MShape a = shape(W1, nact);
+  reserve((size_t)a.gx * W1.cap);
+  for (int q0 = 0, K; q0 < nq; q0 += K) {
+    K = chunk_len(nq, q0, MAXK2);
+    const bool last = q0 + K >= nq;
+    MPtr pk{};
+    for (int k = 0; k < K; k++) pk.p[k] = vp[q0 + k];
+    dim3 grid(a.gx, W1.nblk());
+    with_k<MAXK2>(K, [&](auto kk) {
+      constexpr int KK = decltype(kk)::value;
+      if (last)
+        hipLaunchKernelGGL((k_maxpy2n<KK>), grid, dim3(MB), 0, stream, a, pk, g1, g2, q0, W1.data, W2.data, R.data,
+                           H.data, b, part);
+      else
+        hipLaunchKernelGGL((k_maxpy2<KK, false>), grid, dim3(MB), 0, stream, a, pk, g1, g2, q0, W1.data, W2.data,
+                           part);
+    });
+    QQCD_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_mfinal, dim3(W1.cap, 1), dim3(64), 0, stream, 1, W1.cap, a.gx, nact, (const double *)part, nr,
+                     0, 0);
+  QQCD_HIP(hipGetLastError());
+}
+
+void pythag_scalars(int nq, const double *g1, const double *g2, const double *self, double *sc, const int *nact,
+                    int cap, hipStream_t s) {
+  hipLaunchKernelGGL(k_pythag_scalars, dim3((cap + 63) / 64), dim3(64), 0, s, nq, g1, g2, self, sc, nact, cap);
+  QQCD_HIP(hipGetLastError());
 }
 
 void MBlas::axpy2(int nq, const MultiSpinor *V, const double *g1, const double *g2, MultiSpinor &W1,

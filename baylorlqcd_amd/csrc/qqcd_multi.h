@@ -60,9 +60,11 @@ class MBlas {
   void scale_inv_norm2(const MultiSpinor &X, MultiSpinor &Q, MultiSpinor &H, const double *n2, const int *nact);
   // Two-vector passes of the two-step Arnoldi block (FP32 W1, W2; FP16
   // basis fields V): g1[q*cap + r] = <V_q(r), W1(r)>, g2[...] = <V_q(r),
-  // W2(r)> in one pass over V (per-lane sums in FP32, across lanes in FP64).
+  // W2(r)> in one pass over V (per-lane sums in FP32, across lanes in FP64);
+  // self (optional): self[c*cap + r] = ||W1||^2, Re <W1, W2>, Im <W1, W2>,
+  // ||W2||^2 for c = 0..3 from the same pass.
   void dot2(int nq, const MultiSpinor *V, const MultiSpinor &W1, const MultiSpinor &W2, const int *nact, double *g1,
-            double *g2);
+            double *g2, double *self = nullptr);
   // W1(r) -= sum_q g1[q*cap + r] V_q(r), W2(r) -= sum_q g2[...] V_q(r) in one
   // pass over V; then sc[c*cap + r] = ||W1||^2, Re <W1, W2>, Im <W1, W2>,
   // ||W2||^2 for c = 0..3
@@ -72,6 +74,11 @@ class MBlas {
   // b[4r+3] W1(r) as an FP16 basis field; nr[r] = ||R(r)||^2
   void next2(const MultiSpinor &W1, const MultiSpinor &W2, MultiSpinor &R, MultiSpinor &H, const double *b,
              const int *nact, double *nr);
+  // axpy2 and next2 in one pass: the updated W1, W2 stay in registers (W1, W2
+  // keep the partial updates of earlier chunks when nq exceeds one pass), R
+  // and H are formed from them with the coefficients b, nr[r] = ||R(r)||^2.
+  void axpy2_next(int nq, const MultiSpinor *V, const double *g1, const double *g2, MultiSpinor &W1, MultiSpinor &W2,
+                  const double *b, MultiSpinor &R, MultiSpinor &H, const int *nact, double *nr);
   // swap slots a and b of W
   void swap(MultiSpinor &W, int a, int b);
   // single <-> slot, with precision conversion
@@ -107,6 +114,10 @@ void clear_flags(int *flag, int *any, int cap, hipStream_t s);
 // b receives the per-slot coefficients next2 needs.
 void sstep_columns(double *H, int ldh, int j, const double *g1, const double *g2, const double *sc, double *b,
                    const int *nact, int cap, double hs, hipStream_t s);
+// The four scalars of axpy2 before the pass, from those of W1, W2 (dot2's
+// self) and the projections g1, g2 over nq orthonormal basis fields.
+void pythag_scalars(int nq, const double *g1, const double *g2, const double *self, double *sc, const int *nact,
+                    int cap, hipStream_t s);
 // H[q, j] = sqrt(n2[r]) for one row q of column j (the subdiagonal entry after next2)
 void set_subdiag(double *H, int ldh, int j, const double *n2, const int *nact, int cap, hipStream_t s);
 void arnoldi_column(double *H, int ldh, int j, const double *h1, const double *h2, const int *flag,

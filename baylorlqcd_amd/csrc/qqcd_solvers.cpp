@@ -183,8 +183,12 @@ int arnoldi_steps() { return g_arnoldi_steps; }
 
 static void arnoldi_pair_dev(Context &c, LinOp &A, std::vector<Spinor> &V, int j, DevHess &dh, Spinor &w1,
                              Spinor &w2) {
-  A.apply(V[j], w1);
-  A.apply(w1, w2);
+  if (A.apply_sq) {
+    A.apply_sq(V[j], w1, w2);
+  } else {
+    A.apply(V[j], w1);
+    A.apply(w1, w2);
+  }
   Vec a1 = vv(w1), a2 = vv(w2);
   std::vector<Vec> Vb = vecs(V, j + 1);
   c.blas->pair_dot_dev(j + 1, Vb.data(), a1, a2, dh.col(j), dh.g2());
@@ -773,7 +777,9 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
     if (p.basis == PREC_HALF) o.store = PREC_HALF;
     in_lo.reset(new SpinorBuf(c.g, op_prec));
     out_lo.reset(new SpinorBuf(c.g, op_prec));
-    o.apply = [&](const Spinor &a, Spinor &r) {
+    // an FP16 operator writes an FP32 result itself (the hop's epilogue)
+    const bool direct32 = op_prec == PREC_HALF && work == PREC_SINGLE;
+    o.apply = [&, direct32](const Spinor &a, Spinor &r) {
       const Spinor *in = &a;
       if (a.prec != op_prec) {
         spinor_copy_convert(c.g, a, in_lo->s, c.stream);
@@ -781,11 +787,28 @@ MixedStats solve_mixed(Context &c, const double kappa[2], const Spinor &b, Spino
       }
       if (r.prec == op_prec) {
         (*Alo)(*in, r);
+      } else if (direct32 && r.prec == PREC_SINGLE) {
+        (*Alo)(*in, out_lo->s, &r, true);
       } else {
         (*Alo)(*in, out_lo->s);
         spinor_copy_convert(c.g, out_lo->s, r, c.stream);
       }
     };
+    if (direct32) {
+      // r1 = A a in FP32 and, rounded to FP16 by the same launch, the input
+      // of r2 = A r1: the operator reads its input in FP16, so r2 is A
+      // applied to r1 as the operator sees it
+      o.apply_sq = [&](const Spinor &a, Spinor &r1, Spinor &r2) {
+        if (r1.prec != PREC_SINGLE || r2.prec != PREC_SINGLE) throw std::runtime_error("apply_sq: FP32 results");
+        const Spinor *in = &a;
+        if (a.prec != op_prec) {
+          spinor_copy_convert(c.g, a, in_lo->s, c.stream);
+          in = &in_lo->s;
+        }
+        (*Alo)(*in, out_lo->s, &r1, false);
+        (*Alo)(out_lo->s, in_lo->s, &r2, true);
+      };
+    }
     return o;
   };
   LinOp inner = make_inner();

@@ -20,6 +20,9 @@ struct LinOp {
   Prec prec = PREC_DOUBLE;
   int store = -1;
   std::function<void(const Spinor &, Spinor &)> apply;
+  // optional: r1 = A a, r2 = A r1 in one call (an operator that rounds its
+  // input to a lower storage precision can then pass its own rounded r1 on)
+  std::function<void(const Spinor &, Spinor &, Spinor &)> apply_sq;
   Prec basis() const { return store < 0 ? prec : (Prec)store; }
 };
 
@@ -35,7 +38,9 @@ struct SchurOp {
     kappa[0] = k[0];
     kappa[1] = k[1];
   }
-  void operator()(const Spinor &in, Spinor &out) { schur_apply(c, U, kappa, idag, in, out, tmp.s); }
+  void operator()(const Spinor &in, Spinor &out, Spinor *out32 = nullptr, bool only32 = false) {
+    schur_apply(c, U, kappa, idag, in, out, tmp.s, out32, only32);
+  }
   LinOp op() {
     LinOp o;
     o.prec = tmp.s.prec;

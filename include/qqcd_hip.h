@@ -428,6 +428,14 @@ int qqcd_batch_set(int reorth, int use_graph, double inner_tol, int inner_cycles
  * the batched solver, the two-slot operator and an even cycle length)
  * allows it; 1 one step at a time.  Tuning aid; no reference counterpart. */
 int qqcd_set_arnoldi_steps(int steps);
+
+/* Paired Arnoldi steps of the batched solver: 1 (the default) takes the norm
+ * of u1' = u1 - V V^H u1 and <u1', u2'> from those of u1, u2 (formed in the
+ * pass of dots) less the projections, so that the update pass also writes
+ * the next basis vector and the next operator input; 0 computes them from the
+ * updated vectors, with a third pass over the pair.  Tuning aid; no reference
+ * counterpart. */
+int qqcd_set_pair_norms(int mode);
 /* Statistics of the last batched solve. */
 int qqcd_batch_stats(int *outer, int *cycles, long *slot_matvecs, long *batched_matvecs, int *graph_replays,
                      int *fallback);
