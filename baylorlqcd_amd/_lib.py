@@ -53,6 +53,7 @@ SIGNATURES = {
     "qqcd_set_multi_lds": [ctypes.c_int],
     "qqcd_set_arnoldi_steps": [ctypes.c_int],
     "qqcd_set_pair_norms": [ctypes.c_int],
+    "qqcd_set_batch_mid": [ctypes.c_int],
     "qqcd_set_step_graphs": [ctypes.c_int],
     "qqcd_gauge_random": [ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _dp],
     "qqcd_gauge_get_canonical": [_dp],

@@ -33,11 +33,19 @@ void MultiSchur::operator()(const MultiSpinor &in, MultiSpinor &out, const int *
     e2.a1 = 2.0 * mu / k;
     e2.b0 = -1.0 / (1.0 + a * a);
   }
-  dslash_multi(c.g, U, in, tmp.m, 1, idag, e1, nullptr, nact, tune, c.stream);
-  dslash_multi(c.g, U, tmp.m, out, 0, idag, e2, &in, nact, tune, c.stream);
+  dslash_multi(c.g, U, in, mid(), 1, idag, e1, nullptr, nact, tune, c.stream);
+  dslash_multi(c.g, U, mid(), out, 0, idag, e2, &in, nact, tune, c.stream);
 }
 
 bool MultiSchur::can_scale() const { return dslash_multi_can_scale(c.g, tmp.m.prec, tmp.m.nb, tune); }
+
+void MultiSchur::set_half_mid(bool on) {
+  if (on && tmp.m.prec == PREC_SINGLE && can_scale()) {
+    if (!tmp16) tmp16.reset(new MultiBuf(c.g, PREC_HALF, tmp.m.cap, tmp.m.nb, true));
+  } else {
+    tmp16.reset();
+  }
+}
 
 void MultiSchur::scaled(const MultiSpinor &in, MultiSpinor &out, const int *nact, const double *n2,
                         MultiSpinor *side, double hs) {
@@ -55,8 +63,8 @@ void MultiSchur::scaled(const MultiSpinor &in, MultiSpinor &out, const int *nact
   }
   // the operator is linear: the first hop runs on the unscaled input, the
   // second hop's epilogue scales the whole result
-  dslash_multi(c.g, U, in, tmp.m, 1, idag, e1, nullptr, nact, tune, c.stream);
-  dslash_multi(c.g, U, tmp.m, out, 0, idag, e2, &in, nact, tune, c.stream, n2, side, hs);
+  dslash_multi(c.g, U, in, mid(), 1, idag, e1, nullptr, nact, tune, c.stream);
+  dslash_multi(c.g, U, mid(), out, 0, idag, e2, &in, nact, tune, c.stream, n2, side, hs);
 }
 
 namespace {
@@ -218,6 +226,7 @@ Batch::Batch(Context &c_, const double kappa[2], MultiSpinor &B_, MultiSpinor &X
     paired = fused && p.sstep == 2 && p.reorth == 0 && m % 2 == 0;
     if (paired) U2f.reset(new MultiBuf(c.g, PREC_SINGLE, cap, nb));
   }
+  A32.set_half_mid(p.half_mid != 0);
   mb.reserve(mb.need());
   clear_flags(d_flag(), d_any(), cap, c.stream);
   st.iters.assign(cap, 0);

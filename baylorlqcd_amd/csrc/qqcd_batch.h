@@ -15,8 +15,14 @@ struct MultiSchur {
   int idag;
   MultiBuf tmp;
   MultiTune tune;
+  // the odd-checkerboard intermediate in FP16 with per-(site, slot) norms
+  // (set_half_mid, where the two-slot FP32 kernel runs): the first hop writes
+  // it, the second reads it, half the bytes of FP32
+  std::unique_ptr<MultiBuf> tmp16;
   MultiSchur(Context &c_, Prec p, int recon, const double k[2], int cap, int nb, const MultiTune &t,
              int idag_ = 0);
+  void set_half_mid(bool on);
+  MultiSpinor &mid() { return tmp16 ? tmp16->m : tmp.m; }
   void operator()(const MultiSpinor &in, MultiSpinor &out, const int *nact);
   // out = A in / ||in|| per slot (n2: ||in||^2 per slot in device memory, 0
   // where it is not positive) and side = hs in / ||in|| as an FP16 field
@@ -68,6 +74,9 @@ struct BatchParams {
   // so the update pass also writes the next basis vector and operator input;
   // 0 computes them from the updated vectors, with a third pass for those two
   int pair_norms = 1;
+  // the inner operator's intermediate (odd-checkerboard) field in FP16 with
+  // norms where the two-slot kernel runs (1), or in FP32 (0)
+  int half_mid = 1;
   MultiTune tune;
 };
 

@@ -376,6 +376,13 @@ extern "C" int qqcd_set_pair_norms(int mode) {
   API_END
 }
 
+extern "C" int qqcd_set_batch_mid(int prec) {
+  API_BEGIN
+  if (prec != PREC_SINGLE && prec != PREC_HALF) throw std::runtime_error("intermediate field: 1 (FP32) or 2 (FP16)");
+  g_bp.half_mid = prec == PREC_HALF ? 1 : 0;
+  API_END
+}
+
 extern "C" int qqcd_batch_set(int reorth, int use_graph, double inner_tol, int inner_cycles, int max_outer,
                               int op_prec, int op_recon, int basis_prec) {
   API_BEGIN

@@ -902,17 +902,21 @@ __global__ void __launch_bounds__(256) k_dslash_multi_pipe(const DslashArgs a, c
 // accumulators.  Per right-hand side this halves the load instructions and
 // the link and index arithmetic of the one-lane-per-slot kernel, whose
 // issue rate, not its HBM traffic, bounds it.
+// FP16 input or output (IN16 / OUT16): FP16 fields with one norm per (site,
+// slot), a component of both slots in one 8-byte load, both norms in another.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef f16 f16x4 __attribute__((ext_vector_type(4)));
 
-template <typename L, int RECON> struct HopRaw2 {
+template <typename L, int RECON, bool IN16> struct HopRaw2 {
   typename V2<L>::T u[RECON == 18 ? 9 : 6];
-  f32x4 p[12];  // (re, im) of slot 2q, then (re, im) of slot 2q+1
+  typename std::conditional<IN16, f16x4, f32x4>::type p[12];  // (re, im) of slot 2q, then of slot 2q+1
+  typename V2<float>::T nrm;  // IN16: the norms of the two slots at the neighbour
   float edge;
 };
 
-template <typename L, int RECON, int SIG, bool NT, int K>
+template <typename L, int RECON, int SIG, bool NT, int K, bool IN16>
 __device__ __forceinline__ void hop_fetch2(const DslashArgs &a, const HopIn &h, int i, int x, int y, int z, int t,
-                                           int q2, int lognb, HopRaw2<L, RECON> &r) {
+                                           int q2, int lognb, HopRaw2<L, RECON, IN16> &r) {
   constexpr int MU = K >> 1;
   constexpr bool FWD = !(K & 1);
   constexpr int NC = RECON == 18 ? 9 : 6;
@@ -921,9 +925,12 @@ __device__ __forceinline__ void hop_fetch2(const DslashArgs &a, const HopIn &h, 
   int j;
   hop_nbr<K>(a, x, y, z, t, j, r.edge);
   const int lpar = FWD ? a.par : (a.par ^ 1);
+  constexpr uint32_t ES = IN16 ? 4u : 8u;  // bytes of one complex element
+  typedef typename std::conditional<IN16, f16x4, f32x4>::type PE;
   const uint32_t lo = (uint32_t)(FWD ? i : j) * (uint32_t)sizeof(LE);
-  const uint32_t so = (((uint32_t)j << lognb) + (uint32_t)q2) * 8u;
-  const uint32_t lrun = (uint32_t)h.lstride * (uint32_t)sizeof(LE), srun = (uint32_t)h.sstride * 8u;
+  const uint32_t je = ((uint32_t)j << lognb) + (uint32_t)q2;
+  const uint32_t so = je * ES;
+  const uint32_t lrun = (uint32_t)h.lstride * (uint32_t)sizeof(LE), srun = (uint32_t)h.sstride * ES;
   const __amdgpu_buffer_rsrc_t lr =
       make_rsrc((const char *)h.links + (size_t)((lpar * 4 + MU) * NC) * lrun, NC * lrun);
 #pragma unroll
@@ -932,12 +939,14 @@ __device__ __forceinline__ void hop_fetch2(const DslashArgs &a, const HopIn &h, 
   constexpr int ns = MU == 3 ? 6 : 12;
   const __amdgpu_buffer_rsrc_t sr = make_rsrc(h.in, 12 * srun);
 #pragma unroll
-  for (int k = 0; k < ns; k++) r.p[k] = bld<f32x4, 0>(sr, so, (s0 + k) * srun);
+  for (int k = 0; k < ns; k++) r.p[k] = bld<PE, 0>(sr, so, (s0 + k) * srun);
+  if constexpr (IN16)
+    r.nrm = bld<typename V2<float>::T, 0>(make_rsrc(h.in_norm, 4u * (uint32_t)h.sstride), je * 4u, 0);
 }
 
-template <typename L, int RECON, int SIG, int K>
-__device__ __forceinline__ void hop_consume2(const HopRaw2<L, RECON> &r, cx<float> acc0[12], cx<float> acc1[12],
-                                             float sc) {
+template <typename L, int RECON, int SIG, int K, bool IN16>
+__device__ __forceinline__ void hop_consume2(const HopRaw2<L, RECON, IN16> &r, cx<float> acc0[12],
+                                             cx<float> acc1[12], float sc) {
   constexpr int NC = RECON == 18 ? 9 : 6;
   constexpr int NP = (K >> 1) == 3 ? 6 : 12;
   cx<float> u[9];
@@ -945,49 +954,55 @@ __device__ __forceinline__ void hop_consume2(const HopRaw2<L, RECON> &r, cx<floa
   for (int k = 0; k < NC; k++) u[k] = mk<float>((float)r.u[k].x, (float)r.u[k].y);
   if constexpr (RECON == 12) recon_row2(u);
   const float f = sc * r.edge;
+  float f0 = f, f1 = f;
+  if constexpr (IN16) {
+    f0 *= r.nrm.x;
+    f1 *= r.nrm.y;
+  }
   cx<float> p[12];
 #pragma unroll
-  for (int k = 0; k < NP; k++) p[k] = mk<float>(r.p[k].x, r.p[k].y);
-  hop_math<SIG, K>(u, p, acc0, f);
+  for (int k = 0; k < NP; k++) p[k] = mk<float>((float)r.p[k].x, (float)r.p[k].y);
+  hop_math<SIG, K>(u, p, acc0, f0);
 #pragma unroll
-  for (int k = 0; k < NP; k++) p[k] = mk<float>(r.p[k].z, r.p[k].w);
-  hop_math<SIG, K>(u, p, acc1, f);
+  for (int k = 0; k < NP; k++) p[k] = mk<float>((float)r.p[k].z, (float)r.p[k].w);
+  hop_math<SIG, K>(u, p, acc1, f1);
 }
 
-template <typename L, int RECON> __device__ __forceinline__ void stage_pin2(HopRaw2<L, RECON> &r) {
+template <typename L, int RECON, bool IN16> __device__ __forceinline__ void stage_pin2(HopRaw2<L, RECON, IN16> &r) {
   constexpr int NC = RECON == 18 ? 9 : 6;
   asm volatile("" ::: "memory");
 #pragma unroll
   for (int k = 0; k < NC; k++) asm volatile("" : "+v"(r.u[k]));
 #pragma unroll
   for (int k = 0; k < 12; k++) asm volatile("" : "+v"(r.p[k]));
+  if constexpr (IN16) asm volatile("" : "+v"(r.nrm));
   asm volatile("" : "+v"(r.edge));
 }
 
-template <typename L, int RECON, int SIG, bool NT, int D, int K>
+template <typename L, int RECON, int SIG, bool NT, int D, int K, bool IN16>
 __device__ __forceinline__ void hop_prefetch2(const DslashArgs &a, const HopIn &h, int i, int x, int y, int z, int t,
-                                              int q2, int lognb, HopRaw2<L, RECON> *buf) {
+                                              int q2, int lognb, HopRaw2<L, RECON, IN16> *buf) {
   if constexpr (K < D) {
-    hop_fetch2<L, RECON, SIG, NT, K>(a, h, i, x, y, z, t, q2, lognb, buf[K]);
+    hop_fetch2<L, RECON, SIG, NT, K, IN16>(a, h, i, x, y, z, t, q2, lognb, buf[K]);
     asm volatile("" ::: "memory");
-    hop_prefetch2<L, RECON, SIG, NT, D, K + 1>(a, h, i, x, y, z, t, q2, lognb, buf);
+    hop_prefetch2<L, RECON, SIG, NT, D, K + 1, IN16>(a, h, i, x, y, z, t, q2, lognb, buf);
   }
 }
 
 // hops K .. 7 of the pair, pipelined as hop_pipe: hop K is computed from its
 // buffer, which is refilled with hop K + D, or with the local pair of the
 // epilogue once no hop is left
-template <typename L, int RECON, int SIG, bool NT, int D, int K>
+template <typename L, int RECON, int SIG, bool NT, int D, int K, bool IN16>
 __device__ __forceinline__ void hop_pipe2(const DslashArgs &a, const HopIn &h, int i, int x, int y, int z, int t,
-                                          int q2, int lognb, HopRaw2<L, RECON> *buf, cx<float> acc0[12],
+                                          int q2, int lognb, HopRaw2<L, RECON, IN16> *buf, cx<float> acc0[12],
                                           cx<float> acc1[12], const void *xin, int io, f32x4 *xr) {
   if constexpr (K < 8) {
     stage_pin2(buf[K % D]);
-    hop_consume2<L, RECON, SIG, K>(buf[K % D], acc0, acc1, (float)a.scl[K >> 1]);
+    hop_consume2<L, RECON, SIG, K, IN16>(buf[K % D], acc0, acc1, (float)a.scl[K >> 1]);
     stage_fence(acc0);
     stage_fence(acc1);
     if constexpr (K + D < 8) {
-      hop_fetch2<L, RECON, SIG, NT, K + D>(a, h, i, x, y, z, t, q2, lognb, buf[K % D]);
+      hop_fetch2<L, RECON, SIG, NT, K + D, IN16>(a, h, i, x, y, z, t, q2, lognb, buf[K % D]);
     } else if constexpr (K + D == 8) {
       if (a.has_x) {
         const uint32_t srun = (uint32_t)h.sstride * 8u;
@@ -996,7 +1011,7 @@ __device__ __forceinline__ void hop_pipe2(const DslashArgs &a, const HopIn &h, i
         for (int k = 0; k < 12; k++) xr[k] = bld<f32x4, 0>(xrs, (uint32_t)io * 8u, k * srun);
       }
     }
-    hop_pipe2<L, RECON, SIG, NT, D, K + 1>(a, h, i, x, y, z, t, q2, lognb, buf, acc0, acc1, xin, io, xr);
+    hop_pipe2<L, RECON, SIG, NT, D, K + 1, IN16>(a, h, i, x, y, z, t, q2, lognb, buf, acc0, acc1, xin, io, xr);
   }
 }
 
@@ -1025,9 +1040,10 @@ __device__ __forceinline__ void epilogue_f(const DslashArgs &a, const cx<float> 
 
 // Output site i, slots q2 and q2 + 1 of the lane's RHS block (both: whether
 // slot q2 + 1 is active; a retired slot is computed but not written).
-template <typename L, int RECON, int SIG, bool NT, int D>
-__device__ __forceinline__ void dslash_site_pair(const DslashArgs &a, const HopIn &h, void *out, const void *xin,
-                                                 void *side, int i, int q2, int lognb, int slot0, bool both) {
+template <typename L, int RECON, int SIG, bool NT, int D, bool IN16, bool OUT16>
+__device__ __forceinline__ void dslash_site_pair(const DslashArgs &a, const HopIn &h, void *out, float *out_norm,
+                                                 const void *xin, void *side, int i, int q2, int lognb, int slot0,
+                                                 bool both) {
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
   const int r2 = 2 * i;
   const int x0 = r2 % nx;
@@ -1038,14 +1054,14 @@ __device__ __forceinline__ void dslash_site_pair(const DslashArgs &a, const HopI
   const int t = q / nz;
   const int x = x0 + ((a.par + y + z + t) & 1);
 
-  HopRaw2<L, RECON> buf[D];
-  hop_prefetch2<L, RECON, SIG, NT, D, 0>(a, h, i, x, y, z, t, q2, lognb, buf);
+  HopRaw2<L, RECON, IN16> buf[D];
+  hop_prefetch2<L, RECON, SIG, NT, D, 0, IN16>(a, h, i, x, y, z, t, q2, lognb, buf);
   const int io = (i << lognb) + q2;
   f32x4 xr[12];
   cx<float> acc0[12], acc1[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc0[k] = acc1[k] = mk<float>(0, 0);
-  hop_pipe2<L, RECON, SIG, NT, D, 0>(a, h, i, x, y, z, t, q2, lognb, buf, acc0, acc1, xin, io, xr);
+  hop_pipe2<L, RECON, SIG, NT, D, 0, IN16>(a, h, i, x, y, z, t, q2, lognb, buf, acc0, acc1, xin, io, xr);
 
   cx<float> o0[12], o1[12], xv[12];
 #pragma unroll
@@ -1090,6 +1106,44 @@ __device__ __forceinline__ void dslash_site_pair(const DslashArgs &a, const HopI
       }
     }
   }
+  if constexpr (OUT16) {
+    // per slot: norm = the largest |re|, |im| of its 24 reals, stored values / norm
+    float m0 = 0.f, m1 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      m0 = fmaxf(m0, fmaxf(fabsf(o0[k].re), fabsf(o0[k].im)));
+      m1 = fmaxf(m1, fmaxf(fabsf(o1[k].re), fabsf(o1[k].im)));
+    }
+    const float i0 = m0 > 0.f ? 1.f / m0 : 0.f, i1 = m1 > 0.f ? 1.f / m1 : 0.f;
+    const uint32_t hrun = (uint32_t)h.sstride * 4u;
+    const __amdgpu_buffer_rsrc_t ors = make_rsrc(out, 12 * hrun);
+    const __amdgpu_buffer_rsrc_t nrs = make_rsrc(out_norm, 4u * (uint32_t)h.sstride);
+    if (both) {
+      typename V2<float>::T nn;
+      nn.x = m0;
+      nn.y = m1;
+      bst<typename V2<float>::T, AUXS>(nn, nrs, (uint32_t)io * 4u, 0);
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        f16x4 w;
+        w.x = (f16)(o0[k].re * i0);
+        w.y = (f16)(o0[k].im * i0);
+        w.z = (f16)(o1[k].re * i1);
+        w.w = (f16)(o1[k].im * i1);
+        bst<f16x4, AUXS>(w, ors, (uint32_t)io * 4u, k * hrun);
+      }
+    } else {
+      bst<float, AUXS>(m0, nrs, (uint32_t)io * 4u, 0);
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        h2 w;
+        w.x = (f16)(o0[k].re * i0);
+        w.y = (f16)(o0[k].im * i0);
+        bst<h2, AUXS>(w, ors, (uint32_t)io * 4u, k * hrun);
+      }
+    }
+    return;
+  }
   const __amdgpu_buffer_rsrc_t ors = make_rsrc(out, 12 * srun);
   if (both) {
 #pragma unroll
@@ -1115,7 +1169,7 @@ __device__ __forceinline__ void dslash_site_pair(const DslashArgs &a, const HopI
 // One workgroup per tile of 256 lanes = 512/nb sites x nb/2 slot pairs of one
 // RHS block (nb >= 2), tiles split into eight contiguous ranges, one per XCD,
 // as k_dslash_multi_pipe with per_tile; m.site_tiles counts tiles of pairs.
-template <typename L, int RECON, int SIG, bool NT, int D>
+template <typename L, int RECON, int SIG, bool NT, int D, bool IN16, bool OUT16>
 __global__ void __launch_bounds__(256) k_dslash_multi_pair(const DslashArgs a, const MultiArgs m) {
   const int nact = *m.nact;
   const int nb = 1 << m.lognb;
@@ -1140,10 +1194,14 @@ __global__ void __launch_bounds__(256) k_dslash_multi_pair(const DslashArgs a, c
   if (lidx >= a.Vh || r0 >= nact) return;
   const int i = site_of(a, lidx);
   typedef typename V2<float>::T SE;
-  const long eo = (long)blk * m.blk_elems;
-  const HopIn h{(const SE *)a.in + eo, nullptr, a.links, a.sstride, a.lstride};
-  dslash_site_pair<L, RECON, SIG, NT, D>(a, h, (SE *)a.out + eo, a.x ? (const SE *)a.x + eo : nullptr,
-                                         a.side ? (h2 *)a.side + eo : nullptr, i, q2, m.lognb, r0, r0 + 1 < nact);
+  const long eo = (long)blk * m.blk_elems, no = (long)blk * m.blk_norms;
+  const void *in = IN16 ? (const void *)((const h2 *)a.in + eo) : (const void *)((const SE *)a.in + eo);
+  void *out = OUT16 ? (void *)((h2 *)a.out + eo) : (void *)((SE *)a.out + eo);
+  const HopIn h{in, IN16 ? a.in_norm + no : nullptr, a.links, a.sstride, a.lstride};
+  dslash_site_pair<L, RECON, SIG, NT, D, IN16, OUT16>(a, h, out, OUT16 ? a.out_norm + no : nullptr,
+                                                      a.x ? (const SE *)a.x + eo : nullptr,
+                                                      a.side ? (h2 *)a.side + eo : nullptr, i, q2, m.lognb, r0,
+                                                      r0 + 1 < nact);
 }
 
 HopCoef make_hopcoef(const Geom &g, int idag) {
@@ -1261,21 +1319,31 @@ static void launch_multi_pipe(const DslashArgs &a, const MultiArgs &m, int grid,
 }
 
 template <typename L, int RECON, int SIG, bool NT>
-static void launch_multi_pair(const DslashArgs &a, const MultiArgs &m0, int depth, long max_blk, hipStream_t s) {
+static void launch_multi_pair(const DslashArgs &a, const MultiArgs &m0, int depth, long max_blk, int io16,
+                              hipStream_t s) {
   MultiArgs m = m0;
   m.site_tiles = (int)(((long)a.Vh * (1L << (m.lognb - 1)) + 255) / 256);
   const long gt = 8 * (((long)m.site_tiles * max_blk + 7) / 8);
   if (gt > 0x7fffffffL) throw std::runtime_error("dslash_multi: too many tiles for one launch");
-  if (depth == 1)
-    hipLaunchKernelGGL((k_dslash_multi_pair<L, RECON, SIG, NT, 1>), dim3((int)gt), dim3(256), 0, s, a, m);
+  // io16: 1 FP32 in, FP16 out; 2 FP16 in, FP32 out (two hops in flight)
+  if (io16 == 1)
+    hipLaunchKernelGGL((k_dslash_multi_pair<L, RECON, SIG, NT, 2, false, true>), dim3((int)gt), dim3(256), 0, s, a,
+                       m);
+  else if (io16 == 2)
+    hipLaunchKernelGGL((k_dslash_multi_pair<L, RECON, SIG, NT, 2, true, false>), dim3((int)gt), dim3(256), 0, s, a,
+                       m);
+  else if (depth == 1)
+    hipLaunchKernelGGL((k_dslash_multi_pair<L, RECON, SIG, NT, 1, false, false>), dim3((int)gt), dim3(256), 0, s, a,
+                       m);
   else
-    hipLaunchKernelGGL((k_dslash_multi_pair<L, RECON, SIG, NT, 2>), dim3((int)gt), dim3(256), 0, s, a, m);
+    hipLaunchKernelGGL((k_dslash_multi_pair<L, RECON, SIG, NT, 2, false, false>), dim3((int)gt), dim3(256), 0, s, a,
+                       m);
   QQCD_HIP(hipGetLastError());
 }
 
 template <typename S, typename L, int RECON>
 static void launch_multi(const DslashArgs &a0, const MultiArgs &m0, int sig, const MultiTune &t, long max_blk,
-                         hipStream_t s) {
+                         int io16, hipStream_t s) {
   const long max_tiles = (long)m0.site_tiles * max_blk;
   // the pipelined kernel: projector form, buffer descriptors of < 4 GiB (one
   // RHS block of a field, one (parity, mu) block of links)
@@ -1299,16 +1367,17 @@ static void launch_multi(const DslashArgs &a0, const MultiArgs &m0, int sig, con
       if (t.kernel >= 3 && m.lognb >= 1) {
         const int depth = t.kernel == 3 ? 2 : 1;
         if (sig > 0) {
-          if (t.nt) launch_multi_pair<L, RECON, 1, true>(a, m, depth, max_blk, s);
-          else launch_multi_pair<L, RECON, 1, false>(a, m, depth, max_blk, s);
+          if (t.nt) launch_multi_pair<L, RECON, 1, true>(a, m, depth, max_blk, io16, s);
+          else launch_multi_pair<L, RECON, 1, false>(a, m, depth, max_blk, io16, s);
         } else {
-          if (t.nt) launch_multi_pair<L, RECON, -1, true>(a, m, depth, max_blk, s);
-          else launch_multi_pair<L, RECON, -1, false>(a, m, depth, max_blk, s);
+          if (t.nt) launch_multi_pair<L, RECON, -1, true>(a, m, depth, max_blk, io16, s);
+          else launch_multi_pair<L, RECON, -1, false>(a, m, depth, max_blk, io16, s);
         }
         return;
       }
     }
-    if (a0.slot_n2 || a0.side) throw std::runtime_error("dslash_multi: the scaled epilogue needs the two-slot FP32 kernel");
+    if (a0.slot_n2 || a0.side || io16)
+      throw std::runtime_error("dslash_multi: the scaled epilogue and FP16 in/out need the two-slot FP32 kernel");
     if (sig > 0) {
       if (t.nt) launch_multi_pipe<S, L, RECON, 1, true>(a, m, t.grid, max_tiles, t.lds, s);
       else launch_multi_pipe<S, L, RECON, 1, false>(a, m, t.grid, max_tiles, t.lds, s);
@@ -1318,7 +1387,8 @@ static void launch_multi(const DslashArgs &a0, const MultiArgs &m0, int sig, con
     }
     return;
   }
-  if (a0.slot_n2 || a0.side) throw std::runtime_error("dslash_multi: the scaled epilogue needs the two-slot FP32 kernel");
+  if (a0.slot_n2 || a0.side || io16)
+    throw std::runtime_error("dslash_multi: the scaled epilogue and FP16 in/out need the two-slot FP32 kernel");
   if (sig > 0) launch_multi_sig<S, L, RECON, 1>(a0, m0, t.grid, s);
   else if (sig < 0) launch_multi_sig<S, L, RECON, -1>(a0, m0, t.grid, s);
   else launch_multi_sig<S, L, RECON, 0>(a0, m0, t.grid, s);
@@ -1476,8 +1546,15 @@ void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpi
                   const Epilogue &ep, const MultiSpinor *x, const int *d_nact, const MultiTune &t0, hipStream_t s,
                   const double *slot_n2, MultiSpinor *side, double side_hs) {
   const MultiTune t = multi_tune_of(t0);
-  if (in.prec != out.prec || (x && x->prec != out.prec))
+  // FP16 (with norms) on one side of an FP32 hop: the two-slot kernel's
+  // FP16 intermediate of the Schur operator (io16 1: FP32 -> FP16, 2: FP16 -> FP32)
+  int io16 = 0;
+  if (in.prec == PREC_SINGLE && out.prec == PREC_HALF && out.norm && !x) io16 = 1;
+  if (in.prec == PREC_HALF && in.norm && out.prec == PREC_SINGLE) io16 = 2;
+  if ((in.prec != out.prec && !io16) || (x && x->prec != out.prec))
     throw std::runtime_error("dslash_multi: spinor precisions differ");
+  if (io16 && !dslash_multi_can_scale(g, PREC_SINGLE, in.nb, t0))
+    throw std::runtime_error("dslash_multi: FP16 in or out needs the two-slot FP32 kernel");
   if (in.stride != g.stride || out.stride != g.stride || U.stride != g.stride)
     throw std::runtime_error("dslash_multi: field stride does not match the geometry");
   if (in.nb != out.nb || in.cap != out.cap || (x && (x->nb != in.nb || x->cap != in.cap)))
@@ -1514,12 +1591,12 @@ void dslash_multi(const Geom &g, const Gauge &U, const MultiSpinor &in, MultiSpi
   m.site_tiles = (int)((g.Vh * in.nb + 255) / 256);
   m.blk_elems = in.blk_elems();
   m.blk_norms = g.stride * in.nb;
-  if (in.prec == PREC_HALF) throw std::runtime_error("dslash_multi: FP64 or FP32 right-hand sides");
-  dispatch_prec(in.prec, U, "dslash_multi", [&](auto *sp, auto *lp, auto rc) {
+  if (in.prec == PREC_HALF && !io16) throw std::runtime_error("dslash_multi: FP64 or FP32 right-hand sides");
+  dispatch_prec(io16 ? PREC_SINGLE : in.prec, U, "dslash_multi", [&](auto *sp, auto *lp, auto rc) {
     typedef typename std::remove_pointer<decltype(sp)>::type S;
     typedef typename std::remove_pointer<decltype(lp)>::type L;
     if constexpr (!std::is_same<S, f16>::value)
-      launch_multi<S, L, decltype(rc)::value>(a, m, proj, t, (long)in.nblk(), s);
+      launch_multi<S, L, decltype(rc)::value>(a, m, proj, t, (long)in.nblk(), io16, s);
   });
 }
 

@@ -436,6 +436,12 @@ int qqcd_set_arnoldi_steps(int steps);
  * updated vectors, with a third pass over the pair.  Tuning aid; no reference
  * counterpart. */
 int qqcd_set_pair_norms(int mode);
+
+/* Storage of the batched solver's inner operator's intermediate field (the
+ * odd checkerboard between its two hops): 2 (the default) FP16 with one norm
+ * per site and right-hand side, where the two-slot FP32 hop runs; 1 FP32.
+ * Tuning aid; no reference counterpart. */
+int qqcd_set_batch_mid(int prec);
 /* Statistics of the last batched solve. */
 int qqcd_batch_stats(int *outer, int *cycles, long *slot_matvecs, long *batched_matvecs, int *graph_replays,
                      int *fallback);

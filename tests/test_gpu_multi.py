@@ -135,11 +135,13 @@ def test_tuning_knobs_reject_bad_values(lat):
     assert lib.qqcd_set_multi_lds(1 << 30) != 0
     assert lib.qqcd_set_step_graphs(3) != 0
     assert lib.qqcd_set_pair_norms(2) != 0
+    assert lib.qqcd_set_batch_mid(0) != 0
     # the defaults are left in place
     _lib.check(lib.qqcd_set_multi_tune(-1, -1, -1))
     _lib.check(lib.qqcd_set_multi_lds(-1))
     _lib.check(lib.qqcd_set_step_graphs(2))
     _lib.check(lib.qqcd_set_pair_norms(1))
+    _lib.check(lib.qqcd_set_batch_mid(2))
 
 
 @pytest.mark.parametrize("kappa", [(0.150, 0.0), (0.156, 0.005)])
@@ -278,9 +280,10 @@ def test_fermprop_multi_normalisation_in_operator(cfg1):
     relation) or one at a time; with the one-slot launch form (2) a separate
     pass normalises.  Paired steps take the norms of the updated vectors from
     the projections (the update pass then writes the next basis vector) or
-    from the updated vectors (a third pass).  All reach the same true
-    residuals and solutions to the solve's tolerance, with graph replays and
-    eagerly."""
+    from the updated vectors (a third pass).  The operator's intermediate
+    field is FP16 with per-(site, slot) norms where the two-slot kernel runs,
+    or FP32.  All reach the same true residuals and solutions to the solve's
+    tolerance, with graph replays and eagerly."""
     n, bc, U, u, co, g = cfg1
     kappa = (0.150, 0.0)
     srcs = []
@@ -292,33 +295,35 @@ def test_fermprop_multi_normalisation_in_operator(cfg1):
     lib = _lib.lib()
     sols = {}
     try:
-        for form, steps, norms in ((3, 2, 1), (3, 2, 0), (3, 1, 1), (2, 1, 1)):
+        for form, steps, norms, mid in ((3, 2, 1, 2), (3, 2, 0, 2), (3, 2, 1, 1), (3, 1, 1, 2), (2, 1, 1, 2)):
             for graph in (1, 0):
                 _lib.check(lib.qqcd_set_multi_tune(form, -1, -1))
                 _lib.check(lib.qqcd_set_arnoldi_steps(steps))
                 _lib.check(lib.qqcd_set_pair_norms(norms))
+                _lib.check(lib.qqcd_set_batch_mid(mid))
                 disconloops.batch_set(use_graph=graph, basis_prec=2)
-                print(f"[multi] normalisation in the operator: form {form} steps {steps} norms {norms} graph {graph}",
-                      flush=True)
+                print(f"[multi] normalisation in the operator: form {form} steps {steps} norms {norms} mid {mid} "
+                      f"graph {graph}", flush=True)
                 xe, xo, it, rr = disconloops.fermprop_multi([s[0] for s in srcs], [s[1] for s in srcs], kappa, co,
                                                             bc, 1e-10, (40, 10), u)
                 xs = []
                 for r, (be, bo) in enumerate(srcs):
                     b = O.spinor_ref_to_canon(n, be, bo)
                     x = O.spinor_ref_to_canon(n, xe[r], xo[r])
-                    assert full_rel_res(n, bc, U, kappa, b, x) < 1e-10, (form, steps, norms, graph, r)
+                    assert full_rel_res(n, bc, U, kappa, b, x) < 1e-10, (form, steps, norms, mid, graph, r)
                     xs.append(x)
-                sols[(form, steps, norms, graph)] = xs
+                sols[(form, steps, norms, mid, graph)] = xs
     finally:
         _lib.check(lib.qqcd_set_multi_tune(-1, -1, -1))
         _lib.check(lib.qqcd_set_arnoldi_steps(2))
         _lib.check(lib.qqcd_set_pair_norms(1))
+        _lib.check(lib.qqcd_set_batch_mid(2))
         disconloops.batch_set()
     for key, xs in sols.items():
         for r in range(len(srcs)):
-            assert rel(xs[r], sols[(2, 1, 1, 0)][r]) < 1e-8, (key, r)
+            assert rel(xs[r], sols[(2, 1, 1, 2, 0)][r]) < 1e-8, (key, r)
     # graph replays give the eager launches' result bit for bit
-    for key in ((3, 2, 1), (3, 2, 0), (3, 1, 1)):
+    for key in ((3, 2, 1, 2), (3, 2, 0, 2), (3, 2, 1, 1), (3, 1, 1, 2)):
         for r in range(len(srcs)):
             assert np.array_equal(sols[key + (1,)][r], sols[key + (0,)][r]), (key, r)
 
