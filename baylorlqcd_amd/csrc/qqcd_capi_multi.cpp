@@ -97,6 +97,7 @@ void multi_canonical(int which, int nrhs, int nact, int nb, int prec, int recon,
       dslash_multi(c.g, U, a.m, b.m, par_out, idag, Epilogue(), nullptr, d_n, t, c.stream);
     } else {
       MultiSchur A(c, p, recon, kappa, nrhs, nb, t, idag);
+      A.set_half_mid(g_bp.half_mid != 0);
       A(a.m, b.m, d_n);
     }
     const int out_par = which == 0 ? par_out : 0;

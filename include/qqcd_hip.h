@@ -574,7 +574,9 @@ int qqcd_bench_solve(int method, const int src[4], int spin, int colour, const d
  * as in qqcd_hop_canonical, back to back) through the RHS-innermost fields
  * (prec 0 FP64 or 1 FP32; RHS block nb, 0 = the power of two >= nrhs, at
  * most 64; order 0/1 tile order).  Only the first nact right-hand sides are
- * active: the others are left as they are in out. */
+ * active: the others are left as they are in out.  The FP32 Schur operator
+ * keeps its intermediate field as qqcd_set_batch_mid says (as the batched
+ * solver's inner operator does). */
 int qqcd_hop_multi_canonical(int nrhs, int nact, int nb, int prec, int recon, int idag, int par_out, int order,
                              const double *in, double *out);
 int qqcd_schur_multi_canonical(int nrhs, int nact, int nb, int prec, int recon, const double kappa[2], int idag,

@@ -144,6 +144,36 @@ def test_tuning_knobs_reject_bad_values(lat):
     _lib.check(lib.qqcd_set_batch_mid(2))
 
 
+@pytest.mark.parametrize("recon", [12, 18])
+def test_schur_multi_fp16_intermediate(lat, recon):
+    """The FP32 multi-RHS Schur operator with its odd-checkerboard
+    intermediate in FP16 (one norm per site and slot: the first hop's FP16
+    output form, the second's FP16 input form) or in FP32, against the
+    oracle's FP64 Schur operator: FP16 rounding (2^-11 relative per site)
+    against FP32 rounding, both far apart, the FP16 one well inside 1e-3."""
+    n, bc, U, u, co = lat
+    nrhs = 7
+    psi = spinors(n, nrhs, 23)
+    odd = O.parity_mask(n)
+    psi[:, odd] = 0
+    one = [1.0] * 4
+    kappa = (0.156, 0.005)
+    lib = _lib.lib()
+    errs = {}
+    try:
+        for mid in (1, 2):
+            _lib.check(lib.qqcd_set_batch_mid(mid))
+            out = np.zeros_like(psi)
+            disconloops.schur_multi_canonical(psi, out, kappa, prec=1, recon=recon, nb=4, order=0)
+            errs[mid] = max(rel(out[r][~odd], O.schur(n, U, bc, one, one, 0, kappa, psi[r])[~odd])
+                            for r in range(nrhs))
+    finally:
+        _lib.check(lib.qqcd_set_batch_mid(2))
+    print(f"[multi] Schur operator, FP32 spinors, recon {recon}: max rel error {errs}", flush=True)
+    assert errs[1] < 1e-6, errs
+    assert 1e-5 < errs[2] < 1e-3, errs
+
+
 @pytest.mark.parametrize("kappa", [(0.150, 0.0), (0.156, 0.005)])
 def test_schur_multi_fp64(lat, kappa):
     n, bc, U, u, co = lat
