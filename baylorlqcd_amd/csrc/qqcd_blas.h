@@ -57,6 +57,11 @@ class Blas {
   void pair_dot_dev(int k, const Vec *v, const Vec &w1, const Vec &w2, double *g1, double *g2);
   void pair_axpy_dev(int k, const double *g1, const double *g2, const Vec *v, Vec &w1, Vec &w2, double *sc);
   void pair_comb_dev(const double *b, const Vec &w1, Vec &w2, double *dnorm2);
+  // pair_comb_dev and the two conversions into an FP16 basis with per-site
+  // norms in one pass over the Vh sites: v1 = w1 / sqrt(*n1), v2 = r / ||r||
+  // with r = b0 w2 + (b1 + i b2) w1 (w2 is left as it is), *dnorm2 = ||r||^2
+  void pair_comb_basis(long Vh, const double *b, const Spinor &w1, const Spinor &w2, const double *n1, Spinor &v1,
+                       Spinor &v2, double *dnorm2);
   // y = a x + b y ; returns ||y||^2 when want_norm
   double axpby(std::complex<double> a, const Vec &x, std::complex<double> b, Vec &y, bool want_norm = false);
   void copy(const Vec &x, Vec &y);

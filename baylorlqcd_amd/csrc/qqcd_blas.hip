@@ -354,6 +354,60 @@ __global__ void __launch_bounds__(RB) k_comb2(long nu, const double *b, const vo
   block_sum<1>(acc, part + blockIdx.x);
 }
 
+// The end of a pair of Arnoldi steps over an FP16 basis with per-site norms,
+// one thread per site (grid-stride): r = b0 u2 + (b1 + i b2) u1 from the FP32
+// u1, u2 (12 component runs of stride elements), v1 = u1 / sqrt(*n1) and v2 = r
+// in the basis's FP16 format (the site's 24 reals over their largest
+// magnitude, that magnitude as the site's norm), partials of ||r||^2.  v2 is
+// r unnormalised: its norms are scaled by 1 / ||r|| afterwards (k_scale_norms),
+// the FP16 values of a site being the same for r and r / ||r||.
+__global__ void __launch_bounds__(RB) k_comb2_basis(int Vh, long stride, const double *b, const double *n1,
+                                                    const void *u1, const void *u2, void *v1, float *v1n, void *v2,
+                                                    float *v2n, double *part) {
+  const float s2 = (float)b[0], s1r = (float)b[1], s1i = (float)b[2];
+  const double q = *n1;
+  const float sv = q > 0 ? (float)(1.0 / sqrt(q)) : 0.f;
+  const float2 *a1 = (const float2 *)u1, *a2 = (const float2 *)u2;
+  typedef f16 hv2 __attribute__((ext_vector_type(2)));
+  hv2 *o1 = (hv2 *)v1, *o2 = (hv2 *)v2;
+  float nr = 0.f;
+  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < Vh; i += (long)gridDim.x * RB) {
+    float2 x[12], r[12];
+    float m1 = 0.f, m2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      x[k] = a1[k * stride + i];
+      const float2 y = a2[k * stride + i];
+      r[k] = make_float2(s2 * y.x + (s1r * x[k].x - s1i * x[k].y), s2 * y.y + (s1r * x[k].y + s1i * x[k].x));
+      nr += r[k].x * r[k].x + r[k].y * r[k].y;
+      m1 = fmaxf(m1, fmaxf(fabsf(x[k].x), fabsf(x[k].y)));
+      m2 = fmaxf(m2, fmaxf(fabsf(r[k].x), fabsf(r[k].y)));
+    }
+    const float i1 = m1 > 0.f ? 1.f / m1 : 0.f, i2 = m2 > 0.f ? 1.f / m2 : 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      hv2 h1, h2;
+      h1.x = (f16)(x[k].x * i1);
+      h1.y = (f16)(x[k].y * i1);
+      h2.x = (f16)(r[k].x * i2);
+      h2.y = (f16)(r[k].y * i2);
+      o1[k * stride + i] = h1;
+      o2[k * stride + i] = h2;
+    }
+    v1n[i] = m1 * sv;
+    v2n[i] = m2;
+  }
+  double acc[1] = {(double)nr};
+  block_sum<1>(acc, part + blockIdx.x);
+}
+
+// norm[i] *= 1 / sqrt(*n2) (0 where *n2 is not positive)
+__global__ void __launch_bounds__(RB) k_scale_norms(int Vh, float *norm, const double *n2) {
+  const double q = *n2;
+  const float s = q > 0 ? (float)(1.0 / sqrt(q)) : 0.f;
+  for (long i = blockIdx.x * (long)RB + threadIdx.x; i < Vh; i += (long)gridDim.x * RB) norm[i] *= s;
+}
+
 // Paired steps j, j + 1 on the device Hessenberg of the single-RHS solvers
 // (columns of ld doubles: the complex <v_i, w>, then m squared norms): column
 // j holds g1 = V_j^H A v_j (from pair_dot_dev); with nu^2 = ||u1'||^2 = sc[0],
@@ -892,6 +946,21 @@ void Blas::pair_comb_dev(const double *b, const Vec &w1, Vec &w2, double *dnorm2
   hipLaunchKernelGGL(k_comb2, dim3(RG), dim3(RB), 0, stream, units<f16>(w1.n), b, w1.data, w2.data, part);
   QQCD_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, stream, RG, 1, part, dnorm2);
+  QQCD_HIP(hipGetLastError());
+}
+
+void Blas::pair_comb_basis(long Vh, const double *b, const Spinor &w1, const Spinor &w2, const double *n1, Spinor &v1,
+                           Spinor &v2, double *dnorm2) {
+  if (w1.prec != PREC_SINGLE || w2.prec != PREC_SINGLE || v1.prec != PREC_HALF || v2.prec != PREC_HALF ||
+      !v1.norm || !v2.norm || w1.stride != w2.stride || v1.stride != w1.stride || v2.stride != w1.stride)
+    throw std::runtime_error("pair_comb_basis: FP32 w1, w2 and FP16 v1, v2 with norms, one layout");
+  if (Vh < 1 || Vh > w1.stride) throw std::runtime_error("pair_comb_basis: site count");
+  hipLaunchKernelGGL(k_comb2_basis, dim3(RG), dim3(RB), 0, stream, (int)Vh, w1.stride, b, n1, w1.data, w2.data,
+                     v1.data, v1.norm, v2.data, v2.norm, part);
+  QQCD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RB), 0, stream, RG, 1, part, dnorm2);
+  QQCD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scale_norms, dim3(RG), dim3(RB), 0, stream, (int)Vh, v2.norm, dnorm2);
   QQCD_HIP(hipGetLastError());
 }
 

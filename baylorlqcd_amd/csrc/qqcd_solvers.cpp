@@ -194,6 +194,11 @@ static void arnoldi_pair_dev(Context &c, LinOp &A, std::vector<Spinor> &V, int j
   c.blas->pair_dot_dev(j + 1, Vb.data(), a1, a2, dh.col(j), dh.g2());
   c.blas->pair_axpy_dev(j + 1, dh.col(j), dh.g2(), Vb.data(), a1, a2, dh.sc());
   pair_columns(dh.d, dh.ld, dh.m, j, dh.g2(), dh.sc(), dh.cb(), c.stream);
+  if (V[j + 1].prec == PREC_HALF && V[j + 1].norm && V[j + 2].prec == PREC_HALF && V[j + 2].norm &&
+      w1.prec == PREC_SINGLE && w2.prec == PREC_SINGLE) {
+    c.blas->pair_comb_basis(c.g.Vh, dh.cb(), w1, w2, dh.n2(j), V[j + 1], V[j + 2], dh.n2(j + 1));
+    return;
+  }
   c.blas->pair_comb_dev(dh.cb(), a1, a2, dh.n2(j + 1));
   spinor_convert_rnorm(c.g, w1, V[j + 1], dh.n2(j), c.stream);
   spinor_convert_rnorm(c.g, w2, V[j + 2], dh.n2(j + 1), c.stream);
