@@ -584,7 +584,12 @@ TuneResult autotune_multi(Context &c, Prec p, int recon, int nrhs) {
         float ms = 0;
         QQCD_HIP(hipEventElapsedTime(&ms, e0, e1));
         const double per = ms / reps;
-        if (per < best.ms) {
+        // a larger RHS block must beat the best smaller one by 5%: the hop is
+        // a third of a batched step, and the passes over the slots' bases run
+        // faster in smaller blocks (63 sources at 24^3x48: 10.5 s with block
+        // 16 against 11.0 s with block 32, whose hop was 3% faster)
+        const double need = best.nb > 0 && nb > best.nb ? 0.95 * best.ms : best.ms;
+        if (per < need) {
           best.ms = per;
           best.nb = nb;
           best.t = t;
